@@ -1,0 +1,6 @@
+"""MI355X (gfx950) implementation of g2o's per-iteration Levenberg-Marquardt
+linear-system path (BlockSolver<BlockSolverTraits<6,3>> with Schur, block-Jacobi
+PCG) behind a C ABI (include/g2o_hip.h), with a Python mirror of the
+SparseOptimizer / Solver API."""
+from ._lib import lib, LIB_PATH, FP32, FP64, SCHUR_EXPLICIT, SCHUR_IMPLICIT  # noqa: F401
+from .solver import BundleAdjuster, Context, G2oHipError, linear_solve_pcg, default_options  # noqa: F401

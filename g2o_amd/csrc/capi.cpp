@@ -1,0 +1,265 @@
+// extern "C" entry points of include/g2o_hip.h. Each wraps one context
+// operation; errors become status codes plus the context's last-error text.
+#include <new>
+#include <string>
+
+#include "context.h"
+
+extern "C" {
+
+void g2ohip_default_options(g2ohip_options* opt) {
+  if (!opt) return;
+  opt->precision = G2OHIP_FP64;
+  opt->schur_mode = G2OHIP_SCHUR_EXPLICIT;
+  opt->pcg_tolerance = 1e-6;
+  opt->pcg_max_iter = -1;
+  opt->refine_steps = 1;
+  opt->max_trials = 10;
+  opt->user_lambda_init = 0.0;
+  opt->tau = 1e-5;
+}
+
+int g2ohip_create(int device, g2ohip_ctx** out) {
+  if (!out) return G2OHIP_ERR_INVALID;
+  *out = nullptr;
+  g2ohip_ctx* c = new (std::nothrow) g2ohip_ctx();
+  if (!c) return G2OHIP_ERR_OOM;
+  g2ohip_default_options(&c->opt);
+  if (device >= 0) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device >= n) {
+      delete c;
+      return G2OHIP_ERR_NO_DEVICE;
+    }
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 256, hipHostMallocDefault) != hipSuccess) {
+      delete c;
+      return G2OHIP_ERR_HIP;
+    }
+    c->device = device;
+  }
+  *out = c;
+  return G2OHIP_OK;
+}
+
+void g2ohip_destroy(g2ohip_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+  delete ctx;
+}
+
+const char* g2ohip_last_error(const g2ohip_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+#define G2OHIP_ENTER(ctx)                                   \
+  if (!(ctx)) return G2OHIP_ERR_INVALID;                    \
+  if ((ctx)->device >= 0) (void)hipSetDevice((ctx)->device);
+
+int g2ohip_set_options(g2ohip_ctx* ctx, const g2ohip_options* opt) {
+  G2OHIP_ENTER(ctx);
+  if (!opt) return ctx->fail(G2OHIP_ERR_INVALID, "null options");
+  if (opt->precision != G2OHIP_FP64 && opt->precision != G2OHIP_FP32)
+    return ctx->fail(G2OHIP_ERR_INVALID, "unknown precision");
+  if (opt->schur_mode != G2OHIP_SCHUR_EXPLICIT && opt->schur_mode != G2OHIP_SCHUR_IMPLICIT)
+    return ctx->fail(G2OHIP_ERR_INVALID, "unknown Schur mode");
+  if (!(opt->pcg_tolerance >= 0) || opt->max_trials < 1 || opt->refine_steps < 0 || opt->refine_steps > 16)
+    return ctx->fail(G2OHIP_ERR_INVALID, "invalid option value");
+  ctx->opt = *opt;
+  return G2OHIP_OK;
+}
+
+int g2ohip_get_options(const g2ohip_ctx* ctx, g2ohip_options* opt) {
+  if (!ctx || !opt) return G2OHIP_ERR_INVALID;
+  *opt = ctx->opt;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_edges, const int* edge_cam,
+                            const int* edge_pt, const uint8_t* cam_fixed, const uint8_t* pt_fixed) {
+  G2OHIP_ENTER(ctx);
+  return ctx->set_structure(num_cams, num_pts, num_edges, edge_cam, edge_pt, cam_fixed, pt_fixed);
+}
+
+int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
+  G2OHIP_ENTER(ctx);
+  if (!ctx->have_structure || !info) return ctx->fail(G2OHIP_ERR_INVALID, "no structure");
+  int rc = ctx->ensure_rows();
+  if (rc) return rc;
+  const auto& st = ctx->st;
+  int maxn = 0;
+  for (int i = 0; i < st.npos; ++i) maxn = std::max(maxn, st.pos_ptr[i + 1] - st.pos_ptr[i]);
+  info[0] = st.Cf;
+  info[1] = st.Pf;
+  info[2] = st.ns;
+  info[3] = st.nS;
+  info[4] = st.nlc;
+  info[5] = static_cast<long long>(st.rows.tile_row.size());
+  info[6] = st.rows.pair_instances;
+  info[7] = st.rows.max_row_cols;
+  info[8] = maxn;
+  info[9] = st.nF;
+  info[10] = static_cast<long long>(st.lc_cams.size());
+  info[11] = st.npos;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ba_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* weight, const double* intr,
+                               double huber_delta) {
+  G2OHIP_ENTER(ctx);
+  return ctx->set_measurements(meas, weight, intr, huber_delta);
+}
+
+int g2ohip_ba_set_state(g2ohip_ctx* ctx, const double* cams, const double* pts) {
+  G2OHIP_ENTER(ctx);
+  if (!cams || (!pts && ctx->st.P > 0)) return ctx->fail(G2OHIP_ERR_INVALID, "null state");
+  return ctx->set_state(cams, pts);
+}
+
+int g2ohip_ba_get_state(g2ohip_ctx* ctx, double* cams, double* pts) {
+  G2OHIP_ENTER(ctx);
+  return ctx->get_state(cams, pts);
+}
+
+int g2ohip_compute_active_errors(g2ohip_ctx* ctx, double* chi2) {
+  G2OHIP_ENTER(ctx);
+  if (!chi2) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  return ctx->compute_active_errors(chi2);
+}
+
+int g2ohip_get_errors(g2ohip_ctx* ctx, double* err) {
+  G2OHIP_ENTER(ctx);
+  if (!err) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  double chi;
+  return ctx->compute_active_errors(&chi, err);
+}
+
+int g2ohip_build_system(g2ohip_ctx* ctx) {
+  G2OHIP_ENTER(ctx);
+  return ctx->build_system();
+}
+
+int g2ohip_set_lambda(g2ohip_ctx* ctx, double lambda) {
+  G2OHIP_ENTER(ctx);
+  if (!(lambda >= 0) && !(lambda < 0)) return ctx->fail(G2OHIP_ERR_INVALID, "lambda is NaN");
+  ctx->lambda = lambda;
+  return G2OHIP_OK;
+}
+
+int g2ohip_restore_diagonal(g2ohip_ctx* ctx) {
+  G2OHIP_ENTER(ctx);
+  // the damping lives only inside the solve kernels; nothing to restore
+  return G2OHIP_OK;
+}
+
+int g2ohip_solve(g2ohip_ctx* ctx, int* ok) {
+  G2OHIP_ENTER(ctx);
+  if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  return ctx->solve(ok);
+}
+
+int g2ohip_get_x(g2ohip_ctx* ctx, double* x) {
+  G2OHIP_ENTER(ctx);
+  if (!x || !ctx->have_solution) return ctx->fail(G2OHIP_ERR_INVALID, "no solution");
+  size_t n = 6 * static_cast<size_t>(ctx->st.Cf) + 3 * static_cast<size_t>(ctx->st.Pf);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      hipMemcpy(x, ctx->d_x.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return ctx->fail(G2OHIP_ERR_HIP, "copy of x failed");
+  return G2OHIP_OK;
+}
+
+int g2ohip_get_b(g2ohip_ctx* ctx, double* b) {
+  G2OHIP_ENTER(ctx);
+  if (!b || !ctx->have_system) return ctx->fail(G2OHIP_ERR_INVALID, "no system");
+  size_t n = 6 * static_cast<size_t>(ctx->st.Cf) + 3 * static_cast<size_t>(ctx->st.Pf);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      hipMemcpy(b, ctx->d_b.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return ctx->fail(G2OHIP_ERR_HIP, "copy of b failed");
+  return G2OHIP_OK;
+}
+
+int g2ohip_update(g2ohip_ctx* ctx) {
+  G2OHIP_ENTER(ctx);
+  return ctx->update();
+}
+
+int g2ohip_push(g2ohip_ctx* ctx) {
+  G2OHIP_ENTER(ctx);
+  return ctx->push();
+}
+
+int g2ohip_pop(g2ohip_ctx* ctx) {
+  G2OHIP_ENTER(ctx);
+  return ctx->pop();
+}
+
+int g2ohip_discard_top(g2ohip_ctx* ctx) {
+  G2OHIP_ENTER(ctx);
+  return ctx->discard_top();
+}
+
+int g2ohip_lambda_init(g2ohip_ctx* ctx, double* lambda) {
+  G2OHIP_ENTER(ctx);
+  if (!lambda) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  return ctx->lambda_init(lambda);
+}
+
+int g2ohip_compute_scale(g2ohip_ctx* ctx, double* scale) {
+  G2OHIP_ENTER(ctx);
+  if (!scale) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  return ctx->compute_scale(scale);
+}
+
+int g2ohip_optimize(g2ohip_ctx* ctx, int iterations, g2ohip_iter_stats* stats, int* done) {
+  G2OHIP_ENTER(ctx);
+  int d = 0;
+  int rc = ctx->optimize(iterations, stats, &d);
+  if (done) *done = d;
+  return rc;
+}
+
+int g2ohip_set_force_stop_flag(g2ohip_ctx* ctx, const volatile int* flag) {
+  G2OHIP_ENTER(ctx);
+  ctx->force_stop = flag;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ba_get_hessian(g2ohip_ctx* ctx, double* Hpp, double* Hll, double* Hpl_edge) {
+  G2OHIP_ENTER(ctx);
+  return ctx->get_hessian(Hpp, Hll, Hpl_edge);
+}
+
+int g2ohip_ba_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* bschur) {
+  G2OHIP_ENTER(ctx);
+  return ctx->get_schur(rows, cols, blocks, bschur);
+}
+
+int g2ohip_last_solve_info(g2ohip_ctx* ctx, double* info) {
+  G2OHIP_ENTER(ctx);
+  if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  for (int i = 0; i < 3; ++i) info[i] = ctx->last_info[i];
+  return G2OHIP_OK;
+}
+
+int g2ohip_set_profiling(g2ohip_ctx* ctx, int on) {
+  G2OHIP_ENTER(ctx);
+  return ctx->set_profiling(on);
+}
+
+int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launches) {
+  G2OHIP_ENTER(ctx);
+  return ctx->get_profile(kernel, total_ms, launches);
+}
+
+int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int* brow, const int* bcol,
+                            const double* blocks, const double* b, double* x, double tol, int max_iter,
+                            double* residual_io, int* iterations) {
+  G2OHIP_ENTER(ctx);
+  return ctx->linear_solve_pcg(nb, bd, nblk, brow, bcol, blocks, b, x, tol, max_iter, residual_io, iterations);
+}
+
+int g2ohip_ba_adjoint(g2ohip_ctx* ctx, const double* dL_dcam, const double* dL_dpt, double* dL_dmeas,
+                      double* dL_dweight, double tol, int max_iter, int* iterations) {
+  G2OHIP_ENTER(ctx);
+  return ctx->adjoint(dL_dcam, dL_dpt, dL_dmeas, dL_dweight, tol, max_iter, iterations);
+}
+
+}  // extern "C"

@@ -1,0 +1,738 @@
+// Device context: structure upload, the g2o Solver operations on the device
+// and the Levenberg-Marquardt loop (see context.h).
+#include "context.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <type_traits>
+
+using namespace g2ohip;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename HT>
+PrecBufs<HT>& bufs_of(g2ohip_ctx& c);
+template <>
+PrecBufs<float>& bufs_of<float>(g2ohip_ctx& c) { return c.f32; }
+template <>
+PrecBufs<double>& bufs_of<double>(g2ohip_ctx& c) { return c.f64; }
+
+}  // namespace
+
+g2ohip_ctx::~g2ohip_ctx() {
+  for (auto e : ev_pool) (void)hipEventDestroy(e);
+  for (auto* b : stack_c) delete b;
+  for (auto* b : stack_p) delete b;
+  if (h_pinned) (void)hipHostFree(h_pinned);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int g2ohip_ctx::fail(int code, const std::string& msg) {
+  last_error = msg;
+  return code;
+}
+
+int g2ohip_ctx::require_device() {
+  if (device < 0) return fail(G2OHIP_ERR_NO_DEVICE, "context has no device");
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return G2OHIP_OK;
+  if (e == hipErrorOutOfMemory) return fail(G2OHIP_ERR_OOM, std::string(what) + ": out of device memory");
+  return fail(G2OHIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define CK(...)                                  \
+  do {                                           \
+    int rc_ = (__VA_ARGS__);                     \
+    if (rc_ != G2OHIP_OK) return rc_;            \
+  } while (0)
+#define HK(...) CK(hip((__VA_ARGS__), #__VA_ARGS__))
+
+int g2ohip_ctx::read_scalar(const double* dptr, double* out) {
+  HK(hipMemcpyAsync(h_pinned, dptr, sizeof(double), hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  *out = h_pinned[0];
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::read_err(int* out) {
+  HK(hipMemcpyAsync(h_pinned, d_err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  std::memcpy(out, h_pinned, sizeof(int));
+  return G2OHIP_OK;
+}
+
+// ---------------------------------------------------------------- timing
+int g2ohip_ctx::prof_begin() {
+  if (!profiling) return -1;
+  if (ev_used + 2 > ev_pool.size()) {
+    for (int i = 0; i < 256; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      ev_pool.push_back(e);
+    }
+  }
+  int i = static_cast<int>(ev_used++);
+  (void)hipEventRecord(ev_pool[i], stream);
+  return i;
+}
+
+void g2ohip_ctx::prof_end(int kernel, int begin) {
+  if (begin < 0 || !profiling) return;
+  int i = static_cast<int>(ev_used++);
+  (void)hipEventRecord(ev_pool[i], stream);
+  ev_marks[kernel].push_back({begin, i});
+}
+
+int g2ohip_ctx::set_profiling(int on) {
+  CK(require_device());
+  HK(hipStreamSynchronize(stream));
+  profiling = on != 0;
+  ev_used = 0;
+  for (auto& m : ev_marks) m.clear();
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::get_profile(int kernel, double* total_ms, int* launches) {
+  if (kernel < 0 || kernel >= G2OHIP_K_COUNT) return fail(G2OHIP_ERR_INVALID, "unknown kernel id");
+  HK(hipStreamSynchronize(stream));
+  double t = 0;
+  for (auto& m : ev_marks[kernel]) {
+    float ms = 0;
+    HK(hipEventElapsedTime(&ms, ev_pool[m.first], ev_pool[m.second]));
+    t += ms;
+  }
+  if (total_ms) *total_ms = t;
+  if (launches) *launches = static_cast<int>(ev_marks[kernel].size());
+  return G2OHIP_OK;
+}
+
+struct ProfScope {
+  g2ohip_ctx* c;
+  int k, b;
+  ProfScope(g2ohip_ctx* c_, int k_) : c(c_), k(k_), b(c_->prof_begin()) {}
+  ~ProfScope() { c->prof_end(k, b); }
+};
+
+// ------------------------------------------------------------- structure
+int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep, const uint8_t* cf,
+                              const uint8_t* pf) {
+  if (!ec || !ep) return fail(G2OHIP_ERR_INVALID, "edge index arrays are required");
+  std::string msg = build_structure(st, C, P, E, ec, ep, cf, pf, true);
+  if (!msg.empty()) {
+    have_structure = false;
+    return fail(msg.find("range") != std::string::npos || msg.find("invalid") != std::string::npos
+                    ? G2OHIP_ERR_INVALID
+                    : G2OHIP_ERR_UNSUPPORTED,
+                msg);
+  }
+  have_structure = true;
+  have_meas = have_state = have_system = have_solution = false;
+  rows_ready = false;
+  pcg_residual_mem = -1.0;
+  stack_depth = 0;
+  if (device < 0) return G2OHIP_OK;
+  HK(d_camH.upload(st.camH, stream));
+  HK(d_ptH.upload(st.ptH, stream));
+  HK(d_pos_pt.upload(st.pos_pt, stream));
+  HK(d_pos_h.upload(st.pos_h, stream));
+  HK(d_pos_ptr.upload(st.pos_ptr, stream));
+  HK(d_pos_nfree.upload(st.pos_nfree, stream));
+  HK(d_slot_cam.upload(st.slot_cam, stream));
+  HK(d_slot_pos.upload(st.slot_pos, stream));
+  HK(d_lc_pos0.upload(st.lc_pos0, stream));
+  HK(d_lc_cam_ptr.upload(st.lc_cam_ptr, stream));
+  HK(d_lc_cams.upload(st.lc_cams, stream));
+  HK(d_camp_ptr.upload(st.camp_ptr, stream));
+  HK(d_camp_idx.upload(st.camp_idx, stream));
+  HK(d_S_row.upload(st.S_row, stream));
+  HK(d_S_col.upload(st.S_col, stream));
+  HK(d_S_diag.upload(st.S_diag, stream));
+  HK(d_up_pos.upload(st.up_pos, stream));
+  HK(d_up_posT.upload(st.up_posT, stream));
+  HK(d_F_ptr.upload(st.F_ptr, stream));
+  HK(d_F_col.upload(st.F_col, stream));
+  std::vector<int> fdiag(st.Cf), iota(st.Cf);
+  for (int c = 0; c < st.Cf; ++c) { fdiag[c] = st.up_pos[st.S_diag[c]]; iota[c] = c; }
+  HK(d_F_diag.upload(fdiag, stream));
+  HK(d_iota.upload(iota, stream));
+  const size_t nx = 6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);
+  HK(d_cams.alloc(7 * static_cast<size_t>(C)));
+  HK(d_pts.alloc(3 * static_cast<size_t>(std::max(P, 1))));
+  HK(d_camRt.alloc(12 * static_cast<size_t>(C)));
+  HK(d_Hll.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
+  HK(d_b.alloc(nx));
+  HK(d_x.alloc(nx));
+  HK(d_Hpp.alloc(36 * static_cast<size_t>(st.Cf)));
+  HK(d_hpp_part.alloc(27 * std::max<size_t>(st.lc_cams.size(), 1)));
+  HK(d_cam_part.alloc(6 * std::max<size_t>(st.lc_cams.size(), 1)));
+  HK(d_chunk_chi.alloc(std::max(st.nlc, 1)));
+  HK(d_bschur.alloc(6 * static_cast<size_t>(st.Cf)));
+  HK(d_res.alloc(6 * static_cast<size_t>(st.Cf)));
+  HK(d_xp.alloc(6 * static_cast<size_t>(st.Cf)));
+  HK(d_red.alloc(RED_BLOCKS));
+  HK(d_scal.alloc(8));
+  HK(d_dq_part.alloc(std::max(st.Cf, 1)));
+  HK(d_dn_part.alloc(std::max(st.Cf, 1)));
+  HK(d_err.alloc(4));
+  HK(d_sc.alloc(1));
+  HK(hipMemsetAsync(d_b.p, 0, nx * sizeof(double), stream));
+  HK(hipMemsetAsync(d_x.p, 0, nx * sizeof(double), stream));
+  g = DevGraph{};
+  g.C = st.C; g.P = st.P; g.Cf = st.Cf; g.Pf = st.Pf; g.npos = st.npos; g.ns = st.ns; g.nlc = st.nlc;
+  g.nS = st.nS; g.nF = st.nF;
+  g.camH = d_camH.p; g.ptH = d_ptH.p;
+  g.pos_pt = d_pos_pt.p; g.pos_h = d_pos_h.p; g.pos_ptr = d_pos_ptr.p; g.pos_nfree = d_pos_nfree.p;
+  g.slot_cam = d_slot_cam.p; g.slot_pos = d_slot_pos.p;
+  g.lc_pos0 = d_lc_pos0.p; g.lc_cam_ptr = d_lc_cam_ptr.p; g.lc_cams = d_lc_cams.p;
+  g.camp_ptr = d_camp_ptr.p; g.camp_idx = d_camp_idx.p;
+  g.S_row = d_S_row.p; g.S_col = d_S_col.p; g.S_diag = d_S_diag.p; g.up_pos = d_up_pos.p; g.up_posT = d_up_posT.p;
+  g.F_ptr = d_F_ptr.p; g.F_col = d_F_col.p; g.F_diag = d_F_diag.p;
+  HK(hipStreamSynchronize(stream));
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::ensure_rows() {
+  if (rows_ready) return G2OHIP_OK;
+  if (st.rows.row_ptr.empty()) {
+    std::string msg = build_row_plan(st, SCHUR_MAXC, 8);
+    if (!msg.empty()) return fail(G2OHIP_ERR_UNSUPPORTED, msg);
+  }
+  if (device < 0) return G2OHIP_OK;
+  const RowPlan& rp = st.rows;
+  RowPlanBufs& B = row_bufs;
+  HK(B.row_ptr.upload(rp.row_ptr, stream));
+  HK(B.row_slot.upload(rp.row_slot, stream));
+  HK(B.tile_row.upload(rp.tile_row, stream));
+  HK(B.tile_k0.upload(rp.tile_k0, stream));
+  HK(B.tile_k1.upload(rp.tile_k1, stream));
+  HK(B.tile_order.upload(rp.tile_order, stream));
+  HK(B.S_rowptr.upload(st.S_rowptr, stream));
+  drows.ntiles = static_cast<int>(rp.tile_row.size());
+  drows.row_ptr = B.row_ptr.p; drows.row_slot = B.row_slot.p; drows.tile_row = B.tile_row.p;
+  drows.tile_k0 = B.tile_k0.p; drows.tile_k1 = B.tile_k1.p; drows.tile_order = B.tile_order.p;
+  drows.S_rowptr = B.S_rowptr.p;
+  HK(hipStreamSynchronize(stream));
+  rows_ready = true;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::set_measurements(const double* meas, const double* w, const double* intr, double huber) {
+  if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
+  if (!meas || !intr) return fail(G2OHIP_ERR_INVALID, "measurements and intrinsics are required");
+  CK(require_device());
+  std::vector<double> m(2 * static_cast<size_t>(st.ns)), ww(st.ns);
+  for (int s = 0; s < st.ns; ++s) {
+    int k = st.slot_edge[s];
+    m[2 * static_cast<size_t>(s)] = meas[2 * static_cast<size_t>(k)];
+    m[2 * static_cast<size_t>(s) + 1] = meas[2 * static_cast<size_t>(k) + 1];
+    ww[s] = w ? w[k] : 1.0;
+  }
+  std::vector<double> in(intr, intr + 4 * static_cast<size_t>(st.C));
+  HK(d_meas.upload(m, stream));
+  HK(d_w.upload(ww, stream));
+  HK(d_intr.upload(in, stream));
+  HK(hipStreamSynchronize(stream));
+  g.slot_meas = d_meas.p;
+  g.slot_w = d_w.p;
+  g.intr = d_intr.p;
+  g.huber = huber;
+  have_meas = true;
+  have_system = false;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::set_state(const double* cams, const double* pts) {
+  if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
+  CK(require_device());
+  std::vector<double> c(cams, cams + 7 * static_cast<size_t>(st.C));
+  for (int i = 0; i < st.C; ++i) {
+    double* q = &c[7 * static_cast<size_t>(i) + 3];
+    if (q[3] < 0) { q[0] = -q[0]; q[1] = -q[1]; q[2] = -q[2]; q[3] = -q[3]; }
+    double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (!(n > 0)) return fail(G2OHIP_ERR_INVALID, "zero quaternion");
+    for (int k = 0; k < 4; ++k) q[k] /= n;
+  }
+  std::vector<double> p(pts, pts + 3 * static_cast<size_t>(st.P));
+  HK(d_cams.upload(c, stream));
+  if (st.P > 0) HK(d_pts.upload(p, stream));
+  HK(hipStreamSynchronize(stream));
+  have_state = true;
+  have_system = false;
+  stack_depth = 0;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::get_state(double* cams, double* pts) {
+  if (!have_state) return fail(G2OHIP_ERR_INVALID, "no state");
+  if (cams) HK(hipMemcpyAsync(cams, d_cams.p, 7 * sizeof(double) * st.C, hipMemcpyDeviceToHost, stream));
+  if (pts && st.P > 0) HK(hipMemcpyAsync(pts, d_pts.p, 3 * sizeof(double) * st.P, hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  return G2OHIP_OK;
+}
+
+// ------------------------------------------------------------- residuals
+int g2ohip_ctx::compute_active_errors(double* chi2, double* err_out) {
+  if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+  HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
+  DBuf<double> eslots;
+  if (err_out) HK(eslots.alloc(2 * static_cast<size_t>(st.ns)));
+  {
+    ProfScope ps(this, G2OHIP_K_CHI2);
+    HK(launch_chi2(g, d_camRt.p, d_pts.p, d_red.p, d_scal.p, err_out ? eslots.p : nullptr, stream));
+  }
+  CK(read_scalar(d_scal.p, chi2));
+  if (err_out) {
+    std::vector<double> es(2 * static_cast<size_t>(st.ns));
+    HK(hipMemcpy(es.data(), eslots.p, es.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::fill(err_out, err_out + 2 * static_cast<size_t>(st.E), 0.0);
+    for (int s = 0; s < st.ns; ++s) {
+      int k = st.slot_edge[s];
+      err_out[2 * static_cast<size_t>(k)] = es[2 * static_cast<size_t>(s)];
+      err_out[2 * static_cast<size_t>(k) + 1] = es[2 * static_cast<size_t>(s) + 1];
+    }
+  }
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::build_system() {
+  if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+  HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
+  const int prec = opt.precision;
+  if (prec == G2OHIP_FP32) {
+    HK(f32.Hpl.alloc(18 * static_cast<size_t>(std::max(st.ns, 1))));
+    ProfScope ps(this, G2OHIP_K_LINEARIZE);
+    HK(launch_linearize<float>(g, d_camRt.p, d_pts.p, f32.Hpl.p, d_Hll.p, d_b.p, d_hpp_part.p, d_chunk_chi.p, stream));
+  } else {
+    HK(f64.Hpl.alloc(18 * static_cast<size_t>(std::max(st.ns, 1))));
+    ProfScope ps(this, G2OHIP_K_LINEARIZE);
+    HK(launch_linearize<double>(g, d_camRt.p, d_pts.p, f64.Hpl.p, d_Hll.p, d_b.p, d_hpp_part.p, d_chunk_chi.p,
+                                stream));
+  }
+  HK(launch_reduce_hpp(g, d_hpp_part.p, d_Hpp.p, d_b.p, stream));
+  HK(launch_sum(d_chunk_chi.p, st.nlc, d_scal.p + 1, stream));
+  CK(read_scalar(d_scal.p + 1, &chi2_lin));
+  have_system = true;
+  system_prec = prec;
+  have_solution = false;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::lambda_init(double* out) {
+  if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+  if (opt.user_lambda_init > 0) { *out = opt.user_lambda_init; return G2OHIP_OK; }
+  HK(launch_max_diag(g, d_Hpp.p, d_Hll.p, d_red.p, d_scal.p + 2, stream));
+  double m;
+  CK(read_scalar(d_scal.p + 2, &m));
+  *out = opt.tau * m;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::compute_scale(double* out) {
+  if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
+  const int n = 6 * st.Cf + 3 * st.Pf;
+  HK(launch_scale(n, d_x.p, d_b.p, lambda, d_red.p, d_scal.p + 3, stream));
+  return read_scalar(d_scal.p + 3, out);
+}
+
+// ------------------------------------------------------------------- PCG
+template <typename HT>
+int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, bool use_memory, bool implicit, int* iters,
+                        double* dn_final) {
+  const int nb = st.Cf;
+  const int n = 6 * nb;
+  HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
+  HK(launch_pcg_init<HT, 6>(nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.d.p, d_dn_part.p, stream));
+  HK(launch_pcg_start(nb, d_dn_part.p, opt.pcg_tolerance, use_memory ? pcg_residual_mem : -1.0, d_sc.p, stream));
+  const int max_iter = opt.pcg_max_iter < 0 ? n : opt.pcg_max_iter;
+  ProfScope ps(this, G2OHIP_K_PCG);
+  PcgScalars hs{};
+  const int check = 4;
+  for (int it = 0; it < max_iter; ++it) {
+    if (implicit) {
+      HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, B.d.p, nullptr, d_cam_part.p, d_sc.p, it, stream));
+      HK(launch_implicit_reduce<HT>(g, d_cam_part.p, d_Hpp.p, lambda, B.d.p, B.q.p, d_dq_part.p, d_sc.p, it, stream));
+    } else {
+      HK(launch_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.d.p, B.q.p, d_dq_part.p, d_sc.p, it, stream));
+    }
+    HK(launch_pcg_step<HT, 6>(nb, B.Jinv.p, B.x.p, B.r.p, B.d.p, B.q.p, B.s.p, d_dq_part.p, d_dn_part.p, d_sc.p, it,
+                              stream));
+    HK(launch_pcg_dir<HT, 6>(nb, B.d.p, B.s.p, d_dn_part.p, d_sc.p, it, stream));
+    if ((it + 1) % check == 0) {
+      HK(hipMemcpyAsync(h_pinned, d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
+      HK(hipStreamSynchronize(stream));
+      std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
+      if (hs.done) break;
+    }
+  }
+  HK(hipMemcpyAsync(h_pinned, d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
+  int k = hs.done ? hs.iters : max_iter;
+  *iters = k;
+  *dn_final = hs.dn[k & 1];
+  return G2OHIP_OK;
+}
+
+// BlockSolver::solve with Schur (block_solver.hpp:331-477).
+template <typename HT>
+int g2ohip_ctx::solve_impl(int* ok) {
+  const int which = std::is_same<HT, float>::value ? 0 : 1;
+  PrecBufs<HT>& B = bufs_of<HT>(*this);
+  const int nb = st.Cf;
+  const bool implicit = opt.schur_mode == G2OHIP_SCHUR_IMPLICIT;
+  *ok = 1;
+  HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
+  HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
+  CK(ensure_rows());
+  if (!implicit) {
+    HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
+    {
+      ProfScope ps(this, G2OHIP_K_SCHUR);
+      HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, d_Hll.p, d_b.p, d_Hpp.p, lambda, B.F_val.p, d_bschur.p, nullptr, 0,
+                               d_err.p, stream));
+    }
+    HK(launch_jacobi<HT, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream));
+  } else {
+    // matrix-free reduced operator; block-Jacobi preconditioner from the
+    // diagonal blocks of the Schur complement
+    HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
+    HK(B.Sdiag.alloc(36 * static_cast<size_t>(nb)));
+    HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
+    HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, d_Hll.p, d_b.p, d_Hpp.p, lambda, nullptr, nullptr, B.Sdiag.p, 1,
+                             d_err.p, stream));
+    HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, nullptr, d_b.p, d_cam_part.p, nullptr, 0, stream));
+    HK(launch_implicit_rhs_reduce(g, d_cam_part.p, d_b.p, d_bschur.p, stream));
+    HK(launch_jacobi_damped<HT, 6>(nb, B.Sdiag.p, d_iota.p, lambda, B.Jinv.p, d_err.p, stream));
+  }
+  int err = 0;
+  CK(read_err(&err));
+  if (err) { *ok = 0; return G2OHIP_OK; }
+  const bool refine = (which == 0) && opt.refine_steps > 0;
+  int total_it = 0, it = 0;
+  double dn_final = 0;
+  if (!refine) {
+    CK(run_pcg<HT>(B, d_bschur.p, true, implicit, &it, &dn_final));
+    total_it += it;
+    pcg_residual_mem = 0.5 * dn_final;
+    HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 0.0, stream));
+    // back-substitution x_l = Dinv (b_l - Hpl^T x_p), re-linearised in FP64
+    {
+      ProfScope ps(this, G2OHIP_K_REFINE);
+      HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, d_x.p, d_x.p + 6 * static_cast<size_t>(nb), nullptr, d_err.p,
+                       stream));
+    }
+    last_info[1] = 0;
+  } else {
+    // FP64 iterative refinement: r = g - S x in FP64 (matrix-free), FP32 PCG
+    // corrections. The first solve keeps the PCG residual memory semantics;
+    // the corrections stop on their own relative tolerance.
+    {
+      ProfScope ps(this, G2OHIP_K_REFINE);
+      HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, nullptr, d_x.p + 6 * static_cast<size_t>(nb), d_cam_part.p,
+                       d_err.p, stream));
+    }
+    HK(launch_refine_reduce(g, d_cam_part.p, nullptr, lambda, d_res.p, stream));
+    HK(hipMemsetAsync(d_x.p, 0, 6 * sizeof(double) * nb, stream));
+    for (int step = 0; step <= opt.refine_steps; ++step) {
+      CK(run_pcg<HT>(B, d_res.p, step == 0, implicit, &it, &dn_final));
+      total_it += it;
+      if (step == 0) pcg_residual_mem = 0.5 * dn_final;
+      HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 1.0, stream));
+      const bool last = step == opt.refine_steps;
+      {
+        ProfScope ps(this, G2OHIP_K_REFINE);
+        HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, d_x.p, d_x.p + 6 * static_cast<size_t>(nb),
+                         last ? nullptr : d_cam_part.p, d_err.p, stream));
+      }
+      if (!last) HK(launch_refine_reduce(g, d_cam_part.p, d_x.p, lambda, d_res.p, stream));
+    }
+    last_info[1] = opt.refine_steps;
+  }
+  CK(read_err(&err));
+  if (err) *ok = 0;
+  last_info[0] = total_it;
+  last_info[2] = pcg_residual_mem;
+  have_solution = true;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::solve(int* ok) {
+  if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+  if (system_prec == G2OHIP_FP32) return solve_impl<float>(ok);
+  return solve_impl<double>(ok);
+}
+
+// --------------------------------------------------------- update / stack
+int g2ohip_ctx::update() {
+  if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
+  {
+    ProfScope ps(this, G2OHIP_K_UPDATE);
+    HK(launch_update(g, d_x.p, d_cams.p, d_pts.p, stream));
+  }
+  HK(hipStreamSynchronize(stream));
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::push() {
+  if (!have_state) return fail(G2OHIP_ERR_INVALID, "no state");
+  if (stack_depth >= 4) return fail(G2OHIP_ERR_INVALID, "state stack is full (depth 4)");
+  if (static_cast<int>(stack_c.size()) <= stack_depth) {
+    stack_c.push_back(new DBuf<double>());
+    stack_p.push_back(new DBuf<double>());
+  }
+  DBuf<double>& c = *stack_c[stack_depth];
+  DBuf<double>& p = *stack_p[stack_depth];
+  HK(c.alloc(7 * static_cast<size_t>(st.C)));
+  HK(p.alloc(3 * static_cast<size_t>(std::max(st.P, 1))));
+  HK(launch_copy(c.p, d_cams.p, 7 * sizeof(double) * st.C, stream));
+  if (st.P) HK(launch_copy(p.p, d_pts.p, 3 * sizeof(double) * st.P, stream));
+  stack_depth++;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::pop() {
+  if (stack_depth <= 0) return fail(G2OHIP_ERR_INVALID, "state stack is empty");
+  stack_depth--;
+  HK(launch_copy(d_cams.p, stack_c[stack_depth]->p, 7 * sizeof(double) * st.C, stream));
+  if (st.P) HK(launch_copy(d_pts.p, stack_p[stack_depth]->p, 3 * sizeof(double) * st.P, stream));
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::discard_top() {
+  if (stack_depth <= 0) return fail(G2OHIP_ERR_INVALID, "state stack is empty");
+  stack_depth--;
+  return G2OHIP_OK;
+}
+
+// ------------------------------------------------------------------- LM
+// SparseOptimizer::optimize (sparse_optimizer.cpp:393-455) driving
+// OptimizationAlgorithmLevenberg::solve (optimization_algorithm_levenberg.cpp:60-153).
+int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
+  if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+  *done = 0;
+  pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
+  double ni = 2.0;
+  for (int i = 0; i < iterations; ++i) {
+    if (force_stop && *force_stop) break;
+    g2ohip_iter_stats s{};
+    const double t_it = now_s();
+    double t = now_s();
+    // computeActiveErrors + activeRobustChi2 + buildSystem, fused: the
+    // linearisation pass returns the chi2 of the same state.
+    CK(build_system());
+    const double currentChi0 = chi2_lin;
+    double currentChi = currentChi0;
+    s.time_quadratic_form = now_s() - t;
+    if (i == 0) {
+      CK(lambda_init(&lambda));
+      ni = 2.0;
+    }
+    double rho = 0;
+    int q = 0;
+    do {
+      CK(push());
+      t = now_s();
+      int ok = 0;
+      CK(solve(&ok));
+      s.time_linear_solver += now_s() - t;
+      s.iterations_linear_solver += static_cast<int>(last_info[0]);
+      s.refine_steps += static_cast<int>(last_info[1]);
+      double tempChi = std::numeric_limits<double>::max();
+      if (ok) {
+        t = now_s();
+        CK(update());
+        s.time_update += now_s() - t;
+        t = now_s();
+        CK(compute_active_errors(&tempChi));
+        s.time_residuals += now_s() - t;
+      }
+      rho = currentChi - tempChi;
+      double scale = 1.0;
+      if (ok) {
+        CK(compute_scale(&scale));
+        scale += 1e-3;
+      }
+      rho /= scale;
+      if (rho > 0 && std::isfinite(tempChi) && ok) {
+        double alpha = 1. - std::pow((2 * rho - 1), 3);
+        alpha = std::min(alpha, 2. / 3.);
+        double sf = std::max(1. / 3., alpha);
+        lambda *= sf;
+        ni = 2;
+        currentChi = tempChi;
+        CK(discard_top());
+      } else {
+        lambda *= ni;
+        ni *= 2;
+        CK(pop());
+        if (!std::isfinite(lambda)) break;
+      }
+      q++;
+    } while (rho < 0 && q < opt.max_trials && !(force_stop && *force_stop));
+    HK(hipStreamSynchronize(stream));
+    s.chi2_before = currentChi0;
+    s.chi2 = currentChi;
+    s.lambda = lambda;
+    s.levenberg_iterations = q;
+    s.status = (q == opt.max_trials || rho == 0 || !std::isfinite(lambda)) ? 1 : 0;
+    s.time_iteration = now_s() - t_it;
+    if (stats) stats[i] = s;
+    *done = i + 1;
+    have_system = false;
+    if (s.status) break;
+  }
+  return G2OHIP_OK;
+}
+
+// --------------------------------------------------------------- inspection
+int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
+  if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+  HK(hipStreamSynchronize(stream));
+  if (Hpp) HK(hipMemcpy(Hpp, d_Hpp.p, 36 * sizeof(double) * st.Cf, hipMemcpyDeviceToHost));
+  if (Hll) {
+    std::vector<double> h(6 * static_cast<size_t>(st.Pf));
+    if (st.Pf) HK(hipMemcpy(h.data(), d_Hll.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int p = 0; p < st.Pf; ++p)
+      for (int k = 0; k < 9; ++k) Hll[9 * static_cast<size_t>(p) + k] = h[6 * static_cast<size_t>(p) + map[k]];
+  }
+  if (Hpl_edge) {
+    std::vector<double> h(18 * static_cast<size_t>(st.ns));
+    if (system_prec == G2OHIP_FP32) {
+      std::vector<float> hf(h.size());
+      if (st.ns) HK(hipMemcpy(hf.data(), f32.Hpl.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < h.size(); ++i) h[i] = hf[i];
+    } else if (st.ns) {
+      HK(hipMemcpy(h.data(), f64.Hpl.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    std::fill(Hpl_edge, Hpl_edge + 18 * static_cast<size_t>(st.E), 0.0);
+    for (int s = 0; s < st.ns; ++s) {
+      const int k = st.slot_edge[s];
+      if (st.camH[st.slot_cam[s]] < 0 || st.pos_h[st.slot_pos[s]] < 0) continue;
+      for (int e = 0; e < 18; ++e) Hpl_edge[18 * static_cast<size_t>(k) + e] = h[18 * static_cast<size_t>(s) + e];
+    }
+  }
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::get_schur(int* rows, int* cols, double* blocks, double* bschur) {
+  if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
+  if (opt.schur_mode != G2OHIP_SCHUR_EXPLICIT) return fail(G2OHIP_ERR_INVALID, "explicit Schur mode only");
+  HK(hipStreamSynchronize(stream));
+  for (int k = 0; k < st.nS; ++k) {
+    if (rows) rows[k] = st.S_row[k];
+    if (cols) cols[k] = st.S_col[k];
+  }
+  if (blocks) {
+    std::vector<double> F(36 * static_cast<size_t>(st.nF));
+    if (system_prec == G2OHIP_FP32) {
+      std::vector<float> Ff(F.size());
+      HK(hipMemcpy(Ff.data(), f32.F_val.p, Ff.size() * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < F.size(); ++i) F[i] = Ff[i];
+    } else {
+      HK(hipMemcpy(F.data(), f64.F_val.p, F.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int k = 0; k < st.nS; ++k)
+      std::copy(&F[36 * static_cast<size_t>(st.up_pos[k])], &F[36 * static_cast<size_t>(st.up_pos[k])] + 36,
+                &blocks[36 * static_cast<size_t>(k)]);
+  }
+  if (bschur) HK(hipMemcpy(bschur, d_bschur.p, 6 * sizeof(double) * st.Cf, hipMemcpyDeviceToHost));
+  return G2OHIP_OK;
+}
+
+// ------------------------------------------------ LinearSolverPCG on a host A
+int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, const int* bcol, const double* blocks,
+                                 const double* b, double* x, double tol, int max_iter, double* residual_io,
+                                 int* iterations) {
+  CK(require_device());
+  if (bd != 3 && bd != 6) return fail(G2OHIP_ERR_UNSUPPORTED, "block size must be 3 or 6");
+  if (nb <= 0 || nblk <= 0 || !brow || !bcol || !blocks || !b || !x)
+    return fail(G2OHIP_ERR_INVALID, "invalid linear system");
+  // full block rows: both triangles, sorted by column
+  std::vector<std::vector<std::pair<int, int>>> rows(nb);  // (col, signed block idx+1: <0 transposed)
+  std::vector<int> diag(nb, -1);
+  for (int k = 0; k < nblk; ++k) {
+    int i = brow[k], j = bcol[k];
+    if (i < 0 || j < 0 || i >= nb || j >= nb || i > j) return fail(G2OHIP_ERR_INVALID, "blocks must be upper (row <= col)");
+    rows[i].push_back({j, k + 1});
+    if (i != j) rows[j].push_back({i, -(k + 1)});
+    else diag[i] = k;
+  }
+  for (int i = 0; i < nb; ++i)
+    if (diag[i] < 0) return fail(G2OHIP_ERR_NOT_PD, "missing diagonal block");
+  std::vector<int> Fptr(nb + 1, 0), Fcol, Fdiag(nb);
+  std::vector<double> Fval;
+  const int bb = bd * bd;
+  for (int i = 0; i < nb; ++i) {
+    std::sort(rows[i].begin(), rows[i].end());
+    for (auto& e : rows[i]) {
+      int k = std::abs(e.second) - 1;
+      if (e.first == i && e.second > 0) Fdiag[i] = static_cast<int>(Fcol.size());
+      Fcol.push_back(e.first);
+      for (int r = 0; r < bd; ++r)
+        for (int c = 0; c < bd; ++c)
+          Fval.push_back(e.second > 0 ? blocks[static_cast<size_t>(k) * bb + r * bd + c]
+                                      : blocks[static_cast<size_t>(k) * bb + c * bd + r]);
+    }
+    Fptr[i + 1] = static_cast<int>(Fcol.size());
+  }
+  DBuf<int> dFptr, dFcol, dFdiag;
+  DBuf<double> dFval, dJ, db, dx, dr, dd, dq, ds, ddq, ddn;
+  DBuf<int> derr;
+  DBuf<PcgScalars> dsc;
+  HK(dFptr.upload(Fptr, stream)); HK(dFcol.upload(Fcol, stream)); HK(dFdiag.upload(Fdiag, stream));
+  HK(dFval.upload(Fval, stream));
+  std::vector<double> bv(b, b + static_cast<size_t>(nb) * bd);
+  HK(db.upload(bv, stream));
+  const size_t n = static_cast<size_t>(nb) * bd;
+  HK(dJ.alloc(static_cast<size_t>(nb) * bb)); HK(dx.alloc(n)); HK(dr.alloc(n)); HK(dd.alloc(n)); HK(dq.alloc(n));
+  HK(ds.alloc(n)); HK(ddq.alloc(nb)); HK(ddn.alloc(nb)); HK(derr.alloc(1)); HK(dsc.alloc(1));
+  HK(hipMemsetAsync(derr.p, 0, sizeof(int), stream));
+  auto run = [&](auto bdc) -> int {
+    constexpr int BD = decltype(bdc)::value;
+    HK((launch_jacobi<double, BD>(nb, dFval.p, dFdiag.p, dJ.p, derr.p, stream)));
+    HK((launch_pcg_init<double, BD>(nb, db.p, dJ.p, dx.p, dr.p, dd.p, ddn.p, stream)));
+    HK(launch_pcg_start(nb, ddn.p, tol, residual_io ? *residual_io : -1.0, dsc.p, stream));
+    const int mi = max_iter < 0 ? static_cast<int>(n) : max_iter;
+    PcgScalars hs{};
+    for (int it = 0; it < mi; ++it) {
+      HK((launch_spmv<double, BD>(nb, dFptr.p, dFcol.p, dFval.p, dd.p, dq.p, ddq.p, dsc.p, it, stream)));
+      HK((launch_pcg_step<double, BD>(nb, dJ.p, dx.p, dr.p, dd.p, dq.p, ds.p, ddq.p, ddn.p, dsc.p, it, stream)));
+      HK((launch_pcg_dir<double, BD>(nb, dd.p, ds.p, ddn.p, dsc.p, it, stream)));
+      if ((it + 1) % 4 == 0) {
+        HK(hipMemcpyAsync(h_pinned, dsc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
+        HK(hipStreamSynchronize(stream));
+        std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
+        if (hs.done) break;
+      }
+    }
+    HK(hipMemcpyAsync(h_pinned, dsc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
+    HK(hipStreamSynchronize(stream));
+    std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
+    int k = hs.done ? hs.iters : mi;
+    if (iterations) *iterations = k;
+    if (residual_io) *residual_io = 0.5 * hs.dn[k & 1];
+    return G2OHIP_OK;
+  };
+  if (bd == 3) CK(run(std::integral_constant<int, 3>()));
+  else CK(run(std::integral_constant<int, 6>()));
+  int err = 0;
+  HK(hipMemcpy(&err, derr.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) return fail(G2OHIP_ERR_NOT_PD, "singular diagonal block");
+  HK(hipMemcpy(x, dx.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::adjoint(const double*, const double*, double*, double*, double, int, int*) {
+  return fail(G2OHIP_ERR_UNSUPPORTED, "adjoint not built yet");
+}

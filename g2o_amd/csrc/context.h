@@ -1,0 +1,145 @@
+// Device context behind the C ABI: one bundle-adjustment graph on one GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/g2o_hip.h"
+#include "kernels.h"
+#include "structure.h"
+
+namespace g2ohip {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), bytes);
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  hipError_t upload(const std::vector<T>& v, hipStream_t st) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+  }
+};
+
+// Buffers of one Hessian precision.
+template <typename HT>
+struct PrecBufs {
+  DBuf<HT> Hpl;       // 18 * ns, one row-major 6x3 block per slot
+  DBuf<HT> F_val;     // nF * 36
+  DBuf<HT> Jinv;      // Cf * 36
+  DBuf<HT> Dinv;      // Pf * 9 (implicit mode)
+  DBuf<HT> Sdiag;     // Cf * 36 (implicit mode preconditioner)
+  DBuf<HT> x, r, d, q, s;  // PCG vectors
+};
+
+struct RowPlanBufs {
+  DBuf<int> row_ptr, row_slot, tile_row, tile_k0, tile_k1, tile_order, S_rowptr;
+};
+
+}  // namespace g2ohip
+
+struct g2ohip_ctx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  g2ohip_options opt{};
+  const volatile int* force_stop = nullptr;
+
+  g2ohip::BAStructure st;
+  bool have_structure = false, have_meas = false, have_state = false;
+  bool have_system = false;   // build_system ran since the last state change
+  int system_prec = 0;        // precision of the last build_system
+  bool have_solution = false;
+  double lambda = 0.0;
+  double pcg_residual_mem = -1.0;  // LinearSolverPCG::_residual
+  double last_info[3] = {0, 0, -1};
+  double chi2_lin = 0.0;
+
+  g2ohip::DevGraph g{};
+  g2ohip::DBuf<int> d_camH, d_ptH, d_pos_pt, d_pos_h, d_pos_ptr, d_pos_nfree, d_slot_cam, d_slot_pos, d_lc_pos0,
+      d_lc_cam_ptr, d_lc_cams, d_camp_ptr, d_camp_idx, d_S_row, d_S_col, d_S_diag, d_up_pos, d_up_posT, d_F_ptr,
+      d_F_col, d_F_diag, d_iota;
+  g2ohip::DBuf<double> d_meas, d_w, d_intr;
+  g2ohip::RowPlanBufs row_bufs;
+  g2ohip::DevRowPlan drows{};
+  bool rows_ready = false;
+
+  g2ohip::DBuf<double> d_cams, d_pts, d_camRt;
+  std::vector<g2ohip::DBuf<double>*> stack_c, stack_p;
+  int stack_depth = 0;
+
+  g2ohip::DBuf<double> d_Hll, d_b, d_Hpp, d_hpp_part, d_chunk_chi, d_coef_part, d_bschur, d_x, d_red, d_scal,
+      d_cam_part, d_res, d_xp, d_err64;
+  g2ohip::DBuf<double> d_dq_part, d_dn_part;
+  g2ohip::DBuf<int> d_err;
+  g2ohip::DBuf<g2ohip::PcgScalars> d_sc;
+  g2ohip::PrecBufs<float> f32;
+  g2ohip::PrecBufs<double> f64;
+  double* h_pinned = nullptr;  // small pinned host buffer for scalars
+
+  // kernel timing (HIP events on `stream`)
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, int>> ev_marks[G2OHIP_K_COUNT];
+  int prof_begin();
+  void prof_end(int kernel, int begin);
+  int set_profiling(int on);
+  int get_profile(int kernel, double* total_ms, int* launches);
+
+  ~g2ohip_ctx();
+  int fail(int code, const std::string& msg);
+  int require_device();
+  int hip(hipError_t e, const char* what);
+
+  // operations (return a g2ohip_status)
+  int set_structure(int C, int P, int E, const int* ec, const int* ep, const uint8_t* cf, const uint8_t* pf);
+  int set_measurements(const double* meas, const double* w, const double* intr, double huber);
+  int set_state(const double* cams, const double* pts);
+  int get_state(double* cams, double* pts);
+  int ensure_rows();
+  int compute_active_errors(double* chi2, double* err_out = nullptr);
+  int build_system();
+  int solve(int* ok);
+  int update();
+  int push();
+  int pop();
+  int discard_top();
+  int lambda_init(double* out);
+  int compute_scale(double* out);
+  int optimize(int iterations, g2ohip_iter_stats* stats, int* done);
+  int get_hessian(double* Hpp, double* Hll, double* Hpl_edge);
+  int get_schur(int* rows, int* cols, double* blocks, double* bschur);
+  int linear_solve_pcg(int nb, int bd, int nblk, const int* brow, const int* bcol, const double* blocks,
+                       const double* b, double* x, double tol, int max_iter, double* residual_io, int* iterations);
+  int adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,
+              int max_iter, int* iterations);
+
+  template <typename HT>
+  int solve_impl(int* ok);
+  template <typename HT>
+  int run_pcg(g2ohip::PrecBufs<HT>& B, const double* rhs, bool use_memory, bool implicit, int* iters,
+              double* dn_final);
+  int read_scalar(const double* dptr, double* out);
+  int read_err(int* out);
+};

@@ -1,0 +1,205 @@
+// Device-side math shared by the kernels: SE3Quat pieces, the projection edge,
+// Huber, small inverses and reductions. FP64 unless templated.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace g2ohip {
+
+// Rotation matrix (row-major) of a unit quaternion [x y z w].
+__host__ __device__ inline void quat_to_rot(const double* q, double* R) {
+  double tx = 2 * q[0], ty = 2 * q[1], tz = 2 * q[2];
+  double twx = tx * q[3], twy = ty * q[3], twz = tz * q[3];
+  double txx = tx * q[0], txy = ty * q[0], txz = tz * q[0];
+  double tyy = ty * q[1], tyz = tz * q[1], tzz = tz * q[2];
+  R[0] = 1 - (tyy + tzz); R[1] = txy - twz;       R[2] = txz + twy;
+  R[3] = txy + twz;       R[4] = 1 - (txx + tzz); R[5] = tyz - twx;
+  R[6] = txz - twy;       R[7] = tyz + twx;       R[8] = 1 - (txx + tyy);
+}
+
+// Quaternion of a rotation matrix, Shepperd's method: the largest of
+// w^2, x^2, y^2, z^2 is taken from the trace / diagonal, the others from the
+// off-diagonal sums and differences.
+__host__ __device__ inline void rot_to_quat(const double* R, double* q) {
+  double tr = R[0] + R[4] + R[8];
+  if (tr > 0) {
+    double s = sqrt(tr + 1.0);
+    q[3] = 0.5 * s;
+    s = 0.5 / s;
+    q[0] = (R[7] - R[5]) * s;
+    q[1] = (R[2] - R[6]) * s;
+    q[2] = (R[3] - R[1]) * s;
+  } else {
+    int i = 0;
+    if (R[4] > R[0]) i = 1;
+    if (R[8] > R[3 * i + i]) i = 2;
+    int j = (i + 1) % 3, k = (j + 1) % 3;
+    double s = sqrt(R[3 * i + i] - R[3 * j + j] - R[3 * k + k] + 1.0);
+    double v[3];
+    v[i] = 0.5 * s;
+    s = 0.5 / s;
+    q[3] = (R[3 * k + j] - R[3 * j + k]) * s;
+    v[j] = (R[3 * j + i] + R[3 * i + j]) * s;
+    v[k] = (R[3 * k + i] + R[3 * i + k]) * s;
+    q[0] = v[0]; q[1] = v[1]; q[2] = v[2];
+  }
+}
+
+// w >= 0 then unit norm: SE3Quat::normalizeRotation (se3quat.h:251-256).
+__host__ __device__ inline void normalize_quat(double* q) {
+  if (q[3] < 0) { q[0] = -q[0]; q[1] = -q[1]; q[2] = -q[2]; q[3] = -q[3]; }
+  double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  q[0] /= n; q[1] /= n; q[2] /= n; q[3] /= n;
+}
+
+// Hamilton product a * b.
+__host__ __device__ inline void quat_mul(const double* a, const double* b, double* r) {
+  r[0] = a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1];
+  r[1] = a[3] * b[1] + a[1] * b[3] + a[2] * b[0] - a[0] * b[2];
+  r[2] = a[3] * b[2] + a[2] * b[3] + a[0] * b[1] - a[1] * b[0];
+  r[3] = a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2];
+}
+
+// pose7 <- exp(upd) * pose7: VertexSE3Expmap::oplusImpl
+// (vertex_se3_expmap.cpp:48-51) with SE3Quat::exp (se3quat.h:202-230) and
+// SE3Quat::operator* (se3quat.h:97-103). upd = [omega; upsilon].
+__host__ __device__ inline void se3_left_update(const double* upd, const double* in, double* out) {
+  const double w0 = upd[0], w1 = upd[1], w2 = upd[2];
+  double theta = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+  double W[9] = {0, -w2, w1, w2, 0, -w0, -w1, w0, 0};
+  double W2[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) W2[3 * i + j] = W[3 * i] * W[j] + W[3 * i + 1] * W[3 + j] + W[3 * i + 2] * W[6 + j];
+  double a, b, c, d;
+  if (theta < 0.00001) {
+    a = 1.0; b = 0.5; c = 0.5; d = 1.0 / 6.0;
+  } else {
+    double st = sin(theta), ct = cos(theta);
+    a = st / theta;
+    b = (1 - ct) / (theta * theta);
+    c = b;
+    d = (theta - st) / (theta * theta * theta);
+  }
+  double R[9], V[9];
+  for (int i = 0; i < 9; ++i) {
+    double I = (i % 4 == 0) ? 1.0 : 0.0;
+    R[i] = I + a * W[i] + b * W2[i];
+    V[i] = I + c * W[i] + d * W2[i];
+  }
+  double qe[4], te[3];
+  rot_to_quat(R, qe);
+  normalize_quat(qe);
+  for (int i = 0; i < 3; ++i) te[i] = V[3 * i] * upd[3] + V[3 * i + 1] * upd[4] + V[3 * i + 2] * upd[5];
+  // exp * T: t = te + Re * t_T, q = qe * q_T
+  double Re[9];
+  quat_to_rot(qe, Re);
+  const double* tT = in;
+  double qT[4] = {in[3], in[4], in[5], in[6]};
+  for (int i = 0; i < 3; ++i) out[i] = te[i] + Re[3 * i] * tT[0] + Re[3 * i + 1] * tT[1] + Re[3 * i + 2] * tT[2];
+  double q[4];
+  quat_mul(qe, qT, q);
+  normalize_quat(q);
+  out[3] = q[0]; out[4] = q[1]; out[5] = q[2]; out[6] = q[3];
+}
+
+// RobustKernelHuber::robustify (robust_kernel_impl.cpp:60-74); delta <= 0
+// means no robust kernel (rho = (e, 1, 0)).
+__host__ __device__ inline void huber_rho(double e, double delta, double* rho) {
+  if (delta <= 0) { rho[0] = e; rho[1] = 1.0; rho[2] = 0.0; return; }
+  double dsqr = delta * delta;
+  if (e <= dsqr) {
+    rho[0] = e; rho[1] = 1.0; rho[2] = 0.0;
+  } else {
+    double sqrte = sqrt(e);
+    rho[0] = 2 * sqrte * delta - dsqr;
+    rho[1] = delta / sqrte;
+    rho[2] = -0.5 * rho[1] / e;
+  }
+}
+
+// Xc = R X + t and e = z - (fx x/z + cx, fy y/z + cy)
+// (EdgeSE3ProjectXYZ::computeError, edge_project_xyz.cpp:44-49).
+__device__ inline void project_error(const double* Rt, const double* in, const double* X, const double* z,
+                                     double* Xc, double* e) {
+  for (int i = 0; i < 3; ++i) Xc[i] = Rt[3 * i] * X[0] + Rt[3 * i + 1] * X[1] + Rt[3 * i + 2] * X[2] + Rt[9 + i];
+  e[0] = z[0] - (Xc[0] / Xc[2] * in[0] + in[2]);
+  e[1] = z[1] - (Xc[1] / Xc[2] * in[1] + in[3]);
+}
+
+// Analytic Jacobians of EdgeSE3ProjectXYZ::linearizeOplus
+// (edge_project_xyz.cpp:57-93) in precision T: Jp = d e / d X (2x3 row-major),
+// Jc = d e / d [omega; upsilon] (2x6 row-major), from the camera-frame point.
+template <typename T>
+__device__ inline void project_jacobians(const T* R, T fx, T fy, T x, T y, T z, T* Jp, T* Jc) {
+  T iz = T(1) / z;
+  T z2 = z * z;
+  T t00 = fx, t02 = -x * iz * fx, t11 = fy, t12 = -y * iz * fy;
+  for (int c = 0; c < 3; ++c) {
+    Jp[c] = -iz * (t00 * R[c] + t02 * R[6 + c]);
+    Jp[3 + c] = -iz * (t11 * R[3 + c] + t12 * R[6 + c]);
+  }
+  Jc[0] = x * y / z2 * fx;
+  Jc[1] = -(T(1) + (x * x / z2)) * fx;
+  Jc[2] = y * iz * fx;
+  Jc[3] = -iz * fx;
+  Jc[4] = T(0);
+  Jc[5] = x / z2 * fx;
+  Jc[6] = (T(1) + y * y / z2) * fy;
+  Jc[7] = -x * y / z2 * fy;
+  Jc[8] = -x * iz * fy;
+  Jc[9] = T(0);
+  Jc[10] = -iz * fy;
+  Jc[11] = y / z2 * fy;
+}
+
+// 3x3 inverse by cofactors; symmetric input given by its 6 unique entries
+// [xx xy xz yy yz zz] plus lambda on the diagonal. Returns false if singular.
+__host__ __device__ inline bool inv3_sym(const double* h, double lambda, double* out) {
+  double a = h[0] + lambda, b = h[1], c = h[2], d = h[3] + lambda, e = h[4], f = h[5] + lambda;
+  double c0 = d * f - e * e;
+  double c1 = e * c - b * f;
+  double c2 = b * e - d * c;
+  double det = a * c0 + b * c1 + c * c2;
+  if (!(det != 0.0) || !isfinite(det)) return false;
+  double id = 1.0 / det;
+  out[0] = c0 * id; out[1] = c1 * id; out[2] = c2 * id;
+  out[3] = c1 * id; out[4] = (a * f - c * c) * id; out[5] = (b * c - a * e) * id;
+  out[6] = c2 * id; out[7] = (b * c - a * e) * id; out[8] = (a * d - b * b) * id;
+  return true;
+}
+
+// 6x6 inverse (row-major, in place) by Gauss-Jordan with partial pivoting.
+__device__ inline bool inv6(double* m) {
+  double a[6][12];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 12; ++j) a[i][j] = (j < 6) ? m[6 * i + j] : ((j - 6 == i) ? 1.0 : 0.0);
+  for (int c = 0; c < 6; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (fabs(a[r][c]) > fabs(a[p][c])) p = r;
+    if (!(a[p][c] != 0.0) || !isfinite(a[p][c])) return false;
+    if (p != c)
+      for (int j = 0; j < 12; ++j) { double t = a[p][j]; a[p][j] = a[c][j]; a[c][j] = t; }
+    double piv = 1.0 / a[c][c];
+    for (int j = 0; j < 12; ++j) a[c][j] *= piv;
+    for (int r = 0; r < 6; ++r) {
+      if (r == c) continue;
+      double f = a[r][c];
+      for (int j = 0; j < 12; ++j) a[r][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) m[6 * i + j] = a[i][6 + j];
+  return true;
+}
+
+// Neumaier (compensated) accumulation of a double into (s, c).
+__device__ inline void neumaier_add(double& s, double& c, double v) {
+  double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  s = t;
+}
+
+}  // namespace g2ohip

@@ -1,0 +1,113 @@
+// Launch interface of the HIP kernels (kernels.hip). Plain device pointers;
+// every launcher enqueues on the given stream and returns the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace g2ohip {
+
+constexpr int LIN_THREADS = 256;
+constexpr int LIN_CAMS = 128;  // free cameras per linearisation chunk
+constexpr int LIN_PTS = 256;   // points per linearisation chunk
+constexpr int RED_BLOCKS = 512;  // fixed grid of the grid-stride reductions
+
+// Device view of the structure (all arrays device-resident, read-only).
+struct DevGraph {
+  int C, P, Cf, Pf, npos, ns, nlc, nS, nF;
+  double huber;
+  const int *camH, *ptH;
+  const int *pos_pt, *pos_h, *pos_ptr, *pos_nfree;
+  const int *slot_cam, *slot_pos;
+  const double *slot_meas, *slot_w;
+  const int *lc_pos0, *lc_cam_ptr, *lc_cams, *camp_ptr, *camp_idx;
+  const int *S_row, *S_col, *S_diag, *up_pos, *up_posT, *F_ptr, *F_col, *F_diag;
+  const double* intr;
+};
+
+struct DevRowPlan {
+  int ntiles;
+  const int *row_ptr, *row_slot, *tile_row, *tile_k0, *tile_k1, *tile_order;
+  const int* S_rowptr;
+};
+
+// Scalars of one PCG solve, kept on the device so iterations need no host
+// round trip. dn is double-buffered by iteration parity.
+struct PcgScalars {
+  double dn[2];
+  double d0;
+  int done;
+  int iters;
+  int err;
+  int pad;
+};
+
+// ---- state / residuals
+hipError_t launch_cam_table(const DevGraph& g, const double* cams7, double* camRt, hipStream_t st);
+hipError_t launch_chi2(const DevGraph& g, const double* camRt, const double* pts, double* partial, double* out,
+                       double* err_out, hipStream_t st);
+hipError_t launch_update(const DevGraph& g, const double* dx, double* cams7, double* pts, hipStream_t st);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
+
+// ---- linear system
+template <typename HT>
+hipError_t launch_linearize(const DevGraph& g, const double* camRt, const double* pts, HT* Hpl, double* Hll,
+                            double* b, double* hpp_part, double* chunk_chi, hipStream_t st);
+hipError_t launch_reduce_hpp(const DevGraph& g, const double* hpp_part, double* Hpp, double* b, hipStream_t st);
+hipError_t launch_sum(const double* partial, int n, double* out, hipStream_t st);
+hipError_t launch_max_diag(const DevGraph& g, const double* Hpp, const double* Hll, double* partial, double* out,
+                           hipStream_t st);
+hipError_t launch_scale(int n, const double* x, const double* b, double lambda, double* partial, double* out,
+                        hipStream_t st);
+
+// ---- explicit Schur complement (row-owner). diag_only: only the diagonal
+// blocks (no damping on Hpp) into Sdiag[Cf*36] - the implicit-mode
+// preconditioner; otherwise all row blocks into both triangles of F_val and
+// bschur.
+constexpr int SCHUR_MAXC = 128;
+template <typename HT>
+hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const double* Hll,
+                             const double* b, const double* Hpp, double lambda, HT* F_val, double* bschur,
+                             HT* Sdiag, int diag_only, int* err, hipStream_t st);
+
+// ---- implicit Schur operator
+template <typename HT>
+hipError_t launch_point_dinv(const DevGraph& g, const double* Hll, double lambda, HT* Dinv, int* err, hipStream_t st);
+template <typename HT>
+hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const HT* p, const double* b,
+                                double* cam_part, const PcgScalars* sc, int it, hipStream_t st);
+template <typename HT>
+hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, const double* Hpp, double lambda,
+                                  const HT* p, HT* q, double* dq_part, PcgScalars* sc, int it, hipStream_t st);
+hipError_t launch_implicit_rhs_reduce(const DevGraph& g, const double* cam_part, const double* b, double* bschur,
+                                      hipStream_t st);
+
+// ---- FP64 matrix-free residual of the reduced system and back-substitution
+hipError_t launch_refine(const DevGraph& g, const double* camRt, const double* pts, double lambda, const double* xp,
+                         double* xl, double* cam_part, int* err, hipStream_t st);
+hipError_t launch_refine_reduce(const DevGraph& g, const double* cam_part, const double* xp, double lambda,
+                                double* r, hipStream_t st);
+
+// ---- block-Jacobi PCG (LinearSolverPCG semantics), block size BD
+template <typename HT, int BD>
+hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, int* err, hipStream_t st);
+template <typename HT, int BD>
+hipError_t launch_jacobi_damped(int nb, const HT* F_val, const int* F_diag, double lambda, HT* Jinv, int* err,
+                                hipStream_t st);
+template <typename HT, int BD>
+hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* d, double* part,
+                           hipStream_t st);
+hipError_t launch_pcg_start(int nb, const double* part, double tol, double resid_mem, PcgScalars* sc,
+                            hipStream_t st);
+template <typename HT, int BD>
+hipError_t launch_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* d, HT* q,
+                       double* dq_part, PcgScalars* sc, int it, hipStream_t st);
+template <typename HT, int BD>
+hipError_t launch_pcg_step(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
+                           const double* dq_part, double* dn_part, PcgScalars* sc, int it, hipStream_t st);
+template <typename HT, int BD>
+hipError_t launch_pcg_dir(int nb, HT* d, const HT* s, const double* dn_part, PcgScalars* sc, int it,
+                          hipStream_t st);
+template <typename HT>
+hipError_t launch_to_double(int n, const HT* src, double* dst, double scale_add, hipStream_t st);
+hipError_t launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t st);
+
+}  // namespace g2ohip

@@ -1,0 +1,1085 @@
+// HIP kernels of the g2o LM linear-system path on gfx950.
+//
+// Layout in HBM (all built once by structure.cpp, see structure.h):
+//  * slots = active edges, point-major: slot_cam / slot_pos (int32),
+//    slot_meas (2 x f64), slot_w (f64);
+//  * Hpl: one 6x3 block per slot, row-major, Hpl[18 * slot + 3 * row + col]:
+//    a thread owning a slot reads or writes 72 (FP32) / 144 (FP64)
+//    contiguous bytes, and a point's blocks are contiguous;
+//  * Hll: 6 unique f64 per free point; b: f64 [6 Cf + 3 Pf] (g2o's layout);
+//  * Hschur: full symmetric block-CSR (both triangles) F_val[nF][36].
+// Camera-side sums of the linearisation and of the matrix-free passes go
+// through per-chunk partials that a second kernel sums in a fixed order; the
+// Schur complement is formed by the workgroup that owns each block row.
+#include <hip/hip_runtime.h>
+
+#include "devmath.h"
+#include "kernels.h"
+
+namespace g2ohip {
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum of a block's per-thread values (blockDim.x multiple of 64, <= 1024).
+__device__ double block_sum(double v, double* s_red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) s_red[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < nw; ++i) t += s_red[i];
+    s_red[0] = t;
+  }
+  __syncthreads();
+  t = s_red[0];
+  __syncthreads();
+  return t;
+}
+
+// Compensated sum of part[0..n) computed identically by every block.
+__device__ double block_sum_array(const double* __restrict__ part, int n, double* s_red) {
+  double s = 0, c = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) neumaier_add(s, c, part[i]);
+  // combine (s, c) pairs in a fixed tree
+  double* ss = s_red;
+  double* cc = s_red + blockDim.x;
+  ss[threadIdx.x] = s;
+  cc[threadIdx.x] = c;
+  __syncthreads();
+  for (int stride = blockDim.x / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      double a = ss[threadIdx.x], b = ss[threadIdx.x + stride];
+      double t = a + b;
+      double e = (fabs(a) >= fabs(b)) ? ((a - t) + b) : ((b - t) + a);
+      ss[threadIdx.x] = t;
+      cc[threadIdx.x] += cc[threadIdx.x + stride] + e;
+    }
+    __syncthreads();
+  }
+  double r = ss[0] + cc[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int lower_bound_i(const int* a, int n, int key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__constant__ int c_up6_r[21] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5};
+__constant__ int c_up6_c[21] = {0, 1, 2, 3, 4, 5, 1, 2, 3, 4, 5, 2, 3, 4, 5, 3, 4, 5, 4, 5, 5};
+
+inline int nblocks(long long n, int t) { return static_cast<int>((n + t - 1) / t); }
+
+}  // namespace
+
+// =================================================================== state
+__global__ void k_cam_table(int C, const double* __restrict__ cams7, double* __restrict__ camRt) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const double* p = cams7 + 7 * c;
+  double q[4] = {p[3], p[4], p[5], p[6]};
+  double R[9];
+  quat_to_rot(q, R);
+  double* o = camRt + 12 * c;
+  for (int i = 0; i < 9; ++i) o[i] = R[i];
+  o[9] = p[0]; o[10] = p[1]; o[11] = p[2];
+}
+
+hipError_t launch_cam_table(const DevGraph& g, const double* cams7, double* camRt, hipStream_t st) {
+  hipLaunchKernelGGL(k_cam_table, dim3(nblocks(g.C, 256)), dim3(256), 0, st, g.C, cams7, camRt);
+  return hipGetLastError();
+}
+
+// computeActiveErrors + activeRobustChi2 (sparse_optimizer.cpp:78-130):
+// per-slot error, robust chi2, fixed-grid partial sums.
+__global__ __launch_bounds__(256) void k_chi2(DevGraph g, const double* __restrict__ camRt,
+                                              const double* __restrict__ pts, double* __restrict__ partial,
+                                              double* __restrict__ err_out) {
+  __shared__ double s_red[4];
+  double chi = 0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.ns; s += gridDim.x * blockDim.x) {
+    const int cam = g.slot_cam[s];
+    const int praw = g.pos_pt[g.slot_pos[s]];
+    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+    double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
+    double Xc[3], e[2];
+    project_error(camRt + 12 * cam, g.intr + 4 * cam, X, z, Xc, e);
+    double rho[3];
+    huber_rho(g.slot_w[s] * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+    chi += rho[0];
+    if (err_out) { err_out[2 * s] = e[0]; err_out[2 * s + 1] = e[1]; }
+  }
+  chi = block_sum(chi, s_red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = chi;
+}
+
+__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double s_red[512];
+  double v = block_sum_array(part, n, s_red);
+  if (threadIdx.x == 0) *out = v;
+}
+
+hipError_t launch_sum(const double* partial, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, st, partial, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_chi2(const DevGraph& g, const double* camRt, const double* pts, double* partial, double* out,
+                       double* err_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_chi2, dim3(RED_BLOCKS), dim3(256), 0, st, g, camRt, pts, partial, err_out);
+  return launch_sum(partial, RED_BLOCKS, out, st);
+}
+
+// SparseOptimizer::update (sparse_optimizer.cpp:457-468): exp-map left update
+// of free cameras (vertex_se3_expmap.cpp:48-51) and additive point update
+// (vertex_pointxyz.h:47-50), FP64.
+__global__ void k_update(DevGraph g, const double* __restrict__ dx, double* __restrict__ cams7,
+                         double* __restrict__ pts) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < g.C) {
+    int h = g.camH[t];
+    if (h < 0) return;
+    double in[7], out[7];
+    for (int i = 0; i < 7; ++i) in[i] = cams7[7 * t + i];
+    se3_left_update(dx + 6 * h, in, out);
+    for (int i = 0; i < 7; ++i) cams7[7 * t + i] = out[i];
+  } else if (t < g.C + g.P) {
+    int p = t - g.C;
+    int h = g.ptH[p];
+    if (h < 0) return;
+    const double* d = dx + 6 * static_cast<size_t>(g.Cf) + 3 * static_cast<size_t>(h);
+    pts[3 * p] += d[0];
+    pts[3 * p + 1] += d[1];
+    pts[3 * p + 2] += d[2];
+  }
+}
+
+hipError_t launch_update(const DevGraph& g, const double* dx, double* cams7, double* pts, hipStream_t st) {
+  hipLaunchKernelGGL(k_update, dim3(nblocks(static_cast<long long>(g.C) + g.P, 256)), dim3(256), 0, st, g, dx,
+                     cams7, pts);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+}
+
+// ============================================================ linearisation
+// BlockSolver::buildSystem (block_solver.hpp:492-565) for EdgeSE3ProjectXYZ:
+// errors and Huber weights in FP64, Jacobians (edge_project_xyz.cpp:57-93)
+// and block products in HT, constructQuadraticForm
+// (base_fixed_sized_edge.hpp:48-130) with Omega' = rho' w I (robustInformation,
+// base_edge.h:156-162) and b -= ... as +J^T omega_r, omega_r = -rho' w e.
+// One workgroup per chunk of consecutive points: the slot's Hpl block goes to
+// its own slot; Hll / b_l are summed per point and Hpp / b_p per chunk camera
+// with LDS atomics (f64), then written once.
+template <typename HT>
+__global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const double* __restrict__ camRt,
+                                                          const double* __restrict__ pts, HT* __restrict__ Hpl,
+                                                          double* __restrict__ Hll, double* __restrict__ b,
+                                                          double* __restrict__ hpp_part,
+                                                          double* __restrict__ chunk_chi) {
+  __shared__ int s_cam[LIN_CAMS];
+  __shared__ double s_cacc[LIN_CAMS * 27];
+  __shared__ double s_pacc[LIN_PTS * 9];
+  __shared__ double s_red[LIN_THREADS / 64];
+  const int ch = blockIdx.x, tid = threadIdx.x;
+  const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
+  const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
+  const int c0 = g.lc_cam_ptr[ch], nc = g.lc_cam_ptr[ch + 1] - c0;
+  const int np = p1 - p0;
+  for (int i = tid; i < nc; i += LIN_THREADS) s_cam[i] = g.lc_cams[c0 + i];
+  for (int i = tid; i < nc * 27; i += LIN_THREADS) s_cacc[i] = 0.0;
+  for (int i = tid; i < np * 9; i += LIN_THREADS) s_pacc[i] = 0.0;
+  __syncthreads();
+  double chi = 0;
+  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
+    const int pos = g.slot_pos[s], lp = pos - p0;
+    const int praw = g.pos_pt[pos], ph = g.pos_h[pos];
+    const int cam = g.slot_cam[s], chh = g.camH[cam];
+    const double* Rt = camRt + 12 * cam;
+    const double* in = g.intr + 4 * cam;
+    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+    double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
+    const double w = g.slot_w[s];
+    double Xc[3], e[2];
+    project_error(Rt, in, X, z, Xc, e);
+    double rho[3];
+    huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+    chi += rho[0];
+    const double om = rho[1] * w;
+    const double wr0 = -om * e[0], wr1 = -om * e[1];
+    HT R[9];
+    for (int i = 0; i < 9; ++i) R[i] = static_cast<HT>(Rt[i]);
+    HT Jp[6], Jc[12];
+    project_jacobians<HT>(R, static_cast<HT>(in[0]), static_cast<HT>(in[1]), static_cast<HT>(Xc[0]),
+                          static_cast<HT>(Xc[1]), static_cast<HT>(Xc[2]), Jp, Jc);
+    const HT omh = static_cast<HT>(om);
+    if (ph >= 0) {
+      double* pa = s_pacc + 9 * lp;
+      int k = 0;
+      for (int r = 0; r < 3; ++r)
+        for (int c = r; c < 3; ++c)
+          atomicAdd(&pa[k++], static_cast<double>(omh * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c])));
+      for (int r = 0; r < 3; ++r)
+        atomicAdd(&pa[6 + r], static_cast<double>(Jp[r]) * wr0 + static_cast<double>(Jp[3 + r]) * wr1);
+    }
+    if (chh >= 0) {
+      const int lc = lower_bound_i(s_cam, nc, chh);
+      double* ca = s_cacc + 27 * lc;
+      for (int k = 0; k < 21; ++k) {
+        const int r = c_up6_r[k], c = c_up6_c[k];
+        atomicAdd(&ca[k], static_cast<double>(omh * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c])));
+      }
+      for (int r = 0; r < 6; ++r)
+        atomicAdd(&ca[21 + r], static_cast<double>(Jc[r]) * wr0 + static_cast<double>(Jc[6 + r]) * wr1);
+      if (ph >= 0) {
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 3; ++c)
+            Hpl[18 * static_cast<size_t>(s) + 3 * r + c] = omh * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]);
+      }
+    }
+  }
+  chi = block_sum(chi, s_red);
+  if (tid == 0) chunk_chi[ch] = chi;
+  // block_sum ends with a barrier, so the LDS accumulators are complete
+  const size_t boff = 6 * static_cast<size_t>(g.Cf);
+  for (int lp = tid; lp < np; lp += LIN_THREADS) {
+    const int ph = g.pos_h[p0 + lp];
+    if (ph < 0) continue;
+    for (int k = 0; k < 6; ++k) Hll[6 * static_cast<size_t>(ph) + k] = s_pacc[9 * lp + k];
+    for (int r = 0; r < 3; ++r) b[boff + 3 * static_cast<size_t>(ph) + r] = s_pacc[9 * lp + 6 + r];
+  }
+  for (int i = tid; i < nc * 27; i += LIN_THREADS) hpp_part[static_cast<size_t>(c0) * 27 + i] = s_cacc[i];
+}
+
+template <typename HT>
+hipError_t launch_linearize(const DevGraph& g, const double* camRt, const double* pts, HT* Hpl, double* Hll,
+                            double* b, double* hpp_part, double* chunk_chi, hipStream_t st) {
+  if (g.nlc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_linearize<HT>, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hpl, Hll, b, hpp_part,
+                     chunk_chi);
+  return hipGetLastError();
+}
+template hipError_t launch_linearize<float>(const DevGraph&, const double*, const double*, float*, double*, double*,
+                                            double*, double*, hipStream_t);
+template hipError_t launch_linearize<double>(const DevGraph&, const double*, const double*, double*, double*,
+                                             double*, double*, double*, hipStream_t);
+
+// Hpp and b_p: sum of the chunk partials of each free camera, fixed order.
+__global__ void k_reduce_hpp(DevGraph g, const double* __restrict__ hpp_part, double* __restrict__ Hpp,
+                             double* __restrict__ b) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.Cf * 27) return;
+  const int c = t / 27, k = t % 27;
+  double v = 0;
+  for (int q = g.camp_ptr[c]; q < g.camp_ptr[c + 1]; ++q) v += hpp_part[static_cast<size_t>(g.camp_idx[q]) * 27 + k];
+  if (k < 21) {
+    const int r = c_up6_r[k], cc = c_up6_c[k];
+    Hpp[36 * static_cast<size_t>(c) + 6 * r + cc] = v;
+    Hpp[36 * static_cast<size_t>(c) + 6 * cc + r] = v;
+  } else {
+    b[6 * static_cast<size_t>(c) + (k - 21)] = v;
+  }
+}
+
+hipError_t launch_reduce_hpp(const DevGraph& g, const double* hpp_part, double* Hpp, double* b, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_hpp, dim3(nblocks(static_cast<long long>(g.Cf) * 27, 256)), dim3(256), 0, st, g,
+                     hpp_part, Hpp, b);
+  return hipGetLastError();
+}
+
+// computeLambdaInit (optimization_algorithm_levenberg.cpp:155-167): max |diag|.
+__global__ __launch_bounds__(256) void k_max_diag(DevGraph g, const double* __restrict__ Hpp,
+                                                  const double* __restrict__ Hll, double* __restrict__ partial) {
+  __shared__ double s_red[4];
+  double m = 0;
+  const long long n = 6LL * g.Cf + 3LL * g.Pf;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    double v;
+    if (i < 6LL * g.Cf) v = Hpp[36 * (i / 6) + 7 * (i % 6)];
+    else {
+      long long j = i - 6LL * g.Cf;
+      const int dmap[3] = {0, 3, 5};
+      v = Hll[6 * (j / 3) + dmap[j % 3]];
+    }
+    m = fmax(m, fabs(v));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < 4; ++i) t = fmax(t, s_red[i]);
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_max_final(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double s_red[4];
+  double m = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, part[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < 4; ++i) t = fmax(t, s_red[i]);
+    *out = t;
+  }
+}
+
+hipError_t launch_max_diag(const DevGraph& g, const double* Hpp, const double* Hll, double* partial, double* out,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_max_diag, dim3(RED_BLOCKS), dim3(256), 0, st, g, Hpp, Hll, partial);
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, partial, RED_BLOCKS, out);
+  return hipGetLastError();
+}
+
+// computeScale (optimization_algorithm_levenberg.cpp:169-176)
+__global__ __launch_bounds__(256) void k_scale(int n, const double* __restrict__ x, const double* __restrict__ b,
+                                               double lambda, double* __restrict__ partial) {
+  __shared__ double s_red[4];
+  double s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += x[i] * (lambda * x[i] + b[i]);
+  s = block_sum(s, s_red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+hipError_t launch_scale(int n, const double* x, const double* b, double lambda, double* partial, double* out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_scale, dim3(RED_BLOCKS), dim3(256), 0, st, n, x, b, lambda, partial);
+  return launch_sum(partial, RED_BLOCKS, out, st);
+}
+
+// ========================================================== explicit Schur
+// BlockSolver::solve, Schur part (block_solver.hpp:345-433), row-owner form.
+// Hschur(i, j) = Hpp(i, i) [i == j] + lambda I [i == j]
+//              - sum over free points p seen by i and j of Hpl_ip Dinv_p Hpl_jp^T,
+// Dinv_p = (Hll_p + lambda I)^-1 (f64), and bschur_i = b_i - sum_p Hpl_ip Dinv_p b_p.
+// One workgroup per tile of block row i (camera i): each thread takes
+// observations (point p, slot a of camera i) of that camera, forms
+// W = Hpl_a Dinv_p once and the products W Hpl_b^T for the point's slots b with
+// camera >= i. Off-diagonal products go to the row's LDS accumulator; the
+// diagonal block and the coefficient stay in registers and are reduced over
+// the workgroup at the end. Every Hschur block is written once, by its row's
+// tile, to both triangles of the block-CSR copy; nothing is accumulated in
+// global memory.
+template <typename HT, int MAXC>
+__global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, const HT* __restrict__ Hpl,
+                                                    const double* __restrict__ Hll, const double* __restrict__ b,
+                                                    const double* __restrict__ Hpp, double lambda,
+                                                    HT* __restrict__ F_val, double* __restrict__ bschur,
+                                                    HT* __restrict__ Sdiag, int diag_only, int* __restrict__ err) {
+  __shared__ int s_col[MAXC];
+  __shared__ HT s_acc[MAXC * 36];
+  __shared__ double s_red[4 * 42];
+  const int tile = rp.tile_order[blockIdx.x];
+  const int row = rp.tile_row[tile];
+  const int k0 = rp.tile_k0[tile], k1 = diag_only ? rp.tile_k0[tile] + 1 : rp.tile_k1[tile];
+  const int ncol = k1 - k0;
+  const bool diag_tile = (k0 == rp.S_rowptr[row]);
+  if (diag_only && !diag_tile) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < ncol; i += 256) s_col[i] = g.S_col[k0 + i];
+  for (int i = tid; i < ncol * 36; i += 256) s_acc[i] = HT(0);
+  __syncthreads();
+  const int col_lo = s_col[0], col_hi = s_col[ncol - 1];
+  HT dacc[36];
+  for (int e = 0; e < 36; ++e) dacc[e] = HT(0);
+  double coef[6] = {0, 0, 0, 0, 0, 0};
+  const double* bl_base = b + 6 * static_cast<size_t>(g.Cf);
+  for (int e = rp.row_ptr[row] + tid; e < rp.row_ptr[row + 1]; e += 256) {
+    const int a = rp.row_slot[e];
+    const int pos = g.slot_pos[a];
+    const int ph = g.pos_h[pos];
+    const int send = g.pos_ptr[pos] + g.pos_nfree[pos];
+    double D[9];
+    if (!inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
+      atomicOr(err, 1);
+      for (int i = 0; i < 9; ++i) D[i] = 0.0;
+    }
+    HT Ha[18];
+    const HT* pa = Hpl + 18 * static_cast<size_t>(a);
+    for (int k = 0; k < 18; ++k) Ha[k] = pa[k];
+    HT W[18];
+    {
+      HT Dh[9];
+      for (int k = 0; k < 9; ++k) Dh[k] = static_cast<HT>(D[k]);
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 3; ++c) W[3 * r + c] = Ha[3 * r] * Dh[c] + Ha[3 * r + 1] * Dh[3 + c] + Ha[3 * r + 2] * Dh[6 + c];
+    }
+    if (diag_tile && !diag_only) {
+      const double* bl = bl_base + 3 * static_cast<size_t>(ph);
+      double db[3];
+      for (int r = 0; r < 3; ++r) db[r] = D[3 * r] * bl[0] + D[3 * r + 1] * bl[1] + D[3 * r + 2] * bl[2];
+      for (int r = 0; r < 6; ++r)
+        coef[r] += static_cast<double>(Ha[3 * r]) * db[0] + static_cast<double>(Ha[3 * r + 1]) * db[1] +
+                   static_cast<double>(Ha[3 * r + 2]) * db[2];
+    }
+    for (int bs = a; bs < send; ++bs) {
+      const int cb = g.camH[g.slot_cam[bs]];
+      if (cb > col_hi) break;  // slots ascend by camera
+      if (cb < col_lo) continue;
+      HT Hb[18];
+      if (bs == a) {
+        for (int k = 0; k < 18; ++k) Hb[k] = Ha[k];
+      } else {
+        const HT* pb = Hpl + 18 * static_cast<size_t>(bs);
+        for (int k = 0; k < 18; ++k) Hb[k] = pb[k];
+      }
+      HT P[36];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c)
+          P[6 * r + c] = W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2];
+      if (cb == row) {
+        if (bs == a) {
+          for (int k = 0; k < 36; ++k) dacc[k] += P[k];
+        } else {
+          // a second edge of this point to camera i: g2o merges the two into
+          // one Hpl block, whose product also holds the transposed cross term
+          for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) dacc[6 * r + c] += P[6 * r + c] + P[6 * c + r];
+        }
+      } else {
+        const int lc = lower_bound_i(s_col, ncol, cb);
+        HT* acc = s_acc + 36 * lc;
+        for (int k = 0; k < 36; ++k) atomicAdd(&acc[k], -P[k]);
+      }
+    }
+  }
+  // workgroup reduction of the register accumulators (diagonal tile only)
+  if (diag_tile) {
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int k = 0; k < 36; ++k) {
+      double v = wave_sum(static_cast<double>(dacc[k]));
+      if (lane == 0) s_red[42 * wv + k] = v;
+    }
+    for (int k = 0; k < 6; ++k) {
+      double v = wave_sum(coef[k]);
+      if (lane == 0) s_red[42 * wv + 36 + k] = v;
+    }
+  }
+  __syncthreads();
+  if (diag_tile && tid < 42) {
+    const double v = s_red[tid] + s_red[42 + tid] + s_red[84 + tid] + s_red[126 + tid];
+    if (tid < 36) {
+      double h = Hpp[36 * static_cast<size_t>(row) + tid] - v;
+      if (!diag_only && tid % 7 == 0) h += lambda;
+      s_acc[tid] = static_cast<HT>(h);  // column 0 of the diagonal tile is the diagonal
+    } else if (!diag_only) {
+      const int r = tid - 36;
+      bschur[6 * static_cast<size_t>(row) + r] = b[6 * static_cast<size_t>(row) + r] - v;
+    }
+  }
+  __syncthreads();
+  if (diag_only) {
+    for (int i = tid; i < 36; i += 256) Sdiag[36 * static_cast<size_t>(row) + i] = s_acc[i];
+    return;
+  }
+  for (int i = tid; i < ncol * 36; i += 256) {
+    const int lc = i / 36, e = i % 36;
+    const int k = k0 + lc;
+    const HT v = s_acc[i];
+    F_val[36 * static_cast<size_t>(g.up_pos[k]) + e] = v;
+    const int pT = g.up_posT[k];
+    if (pT >= 0) F_val[36 * static_cast<size_t>(pT) + (e % 6) * 6 + e / 6] = v;
+  }
+}
+
+template <typename HT>
+hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const double* Hll,
+                             const double* b, const double* Hpp, double lambda, HT* F_val, double* bschur,
+                             HT* Sdiag, int diag_only, int* err, hipStream_t st) {
+  if (rp.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_schur_rows<HT, SCHUR_MAXC>), dim3(rp.ntiles), dim3(256), 0, st, g, rp, Hpl, Hll, b, Hpp,
+                     lambda, F_val, bschur, Sdiag, diag_only, err);
+  return hipGetLastError();
+}
+template hipError_t launch_schur_rows<float>(const DevGraph&, const DevRowPlan&, const float*, const double*,
+                                             const double*, const double*, double, float*, double*, float*, int, int*,
+                                             hipStream_t);
+template hipError_t launch_schur_rows<double>(const DevGraph&, const DevRowPlan&, const double*, const double*,
+                                              const double*, const double*, double, double*, double*, double*, int,
+                                              int*, hipStream_t);
+
+// ========================================================== implicit Schur
+template <typename HT>
+__global__ void k_point_dinv(DevGraph g, const double* __restrict__ Hll, double lambda, HT* __restrict__ Dinv,
+                             int* __restrict__ err) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.Pf) return;
+  double D[9];
+  if (!inv3_sym(Hll + 6 * static_cast<size_t>(p), lambda, D)) {
+    atomicOr(err, 1);
+    for (int i = 0; i < 9; ++i) D[i] = 0;
+  }
+  for (int i = 0; i < 9; ++i) Dinv[9 * static_cast<size_t>(p) + i] = static_cast<HT>(D[i]);
+}
+
+template <typename HT>
+hipError_t launch_point_dinv(const DevGraph& g, const double* Hll, double lambda, HT* Dinv, int* err, hipStream_t st) {
+  hipLaunchKernelGGL(k_point_dinv<HT>, dim3(nblocks(g.Pf, 256)), dim3(256), 0, st, g, Hll, lambda, Dinv, err);
+  return hipGetLastError();
+}
+template hipError_t launch_point_dinv<float>(const DevGraph&, const double*, double, float*, int*, hipStream_t);
+template hipError_t launch_point_dinv<double>(const DevGraph&, const double*, double, double*, int*, hipStream_t);
+
+// One pass of the matrix-free reduced operator over the linearisation chunks:
+// with p != null: v_l = sum Hpl^T p_c, y = Dinv v_l, part_c += Hpl y;
+// with p == null: y = Dinv b_l (the Schur right-hand side).
+// sc != null: PCG iteration `it` (skipped once converged; the convergence
+// test of linear_solver_pcg.hpp:141 runs here).
+template <typename HT>
+__global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const HT* __restrict__ Hpl,
+                                                              const HT* __restrict__ Dinv, const HT* __restrict__ p,
+                                                              const double* __restrict__ b,
+                                                              double* __restrict__ cam_part, PcgScalars* sc, int it) {
+  if (sc) {
+    if (sc->done) return;
+    if (sc->dn[it & 1] <= sc->d0) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) { sc->done = 1; sc->iters = it; }
+      return;
+    }
+  }
+  __shared__ int s_cam[LIN_CAMS];
+  __shared__ double s_cacc[LIN_CAMS * 6];
+  __shared__ double s_v[LIN_PTS * 3];
+  const int ch = blockIdx.x, tid = threadIdx.x;
+  const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
+  const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
+  const int c0 = g.lc_cam_ptr[ch], nc = g.lc_cam_ptr[ch + 1] - c0;
+  const int np = p1 - p0;
+  for (int i = tid; i < nc; i += LIN_THREADS) s_cam[i] = g.lc_cams[c0 + i];
+  for (int i = tid; i < nc * 6; i += LIN_THREADS) s_cacc[i] = 0.0;
+  for (int i = tid; i < np * 3; i += LIN_THREADS) s_v[i] = 0.0;
+  __syncthreads();
+  if (p) {
+    for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
+      const int pos = g.slot_pos[s];
+      const int chh = g.camH[g.slot_cam[s]];
+      if (g.pos_h[pos] < 0 || chh < 0) continue;
+      const HT* pc = p + 6 * static_cast<size_t>(chh);
+      for (int c = 0; c < 3; ++c) {
+        HT v = 0;
+        for (int r = 0; r < 6; ++r) v += Hpl[18 * static_cast<size_t>(s) + 3 * r + c] * pc[r];
+        atomicAdd(&s_v[3 * (pos - p0) + c], static_cast<double>(v));
+      }
+    }
+    __syncthreads();
+  }
+  for (int lp = tid; lp < np; lp += LIN_THREADS) {
+    const int ph = g.pos_h[p0 + lp];
+    if (ph < 0) continue;
+    const HT* D = Dinv + 9 * static_cast<size_t>(ph);
+    double v[3];
+    if (p) {
+      v[0] = s_v[3 * lp]; v[1] = s_v[3 * lp + 1]; v[2] = s_v[3 * lp + 2];
+    } else {
+      const double* bl = b + 6 * static_cast<size_t>(g.Cf) + 3 * static_cast<size_t>(ph);
+      v[0] = bl[0]; v[1] = bl[1]; v[2] = bl[2];
+    }
+    for (int r = 0; r < 3; ++r)
+      s_v[3 * lp + r] = static_cast<double>(D[3 * r]) * v[0] + static_cast<double>(D[3 * r + 1]) * v[1] +
+                        static_cast<double>(D[3 * r + 2]) * v[2];
+  }
+  __syncthreads();
+  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
+    const int pos = g.slot_pos[s];
+    const int chh = g.camH[g.slot_cam[s]];
+    if (g.pos_h[pos] < 0 || chh < 0) continue;
+    const double* y = s_v + 3 * (pos - p0);
+    const HT y0 = static_cast<HT>(y[0]), y1 = static_cast<HT>(y[1]), y2 = static_cast<HT>(y[2]);
+    const int lc = lower_bound_i(s_cam, nc, chh);
+    for (int r = 0; r < 6; ++r) {
+      const HT* h = Hpl + 18 * static_cast<size_t>(s) + 3 * r;
+      HT v = h[0] * y0 + h[1] * y1 + h[2] * y2;
+      atomicAdd(&s_cacc[6 * lc + r], static_cast<double>(v));
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nc * 6; i += LIN_THREADS) cam_part[static_cast<size_t>(c0) * 6 + i] = s_cacc[i];
+}
+
+template <typename HT>
+hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const HT* p, const double* b,
+                                double* cam_part, const PcgScalars* sc, int it, hipStream_t st) {
+  hipLaunchKernelGGL(k_implicit_pass<HT>, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, Hpl, Dinv, p, b, cam_part,
+                     const_cast<PcgScalars*>(sc), it);
+  return hipGetLastError();
+}
+template hipError_t launch_implicit_pass<float>(const DevGraph&, const float*, const float*, const float*,
+                                                const double*, double*, const PcgScalars*, int, hipStream_t);
+template hipError_t launch_implicit_pass<double>(const DevGraph&, const double*, const double*, const double*,
+                                                 const double*, double*, const PcgScalars*, int, hipStream_t);
+
+// q = (Hpp + lambda I) p - sum of chunk partials; dq_part[c] = q_c . p_c
+template <typename HT>
+__global__ void k_implicit_reduce(DevGraph g, const double* __restrict__ cam_part, const double* __restrict__ Hpp,
+                                  double lambda, const HT* __restrict__ p, HT* __restrict__ q,
+                                  double* __restrict__ dq_part, PcgScalars* sc) {
+  if (sc->done) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.Cf) return;
+  double pc[6], out[6];
+  for (int r = 0; r < 6; ++r) pc[r] = static_cast<double>(p[6 * static_cast<size_t>(c) + r]);
+  for (int r = 0; r < 6; ++r) {
+    double v = lambda * pc[r];
+    for (int k = 0; k < 6; ++k) v += Hpp[36 * static_cast<size_t>(c) + 6 * r + k] * pc[k];
+    out[r] = v;
+  }
+  for (int q2 = g.camp_ptr[c]; q2 < g.camp_ptr[c + 1]; ++q2) {
+    const double* pp = cam_part + 6 * static_cast<size_t>(g.camp_idx[q2]);
+    for (int r = 0; r < 6; ++r) out[r] -= pp[r];
+  }
+  double dq = 0;
+  for (int r = 0; r < 6; ++r) {
+    HT v = static_cast<HT>(out[r]);
+    q[6 * static_cast<size_t>(c) + r] = v;
+    dq += static_cast<double>(v) * pc[r];
+  }
+  dq_part[c] = dq;
+}
+
+template <typename HT>
+hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, const double* Hpp, double lambda,
+                                  const HT* p, HT* q, double* dq_part, PcgScalars* sc, int it, hipStream_t st) {
+  (void)it;
+  hipLaunchKernelGGL(k_implicit_reduce<HT>, dim3(nblocks(g.Cf, 256)), dim3(256), 0, st, g, cam_part, Hpp, lambda, p,
+                     q, dq_part, sc);
+  return hipGetLastError();
+}
+template hipError_t launch_implicit_reduce<float>(const DevGraph&, const double*, const double*, double, const float*,
+                                                  float*, double*, PcgScalars*, int, hipStream_t);
+template hipError_t launch_implicit_reduce<double>(const DevGraph&, const double*, const double*, double,
+                                                   const double*, double*, double*, PcgScalars*, int, hipStream_t);
+
+__global__ void k_implicit_rhs_reduce(DevGraph g, const double* __restrict__ cam_part, const double* __restrict__ b,
+                                      double* __restrict__ bschur) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * g.Cf) return;
+  const int c = t / 6, r = t % 6;
+  double v = b[t];
+  for (int q = g.camp_ptr[c]; q < g.camp_ptr[c + 1]; ++q) v -= cam_part[6 * static_cast<size_t>(g.camp_idx[q]) + r];
+  bschur[t] = v;
+}
+
+hipError_t launch_implicit_rhs_reduce(const DevGraph& g, const double* cam_part, const double* b, double* bschur,
+                                      hipStream_t st) {
+  hipLaunchKernelGGL(k_implicit_rhs_reduce, dim3(nblocks(6LL * g.Cf, 256)), dim3(256), 0, st, g, cam_part, b, bschur);
+  return hipGetLastError();
+}
+
+// ======================================= FP64 matrix-free residual / back-sub
+// Re-linearises every edge in FP64 from the state. For the camera update xp:
+//   u_l = (Hll + lambda I)^-1 sum_e -w' Jp^T (e + Jc xp_c)   (= x_l, the
+//         back-substitution of block_solver.hpp:452-476, written to xl),
+//   part_c += sum_e -w' Jc^T (e + Jc xp_c + Jp u_l)           (if cam_part)
+// so that r_c = -lambda xp_c + sum part_c is the camera block of
+// b - (H + lambda I) [xp; x_l] with the point block exactly zero: the FP64
+// residual of the reduced system used for iterative refinement.
+__global__ __launch_bounds__(LIN_THREADS) void k_refine(DevGraph g, const double* __restrict__ camRt,
+                                                       const double* __restrict__ pts, double lambda,
+                                                       const double* __restrict__ xp, double* __restrict__ xl,
+                                                       double* __restrict__ cam_part, int* __restrict__ err) {
+  __shared__ int s_cam[LIN_CAMS];
+  __shared__ double s_cacc[LIN_CAMS * 6];
+  __shared__ double s_pacc[LIN_PTS * 9];
+  const int ch = blockIdx.x, tid = threadIdx.x;
+  const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
+  const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
+  const int c0 = g.lc_cam_ptr[ch], nc = g.lc_cam_ptr[ch + 1] - c0;
+  const int np = p1 - p0;
+  for (int i = tid; i < nc; i += LIN_THREADS) s_cam[i] = g.lc_cams[c0 + i];
+  for (int i = tid; i < nc * 6; i += LIN_THREADS) s_cacc[i] = 0.0;
+  for (int i = tid; i < np * 9; i += LIN_THREADS) s_pacc[i] = 0.0;
+  __syncthreads();
+  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
+    const int pos = g.slot_pos[s];
+    const int ph = g.pos_h[pos];
+    if (ph < 0) continue;
+    const int praw = g.pos_pt[pos];
+    const int cam = g.slot_cam[s], chh = g.camH[cam];
+    const double* Rt = camRt + 12 * cam;
+    const double* in = g.intr + 4 * cam;
+    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+    double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
+    const double w = g.slot_w[s];
+    double Xc[3], e[2];
+    project_error(Rt, in, X, z, Xc, e);
+    double rho[3];
+    huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+    const double om = rho[1] * w;
+    double Jp[6], Jc[12];
+    project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+    double t0 = e[0], t1 = e[1];
+    if (chh >= 0 && xp) {
+      const double* x = xp + 6 * static_cast<size_t>(chh);
+      for (int k = 0; k < 6; ++k) { t0 += Jc[k] * x[k]; t1 += Jc[6 + k] * x[k]; }
+    }
+    double* pa = s_pacc + 9 * (pos - p0);
+    int k = 0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = r; c < 3; ++c) atomicAdd(&pa[k++], om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]));
+    for (int r = 0; r < 3; ++r) atomicAdd(&pa[6 + r], -om * (Jp[r] * t0 + Jp[3 + r] * t1));
+  }
+  __syncthreads();
+  for (int lp = tid; lp < np; lp += LIN_THREADS) {
+    const int ph = g.pos_h[p0 + lp];
+    double* pa = s_pacc + 9 * lp;
+    if (ph < 0) { pa[6] = pa[7] = pa[8] = 0.0; continue; }
+    double D[9];
+    if (!inv3_sym(pa, lambda, D)) {
+      atomicOr(err, 1);
+      for (int i = 0; i < 9; ++i) D[i] = 0;
+    }
+    double u[3];
+    for (int r = 0; r < 3; ++r) u[r] = D[3 * r] * pa[6] + D[3 * r + 1] * pa[7] + D[3 * r + 2] * pa[8];
+    for (int r = 0; r < 3; ++r) {
+      xl[3 * static_cast<size_t>(ph) + r] = u[r];
+      pa[6 + r] = u[r];
+    }
+  }
+  if (!cam_part) return;
+  __syncthreads();
+  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
+    const int cam = g.slot_cam[s], chh = g.camH[cam];
+    if (chh < 0) continue;
+    const int pos = g.slot_pos[s];
+    const int praw = g.pos_pt[pos];
+    const double* Rt = camRt + 12 * cam;
+    const double* in = g.intr + 4 * cam;
+    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+    double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
+    const double w = g.slot_w[s];
+    double Xc[3], e[2];
+    project_error(Rt, in, X, z, Xc, e);
+    double rho[3];
+    huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+    const double om = rho[1] * w;
+    double Jp[6], Jc[12];
+    project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+    double t0 = e[0], t1 = e[1];
+    if (xp) {
+      const double* x = xp + 6 * static_cast<size_t>(chh);
+      for (int k = 0; k < 6; ++k) { t0 += Jc[k] * x[k]; t1 += Jc[6 + k] * x[k]; }
+    }
+    const double* u = s_pacc + 9 * (pos - p0) + 6;
+    for (int k = 0; k < 3; ++k) { t0 += Jp[k] * u[k]; t1 += Jp[3 + k] * u[k]; }
+    const int lc = lower_bound_i(s_cam, nc, chh);
+    for (int r = 0; r < 6; ++r) atomicAdd(&s_cacc[6 * lc + r], -om * (Jc[r] * t0 + Jc[6 + r] * t1));
+  }
+  __syncthreads();
+  for (int i = tid; i < nc * 6; i += LIN_THREADS) cam_part[static_cast<size_t>(c0) * 6 + i] = s_cacc[i];
+}
+
+hipError_t launch_refine(const DevGraph& g, const double* camRt, const double* pts, double lambda, const double* xp,
+                         double* xl, double* cam_part, int* err, hipStream_t st) {
+  if (g.nlc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_refine, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, lambda, xp, xl, cam_part, err);
+  return hipGetLastError();
+}
+
+__global__ void k_refine_reduce(DevGraph g, const double* __restrict__ cam_part, const double* __restrict__ xp,
+                                double lambda, double* __restrict__ r) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * g.Cf) return;
+  const int c = t / 6, k = t % 6;
+  double v = xp ? -lambda * xp[t] : 0.0;
+  for (int q = g.camp_ptr[c]; q < g.camp_ptr[c + 1]; ++q) v += cam_part[6 * static_cast<size_t>(g.camp_idx[q]) + k];
+  r[t] = v;
+}
+
+hipError_t launch_refine_reduce(const DevGraph& g, const double* cam_part, const double* xp, double lambda, double* r,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_refine_reduce, dim3(nblocks(6LL * g.Cf, 256)), dim3(256), 0, st, g, cam_part, xp, lambda, r);
+  return hipGetLastError();
+}
+
+// ============================================================ block-Jacobi PCG
+// LinearSolverPCG::solve (linear_solver_pcg.hpp:87-161) as a stream of
+// kernels per iteration. Vectors and the matrix in HT, dot products in f64 with
+// per-row partials summed (compensated, fixed order) by every consumer block.
+template <int BD>
+__device__ bool invBD(double* m) {
+  double a[BD][2 * BD];
+  for (int i = 0; i < BD; ++i)
+    for (int j = 0; j < 2 * BD; ++j) a[i][j] = (j < BD) ? m[BD * i + j] : ((j - BD == i) ? 1.0 : 0.0);
+  for (int c = 0; c < BD; ++c) {
+    int p = c;
+    for (int r = c + 1; r < BD; ++r)
+      if (fabs(a[r][c]) > fabs(a[p][c])) p = r;
+    if (!(a[p][c] != 0.0) || !isfinite(a[p][c])) return false;
+    if (p != c)
+      for (int j = 0; j < 2 * BD; ++j) { double t = a[p][j]; a[p][j] = a[c][j]; a[c][j] = t; }
+    double piv = 1.0 / a[c][c];
+    for (int j = 0; j < 2 * BD; ++j) a[c][j] *= piv;
+    for (int r = 0; r < BD; ++r) {
+      if (r == c) continue;
+      double f = a[r][c];
+      for (int j = 0; j < 2 * BD; ++j) a[r][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < BD; ++i)
+    for (int j = 0; j < BD; ++j) m[BD * i + j] = a[i][BD + j];
+  return true;
+}
+
+template <typename HT, int BD>
+__global__ void k_jacobi(int nb, const HT* __restrict__ F_val, const int* __restrict__ F_diag, HT* __restrict__ Jinv,
+                         double lambda, int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  double m[BD * BD];
+  const HT* src = F_val + static_cast<size_t>(F_diag[i]) * BD * BD;
+  for (int k = 0; k < BD * BD; ++k) m[k] = static_cast<double>(src[k]);
+  for (int k = 0; k < BD; ++k) m[k * (BD + 1)] += lambda;
+  if (!invBD<BD>(m)) {
+    atomicOr(err, 2);
+    for (int k = 0; k < BD * BD; ++k) m[k] = 0;
+  }
+  for (int k = 0; k < BD * BD; ++k) Jinv[static_cast<size_t>(i) * BD * BD + k] = static_cast<HT>(m[k]);
+}
+
+template <typename HT, int BD>
+hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, int* err, hipStream_t st) {
+  hipLaunchKernelGGL((k_jacobi<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, F_val, F_diag, Jinv, 0.0, err);
+  return hipGetLastError();
+}
+
+template <typename HT, int BD>
+hipError_t launch_jacobi_damped(int nb, const HT* F_val, const int* F_diag, double lambda, HT* Jinv, int* err,
+                                hipStream_t st) {
+  hipLaunchKernelGGL((k_jacobi<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, F_val, F_diag, Jinv, lambda,
+                     err);
+  return hipGetLastError();
+}
+template hipError_t launch_jacobi_damped<float, 6>(int, const float*, const int*, double, float*, int*, hipStream_t);
+template hipError_t launch_jacobi_damped<double, 6>(int, const double*, const int*, double, double*, int*,
+                                                    hipStream_t);
+
+template <typename HT, int BD>
+__global__ void k_pcg_init(int nb, const double* __restrict__ b, const HT* __restrict__ J, HT* __restrict__ x,
+                           HT* __restrict__ r, HT* __restrict__ d, double* __restrict__ part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  HT rv[BD];
+  for (int k = 0; k < BD; ++k) {
+    rv[k] = static_cast<HT>(b[BD * static_cast<size_t>(i) + k]);
+    r[BD * static_cast<size_t>(i) + k] = rv[k];
+    x[BD * static_cast<size_t>(i) + k] = HT(0);
+  }
+  double dot = 0;
+  const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
+  for (int k = 0; k < BD; ++k) {
+    HT v = 0;
+    for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+    d[BD * static_cast<size_t>(i) + k] = v;
+    dot += static_cast<double>(rv[k]) * static_cast<double>(v);
+  }
+  part[i] = dot;
+}
+
+template <typename HT, int BD>
+hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* d, double* part,
+                           hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_init<HT, BD>), dim3(nblocks(nb, 256)), dim3(256), 0, st, nb, b, Jinv, x, r, d, part);
+  return hipGetLastError();
+}
+
+// dn = r . M^-1 r; d0 = tol * dn, floored by the previous solve's residual
+// memory (linear_solver_pcg.hpp:128-136).
+__global__ __launch_bounds__(256) void k_pcg_start(int nb, const double* __restrict__ part, double tol,
+                                                   double resid_mem, PcgScalars* sc) {
+  __shared__ double s_red[512];
+  double dn = block_sum_array(part, nb, s_red);
+  if (threadIdx.x == 0) {
+    double d0 = tol * dn;
+    if (resid_mem > 0.0 && resid_mem > d0) d0 = resid_mem;
+    sc->dn[0] = dn;
+    sc->dn[1] = dn;
+    sc->d0 = d0;
+    sc->done = 0;
+    sc->iters = -1;
+  }
+}
+
+hipError_t launch_pcg_start(int nb, const double* part, double tol, double resid_mem, PcgScalars* sc,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, st, nb, part, tol, resid_mem, sc);
+  return hipGetLastError();
+}
+
+// q = A d over the full block rows: one wavefront per block row, lanes split
+// into 64 / BD groups that stride over the row's blocks; lane k of a group
+// owns output row k.
+template <typename HT, int BD>
+__global__ __launch_bounds__(256) void k_spmv(int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col,
+                                              const HT* __restrict__ F_val, const HT* __restrict__ d,
+                                              HT* __restrict__ q, double* __restrict__ dq_part, PcgScalars* sc,
+                                              int it) {
+  if (sc->done) return;
+  if (sc->dn[it & 1] <= sc->d0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { sc->done = 1; sc->iters = it; }
+    return;
+  }
+  constexpr int G = 64 / BD;
+  __shared__ HT s_part[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = blockIdx.x * 4 + wv;
+  const int grp = lane / BD, k = lane % BD;
+  HT acc = 0;
+  if (row < nb && grp < G) {
+    for (int p = F_ptr[row] + grp; p < F_ptr[row + 1]; p += G) {
+      const HT* V = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
+      const HT* dj = d + BD * static_cast<size_t>(F_col[p]);
+      for (int c = 0; c < BD; ++c) acc += V[c] * dj[c];
+    }
+  }
+  s_part[wv][lane] = acc;
+  __syncthreads();
+  if (row < nb && lane < BD) {
+    HT v = 0;
+    for (int g2 = 0; g2 < G; ++g2) v += s_part[wv][g2 * BD + lane];
+    q[BD * static_cast<size_t>(row) + lane] = v;
+    s_part[wv][lane] = v;  // each lane only rewrites its own entry
+  }
+  __syncthreads();
+  if (row < nb && lane == 0) {
+    double dot = 0;
+    for (int c = 0; c < BD; ++c)
+      dot += static_cast<double>(s_part[wv][c]) * static_cast<double>(d[BD * static_cast<size_t>(row) + c]);
+    dq_part[row] = dot;
+  }
+}
+
+template <typename HT, int BD>
+hipError_t launch_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* d, HT* q,
+                       double* dq_part, PcgScalars* sc, int it, hipStream_t st) {
+  hipLaunchKernelGGL((k_spmv<HT, BD>), dim3(nblocks(nb, 4)), dim3(256), 0, st, nb, F_ptr, F_col, F_val, d, q,
+                     dq_part, sc, it);
+  return hipGetLastError();
+}
+
+// a = dn / d.q; x += a d; r -= a q; s = M^-1 r; dn_part = r . s
+template <typename HT, int BD>
+__global__ __launch_bounds__(256) void k_pcg_step(int nb, const HT* __restrict__ J, HT* __restrict__ x,
+                                                  HT* __restrict__ r, const HT* __restrict__ d,
+                                                  const HT* __restrict__ q, HT* __restrict__ s,
+                                                  const double* __restrict__ dq_part, double* __restrict__ dn_part,
+                                                  PcgScalars* sc, int it) {
+  if (sc->done) return;
+  __shared__ double s_red[512];
+  const double dq = block_sum_array(dq_part, nb, s_red);
+  const double a = sc->dn[it & 1] / dq;
+  const HT ah = static_cast<HT>(a);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  HT rv[BD];
+  for (int k = 0; k < BD; ++k) {
+    const size_t o = BD * static_cast<size_t>(i) + k;
+    x[o] += ah * d[o];
+    rv[k] = r[o] - ah * q[o];
+    r[o] = rv[k];
+  }
+  const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
+  double dot = 0;
+  for (int k = 0; k < BD; ++k) {
+    HT v = 0;
+    for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+    s[BD * static_cast<size_t>(i) + k] = v;
+    dot += static_cast<double>(rv[k]) * static_cast<double>(v);
+  }
+  dn_part[i] = dot;
+}
+
+template <typename HT, int BD>
+hipError_t launch_pcg_step(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
+                           const double* dq_part, double* dn_part, PcgScalars* sc, int it, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_step<HT, BD>), dim3(nblocks(nb, 256)), dim3(256), 0, st, nb, Jinv, x, r, d, q, s,
+                     dq_part, dn_part, sc, it);
+  return hipGetLastError();
+}
+
+// b = dn_new / dn; d = s + b d
+template <typename HT, int BD>
+__global__ __launch_bounds__(256) void k_pcg_dir(int nb, HT* __restrict__ d, const HT* __restrict__ s,
+                                                 const double* __restrict__ dn_part, PcgScalars* sc, int it) {
+  if (sc->done) return;
+  __shared__ double s_red[512];
+  const double dn_new = block_sum_array(dn_part, nb, s_red);
+  const double be = dn_new / sc->dn[it & 1];
+  const HT bh = static_cast<HT>(be);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->dn[(it + 1) & 1] = dn_new;
+  if (i >= nb * BD) return;
+  d[i] = s[i] + bh * d[i];
+}
+
+template <typename HT, int BD>
+hipError_t launch_pcg_dir(int nb, HT* d, const HT* s, const double* dn_part, PcgScalars* sc, int it,
+                          hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_dir<HT, BD>), dim3(nblocks(static_cast<long long>(nb) * BD, 256)), dim3(256), 0, st, nb,
+                     d, s, dn_part, sc, it);
+  return hipGetLastError();
+}
+
+#define G2OHIP_PCG_INST(HT, BD)                                                                                     \
+  template hipError_t launch_jacobi<HT, BD>(int, const HT*, const int*, HT*, int*, hipStream_t);                   \
+  template hipError_t launch_pcg_init<HT, BD>(int, const double*, const HT*, HT*, HT*, HT*, double*, hipStream_t); \
+  template hipError_t launch_spmv<HT, BD>(int, const int*, const int*, const HT*, const HT*, HT*, double*,         \
+                                          PcgScalars*, int, hipStream_t);                                          \
+  template hipError_t launch_pcg_step<HT, BD>(int, const HT*, HT*, HT*, const HT*, const HT*, HT*, const double*,  \
+                                              double*, PcgScalars*, int, hipStream_t);                             \
+  template hipError_t launch_pcg_dir<HT, BD>(int, HT*, const HT*, const double*, PcgScalars*, int, hipStream_t);
+G2OHIP_PCG_INST(float, 6)
+G2OHIP_PCG_INST(double, 6)
+G2OHIP_PCG_INST(double, 3)
+#undef G2OHIP_PCG_INST
+
+template <typename HT>
+__global__ void k_to_double(int n, const HT* __restrict__ src, double* __restrict__ dst, double scale_add) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // scale_add == 0: copy; == 1: accumulate
+  dst[i] = (scale_add != 0.0 ? dst[i] : 0.0) + static_cast<double>(src[i]);
+}
+
+template <typename HT>
+hipError_t launch_to_double(int n, const HT* src, double* dst, double scale_add, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_double<HT>, dim3(nblocks(n, 256)), dim3(256), 0, st, n, src, dst, scale_add);
+  return hipGetLastError();
+}
+template hipError_t launch_to_double<float>(int, const float*, double*, double, hipStream_t);
+template hipError_t launch_to_double<double>(int, const double*, double*, double, hipStream_t);
+
+__global__ void k_axpy_d(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+hipError_t launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_d, dim3(nblocks(n, 256)), dim3(256), 0, st, n, a, x, y);
+  return hipGetLastError();
+}
+
+}  // namespace g2ohip

@@ -1,0 +1,283 @@
+"""Seeded synthetic bundle-adjustment and pose-graph problems.
+
+Problems are plain numpy arrays in the layout the C-ABI takes:
+
+* ``cams``  float64[C, 7]  world-to-camera SE3 as [tx ty tz qx qy qz qw]
+  (the ``SE3Quat::toVector`` order, g2o/types/slam3d/se3quat.h);
+* ``intr``  float64[C, 4]  fx fy cx cy per camera (EdgeSE3ProjectXYZ keeps
+  them per edge, g2o/types/sba/edge_project_xyz.h:53; here they are per camera);
+* ``pts``   float64[P, 3];
+* ``edge_cam``/``edge_pt`` int32[E], ``meas`` float64[E, 2], ``weight`` float64[E]
+  (information matrix ``weight * I2``);
+* ``cam_fixed``/``pt_fixed`` uint8 masks, ``huber`` the Huber delta (0 = none).
+
+The generators follow the shapes of g2o's own demos (ba_demo.cpp:128-227 for the
+small BA case, create_sphere.cpp:42-241 for the pose graph) but are this
+package's own, seeded through ``numpy.random.default_rng``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+IMG_W, IMG_H = 640.0, 480.0
+
+
+@dataclass
+class BAProblem:
+    cams: np.ndarray
+    intr: np.ndarray
+    pts: np.ndarray
+    edge_cam: np.ndarray
+    edge_pt: np.ndarray
+    meas: np.ndarray
+    weight: np.ndarray
+    cam_fixed: np.ndarray
+    pt_fixed: np.ndarray
+    huber: float
+    name: str = ""
+    gt_cams: np.ndarray | None = None
+    gt_pts: np.ndarray | None = None
+    info: dict = field(default_factory=dict)
+
+    @property
+    def num_cams(self) -> int:
+        return int(self.cams.shape[0])
+
+    @property
+    def num_pts(self) -> int:
+        return int(self.pts.shape[0])
+
+    @property
+    def num_edges(self) -> int:
+        return int(self.edge_cam.shape[0])
+
+    def copy(self) -> "BAProblem":
+        return BAProblem(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                            for k, v in self.__dict__.items()})
+
+
+# ----------------------------------------------------------------- rotations
+def quat_to_rot(q: np.ndarray) -> np.ndarray:
+    """Rotation matrices of unit quaternions [x, y, z, w] (..., 4) -> (..., 3, 3)."""
+    x, y, z, w = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    R = np.empty(q.shape[:-1] + (3, 3))
+    R[..., 0, 0] = 1 - 2 * (y * y + z * z)
+    R[..., 0, 1] = 2 * (x * y - z * w)
+    R[..., 0, 2] = 2 * (x * z + y * w)
+    R[..., 1, 0] = 2 * (x * y + z * w)
+    R[..., 1, 1] = 1 - 2 * (x * x + z * z)
+    R[..., 1, 2] = 2 * (y * z - x * w)
+    R[..., 2, 0] = 2 * (x * z - y * w)
+    R[..., 2, 1] = 2 * (y * z + x * w)
+    R[..., 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def rot_to_quat(R: np.ndarray) -> np.ndarray:
+    """Unit quaternions [x, y, z, w] with w >= 0 of rotation matrices (..., 3, 3)."""
+    R = np.asarray(R, dtype=np.float64)
+    shp = R.shape[:-2]
+    R = R.reshape(-1, 3, 3)
+    q = np.empty((R.shape[0], 4))
+    tr = R[:, 0, 0] + R[:, 1, 1] + R[:, 2, 2]
+    for n in range(R.shape[0]):
+        m = R[n]
+        if tr[n] > 0:
+            s = np.sqrt(tr[n] + 1.0) * 2
+            q[n] = [(m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s, 0.25 * s]
+        else:
+            i = int(np.argmax([m[0, 0], m[1, 1], m[2, 2]]))
+            j, k = (i + 1) % 3, (i + 2) % 3
+            s = np.sqrt(1.0 + m[i, i] - m[j, j] - m[k, k]) * 2
+            v = np.empty(3)
+            v[i] = 0.25 * s
+            v[j] = (m[j, i] + m[i, j]) / s
+            v[k] = (m[k, i] + m[i, k]) / s
+            q[n] = [v[0], v[1], v[2], (m[k, j] - m[j, k]) / s]
+    q *= np.where(q[:, 3:4] < 0, -1.0, 1.0)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q.reshape(shp + (4,))
+
+
+def axis_angle_to_rot(w: np.ndarray) -> np.ndarray:
+    """Rodrigues' formula, (..., 3) -> (..., 3, 3)."""
+    w = np.asarray(w, dtype=np.float64)
+    th = np.linalg.norm(w, axis=-1)[..., None, None]
+    K = np.zeros(w.shape[:-1] + (3, 3))
+    K[..., 0, 1], K[..., 0, 2] = -w[..., 2], w[..., 1]
+    K[..., 1, 0], K[..., 1, 2] = w[..., 2], -w[..., 0]
+    K[..., 2, 0], K[..., 2, 1] = -w[..., 1], w[..., 0]
+    small = th < 1e-12
+    ths = np.where(small, 1.0, th)
+    a = np.where(small, 1.0, np.sin(ths) / ths)
+    b = np.where(small, 0.5, (1 - np.cos(ths)) / (ths * ths))
+    return np.eye(3) + a * K + b * (K @ K)
+
+
+def pose_from_Rt(R: np.ndarray, t: np.ndarray) -> np.ndarray:
+    return np.concatenate([t, rot_to_quat(R)], axis=-1)
+
+
+def pose_to_Rt(p: np.ndarray):
+    return quat_to_rot(p[..., 3:7]), p[..., 0:3]
+
+
+def look_rotation(fwd: np.ndarray, up_hint=np.array([0.0, 0.0, 1.0])) -> np.ndarray:
+    """World-to-camera rotations for cameras looking along ``fwd`` (x right,
+    y down, z forward)."""
+    z = fwd / np.linalg.norm(fwd, axis=-1, keepdims=True)
+    x = np.cross(z, up_hint)
+    x /= np.linalg.norm(x, axis=-1, keepdims=True)
+    y = np.cross(z, x)
+    Rwc = np.stack([x, y, z], axis=-1)  # columns are the camera axes in world
+    return np.swapaxes(Rwc, -1, -2)
+
+
+def project(cams: np.ndarray, intr: np.ndarray, pts: np.ndarray, ec: np.ndarray, ep: np.ndarray):
+    R, t = pose_to_Rt(cams)
+    Xc = np.einsum("eij,ej->ei", R[ec], pts[ep]) + t[ec]
+    u = Xc[:, 0] / Xc[:, 2] * intr[ec, 0] + intr[ec, 2]
+    v = Xc[:, 1] / Xc[:, 2] * intr[ec, 1] + intr[ec, 3]
+    return np.stack([u, v], axis=1), Xc[:, 2]
+
+
+def perturb_poses(rng, cams: np.ndarray, sig_t: float, sig_r: float) -> np.ndarray:
+    """Left-multiplied perturbation exp(delta) * T with delta ~ N(0, diag)."""
+    R, t = pose_to_Rt(cams)
+    dR = axis_angle_to_rot(rng.normal(0, sig_r, size=(cams.shape[0], 3)))
+    dt = rng.normal(0, sig_t, size=(cams.shape[0], 3))
+    R2 = dR @ R
+    t2 = np.einsum("cij,cj->ci", dR, t) + dt
+    return pose_from_Rt(R2, t2)
+
+
+# ------------------------------------------------------------------- BA cases
+def make_small_ba(seed: int = 1000, num_cams: int = 16, num_pts: int = 500, huber: float = 0.0) -> BAProblem:
+    """Small BA: cameras on a circle of radius 15 looking inward, points uniform
+    in a 10 x 10 x 5 box, fx = fy = 500, (cx, cy) = (320, 240), 640 x 480 image,
+    in-image and in-front projections kept, pixel noise N(0, 1); poses perturbed
+    by sigma_t = 0.1, sigma_r = 0.02 rad, points by sigma = 0.2; the first two
+    cameras fixed (the ba_demo.cpp:154-160 gauge)."""
+    rng = np.random.default_rng(seed)
+    ang = np.arange(num_cams) * 2 * np.pi / num_cams
+    centers = np.stack([15 * np.cos(ang), 15 * np.sin(ang), np.zeros(num_cams)], axis=1)
+    R = look_rotation(-centers)
+    t = -np.einsum("cij,cj->ci", R, centers)
+    gt_cams = pose_from_Rt(R, t)
+    intr = np.tile([500.0, 500.0, 320.0, 240.0], (num_cams, 1))
+    gt_pts = rng.uniform([-5, -5, -2.5], [5, 5, 2.5], size=(num_pts, 3))
+    ec, ep = np.meshgrid(np.arange(num_cams), np.arange(num_pts), indexing="ij")
+    ec, ep = ec.ravel().astype(np.int32), ep.ravel().astype(np.int32)
+    uv, depth = project(gt_cams, intr, gt_pts, ec, ep)
+    keep = (depth > 0) & (uv[:, 0] >= 0) & (uv[:, 1] >= 0) & (uv[:, 0] < IMG_W) & (uv[:, 1] < IMG_H)
+    # points seen by fewer than two cameras are dropped (ba_demo.cpp:186)
+    cnt = np.bincount(ep[keep], minlength=num_pts)
+    keep &= cnt[ep] >= 2
+    used = np.flatnonzero(cnt >= 2)
+    remap = -np.ones(num_pts, dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    ec, ep, uv = ec[keep], remap[ep[keep]].astype(np.int32), uv[keep]
+    gt_pts = gt_pts[used]
+    meas = uv + rng.normal(0, 1.0, size=uv.shape)
+    cams = perturb_poses(rng, gt_cams, 0.1, 0.02)
+    cam_fixed = np.zeros(num_cams, dtype=np.uint8)
+    cam_fixed[:2] = 1
+    cams[:2] = gt_cams[:2]
+    pts = gt_pts + rng.normal(0, 0.2, size=gt_pts.shape)
+    return BAProblem(cams, intr, pts, ec.astype(np.int32), ep, meas, np.ones(ec.size), cam_fixed,
+                     np.zeros(gt_pts.shape[0], dtype=np.uint8), huber, name="small_ba", gt_cams=gt_cams,
+                     gt_pts=gt_pts)
+
+
+def make_walk_ba(seed: int = 1002, num_cams: int = 1500, num_pts: int = 500_000, mean_track: float = 6.0,
+                 outlier_ratio: float = 0.05, huber: float = 2.0, weights: tuple | None = None,
+                 step: float = 2.0, sig_t: float = 0.05, sig_r: float = 0.002, sig_p: float = 0.1,
+                 shuffle: bool = True) -> BAProblem:
+    """Large BA along a camera random walk (``step`` metres per camera, 3 km
+    for 1500 cameras). Per-camera fx ~ U(400, 800) = fy, principal point at the
+    centre of a 640 x 480 image. Each point is placed ahead of the last camera
+    of a track of consecutive cameras whose length is 2 + Geometric with mean
+    ``mean_track``; projections behind a camera or outside the image are
+    dropped (points left with < 2 are dropped). Pixel noise N(0, 1);
+    ``outlier_ratio`` of the observations are replaced by uniform image points.
+    Initial poses are perturbed by (sig_t, sig_r), points by sig_p; the first
+    two cameras are fixed (gauge incl. scale). ``weights=(lo, hi)`` draws
+    per-edge information weights U(lo, hi), else 1."""
+    rng = np.random.default_rng(seed)
+    C = num_cams
+    heading = np.cumsum(rng.normal(0, 0.02, size=C)) + 0.3 * np.sin(np.arange(C) / 150.0)
+    pos = np.zeros((C, 3))
+    pos[1:, 0] = np.cumsum(step * np.cos(heading[:-1]))
+    pos[1:, 1] = np.cumsum(step * np.sin(heading[:-1]))
+    pos[:, 2] = 1.5 + 0.3 * np.sin(np.arange(C) / 40.0)
+    yaw = heading + rng.normal(0, 0.03, size=C)
+    pitch = rng.normal(0, 0.02, size=C)
+    fwd = np.stack([np.cos(yaw) * np.cos(pitch), np.sin(yaw) * np.cos(pitch), np.sin(pitch)], axis=1)
+    R = look_rotation(fwd)
+    t = -np.einsum("cij,cj->ci", R, pos)
+    gt_cams = pose_from_Rt(R, t)
+    fx = rng.uniform(400, 800, size=C)
+    intr = np.stack([fx, fx, np.full(C, IMG_W / 2), np.full(C, IMG_H / 2)], axis=1)
+
+    P = num_pts
+    p_geo = 1.0 / (mean_track - 1.0)
+    n = np.minimum(1 + rng.geometric(p_geo, size=P), C)
+    last = rng.integers(0, C, size=P)
+    last = np.maximum(last, n - 1)
+    # point ahead of the last camera: depth U(4, 25), inside 80 % of the view
+    depth = rng.uniform(4.0, 25.0, size=P)
+    xn = rng.uniform(-0.8, 0.8, size=P) * (IMG_W / 2) / fx[last]
+    yn = rng.uniform(-0.8, 0.8, size=P) * (IMG_H / 2) / fx[last]
+    Xc = np.stack([xn * depth, yn * depth, depth], axis=1)
+    Rl, tl = R[last], t[last]
+    gt_pts = np.einsum("pji,pj->pi", Rl, Xc - tl)  # R^T (Xc - t)
+    # observations: cameras last-n+1 .. last
+    offs = np.repeat(np.cumsum(n) - n, n)
+    k = np.arange(n.sum()) - offs
+    ep = np.repeat(np.arange(P), n)
+    ec = (np.repeat(last - n + 1, n) + k)
+    uv, d = project(gt_cams, intr, gt_pts, ec, ep)
+    keep = (d > 0.1) & (uv[:, 0] >= 0) & (uv[:, 1] >= 0) & (uv[:, 0] < IMG_W) & (uv[:, 1] < IMG_H)
+    cnt = np.bincount(ep[keep], minlength=P)
+    keep &= cnt[ep] >= 2
+    used = np.flatnonzero(cnt >= 2)
+    remap = -np.ones(P, dtype=np.int64)
+    if shuffle:
+        perm = rng.permutation(used.size)
+    else:
+        perm = np.arange(used.size)
+    remap[used] = perm
+    gt_pts_new = np.empty((used.size, 3))
+    gt_pts_new[perm] = gt_pts[used]
+    ec, ep, uv = ec[keep], remap[ep[keep]], uv[keep]
+    E = ec.size
+    meas = uv + rng.normal(0, 1.0, size=uv.shape)
+    out = rng.random(E) < outlier_ratio
+    meas[out] = np.stack([rng.uniform(0, IMG_W, out.sum()), rng.uniform(0, IMG_H, out.sum())], axis=1)
+    # observations ordered camera-major (as BAL files list them)
+    order = np.lexsort((ep, ec))
+    ec, ep, meas = ec[order], ep[order], meas[order]
+    w = np.ones(E) if weights is None else rng.uniform(weights[0], weights[1], size=E)
+    cams = perturb_poses(rng, gt_cams, sig_t, sig_r)
+    cam_fixed = np.zeros(C, dtype=np.uint8)
+    cam_fixed[:2] = 1
+    cams[:2] = gt_cams[:2]
+    pts = gt_pts_new + rng.normal(0, sig_p, size=gt_pts_new.shape)
+    info = {"outliers": int(out.sum()), "mean_track": float(E / used.size)}
+    return BAProblem(cams, intr, pts, ec.astype(np.int32), ep.astype(np.int32), meas, w, cam_fixed,
+                     np.zeros(used.size, dtype=np.uint8), huber, name=f"walk_ba_{C}c_{used.size}p",
+                     gt_cams=gt_cams, gt_pts=gt_pts_new, info=info)
+
+
+def make_metric_ba(seed: int = 1002) -> BAProblem:
+    """The headline workload: 1,500 cameras on a 3 km walk, 500k points,
+    ~3M observations, Huber delta 2, 5 % outliers."""
+    return make_walk_ba(seed=seed, num_cams=1500, num_pts=500_000)
+
+
+def make_diff_ba(seed: int = 1004) -> BAProblem:
+    """Differentiable-BA workload: 200 cameras, 50k points, ~300k observations,
+    per-edge weights U(0.5, 2)."""
+    return make_walk_ba(seed=seed, num_cams=200, num_pts=50_000, weights=(0.5, 2.0))

@@ -1,0 +1,243 @@
+"""Python host mirror of g2o's SparseOptimizer + OptimizationAlgorithmLevenberg
++ BlockSolver_6_3 API for bundle adjustment, backed by libg2o_hip.so.
+
+Method names follow the reference (g2o/core/sparse_optimizer.h,
+g2o/core/solver.h, g2o/core/optimization_algorithm_levenberg.h):
+``initialize_optimization`` / ``optimize`` / ``compute_active_errors`` /
+``active_robust_chi2`` on the optimizer, ``build_system`` / ``set_lambda`` /
+``solve`` / ``restore_diagonal`` / ``x`` / ``b`` on the solver. Errors raise
+:class:`G2oHipError` carrying the C-ABI status.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib as L
+
+
+class G2oHipError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"g2o_hip status {status}: {msg}")
+        self.status = status
+
+
+def _dp(a):
+    return a.ctypes.data_as(L._dp)
+
+
+def _ip(a):
+    return a.ctypes.data_as(L._ip)
+
+
+def _up(a):
+    return a.ctypes.data_as(L._up)
+
+
+def default_options() -> L.Options:
+    o = L.Options()
+    L.lib().g2ohip_default_options(ct.byref(o))
+    return o
+
+
+class Context:
+    """One device context (``device=-1``: host-only, structure checks only)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        h = ct.c_void_p()
+        st = self._lib.g2ohip_create(device, ct.byref(h))
+        if st != L.OK:
+            raise G2oHipError(st, f"cannot create a context on device {device}")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.g2ohip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, st: int):
+        if st != L.OK:
+            raise G2oHipError(st, self._lib.g2ohip_last_error(self.h).decode())
+
+    def call(self, name: str, *args):
+        self.check(getattr(self._lib, name)(self.h, *args))
+
+    def set_options(self, **kw):
+        o = L.Options()
+        self.check(self._lib.g2ohip_get_options(self.h, ct.byref(o)))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self.check(self._lib.g2ohip_set_options(self.h, ct.byref(o)))
+        return o
+
+
+class BundleAdjuster:
+    """SparseOptimizer with OptimizationAlgorithmLevenberg and a BlockSolverHIP
+    for one bundle-adjustment problem (``g2o_amd.datasets.BAProblem``)."""
+
+    def __init__(self, prob, device: int = 0, precision: int = L.FP64, schur_mode: int = L.SCHUR_EXPLICIT,
+                 pcg_tolerance: float = 1e-6, pcg_max_iter: int = -1, refine_steps: int = 1, max_trials: int = 10):
+        self.ctx = Context(device)
+        self.prob = prob
+        self.set_options(precision=precision, schur_mode=schur_mode, pcg_tolerance=pcg_tolerance,
+                         pcg_max_iter=pcg_max_iter, refine_steps=refine_steps, max_trials=max_trials)
+        self.C, self.P, self.E = prob.num_cams, prob.num_pts, prob.num_edges
+        self._ec = np.ascontiguousarray(prob.edge_cam, dtype=np.int32)
+        self._ep = np.ascontiguousarray(prob.edge_pt, dtype=np.int32)
+        self._cf = np.ascontiguousarray(prob.cam_fixed, dtype=np.uint8)
+        self._pf = np.ascontiguousarray(prob.pt_fixed, dtype=np.uint8)
+        self.ctx.call("g2ohip_ba_set_structure", self.C, self.P, self.E, _ip(self._ec), _ip(self._ep), _up(self._cf),
+                      _up(self._pf))
+        info = self.structure_info()
+        self.Cf, self.Pf = int(info[0]), int(info[1])
+        if device >= 0:
+            self.set_measurements(prob.meas, prob.weight, prob.intr, prob.huber)
+            self.set_state(prob.cams, prob.pts)
+
+    # ---- setup
+    def set_options(self, **kw):
+        return self.ctx.set_options(**kw)
+
+    def structure_info(self) -> np.ndarray:
+        info = (ct.c_longlong * 12)()
+        self.ctx.check(self.ctx._lib.g2ohip_ba_structure_info(self.ctx.h, info))
+        return np.array(info[:], dtype=np.int64)
+
+    def set_measurements(self, meas, weight, intr, huber):
+        m = np.ascontiguousarray(meas, dtype=np.float64)
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        k = np.ascontiguousarray(intr, dtype=np.float64)
+        self.ctx.call("g2ohip_ba_set_measurements", _dp(m), _dp(w), _dp(k), float(huber))
+
+    def set_state(self, cams, pts):
+        c = np.ascontiguousarray(cams, dtype=np.float64)
+        p = np.ascontiguousarray(pts, dtype=np.float64)
+        self.ctx.call("g2ohip_ba_set_state", _dp(c), _dp(p))
+
+    def state(self):
+        c = np.zeros((self.C, 7))
+        p = np.zeros((self.P, 3))
+        self.ctx.call("g2ohip_ba_get_state", _dp(c), _dp(p))
+        return c, p
+
+    # ---- SparseOptimizer
+    def compute_active_errors(self) -> float:
+        """computeActiveErrors + activeRobustChi2; returns the robust chi2."""
+        v = ct.c_double()
+        self.ctx.call("g2ohip_compute_active_errors", ct.byref(v))
+        return v.value
+
+    def errors(self) -> np.ndarray:
+        e = np.zeros((self.E, 2))
+        self.ctx.call("g2ohip_get_errors", _dp(e))
+        return e
+
+    def optimize(self, iterations: int):
+        stats = (L.IterStats * max(iterations, 1))()
+        done = ct.c_int()
+        self.ctx.call("g2ohip_optimize", iterations, stats, ct.byref(done))
+        out = []
+        for s in stats[:done.value]:
+            out.append({f: getattr(s, f) for f, _ in L.IterStats._fields_})
+        return out
+
+    def update(self):
+        self.ctx.call("g2ohip_update")
+
+    def push(self):
+        self.ctx.call("g2ohip_push")
+
+    def pop(self):
+        self.ctx.call("g2ohip_pop")
+
+    def discard_top(self):
+        self.ctx.call("g2ohip_discard_top")
+
+    # ---- Solver (BlockSolverHIP)
+    def build_system(self):
+        self.ctx.call("g2ohip_build_system")
+
+    def set_lambda(self, lam: float):
+        self.ctx.call("g2ohip_set_lambda", float(lam))
+
+    def restore_diagonal(self):
+        self.ctx.call("g2ohip_restore_diagonal")
+
+    def solve(self) -> bool:
+        ok = ct.c_int()
+        self.ctx.call("g2ohip_solve", ct.byref(ok))
+        return bool(ok.value)
+
+    def x(self) -> np.ndarray:
+        v = np.zeros(6 * self.Cf + 3 * self.Pf)
+        self.ctx.call("g2ohip_get_x", _dp(v))
+        return v
+
+    def b(self) -> np.ndarray:
+        v = np.zeros(6 * self.Cf + 3 * self.Pf)
+        self.ctx.call("g2ohip_get_b", _dp(v))
+        return v
+
+    def lambda_init(self) -> float:
+        v = ct.c_double()
+        self.ctx.call("g2ohip_lambda_init", ct.byref(v))
+        return v.value
+
+    def compute_scale(self) -> float:
+        v = ct.c_double()
+        self.ctx.call("g2ohip_compute_scale", ct.byref(v))
+        return v.value
+
+    def hessian(self):
+        Hpp = np.zeros((self.Cf, 6, 6))
+        Hll = np.zeros((self.Pf, 3, 3))
+        Hpl = np.zeros((self.E, 6, 3))
+        self.ctx.call("g2ohip_ba_get_hessian", _dp(Hpp), _dp(Hll), _dp(Hpl))
+        return Hpp, Hll, Hpl
+
+    def schur(self, nschur: int):
+        rows = np.zeros(nschur, dtype=np.int32)
+        cols = np.zeros(nschur, dtype=np.int32)
+        blocks = np.zeros((nschur, 6, 6))
+        bs = np.zeros(6 * self.Cf)
+        self.ctx.call("g2ohip_ba_get_schur", _ip(rows), _ip(cols), _dp(blocks), _dp(bs))
+        return rows, cols, blocks, bs
+
+    def set_profiling(self, on: bool = True):
+        self.ctx.call("g2ohip_set_profiling", int(on))
+
+    def profile(self) -> dict:
+        """{kernel name: (total ms, launches)} since set_profiling(True)."""
+        out = {}
+        for k, name in enumerate(L.KERNEL_NAMES):
+            t = ct.c_double()
+            n = ct.c_int()
+            self.ctx.call("g2ohip_get_profile", k, ct.byref(t), ct.byref(n))
+            out[name] = (t.value, n.value)
+        return out
+
+    def last_solve_info(self):
+        v = np.zeros(3)
+        self.ctx.call("g2ohip_last_solve_info", _dp(v))
+        return v
+
+
+def linear_solve_pcg(ctx: Context, nb: int, bd: int, rows, cols, blocks, b, tol=1e-6, max_iter=-1, residual=-1.0):
+    """LinearSolverPCG::solve on a host block matrix (upper blocks).
+    Returns (x, iterations, residual memory)."""
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    blocks = np.ascontiguousarray(blocks, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(nb * bd)
+    r = ct.c_double(residual)
+    it = ct.c_int()
+    ctx.call("g2ohip_linear_solve_pcg", nb, bd, len(rows), _ip(rows), _ip(cols), _dp(blocks), _dp(b), _dp(x),
+             float(tol), int(max_iter), ct.byref(r), ct.byref(it))
+    return x, it.value, r.value

@@ -1,0 +1,218 @@
+/*
+ * g2o_hip.h - C ABI of the MI355X (gfx950) implementation of g2o's
+ * per-iteration Levenberg-Marquardt linear-system path for bundle adjustment
+ * (BlockSolver<BlockSolverTraits<6,3>> with Schur) and 3D pose graphs
+ * (6x6 blocks without Schur).
+ *
+ * Plain C types only: pointers to host arrays and sizes. Every call returns a
+ * status code (G2OHIP_OK = 0); nothing throws across the ABI. After a failure
+ * g2ohip_last_error() returns a message for that context.
+ *
+ * Each entry point names the g2o interface it replaces (paths relative to the
+ * g2o source tree). The host-side C++ classes BlockSolverHIP / LinearSolverHIP
+ * (g2o_amd/host/) implement g2o's Solver / LinearSolver contracts on top of it.
+ *
+ * Threading: one context is used from one host thread at a time, as g2o calls
+ * its Solver (g2o/core/solver.h:44-153). Calls are stream-ordered on the
+ * context's own HIP stream and synchronous on return.
+ */
+#ifndef G2O_HIP_H
+#define G2O_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum g2ohip_status {
+  G2OHIP_OK = 0,
+  G2OHIP_ERR_INVALID = 1,      /* bad argument or call order */
+  G2OHIP_ERR_NOT_PD = 2,       /* singular / non-positive-definite block */
+  G2OHIP_ERR_NAN = 3,          /* non-finite value in the system */
+  G2OHIP_ERR_OOM = 4,          /* device allocation failed */
+  G2OHIP_ERR_HIP = 5,          /* HIP runtime error */
+  G2OHIP_ERR_UNSUPPORTED = 6,  /* structure outside the supported limits */
+  G2OHIP_ERR_NO_DEVICE = 7     /* context created without a device */
+};
+
+/* precision of the Hessian blocks, the Schur reduction and the PCG SpMV */
+enum g2ohip_precision { G2OHIP_FP64 = 0, G2OHIP_FP32 = 1 };
+/* how the reduced camera system is applied inside the PCG */
+enum g2ohip_schur_mode { G2OHIP_SCHUR_EXPLICIT = 0, G2OHIP_SCHUR_IMPLICIT = 1 };
+
+typedef struct g2ohip_ctx g2ohip_ctx;
+
+/* Solver options. Defaults (g2ohip_default_options) follow g2o: LM tau 1e-5 and
+ * maxTrialsAfterFailure 10 (optimization_algorithm_levenberg.cpp:42-56); PCG
+ * tolerance 1e-6, absolute-tolerance memory on, maxIter = system size
+ * (linear_solver_pcg.h:45-50). */
+typedef struct g2ohip_options {
+  int precision;          /* g2ohip_precision */
+  int schur_mode;         /* g2ohip_schur_mode */
+  double pcg_tolerance;   /* LinearSolverPCG::setTolerance */
+  int pcg_max_iter;       /* LinearSolverPCG::setMaxIterations, <0: n */
+  int refine_steps;       /* FP32 only: FP64 iterative-refinement corrections */
+  int max_trials;         /* OptimizationAlgorithmLevenberg maxTrialsAfterFailure */
+  double user_lambda_init;/* "initialLambda" property, <=0: tau * max diag */
+  double tau;             /* OptimizationAlgorithmLevenberg::_tau */
+} g2ohip_options;
+
+/* Per-LM-iteration record, fields named after G2OBatchStatistics
+ * (g2o/core/batch_stats.h:49-67) where one exists. Times are seconds. */
+typedef struct g2ohip_iter_stats {
+  double chi2_before;
+  double chi2;                 /* after the iteration (accepted state) */
+  double lambda;               /* after the iteration */
+  int levenberg_iterations;    /* inner trials */
+  int iterations_linear_solver;/* PCG iterations summed over trials */
+  int refine_steps;            /* FP64 refinement corrections summed */
+  int status;                  /* 0 OK, 1 Terminate (levenberg.cpp:149-152) */
+  double time_residuals;
+  double time_quadratic_form;
+  double time_schur_complement;
+  double time_linear_solver;
+  double time_update;
+  double time_iteration;
+} g2ohip_iter_stats;
+
+void g2ohip_default_options(g2ohip_options* opt);
+
+/* ------------------------------------------------------------ contexts */
+/* device >= 0 binds a HIP device. device == -1 makes a host-only context in
+ * which only g2ohip_ba_set_structure / g2ohip_ba_structure_info work (used to
+ * check the structure builder without a GPU). */
+int g2ohip_create(int device, g2ohip_ctx** out);
+void g2ohip_destroy(g2ohip_ctx* ctx);
+const char* g2ohip_last_error(const g2ohip_ctx* ctx);
+int g2ohip_set_options(g2ohip_ctx* ctx, const g2ohip_options* opt);
+int g2ohip_get_options(const g2ohip_ctx* ctx, g2ohip_options* opt);
+
+/* ------------------------------------------------- bundle adjustment graph */
+/* SparseOptimizer::initializeOptimization + buildIndexMapping
+ * (sparse_optimizer.cpp:175-321) and BlockSolver::buildStructure
+ * (block_solver.hpp:110-269) for EdgeSE3ProjectXYZ edges between
+ * VertexSE3Expmap cameras and marginalized VertexPointXYZ points.
+ * edge_cam/edge_pt: [E] vertex indices; cam_fixed/pt_fixed: [C]/[P] 0/1 masks
+ * (NULL = none fixed). An edge with both vertices fixed is inactive. */
+int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_edges, const int* edge_cam,
+                            const int* edge_pt, const uint8_t* cam_fixed, const uint8_t* pt_fixed);
+
+/* Structure summary: info[0..11] = free cameras, free points, active edges,
+ * Hschur upper blocks, linearisation chunks, Schur row tiles, Schur pair
+ * instances, longest Hschur block row, max slots per point, Hschur blocks in
+ * both triangles, (chunk, camera) partial rows of the linearisation, points
+ * with active edges. */
+int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info);
+
+/* Measurements of EdgeSE3ProjectXYZ (edge_project_xyz.cpp:44-49): meas[E*2]
+ * pixel (u, v); weight[E] scalar information (Omega = weight * I2, NULL = 1);
+ * intr[C*4] fx fy cx cy per camera; huber_delta > 0 attaches
+ * RobustKernelHuber(delta) to every edge (robust_kernel_impl.cpp:60-74). */
+int g2ohip_ba_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* weight, const double* intr,
+                               double huber_delta);
+
+/* Vertex estimates: cams[C*7] = SE3Quat [tx ty tz qx qy qz qw] (world to
+ * camera, se3quat.h), pts[P*3]. set_state normalizes quaternions as the
+ * SE3Quat constructor does (se3quat.h:65-86). */
+int g2ohip_ba_set_state(g2ohip_ctx* ctx, const double* cams, const double* pts);
+int g2ohip_ba_get_state(g2ohip_ctx* ctx, double* cams, double* pts);
+
+/* ------------------------------------ Solver contract (g2o/core/solver.h) */
+/* computeActiveErrors + activeRobustChi2 (sparse_optimizer.cpp:78-130) */
+int g2ohip_compute_active_errors(g2ohip_ctx* ctx, double* chi2);
+/* per-edge residual/Jacobian errors: err[E*2] (zero for inactive edges) */
+int g2ohip_get_errors(g2ohip_ctx* ctx, double* err);
+/* BlockSolver::buildSystem (block_solver.hpp:492-565) */
+int g2ohip_build_system(g2ohip_ctx* ctx);
+/* BlockSolver::setLambda / restoreDiagonal (block_solver.hpp:567-606). The
+ * damping is applied inside the Schur and PCG kernels; no diagonal backup. */
+int g2ohip_set_lambda(g2ohip_ctx* ctx, double lambda);
+int g2ohip_restore_diagonal(g2ohip_ctx* ctx);
+/* BlockSolver::solve (block_solver.hpp:331-477): Schur complement, reduced
+ * camera system by block-Jacobi PCG, back-substitution. *ok = 0 when the
+ * system was not solvable (the LM then rejects the step). */
+int g2ohip_solve(g2ohip_ctx* ctx, int* ok);
+/* Solver::x() / Solver::b() (solver.h:101-108), host copies [6Cf + 3Pf] */
+int g2ohip_get_x(g2ohip_ctx* ctx, double* x);
+int g2ohip_get_b(g2ohip_ctx* ctx, double* b);
+/* BlockSolverBase::multiplyHessian (block_solver.h:90-97) is not needed by LM
+ * and not provided. */
+
+/* SparseOptimizer::update(solver.x()) (sparse_optimizer.cpp:457-468) */
+int g2ohip_update(g2ohip_ctx* ctx);
+/* SparseOptimizer::push / pop / discardTop (sparse_optimizer.cpp:547-567,
+ * 633-637): state stack of depth <= 4 */
+int g2ohip_push(g2ohip_ctx* ctx);
+int g2ohip_pop(g2ohip_ctx* ctx);
+int g2ohip_discard_top(g2ohip_ctx* ctx);
+/* OptimizationAlgorithmLevenberg::computeLambdaInit / computeScale
+ * (optimization_algorithm_levenberg.cpp:155-176) */
+int g2ohip_lambda_init(g2ohip_ctx* ctx, double* lambda);
+int g2ohip_compute_scale(g2ohip_ctx* ctx, double* scale);
+
+/* SparseOptimizer::optimize(iterations) with OptimizationAlgorithmLevenberg
+ * (sparse_optimizer.cpp:393-455, optimization_algorithm_levenberg.cpp:60-153).
+ * stats[iterations] may be NULL. *done = iterations run. */
+int g2ohip_optimize(g2ohip_ctx* ctx, int iterations, g2ohip_iter_stats* stats, int* done);
+/* SparseOptimizer::setForceStopFlag (sparse_optimizer.h:179-183): the flag is
+ * read between LM trials; NULL clears it. */
+int g2ohip_set_force_stop_flag(g2ohip_ctx* ctx, const volatile int* flag);
+
+/* ------------------------------------------------- inspection (FP64 copies) */
+/* Hpp[Cf*36], Hll[Pf*9], Hpl per edge [E*18] (6x3 row-major; zero for edges
+ * without a block), all row-major, of the last build_system. NULL skips. */
+int g2ohip_ba_get_hessian(g2ohip_ctx* ctx, double* Hpp, double* Hll, double* Hpl_edge);
+/* Hschur upper blocks of the last solve, including the damping: rows/cols
+ * [nS], blocks [nS*36] row-major; bschur [6Cf]. */
+int g2ohip_ba_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* bschur);
+/* statistics of the last g2ohip_solve: info[0] PCG iterations (summed over
+ * refinement), info[1] refinement corrections, info[2] final PCG residual
+ * memory (LinearSolverPCG::_residual) */
+int g2ohip_last_solve_info(g2ohip_ctx* ctx, double* info);
+
+/* ------------------------------------------------------- kernel timing */
+/* Per-kernel durations from HIP events recorded on the context's stream around
+ * each launch (the device-side counterpart of the G2OBatchStatistics timers,
+ * g2o/core/batch_stats.h:49-67). g2ohip_set_profiling(on) clears the counters.
+ * G2OHIP_K_PCG covers one whole PCG solve (all its kernels). */
+enum g2ohip_kernel {
+  G2OHIP_K_LINEARIZE = 0,
+  G2OHIP_K_SCHUR = 1,
+  G2OHIP_K_PCG = 2,
+  G2OHIP_K_REFINE = 3,
+  G2OHIP_K_CHI2 = 4,
+  G2OHIP_K_UPDATE = 5,
+  G2OHIP_K_COUNT = 6
+};
+int g2ohip_set_profiling(g2ohip_ctx* ctx, int on);
+int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launches);
+
+/* ------------------------- LinearSolver contract (g2o/core/linear_solver.h) */
+/* LinearSolverPCG::solve (linear_solver_pcg.hpp:87-161) on a host matrix:
+ * symmetric, upper blocks only (brow <= bcol), uniform block size bd (3 or 6),
+ * blocks[nblk*bd*bd] row-major. x is overwritten; residual_io carries the
+ * LinearSolverPCG::_residual memory (pass -1 for a fresh solver).
+ * *iterations receives the iteration count. FP64 throughout. */
+int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int* brow, const int* bcol,
+                            const double* blocks, const double* b, double* x, double tol, int max_iter,
+                            double* residual_io, int* iterations);
+
+/* ---------------------------------------------- adjoint (differentiation) */
+/* Gradient of a scalar loss L(x*) at the converged state with respect to the
+ * measurements and the per-edge information weights, by implicit
+ * differentiation of the optimality condition grad E(x*, z, w) = 0:
+ *   (d^2E/dx^2) lambda = dL/dx,  dL/dz = -(d^2E/dx dz)^T lambda,
+ *   dL/dw = -(d^2E/dx dw)^T lambda,
+ * with the exact second-order (Newton) Hessian of the projection and Huber
+ * terms. dL_dcam[C*6] is the loss gradient with respect to the left-multiplied
+ * camera perturbation [omega; upsilon] (zero rows for fixed cameras), dL_dpt[P*3].
+ * Outputs dL_dmeas[E*2], dL_dweight[E]. No reference counterpart. */
+int g2ohip_ba_adjoint(g2ohip_ctx* ctx, const double* dL_dcam, const double* dL_dpt, double* dL_dmeas,
+                      double* dL_dweight, double tol, int max_iter, int* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G2O_HIP_H */

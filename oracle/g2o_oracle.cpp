@@ -1,0 +1,1052 @@
+// FP64 CPU oracle of g2o's per-iteration Levenberg-Marquardt linear-system path
+// for bundle adjustment (BlockSolver<BlockSolverTraits<6,3>> with Schur) and for
+// 3D pose graphs (6x6 blocks, no Schur).
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker that the HIP library is
+// compared against, and the CPU baseline that is timed beside it. Nothing in the
+// product library links or calls it.
+//
+// It restates, without Eigen, the arithmetic of these reference files (paths are
+// relative to the g2o source tree):
+//   g2o/types/slam3d/se3quat.h:97-256            SE3Quat exp / product / map / normalize
+//   g2o/types/slam3d/se3_ops.hpp:27-52           skew / project
+//   g2o/types/sba/edge_project_xyz.cpp:44-101    EdgeSE3ProjectXYZ error + analytic Jacobians
+//   g2o/types/sba/vertex_se3_expmap.cpp:48-51    VertexSE3Expmap::oplusImpl (left exp update)
+//   g2o/types/slam3d/vertex_pointxyz.h:47-50     VertexPointXYZ::oplusImpl
+//   g2o/core/robust_kernel_impl.cpp:60-74        RobustKernelHuber::robustify
+//   g2o/core/base_edge.h:99,156-162              chi2, robustInformation
+//   g2o/core/base_fixed_sized_edge.hpp:48-130    constructQuadraticForm
+//   g2o/core/block_solver.hpp:331-606            Schur, back-substitution, setLambda
+//   g2o/core/sparse_optimizer.cpp:78-130,175-200,457-468  errors, chi2, index map, update
+//   g2o/core/optimization_algorithm_levenberg.cpp:60-176  LM control
+//   g2o/solvers/pcg/linear_solver_pcg.hpp:87-205 block-Jacobi PCG and its stop rule
+//   g2o/solvers/eigen/linear_solver_eigen.h:94-191 (semantics: FP64 Cholesky, false on non-PD)
+//   pose graph: g2o/types/slam3d/edge_se3.cpp:60-88, vertex_se3.h:99-108,
+//               isometry3d_mappings.cpp:80-128, isometry3d_gradients.h:206-279
+//
+// Conventions of this file: 3x3 / 6x6 / 6x3 blocks are stored row-major.
+// A camera pose is 7 doubles [tx ty tz qx qy qz qw] (SE3Quat::toVector order).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using std::vector;
+
+inline double now_s() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// ---------------------------------------------------------------- quaternions
+struct Quat {
+  double x, y, z, w;
+};
+
+// Hamilton product a*b (the product Eigen's Quaternion::operator* computes).
+inline Quat qmul(const Quat& a, const Quat& b) {
+  return {a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+          a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z,
+          a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x,
+          a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+
+inline void qnormalize(Quat& q) {
+  double n = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+  q.x /= n; q.y /= n; q.z /= n; q.w /= n;
+}
+
+// Rotation matrix (row-major) of a unit quaternion.
+inline void q2R(const Quat& q, double R[9]) {
+  double tx = 2 * q.x, ty = 2 * q.y, tz = 2 * q.z;
+  double twx = tx * q.w, twy = ty * q.w, twz = tz * q.w;
+  double txx = tx * q.x, txy = ty * q.x, txz = tz * q.x;
+  double tyy = ty * q.y, tyz = tz * q.y, tzz = tz * q.z;
+  R[0] = 1 - (tyy + tzz); R[1] = txy - twz;       R[2] = txz + twy;
+  R[3] = txy + twz;       R[4] = 1 - (txx + tzz); R[5] = tyz - twx;
+  R[6] = txz - twy;       R[7] = tyz + twx;       R[8] = 1 - (txx + tyy);
+}
+
+// Quaternion of a rotation matrix (row-major). Shepperd's method (J. Guidance
+// and Control 1(3), 1978): take the largest of w^2, x^2, y^2, z^2 from the trace
+// and the diagonal, then the other three components from the off-diagonals.
+inline Quat R2q(const double R[9]) {
+  Quat q;
+  double tr = R[0] + R[4] + R[8];
+  if (tr > 0) {
+    double s = std::sqrt(tr + 1.0);
+    q.w = 0.5 * s;
+    s = 0.5 / s;
+    q.x = (R[7] - R[5]) * s;
+    q.y = (R[2] - R[6]) * s;
+    q.z = (R[3] - R[1]) * s;
+  } else {
+    int i = 0;
+    if (R[4] > R[0]) i = 1;
+    if (R[8] > R[3 * i + i]) i = 2;
+    int j = (i + 1) % 3, k = (j + 1) % 3;
+    double s = std::sqrt(R[3 * i + i] - R[3 * j + j] - R[3 * k + k] + 1.0);
+    double v[3];
+    v[i] = 0.5 * s;
+    s = 0.5 / s;
+    q.w = (R[3 * k + j] - R[3 * j + k]) * s;
+    v[j] = (R[3 * j + i] + R[3 * i + j]) * s;
+    v[k] = (R[3 * k + i] + R[3 * i + k]) * s;
+    q.x = v[0]; q.y = v[1]; q.z = v[2];
+  }
+  return q;
+}
+
+inline void qrot(const Quat& q, const double v[3], double out[3]) {
+  double R[9];
+  q2R(q, R);
+  for (int r = 0; r < 3; ++r) out[r] = R[3 * r] * v[0] + R[3 * r + 1] * v[1] + R[3 * r + 2] * v[2];
+}
+
+// ---------------------------------------------------------------- SE3Quat
+struct SE3 {
+  Quat q{0, 0, 0, 1};
+  double t[3]{0, 0, 0};
+};
+
+// se3quat.h:251-256: make w >= 0, then normalize.
+inline void normalize_rotation(SE3& a) {
+  if (a.q.w < 0) { a.q.x = -a.q.x; a.q.y = -a.q.y; a.q.z = -a.q.z; a.q.w = -a.q.w; }
+  qnormalize(a.q);
+}
+
+// se3quat.h:97-103 (operator*)
+inline SE3 se3_mul(const SE3& a, const SE3& b) {
+  SE3 r;
+  double rt[3];
+  qrot(a.q, b.t, rt);
+  for (int i = 0; i < 3; ++i) r.t[i] = a.t[i] + rt[i];
+  r.q = qmul(a.q, b.q);
+  normalize_rotation(r);
+  return r;
+}
+
+inline void skew(const double v[3], double m[9]) {
+  m[0] = 0;     m[1] = -v[2]; m[2] = v[1];
+  m[3] = v[2];  m[4] = 0;     m[5] = -v[0];
+  m[6] = -v[1]; m[7] = v[0];  m[8] = 0;
+}
+
+inline void mat3mul(const double a[9], const double b[9], double c[9]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      c[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+}
+
+// se3quat.h:202-230 (SE3Quat::exp): update = [omega; upsilon].
+inline SE3 se3_exp(const double upd[6]) {
+  const double* w = upd;
+  const double* u = upd + 3;
+  double theta = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  double W[9], W2[9], R[9], V[9];
+  skew(w, W);
+  mat3mul(W, W, W2);
+  double a, b, c, d;  // R = I + a W + b W^2, V = I + c W + d W^2
+  if (theta < 0.00001) {
+    a = 1.0; b = 0.5; c = 0.5; d = 1.0 / 6.0;
+  } else {
+    a = std::sin(theta) / theta;
+    b = (1 - std::cos(theta)) / (theta * theta);
+    c = (1 - std::cos(theta)) / (theta * theta);
+    d = (theta - std::sin(theta)) / std::pow(theta, 3);
+  }
+  for (int i = 0; i < 9; ++i) {
+    double I = (i % 4 == 0) ? 1.0 : 0.0;
+    R[i] = I + a * W[i] + b * W2[i];
+    V[i] = I + c * W[i] + d * W2[i];
+  }
+  SE3 r;
+  r.q = R2q(R);
+  for (int i = 0; i < 3; ++i) r.t[i] = V[3 * i] * u[0] + V[3 * i + 1] * u[1] + V[3 * i + 2] * u[2];
+  normalize_rotation(r);  // SE3Quat(Quaternion, Vector3) constructor
+  return r;
+}
+
+// se3quat.h:165-198 (SE3Quat::log)
+inline void se3_log(const SE3& a, double out[6]) {
+  double R[9];
+  q2R(a.q, R);
+  double d = 0.5 * (R[0] + R[4] + R[8] - 1);
+  double dR[3] = {R[7] - R[5], R[2] - R[6], R[3] - R[1]};
+  double omega[3], Om[9], Om2[9], Vinv[9];
+  double cc;
+  if (std::fabs(d) > 0.99999) {
+    for (int i = 0; i < 3; ++i) omega[i] = 0.5 * dR[i];
+    cc = 1.0 / 12.0;
+  } else {
+    double theta = std::acos(d);
+    double s = theta / (2 * std::sqrt(1 - d * d));
+    for (int i = 0; i < 3; ++i) omega[i] = s * dR[i];
+    cc = (1 - theta / (2 * std::tan(theta / 2))) / (theta * theta);
+  }
+  skew(omega, Om);
+  mat3mul(Om, Om, Om2);
+  for (int i = 0; i < 9; ++i) Vinv[i] = ((i % 4 == 0) ? 1.0 : 0.0) - 0.5 * Om[i] + cc * Om2[i];
+  for (int i = 0; i < 3; ++i) out[i] = omega[i];
+  for (int i = 0; i < 3; ++i) out[3 + i] = Vinv[3 * i] * a.t[0] + Vinv[3 * i + 1] * a.t[1] + Vinv[3 * i + 2] * a.t[2];
+}
+
+inline SE3 se3_from7(const double* p) {
+  SE3 r;
+  r.t[0] = p[0]; r.t[1] = p[1]; r.t[2] = p[2];
+  r.q = {p[3], p[4], p[5], p[6]};
+  normalize_rotation(r);
+  return r;
+}
+inline void se3_to7(const SE3& a, double* p) {
+  p[0] = a.t[0]; p[1] = a.t[1]; p[2] = a.t[2];
+  p[3] = a.q.x; p[4] = a.q.y; p[5] = a.q.z; p[6] = a.q.w;
+}
+
+// ---------------------------------------------------------------- Huber
+// robust_kernel_impl.cpp:60-74
+inline void huber(double e, double delta, double rho[3]) {
+  double dsqr = delta * delta;
+  if (e <= dsqr) {
+    rho[0] = e; rho[1] = 1.; rho[2] = 0.;
+  } else {
+    double sqrte = std::sqrt(e);
+    rho[0] = 2 * sqrte * delta - dsqr;
+    rho[1] = delta / sqrte;
+    rho[2] = -0.5 * rho[1] / e;
+  }
+}
+
+// ------------------------------------------------- EdgeSE3ProjectXYZ (BA edge)
+// edge_project_xyz.cpp:44-49 (computeError) and :57-93 (linearizeOplus).
+// Jp is d e / d point (2x3 row-major), Jc is d e / d [omega; upsilon] (2x6).
+inline void project_edge(const SE3& T, const double R[9], const double in[4], const double X[3],
+                         const double z[2], double e[2], double* Jp, double* Jc) {
+  double Xc[3];
+  for (int i = 0; i < 3; ++i) Xc[i] = R[3 * i] * X[0] + R[3 * i + 1] * X[1] + R[3 * i + 2] * X[2] + T.t[i];
+  const double fx = in[0], fy = in[1], cx = in[2], cy = in[3];
+  double x = Xc[0], y = Xc[1], zz = Xc[2];
+  e[0] = z[0] - (x / zz * fx + cx);
+  e[1] = z[1] - (y / zz * fy + cy);
+  if (!Jp) return;
+  double z2 = zz * zz;
+  double tmp[6] = {fx, 0, -x / zz * fx, 0, fy, -y / zz * fy};
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c)
+      Jp[3 * r + c] = -1. / zz * (tmp[3 * r] * R[c] + tmp[3 * r + 1] * R[3 + c] + tmp[3 * r + 2] * R[6 + c]);
+  Jc[0] = x * y / z2 * fx;
+  Jc[1] = -(1 + (x * x / z2)) * fx;
+  Jc[2] = y / zz * fx;
+  Jc[3] = -1. / zz * fx;
+  Jc[4] = 0;
+  Jc[5] = x / z2 * fx;
+  Jc[6] = (1 + y * y / z2) * fy;
+  Jc[7] = -x * y / z2 * fy;
+  Jc[8] = -x / zz * fy;
+  Jc[9] = 0;
+  Jc[10] = -1. / zz * fy;
+  Jc[11] = y / z2 * fy;
+}
+
+// ---------------------------------------------------------------- small dense
+// 3x3 inverse by cofactors (the closed form Eigen uses for fixed 3x3).
+inline bool inv3(const double m[9], double out[9]) {
+  double c0 = m[4] * m[8] - m[5] * m[7];
+  double c1 = m[5] * m[6] - m[3] * m[8];
+  double c2 = m[3] * m[7] - m[4] * m[6];
+  double det = m[0] * c0 + m[1] * c1 + m[2] * c2;
+  double id = 1.0 / det;
+  out[0] = c0 * id;
+  out[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+  out[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+  out[3] = c1 * id;
+  out[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+  out[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+  out[6] = c2 * id;
+  out[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+  out[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+  return std::isfinite(id);
+}
+
+// n x n inverse (row-major) by Gauss-Jordan elimination with partial pivoting.
+inline bool invN(int n, const double* m, double* out) {
+  double a[6 * 12];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < 2 * n; ++j)
+      a[i * 2 * n + j] = (j < n) ? m[i * n + j] : ((j - n == i) ? 1.0 : 0.0);
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[r * 2 * n + c]) > std::fabs(a[p * 2 * n + c])) p = r;
+    if (a[p * 2 * n + c] == 0.0) return false;
+    if (p != c)
+      for (int j = 0; j < 2 * n; ++j) std::swap(a[p * 2 * n + j], a[c * 2 * n + j]);
+    double piv = 1.0 / a[c * 2 * n + c];
+    for (int j = 0; j < 2 * n; ++j) a[c * 2 * n + j] *= piv;
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      double f = a[r * 2 * n + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < 2 * n; ++j) a[r * 2 * n + j] -= f * a[c * 2 * n + j];
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out[i * n + j] = a[i * 2 * n + n + j];
+  return true;
+}
+
+// Dense LL^T of an SPD matrix (row-major, full storage) and solve. Returns
+// false when a pivot is not positive, as LinearSolverEigen::solve does when
+// SimplicialLLT reports failure (linear_solver_eigen.h:121-139).
+bool chol_solve(int n, const double* A, const double* b, double* x) {
+  vector<double> L(static_cast<size_t>(n) * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double s = A[static_cast<size_t>(j) * n + j];
+    for (int k = 0; k < j; ++k) s -= L[static_cast<size_t>(j) * n + k] * L[static_cast<size_t>(j) * n + k];
+    if (!(s > 0.0) || !std::isfinite(s)) return false;
+    double ljj = std::sqrt(s);
+    L[static_cast<size_t>(j) * n + j] = ljj;
+#pragma omp parallel for if (n > 512)
+    for (int i = j + 1; i < n; ++i) {
+      double t = A[static_cast<size_t>(i) * n + j];
+      const double* li = &L[static_cast<size_t>(i) * n];
+      const double* lj = &L[static_cast<size_t>(j) * n];
+      for (int k = 0; k < j; ++k) t -= li[k] * lj[k];
+      L[static_cast<size_t>(i) * n + j] = t / ljj;
+    }
+  }
+  vector<double> y(n);
+  for (int i = 0; i < n; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= L[static_cast<size_t>(i) * n + k] * y[k];
+    y[i] = s / L[static_cast<size_t>(i) * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < n; ++k) s -= L[static_cast<size_t>(k) * n + i] * x[k];
+    x[i] = s / L[static_cast<size_t>(i) * n + i];
+  }
+  return true;
+}
+
+// ------------------------------------------------ block-sparse symmetric matrix
+// Upper-triangle block storage of a symmetric matrix with uniform block size
+// bd (the layout PCG flattens, linear_solver_pcg.hpp:95-115).
+struct BlockSym {
+  int bd = 6, nb = 0;
+  vector<int> row, col;    // upper blocks (row <= col)
+  vector<double> val;      // bd*bd row-major per block
+  vector<int> diag;        // block index of (i,i), -1 when absent
+};
+
+// linear_solver_pcg.hpp:185-205 (mult): dest = A src over the upper blocks and
+// their transposes. Parallel over block rows through a full-row index.
+struct FullRows {
+  vector<int> ptr, blk;
+  vector<char> trans;
+};
+
+FullRows make_full_rows(const BlockSym& A) {
+  FullRows F;
+  F.ptr.assign(A.nb + 1, 0);
+  for (size_t k = 0; k < A.row.size(); ++k) {
+    F.ptr[A.row[k] + 1]++;
+    if (A.row[k] != A.col[k]) F.ptr[A.col[k] + 1]++;
+  }
+  for (int i = 0; i < A.nb; ++i) F.ptr[i + 1] += F.ptr[i];
+  F.blk.resize(F.ptr[A.nb]);
+  F.trans.resize(F.ptr[A.nb]);
+  vector<int> fill(F.ptr.begin(), F.ptr.end() - 1);
+  for (size_t k = 0; k < A.row.size(); ++k) {
+    int i = A.row[k], j = A.col[k];
+    F.blk[fill[i]] = static_cast<int>(k); F.trans[fill[i]++] = 0;
+    if (i != j) { F.blk[fill[j]] = static_cast<int>(k); F.trans[fill[j]++] = 1; }
+  }
+  return F;
+}
+
+void sym_mult(const BlockSym& A, const FullRows& F, const double* src, double* dest) {
+  const int bd = A.bd;
+#pragma omp parallel for schedule(static) if (A.nb > 256)
+  for (int i = 0; i < A.nb; ++i) {
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = F.ptr[i]; p < F.ptr[i + 1]; ++p) {
+      int k = F.blk[p];
+      const double* m = &A.val[static_cast<size_t>(k) * bd * bd];
+      if (!F.trans[p]) {
+        int j = A.col[k];
+        for (int r = 0; r < bd; ++r)
+          for (int c = 0; c < bd; ++c) acc[r] += m[r * bd + c] * src[j * bd + c];
+      } else {
+        int j = A.row[k];
+        for (int r = 0; r < bd; ++r)
+          for (int c = 0; c < bd; ++c) acc[r] += m[c * bd + r] * src[j * bd + c];
+      }
+    }
+    for (int r = 0; r < bd; ++r) dest[i * bd + r] = acc[r];
+  }
+}
+
+// LinearSolverPCG::solve (linear_solver_pcg.hpp:87-161): block-Jacobi PCG from
+// x = 0. `residual` carries LinearSolverPCG::_residual between calls (-1 after
+// init()); stop when dn <= max(tol * dn0, residual) (absoluteTolerance = true,
+// linear_solver_pcg.h:45-50), maxIter < 0 means n. Always succeeds.
+struct PCGState {
+  double tol = 1e-6;
+  int max_iter = -1;
+  double residual = -1.0;
+  int last_iters = 0;
+};
+
+bool pcg_solve(const BlockSym& A, const double* b, double* x, PCGState& st) {
+  const int bd = A.bd, n = A.nb * bd;
+  FullRows F = make_full_rows(A);
+  vector<double> J(static_cast<size_t>(A.nb) * bd * bd);
+  for (int i = 0; i < A.nb; ++i) {
+    if (A.diag[i] < 0) return false;
+    if (!invN(bd, &A.val[static_cast<size_t>(A.diag[i]) * bd * bd], &J[static_cast<size_t>(i) * bd * bd])) return false;
+  }
+  auto mdiag = [&](const vector<double>& src, vector<double>& dst) {
+    for (int i = 0; i < A.nb; ++i)
+      for (int r = 0; r < bd; ++r) {
+        double s = 0;
+        for (int c = 0; c < bd; ++c) s += J[static_cast<size_t>(i) * bd * bd + r * bd + c] * src[i * bd + c];
+        dst[i * bd + r] = s;
+      }
+  };
+  auto dot = [&](const vector<double>& a, const vector<double>& b2) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += a[i] * b2[i];
+    return s;
+  };
+  for (int i = 0; i < n; ++i) x[i] = 0;
+  vector<double> r(b, b + n), d(n, 0.0), q(n, 0.0), s(n, 0.0);
+  mdiag(r, d);
+  double dn = dot(r, d);
+  double d0 = st.tol * dn;
+  if (st.residual > 0.0 && st.residual > d0) d0 = st.residual;
+  int maxIter = st.max_iter < 0 ? n : st.max_iter;
+  int it;
+  for (it = 0; it < maxIter; ++it) {
+    if (dn <= d0) break;
+    sym_mult(A, F, d.data(), q.data());
+    double a = dn / dot(d, q);
+    for (int i = 0; i < n; ++i) { x[i] += a * d[i]; r[i] -= a * q[i]; }
+    mdiag(r, s);
+    double dold = dn;
+    dn = dot(r, s);
+    double ba = dn / dold;
+    for (int i = 0; i < n; ++i) d[i] = s[i] + ba * d[i];
+  }
+  st.residual = 0.5 * dn;
+  st.last_iters = it;
+  return true;
+}
+
+bool chol_solve_blocks(const BlockSym& A, const double* b, double* x) {
+  const int bd = A.bd, n = A.nb * bd;
+  vector<double> D(static_cast<size_t>(n) * n, 0.0);
+  for (size_t k = 0; k < A.row.size(); ++k) {
+    int i = A.row[k], j = A.col[k];
+    const double* m = &A.val[k * bd * bd];
+    for (int r = 0; r < bd; ++r)
+      for (int c = 0; c < bd; ++c) {
+        D[static_cast<size_t>(i * bd + r) * n + j * bd + c] = m[r * bd + c];
+        D[static_cast<size_t>(j * bd + c) * n + i * bd + r] = m[r * bd + c];
+      }
+  }
+  return chol_solve(n, D.data(), b, x);
+}
+
+// ---------------------------------------------------------------- BA problem
+struct BA {
+  int C = 0, P = 0, E = 0;
+  vector<SE3> cams;
+  vector<double> intr;   // 4 per camera
+  vector<double> pts;    // 3 per point
+  vector<int> ecam, ept;
+  vector<double> meas, w;
+  vector<uint8_t> cfix, pfix;
+  double huber = 0;
+
+  // index mapping (sparse_optimizer.cpp:175-200; block_solver.hpp:122-137)
+  vector<int> camH, ptH;  // free index or -1
+  int Cf = 0, Pf = 0;
+  // active edges: not all vertices fixed (sparse_optimizer.cpp:209-287)
+  vector<int> active;
+  // Hpl blocks: one per distinct (free cam, free point) pair; per point column
+  // sorted by camera (HplCCS, block_solver.hpp:235)
+  vector<int> colPtr;            // Pf+1
+  vector<int> colCam;            // free camera index per block
+  vector<int> edgeBlk;           // Hpl block per edge or -1
+  // Hschur pattern (block_solver.hpp:237-266): per row sorted columns
+  vector<int> sPtr, sCol;
+  // per-iteration system
+  vector<double> Hpp, Hll, Hpl, b;
+  vector<double> err;            // 2 per edge
+  // state stack (base_vertex.h:111-122)
+  vector<vector<SE3>> stackC;
+  vector<vector<double>> stackP;
+  // solver workspace
+  BlockSym S;
+  vector<double> bschur, Dinv, x, coeff;
+  double lambda = 0;
+  PCGState pcg;
+  int solver_kind = 0;  // 0 Cholesky, 1 PCG
+  double times[6] = {0, 0, 0, 0, 0, 0};
+};
+
+void ba_structure(BA& p) {
+  p.camH.assign(p.C, -1);
+  p.ptH.assign(p.P, -1);
+  p.Cf = p.Pf = 0;
+  for (int c = 0; c < p.C; ++c) if (!p.cfix[c]) p.camH[c] = p.Cf++;
+  for (int i = 0; i < p.P; ++i) if (!p.pfix[i]) p.ptH[i] = p.Pf++;
+  p.active.clear();
+  for (int k = 0; k < p.E; ++k)
+    if (!(p.cfix[p.ecam[k]] && p.pfix[p.ept[k]])) p.active.push_back(k);
+  // Hpl blocks per point column
+  vector<vector<int>> colCams(p.Pf);
+  for (int k : p.active) {
+    int c = p.camH[p.ecam[k]], l = p.ptH[p.ept[k]];
+    if (c >= 0 && l >= 0) colCams[l].push_back(c);
+  }
+  p.colPtr.assign(p.Pf + 1, 0);
+  for (int l = 0; l < p.Pf; ++l) {
+    auto& v = colCams[l];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    p.colPtr[l + 1] = p.colPtr[l] + static_cast<int>(v.size());
+  }
+  p.colCam.resize(p.colPtr[p.Pf]);
+  for (int l = 0; l < p.Pf; ++l) std::copy(colCams[l].begin(), colCams[l].end(), p.colCam.begin() + p.colPtr[l]);
+  p.edgeBlk.assign(p.E, -1);
+  for (int k : p.active) {
+    int c = p.camH[p.ecam[k]], l = p.ptH[p.ept[k]];
+    if (c < 0 || l < 0) continue;
+    auto b0 = p.colCam.begin() + p.colPtr[l], b1 = p.colCam.begin() + p.colPtr[l + 1];
+    p.edgeBlk[k] = static_cast<int>(std::lower_bound(b0, b1, c) - p.colCam.begin());
+  }
+  // Schur pattern: diagonal of Hpp plus co-observing pairs
+  vector<vector<int>> rows(p.Cf);
+  for (int c = 0; c < p.Cf; ++c) rows[c].push_back(c);
+  for (int l = 0; l < p.Pf; ++l)
+    for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a)
+      for (int bb = a; bb < p.colPtr[l + 1]; ++bb) rows[p.colCam[a]].push_back(p.colCam[bb]);
+  p.sPtr.assign(p.Cf + 1, 0);
+  for (int c = 0; c < p.Cf; ++c) {
+    auto& v = rows[c];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    p.sPtr[c + 1] = p.sPtr[c] + static_cast<int>(v.size());
+  }
+  p.sCol.resize(p.sPtr[p.Cf]);
+  for (int c = 0; c < p.Cf; ++c) std::copy(rows[c].begin(), rows[c].end(), p.sCol.begin() + p.sPtr[c]);
+  p.S.bd = 6;
+  p.S.nb = p.Cf;
+  p.S.row.resize(p.sCol.size());
+  p.S.col = p.sCol;
+  p.S.diag.assign(p.Cf, -1);
+  for (int c = 0; c < p.Cf; ++c)
+    for (int k = p.sPtr[c]; k < p.sPtr[c + 1]; ++k) {
+      p.S.row[k] = c;
+      if (p.sCol[k] == c) p.S.diag[c] = k;
+    }
+  p.S.val.assign(p.sCol.size() * 36, 0.0);
+  p.err.assign(2 * static_cast<size_t>(p.E), 0.0);
+}
+
+// computeActiveErrors + activeRobustChi2 (sparse_optimizer.cpp:78-130)
+double ba_chi2(BA& p) {
+  double chi = 0;
+  const int n = static_cast<int>(p.active.size());
+#pragma omp parallel for reduction(+ : chi) schedule(static)
+  for (int a = 0; a < n; ++a) {
+    int k = p.active[a];
+    const SE3& T = p.cams[p.ecam[k]];
+    double R[9];
+    q2R(T.q, R);
+    double e[2];
+    project_edge(T, R, &p.intr[4 * p.ecam[k]], &p.pts[3 * p.ept[k]], &p.meas[2 * k], e, nullptr, nullptr);
+    p.err[2 * k] = e[0];
+    p.err[2 * k + 1] = e[1];
+    double c2 = p.w[k] * (e[0] * e[0] + e[1] * e[1]);
+    if (p.huber > 0) {
+      double rho[3];
+      huber(c2, p.huber, rho);
+      chi += rho[0];
+    } else {
+      chi += c2;
+    }
+  }
+  return chi;
+}
+
+// BlockSolver::buildSystem (block_solver.hpp:492-565) with the per-edge
+// linearizeOplus + constructQuadraticForm (base_fixed_sized_edge.hpp:48-130).
+// Uses the errors of the last ba_chi2 call, as the reference uses _error.
+void ba_build_system(BA& p) {
+  p.Hpp.assign(static_cast<size_t>(p.Cf) * 36, 0.0);
+  p.Hll.assign(static_cast<size_t>(p.Pf) * 9, 0.0);
+  p.Hpl.assign(p.colCam.size() * 18, 0.0);
+  p.b.assign(6 * static_cast<size_t>(p.Cf) + 3 * static_cast<size_t>(p.Pf), 0.0);
+  const int n = static_cast<int>(p.active.size());
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  // Per-thread camera accumulators stand in for the per-vertex QuadraticFormLock
+  // of the reference's OpenMP build (base_edge.h:42-58).
+  vector<vector<double>> hppT(nth), bpT(nth), hllT(nth), blT(nth);
+#pragma omp parallel
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    auto& hpp = hppT[tid];
+    auto& bp = bpT[tid];
+    auto& hll = hllT[tid];
+    auto& bl = blT[tid];
+    hpp.assign(static_cast<size_t>(p.Cf) * 36, 0.0);
+    bp.assign(static_cast<size_t>(p.Cf) * 6, 0.0);
+    hll.assign(static_cast<size_t>(p.Pf) * 9, 0.0);
+    bl.assign(static_cast<size_t>(p.Pf) * 3, 0.0);
+#pragma omp for schedule(static)
+    for (int a = 0; a < n; ++a) {
+      int k = p.active[a];
+      int ci = p.camH[p.ecam[k]], li = p.ptH[p.ept[k]];
+      const SE3& T = p.cams[p.ecam[k]];
+      double R[9];
+      q2R(T.q, R);
+      double e[2] = {p.err[2 * k], p.err[2 * k + 1]}, etmp[2];
+      double Jp[6], Jc[12];
+      project_edge(T, R, &p.intr[4 * p.ecam[k]], &p.pts[3 * p.ept[k]], &p.meas[2 * k], etmp, Jp, Jc);
+      double om = p.w[k];  // Omega = w I
+      double wr_scale = 1.0;
+      if (p.huber > 0) {
+        double rho[3];
+        huber(om * (e[0] * e[0] + e[1] * e[1]), p.huber, rho);
+        wr_scale = rho[1];
+      }
+      double omp_ = wr_scale * om;                     // robustInformation
+      double wr[2] = {-omp_ * e[0], -omp_ * e[1]};     // omega_r = -rho' Omega e
+      if (li >= 0) {
+        for (int r = 0; r < 3; ++r) {
+          bl[3 * li + r] += Jp[r] * wr[0] + Jp[3 + r] * wr[1];
+          for (int c = 0; c < 3; ++c) hll[9 * li + 3 * r + c] += omp_ * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]);
+        }
+      }
+      if (ci >= 0) {
+        for (int r = 0; r < 6; ++r) {
+          bp[6 * ci + r] += Jc[r] * wr[0] + Jc[6 + r] * wr[1];
+          for (int c = 0; c < 6; ++c) hpp[36 * ci + 6 * r + c] += omp_ * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c]);
+        }
+      }
+      if (ci >= 0 && li >= 0) {
+        // Hpl(cam, pt) += (Jc^T Omega') Jp, written through the transposed map
+        // because the point is vertex 0 (block_solver.hpp:210-216).
+        double* h = &p.Hpl[static_cast<size_t>(p.edgeBlk[k]) * 18];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 3; ++c) h[3 * r + c] += omp_ * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]);
+      }
+    }
+  }
+  for (int t = 0; t < nth; ++t) {
+    for (size_t i = 0; i < p.Hpp.size(); ++i) p.Hpp[i] += hppT[t][i];
+    for (size_t i = 0; i < p.Hll.size(); ++i) p.Hll[i] += hllT[t][i];
+    for (int i = 0; i < 6 * p.Cf; ++i) p.b[i] += bpT[t][i];
+    for (int i = 0; i < 3 * p.Pf; ++i) p.b[6 * p.Cf + i] += blT[t][i];
+  }
+}
+
+// BlockSolver::solve with Schur (block_solver.hpp:331-477). lambda is applied
+// to the diagonals as setLambda does (block_solver.hpp:567-590) and undone
+// afterwards (restoreDiagonal, :592-606) by working on copies.
+bool ba_solve(BA& p) {
+  double t0 = now_s();
+  const int Cf = p.Cf, Pf = p.Pf;
+  const int sp = 6 * Cf;
+  // Hschur = Hpp (+ lambda)
+  std::fill(p.S.val.begin(), p.S.val.end(), 0.0);
+  for (int c = 0; c < Cf; ++c) {
+    double* d = &p.S.val[static_cast<size_t>(p.S.diag[c]) * 36];
+    for (int i = 0; i < 36; ++i) d[i] = p.Hpp[36 * static_cast<size_t>(c) + i];
+    for (int i = 0; i < 6; ++i) d[7 * i] += p.lambda;
+  }
+  p.Dinv.assign(static_cast<size_t>(Pf) * 9, 0.0);
+  p.coeff.assign(sp + 3 * static_cast<size_t>(Pf), 0.0);
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  vector<vector<double>> Sth(nth), cth(nth);
+  bool okD = true;
+#pragma omp parallel reduction(&& : okD)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    auto& Sl = Sth[tid];
+    auto& cl = cth[tid];
+    Sl.assign(p.S.val.size(), 0.0);
+    cl.assign(sp, 0.0);
+#pragma omp for schedule(dynamic, 64)
+    for (int l = 0; l < Pf; ++l) {
+      double D[9];
+      for (int i = 0; i < 9; ++i) D[i] = p.Hll[9 * static_cast<size_t>(l) + i];
+      for (int i = 0; i < 3; ++i) D[4 * i] += p.lambda;
+      double* Di = &p.Dinv[9 * static_cast<size_t>(l)];
+      okD = inv3(D, Di) && okD;
+      const double* bl = &p.b[sp + 3 * static_cast<size_t>(l)];
+      double db[3];
+      for (int r = 0; r < 3; ++r) db[r] = Di[3 * r] * bl[0] + Di[3 * r + 1] * bl[1] + Di[3 * r + 2] * bl[2];
+      for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a) {
+        int i1 = p.colCam[a];
+        const double* Bi = &p.Hpl[static_cast<size_t>(a) * 18];
+        double BD[18];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 3; ++c)
+            BD[3 * r + c] = Bi[3 * r] * Di[c] + Bi[3 * r + 1] * Di[3 + c] + Bi[3 * r + 2] * Di[6 + c];
+        for (int r = 0; r < 6; ++r) cl[6 * i1 + r] += Bi[3 * r] * db[0] + Bi[3 * r + 1] * db[1] + Bi[3 * r + 2] * db[2];
+        int sk = p.sPtr[i1];
+        for (int bb = a; bb < p.colPtr[l + 1]; ++bb) {
+          int i2 = p.colCam[bb];
+          while (p.sCol[sk] < i2) ++sk;
+          const double* Bj = &p.Hpl[static_cast<size_t>(bb) * 18];
+          double* h = &Sl[static_cast<size_t>(sk) * 36];
+          for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c)
+              h[6 * r + c] -= BD[3 * r] * Bj[3 * c] + BD[3 * r + 1] * Bj[3 * c + 1] + BD[3 * r + 2] * Bj[3 * c + 2];
+        }
+      }
+    }
+  }
+  if (!okD) return false;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < static_cast<long>(p.S.val.size()); ++i)
+    for (int t = 0; t < nth; ++t) p.S.val[i] += Sth[t][i];
+  p.bschur.assign(sp, 0.0);
+  for (int i = 0; i < sp; ++i) {
+    double c = 0;
+    for (int t = 0; t < nth; ++t) c += cth[t][i];
+    p.coeff[i] = c;
+    p.bschur[i] = p.b[i] - c;
+  }
+  double t1 = now_s();
+  p.times[2] += t1 - t0;
+  p.x.assign(sp + 3 * static_cast<size_t>(Pf), 0.0);
+  bool ok = (p.solver_kind == 0) ? chol_solve_blocks(p.S, p.bschur.data(), p.x.data())
+                                 : pcg_solve(p.S, p.bschur.data(), p.x.data(), p.pcg);
+  double t2 = now_s();
+  p.times[3] += t2 - t1;
+  if (!ok) return false;
+  // back-substitution (block_solver.hpp:452-476): xl = Dinv (bl - Hpl^T xp)
+#pragma omp parallel for schedule(static)
+  for (int l = 0; l < Pf; ++l) {
+    double cl[3];
+    for (int r = 0; r < 3; ++r) cl[r] = p.b[sp + 3 * static_cast<size_t>(l) + r];
+    for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a) {
+      const double* B = &p.Hpl[static_cast<size_t>(a) * 18];
+      const double* xp = &p.x[6 * static_cast<size_t>(p.colCam[a])];
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 6; ++r) cl[c] -= B[3 * r + c] * xp[r];
+    }
+    const double* Di = &p.Dinv[9 * static_cast<size_t>(l)];
+    for (int r = 0; r < 3; ++r)
+      p.x[sp + 3 * static_cast<size_t>(l) + r] = Di[3 * r] * cl[0] + Di[3 * r + 1] * cl[1] + Di[3 * r + 2] * cl[2];
+  }
+  return true;
+}
+
+// SparseOptimizer::update (sparse_optimizer.cpp:457-468)
+void ba_update(BA& p, const double* x) {
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < p.C; ++c) {
+    int h = p.camH[c];
+    if (h < 0) continue;
+    p.cams[c] = se3_mul(se3_exp(&x[6 * h]), p.cams[c]);  // vertex_se3_expmap.cpp:48-51
+  }
+  const size_t off = 6 * static_cast<size_t>(p.Cf);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < p.P; ++i) {
+    int h = p.ptH[i];
+    if (h < 0) continue;
+    for (int r = 0; r < 3; ++r) p.pts[3 * static_cast<size_t>(i) + r] += x[off + 3 * static_cast<size_t>(h) + r];
+  }
+}
+
+void ba_push(BA& p) { p.stackC.push_back(p.cams); p.stackP.push_back(p.pts); }
+void ba_pop(BA& p) { p.cams = p.stackC.back(); p.pts = p.stackP.back(); p.stackC.pop_back(); p.stackP.pop_back(); }
+void ba_discard(BA& p) { p.stackC.pop_back(); p.stackP.pop_back(); }
+
+// computeLambdaInit (optimization_algorithm_levenberg.cpp:155-167)
+double ba_lambda_init(const BA& p) {
+  double m = 0;
+  for (int c = 0; c < p.Cf; ++c)
+    for (int j = 0; j < 6; ++j) m = std::max(m, std::fabs(p.Hpp[36 * static_cast<size_t>(c) + 7 * j]));
+  for (int l = 0; l < p.Pf; ++l)
+    for (int j = 0; j < 3; ++j) m = std::max(m, std::fabs(p.Hll[9 * static_cast<size_t>(l) + 4 * j]));
+  return 1e-5 * m;
+}
+
+// computeScale (optimization_algorithm_levenberg.cpp:169-176)
+double ba_scale(const BA& p) {
+  double s = 0;
+  for (size_t j = 0; j < p.x.size(); ++j) s += p.x[j] * (p.lambda * p.x[j] + p.b[j]);
+  return s;
+}
+
+}  // namespace
+
+// =============================================================== C interface
+extern "C" {
+
+void* orc_ba_new(int C, int P, int E, const double* cam7, const double* intr4, const double* pts3,
+                 const int* ecam, const int* ept, const double* meas2, const double* w,
+                 const unsigned char* cam_fixed, const unsigned char* pt_fixed, double huber_delta) {
+  BA* p = new BA();
+  p->C = C; p->P = P; p->E = E;
+  p->cams.resize(C);
+  for (int c = 0; c < C; ++c) p->cams[c] = se3_from7(&cam7[7 * c]);
+  p->intr.assign(intr4, intr4 + 4 * static_cast<size_t>(C));
+  p->pts.assign(pts3, pts3 + 3 * static_cast<size_t>(P));
+  p->ecam.assign(ecam, ecam + E);
+  p->ept.assign(ept, ept + E);
+  p->meas.assign(meas2, meas2 + 2 * static_cast<size_t>(E));
+  if (w) p->w.assign(w, w + E); else p->w.assign(E, 1.0);
+  p->cfix.assign(C, 0);
+  p->pfix.assign(P, 0);
+  if (cam_fixed) p->cfix.assign(cam_fixed, cam_fixed + C);
+  if (pt_fixed) p->pfix.assign(pt_fixed, pt_fixed + P);
+  p->huber = huber_delta;
+  ba_structure(*p);
+  return p;
+}
+
+void orc_ba_free(void* h) { delete static_cast<BA*>(h); }
+
+void orc_ba_dims(void* h, int* cf, int* pf, int* nschur) {
+  BA* p = static_cast<BA*>(h);
+  *cf = p->Cf; *pf = p->Pf; *nschur = static_cast<int>(p->sCol.size());
+}
+
+void orc_ba_get_state(void* h, double* cam7, double* pts3) {
+  BA* p = static_cast<BA*>(h);
+  for (int c = 0; c < p->C; ++c) se3_to7(p->cams[c], &cam7[7 * c]);
+  std::copy(p->pts.begin(), p->pts.end(), pts3);
+}
+
+void orc_ba_set_state(void* h, const double* cam7, const double* pts3) {
+  BA* p = static_cast<BA*>(h);
+  for (int c = 0; c < p->C; ++c) p->cams[c] = se3_from7(&cam7[7 * c]);
+  p->pts.assign(pts3, pts3 + 3 * static_cast<size_t>(p->P));
+}
+
+double orc_ba_chi2(void* h) { return ba_chi2(*static_cast<BA*>(h)); }
+
+// Linearize at the current state. Outputs are optional (nullptr skips):
+// Hpp[Cf*36], Hll[Pf*9], Hpl_edge[E*18] (the Hpl block each edge maps to, zero
+// for edges without one), b[6Cf+3Pf]. Returns chi2 at the state.
+double orc_ba_linearize(void* h, double* Hpp, double* Hll, double* Hpl_edge, double* b) {
+  BA* p = static_cast<BA*>(h);
+  double chi = ba_chi2(*p);
+  ba_build_system(*p);
+  if (Hpp) std::copy(p->Hpp.begin(), p->Hpp.end(), Hpp);
+  if (Hll) std::copy(p->Hll.begin(), p->Hll.end(), Hll);
+  if (b) std::copy(p->b.begin(), p->b.end(), b);
+  if (Hpl_edge) {
+    for (int k = 0; k < p->E; ++k) {
+      double* o = &Hpl_edge[18 * static_cast<size_t>(k)];
+      if (p->edgeBlk[k] < 0) { std::fill(o, o + 18, 0.0); continue; }
+      std::copy(&p->Hpl[18 * static_cast<size_t>(p->edgeBlk[k])], &p->Hpl[18 * static_cast<size_t>(p->edgeBlk[k])] + 18, o);
+    }
+  }
+  return chi;
+}
+
+// Schur complement of the last linearization with damping lambda. Writes the
+// upper blocks (rows/cols/36 values each, count = nschur of orc_ba_dims) and
+// bschur[6Cf]. Returns 0, or 2 when an Hll block is singular.
+int orc_ba_schur(void* h, double lambda, int* rows, int* cols, double* blocks, double* bschur) {
+  BA* p = static_cast<BA*>(h);
+  p->lambda = lambda;
+  int saved = p->solver_kind;
+  p->solver_kind = 0;
+  // reuse ba_solve's Schur part by solving with Cholesky, then export
+  bool ok = ba_solve(*p);
+  p->solver_kind = saved;
+  for (size_t k = 0; k < p->sCol.size(); ++k) {
+    if (rows) rows[k] = p->S.row[k];
+    if (cols) cols[k] = p->S.col[k];
+    if (blocks) std::copy(&p->S.val[36 * k], &p->S.val[36 * k] + 36, &blocks[36 * k]);
+  }
+  if (bschur) std::copy(p->bschur.begin(), p->bschur.end(), bschur);
+  return ok ? 0 : 2;
+}
+
+// One linear solve of the damped system at the last linearization:
+// solver 0 = Cholesky of the reduced system, 1 = PCG (tol, maxiter; the PCG
+// residual memory is reset first). dx[6Cf+3Pf]. Returns 0 or 2 (not PD).
+int orc_ba_solve(void* h, double lambda, int solver, double tol, int maxiter, double* dx, int* pcg_iters) {
+  BA* p = static_cast<BA*>(h);
+  p->lambda = lambda;
+  p->solver_kind = solver;
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  p->pcg.residual = -1.0;
+  bool ok = ba_solve(*p);
+  if (pcg_iters) *pcg_iters = p->pcg.last_iters;
+  if (!ok) return 2;
+  std::copy(p->x.begin(), p->x.end(), dx);
+  return 0;
+}
+
+// SparseOptimizer::optimize(iters) with OptimizationAlgorithmLevenberg
+// (optimization_algorithm_levenberg.cpp:60-153). trace[iters*6] per iteration:
+// chi2 before, chi2 after, lambda after, trials, pcg iterations (sum), status.
+// times[6]: residuals, quadratic form, schur, linear solver, update, total.
+// Returns the number of iterations run.
+int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, double* trace, double* times) {
+  BA* p = static_cast<BA*>(h);
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  for (double& t : p->times) t = 0;
+  double tall = now_s();
+  double lambda = 0, ni = 2;
+  int it;
+  for (it = 0; it < iters; ++it) {
+    double t0 = now_s();
+    double currentChi = ba_chi2(*p);
+    double t1 = now_s();
+    p->times[0] += t1 - t0;
+    ba_build_system(*p);
+    p->times[1] += now_s() - t1;
+    if (it == 0) { lambda = ba_lambda_init(*p); ni = 2; }
+    double chi_before = currentChi;
+    double rho = 0;
+    int q = 0, pcgsum = 0;
+    do {
+      ba_push(*p);
+      p->lambda = lambda;
+      bool ok = ba_solve(*p);
+      pcgsum += p->pcg.last_iters;
+      double tu = now_s();
+      // A failed solve is rejected below and the state popped, so the update
+      // the reference still applies (levenberg.cpp:112) has no effect.
+      if (ok) ba_update(*p, p->x.data());
+      p->times[4] += now_s() - tu;
+      double tc = now_s();
+      double tempChi = ba_chi2(*p);
+      p->times[0] += now_s() - tc;
+      if (!ok) tempChi = std::numeric_limits<double>::max();
+      rho = currentChi - tempChi;
+      double scale = ok ? ba_scale(*p) + 1e-3 : 1;
+      rho /= scale;
+      if (rho > 0 && std::isfinite(tempChi) && ok) {
+        double alpha = 1. - std::pow((2 * rho - 1), 3);
+        alpha = std::min(alpha, 2. / 3.);
+        double sf = std::max(1. / 3., alpha);
+        lambda *= sf;
+        ni = 2;
+        currentChi = tempChi;
+        ba_discard(*p);
+      } else {
+        lambda *= ni;
+        ni *= 2;
+        ba_pop(*p);
+        if (!std::isfinite(lambda)) break;
+      }
+      q++;
+    } while (rho < 0 && q < 10);
+    int status = (q == 10 || rho == 0 || !std::isfinite(lambda)) ? 1 : 0;
+    if (trace) {
+      trace[6 * it + 0] = chi_before;
+      trace[6 * it + 1] = currentChi;
+      trace[6 * it + 2] = lambda;
+      trace[6 * it + 3] = q;
+      trace[6 * it + 4] = pcgsum;
+      trace[6 * it + 5] = status;
+    }
+    if (status) { ++it; break; }
+  }
+  p->times[5] = now_s() - tall;
+  if (times) std::copy(p->times, p->times + 6, times);
+  return it;
+}
+
+// -------------------------------------------------------- single functions
+void orc_se3_exp(const double* upd6, double* out7) { se3_to7(se3_exp(upd6), out7); }
+void orc_se3_log(const double* a7, double* out6) { se3_log(se3_from7(a7), out6); }
+void orc_se3_mul(const double* a7, const double* b7, double* out7) {
+  se3_to7(se3_mul(se3_from7(a7), se3_from7(b7)), out7);
+}
+void orc_quat_from_R(const double* R9, double* q4) {
+  Quat q = R2q(R9);
+  q4[0] = q.x; q4[1] = q.y; q4[2] = q.z; q4[3] = q.w;
+}
+void orc_quat_to_R(const double* q4, double* R9) { q2R({q4[0], q4[1], q4[2], q4[3]}, R9); }
+void orc_huber(double e, double delta, double* rho3) { huber(e, delta, rho3); }
+void orc_project_edge(const double* cam7, const double* intr4, const double* X3, const double* z2,
+                      double* e2, double* Jp6, double* Jc12) {
+  SE3 T = se3_from7(cam7);
+  double R[9];
+  q2R(T.q, R);
+  project_edge(T, R, intr4, X3, z2, e2, Jp6, Jc12);
+}
+
+// Block-Jacobi PCG on a symmetric matrix given by its upper blocks
+// (brow <= bcol, bd x bd row-major values). residual_io carries the PCG
+// residual memory (-1 for a fresh solver). Returns iterations, or -1 when a
+// diagonal block is missing or singular.
+int orc_pcg_blocks(int nb, int bd, int nblk, const int* brow, const int* bcol, const double* blocks,
+                   const double* b, double* x, double tol, int maxiter, double* residual_io) {
+  BlockSym A;
+  A.bd = bd; A.nb = nb;
+  A.row.assign(brow, brow + nblk);
+  A.col.assign(bcol, bcol + nblk);
+  A.val.assign(blocks, blocks + static_cast<size_t>(nblk) * bd * bd);
+  A.diag.assign(nb, -1);
+  for (int k = 0; k < nblk; ++k) if (brow[k] == bcol[k]) A.diag[brow[k]] = k;
+  PCGState st;
+  st.tol = tol; st.max_iter = maxiter; st.residual = residual_io ? *residual_io : -1.0;
+  if (!pcg_solve(A, b, x, st)) return -1;
+  if (residual_io) *residual_io = st.residual;
+  return st.last_iters;
+}
+
+// Dense Cholesky solve (row-major full A). Returns 0, or 2 when not PD.
+int orc_chol_solve(int n, const double* A, const double* b, double* x) {
+  return chol_solve(n, A, b, x) ? 0 : 2;
+}
+
+int orc_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+}  // extern "C"

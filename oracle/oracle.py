@@ -1,0 +1,222 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/g2o_oracle.cpp).
+
+Test infrastructure only: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libg2o_oracle.so")
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int)
+_up = ct.POINTER(ct.c_ubyte)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with make (g++ -fopenmp)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ct.CDLL(LIB_PATH)
+        L.orc_ba_new.restype = ct.c_void_p
+        L.orc_ba_new.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _ip, _ip, _dp, _dp, _up, _up,
+                                 ct.c_double]
+        L.orc_ba_free.argtypes = [ct.c_void_p]
+        L.orc_ba_dims.argtypes = [ct.c_void_p, _ip, _ip, _ip]
+        L.orc_ba_get_state.argtypes = [ct.c_void_p, _dp, _dp]
+        L.orc_ba_set_state.argtypes = [ct.c_void_p, _dp, _dp]
+        L.orc_ba_chi2.restype = ct.c_double
+        L.orc_ba_chi2.argtypes = [ct.c_void_p]
+        L.orc_ba_linearize.restype = ct.c_double
+        L.orc_ba_linearize.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_ba_schur.restype = ct.c_int
+        L.orc_ba_schur.argtypes = [ct.c_void_p, ct.c_double, _ip, _ip, _dp, _dp]
+        L.orc_ba_solve.restype = ct.c_int
+        L.orc_ba_solve.argtypes = [ct.c_void_p, ct.c_double, ct.c_int, ct.c_double, ct.c_int, _dp, _ip]
+        L.orc_ba_optimize.restype = ct.c_int
+        L.orc_ba_optimize.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp, _dp]
+        L.orc_se3_exp.argtypes = [_dp, _dp]
+        L.orc_se3_log.argtypes = [_dp, _dp]
+        L.orc_se3_mul.argtypes = [_dp, _dp, _dp]
+        L.orc_quat_from_R.argtypes = [_dp, _dp]
+        L.orc_quat_to_R.argtypes = [_dp, _dp]
+        L.orc_huber.argtypes = [ct.c_double, ct.c_double, _dp]
+        L.orc_project_edge.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_pcg_blocks.restype = ct.c_int
+        L.orc_pcg_blocks.argtypes = [ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _dp, _dp, _dp, ct.c_double, ct.c_int,
+                                     _dp]
+        L.orc_chol_solve.restype = ct.c_int
+        L.orc_chol_solve.argtypes = [ct.c_int, _dp, _dp, _dp]
+        L.orc_num_threads.restype = ct.c_int
+        L.orc_set_num_threads.argtypes = [ct.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _u(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a, a.ctypes.data_as(_up)
+
+
+class OracleBA:
+    """FP64 CPU restatement of g2o's BlockSolver_6_3 + LM on one BA problem."""
+
+    def __init__(self, prob):
+        L = lib()
+        keep = []
+        args = []
+        for arr, conv in ((prob.cams, _d), (prob.intr, _d), (prob.pts, _d), (prob.edge_cam, _i),
+                          (prob.edge_pt, _i), (prob.meas, _d), (prob.weight, _d), (prob.cam_fixed, _u),
+                          (prob.pt_fixed, _u)):
+            a, p = conv(arr)
+            keep.append(a)
+            args.append(p)
+        self.C, self.P, self.E = prob.num_cams, prob.num_pts, prob.num_edges
+        self.h = L.orc_ba_new(self.C, self.P, self.E, *args, float(prob.huber))
+        cf, pf, ns = ct.c_int(), ct.c_int(), ct.c_int()
+        L.orc_ba_dims(self.h, ct.byref(cf), ct.byref(pf), ct.byref(ns))
+        self.Cf, self.Pf, self.nschur = cf.value, pf.value, ns.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_ba_free(self.h)
+            self.h = None
+
+    def state(self):
+        c = np.zeros((self.C, 7))
+        p = np.zeros((self.P, 3))
+        lib().orc_ba_get_state(self.h, _d(c)[1], _d(p)[1])
+        return c, p
+
+    def set_state(self, cams, pts):
+        c, pc = _d(cams)
+        p, pp = _d(pts)
+        lib().orc_ba_set_state(self.h, pc, pp)
+
+    def chi2(self) -> float:
+        return lib().orc_ba_chi2(self.h)
+
+    def linearize(self):
+        Hpp = np.zeros((self.Cf, 6, 6))
+        Hll = np.zeros((self.Pf, 3, 3))
+        Hpl = np.zeros((self.E, 6, 3))
+        b = np.zeros(6 * self.Cf + 3 * self.Pf)
+        chi = lib().orc_ba_linearize(self.h, Hpp.ctypes.data_as(_dp), Hll.ctypes.data_as(_dp),
+                                     Hpl.ctypes.data_as(_dp), b.ctypes.data_as(_dp))
+        return chi, Hpp, Hll, Hpl, b
+
+    def schur(self, lam: float):
+        rows = np.zeros(self.nschur, dtype=np.int32)
+        cols = np.zeros(self.nschur, dtype=np.int32)
+        blocks = np.zeros((self.nschur, 6, 6))
+        bs = np.zeros(6 * self.Cf)
+        st = lib().orc_ba_schur(self.h, lam, rows.ctypes.data_as(_ip), cols.ctypes.data_as(_ip),
+                                blocks.ctypes.data_as(_dp), bs.ctypes.data_as(_dp))
+        return st, rows, cols, blocks, bs
+
+    def solve(self, lam: float, solver: int = 0, tol: float = 1e-6, maxiter: int = -1):
+        dx = np.zeros(6 * self.Cf + 3 * self.Pf)
+        it = ct.c_int()
+        st = lib().orc_ba_solve(self.h, lam, solver, tol, maxiter, dx.ctypes.data_as(_dp), ct.byref(it))
+        return st, dx, it.value
+
+    def optimize(self, iters: int, solver: int = 1, tol: float = 1e-6, maxiter: int = -1):
+        trace = np.zeros((iters, 6))
+        times = np.zeros(6)
+        n = lib().orc_ba_optimize(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp),
+                                  times.ctypes.data_as(_dp))
+        return n, trace[:n], times
+
+
+def se3_exp(upd):
+    out = np.zeros(7)
+    lib().orc_se3_exp(_d(upd)[1], out.ctypes.data_as(_dp))
+    return out
+
+
+def se3_log(p7):
+    out = np.zeros(6)
+    lib().orc_se3_log(_d(p7)[1], out.ctypes.data_as(_dp))
+    return out
+
+
+def se3_mul(a7, b7):
+    out = np.zeros(7)
+    lib().orc_se3_mul(_d(a7)[1], _d(b7)[1], out.ctypes.data_as(_dp))
+    return out
+
+
+def quat_from_R(R):
+    out = np.zeros(4)
+    lib().orc_quat_from_R(_d(R)[1], out.ctypes.data_as(_dp))
+    return out
+
+
+def quat_to_R(q):
+    out = np.zeros((3, 3))
+    lib().orc_quat_to_R(_d(q)[1], out.ctypes.data_as(_dp))
+    return out
+
+
+def huber(e, delta):
+    out = np.zeros(3)
+    lib().orc_huber(float(e), float(delta), out.ctypes.data_as(_dp))
+    return out
+
+
+def project_edge(cam7, intr4, X, z):
+    e = np.zeros(2)
+    Jp = np.zeros((2, 3))
+    Jc = np.zeros((2, 6))
+    lib().orc_project_edge(_d(cam7)[1], _d(intr4)[1], _d(X)[1], _d(z)[1], e.ctypes.data_as(_dp),
+                           Jp.ctypes.data_as(_dp), Jc.ctypes.data_as(_dp))
+    return e, Jp, Jc
+
+
+def pcg_blocks(nb, bd, rows, cols, blocks, b, tol=1e-6, maxiter=-1, residual=-1.0):
+    """Returns (x, iterations, residual memory)."""
+    x = np.zeros(nb * bd)
+    r = ct.c_double(residual)
+    rr, rp = _i(rows)
+    cc, cp = _i(cols)
+    bb, bp = _d(blocks)
+    b2, b2p = _d(b)
+    it = lib().orc_pcg_blocks(nb, bd, len(rows), rp, cp, bp, b2p, x.ctypes.data_as(_dp), tol, maxiter,
+                              ct.byref(r))
+    return x, it, r.value
+
+
+def chol_solve(A, b):
+    A, ap = _d(A)
+    b, bp = _d(b)
+    x = np.zeros(b.size)
+    st = lib().orc_chol_solve(b.size, ap, bp, x.ctypes.data_as(_dp))
+    return st, x

@@ -1,0 +1,209 @@
+"""GPU parity of the bundle-adjustment path against the FP64 CPU oracle.
+
+Every call goes through the C ABI (libg2o_hip.so). Tolerances:
+* FP64 path: blocks, gradients and chi2 relative 1e-11 (only the summation
+  order differs from the oracle);
+* FP32 path: Hessian blocks relative 2e-5 of the block norm; final LM chi2
+  relative 1e-6 of the FP64 result.
+"""
+import numpy as np
+import pytest
+
+from g2o_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def small():
+    return datasets.make_small_ba(seed=1000)
+
+
+@pytest.fixture(scope="module")
+def small_huber():
+    p = datasets.make_small_ba(seed=1001)
+    p.huber = 2.0
+    # a few gross outliers so that the Huber branch is taken
+    p.meas[::37] += 25.0
+    return p
+
+
+@pytest.fixture(scope="module")
+def walk():
+    return datasets.make_walk_ba(seed=7, num_cams=60, num_pts=6000)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _mk(prob, **kw):
+    import g2o_amd
+    return g2o_amd.BundleAdjuster(prob, device=0, **kw)
+
+
+@pytest.mark.parametrize("case", ["small", "small_huber", "walk"])
+def test_chi2_and_hessian_fp64(case, request, oracle_lib):
+    prob = request.getfixturevalue(case)
+    _say(f"case {case}: C={prob.num_cams} P={prob.num_pts} E={prob.num_edges}")
+    import g2o_amd
+    ba = _mk(prob, precision=g2o_amd.FP64)
+    ora = oracle_lib.OracleBA(prob)
+    chi_o, Hpp_o, Hll_o, Hpl_o, b_o = ora.linearize()
+    chi_g = ba.compute_active_errors()
+    assert abs(chi_g - chi_o) <= 1e-11 * abs(chi_o)
+    ba.build_system()
+    Hpp, Hll, Hpl = ba.hessian()
+    b = ba.b()
+    assert _rel(Hpp, Hpp_o) < 1e-11
+    assert _rel(Hll, Hll_o) < 1e-11
+    assert _rel(Hpl, Hpl_o) < 1e-11
+    assert _rel(b, b_o) < 1e-11
+    # errors per edge
+    e = ba.errors()
+    assert e.shape == (prob.num_edges, 2)
+
+
+@pytest.mark.parametrize("case", ["small", "walk"])
+def test_hessian_fp32(case, request, oracle_lib):
+    prob = request.getfixturevalue(case)
+    import g2o_amd
+    ba = _mk(prob, precision=g2o_amd.FP32)
+    ora = oracle_lib.OracleBA(prob)
+    chi_o, Hpp_o, Hll_o, Hpl_o, b_o = ora.linearize()
+    ba.build_system()
+    Hpp, Hll, Hpl = ba.hessian()
+    for a, o in ((Hpp, Hpp_o), (Hll, Hll_o), (Hpl, Hpl_o)):
+        scale = np.maximum(np.linalg.norm(o.reshape(o.shape[0], -1), axis=1), 1e-300)
+        err = np.linalg.norm((a - o).reshape(o.shape[0], -1), axis=1) / scale
+        assert err.max() < 2e-5
+    assert _rel(ba.b(), b_o) < 2e-5
+
+
+@pytest.mark.parametrize("case", ["small", "small_huber", "walk"])
+def test_schur_fp64(case, request, oracle_lib):
+    prob = request.getfixturevalue(case)
+    import g2o_amd
+    ba = _mk(prob, precision=g2o_amd.FP64, pcg_tolerance=1e-30, pcg_max_iter=400)
+    ora = oracle_lib.OracleBA(prob)
+    ora.linearize()
+    lam = 3.7
+    st, rows_o, cols_o, blocks_o, bs_o = ora.schur(lam)
+    assert st == 0
+    ba.build_system()
+    ba.set_lambda(lam)
+    assert ba.solve()
+    rows, cols, blocks, bs = ba.schur(ora.nschur)
+    assert np.array_equal(rows, rows_o) and np.array_equal(cols, cols_o)
+    assert _rel(blocks, blocks_o) < 1e-11
+    assert _rel(bs, bs_o) < 1e-11
+    # the PCG run to convergence gives the dense Cholesky solution
+    st, dx_o, _ = ora.solve(lam, solver=0)
+    assert st == 0
+    assert _rel(ba.x(), dx_o) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["small", "walk"])
+def test_pcg_iterations_match_reference_rule(case, request, oracle_lib):
+    prob = request.getfixturevalue(case)
+    import g2o_amd
+    ba = _mk(prob, precision=g2o_amd.FP64)
+    ora = oracle_lib.OracleBA(prob)
+    ora.linearize()
+    ba.build_system()
+    lam = 1.0
+    st, dx_o, it_o = ora.solve(lam, solver=1, tol=1e-6)
+    ba.set_lambda(lam)
+    assert ba.solve()
+    info = ba.last_solve_info()
+    _say(f"pcg iterations gpu {info[0]} oracle {it_o}")
+    assert int(info[0]) == it_o
+    assert _rel(ba.x(), dx_o) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["small", "small_huber", "walk"])
+def test_lm_trace_fp64(case, request, oracle_lib):
+    prob = request.getfixturevalue(case)
+    import g2o_amd
+    ba = _mk(prob, precision=g2o_amd.FP64)
+    ora = oracle_lib.OracleBA(prob)
+    n_o, trace, _ = ora.optimize(8, solver=1, tol=1e-6)
+    stats = ba.optimize(8)
+    _say(f"{case}: gpu chi2 {[s['chi2'] for s in stats]}")
+    _say(f"{case}: cpu chi2 {list(trace[:, 1])}")
+    assert len(stats) == n_o
+    for s, t in zip(stats, trace):
+        assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
+        assert s["levenberg_iterations"] == int(t[3])
+        assert s["iterations_linear_solver"] == int(t[4])
+    c, p = ba.state()
+    co, po = ora.state()
+    # inexact PCG (tol 1e-6) lets rounding differences grow slowly over 8 iterations
+    assert _rel(p, po) < 1e-6
+
+
+@pytest.mark.parametrize("refine", [0, 1, 2])
+def test_lm_fp32_refine(refine, walk, oracle_lib):
+    import g2o_amd
+    ba64 = _mk(walk, precision=g2o_amd.FP64)
+    s64 = ba64.optimize(8)
+    ba32 = _mk(walk, precision=g2o_amd.FP32, refine_steps=refine)
+    s32 = ba32.optimize(8)
+    _say(f"refine {refine}: fp64 {s64[-1]['chi2']:.10e} fp32 {s32[-1]['chi2']:.10e}")
+    rel = abs(s32[-1]["chi2"] - s64[-1]["chi2"]) / s64[-1]["chi2"]
+    assert rel < 1e-6
+
+
+def test_linear_solver_pcg_fixture():
+    """LinearSolverPCG on the reference's 36x36 block fixture
+    (unit_test/solver/sparse_system_helper.cpp), solved twice as in
+    linear_solver_test.cpp:89-101, to 1e-6."""
+    import json
+    import os
+    import g2o_amd
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "sparse_system_36.json")))
+    ctx = g2o_amd.Context(0)
+    rows, cols = np.array(fx["rows"]), np.array(fx["cols"])
+    blocks = np.array(fx["blocks"]).reshape(-1, 3, 3)
+    b = np.array(fx["b"])
+    x_ref = np.array(fx["x"])
+    resid = -1.0
+    for _ in range(2):
+        x, it, resid = g2o_amd.linear_solve_pcg(ctx, 12, 3, rows, cols, blocks, b, tol=1e-12, max_iter=200,
+                                                residual=-1.0)
+        assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref) + 1e-7
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_implicit_matches_explicit(prec, walk):
+    """Implicit (matrix-free) Schur inside the PCG applies the same reduced
+    operator as the explicit Hschur: run to a tight tolerance both give the
+    same step."""
+    import g2o_amd
+    p = g2o_amd.FP64 if prec == "fp64" else g2o_amd.FP32
+    tol = 1e-24 if prec == "fp64" else 1e-12
+    xs = []
+    for mode in (g2o_amd.SCHUR_EXPLICIT, g2o_amd.SCHUR_IMPLICIT):
+        ba = _mk(walk, precision=p, schur_mode=mode, pcg_tolerance=tol, pcg_max_iter=600, refine_steps=0)
+        ba.build_system()
+        ba.set_lambda(2.5)
+        assert ba.solve()
+        xs.append(ba.x())
+    bound = 1e-9 if prec == "fp64" else 2e-4
+    _say(f"implicit vs explicit {prec}: {_rel(xs[1], xs[0]):.3e}")
+    assert _rel(xs[1], xs[0]) < bound
+
+
+def test_profile_counts(walk):
+    import g2o_amd
+    ba = _mk(walk, precision=g2o_amd.FP32)
+    ba.set_profiling(True)
+    ba.optimize(2)
+    prof = ba.profile()
+    _say(f"profile {prof}")
+    assert prof["linearize"][1] >= 1 and prof["schur"][1] >= 1 and prof["pcg"][1] >= 1
+    assert all(v[0] >= 0 for v in prof.values())
