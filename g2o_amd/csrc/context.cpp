@@ -351,6 +351,37 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, bool use_memory, boo
   const int nb = st.Cf;
   const int n = 6 * nb;
   HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
+  HK(hipMemsetAsync(d_sc.p, 0, sizeof(PcgScalars), stream));
+  const int max_iter_p = opt.pcg_max_iter < 0 ? n : opt.pcg_max_iter;
+  if (!implicit) {
+    const int wi = std::is_same<HT, float>::value ? 0 : 1;
+    if (pcg_grid[wi] < 0) {
+      const int mx = pcg_persistent_max_grid<HT>(device);
+      pcg_grid[wi] = std::min({mx, 256, (nb + 3) / 4});
+    }
+    if (pcg_grid[wi] > 0) {
+      HK(B.d2.alloc(n));
+      HK(d_bar.alloc(2));
+      HK(d_gpart.alloc(2 * static_cast<size_t>(pcg_grid[wi])));
+      ProfScope ps(this, G2OHIP_K_PCG);
+      hipError_t e = launch_pcg_persistent<HT>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.Jinv.p, rhs, B.x.p, B.r.p,
+                                               B.d.p, B.d2.p, B.q.p, B.s.p, d_gpart.p, d_bar.p, d_sc.p,
+                                               opt.pcg_tolerance, use_memory ? pcg_residual_mem : -1.0, max_iter_p,
+                                               pcg_grid[wi], stream);
+      if (e == hipSuccess) {
+        PcgScalars hs{};
+        HK(hipMemcpyAsync(h_pinned, d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
+        HK(hipStreamSynchronize(stream));
+        std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
+        if (hs.err) return fail(G2OHIP_ERR_HIP, "persistent PCG grid barrier timed out");
+        *iters = hs.iters;
+        *dn_final = hs.dn[hs.iters & 1];
+        return G2OHIP_OK;
+      }
+      (void)hipGetLastError();
+      pcg_grid[wi] = 0;  // not co-resident on this device: use the kernel-per-step loop
+    }
+  }
   HK(launch_pcg_init<HT, 6>(nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.d.p, d_dn_part.p, stream));
   HK(launch_pcg_start(nb, d_dn_part.p, opt.pcg_tolerance, use_memory ? pcg_residual_mem : -1.0, d_sc.p, stream));
   const int max_iter = opt.pcg_max_iter < 0 ? n : opt.pcg_max_iter;

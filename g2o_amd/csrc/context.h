@@ -49,7 +49,7 @@ struct PrecBufs {
   DBuf<HT> Jinv;      // Cf * 36
   DBuf<HT> Dinv;      // Pf * 9 (implicit mode)
   DBuf<HT> Sdiag;     // Cf * 36 (implicit mode preconditioner)
-  DBuf<HT> x, r, d, q, s;  // PCG vectors
+  DBuf<HT> x, r, d, d2, q, s;  // PCG vectors
 };
 
 struct RowPlanBufs {
@@ -92,6 +92,9 @@ struct g2ohip_ctx {
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;
   g2ohip::DBuf<int> d_err;
+  g2ohip::DBuf<unsigned> d_bar;
+  g2ohip::DBuf<double> d_gpart;
+  int pcg_grid[2] = {-1, -1};  // persistent PCG grid per precision (0: unavailable)
   g2ohip::DBuf<g2ohip::PcgScalars> d_sc;
   g2ohip::PrecBufs<float> f32;
   g2ohip::PrecBufs<double> f64;

@@ -106,6 +106,19 @@ hipError_t launch_pcg_step(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, co
 template <typename HT, int BD>
 hipError_t launch_pcg_dir(int nb, HT* d, const HT* s, const double* dn_part, PcgScalars* sc, int it,
                           hipStream_t st);
+// Whole PCG solve in one cooperative launch (explicit Hschur, BD = 6). The
+// workgroups own block rows and meet at two grid barriers per iteration; the
+// convergence test and all dot products are evaluated identically by every
+// workgroup. rhs in f64; x, r, d0, d1, q, s in HT; part: 2 * grid doubles;
+// bar: 2 unsigned (zeroed by the launcher). The launch fails (and the caller
+// uses the kernel-per-step loop) if the grid cannot be co-resident.
+template <typename HT>
+hipError_t launch_pcg_persistent(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* Jinv,
+                                 const double* rhs, HT* x, HT* r, HT* d0, HT* d1, HT* q, HT* s, double* part,
+                                 unsigned* bar, PcgScalars* sc, double tol, double resid_mem, int max_iter,
+                                 int grid, hipStream_t st);
+template <typename HT>
+int pcg_persistent_max_grid(int device);
 template <typename HT>
 hipError_t launch_to_double(int n, const HT* src, double* dst, double scale_add, hipStream_t st);
 hipError_t launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t st);

@@ -390,7 +390,9 @@ __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, c
                                                     const double* __restrict__ Hpp, double lambda,
                                                     HT* __restrict__ F_val, double* __restrict__ bschur,
                                                     HT* __restrict__ Sdiag, int diag_only, int* __restrict__ err) {
+  constexpr int MAPW = 1024;  // direct column map when the row's columns span < MAPW cameras
   __shared__ int s_col[MAXC];
+  __shared__ int s_map[MAPW];
   __shared__ HT s_acc[MAXC * 36];
   __shared__ double s_red[4 * 42];
   const int tile = rp.tile_order[blockIdx.x];
@@ -400,12 +402,17 @@ __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, c
   const bool diag_tile = (k0 == rp.S_rowptr[row]);
   if (diag_only && !diag_tile) return;
   const int tid = threadIdx.x;
+  const int col_lo = g.S_col[k0], col_hi = g.S_col[k1 - 1];
+  const bool use_map = (col_hi - col_lo) < MAPW;
   for (int i = tid; i < ncol; i += 256) s_col[i] = g.S_col[k0 + i];
   for (int i = tid; i < ncol * 36; i += 256) s_acc[i] = HT(0);
+  if (use_map) {
+    __syncthreads();
+    for (int i = tid; i < ncol; i += 256) s_map[s_col[i] - col_lo] = i;
+  }
   __syncthreads();
-  const int col_lo = s_col[0], col_hi = s_col[ncol - 1];
-  HT dacc[36];
-  for (int e = 0; e < 36; ++e) dacc[e] = HT(0);
+  HT dacc[21];  // upper triangle of the diagonal block (symmetric)
+  for (int e = 0; e < 21; ++e) dacc[e] = HT(0);
   double coef[6] = {0, 0, 0, 0, 0, 0};
   const double* bl_base = b + 6 * static_cast<size_t>(g.Cf);
   for (int e = rp.row_ptr[row] + tid; e < rp.row_ptr[row + 1]; e += 256) {
@@ -418,76 +425,80 @@ __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, c
       atomicOr(err, 1);
       for (int i = 0; i < 9; ++i) D[i] = 0.0;
     }
-    HT Ha[18];
-    const HT* pa = Hpl + 18 * static_cast<size_t>(a);
-    for (int k = 0; k < 18; ++k) Ha[k] = pa[k];
     HT W[18];
     {
+      const HT* pa = Hpl + 18 * static_cast<size_t>(a);
+      HT Ha[18];
+      for (int k = 0; k < 18; ++k) Ha[k] = pa[k];
       HT Dh[9];
       for (int k = 0; k < 9; ++k) Dh[k] = static_cast<HT>(D[k]);
       for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 3; ++c) W[3 * r + c] = Ha[3 * r] * Dh[c] + Ha[3 * r + 1] * Dh[3 + c] + Ha[3 * r + 2] * Dh[6 + c];
-    }
-    if (diag_tile && !diag_only) {
-      const double* bl = bl_base + 3 * static_cast<size_t>(ph);
-      double db[3];
-      for (int r = 0; r < 3; ++r) db[r] = D[3 * r] * bl[0] + D[3 * r + 1] * bl[1] + D[3 * r + 2] * bl[2];
-      for (int r = 0; r < 6; ++r)
-        coef[r] += static_cast<double>(Ha[3 * r]) * db[0] + static_cast<double>(Ha[3 * r + 1]) * db[1] +
-                   static_cast<double>(Ha[3 * r + 2]) * db[2];
-    }
-    for (int bs = a; bs < send; ++bs) {
-      const int cb = g.camH[g.slot_cam[bs]];
-      if (cb > col_hi) break;  // slots ascend by camera
-      if (cb < col_lo) continue;
-      HT Hb[18];
-      if (bs == a) {
-        for (int k = 0; k < 18; ++k) Hb[k] = Ha[k];
-      } else {
-        const HT* pb = Hpl + 18 * static_cast<size_t>(bs);
-        for (int k = 0; k < 18; ++k) Hb[k] = pb[k];
+      if (diag_tile && !diag_only) {
+        const double* bl = bl_base + 3 * static_cast<size_t>(ph);
+        double db[3];
+        for (int r = 0; r < 3; ++r) db[r] = D[3 * r] * bl[0] + D[3 * r + 1] * bl[1] + D[3 * r + 2] * bl[2];
+        for (int r = 0; r < 6; ++r)
+          coef[r] += static_cast<double>(Ha[3 * r]) * db[0] + static_cast<double>(Ha[3 * r + 1]) * db[1] +
+                     static_cast<double>(Ha[3 * r + 2]) * db[2];
       }
-      HT P[36];
-      for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c)
-          P[6 * r + c] = W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2];
+    }
+    // Walk the point's slots b >= a starting at a lane-dependent offset:
+    // neighbouring lanes hold points with near-identical camera tracks, and
+    // the rotation keeps them from hitting the same block column together.
+    const int m = send - a;
+    const int rot = tid % m;
+    for (int j = 0; j < m; ++j) {
+      int bs = a + j + rot;
+      if (bs >= send) bs -= m;
+      const int cb = g.camH[g.slot_cam[bs]];
+      if (cb > col_hi || cb < col_lo) continue;
+      HT Hb[18];
+      const HT* pb = Hpl + 18 * static_cast<size_t>(bs);
+      for (int k = 0; k < 18; ++k) Hb[k] = pb[k];
       if (cb == row) {
-        if (bs == a) {
-          for (int k = 0; k < 36; ++k) dacc[k] += P[k];
-        } else {
-          // a second edge of this point to camera i: g2o merges the two into
-          // one Hpl block, whose product also holds the transposed cross term
-          for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) dacc[6 * r + c] += P[6 * r + c] + P[6 * c + r];
-        }
+        // diagonal block: W Hb^T, symmetric for b == a; a second edge of this
+        // point to camera i adds the product and its transpose (g2o merges
+        // the two edges into one Hpl block)
+        int k = 0;
+        for (int r = 0; r < 6; ++r)
+          for (int c = r; c < 6; ++c, ++k) {
+            HT v = W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2];
+            if (bs != a) v += W[3 * c] * Hb[3 * r] + W[3 * c + 1] * Hb[3 * r + 1] + W[3 * c + 2] * Hb[3 * r + 2];
+            dacc[k] += v;
+          }
       } else {
-        const int lc = lower_bound_i(s_col, ncol, cb);
+        const int lc = use_map ? s_map[cb - col_lo] : lower_bound_i(s_col, ncol, cb);
         HT* acc = s_acc + 36 * lc;
-        for (int k = 0; k < 36; ++k) atomicAdd(&acc[k], -P[k]);
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 6; ++c)
+            atomicAdd(&acc[6 * r + c], -(W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2]));
       }
     }
   }
   // workgroup reduction of the register accumulators (diagonal tile only)
   if (diag_tile) {
     const int lane = tid & 63, wv = tid >> 6;
-    for (int k = 0; k < 36; ++k) {
+    for (int k = 0; k < 21; ++k) {
       double v = wave_sum(static_cast<double>(dacc[k]));
       if (lane == 0) s_red[42 * wv + k] = v;
     }
     for (int k = 0; k < 6; ++k) {
       double v = wave_sum(coef[k]);
-      if (lane == 0) s_red[42 * wv + 36 + k] = v;
+      if (lane == 0) s_red[42 * wv + 21 + k] = v;
     }
   }
   __syncthreads();
-  if (diag_tile && tid < 42) {
+  if (diag_tile && tid < 27) {
     const double v = s_red[tid] + s_red[42 + tid] + s_red[84 + tid] + s_red[126 + tid];
-    if (tid < 36) {
-      double h = Hpp[36 * static_cast<size_t>(row) + tid] - v;
-      if (!diag_only && tid % 7 == 0) h += lambda;
-      s_acc[tid] = static_cast<HT>(h);  // column 0 of the diagonal tile is the diagonal
+    if (tid < 21) {
+      const int r = c_up6_r[tid], c = c_up6_c[tid];
+      double h = Hpp[36 * static_cast<size_t>(row) + 6 * r + c] - v;
+      if (!diag_only && r == c) h += lambda;
+      s_acc[6 * r + c] = static_cast<HT>(h);  // column 0 of the diagonal tile is the diagonal
+      s_acc[6 * c + r] = static_cast<HT>(h);
     } else if (!diag_only) {
-      const int r = tid - 36;
+      const int r = tid - 21;
       bschur[6 * static_cast<size_t>(row) + r] = b[6 * static_cast<size_t>(row) + r] - v;
     }
   }
@@ -1055,6 +1066,192 @@ G2OHIP_PCG_INST(float, 6)
 G2OHIP_PCG_INST(double, 6)
 G2OHIP_PCG_INST(double, 3)
 #undef G2OHIP_PCG_INST
+
+// ----------------------------------------------- persistent (one launch) PCG
+__global__ void k_zero_u32(unsigned* p, int n) {
+  if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0u;
+}
+
+// Grid barrier: one arrival counter and one generation word. The last
+// workgroup to arrive resets the counter and bumps the generation; the others
+// poll it with a bounded spin (a stuck barrier sets sc->err and every
+// workgroup leaves the solve instead of waiting forever).
+__device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, PcgScalars* sc) {
+  __shared__ int s_ok;
+  __threadfence();  // every wave's global stores are complete and visible at agent scope
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_ok = 1;
+    __threadfence();
+    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == nblocks - 1) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar[1], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (++spins > (1L << 26)) {
+          __hip_atomic_store(&sc->err, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          s_ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+template <typename HT>
+__global__ __launch_bounds__(256) void k_pcg_persistent(int nb, const int* __restrict__ F_ptr,
+                                                        const int* __restrict__ F_col, const HT* __restrict__ F_val,
+                                                        const HT* __restrict__ J, const double* __restrict__ rhs,
+                                                        HT* __restrict__ x, HT* __restrict__ r, HT* d0, HT* d1,
+                                                        HT* __restrict__ q, HT* __restrict__ s, double* part,
+                                                        unsigned* bar, PcgScalars* sc, double tol, double resid_mem,
+                                                        int max_iter) {
+  constexpr int BD = 6, G = 64 / BD;
+  __shared__ double s_red[512];
+  __shared__ HT s_part[4][64];
+  __shared__ double s_wsum[4];
+  const unsigned nblk = gridDim.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double* part_a = part;         // d . q per workgroup
+  double* part_b = part + nblk;  // r . s per workgroup
+  // ---- x = 0, r = b, d = M^-1 r, dn = r . d   (linear_solver_pcg.hpp:120-127)
+  double loc = 0;
+  for (int i = blockIdx.x * 256 + tid; i < nb; i += nblk * 256) {
+    HT rv[BD];
+    for (int k = 0; k < BD; ++k) {
+      rv[k] = static_cast<HT>(rhs[BD * static_cast<size_t>(i) + k]);
+      r[BD * static_cast<size_t>(i) + k] = rv[k];
+      x[BD * static_cast<size_t>(i) + k] = HT(0);
+    }
+    const HT* Ji = J + static_cast<size_t>(i) * 36;
+    for (int k = 0; k < BD; ++k) {
+      HT v = 0;
+      for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+      d0[BD * static_cast<size_t>(i) + k] = v;
+      loc += static_cast<double>(rv[k]) * static_cast<double>(v);
+    }
+  }
+  loc = block_sum(loc, s_wsum);
+  if (tid == 0) part_b[blockIdx.x] = loc;
+  if (!grid_barrier(bar, nblk, sc)) return;
+  double dn = block_sum_array(part_b, nblk, s_red);
+  double dlim = tol * dn;
+  if (resid_mem > 0.0 && resid_mem > dlim) dlim = resid_mem;
+  double beta = 0.0;
+  int it = 0;
+  for (; it < max_iter; ++it) {
+    if (dn <= dlim) break;  // linear_solver_pcg.hpp:141
+    HT* dold = (it & 1) ? d1 : d0;
+    HT* dnew = (it & 1) ? d0 : d1;
+    const HT bh = static_cast<HT>(beta);
+    // ---- phase A: d = s + beta d (formed on the fly from the previous
+    // iteration's s and d, both complete), q = A d for the owned block rows;
+    // one wavefront per block row, 64 / 6 lane groups over the row's blocks.
+    double dq_loc = 0;
+    for (int row0 = blockIdx.x * 4; row0 < nb; row0 += nblk * 4) {
+      const int row = row0 + wv;
+      const int grp = lane / BD, k = lane % BD;
+      HT acc = 0;
+      if (row < nb && grp < G) {
+        for (int p = F_ptr[row] + grp; p < F_ptr[row + 1]; p += G) {
+          const HT* V = F_val + static_cast<size_t>(p) * 36 + BD * k;
+          const size_t cj = BD * static_cast<size_t>(F_col[p]);
+          for (int c = 0; c < BD; ++c) {
+            const HT dv = (it == 0) ? dold[cj + c] : s[cj + c] + bh * dold[cj + c];
+            acc += V[c] * dv;
+          }
+        }
+      }
+      s_part[wv][lane] = acc;
+      __syncthreads();
+      if (row < nb && lane < BD) {
+        HT v = 0;
+        for (int g2 = 0; g2 < G; ++g2) v += s_part[wv][g2 * BD + lane];
+        const size_t o = BD * static_cast<size_t>(row) + lane;
+        const HT dv = (it == 0) ? dold[o] : s[o] + bh * dold[o];
+        dnew[o] = dv;
+        q[o] = v;
+        dq_loc += static_cast<double>(v) * static_cast<double>(dv);
+      }
+      __syncthreads();
+    }
+    dq_loc = block_sum(dq_loc, s_wsum);
+    if (tid == 0) part_a[blockIdx.x] = dq_loc;
+    if (!grid_barrier(bar, nblk, sc)) return;
+    // ---- phase B: a = dn / d.q; x += a d; r -= a q; s = M^-1 r; dn' = r . s
+    const double dq = block_sum_array(part_a, nblk, s_red);
+    const double alpha = dn / dq;
+    const HT ah = static_cast<HT>(alpha);
+    double dn_loc = 0;
+    for (int i = blockIdx.x * 256 + tid; i < nb; i += nblk * 256) {
+      HT rv[BD];
+      for (int k = 0; k < BD; ++k) {
+        const size_t o = BD * static_cast<size_t>(i) + k;
+        x[o] += ah * dnew[o];
+        rv[k] = r[o] - ah * q[o];
+        r[o] = rv[k];
+      }
+      const HT* Ji = J + static_cast<size_t>(i) * 36;
+      for (int k = 0; k < BD; ++k) {
+        HT v = 0;
+        for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+        s[BD * static_cast<size_t>(i) + k] = v;
+        dn_loc += static_cast<double>(rv[k]) * static_cast<double>(v);
+      }
+    }
+    dn_loc = block_sum(dn_loc, s_wsum);
+    if (tid == 0) part_b[blockIdx.x] = dn_loc;
+    if (!grid_barrier(bar, nblk, sc)) return;
+    const double dn_new = block_sum_array(part_b, nblk, s_red);
+    beta = dn_new / dn;
+    dn = dn_new;
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    sc->dn[it & 1] = dn;
+    sc->d0 = dlim;
+    sc->iters = it;
+    sc->done = 1;
+  }
+}
+
+template <typename HT>
+int pcg_persistent_max_grid(int device) {
+  int ncu = 0, per = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pcg_persistent<HT>, 256, 0) != hipSuccess) return 0;
+  return ncu * per;
+}
+template int pcg_persistent_max_grid<float>(int);
+template int pcg_persistent_max_grid<double>(int);
+
+template <typename HT>
+hipError_t launch_pcg_persistent(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* Jinv,
+                                 const double* rhs, HT* x, HT* r, HT* d0, HT* d1, HT* q, HT* s, double* part,
+                                 unsigned* bar, PcgScalars* sc, double tol, double resid_mem, int max_iter, int grid,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, bar, 2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  void* args[] = {&nb, &F_ptr, &F_col, &F_val, &Jinv, &rhs, &x, &r, &d0, &d1, &q, &s, &part, &bar, &sc, &tol,
+                  &resid_mem, &max_iter};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<HT>), dim3(grid), dim3(256),
+                                    args, 0, st);
+}
+template hipError_t launch_pcg_persistent<float>(int, const int*, const int*, const float*, const float*,
+                                                 const double*, float*, float*, float*, float*, float*, float*,
+                                                 double*, unsigned*, PcgScalars*, double, double, int, int,
+                                                 hipStream_t);
+template hipError_t launch_pcg_persistent<double>(int, const int*, const int*, const double*, const double*,
+                                                  const double*, double*, double*, double*, double*, double*,
+                                                  double*, double*, unsigned*, PcgScalars*, double, double, int, int,
+                                                  hipStream_t);
 
 template <typename HT>
 __global__ void k_to_double(int n, const HT* __restrict__ src, double* __restrict__ dst, double scale_add) {

@@ -1,0 +1,94 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol
+include/g2o_hip.h declares, and the host-side structure builder (device -1
+context) produces the reference's index mapping and Hschur pattern."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import g2o_amd
+from g2o_amd import datasets
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "g2o_hip.h")).read()
+    return sorted(set(re.findall(r"\b(g2ohip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = g2o_amd.lib()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(g2o_amd._lib.EXPORTS) == names
+
+
+def test_default_options_follow_reference():
+    o = g2o_amd.default_options()
+    assert o.pcg_tolerance == 1e-6 and o.pcg_max_iter == -1  # linear_solver_pcg.h:45-50
+    assert o.max_trials == 10 and o.tau == 1e-5              # levenberg.cpp:42-56
+    assert o.precision == g2o_amd.FP64 and o.schur_mode == g2o_amd.SCHUR_EXPLICIT
+
+
+def test_host_only_context_rejects_device_calls():
+    ctx = g2o_amd.Context(-1)
+    with pytest.raises(g2o_amd.G2oHipError):
+        ctx.call("g2ohip_build_system")
+
+
+def _pairs_numpy(prob):
+    free = np.where(prob.cam_fixed == 0, np.cumsum(prob.cam_fixed == 0) - 1, -1)
+    by_pt = {}
+    for c, p in zip(prob.edge_cam, prob.edge_pt):
+        if free[c] >= 0 and prob.pt_fixed[p] == 0:
+            by_pt.setdefault(int(p), []).append(int(free[c]))
+    keys = set(range(0, 0))
+    pairs = 0
+    Cf = int((prob.cam_fixed == 0).sum())
+    keys = {(i, i) for i in range(Cf)}
+    for cams in by_pt.values():
+        cams = sorted(cams)
+        pairs += len(cams) * (len(cams) + 1) // 2
+        for a in range(len(cams)):
+            for b in range(a, len(cams)):
+                keys.add((cams[a], cams[b]))
+    return pairs, len(keys)
+
+
+@pytest.mark.parametrize("seed,cams,pts", [(3, 40, 3000), (9, 120, 8000)])
+def test_structure_matches_reference_pattern(seed, cams, pts):
+    prob = datasets.make_walk_ba(seed=seed, num_cams=cams, num_pts=pts)
+    ba = g2o_amd.BundleAdjuster(prob, device=-1)
+    info = ba.structure_info()
+    Cf = int((prob.cam_fixed == 0).sum())
+    assert info[0] == Cf and info[1] == prob.num_pts and info[2] == prob.num_edges
+    pairs, nS = _pairs_numpy(prob)
+    assert info[6] == pairs      # Schur pair instances
+    assert info[3] == nS         # upper Hschur blocks (block_solver.hpp:237-266)
+    assert info[9] == 2 * nS - Cf  # both triangles
+    assert info[5] >= Cf         # one or more tiles per block row
+
+
+def test_structure_rejects_bad_indices():
+    prob = datasets.make_small_ba(seed=1)
+    prob.edge_cam = prob.edge_cam.copy()
+    prob.edge_cam[5] = prob.num_cams + 3
+    with pytest.raises(g2o_amd.G2oHipError) as ei:
+        g2o_amd.BundleAdjuster(prob, device=-1)
+    assert ei.value.status == g2o_amd._lib.ERR_INVALID
+
+
+def test_fixed_points_and_cameras_are_excluded():
+    prob = datasets.make_walk_ba(seed=5, num_cams=30, num_pts=2000)
+    prob.pt_fixed = prob.pt_fixed.copy()
+    prob.pt_fixed[::10] = 1
+    ba = g2o_amd.BundleAdjuster(prob, device=-1)
+    info = ba.structure_info()
+    assert info[1] == prob.num_pts - int(prob.pt_fixed.sum())
+    # edges with both ends fixed are inactive
+    both = (prob.cam_fixed[prob.edge_cam] == 1) & (prob.pt_fixed[prob.edge_pt] == 1)
+    assert info[2] == prob.num_edges - int(both.sum())

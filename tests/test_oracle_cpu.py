@@ -1,0 +1,317 @@
+"""Pinning the FP64 CPU oracle (no GPU).
+
+Each check is independent of the formula it checks:
+* the reference's own 36x36 block system (unit_test/solver/sparse_system_helper.cpp)
+  solved by the oracle's Cholesky and PCG to 1e-6 (linear_solver_test.cpp:89-101);
+* Huber: closed form values and numeric derivatives at the reference test's
+  delta and error values (robust_kernel_tests.cpp:77-146);
+* SE3 exp against scipy's rotation-vector map and the series of V; exp/log
+  round trip; unit quaternion with w >= 0; continuity across the small-angle
+  branch (se3quat.h:202-230);
+* projection-edge Jacobians against central differences taken through the
+  vertices' oplus (base_fixed_sized_edge.hpp:152-193 scheme);
+* the whole quadratic form (Hpp, Hll, Hpl, b) against a dense numeric
+  Jacobian of the stacked residuals;
+* Schur elimination + back-substitution against a dense solve of the full
+  damped system;
+* LM on the small BA reaches the statistical chi2 of a sigma = 1 problem.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from g2o_amd import datasets
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+@pytest.fixture(scope="module")
+def fixture36():
+    d = json.load(open(os.path.join(HERE, "golden", "sparse_system_36.json")))
+    rows, cols = np.array(d["rows"]), np.array(d["cols"])
+    blocks = np.array(d["blocks"]).reshape(-1, 3, 3)
+    return rows, cols, blocks, np.array(d["b"]), np.array(d["x"]), np.array(d["inverse"]).reshape(36, 36)
+
+
+def _dense(rows, cols, blocks, bd, nb):
+    A = np.zeros((nb * bd, nb * bd))
+    for r, c, v in zip(rows, cols, blocks):
+        A[bd * r:bd * r + bd, bd * c:bd * c + bd] = v
+        if r != c:
+            A[bd * c:bd * c + bd, bd * r:bd * r + bd] = v.T
+    return A
+
+
+def test_fixture_is_consistent(fixture36):
+    rows, cols, blocks, b, x, inv = fixture36
+    A = _dense(rows, cols, blocks, 3, 12)
+    assert np.allclose(inv @ A, np.eye(36), atol=1e-9)
+    # x is printed with 8 digits: compare solutions, as the reference test does
+    assert np.linalg.norm(np.linalg.solve(A, b) - x) <= 1e-6 * np.linalg.norm(x)
+
+
+def test_cholesky_on_reference_fixture(O, fixture36):
+    rows, cols, blocks, b, x, _ = fixture36
+    A = _dense(rows, cols, blocks, 3, 12)
+    for _ in range(2):  # solve twice, as linear_solver_test.cpp:89-101
+        st, sol = O.chol_solve(A, b)
+        assert st == 0
+        assert np.linalg.norm(sol - x) <= 1e-6 * np.linalg.norm(x)
+
+
+def test_pcg_on_reference_fixture(O, fixture36):
+    rows, cols, blocks, b, x, _ = fixture36
+    resid = -1.0
+    for _ in range(2):
+        sol, it, resid_new = O.pcg_blocks(12, 3, rows, cols, blocks, b, tol=1e-14, maxiter=500, residual=-1.0)
+        assert it > 0
+        assert np.linalg.norm(sol - x) <= 1e-6 * np.linalg.norm(x) + 1e-7
+    # residual memory (linear_solver_pcg.hpp:134-136): a second solve floored
+    # by half the previous final dn stops no later than a fresh one
+    sol1, it1, mem = O.pcg_blocks(12, 3, rows, cols, blocks, b, tol=1e-3, maxiter=500, residual=-1.0)
+    sol2, it2, _ = O.pcg_blocks(12, 3, rows, cols, blocks, b, tol=1e-3, maxiter=500, residual=mem * 1e6)
+    assert it2 <= it1 and it2 == 0
+    sol3, it3, _ = O.pcg_blocks(12, 3, rows, cols, blocks, b, tol=1e-3, maxiter=2, residual=-1.0)
+    assert it3 == 2
+
+
+def test_not_pd_is_reported(O):
+    A = np.eye(6)
+    A[3, 3] = -1.0
+    st, _ = O.chol_solve(A, np.ones(6))
+    assert st == 2
+
+
+@pytest.mark.parametrize("e", [0.99 * 0.07 * 0.07, 0.99 * 0.07, 1.5 * 0.07])
+def test_huber_matches_reference_test(O, e):
+    d = 0.07
+    rho = O.huber(e, d)
+    assert rho[0] != 0.0
+    ev = lambda v: O.huber(v, d)[0]
+    h = 1e-9
+    num1 = (ev(e + h) - ev(e - h)) / (2 * h)
+    assert abs(rho[1] - num1) <= 1e-5
+    k = 1e-5
+    r1 = (2 * ev(e) - 5 * ev(e - k) + 4 * ev(e - 2 * k) - ev(e - 3 * k)) / (k * k)
+    r2 = (2 * ev(e) - 5 * ev(e + k) + 4 * ev(e + 2 * k) - ev(e + 3 * k)) / (k * k)
+    assert abs(rho[2] - 0.5 * (r1 + r2)) <= 1e-3
+    # closed form
+    if e <= d * d:
+        assert np.allclose(rho, [e, 1.0, 0.0])
+    else:
+        assert np.isclose(rho[0], 2 * d * np.sqrt(e) - d * d)
+        assert np.isclose(rho[1], d / np.sqrt(e))
+
+
+def test_se3_exp_against_scipy_and_series(O):
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        w = rng.normal(size=3) * rng.choice([1e-7, 1e-3, 0.5, 2.0])
+        u = rng.normal(size=3)
+        p = O.se3_exp(np.concatenate([w, u]))
+        q = p[3:]
+        assert abs(np.linalg.norm(q) - 1) < 1e-14 and q[3] >= 0
+        R_ref = Rotation.from_rotvec(w).as_matrix()
+        assert np.allclose(O.quat_to_R(q), R_ref, atol=1e-12)
+        # V = sum_k W^k / (k+1)!
+        W = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+        V = np.eye(3)
+        term = np.eye(3)
+        for k in range(1, 80):  # |w| up to ~9: the series needs many terms
+            term = term @ W / (k + 1)
+            V = V + term
+        assert np.allclose(p[:3], V @ u, atol=1e-9)
+
+
+def test_se3_log_roundtrip_and_small_angle(O):
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        v = np.concatenate([rng.normal(size=3) * 0.8, rng.normal(size=3)])
+        assert np.allclose(O.se3_log(O.se3_exp(v)), v, atol=1e-8)
+    # continuity at the theta = 1e-5 branch point
+    d = np.array([1.0, 2.0, -1.0]) / np.sqrt(6)
+    a = O.se3_exp(np.concatenate([d * (1e-5 * (1 - 1e-9)), [0.3, 0.1, 0.2]]))
+    b = O.se3_exp(np.concatenate([d * (1e-5 * (1 + 1e-9)), [0.3, 0.1, 0.2]]))
+    assert np.allclose(a, b, atol=1e-12)
+
+
+def test_se3_product_matches_matrices(O):
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        a = O.se3_exp(rng.normal(size=6))
+        b = O.se3_exp(rng.normal(size=6))
+        c = O.se3_mul(a, b)
+        Ta, Tb, Tc = (np.eye(4) for _ in range(3))
+        for T, p in ((Ta, a), (Tb, b), (Tc, c)):
+            T[:3, :3] = O.quat_to_R(p[3:])
+            T[:3, 3] = p[:3]
+        assert np.allclose(Ta @ Tb, Tc, atol=1e-12)
+
+
+def _oplus_cam(O, cam7, d6):
+    return O.se3_mul(O.se3_exp(d6), cam7)
+
+
+def test_projection_jacobians_vs_central_differences(O):
+    """Analytic EdgeSE3ProjectXYZ Jacobians vs central differences through
+    oplus (delta 1e-9 as base_fixed_sized_edge.hpp:161), 2000 random states."""
+    rng = np.random.default_rng(6)
+    h = 1e-9
+    worst = 0.0
+    for _ in range(2000):
+        cam = O.se3_exp(np.concatenate([rng.normal(size=3) * 0.3, rng.normal(size=3)]))
+        R = O.quat_to_R(cam[3:])
+        Xc = np.array([rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(2, 10)])
+        X = R.T @ (Xc - cam[:3])
+        intr = np.array([rng.uniform(300, 900), rng.uniform(300, 900), 320.0, 240.0])
+        z = rng.uniform(0, 640, size=2)
+        e, Jp, Jc = O.project_edge(cam, intr, X, z)
+        Jp_n = np.zeros((2, 3))
+        Jc_n = np.zeros((2, 6))
+        for d in range(3):
+            dv = np.zeros(3)
+            dv[d] = h
+            Jp_n[:, d] = (O.project_edge(cam, intr, X + dv, z)[0] - O.project_edge(cam, intr, X - dv, z)[0]) / (2 * h)
+        for d in range(6):
+            dv = np.zeros(6)
+            dv[d] = h
+            ep = O.project_edge(_oplus_cam(O, cam, dv), intr, X, z)[0]
+            em = O.project_edge(_oplus_cam(O, cam, -dv), intr, X, z)[0]
+            Jc_n[:, d] = (ep - em) / (2 * h)
+        scale = max(np.abs(Jc).max(), 1.0)
+        worst = max(worst, np.abs(Jp - Jp_n).max() / scale, np.abs(Jc - Jc_n).max() / scale)
+    assert worst < 1e-6
+
+
+def _tiny_ba(huber):
+    p = datasets.make_small_ba(seed=11, num_cams=6, num_pts=40)
+    p.huber = huber
+    if huber:
+        p.meas[::7] += 15.0
+    return p
+
+
+@pytest.mark.parametrize("huber", [0.0, 2.0])
+def test_quadratic_form_vs_dense_numeric_jacobian(O, huber):
+    """H = J^T W J and b = -J^T W e with J the numeric Jacobian of all
+    residuals with respect to all free parameters (oplus perturbations) and
+    W the per-edge Huber weights rho'(e^T Omega e) * Omega."""
+    prob = _tiny_ba(huber)
+    ora = O.OracleBA(prob)
+    chi, Hpp, Hll, Hpl, b = ora.linearize()
+    cams0, pts0 = ora.state()
+    free_c = np.flatnonzero(prob.cam_fixed == 0)
+    Cf, Pf = len(free_c), prob.num_pts
+    n = 6 * Cf + 3 * Pf
+    E = prob.num_edges
+
+    def residuals(cams, pts):
+        r = np.zeros(2 * E)
+        for k in range(E):
+            r[2 * k:2 * k + 2] = O.project_edge(cams[prob.edge_cam[k]], prob.intr[prob.edge_cam[k]],
+                                                pts[prob.edge_pt[k]], prob.meas[k])[0]
+        return r
+
+    e0 = residuals(cams0, pts0)
+    J = np.zeros((2 * E, n))
+    h = 1e-6
+    for i, c in enumerate(free_c):
+        for d in range(6):
+            dv = np.zeros(6)
+            dv[d] = h
+            cp, cm = cams0.copy(), cams0.copy()
+            cp[c] = _oplus_cam(O, cams0[c], dv)
+            cm[c] = _oplus_cam(O, cams0[c], -dv)
+            J[:, 6 * i + d] = (residuals(cp, pts0) - residuals(cm, pts0)) / (2 * h)
+    for p in range(Pf):
+        for d in range(3):
+            pp, pm = pts0.copy(), pts0.copy()
+            pp[p, d] += h
+            pm[p, d] -= h
+            J[:, 6 * Cf + 3 * p + d] = (residuals(cams0, pp) - residuals(cams0, pm)) / (2 * h)
+    w = np.ones(E)
+    if huber:
+        s = (e0.reshape(-1, 2) ** 2).sum(1)
+        w = np.where(s <= huber ** 2, 1.0, huber / np.sqrt(s))
+    Wd = np.repeat(w, 2)
+    H = J.T @ (Wd[:, None] * J)
+    bb = -J.T @ (Wd * e0)
+    Hd = np.zeros((n, n))
+    for i in range(Cf):
+        Hd[6 * i:6 * i + 6, 6 * i:6 * i + 6] = Hpp[i]
+    for p in range(Pf):
+        o = 6 * Cf + 3 * p
+        Hd[o:o + 3, o:o + 3] = Hll[p]
+    for k in range(E):
+        ci = np.searchsorted(free_c, prob.edge_cam[k])
+        if ci < Cf and free_c[ci] == prob.edge_cam[k]:
+            o = 6 * Cf + 3 * prob.edge_pt[k]
+            Hd[6 * ci:6 * ci + 6, o:o + 3] = Hpl[k]
+            Hd[o:o + 3, 6 * ci:6 * ci + 6] = Hpl[k].T
+    assert np.abs(Hd - H).max() <= 2e-5 * np.abs(H).max()
+    assert np.abs(b - bb).max() <= 2e-5 * np.abs(bb).max()
+    # robust chi2 of the same state
+    s = (e0.reshape(-1, 2) ** 2).sum(1)
+    rho0 = s if not huber else np.where(s <= huber ** 2, s, 2 * huber * np.sqrt(s) - huber ** 2)
+    assert np.isclose(chi, rho0.sum(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("lam", [0.0, 1.0, 100.0])
+def test_schur_backsubstitution_equals_dense_solve(O, lam):
+    prob = _tiny_ba(2.0)
+    ora = O.OracleBA(prob)
+    chi, Hpp, Hll, Hpl, b = ora.linearize()
+    st, dx, _ = ora.solve(lam, solver=0)
+    assert st == 0
+    free_c = np.flatnonzero(prob.cam_fixed == 0)
+    Cf, Pf = len(free_c), prob.num_pts
+    n = 6 * Cf + 3 * Pf
+    H = np.zeros((n, n))
+    for i in range(Cf):
+        H[6 * i:6 * i + 6, 6 * i:6 * i + 6] = Hpp[i]
+    for p in range(Pf):
+        o = 6 * Cf + 3 * p
+        H[o:o + 3, o:o + 3] = Hll[p]
+    for k in range(prob.num_edges):
+        ci = np.searchsorted(free_c, prob.edge_cam[k])
+        if ci < Cf and free_c[ci] == prob.edge_cam[k]:
+            o = 6 * Cf + 3 * prob.edge_pt[k]
+            H[6 * ci:6 * ci + 6, o:o + 3] += Hpl[k]
+            H[o:o + 3, 6 * ci:6 * ci + 6] += Hpl[k].T
+    x = np.linalg.solve(H + lam * np.eye(n), b)
+    assert np.abs(dx - x).max() <= 1e-8 * np.abs(x).max()
+    # PCG run to a tiny tolerance converges to the same step
+    st, dx2, it = ora.solve(lam, solver=1, tol=1e-24, maxiter=2000)
+    assert np.abs(dx2 - x).max() <= 1e-7 * np.abs(x).max()
+
+
+def test_lm_reaches_statistical_chi2(O):
+    """With N(0, 1) pixel noise and no outliers, the optimum chi2 of the
+    small BA is chi-square with (2E - #params) degrees of freedom."""
+    prob = datasets.make_small_ba(seed=1000)
+    ora = O.OracleBA(prob)
+    n, trace, _ = ora.optimize(30, solver=0)
+    dof = 2 * prob.num_edges - (6 * (prob.num_cams - 2) + 3 * prob.num_pts)
+    final = trace[-1, 1]
+    assert abs(final - dof) < 5 * np.sqrt(2 * dof)
+    accepted = trace[:, 1]
+    assert np.all(np.diff(accepted) <= 1e-9 * accepted[:-1])
+
+
+def test_lm_trace_pcg_vs_cholesky(O):
+    """The PCG-based LM (reference lm_pcg6_3 semantics) follows the
+    Cholesky-based one closely on a well-conditioned problem."""
+    prob = datasets.make_small_ba(seed=1000)
+    a = O.OracleBA(prob)
+    b = O.OracleBA(prob)
+    _, ta, _ = a.optimize(6, solver=0)
+    _, tb, _ = b.optimize(6, solver=1, tol=1e-12)
+    assert np.allclose(ta[:, 1], tb[:, 1], rtol=1e-6)
