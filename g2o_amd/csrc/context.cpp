@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <type_traits>
@@ -149,6 +150,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_pos_nfree.upload(st.pos_nfree, stream));
   HK(d_slot_cam.upload(st.slot_cam, stream));
   HK(d_slot_pos.upload(st.slot_pos, stream));
+  HK(d_slot_camh.upload(st.slot_camh, stream));
   HK(d_lc_pos0.upload(st.lc_pos0, stream));
   HK(d_lc_cam_ptr.upload(st.lc_cam_ptr, stream));
   HK(d_lc_cams.upload(st.lc_cams, stream));
@@ -192,7 +194,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   g.nS = st.nS; g.nF = st.nF;
   g.camH = d_camH.p; g.ptH = d_ptH.p;
   g.pos_pt = d_pos_pt.p; g.pos_h = d_pos_h.p; g.pos_ptr = d_pos_ptr.p; g.pos_nfree = d_pos_nfree.p;
-  g.slot_cam = d_slot_cam.p; g.slot_pos = d_slot_pos.p;
+  g.slot_cam = d_slot_cam.p; g.slot_pos = d_slot_pos.p; g.slot_camh = d_slot_camh.p;
   g.lc_pos0 = d_lc_pos0.p; g.lc_cam_ptr = d_lc_cam_ptr.p; g.lc_cams = d_lc_cams.p;
   g.camp_ptr = d_camp_ptr.p; g.camp_idx = d_camp_idx.p;
   g.S_row = d_S_row.p; g.S_col = d_S_col.p; g.S_diag = d_S_diag.p; g.up_pos = d_up_pos.p; g.up_posT = d_up_posT.p;
@@ -203,6 +205,8 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
 
 int g2ohip_ctx::ensure_rows() {
   if (rows_ready) return G2OHIP_OK;
+  for (int i = 0; i < st.npos; ++i)
+    if (st.pos_nfree[i] > 1023) return fail(G2OHIP_ERR_UNSUPPORTED, "a point has more than 1023 free cameras");
   if (st.rows.row_ptr.empty()) {
     std::string msg = build_row_plan(st, SCHUR_MAXC, 8);
     if (!msg.empty()) return fail(G2OHIP_ERR_UNSUPPORTED, msg);
@@ -356,7 +360,10 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, bool use_memory, boo
   if (!implicit) {
     const int wi = std::is_same<HT, float>::value ? 0 : 1;
     if (pcg_grid[wi] < 0) {
-      const int mx = pcg_persistent_max_grid<HT>(device);
+      // The one-launch PCG is opt-in (G2OHIP_PCG_PERSISTENT=1): its grid
+      // barriers cost more than the kernel-per-step loop on gfx950.
+      const char* env = std::getenv("G2OHIP_PCG_PERSISTENT");
+      const int mx = (env && env[0] == '1') ? pcg_persistent_max_grid<HT>(device) : 0;
       pcg_grid[wi] = std::min({mx, 256, (nb + 3) / 4});
     }
     if (pcg_grid[wi] > 0) {

@@ -16,7 +16,7 @@ struct DevGraph {
   double huber;
   const int *camH, *ptH;
   const int *pos_pt, *pos_h, *pos_ptr, *pos_nfree;
-  const int *slot_cam, *slot_pos;
+  const int *slot_cam, *slot_pos, *slot_camh;
   const double *slot_meas, *slot_w;
   const int *lc_pos0, *lc_cam_ptr, *lc_cams, *camp_ptr, *camp_idx;
   const int *S_row, *S_col, *S_diag, *up_pos, *up_posT, *F_ptr, *F_col, *F_diag;

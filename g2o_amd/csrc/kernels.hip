@@ -384,17 +384,27 @@ hipError_t launch_scale(int n, const double* x, const double* b, double lambda, 
 // the workgroup at the end. Every Hschur block is written once, by its row's
 // tile, to both triangles of the block-CSR copy; nothing is accumulated in
 // global memory.
-template <typename HT, int MAXC>
+template <typename HT, int MAXC, int EB, int MAXP>
 __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, const HT* __restrict__ Hpl,
                                                     const double* __restrict__ Hll, const double* __restrict__ b,
                                                     const double* __restrict__ Hpp, double lambda,
                                                     HT* __restrict__ F_val, double* __restrict__ bschur,
                                                     HT* __restrict__ Sdiag, int diag_only, int* __restrict__ err) {
   constexpr int MAPW = 1024;  // direct column map when the row's columns span < MAPW cameras
+  static_assert(EB <= 256, "one thread per batch entry");
+  static_assert(MAXC < 255 && EB < (1 << 13) && MAXP >= 1024, "pair packing: column 8 bits, j 10 bits");
   __shared__ int s_col[MAXC];
   __shared__ int s_map[MAPW];
   __shared__ HT s_acc[MAXC * 36];
-  __shared__ double s_red[4 * 42];
+  __shared__ HT s_W[EB * 18];
+  __shared__ int s_a[EB];
+  __shared__ int s_pref[EB + 1];
+  __shared__ int s_cnt[MAXC + 1];
+  __shared__ int s_off[MAXC + 2];
+  __shared__ int s_pk[MAXP];      // packed pair: entry << 18 | j << 8 | column
+  __shared__ int s_sorted[MAXP];  // packed pairs grouped by column
+  __shared__ double s_red[4 * 8];
+  __shared__ int s_ne;
   const int tile = rp.tile_order[blockIdx.x];
   const int row = rp.tile_row[tile];
   const int k0 = rp.tile_k0[tile], k1 = diag_only ? rp.tile_k0[tile] + 1 : rp.tile_k1[tile];
@@ -410,28 +420,34 @@ __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, c
     __syncthreads();
     for (int i = tid; i < ncol; i += 256) s_map[s_col[i] - col_lo] = i;
   }
-  __syncthreads();
-  HT dacc[21];  // upper triangle of the diagonal block (symmetric)
-  for (int e = 0; e < 21; ++e) dacc[e] = HT(0);
   double coef[6] = {0, 0, 0, 0, 0, 0};
   const double* bl_base = b + 6 * static_cast<size_t>(g.Cf);
-  for (int e = rp.row_ptr[row] + tid; e < rp.row_ptr[row + 1]; e += 256) {
-    const int a = rp.row_slot[e];
-    const int pos = g.slot_pos[a];
-    const int ph = g.pos_h[pos];
-    const int send = g.pos_ptr[pos] + g.pos_nfree[pos];
-    double D[9];
-    if (!inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
-      atomicOr(err, 1);
-      for (int i = 0; i < 9; ++i) D[i] = 0.0;
-    }
-    HT W[18];
-    {
+  const int e_end = rp.row_ptr[row + 1];
+  int e0 = rp.row_ptr[row];
+  while (e0 < e_end) {
+    const int cand = min(EB, e_end - e0);
+    __syncthreads();  // previous batch finished with the staging arrays
+    // ---- phase 1, one thread per observation (point p, slot a of this
+    // camera): Dinv_p = (Hll_p + lambda I)^-1 in f64, W = Hpl_a Dinv_p to
+    // LDS, the coefficient Hpl_a Dinv_p b_p, and m = slots b >= a of p.
+    int m = 0;
+    double cf[6] = {0, 0, 0, 0, 0, 0};
+    if (tid < cand) {
+      const int a = rp.row_slot[e0 + tid];
+      const int pos = g.slot_pos[a];
+      const int ph = g.pos_h[pos];
+      m = g.pos_ptr[pos] + g.pos_nfree[pos] - a;
+      double D[9];
+      if (!inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
+        atomicOr(err, 1);
+        for (int i = 0; i < 9; ++i) D[i] = 0.0;
+      }
       const HT* pa = Hpl + 18 * static_cast<size_t>(a);
       HT Ha[18];
       for (int k = 0; k < 18; ++k) Ha[k] = pa[k];
       HT Dh[9];
       for (int k = 0; k < 9; ++k) Dh[k] = static_cast<HT>(D[k]);
+      HT* W = s_W + 18 * tid;
       for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 3; ++c) W[3 * r + c] = Ha[3 * r] * Dh[c] + Ha[3 * r + 1] * Dh[3 + c] + Ha[3 * r + 2] * Dh[6 + c];
       if (diag_tile && !diag_only) {
@@ -439,68 +455,136 @@ __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, c
         double db[3];
         for (int r = 0; r < 3; ++r) db[r] = D[3 * r] * bl[0] + D[3 * r + 1] * bl[1] + D[3 * r + 2] * bl[2];
         for (int r = 0; r < 6; ++r)
-          coef[r] += static_cast<double>(Ha[3 * r]) * db[0] + static_cast<double>(Ha[3 * r + 1]) * db[1] +
-                     static_cast<double>(Ha[3 * r + 2]) * db[2];
+          cf[r] = static_cast<double>(Ha[3 * r]) * db[0] + static_cast<double>(Ha[3 * r + 1]) * db[1] +
+                  static_cast<double>(Ha[3 * r + 2]) * db[2];
       }
+      s_a[tid] = a;
     }
-    // Walk the point's slots b >= a starting at a lane-dependent offset:
-    // neighbouring lanes hold points with near-identical camera tracks, and
-    // the rotation keeps them from hitting the same block column together.
-    const int m = send - a;
-    const int rot = tid % m;
-    for (int j = 0; j < m; ++j) {
-      int bs = a + j + rot;
-      if (bs >= send) bs -= m;
-      const int cb = g.camH[g.slot_cam[bs]];
-      if (cb > col_hi || cb < col_lo) continue;
+    // inclusive scan of m over the batch (Hillis-Steele in LDS); only the
+    // first EB threads own an entry of s_pref
+    if (tid < EB) s_pref[tid + 1] = m;
+    if (tid == 0) s_pref[0] = 0;
+    __syncthreads();
+    for (int off = 1; off < EB; off <<= 1) {
+      const int v = (tid < EB && tid + 1 > off) ? s_pref[tid + 1 - off] : 0;
+      __syncthreads();
+      if (tid < EB) s_pref[tid + 1] += v;
+      __syncthreads();
+    }
+    // the batch ends where its pairs would exceed MAXP (at least one entry)
+    if (tid == 0) {
+      int lo = 1, hi = cand;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pref[mid] <= MAXP) lo = mid;
+        else hi = mid - 1;
+      }
+      s_ne = lo;
+    }
+    for (int i = tid; i <= ncol; i += 256) s_cnt[i] = 0;
+    __syncthreads();
+    const int ne = s_ne;
+    const int total = s_pref[ne];
+    if (tid < ne)
+      for (int r = 0; r < 6; ++r) coef[r] += cf[r];
+    // ---- phase 2a: column of every pair (a, b = a + j) and a histogram
+    for (int k = tid; k < total; k += 256) {
+      int lo = 0, hi = ne;  // last e with s_pref[e] <= k
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= k) lo = mid;
+        else hi = mid;
+      }
+      const int j = k - s_pref[lo];
+      const int cb = g.slot_camh[s_a[lo] + j];
+      int lc = ncol;  // outside this tile
+      if (cb >= col_lo && cb <= col_hi) lc = use_map ? s_map[cb - col_lo] : lower_bound_i(s_col, ncol, cb);
+      s_pk[k] = (lo << 18) | (j << 8) | lc;
+      atomicAdd(&s_cnt[lc], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int c = 0; c <= ncol; ++c) {
+        s_off[c] = acc;
+        acc += s_cnt[c];
+      }
+      s_off[ncol + 1] = acc;
+    }
+    __syncthreads();
+    for (int i = tid; i <= ncol; i += 256) s_cnt[i] = s_off[i];  // scatter cursors
+    __syncthreads();
+    for (int k = tid; k < total; k += 256) {
+      const int v = s_pk[k];
+      const int lc = v & 255;
+      if (lc < ncol) s_sorted[atomicAdd(&s_cnt[lc], 1)] = v;
+    }
+    __syncthreads();
+    // ---- phase 2b: each thread walks a contiguous run of the column-sorted
+    // pairs, accumulates W Hb^T in registers and flushes when the column
+    // changes. A pair b != a on the diagonal column (two edges of one point to
+    // camera i) adds the product and its transpose: g2o merges such edges into
+    // one Hpl block.
+    const int nvalid = s_off[ncol];
+    const int chunk = (nvalid + 255) / 256;
+    const int p_begin = min(tid * chunk, nvalid), p_end = min(p_begin + chunk, nvalid);
+    HT acc[36];
+    for (int q = 0; q < 36; ++q) acc[q] = HT(0);
+    int cur = -1;
+    for (int p = p_begin; p < p_end; ++p) {
+      const int v = s_sorted[p];
+      const int lc = v & 255, j = (v >> 8) & 1023, e = v >> 18;
+      if (lc != cur) {
+        if (cur >= 0) {
+          HT* dst = s_acc + 36 * cur;
+          for (int q = 0; q < 36; ++q) atomicAdd(&dst[q], -acc[q]);
+          for (int q = 0; q < 36; ++q) acc[q] = HT(0);
+        }
+        cur = lc;
+      }
+      const HT* W = s_W + 18 * e;
+      const HT* pb = Hpl + 18 * static_cast<size_t>(s_a[e] + j);
       HT Hb[18];
-      const HT* pb = Hpl + 18 * static_cast<size_t>(bs);
-      for (int k = 0; k < 18; ++k) Hb[k] = pb[k];
-      if (cb == row) {
-        // diagonal block: W Hb^T, symmetric for b == a; a second edge of this
-        // point to camera i adds the product and its transpose (g2o merges
-        // the two edges into one Hpl block)
-        int k = 0;
-        for (int r = 0; r < 6; ++r)
-          for (int c = r; c < 6; ++c, ++k) {
-            HT v = W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2];
-            if (bs != a) v += W[3 * c] * Hb[3 * r] + W[3 * c + 1] * Hb[3 * r + 1] + W[3 * c + 2] * Hb[3 * r + 2];
-            dacc[k] += v;
-          }
-      } else {
-        const int lc = use_map ? s_map[cb - col_lo] : lower_bound_i(s_col, ncol, cb);
-        HT* acc = s_acc + 36 * lc;
-        for (int r = 0; r < 6; ++r)
-          for (int c = 0; c < 6; ++c)
-            atomicAdd(&acc[6 * r + c], -(W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2]));
-      }
+      for (int q = 0; q < 18; ++q) Hb[q] = pb[q];
+      const bool sym = diag_tile && lc == 0 && j != 0;
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+          HT t = W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2];
+          if (sym) t += W[3 * c] * Hb[3 * r] + W[3 * c + 1] * Hb[3 * r + 1] + W[3 * c + 2] * Hb[3 * r + 2];
+          acc[6 * r + c] += t;
+        }
     }
+    if (cur >= 0) {
+      HT* dst = s_acc + 36 * cur;
+      for (int q = 0; q < 36; ++q) atomicAdd(&dst[q], -acc[q]);
+    }
+    e0 += ne;
   }
-  // workgroup reduction of the register accumulators (diagonal tile only)
+  // workgroup reduction of the coefficient (diagonal tile only)
   if (diag_tile) {
     const int lane = tid & 63, wv = tid >> 6;
-    for (int k = 0; k < 21; ++k) {
-      double v = wave_sum(static_cast<double>(dacc[k]));
-      if (lane == 0) s_red[42 * wv + k] = v;
-    }
     for (int k = 0; k < 6; ++k) {
-      double v = wave_sum(coef[k]);
-      if (lane == 0) s_red[42 * wv + 21 + k] = v;
+      const double v = wave_sum(coef[k]);
+      if (lane == 0) s_red[8 * wv + k] = v;
     }
   }
   __syncthreads();
-  if (diag_tile && tid < 27) {
-    const double v = s_red[tid] + s_red[42 + tid] + s_red[84 + tid] + s_red[126 + tid];
-    if (tid < 21) {
-      const int r = c_up6_r[tid], c = c_up6_c[tid];
-      double h = Hpp[36 * static_cast<size_t>(row) + 6 * r + c] - v;
-      if (!diag_only && r == c) h += lambda;
-      s_acc[6 * r + c] = static_cast<HT>(h);  // column 0 of the diagonal tile is the diagonal
-      s_acc[6 * c + r] = static_cast<HT>(h);
-    } else if (!diag_only) {
-      const int r = tid - 21;
-      bschur[6 * static_cast<size_t>(row) + r] = b[6 * static_cast<size_t>(row) + r] - v;
-    }
+  // diagonal block (column 0 of the diagonal tile): Hpp + lambda I minus the
+  // accumulated products, symmetrised (the float products W Hb^T of one block
+  // are symmetric only up to rounding)
+  double hv = 0.0;
+  if (diag_tile && tid < 36) {
+    const int r = tid / 6, c = tid % 6;
+    hv = Hpp[36 * static_cast<size_t>(row) + tid] +
+         0.5 * (static_cast<double>(s_acc[6 * r + c]) + static_cast<double>(s_acc[6 * c + r]));
+    if (!diag_only && r == c) hv += lambda;
+  }
+  __syncthreads();
+  if (diag_tile && tid < 36) s_acc[tid] = static_cast<HT>(hv);
+  if (diag_tile && !diag_only && tid >= 64 && tid < 70) {
+    const int r = tid - 64;
+    const double v = s_red[r] + s_red[8 + r] + s_red[16 + r] + s_red[24 + r];
+    bschur[6 * static_cast<size_t>(row) + r] = b[6 * static_cast<size_t>(row) + r] - v;
   }
   __syncthreads();
   if (diag_only) {
@@ -522,8 +606,9 @@ hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* 
                              const double* b, const double* Hpp, double lambda, HT* F_val, double* bschur,
                              HT* Sdiag, int diag_only, int* err, hipStream_t st) {
   if (rp.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_schur_rows<HT, SCHUR_MAXC>), dim3(rp.ntiles), dim3(256), 0, st, g, rp, Hpl, Hll, b, Hpp,
-                     lambda, F_val, bschur, Sdiag, diag_only, err);
+  constexpr int EB = sizeof(HT) == 4 ? 256 : 128;
+  hipLaunchKernelGGL((k_schur_rows<HT, SCHUR_MAXC, EB, 2048>), dim3(rp.ntiles), dim3(256), 0, st, g, rp, Hpl, Hll, b,
+                     Hpp, lambda, F_val, bschur, Sdiag, diag_only, err);
   return hipGetLastError();
 }
 template hipError_t launch_schur_rows<float>(const DevGraph&, const DevRowPlan&, const float*, const double*,
@@ -830,29 +915,63 @@ hipError_t launch_refine_reduce(const DevGraph& g, const double* cam_part, const
 // LinearSolverPCG::solve (linear_solver_pcg.hpp:87-161) as a stream of
 // kernels per iteration. Vectors and the matrix in HT, dot products in f64 with
 // per-row partials summed (compensated, fixed order) by every consumer block.
+// Inverse of a symmetric positive-definite BD x BD block (row-major, in
+// place): A = L L^T, then A^-1 = L^-T L^-1. Constant loop bounds keep
+// everything in registers. Returns false unless every pivot is positive.
 template <int BD>
-__device__ bool invBD(double* m) {
-  double a[BD][2 * BD];
+__device__ __forceinline__ bool inv_spd(double* m) {
+  double L[BD][BD];
+#pragma unroll
   for (int i = 0; i < BD; ++i)
-    for (int j = 0; j < 2 * BD; ++j) a[i][j] = (j < BD) ? m[BD * i + j] : ((j - BD == i) ? 1.0 : 0.0);
-  for (int c = 0; c < BD; ++c) {
-    int p = c;
-    for (int r = c + 1; r < BD; ++r)
-      if (fabs(a[r][c]) > fabs(a[p][c])) p = r;
-    if (!(a[p][c] != 0.0) || !isfinite(a[p][c])) return false;
-    if (p != c)
-      for (int j = 0; j < 2 * BD; ++j) { double t = a[p][j]; a[p][j] = a[c][j]; a[c][j] = t; }
-    double piv = 1.0 / a[c][c];
-    for (int j = 0; j < 2 * BD; ++j) a[c][j] *= piv;
-    for (int r = 0; r < BD; ++r) {
-      if (r == c) continue;
-      double f = a[r][c];
-      for (int j = 0; j < 2 * BD; ++j) a[r][j] -= f * a[c][j];
+#pragma unroll
+    for (int j = 0; j < BD; ++j) L[i][j] = 0.0;
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < BD; ++j) {
+    double s = m[BD * j + j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+    ok = ok && (s > 0.0) && isfinite(s);
+    const double ljj = sqrt(fmax(s, 1e-300));
+    L[j][j] = ljj;
+    const double il = 1.0 / ljj;
+#pragma unroll
+    for (int i = j + 1; i < BD; ++i) {
+      double t = m[BD * i + j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+      L[i][j] = t * il;
     }
   }
+  // N = L^-1 (lower triangular)
+  double N[BD][BD];
+#pragma unroll
   for (int i = 0; i < BD; ++i)
-    for (int j = 0; j < BD; ++j) m[BD * i + j] = a[i][BD + j];
-  return true;
+#pragma unroll
+    for (int j = 0; j < BD; ++j) N[i][j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < BD; ++j) {
+    N[j][j] = 1.0 / L[j][j];
+#pragma unroll
+    for (int i = j + 1; i < BD; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = j; k < i; ++k) t -= L[i][k] * N[k][j];
+      N[i][j] = t / L[i][i];
+    }
+  }
+  // A^-1 = N^T N
+#pragma unroll
+  for (int i = 0; i < BD; ++i)
+#pragma unroll
+    for (int j = i; j < BD; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = j; k < BD; ++k) t += N[k][i] * N[k][j];
+      m[BD * i + j] = t;
+      m[BD * j + i] = t;
+    }
+  return ok;
 }
 
 template <typename HT, int BD>
@@ -862,12 +981,16 @@ __global__ void k_jacobi(int nb, const HT* __restrict__ F_val, const int* __rest
   if (i >= nb) return;
   double m[BD * BD];
   const HT* src = F_val + static_cast<size_t>(F_diag[i]) * BD * BD;
+#pragma unroll
   for (int k = 0; k < BD * BD; ++k) m[k] = static_cast<double>(src[k]);
+#pragma unroll
   for (int k = 0; k < BD; ++k) m[k * (BD + 1)] += lambda;
-  if (!invBD<BD>(m)) {
+  if (!inv_spd<BD>(m)) {
     atomicOr(err, 2);
+#pragma unroll
     for (int k = 0; k < BD * BD; ++k) m[k] = 0;
   }
+#pragma unroll
   for (int k = 0; k < BD * BD; ++k) Jinv[static_cast<size_t>(i) * BD * BD + k] = static_cast<HT>(m[k]);
 }
 

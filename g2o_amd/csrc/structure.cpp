@@ -106,6 +106,8 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
     }
   }
   st.pos_ptr[st.npos] = s;
+  st.slot_camh.resize(st.ns);
+  for (int q = 0; q < st.ns; ++q) st.slot_camh[q] = st.camH[st.slot_cam[q]];
 
   // linearisation chunks: consecutive points, bounded slots / cameras / points
   const LinPlanCaps& lc = st.lcaps;

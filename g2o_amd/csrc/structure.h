@@ -57,6 +57,7 @@ struct BAStructure {
   // slots
   int ns = 0;
   std::vector<int> slot_edge, slot_cam, slot_pos;
+  std::vector<int> slot_camh;        // free index of the slot's camera or -1
   std::vector<int> edge_slot;        // E -> slot or -1 (inactive edge)
   // linearisation chunks
   LinPlanCaps lcaps;
