@@ -92,9 +92,10 @@ int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
   info[2] = st.ns;
   info[3] = st.nS;
   info[4] = st.nlc;
-  info[5] = static_cast<long long>(st.rows.tile_row.size());
-  info[6] = st.rows.pair_instances;
-  info[7] = st.rows.max_row_cols;
+  const auto& rp = st.rows;
+  info[5] = static_cast<long long>(rp.tile_row.size());
+  info[6] = rp.pair_instances;
+  info[7] = rp.max_row_cols;
   info[8] = maxn;
   info[9] = st.nF;
   info[10] = static_cast<long long>(st.lc_cams.size());

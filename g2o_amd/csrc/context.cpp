@@ -151,6 +151,10 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_slot_cam.upload(st.slot_cam, stream));
   HK(d_slot_pos.upload(st.slot_pos, stream));
   HK(d_slot_camh.upload(st.slot_camh, stream));
+  HK(d_slot_ph.upload(st.slot_ph, stream));
+  HK(d_slot_q.upload(st.slot_q, stream));
+  HK(d_cam_qptr.upload(st.cam_qptr, stream));
+  HK(d_q_ph.upload(st.q_ph, stream));
   HK(d_lc_pos0.upload(st.lc_pos0, stream));
   HK(d_lc_cam_ptr.upload(st.lc_cam_ptr, stream));
   HK(d_lc_cams.upload(st.lc_cams, stream));
@@ -194,7 +198,8 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   g.nS = st.nS; g.nF = st.nF;
   g.camH = d_camH.p; g.ptH = d_ptH.p;
   g.pos_pt = d_pos_pt.p; g.pos_h = d_pos_h.p; g.pos_ptr = d_pos_ptr.p; g.pos_nfree = d_pos_nfree.p;
-  g.slot_cam = d_slot_cam.p; g.slot_pos = d_slot_pos.p; g.slot_camh = d_slot_camh.p;
+  g.slot_cam = d_slot_cam.p; g.slot_pos = d_slot_pos.p; g.slot_camh = d_slot_camh.p; g.slot_ph = d_slot_ph.p;
+  g.nq = st.nq; g.slot_q = d_slot_q.p; g.cam_qptr = d_cam_qptr.p; g.q_ph = d_q_ph.p;
   g.lc_pos0 = d_lc_pos0.p; g.lc_cam_ptr = d_lc_cam_ptr.p; g.lc_cams = d_lc_cams.p;
   g.camp_ptr = d_camp_ptr.p; g.camp_idx = d_camp_idx.p;
   g.S_row = d_S_row.p; g.S_col = d_S_col.p; g.S_diag = d_S_diag.p; g.up_pos = d_up_pos.p; g.up_posT = d_up_posT.p;
@@ -205,26 +210,25 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
 
 int g2ohip_ctx::ensure_rows() {
   if (rows_ready) return G2OHIP_OK;
-  for (int i = 0; i < st.npos; ++i)
-    if (st.pos_nfree[i] > 1023) return fail(G2OHIP_ERR_UNSUPPORTED, "a point has more than 1023 free cameras");
-  if (st.rows.row_ptr.empty()) {
-    std::string msg = build_row_plan(st, SCHUR_MAXC, 8);
+  if (st.rows.tile_row.empty()) {
+    std::string msg = build_row_plan(st, SCHUR_MAXC, SCHUR_CHUNK, 8);
     if (!msg.empty()) return fail(G2OHIP_ERR_UNSUPPORTED, msg);
   }
   if (device < 0) return G2OHIP_OK;
   const RowPlan& rp = st.rows;
   RowPlanBufs& B = row_bufs;
-  HK(B.row_ptr.upload(rp.row_ptr, stream));
-  HK(B.row_slot.upload(rp.row_slot, stream));
   HK(B.tile_row.upload(rp.tile_row, stream));
   HK(B.tile_k0.upload(rp.tile_k0, stream));
   HK(B.tile_k1.upload(rp.tile_k1, stream));
   HK(B.tile_order.upload(rp.tile_order, stream));
   HK(B.S_rowptr.upload(st.S_rowptr, stream));
+  HK(B.blk_pair_ptr.upload(rp.blk_pair_ptr, stream));
+  HK(B.pair_qa.upload(rp.pair_qa, stream));
+  HK(B.pair_qb.upload(rp.pair_qb, stream));
   drows.ntiles = static_cast<int>(rp.tile_row.size());
-  drows.row_ptr = B.row_ptr.p; drows.row_slot = B.row_slot.p; drows.tile_row = B.tile_row.p;
-  drows.tile_k0 = B.tile_k0.p; drows.tile_k1 = B.tile_k1.p; drows.tile_order = B.tile_order.p;
-  drows.S_rowptr = B.S_rowptr.p;
+  drows.tile_row = B.tile_row.p; drows.tile_k0 = B.tile_k0.p; drows.tile_k1 = B.tile_k1.p;
+  drows.tile_order = B.tile_order.p; drows.S_rowptr = B.S_rowptr.p;
+  drows.blk_pair_ptr = B.blk_pair_ptr.p; drows.pair_qa = B.pair_qa.p; drows.pair_qb = B.pair_qb.p;
   HK(hipStreamSynchronize(stream));
   rows_ready = true;
   return G2OHIP_OK;
@@ -313,11 +317,11 @@ int g2ohip_ctx::build_system() {
   HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
   const int prec = opt.precision;
   if (prec == G2OHIP_FP32) {
-    HK(f32.Hpl.alloc(18 * static_cast<size_t>(std::max(st.ns, 1))));
+    HK(f32.Hpl.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
     ProfScope ps(this, G2OHIP_K_LINEARIZE);
     HK(launch_linearize<float>(g, d_camRt.p, d_pts.p, f32.Hpl.p, d_Hll.p, d_b.p, d_hpp_part.p, d_chunk_chi.p, stream));
   } else {
-    HK(f64.Hpl.alloc(18 * static_cast<size_t>(std::max(st.ns, 1))));
+    HK(f64.Hpl.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
     ProfScope ps(this, G2OHIP_K_LINEARIZE);
     HK(launch_linearize<double>(g, d_camRt.p, d_pts.p, f64.Hpl.p, d_Hll.p, d_b.p, d_hpp_part.p, d_chunk_chi.p,
                                 stream));
@@ -364,7 +368,7 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, bool use_memory, boo
       // barriers cost more than the kernel-per-step loop on gfx950.
       const char* env = std::getenv("G2OHIP_PCG_PERSISTENT");
       const int mx = (env && env[0] == '1') ? pcg_persistent_max_grid<HT>(device) : 0;
-      pcg_grid[wi] = std::min({mx, 256, (nb + 3) / 4});
+      pcg_grid[wi] = std::min({mx, 128, (nb + 3) / 4});
     }
     if (pcg_grid[wi] > 0) {
       HK(B.d2.alloc(n));
@@ -432,22 +436,27 @@ int g2ohip_ctx::solve_impl(int* ok) {
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
   CK(ensure_rows());
-  if (!implicit) {
-    HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
-    {
-      ProfScope ps(this, G2OHIP_K_SCHUR);
-      HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, d_Hll.p, d_b.p, d_Hpp.p, lambda, B.F_val.p, d_bschur.p, nullptr, 0,
-                               d_err.p, stream));
+  HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
+  HK(B.W.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
+  {
+    ProfScope ps(this, G2OHIP_K_SCHUR);
+    HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
+    HK(launch_schur_w<HT>(g, B.Hpl.p, B.Dinv.p, B.W.p, stream));
+    if (!implicit) {
+      HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
+      HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lambda, B.F_val.p, d_bschur.p, nullptr, 0,
+                               stream));
+    } else {
+      HK(B.Sdiag.alloc(36 * static_cast<size_t>(nb)));
+      HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lambda, nullptr, nullptr, B.Sdiag.p, 1,
+                               stream));
     }
+  }
+  if (!implicit) {
     HK(launch_jacobi<HT, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream));
   } else {
     // matrix-free reduced operator; block-Jacobi preconditioner from the
     // diagonal blocks of the Schur complement
-    HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
-    HK(B.Sdiag.alloc(36 * static_cast<size_t>(nb)));
-    HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
-    HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, d_Hll.p, d_b.p, d_Hpp.p, lambda, nullptr, nullptr, B.Sdiag.p, 1,
-                             d_err.p, stream));
     HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, nullptr, d_b.p, d_cam_part.p, nullptr, 0, stream));
     HK(launch_implicit_rhs_reduce(g, d_cam_part.p, d_b.p, d_bschur.p, stream));
     HK(launch_jacobi_damped<HT, 6>(nb, B.Sdiag.p, d_iota.p, lambda, B.Jinv.p, d_err.p, stream));
@@ -645,19 +654,19 @@ int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
       for (int k = 0; k < 9; ++k) Hll[9 * static_cast<size_t>(p) + k] = h[6 * static_cast<size_t>(p) + map[k]];
   }
   if (Hpl_edge) {
-    std::vector<double> h(18 * static_cast<size_t>(st.ns));
+    std::vector<double> h(18 * static_cast<size_t>(st.nq));
     if (system_prec == G2OHIP_FP32) {
       std::vector<float> hf(h.size());
-      if (st.ns) HK(hipMemcpy(hf.data(), f32.Hpl.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost));
+      if (st.nq) HK(hipMemcpy(hf.data(), f32.Hpl.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < h.size(); ++i) h[i] = hf[i];
-    } else if (st.ns) {
+    } else if (st.nq) {
       HK(hipMemcpy(h.data(), f64.Hpl.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     }
     std::fill(Hpl_edge, Hpl_edge + 18 * static_cast<size_t>(st.E), 0.0);
     for (int s = 0; s < st.ns; ++s) {
-      const int k = st.slot_edge[s];
-      if (st.camH[st.slot_cam[s]] < 0 || st.pos_h[st.slot_pos[s]] < 0) continue;
-      for (int e = 0; e < 18; ++e) Hpl_edge[18 * static_cast<size_t>(k) + e] = h[18 * static_cast<size_t>(s) + e];
+      const int k = st.slot_edge[s], q = st.slot_q[s];
+      if (q < 0) continue;
+      for (int e = 0; e < 18; ++e) Hpl_edge[18 * static_cast<size_t>(k) + e] = h[18 * static_cast<size_t>(q) + e];
     }
   }
   return G2OHIP_OK;

@@ -44,16 +44,17 @@ struct DBuf {
 // Buffers of one Hessian precision.
 template <typename HT>
 struct PrecBufs {
-  DBuf<HT> Hpl;       // 18 * ns, one row-major 6x3 block per slot
+  DBuf<HT> Hpl;       // 18 * nq, one row-major 6x3 block per observation, camera-major
+  DBuf<HT> W;         // 18 * nq, Hpl Dinv of the current solve
   DBuf<HT> F_val;     // nF * 36
   DBuf<HT> Jinv;      // Cf * 36
-  DBuf<HT> Dinv;      // Pf * 9 (implicit mode)
+  DBuf<HT> Dinv;      // Pf * 9, (Hll + lambda I)^-1 of the current solve
   DBuf<HT> Sdiag;     // Cf * 36 (implicit mode preconditioner)
   DBuf<HT> x, r, d, d2, q, s;  // PCG vectors
 };
 
 struct RowPlanBufs {
-  DBuf<int> row_ptr, row_slot, tile_row, tile_k0, tile_k1, tile_order, S_rowptr;
+  DBuf<int> tile_row, tile_k0, tile_k1, tile_order, S_rowptr, blk_pair_ptr, pair_qa, pair_qb;
 };
 
 }  // namespace g2ohip
@@ -76,7 +77,7 @@ struct g2ohip_ctx {
   double chi2_lin = 0.0;
 
   g2ohip::DevGraph g{};
-  g2ohip::DBuf<int> d_camH, d_ptH, d_pos_pt, d_pos_h, d_pos_ptr, d_pos_nfree, d_slot_cam, d_slot_pos, d_slot_camh, d_lc_pos0,
+  g2ohip::DBuf<int> d_camH, d_ptH, d_pos_pt, d_pos_h, d_pos_ptr, d_pos_nfree, d_slot_cam, d_slot_pos, d_slot_camh, d_slot_ph, d_slot_q, d_cam_qptr, d_q_ph, d_lc_pos0,
       d_lc_cam_ptr, d_lc_cams, d_camp_ptr, d_camp_idx, d_S_row, d_S_col, d_S_diag, d_up_pos, d_up_posT, d_F_ptr,
       d_F_col, d_F_diag, d_iota;
   g2ohip::DBuf<double> d_meas, d_w, d_intr;

@@ -16,7 +16,9 @@ struct DevGraph {
   double huber;
   const int *camH, *ptH;
   const int *pos_pt, *pos_h, *pos_ptr, *pos_nfree;
-  const int *slot_cam, *slot_pos, *slot_camh;
+  const int *slot_cam, *slot_pos, *slot_camh, *slot_ph;
+  int nq;                                    // observations with a free camera and a free point
+  const int *slot_q, *cam_qptr, *q_ph;       // camera-major order of Hpl
   const double *slot_meas, *slot_w;
   const int *lc_pos0, *lc_cam_ptr, *lc_cams, *camp_ptr, *camp_idx;
   const int *S_row, *S_col, *S_diag, *up_pos, *up_posT, *F_ptr, *F_col, *F_diag;
@@ -25,8 +27,9 @@ struct DevGraph {
 
 struct DevRowPlan {
   int ntiles;
-  const int *row_ptr, *row_slot, *tile_row, *tile_k0, *tile_k1, *tile_order;
+  const int *tile_row, *tile_k0, *tile_k1, *tile_order;
   const int* S_rowptr;
+  const int *blk_pair_ptr, *pair_qa, *pair_qb;
 };
 
 // Scalars of one PCG solve, kept on the device so iterations need no host
@@ -62,11 +65,15 @@ hipError_t launch_scale(int n, const double* x, const double* b, double lambda, 
 // blocks (no damping on Hpp) into Sdiag[Cf*36] - the implicit-mode
 // preconditioner; otherwise all row blocks into both triangles of F_val and
 // bschur.
-constexpr int SCHUR_MAXC = 128;
+constexpr int SCHUR_MAXC = 64;      // Hschur blocks per row tile
+constexpr int SCHUR_CHUNK = 256;    // pair products per work item
+// W = Hpl Dinv per observation, camera-major (Dinv from launch_point_dinv).
 template <typename HT>
-hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const double* Hll,
-                             const double* b, const double* Hpp, double lambda, HT* F_val, double* bschur,
-                             HT* Sdiag, int diag_only, int* err, hipStream_t st);
+hipError_t launch_schur_w(const DevGraph& g, const HT* Hpl, const HT* Dinv, HT* W, hipStream_t st);
+template <typename HT>
+hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const HT* W, const double* b,
+                             const double* Hpp, double lambda, HT* F_val, double* bschur, HT* Sdiag, int diag_only,
+                             hipStream_t st);
 
 // ---- implicit Schur operator
 template <typename HT>

@@ -3,9 +3,10 @@
 // Layout in HBM (all built once by structure.cpp, see structure.h):
 //  * slots = active edges, point-major: slot_cam / slot_pos (int32),
 //    slot_meas (2 x f64), slot_w (f64);
-//  * Hpl: one 6x3 block per slot, row-major, Hpl[18 * slot + 3 * row + col]:
-//    a thread owning a slot reads or writes 72 (FP32) / 144 (FP64)
-//    contiguous bytes, and a point's blocks are contiguous;
+//  * Hpl: one row-major 6x3 block per observation with a free camera and a
+//    free point, in camera-major order (q index: camera by camera, points in
+//    processing order), Hpl[18 * q + 3 * row + col]: the blocks one Hschur
+//    block row needs are contiguous runs;
 //  * Hll: 6 unique f64 per free point; b: f64 [6 Cf + 3 Pf] (g2o's layout);
 //  * Hschur: full symmetric block-CSR (both triangles) F_val[nF][36].
 // Camera-side sums of the linearisation and of the matrix-free passes go
@@ -251,9 +252,9 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
       for (int r = 0; r < 6; ++r)
         atomicAdd(&ca[21 + r], static_cast<double>(Jc[r]) * wr0 + static_cast<double>(Jc[6 + r]) * wr1);
       if (ph >= 0) {
+        HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]);
         for (int r = 0; r < 6; ++r)
-          for (int c = 0; c < 3; ++c)
-            Hpl[18 * static_cast<size_t>(s) + 3 * r + c] = omh * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]);
+          for (int c = 0; c < 3; ++c) h[3 * r + c] = omh * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]);
       }
     }
   }
@@ -384,27 +385,72 @@ hipError_t launch_scale(int n, const double* x, const double* b, double lambda, 
 // the workgroup at the end. Every Hschur block is written once, by its row's
 // tile, to both triangles of the block-CSR copy; nothing is accumulated in
 // global memory.
-template <typename HT, int MAXC, int EB, int MAXP>
+// DPP move within 16-lane rows (quad_perm / row_ror controls).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long x = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(x >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+// Sum over the 16 lanes of a DPP row, result in every lane of the row.
+template <typename T>
+__device__ __forceinline__ T row16_sum(T v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm(1, 0, 3, 2)
+  v += dpp_mov<0x4E>(v);   // quad_perm(2, 3, 0, 1)
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  return v;
+}
+
+// W_q = Hpl_q Dinv_p for every observation q (camera-major), Dinv_p =
+// (Hll_p + lambda I)^-1 of the observation's point.
+template <typename HT>
+__global__ __launch_bounds__(256) void k_schur_w(int nq, const int* __restrict__ q_ph, const HT* __restrict__ Hpl,
+                                                 const HT* __restrict__ Dinv, HT* __restrict__ W) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const HT* D = Dinv + 9 * static_cast<size_t>(q_ph[q]);
+  const HT* H = Hpl + 18 * static_cast<size_t>(q);
+  HT d[9], h[18];
+  for (int k = 0; k < 9; ++k) d[k] = D[k];
+  for (int k = 0; k < 18; ++k) h[k] = H[k];
+  HT* o = W + 18 * static_cast<size_t>(q);
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 3; ++c) o[3 * r + c] = h[3 * r] * d[c] + h[3 * r + 1] * d[3 + c] + h[3 * r + 2] * d[6 + c];
+}
+
+template <typename HT>
+hipError_t launch_schur_w(const DevGraph& g, const HT* Hpl, const HT* Dinv, HT* W, hipStream_t st) {
+  if (g.nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_schur_w<HT>, dim3(nblocks(g.nq, 256)), dim3(256), 0, st, g.nq, g.q_ph, Hpl, Dinv, W);
+  return hipGetLastError();
+}
+template hipError_t launch_schur_w<float>(const DevGraph&, const float*, const float*, float*, hipStream_t);
+template hipError_t launch_schur_w<double>(const DevGraph&, const double*, const double*, double*, hipStream_t);
+
+// Hschur block row i, row-owner form. Block (i, j) = sum over the points p
+// seen by both of W_ip Hpl_jp^T, where W_ip and Hpl_jp sit at positions qa
+// and qb of the camera-major lists (runs of consecutive points give
+// consecutive q on both sides). The tile's blocks are cut into work items of
+// at most CH pair products; a 16-lane group takes an item, its lanes stride
+// over the pairs accumulating in registers, the group sums them with DPP row
+// operations and adds the item into the block's LDS accumulator (one atomic
+// add per element per item). The first tile of a row also forms the
+// right-hand-side coefficient sum_p W_ip b_p.
+template <typename HT, int MAXC, int CH>
 __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, const HT* __restrict__ Hpl,
-                                                    const double* __restrict__ Hll, const double* __restrict__ b,
+                                                    const HT* __restrict__ W, const double* __restrict__ b,
                                                     const double* __restrict__ Hpp, double lambda,
                                                     HT* __restrict__ F_val, double* __restrict__ bschur,
-                                                    HT* __restrict__ Sdiag, int diag_only, int* __restrict__ err) {
-  constexpr int MAPW = 1024;  // direct column map when the row's columns span < MAPW cameras
-  static_assert(EB <= 256, "one thread per batch entry");
-  static_assert(MAXC < 255 && EB < (1 << 13) && MAXP >= 1024, "pair packing: column 8 bits, j 10 bits");
-  __shared__ int s_col[MAXC];
-  __shared__ int s_map[MAPW];
+                                                    HT* __restrict__ Sdiag, int diag_only) {
   __shared__ HT s_acc[MAXC * 36];
-  __shared__ HT s_W[EB * 18];
-  __shared__ int s_a[EB];
-  __shared__ int s_pref[EB + 1];
-  __shared__ int s_cnt[MAXC + 1];
-  __shared__ int s_off[MAXC + 2];
-  __shared__ int s_pk[MAXP];      // packed pair: entry << 18 | j << 8 | column
-  __shared__ int s_sorted[MAXP];  // packed pairs grouped by column
+  __shared__ int s_item[MAXC + 1];
   __shared__ double s_red[4 * 8];
-  __shared__ int s_ne;
   const int tile = rp.tile_order[blockIdx.x];
   const int row = rp.tile_row[tile];
   const int k0 = rp.tile_k0[tile], k1 = diag_only ? rp.tile_k0[tile] + 1 : rp.tile_k1[tile];
@@ -412,153 +458,68 @@ __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, c
   const bool diag_tile = (k0 == rp.S_rowptr[row]);
   if (diag_only && !diag_tile) return;
   const int tid = threadIdx.x;
-  const int col_lo = g.S_col[k0], col_hi = g.S_col[k1 - 1];
-  const bool use_map = (col_hi - col_lo) < MAPW;
-  for (int i = tid; i < ncol; i += 256) s_col[i] = g.S_col[k0 + i];
+  const int grp = tid >> 4, gl = tid & 15;  // 16 groups of 16 lanes (one DPP row each)
   for (int i = tid; i < ncol * 36; i += 256) s_acc[i] = HT(0);
-  if (use_map) {
-    __syncthreads();
-    for (int i = tid; i < ncol; i += 256) s_map[s_col[i] - col_lo] = i;
+  if (tid == 0) {
+    int acc = 0;
+    for (int c = 0; c < ncol; ++c) {
+      s_item[c] = acc;
+      acc += (rp.blk_pair_ptr[k0 + c + 1] - rp.blk_pair_ptr[k0 + c] + CH - 1) / CH;
+    }
+    s_item[ncol] = acc;
   }
+  // ---- right-hand-side coefficient over the camera's observations
   double coef[6] = {0, 0, 0, 0, 0, 0};
-  const double* bl_base = b + 6 * static_cast<size_t>(g.Cf);
-  const int e_end = rp.row_ptr[row + 1];
-  int e0 = rp.row_ptr[row];
-  while (e0 < e_end) {
-    const int cand = min(EB, e_end - e0);
-    __syncthreads();  // previous batch finished with the staging arrays
-    // ---- phase 1, one thread per observation (point p, slot a of this
-    // camera): Dinv_p = (Hll_p + lambda I)^-1 in f64, W = Hpl_a Dinv_p to
-    // LDS, the coefficient Hpl_a Dinv_p b_p, and m = slots b >= a of p.
-    int m = 0;
-    double cf[6] = {0, 0, 0, 0, 0, 0};
-    if (tid < cand) {
-      const int a = rp.row_slot[e0 + tid];
-      const int pos = g.slot_pos[a];
-      const int ph = g.pos_h[pos];
-      m = g.pos_ptr[pos] + g.pos_nfree[pos] - a;
-      double D[9];
-      if (!inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
-        atomicOr(err, 1);
-        for (int i = 0; i < 9; ++i) D[i] = 0.0;
-      }
-      const HT* pa = Hpl + 18 * static_cast<size_t>(a);
-      HT Ha[18];
-      for (int k = 0; k < 18; ++k) Ha[k] = pa[k];
-      HT Dh[9];
-      for (int k = 0; k < 9; ++k) Dh[k] = static_cast<HT>(D[k]);
-      HT* W = s_W + 18 * tid;
+  if (diag_tile && !diag_only) {
+    const double* bl_base = b + 6 * static_cast<size_t>(g.Cf);
+    for (int q = g.cam_qptr[row] + tid; q < g.cam_qptr[row + 1]; q += 256) {
+      const HT* w = W + 18 * static_cast<size_t>(q);
+      const double* bl = bl_base + 3 * static_cast<size_t>(g.q_ph[q]);
       for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 3; ++c) W[3 * r + c] = Ha[3 * r] * Dh[c] + Ha[3 * r + 1] * Dh[3 + c] + Ha[3 * r + 2] * Dh[6 + c];
-      if (diag_tile && !diag_only) {
-        const double* bl = bl_base + 3 * static_cast<size_t>(ph);
-        double db[3];
-        for (int r = 0; r < 3; ++r) db[r] = D[3 * r] * bl[0] + D[3 * r + 1] * bl[1] + D[3 * r + 2] * bl[2];
-        for (int r = 0; r < 6; ++r)
-          cf[r] = static_cast<double>(Ha[3 * r]) * db[0] + static_cast<double>(Ha[3 * r + 1]) * db[1] +
-                  static_cast<double>(Ha[3 * r + 2]) * db[2];
-      }
-      s_a[tid] = a;
+        coef[r] += static_cast<double>(w[3 * r]) * bl[0] + static_cast<double>(w[3 * r + 1]) * bl[1] +
+                   static_cast<double>(w[3 * r + 2]) * bl[2];
     }
-    // inclusive scan of m over the batch (Hillis-Steele in LDS); only the
-    // first EB threads own an entry of s_pref
-    if (tid < EB) s_pref[tid + 1] = m;
-    if (tid == 0) s_pref[0] = 0;
-    __syncthreads();
-    for (int off = 1; off < EB; off <<= 1) {
-      const int v = (tid < EB && tid + 1 > off) ? s_pref[tid + 1 - off] : 0;
-      __syncthreads();
-      if (tid < EB) s_pref[tid + 1] += v;
-      __syncthreads();
-    }
-    // the batch ends where its pairs would exceed MAXP (at least one entry)
-    if (tid == 0) {
-      int lo = 1, hi = cand;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_pref[mid] <= MAXP) lo = mid;
-        else hi = mid - 1;
-      }
-      s_ne = lo;
-    }
-    for (int i = tid; i <= ncol; i += 256) s_cnt[i] = 0;
-    __syncthreads();
-    const int ne = s_ne;
-    const int total = s_pref[ne];
-    if (tid < ne)
-      for (int r = 0; r < 6; ++r) coef[r] += cf[r];
-    // ---- phase 2a: column of every pair (a, b = a + j) and a histogram
-    for (int k = tid; k < total; k += 256) {
-      int lo = 0, hi = ne;  // last e with s_pref[e] <= k
+  }
+  __syncthreads();
+  const int nitems = s_item[ncol];
+  for (int it = grp; it < nitems; it += 16) {
+    int c = 0;  // last column with s_item[c] <= it
+    {
+      int lo = 0, hi = ncol;
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= k) lo = mid;
+        if (s_item[mid] <= it) lo = mid;
         else hi = mid;
       }
-      const int j = k - s_pref[lo];
-      const int cb = g.slot_camh[s_a[lo] + j];
-      int lc = ncol;  // outside this tile
-      if (cb >= col_lo && cb <= col_hi) lc = use_map ? s_map[cb - col_lo] : lower_bound_i(s_col, ncol, cb);
-      s_pk[k] = (lo << 18) | (j << 8) | lc;
-      atomicAdd(&s_cnt[lc], 1);
+      c = lo;
     }
-    __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int c = 0; c <= ncol; ++c) {
-        s_off[c] = acc;
-        acc += s_cnt[c];
-      }
-      s_off[ncol + 1] = acc;
-    }
-    __syncthreads();
-    for (int i = tid; i <= ncol; i += 256) s_cnt[i] = s_off[i];  // scatter cursors
-    __syncthreads();
-    for (int k = tid; k < total; k += 256) {
-      const int v = s_pk[k];
-      const int lc = v & 255;
-      if (lc < ncol) s_sorted[atomicAdd(&s_cnt[lc], 1)] = v;
-    }
-    __syncthreads();
-    // ---- phase 2b: each thread walks a contiguous run of the column-sorted
-    // pairs, accumulates W Hb^T in registers and flushes when the column
-    // changes. A pair b != a on the diagonal column (two edges of one point to
-    // camera i) adds the product and its transpose: g2o merges such edges into
-    // one Hpl block.
-    const int nvalid = s_off[ncol];
-    const int chunk = (nvalid + 255) / 256;
-    const int p_begin = min(tid * chunk, nvalid), p_end = min(p_begin + chunk, nvalid);
+    const int pbeg = rp.blk_pair_ptr[k0 + c] + (it - s_item[c]) * CH;
+    const int pend = min(pbeg + CH, rp.blk_pair_ptr[k0 + c + 1]);
+    const bool diag_col = diag_tile && c == 0;
     HT acc[36];
     for (int q = 0; q < 36; ++q) acc[q] = HT(0);
-    int cur = -1;
-    for (int p = p_begin; p < p_end; ++p) {
-      const int v = s_sorted[p];
-      const int lc = v & 255, j = (v >> 8) & 1023, e = v >> 18;
-      if (lc != cur) {
-        if (cur >= 0) {
-          HT* dst = s_acc + 36 * cur;
-          for (int q = 0; q < 36; ++q) atomicAdd(&dst[q], -acc[q]);
-          for (int q = 0; q < 36; ++q) acc[q] = HT(0);
-        }
-        cur = lc;
-      }
-      const HT* W = s_W + 18 * e;
-      const HT* pb = Hpl + 18 * static_cast<size_t>(s_a[e] + j);
-      HT Hb[18];
-      for (int q = 0; q < 18; ++q) Hb[q] = pb[q];
-      const bool sym = diag_tile && lc == 0 && j != 0;
+    for (int p = pbeg + gl; p < pend; p += 16) {
+      const int qa = rp.pair_qa[p], qb = rp.pair_qb[p];
+      const HT* w = W + 18 * static_cast<size_t>(qa);
+      const HT* h = Hpl + 18 * static_cast<size_t>(qb);
+      HT Wa[18], Hb[18];
+      for (int k = 0; k < 18; ++k) { Wa[k] = w[k]; Hb[k] = h[k]; }
+      // a second edge of the point to camera i (qb != qa on the diagonal
+      // block) adds the product and its transpose: g2o merges such edges into
+      // one Hpl block
+      const bool sym = diag_col && qa != qb;
       for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c) {
-          HT t = W[3 * r] * Hb[3 * c] + W[3 * r + 1] * Hb[3 * c + 1] + W[3 * r + 2] * Hb[3 * c + 2];
-          if (sym) t += W[3 * c] * Hb[3 * r] + W[3 * c + 1] * Hb[3 * r + 1] + W[3 * c + 2] * Hb[3 * r + 2];
-          acc[6 * r + c] += t;
+        for (int cc = 0; cc < 6; ++cc) {
+          HT x = Wa[3 * r] * Hb[3 * cc] + Wa[3 * r + 1] * Hb[3 * cc + 1] + Wa[3 * r + 2] * Hb[3 * cc + 2];
+          if (sym) x += Wa[3 * cc] * Hb[3 * r] + Wa[3 * cc + 1] * Hb[3 * r + 1] + Wa[3 * cc + 2] * Hb[3 * r + 2];
+          acc[6 * r + cc] += x;
         }
     }
-    if (cur >= 0) {
-      HT* dst = s_acc + 36 * cur;
-      for (int q = 0; q < 36; ++q) atomicAdd(&dst[q], -acc[q]);
+    HT* dst = s_acc + 36 * c;
+    for (int q = 0; q < 36; ++q) {
+      const HT sum = row16_sum(acc[q]);
+      if ((q & 15) == gl) atomicAdd(&dst[q], -sum);
     }
-    e0 += ne;
   }
   // workgroup reduction of the coefficient (diagonal tile only)
   if (diag_tile) {
@@ -602,21 +563,20 @@ __global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, c
 }
 
 template <typename HT>
-hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const double* Hll,
-                             const double* b, const double* Hpp, double lambda, HT* F_val, double* bschur,
-                             HT* Sdiag, int diag_only, int* err, hipStream_t st) {
+hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const HT* W, const double* b,
+                             const double* Hpp, double lambda, HT* F_val, double* bschur, HT* Sdiag, int diag_only,
+                             hipStream_t st) {
   if (rp.ntiles == 0) return hipSuccess;
-  constexpr int EB = sizeof(HT) == 4 ? 256 : 128;
-  hipLaunchKernelGGL((k_schur_rows<HT, SCHUR_MAXC, EB, 2048>), dim3(rp.ntiles), dim3(256), 0, st, g, rp, Hpl, Hll, b,
-                     Hpp, lambda, F_val, bschur, Sdiag, diag_only, err);
+  hipLaunchKernelGGL((k_schur_rows<HT, SCHUR_MAXC, SCHUR_CHUNK>), dim3(rp.ntiles), dim3(256), 0, st, g, rp, Hpl, W, b,
+                     Hpp, lambda, F_val, bschur, Sdiag, diag_only);
   return hipGetLastError();
 }
-template hipError_t launch_schur_rows<float>(const DevGraph&, const DevRowPlan&, const float*, const double*,
-                                             const double*, const double*, double, float*, double*, float*, int, int*,
+template hipError_t launch_schur_rows<float>(const DevGraph&, const DevRowPlan&, const float*, const float*,
+                                             const double*, const double*, double, float*, double*, float*, int,
                                              hipStream_t);
 template hipError_t launch_schur_rows<double>(const DevGraph&, const DevRowPlan&, const double*, const double*,
                                               const double*, const double*, double, double*, double*, double*, int,
-                                              int*, hipStream_t);
+                                              hipStream_t);
 
 // ========================================================== implicit Schur
 template <typename HT>
@@ -675,9 +635,10 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
       const int chh = g.camH[g.slot_cam[s]];
       if (g.pos_h[pos] < 0 || chh < 0) continue;
       const HT* pc = p + 6 * static_cast<size_t>(chh);
+      const HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]);
       for (int c = 0; c < 3; ++c) {
         HT v = 0;
-        for (int r = 0; r < 6; ++r) v += Hpl[18 * static_cast<size_t>(s) + 3 * r + c] * pc[r];
+        for (int r = 0; r < 6; ++r) v += h[3 * r + c] * pc[r];
         atomicAdd(&s_v[3 * (pos - p0) + c], static_cast<double>(v));
       }
     }
@@ -707,7 +668,7 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
     const HT y0 = static_cast<HT>(y[0]), y1 = static_cast<HT>(y[1]), y2 = static_cast<HT>(y[2]);
     const int lc = lower_bound_i(s_cam, nc, chh);
     for (int r = 0; r < 6; ++r) {
-      const HT* h = Hpl + 18 * static_cast<size_t>(s) + 3 * r;
+      const HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]) + 3 * r;
       HT v = h[0] * y0 + h[1] * y1 + h[2] * y2;
       atomicAdd(&s_cacc[6 * lc + r], static_cast<double>(v));
     }
@@ -1201,20 +1162,22 @@ __global__ void k_zero_u32(unsigned* p, int n) {
 // workgroup leaves the solve instead of waiting forever).
 __device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, PcgScalars* sc) {
   __shared__ int s_ok;
-  __threadfence();  // every wave's global stores are complete and visible at agent scope
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's global stores have reached L2
   __syncthreads();
   if (threadIdx.x == 0) {
     s_ok = 1;
-    __threadfence();
-    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish the workgroup's stores
+    const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == nblocks - 1) {
       __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&bar[1], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
+      // relaxed polling: no cache invalidation per poll; the acquire fence
+      // below runs once the generation moved
       long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (++spins > (1L << 26)) {
+      while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (++spins > (1L << 24)) {
           __hip_atomic_store(&sc->err, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           s_ok = 0;
           break;
@@ -1222,7 +1185,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, Pc
         __builtin_amdgcn_s_sleep(1);
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
   return s_ok != 0;

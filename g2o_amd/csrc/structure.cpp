@@ -107,7 +107,30 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   }
   st.pos_ptr[st.npos] = s;
   st.slot_camh.resize(st.ns);
-  for (int q = 0; q < st.ns; ++q) st.slot_camh[q] = st.camH[st.slot_cam[q]];
+  st.slot_ph.resize(st.ns);
+  for (int q = 0; q < st.ns; ++q) {
+    st.slot_camh[q] = st.camH[st.slot_cam[q]];
+    st.slot_ph[q] = st.pos_h[st.slot_pos[q]];
+  }
+  st.cam_qptr.assign(st.Cf + 1, 0);
+  for (int q = 0; q < st.ns; ++q)
+    if (st.slot_camh[q] >= 0 && st.slot_ph[q] >= 0) st.cam_qptr[st.slot_camh[q] + 1]++;
+  for (int c = 0; c < st.Cf; ++c) st.cam_qptr[c + 1] += st.cam_qptr[c];
+  st.nq = st.cam_qptr[st.Cf];
+  st.q_slot.resize(st.nq);
+  st.q_ph.resize(st.nq);
+  st.slot_q.assign(st.ns, -1);
+  {
+    std::vector<int> fill(st.cam_qptr.begin(), st.cam_qptr.end() - 1);
+    for (int q = 0; q < st.ns; ++q) {  // slots ascend: each camera lists its points in processing order
+      const int c = st.slot_camh[q];
+      if (c < 0 || st.slot_ph[q] < 0) continue;
+      const int d = fill[c]++;
+      st.q_slot[d] = q;
+      st.q_ph[d] = st.slot_ph[q];
+      st.slot_q[q] = d;
+    }
+  }
 
   // linearisation chunks: consecutive points, bounded slots / cameras / points
   const LinPlanCaps& lc = st.lcaps;
@@ -216,27 +239,15 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   return "";
 }
 
-std::string build_row_plan(BAStructure& st, int max_cols, int num_xcds) {
+std::string build_row_plan(BAStructure& st, int max_cols, int chunk, int num_xcds) {
   RowPlan& rp = st.rows;
   rp = RowPlan();
   rp.max_cols = max_cols;
+  rp.chunk = chunk;
   const int Cf = st.Cf;
-  rp.row_ptr.assign(Cf + 1, 0);
-  for (int s = 0; s < st.ns; ++s) {
-    int c = st.camH[st.slot_cam[s]];
-    if (c >= 0 && st.pos_h[st.slot_pos[s]] >= 0) rp.row_ptr[c + 1]++;
-  }
-  for (int c = 0; c < Cf; ++c) rp.row_ptr[c + 1] += rp.row_ptr[c];
-  rp.row_slot.resize(rp.row_ptr[Cf]);
-  {
-    std::vector<int> fill(rp.row_ptr.begin(), rp.row_ptr.end() - 1);
-    for (int s = 0; s < st.ns; ++s) {  // slots ascend, so each row lists its points in slot order
-      int c = st.camH[st.slot_cam[s]];
-      if (c >= 0 && st.pos_h[st.slot_pos[s]] >= 0) rp.row_slot[fill[c]++] = s;
-    }
-  }
   for (int i = 0; i < st.npos; ++i)
     if (st.pos_h[i] >= 0) rp.pair_instances += static_cast<long long>(st.pos_nfree[i]) * (st.pos_nfree[i] + 1) / 2;
+  if (rp.pair_instances >= (1LL << 31)) return "more than 2^31 Schur pair products";
   for (int r = 0; r < Cf; ++r) {
     const int k0 = st.S_rowptr[r], k1 = st.S_rowptr[r + 1];
     if (k1 <= k0 || st.S_col[k0] != r) return "Hschur row without its diagonal block";
@@ -245,6 +256,40 @@ std::string build_row_plan(BAStructure& st, int max_cols, int num_xcds) {
       rp.tile_row.push_back(r);
       rp.tile_k0.push_back(k);
       rp.tile_k1.push_back(std::min(k1, k + max_cols));
+    }
+  }
+  // pair products grouped by upper block (counting sort by block)
+  rp.blk_pair_ptr.assign(st.nS + 1, 0);
+  std::vector<int> pair_blk;
+  pair_blk.reserve(static_cast<size_t>(rp.pair_instances));
+  for (int i = 0; i < st.npos; ++i) {
+    if (st.pos_h[i] < 0) continue;
+    const int a0 = st.pos_ptr[i], nf = st.pos_nfree[i];
+    for (int a = 0; a < nf; ++a) {
+      int k = st.S_rowptr[st.slot_camh[a0 + a]];
+      for (int bb = a; bb < nf; ++bb) {
+        const int cb = st.slot_camh[a0 + bb];
+        while (st.S_col[k] < cb) ++k;  // cameras ascend within the point
+        pair_blk.push_back(k);
+        rp.blk_pair_ptr[k + 1]++;
+      }
+    }
+  }
+  for (int k = 0; k < st.nS; ++k) rp.blk_pair_ptr[k + 1] += rp.blk_pair_ptr[k];
+  rp.pair_qa.resize(static_cast<size_t>(rp.pair_instances));
+  rp.pair_qb.resize(static_cast<size_t>(rp.pair_instances));
+  {
+    std::vector<int> fill(rp.blk_pair_ptr.begin(), rp.blk_pair_ptr.end() - 1);
+    size_t q = 0;
+    for (int i = 0; i < st.npos; ++i) {
+      if (st.pos_h[i] < 0) continue;
+      const int a0 = st.pos_ptr[i], nf = st.pos_nfree[i];
+      for (int a = 0; a < nf; ++a)
+        for (int bb = a; bb < nf; ++bb) {
+          const int dst = fill[pair_blk[q++]]++;
+          rp.pair_qa[dst] = st.slot_q[a0 + a];
+          rp.pair_qb[dst] = st.slot_q[a0 + bb];
+        }
     }
   }
   // Workgroups are dealt to the XCDs round-robin by blockIdx; map blockIdx so

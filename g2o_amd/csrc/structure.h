@@ -34,11 +34,16 @@ struct LinPlanCaps {
 // [k0, k1) (indices into the upper block list) in LDS; the first tile of a row
 // also owns the diagonal block and the right-hand-side coefficient.
 struct RowPlan {
-  int max_cols = 128;                // columns per tile (LDS accumulator)
-  std::vector<int> row_ptr;          // Cf+1
-  std::vector<int> row_slot;         // slots (free camera, free point), camera-major
+  int max_cols = 64;                 // Hschur blocks per row tile
+  int chunk = 256;                   // pair products per work item
   std::vector<int> tile_row, tile_k0, tile_k1;
   std::vector<int> tile_order;       // blockIdx -> tile, XCD-aware
+  // pair products of each upper Hschur block (i, j): positions (qa, qb) of
+  // the point's observations in camera i's and camera j's observation lists
+  // (the camera-major order of Hpl), grouped by block, points in processing
+  // order; CSR over the nS upper blocks
+  std::vector<int> blk_pair_ptr;     // nS+1 (pair count < 2^31)
+  std::vector<int> pair_qa, pair_qb;
   long long pair_instances = 0;
   int max_row_cols = 0;
 };
@@ -58,6 +63,14 @@ struct BAStructure {
   int ns = 0;
   std::vector<int> slot_edge, slot_cam, slot_pos;
   std::vector<int> slot_camh;        // free index of the slot's camera or -1
+  std::vector<int> slot_ph;          // free index of the slot's point or -1
+  // camera-major observation lists: slots with a free camera and a free
+  // point, camera by camera, points in processing order. Hpl is stored in
+  // this order ("q" index).
+  int nq = 0;
+  std::vector<int> cam_qptr;         // Cf+1
+  std::vector<int> q_slot, q_ph;     // nq
+  std::vector<int> slot_q;           // ns -> q or -1
   std::vector<int> edge_slot;        // E -> slot or -1 (inactive edge)
   // linearisation chunks
   LinPlanCaps lcaps;
@@ -83,6 +96,6 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
                             const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur);
 
 // Builds st.rows; num_xcds spreads consecutive block rows over the same XCD.
-std::string build_row_plan(BAStructure& st, int max_cols, int num_xcds);
+std::string build_row_plan(BAStructure& st, int max_cols, int chunk, int num_xcds);
 
 }  // namespace g2ohip
