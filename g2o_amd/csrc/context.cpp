@@ -353,75 +353,81 @@ int g2ohip_ctx::compute_scale(double* out) {
 }
 
 // ------------------------------------------------------------------- PCG
+namespace {
+
+// One LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) on the device:
+// init, then `op` (direction + operator) and the update kernel per
+// iteration. The stop test runs on the device and later launches return at
+// once, so the host only polls the scalars: first after `hint` iterations
+// (the previous solve's count), then every 4. If the loop reaches max_iter
+// the head runs once more to record the final residual.
+template <typename HT, int BD, typename OpFn>
+hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const double* rhs, const HT* J, HT* x, HT* r,
+                     HT* s, HT* const dirs[2], HT* q, double* part_rs, double* part_dq, int n_dq,
+                     g2ohip::PcgScalars* sc, int max_iter, int hint, OpFn op, g2ohip::PcgRun* out) {
+  using namespace g2ohip;
+  hipError_t e;
+  PcgScalars hs{};
+  auto poll = [&]() -> hipError_t {
+    hipError_t e2 = hipMemcpyAsync(h_pinned, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream);
+    if (e2 == hipSuccess) e2 = hipStreamSynchronize(stream);
+    if (e2 == hipSuccess) std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
+    return e2;
+  };
+  if ((e = hipMemsetAsync(sc, 0, sizeof(PcgScalars), stream)) != hipSuccess) return e;
+  if ((e = launch_pcg_init<HT, BD>(nb, rhs, J, x, r, s, part_rs, stream)) != hipSuccess) return e;
+  int next = std::max(4, hint);
+  bool done = false;
+  for (int it = 0; it < max_iter; ++it) {
+    if ((e = op(it, 0, dirs[(it + 1) & 1], dirs[it & 1])) != hipSuccess) return e;
+    if ((e = launch_pcg_update<HT, BD>(nb, J, x, r, dirs[it & 1], q, s, part_dq, n_dq, part_rs, sc, it, stream)) !=
+        hipSuccess)
+      return e;
+    if (it + 1 == next) {
+      if ((e = poll()) != hipSuccess) return e;
+      if (hs.done) { done = true; break; }
+      next += 4;
+    }
+  }
+  if (!done) {
+    if ((e = op(max_iter, 1, dirs[(max_iter + 1) & 1], dirs[max_iter & 1])) != hipSuccess) return e;
+    if ((e = poll()) != hipSuccess) return e;
+  }
+  out->iters = hs.iters;
+  out->dn_final = hs.dn[hs.iters & 1];
+  out->d0 = hs.d0;
+  out->dn0 = hs.dn0;
+  return hipSuccess;
+}
+
+}  // namespace
+
 template <typename HT>
-int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, bool use_memory, bool implicit, int* iters,
-                        double* dn_final) {
+int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ctl, bool implicit, int max_iter,
+                        int hint, PcgRun* out) {
   const int nb = st.Cf;
   const int n = 6 * nb;
-  HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
-  HK(hipMemsetAsync(d_sc.p, 0, sizeof(PcgScalars), stream));
-  const int max_iter_p = opt.pcg_max_iter < 0 ? n : opt.pcg_max_iter;
-  if (!implicit) {
-    const int wi = std::is_same<HT, float>::value ? 0 : 1;
-    if (pcg_grid[wi] < 0) {
-      // The one-launch PCG is opt-in (G2OHIP_PCG_PERSISTENT=1): its grid
-      // barriers cost more than the kernel-per-step loop on gfx950.
-      const char* env = std::getenv("G2OHIP_PCG_PERSISTENT");
-      const int mx = (env && env[0] == '1') ? pcg_persistent_max_grid<HT>(device) : 0;
-      pcg_grid[wi] = std::min({mx, 128, (nb + 3) / 4});
-    }
-    if (pcg_grid[wi] > 0) {
-      HK(B.d2.alloc(n));
-      HK(d_bar.alloc(2));
-      HK(d_gpart.alloc(2 * static_cast<size_t>(pcg_grid[wi])));
-      ProfScope ps(this, G2OHIP_K_PCG);
-      hipError_t e = launch_pcg_persistent<HT>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.Jinv.p, rhs, B.x.p, B.r.p,
-                                               B.d.p, B.d2.p, B.q.p, B.s.p, d_gpart.p, d_bar.p, d_sc.p,
-                                               opt.pcg_tolerance, use_memory ? pcg_residual_mem : -1.0, max_iter_p,
-                                               pcg_grid[wi], stream);
-      if (e == hipSuccess) {
-        PcgScalars hs{};
-        HK(hipMemcpyAsync(h_pinned, d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
-        HK(hipStreamSynchronize(stream));
-        std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
-        if (hs.err) return fail(G2OHIP_ERR_HIP, "persistent PCG grid barrier timed out");
-        *iters = hs.iters;
-        *dn_final = hs.dn[hs.iters & 1];
-        return G2OHIP_OK;
-      }
-      (void)hipGetLastError();
-      pcg_grid[wi] = 0;  // not co-resident on this device: use the kernel-per-step loop
-    }
-  }
-  HK(launch_pcg_init<HT, 6>(nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.d.p, d_dn_part.p, stream));
-  HK(launch_pcg_start(nb, d_dn_part.p, opt.pcg_tolerance, use_memory ? pcg_residual_mem : -1.0, d_sc.p, stream));
-  const int max_iter = opt.pcg_max_iter < 0 ? n : opt.pcg_max_iter;
+  HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.d2.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
+  HT* const dirs[2] = {B.d.p, B.d2.p};
   ProfScope ps(this, G2OHIP_K_PCG);
-  PcgScalars hs{};
-  const int check = 4;
-  for (int it = 0; it < max_iter; ++it) {
-    if (implicit) {
-      HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, B.d.p, nullptr, d_cam_part.p, d_sc.p, it, stream));
-      HK(launch_implicit_reduce<HT>(g, d_cam_part.p, d_Hpp.p, lambda, B.d.p, B.q.p, d_dq_part.p, d_sc.p, it, stream));
-    } else {
-      HK(launch_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.d.p, B.q.p, d_dq_part.p, d_sc.p, it, stream));
-    }
-    HK(launch_pcg_step<HT, 6>(nb, B.Jinv.p, B.x.p, B.r.p, B.d.p, B.q.p, B.s.p, d_dq_part.p, d_dn_part.p, d_sc.p, it,
-                              stream));
-    HK(launch_pcg_dir<HT, 6>(nb, B.d.p, B.s.p, d_dn_part.p, d_sc.p, it, stream));
-    if ((it + 1) % check == 0) {
-      HK(hipMemcpyAsync(h_pinned, d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
-      HK(hipStreamSynchronize(stream));
-      std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
-      if (hs.done) break;
-    }
+  if (implicit) {
+    auto op = [&](int it, int last, const HT* d_old, HT* d) -> hipError_t {
+      hipError_t e = launch_pcg_dir<HT, 6>(nb, B.s.p, d_old, d, d_dn_part.p, d_sc.p, ctl, it, last, stream);
+      if (e != hipSuccess || last) return e;
+      e = launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, d, nullptr, d_cam_part.p, d_sc.p, it, stream);
+      if (e != hipSuccess) return e;
+      return launch_implicit_reduce<HT>(g, d_cam_part.p, d_Hpp.p, lambda, d, B.q.p, d_dq_part.p, d_sc.p, it, stream);
+    };
+    HK((pcg_drive<HT, 6>(stream, h_pinned, nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p, d_dn_part.p,
+                         d_dq_part.p, pcg_row_blocks(nb), d_sc.p, max_iter, hint, op, out)));
+  } else {
+    auto op = [&](int it, int last, const HT* d_old, HT* d) -> hipError_t {
+      return launch_pcg_dir_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, d_old, d, B.q.p, d_dn_part.p,
+                                        d_dq_part.p, d_sc.p, ctl, it, last, stream);
+    };
+    HK((pcg_drive<HT, 6>(stream, h_pinned, nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p, d_dn_part.p,
+                         d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, hint, op, out)));
   }
-  HK(hipMemcpyAsync(h_pinned, d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
-  HK(hipStreamSynchronize(stream));
-  std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
-  int k = hs.done ? hs.iters : max_iter;
-  *iters = k;
-  *dn_final = hs.dn[k & 1];
   return G2OHIP_OK;
 }
 
@@ -465,46 +471,47 @@ int g2ohip_ctx::solve_impl(int* ok) {
   CK(read_err(&err));
   if (err) { *ok = 0; return G2OHIP_OK; }
   const bool refine = (which == 0) && opt.refine_steps > 0;
-  int total_it = 0, it = 0;
-  double dn_final = 0;
-  if (!refine) {
-    CK(run_pcg<HT>(B, d_bschur.p, true, implicit, &it, &dn_final));
-    total_it += it;
-    pcg_residual_mem = 0.5 * dn_final;
-    HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 0.0, stream));
-    // back-substitution x_l = Dinv (b_l - Hpl^T x_p), re-linearised in FP64
-    {
-      ProfScope ps(this, G2OHIP_K_REFINE);
-      HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, d_x.p, d_x.p + 6 * static_cast<size_t>(nb), nullptr, d_err.p,
-                       stream));
-    }
-    last_info[1] = 0;
-  } else {
-    // FP64 iterative refinement: r = g - S x in FP64 (matrix-free), FP32 PCG
-    // corrections. The first solve keeps the PCG residual memory semantics;
-    // the corrections stop on their own relative tolerance.
-    {
-      ProfScope ps(this, G2OHIP_K_REFINE);
-      HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, nullptr, d_x.p + 6 * static_cast<size_t>(nb), d_cam_part.p,
-                       d_err.p, stream));
-    }
-    HK(launch_refine_reduce(g, d_cam_part.p, nullptr, lambda, d_res.p, stream));
-    HK(hipMemsetAsync(d_x.p, 0, 6 * sizeof(double) * nb, stream));
-    for (int step = 0; step <= opt.refine_steps; ++step) {
-      CK(run_pcg<HT>(B, d_res.p, step == 0, implicit, &it, &dn_final));
-      total_it += it;
-      if (step == 0) pcg_residual_mem = 0.5 * dn_final;
-      HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 1.0, stream));
-      const bool last = step == opt.refine_steps;
+  const int max_iter = opt.pcg_max_iter < 0 ? 6 * nb : opt.pcg_max_iter;
+  const double* xp64 = d_x.p;
+  double* xl64 = d_x.p + 6 * static_cast<size_t>(nb);
+  // reduced system on the Schur right-hand side, with the residual memory of
+  // the previous solve (linear_solver_pcg.hpp:128-136)
+  PcgRun pr;
+  CK(run_pcg<HT>(B, d_bschur.p, PcgControl{opt.pcg_tolerance, pcg_residual_mem, -1.0}, implicit, max_iter,
+                 pcg_hint[0], &pr));
+  pcg_hint[0] = pr.iters;
+  int total_it = pr.iters;
+  double dn_final = pr.dn_final;
+  HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 0.0, stream));
+  if (refine) {
+    // FP64 iterative refinement: r = g - S x evaluated in FP64 matrix-free from
+    // the state (no FP32 block enters it), then an FP32 PCG correction S dx = r
+    // that stops on the first solve's absolute threshold d0, so the refined x
+    // meets the reference stop rule measured on the FP64 residual. A
+    // correction never runs longer than the first solve plus 10 iterations.
+    for (int step = 0; step < opt.refine_steps; ++step) {
       {
         ProfScope ps(this, G2OHIP_K_REFINE);
-        HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, d_x.p, d_x.p + 6 * static_cast<size_t>(nb),
-                         last ? nullptr : d_cam_part.p, d_err.p, stream));
+        HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, xp64, xl64, d_cam_part.p, d_err.p, stream));
       }
-      if (!last) HK(launch_refine_reduce(g, d_cam_part.p, d_x.p, lambda, d_res.p, stream));
+      HK(launch_refine_reduce(g, d_cam_part.p, xp64, lambda, d_res.p, stream));
+      PcgRun pc;
+      CK(run_pcg<HT>(B, d_res.p, PcgControl{opt.pcg_tolerance, -1.0, pr.d0}, implicit,
+                     std::min(max_iter, pr.iters + 10), pcg_hint[1], &pc));
+      pcg_hint[1] = pc.iters;
+      total_it += pc.iters;
+      dn_final = pc.dn_final;
+      if (pc.iters > 0) HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 1.0, stream));
     }
-    last_info[1] = opt.refine_steps;
   }
+  pcg_residual_mem = 0.5 * dn_final;
+  // back-substitution x_l = Dinv (b_l - Hpl^T x_p), re-linearised in FP64
+  // (block_solver.hpp:452-476)
+  {
+    ProfScope ps(this, G2OHIP_K_REFINE);
+    HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, xp64, xl64, nullptr, d_err.p, stream));
+  }
+  last_info[1] = refine ? opt.refine_steps : 0;
   CK(read_err(&err));
   if (err) *ok = 0;
   last_info[0] = total_it;
@@ -568,6 +575,7 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   *done = 0;
   pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
+  pcg_hint[0] = pcg_hint[1] = 0;
   double ni = 2.0;
   for (int i = 0; i < iterations; ++i) {
     if (force_stop && *force_stop) break;
@@ -734,7 +742,7 @@ int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, cons
     Fptr[i + 1] = static_cast<int>(Fcol.size());
   }
   DBuf<int> dFptr, dFcol, dFdiag;
-  DBuf<double> dFval, dJ, db, dx, dr, dd, dq, ds, ddq, ddn;
+  DBuf<double> dFval, dJ, db, dx, dr, dd, dd2, dq, ds, ddq, ddn;
   DBuf<int> derr;
   DBuf<PcgScalars> dsc;
   HK(dFptr.upload(Fptr, stream)); HK(dFcol.upload(Fcol, stream)); HK(dFdiag.upload(Fdiag, stream));
@@ -742,33 +750,25 @@ int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, cons
   std::vector<double> bv(b, b + static_cast<size_t>(nb) * bd);
   HK(db.upload(bv, stream));
   const size_t n = static_cast<size_t>(nb) * bd;
-  HK(dJ.alloc(static_cast<size_t>(nb) * bb)); HK(dx.alloc(n)); HK(dr.alloc(n)); HK(dd.alloc(n)); HK(dq.alloc(n));
-  HK(ds.alloc(n)); HK(ddq.alloc(nb)); HK(ddn.alloc(nb)); HK(derr.alloc(1)); HK(dsc.alloc(1));
+  HK(dJ.alloc(static_cast<size_t>(nb) * bb)); HK(dx.alloc(n)); HK(dr.alloc(n)); HK(dd.alloc(n)); HK(dd2.alloc(n));
+  HK(dq.alloc(n)); HK(ds.alloc(n)); HK(ddq.alloc(pcg_spmv_blocks(nb))); HK(ddn.alloc(pcg_row_blocks(nb)));
+  HK(derr.alloc(1)); HK(dsc.alloc(1));
   HK(hipMemsetAsync(derr.p, 0, sizeof(int), stream));
+  const PcgControl ctl{tol, residual_io ? *residual_io : -1.0, -1.0};
+  double* const dirs[2] = {dd.p, dd2.p};
   auto run = [&](auto bdc) -> int {
     constexpr int BD = decltype(bdc)::value;
     HK((launch_jacobi<double, BD>(nb, dFval.p, dFdiag.p, dJ.p, derr.p, stream)));
-    HK((launch_pcg_init<double, BD>(nb, db.p, dJ.p, dx.p, dr.p, dd.p, ddn.p, stream)));
-    HK(launch_pcg_start(nb, ddn.p, tol, residual_io ? *residual_io : -1.0, dsc.p, stream));
     const int mi = max_iter < 0 ? static_cast<int>(n) : max_iter;
-    PcgScalars hs{};
-    for (int it = 0; it < mi; ++it) {
-      HK((launch_spmv<double, BD>(nb, dFptr.p, dFcol.p, dFval.p, dd.p, dq.p, ddq.p, dsc.p, it, stream)));
-      HK((launch_pcg_step<double, BD>(nb, dJ.p, dx.p, dr.p, dd.p, dq.p, ds.p, ddq.p, ddn.p, dsc.p, it, stream)));
-      HK((launch_pcg_dir<double, BD>(nb, dd.p, ds.p, ddn.p, dsc.p, it, stream)));
-      if ((it + 1) % 4 == 0) {
-        HK(hipMemcpyAsync(h_pinned, dsc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
-        HK(hipStreamSynchronize(stream));
-        std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
-        if (hs.done) break;
-      }
-    }
-    HK(hipMemcpyAsync(h_pinned, dsc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream));
-    HK(hipStreamSynchronize(stream));
-    std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
-    int k = hs.done ? hs.iters : mi;
-    if (iterations) *iterations = k;
-    if (residual_io) *residual_io = 0.5 * hs.dn[k & 1];
+    auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
+      return launch_pcg_dir_spmv<double, BD>(nb, dFptr.p, dFcol.p, dFval.p, ds.p, d_old, d, dq.p, ddn.p, ddq.p,
+                                             dsc.p, ctl, it, last, stream);
+    };
+    PcgRun pr;
+    HK((pcg_drive<double, BD>(stream, h_pinned, nb, db.p, dJ.p, dx.p, dr.p, ds.p, dirs, dq.p, ddn.p, ddq.p,
+                              pcg_spmv_blocks(nb), dsc.p, mi, 0, op, &pr)));
+    if (iterations) *iterations = pr.iters;
+    if (residual_io) *residual_io = 0.5 * pr.dn_final;
     return G2OHIP_OK;
   };
   if (bd == 3) CK(run(std::integral_constant<int, 3>()));

@@ -50,7 +50,15 @@ struct PrecBufs {
   DBuf<HT> Jinv;      // Cf * 36
   DBuf<HT> Dinv;      // Pf * 9, (Hll + lambda I)^-1 of the current solve
   DBuf<HT> Sdiag;     // Cf * 36 (implicit mode preconditioner)
-  DBuf<HT> x, r, d, d2, q, s;  // PCG vectors
+  DBuf<HT> x, r, d, d2, q, s;  // PCG vectors (d, d2: directions of odd / even iterations)
+};
+
+// Outcome of one PCG solve (read back from PcgScalars).
+struct PcgRun {
+  int iters = 0;
+  double dn_final = 0;  // r . M^-1 r at exit
+  double d0 = 0;        // the absolute stop threshold used
+  double dn0 = 0;       // r . M^-1 r of the right-hand side
 };
 
 struct RowPlanBufs {
@@ -93,9 +101,7 @@ struct g2ohip_ctx {
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;
   g2ohip::DBuf<int> d_err;
-  g2ohip::DBuf<unsigned> d_bar;
-  g2ohip::DBuf<double> d_gpart;
-  int pcg_grid[2] = {-1, -1};  // persistent PCG grid per precision (0: unavailable)
+  int pcg_hint[2] = {0, 0};  // iterations of the previous main / correction solve (first host poll)
   g2ohip::DBuf<g2ohip::PcgScalars> d_sc;
   g2ohip::PrecBufs<float> f32;
   g2ohip::PrecBufs<double> f64;
@@ -142,8 +148,8 @@ struct g2ohip_ctx {
   template <typename HT>
   int solve_impl(int* ok);
   template <typename HT>
-  int run_pcg(g2ohip::PrecBufs<HT>& B, const double* rhs, bool use_memory, bool implicit, int* iters,
-              double* dn_final);
+  int run_pcg(g2ohip::PrecBufs<HT>& B, const double* rhs, const g2ohip::PcgControl& ctl, bool implicit, int max_iter,
+              int hint, g2ohip::PcgRun* out);
   int read_scalar(const double* dptr, double* out);
   int read_err(int* out);
 };

@@ -37,10 +37,19 @@ struct DevRowPlan {
 struct PcgScalars {
   double dn[2];
   double d0;
+  double dn0;
   int done;
   int iters;
   int err;
   int pad;
+};
+
+// Stop rule of one solve: d0 = tol * dn0 floored by resid_mem (the previous
+// solve's residual memory, ignored when <= 0), or exactly d0_abs when >= 0.
+struct PcgControl {
+  double tol;
+  double resid_mem;
+  double d0_abs;
 };
 
 // ---- state / residuals
@@ -99,33 +108,26 @@ hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, i
 template <typename HT, int BD>
 hipError_t launch_jacobi_damped(int nb, const HT* F_val, const int* F_diag, double lambda, HT* Jinv, int* err,
                                 hipStream_t st);
+// Two launches per iteration: pcg_dir_spmv (or pcg_dir + the matrix-free
+// operator) then pcg_update. Partials: part_rs has pcg_row_blocks(nb)
+// entries, part_dq pcg_spmv_blocks(nb) (explicit) or pcg_row_blocks(Cf)
+// (implicit). d and d_old swap every iteration.
+int pcg_row_blocks(int nb);
+int pcg_spmv_blocks(int nb);
 template <typename HT, int BD>
-hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* d, double* part,
+hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* s, double* part_rs,
                            hipStream_t st);
-hipError_t launch_pcg_start(int nb, const double* part, double tol, double resid_mem, PcgScalars* sc,
-                            hipStream_t st);
 template <typename HT, int BD>
-hipError_t launch_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* d, HT* q,
-                       double* dq_part, PcgScalars* sc, int it, hipStream_t st);
+hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* s,
+                               const HT* d_old, HT* d, HT* q, const double* part_rs, double* part_dq,
+                               PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);
 template <typename HT, int BD>
-hipError_t launch_pcg_step(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
-                           const double* dq_part, double* dn_part, PcgScalars* sc, int it, hipStream_t st);
+hipError_t launch_pcg_dir(int nb, const HT* s, const HT* d_old, HT* d, const double* part_rs, PcgScalars* sc,
+                          const PcgControl& ctl, int it, int last, hipStream_t st);
 template <typename HT, int BD>
-hipError_t launch_pcg_dir(int nb, HT* d, const HT* s, const double* dn_part, PcgScalars* sc, int it,
-                          hipStream_t st);
-// Whole PCG solve in one cooperative launch (explicit Hschur, BD = 6). The
-// workgroups own block rows and meet at two grid barriers per iteration; the
-// convergence test and all dot products are evaluated identically by every
-// workgroup. rhs in f64; x, r, d0, d1, q, s in HT; part: 2 * grid doubles;
-// bar: 2 unsigned (zeroed by the launcher). The launch fails (and the caller
-// uses the kernel-per-step loop) if the grid cannot be co-resident.
-template <typename HT>
-hipError_t launch_pcg_persistent(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* Jinv,
-                                 const double* rhs, HT* x, HT* r, HT* d0, HT* d1, HT* q, HT* s, double* part,
-                                 unsigned* bar, PcgScalars* sc, double tol, double resid_mem, int max_iter,
-                                 int grid, hipStream_t st);
-template <typename HT>
-int pcg_persistent_max_grid(int device);
+hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
+                             const double* part_dq, int n_dq, double* part_rs, const PcgScalars* sc, int it,
+                             hipStream_t st);
 template <typename HT>
 hipError_t launch_to_double(int n, const HT* src, double* dst, double scale_add, hipStream_t st);
 hipError_t launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t st);

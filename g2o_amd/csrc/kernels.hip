@@ -603,20 +603,13 @@ template hipError_t launch_point_dinv<double>(const DevGraph&, const double*, do
 // One pass of the matrix-free reduced operator over the linearisation chunks:
 // with p != null: v_l = sum Hpl^T p_c, y = Dinv v_l, part_c += Hpl y;
 // with p == null: y = Dinv b_l (the Schur right-hand side).
-// sc != null: PCG iteration `it` (skipped once converged; the convergence
-// test of linear_solver_pcg.hpp:141 runs here).
+// sc != null: PCG iteration `it` (skipped once the solve is over).
 template <typename HT>
 __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const HT* __restrict__ Hpl,
                                                               const HT* __restrict__ Dinv, const HT* __restrict__ p,
                                                               const double* __restrict__ b,
                                                               double* __restrict__ cam_part, PcgScalars* sc, int it) {
-  if (sc) {
-    if (sc->done) return;
-    if (sc->dn[it & 1] <= sc->d0) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) { sc->done = 1; sc->iters = it; }
-      return;
-    }
-  }
+  if (sc && sc->done) return;
   __shared__ int s_cam[LIN_CAMS];
   __shared__ double s_cacc[LIN_CAMS * 6];
   __shared__ double s_v[LIN_PTS * 3];
@@ -689,39 +682,42 @@ template hipError_t launch_implicit_pass<float>(const DevGraph&, const float*, c
 template hipError_t launch_implicit_pass<double>(const DevGraph&, const double*, const double*, const double*,
                                                  const double*, double*, const PcgScalars*, int, hipStream_t);
 
-// q = (Hpp + lambda I) p - sum of chunk partials; dq_part[c] = q_c . p_c
+// q = (Hpp + lambda I) p - sum of chunk partials; dq_part[block] = sum of q_c . p_c
 template <typename HT>
-__global__ void k_implicit_reduce(DevGraph g, const double* __restrict__ cam_part, const double* __restrict__ Hpp,
-                                  double lambda, const HT* __restrict__ p, HT* __restrict__ q,
-                                  double* __restrict__ dq_part, PcgScalars* sc) {
+__global__ __launch_bounds__(64) void k_implicit_reduce(DevGraph g, const double* __restrict__ cam_part,
+                                                        const double* __restrict__ Hpp, double lambda,
+                                                        const HT* __restrict__ p, HT* __restrict__ q,
+                                                        double* __restrict__ dq_part, const PcgScalars* sc) {
   if (sc->done) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.Cf) return;
-  double pc[6], out[6];
-  for (int r = 0; r < 6; ++r) pc[r] = static_cast<double>(p[6 * static_cast<size_t>(c) + r]);
-  for (int r = 0; r < 6; ++r) {
-    double v = lambda * pc[r];
-    for (int k = 0; k < 6; ++k) v += Hpp[36 * static_cast<size_t>(c) + 6 * r + k] * pc[k];
-    out[r] = v;
-  }
-  for (int q2 = g.camp_ptr[c]; q2 < g.camp_ptr[c + 1]; ++q2) {
-    const double* pp = cam_part + 6 * static_cast<size_t>(g.camp_idx[q2]);
-    for (int r = 0; r < 6; ++r) out[r] -= pp[r];
-  }
+  const int c = blockIdx.x * 64 + threadIdx.x;
   double dq = 0;
-  for (int r = 0; r < 6; ++r) {
-    HT v = static_cast<HT>(out[r]);
-    q[6 * static_cast<size_t>(c) + r] = v;
-    dq += static_cast<double>(v) * pc[r];
+  if (c < g.Cf) {
+    double pc[6], out[6];
+    for (int r = 0; r < 6; ++r) pc[r] = static_cast<double>(p[6 * static_cast<size_t>(c) + r]);
+    for (int r = 0; r < 6; ++r) {
+      double v = lambda * pc[r];
+      for (int k = 0; k < 6; ++k) v += Hpp[36 * static_cast<size_t>(c) + 6 * r + k] * pc[k];
+      out[r] = v;
+    }
+    for (int q2 = g.camp_ptr[c]; q2 < g.camp_ptr[c + 1]; ++q2) {
+      const double* pp = cam_part + 6 * static_cast<size_t>(g.camp_idx[q2]);
+      for (int r = 0; r < 6; ++r) out[r] -= pp[r];
+    }
+    for (int r = 0; r < 6; ++r) {
+      HT v = static_cast<HT>(out[r]);
+      q[6 * static_cast<size_t>(c) + r] = v;
+      dq += static_cast<double>(v) * pc[r];
+    }
   }
-  dq_part[c] = dq;
+  dq = wave_sum(dq);
+  if (threadIdx.x == 0) dq_part[blockIdx.x] = dq;
 }
 
 template <typename HT>
 hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, const double* Hpp, double lambda,
                                   const HT* p, HT* q, double* dq_part, PcgScalars* sc, int it, hipStream_t st) {
   (void)it;
-  hipLaunchKernelGGL(k_implicit_reduce<HT>, dim3(nblocks(g.Cf, 256)), dim3(256), 0, st, g, cam_part, Hpp, lambda, p,
+  hipLaunchKernelGGL(k_implicit_reduce<HT>, dim3(nblocks(g.Cf, 64)), dim3(64), 0, st, g, cam_part, Hpp, lambda, p,
                      q, dq_part, sc);
   return hipGetLastError();
 }
@@ -972,73 +968,113 @@ template hipError_t launch_jacobi_damped<float, 6>(int, const float*, const int*
 template hipError_t launch_jacobi_damped<double, 6>(int, const double*, const int*, double, double*, int*,
                                                     hipStream_t);
 
-template <typename HT, int BD>
-__global__ void k_pcg_init(int nb, const double* __restrict__ b, const HT* __restrict__ J, HT* __restrict__ x,
-                           HT* __restrict__ r, HT* __restrict__ d, double* __restrict__ part) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  HT rv[BD];
-  for (int k = 0; k < BD; ++k) {
-    rv[k] = static_cast<HT>(b[BD * static_cast<size_t>(i) + k]);
-    r[BD * static_cast<size_t>(i) + k] = rv[k];
-    x[BD * static_cast<size_t>(i) + k] = HT(0);
+// ----------------------------------------------------- block-Jacobi PCG
+// LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) as two launches per
+// iteration. Direction update, convergence test and SpMV are fused: the new
+// direction d = s + beta d_old is formed on the fly from s and the previous
+// direction (double-buffered by iteration parity), so no kernel has to wait
+// for a separate direction pass. Dot products are per-workgroup FP64 partials
+// summed with compensation, identically by every wavefront that needs them.
+
+// Compensated sum of part[0..n); every lane of the calling wavefront returns
+// the same value (pairs of lanes combine with the same commutative ops).
+__device__ double wave_sum_array(const double* __restrict__ part, int n) {
+  const int lane = threadIdx.x & 63;
+  double s = 0, c = 0;
+  for (int i = lane; i < n; i += 64) neumaier_add(s, c, part[i]);
+  for (int off = 32; off > 0; off >>= 1) {
+    const double os = __shfl_xor(s, off, 64), oc = __shfl_xor(c, off, 64);
+    const double t = s + os;
+    const double e = (fabs(s) >= fabs(os)) ? ((s - t) + os) : ((os - t) + s);
+    c = (c + oc) + e;
+    s = t;
   }
+  return s + c;
+}
+
+template <typename HT>
+__device__ __forceinline__ HT pcg_dir_val(HT s, HT d_old, HT beta, bool first) {
+  return first ? s : fma(beta, d_old, s);
+}
+
+// x = 0, r = b, s = M^-1 r, part[block] = r . s   (linear_solver_pcg.hpp:120-130)
+template <typename HT, int BD>
+__global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restrict__ b, const HT* __restrict__ J,
+                                                 HT* __restrict__ x, HT* __restrict__ r, HT* __restrict__ s,
+                                                 double* __restrict__ part) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
   double dot = 0;
-  const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
-  for (int k = 0; k < BD; ++k) {
-    HT v = 0;
-    for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
-    d[BD * static_cast<size_t>(i) + k] = v;
-    dot += static_cast<double>(rv[k]) * static_cast<double>(v);
+  if (i < nb) {
+    HT rv[BD];
+    for (int k = 0; k < BD; ++k) {
+      rv[k] = static_cast<HT>(b[BD * static_cast<size_t>(i) + k]);
+      r[BD * static_cast<size_t>(i) + k] = rv[k];
+      x[BD * static_cast<size_t>(i) + k] = HT(0);
+    }
+    const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
+    for (int k = 0; k < BD; ++k) {
+      HT v = 0;
+      for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+      s[BD * static_cast<size_t>(i) + k] = v;
+      dot += static_cast<double>(rv[k]) * static_cast<double>(v);
+    }
   }
-  part[i] = dot;
+  dot = wave_sum(dot);
+  if (threadIdx.x == 0) part[blockIdx.x] = dot;
 }
 
+// Head of iteration `it`: dn = r . s from the previous update's partials; the
+// stop test `dn <= d0` of linear_solver_pcg.hpp:141 (d0 fixed at it == 0 as
+// tol * dn0 floored by the residual memory, or d0_abs when >= 0); then
+// beta = dn / dn_old. Returns false (and records the iteration count) when
+// the solve is over. Every workgroup evaluates this identically.
+__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, double tol, double resid_mem,
+                         double d0_abs, int it, int last, double* beta) {
+  if (sc->done) return false;
+  const double dn = wave_sum_array(part_rs, n_rs);
+  double d0;
+  if (it == 0) {
+    if (d0_abs >= 0.0) {
+      d0 = d0_abs;
+    } else {
+      d0 = tol * dn;
+      if (resid_mem > 0.0 && resid_mem > d0) d0 = resid_mem;
+    }
+  } else {
+    d0 = sc->d0;
+  }
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (first) {
+    if (it == 0) { sc->d0 = d0; sc->dn0 = dn; }
+    sc->dn[it & 1] = dn;  // the other slot still holds dn_old for this launch
+  }
+  if (dn <= d0 || last) {
+    if (first) { sc->done = 1; sc->iters = it; }
+    return false;
+  }
+  *beta = it == 0 ? 0.0 : dn / sc->dn[(it + 1) & 1];
+  return true;
+}
+
+// Explicit operator: d = s + beta d_old for the owned rows and q = A d over
+// the full block rows (both triangles). One wavefront per block row; lanes
+// split into 64 / BD groups striding over the row's blocks, lane k of a group
+// owns output row k. part_dq[block] = sum over its rows of d . q.
 template <typename HT, int BD>
-hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* d, double* part,
-                           hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_init<HT, BD>), dim3(nblocks(nb, 256)), dim3(256), 0, st, nb, b, Jinv, x, r, d, part);
-  return hipGetLastError();
-}
-
-// dn = r . M^-1 r; d0 = tol * dn, floored by the previous solve's residual
-// memory (linear_solver_pcg.hpp:128-136).
-__global__ __launch_bounds__(256) void k_pcg_start(int nb, const double* __restrict__ part, double tol,
-                                                   double resid_mem, PcgScalars* sc) {
-  __shared__ double s_red[512];
-  double dn = block_sum_array(part, nb, s_red);
-  if (threadIdx.x == 0) {
-    double d0 = tol * dn;
-    if (resid_mem > 0.0 && resid_mem > d0) d0 = resid_mem;
-    sc->dn[0] = dn;
-    sc->dn[1] = dn;
-    sc->d0 = d0;
-    sc->done = 0;
-    sc->iters = -1;
-  }
-}
-
-hipError_t launch_pcg_start(int nb, const double* part, double tol, double resid_mem, PcgScalars* sc,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, st, nb, part, tol, resid_mem, sc);
-  return hipGetLastError();
-}
-
-// q = A d over the full block rows: one wavefront per block row, lanes split
-// into 64 / BD groups that stride over the row's blocks; lane k of a group
-// owns output row k.
-template <typename HT, int BD>
-__global__ __launch_bounds__(256) void k_spmv(int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col,
-                                              const HT* __restrict__ F_val, const HT* __restrict__ d,
-                                              HT* __restrict__ q, double* __restrict__ dq_part, PcgScalars* sc,
-                                              int it) {
-  if (sc->done) return;
-  if (sc->dn[it & 1] <= sc->d0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { sc->done = 1; sc->iters = it; }
-    return;
-  }
+__global__ __launch_bounds__(256) void k_pcg_dir_spmv(int nb, const int* __restrict__ F_ptr,
+                                                      const int* __restrict__ F_col, const HT* __restrict__ F_val,
+                                                      const HT* __restrict__ s, const HT* __restrict__ d_old,
+                                                      HT* __restrict__ d, HT* __restrict__ q,
+                                                      const double* __restrict__ part_rs, int n_rs,
+                                                      double* __restrict__ part_dq, PcgScalars* sc, double tol,
+                                                      double resid_mem, double d0_abs, int it, int last) {
+  double beta_d;
+  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta_d)) return;
+  const bool firstit = it == 0;
+  const HT beta = static_cast<HT>(beta_d);
   constexpr int G = 64 / BD;
   __shared__ HT s_part[4][64];
+  __shared__ double s_dot[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int row = blockIdx.x * 4 + wv;
   const int grp = lane / BD, k = lane % BD;
@@ -1046,298 +1082,128 @@ __global__ __launch_bounds__(256) void k_spmv(int nb, const int* __restrict__ F_
   if (row < nb && grp < G) {
     for (int p = F_ptr[row] + grp; p < F_ptr[row + 1]; p += G) {
       const HT* V = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
-      const HT* dj = d + BD * static_cast<size_t>(F_col[p]);
-      for (int c = 0; c < BD; ++c) acc += V[c] * dj[c];
+      const size_t j = BD * static_cast<size_t>(F_col[p]);
+      for (int c = 0; c < BD; ++c) acc += V[c] * pcg_dir_val(s[j + c], firstit ? HT(0) : d_old[j + c], beta, firstit);
     }
   }
   s_part[wv][lane] = acc;
   __syncthreads();
+  double dot = 0;
   if (row < nb && lane < BD) {
     HT v = 0;
     for (int g2 = 0; g2 < G; ++g2) v += s_part[wv][g2 * BD + lane];
-    q[BD * static_cast<size_t>(row) + lane] = v;
-    s_part[wv][lane] = v;  // each lane only rewrites its own entry
+    const size_t o = BD * static_cast<size_t>(row) + lane;
+    const HT dv = pcg_dir_val(s[o], firstit ? HT(0) : d_old[o], beta, firstit);
+    q[o] = v;
+    d[o] = dv;
+    dot = static_cast<double>(v) * static_cast<double>(dv);
   }
+  dot = wave_sum(dot);
+  if (lane == 0) s_dot[wv] = dot;
   __syncthreads();
-  if (row < nb && lane == 0) {
-    double dot = 0;
-    for (int c = 0; c < BD; ++c)
-      dot += static_cast<double>(s_part[wv][c]) * static_cast<double>(d[BD * static_cast<size_t>(row) + c]);
-    dq_part[row] = dot;
-  }
+  if (threadIdx.x == 0) part_dq[blockIdx.x] = ((s_dot[0] + s_dot[1]) + s_dot[2]) + s_dot[3];
 }
 
+// Direction only (matrix-free operator): d = s + beta d_old, thread per row.
 template <typename HT, int BD>
-hipError_t launch_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* d, HT* q,
-                       double* dq_part, PcgScalars* sc, int it, hipStream_t st) {
-  hipLaunchKernelGGL((k_spmv<HT, BD>), dim3(nblocks(nb, 4)), dim3(256), 0, st, nb, F_ptr, F_col, F_val, d, q,
-                     dq_part, sc, it);
-  return hipGetLastError();
-}
-
-// a = dn / d.q; x += a d; r -= a q; s = M^-1 r; dn_part = r . s
-template <typename HT, int BD>
-__global__ __launch_bounds__(256) void k_pcg_step(int nb, const HT* __restrict__ J, HT* __restrict__ x,
-                                                  HT* __restrict__ r, const HT* __restrict__ d,
-                                                  const HT* __restrict__ q, HT* __restrict__ s,
-                                                  const double* __restrict__ dq_part, double* __restrict__ dn_part,
-                                                  PcgScalars* sc, int it) {
-  if (sc->done) return;
-  __shared__ double s_red[512];
-  const double dq = block_sum_array(dq_part, nb, s_red);
-  const double a = sc->dn[it & 1] / dq;
-  const HT ah = static_cast<HT>(a);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_pcg_dir(int nb, const HT* __restrict__ s, const HT* __restrict__ d_old,
+                                                HT* __restrict__ d, const double* __restrict__ part_rs, int n_rs,
+                                                PcgScalars* sc, double tol, double resid_mem, double d0_abs, int it,
+                                                int last) {
+  double beta_d;
+  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta_d)) return;
+  const HT beta = static_cast<HT>(beta_d);
+  const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= nb) return;
-  HT rv[BD];
   for (int k = 0; k < BD; ++k) {
     const size_t o = BD * static_cast<size_t>(i) + k;
-    x[o] += ah * d[o];
-    rv[k] = r[o] - ah * q[o];
-    r[o] = rv[k];
+    d[o] = pcg_dir_val(s[o], it == 0 ? HT(0) : d_old[o], beta, it == 0);
   }
-  const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
-  double dot = 0;
-  for (int k = 0; k < BD; ++k) {
-    HT v = 0;
-    for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
-    s[BD * static_cast<size_t>(i) + k] = v;
-    dot += static_cast<double>(rv[k]) * static_cast<double>(v);
-  }
-  dn_part[i] = dot;
 }
 
+// a = dn / d.q; x += a d; r -= a q; s = M^-1 r; part_rs[block] = r . s
 template <typename HT, int BD>
-hipError_t launch_pcg_step(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
-                           const double* dq_part, double* dn_part, PcgScalars* sc, int it, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_step<HT, BD>), dim3(nblocks(nb, 256)), dim3(256), 0, st, nb, Jinv, x, r, d, q, s,
-                     dq_part, dn_part, sc, it);
+__global__ __launch_bounds__(64) void k_pcg_update(int nb, const HT* __restrict__ J, HT* __restrict__ x,
+                                                   HT* __restrict__ r, const HT* __restrict__ d,
+                                                   const HT* __restrict__ q, HT* __restrict__ s,
+                                                   const double* __restrict__ part_dq, int n_dq,
+                                                   double* __restrict__ part_rs, const PcgScalars* sc, int it) {
+  if (sc->done) return;
+  const double dq = wave_sum_array(part_dq, n_dq);
+  const HT a = static_cast<HT>(sc->dn[it & 1] / dq);
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  double dot = 0;
+  if (i < nb) {
+    HT rv[BD];
+    for (int k = 0; k < BD; ++k) {
+      const size_t o = BD * static_cast<size_t>(i) + k;
+      x[o] += a * d[o];
+      rv[k] = r[o] - a * q[o];
+      r[o] = rv[k];
+    }
+    const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
+    for (int k = 0; k < BD; ++k) {
+      HT v = 0;
+      for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+      s[BD * static_cast<size_t>(i) + k] = v;
+      dot += static_cast<double>(rv[k]) * static_cast<double>(v);
+    }
+  }
+  dot = wave_sum(dot);
+  if (threadIdx.x == 0) part_rs[blockIdx.x] = dot;
+}
+
+int pcg_row_blocks(int nb) { return nblocks(nb, 64); }
+int pcg_spmv_blocks(int nb) { return nblocks(nb, 4); }
+
+template <typename HT, int BD>
+hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* s, double* part_rs,
+                           hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_init<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs);
   return hipGetLastError();
 }
 
-// b = dn_new / dn; d = s + b d
 template <typename HT, int BD>
-__global__ __launch_bounds__(256) void k_pcg_dir(int nb, HT* __restrict__ d, const HT* __restrict__ s,
-                                                 const double* __restrict__ dn_part, PcgScalars* sc, int it) {
-  if (sc->done) return;
-  __shared__ double s_red[512];
-  const double dn_new = block_sum_array(dn_part, nb, s_red);
-  const double be = dn_new / sc->dn[it & 1];
-  const HT bh = static_cast<HT>(be);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->dn[(it + 1) & 1] = dn_new;
-  if (i >= nb * BD) return;
-  d[i] = s[i] + bh * d[i];
+hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* s,
+                               const HT* d_old, HT* d, HT* q, const double* part_rs, double* part_dq,
+                               PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_dir_spmv<HT, BD>), dim3(nblocks(nb, 4)), dim3(256), 0, st, nb, F_ptr, F_col, F_val, s,
+                     d_old, d, q, part_rs, nblocks(nb, 64), part_dq, sc, ctl.tol, ctl.resid_mem, ctl.d0_abs, it,
+                     last);
+  return hipGetLastError();
 }
 
 template <typename HT, int BD>
-hipError_t launch_pcg_dir(int nb, HT* d, const HT* s, const double* dn_part, PcgScalars* sc, int it,
-                          hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_dir<HT, BD>), dim3(nblocks(static_cast<long long>(nb) * BD, 256)), dim3(256), 0, st, nb,
-                     d, s, dn_part, sc, it);
+hipError_t launch_pcg_dir(int nb, const HT* s, const HT* d_old, HT* d, const double* part_rs, PcgScalars* sc,
+                          const PcgControl& ctl, int it, int last, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_dir<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, s, d_old, d, part_rs,
+                     nblocks(nb, 64), sc, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
+  return hipGetLastError();
+}
+
+template <typename HT, int BD>
+hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
+                             const double* part_dq, int n_dq, double* part_rs, const PcgScalars* sc, int it,
+                             hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_update<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, Jinv, x, r, d, q, s,
+                     part_dq, n_dq, part_rs, sc, it);
   return hipGetLastError();
 }
 
 #define G2OHIP_PCG_INST(HT, BD)                                                                                     \
   template hipError_t launch_jacobi<HT, BD>(int, const HT*, const int*, HT*, int*, hipStream_t);                   \
   template hipError_t launch_pcg_init<HT, BD>(int, const double*, const HT*, HT*, HT*, HT*, double*, hipStream_t); \
-  template hipError_t launch_spmv<HT, BD>(int, const int*, const int*, const HT*, const HT*, HT*, double*,         \
-                                          PcgScalars*, int, hipStream_t);                                          \
-  template hipError_t launch_pcg_step<HT, BD>(int, const HT*, HT*, HT*, const HT*, const HT*, HT*, const double*,  \
-                                              double*, PcgScalars*, int, hipStream_t);                             \
-  template hipError_t launch_pcg_dir<HT, BD>(int, HT*, const HT*, const double*, PcgScalars*, int, hipStream_t);
+  template hipError_t launch_pcg_dir_spmv<HT, BD>(int, const int*, const int*, const HT*, const HT*, const HT*,   \
+                                                  HT*, HT*, const double*, double*, PcgScalars*,                 \
+                                                  const PcgControl&, int, int, hipStream_t);                     \
+  template hipError_t launch_pcg_dir<HT, BD>(int, const HT*, const HT*, HT*, const double*, PcgScalars*,          \
+                                             const PcgControl&, int, int, hipStream_t);                          \
+  template hipError_t launch_pcg_update<HT, BD>(int, const HT*, HT*, HT*, const HT*, const HT*, HT*,              \
+                                                const double*, int, double*, const PcgScalars*, int, hipStream_t);
 G2OHIP_PCG_INST(float, 6)
 G2OHIP_PCG_INST(double, 6)
 G2OHIP_PCG_INST(double, 3)
 #undef G2OHIP_PCG_INST
-
-// ----------------------------------------------- persistent (one launch) PCG
-__global__ void k_zero_u32(unsigned* p, int n) {
-  if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0u;
-}
-
-// Grid barrier: one arrival counter and one generation word. The last
-// workgroup to arrive resets the counter and bumps the generation; the others
-// poll it with a bounded spin (a stuck barrier sets sc->err and every
-// workgroup leaves the solve instead of waiting forever).
-__device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, PcgScalars* sc) {
-  __shared__ int s_ok;
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's global stores have reached L2
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    s_ok = 1;
-    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish the workgroup's stores
-    const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nblocks - 1) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&bar[1], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      // relaxed polling: no cache invalidation per poll; the acquire fence
-      // below runs once the generation moved
-      long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (++spins > (1L << 24)) {
-          __hip_atomic_store(&sc->err, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          s_ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-  return s_ok != 0;
-}
-
-template <typename HT>
-__global__ __launch_bounds__(256) void k_pcg_persistent(int nb, const int* __restrict__ F_ptr,
-                                                        const int* __restrict__ F_col, const HT* __restrict__ F_val,
-                                                        const HT* __restrict__ J, const double* __restrict__ rhs,
-                                                        HT* __restrict__ x, HT* __restrict__ r, HT* d0, HT* d1,
-                                                        HT* __restrict__ q, HT* __restrict__ s, double* part,
-                                                        unsigned* bar, PcgScalars* sc, double tol, double resid_mem,
-                                                        int max_iter) {
-  constexpr int BD = 6, G = 64 / BD;
-  __shared__ double s_red[512];
-  __shared__ HT s_part[4][64];
-  __shared__ double s_wsum[4];
-  const unsigned nblk = gridDim.x;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  double* part_a = part;         // d . q per workgroup
-  double* part_b = part + nblk;  // r . s per workgroup
-  // ---- x = 0, r = b, d = M^-1 r, dn = r . d   (linear_solver_pcg.hpp:120-127)
-  double loc = 0;
-  for (int i = blockIdx.x * 256 + tid; i < nb; i += nblk * 256) {
-    HT rv[BD];
-    for (int k = 0; k < BD; ++k) {
-      rv[k] = static_cast<HT>(rhs[BD * static_cast<size_t>(i) + k]);
-      r[BD * static_cast<size_t>(i) + k] = rv[k];
-      x[BD * static_cast<size_t>(i) + k] = HT(0);
-    }
-    const HT* Ji = J + static_cast<size_t>(i) * 36;
-    for (int k = 0; k < BD; ++k) {
-      HT v = 0;
-      for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
-      d0[BD * static_cast<size_t>(i) + k] = v;
-      loc += static_cast<double>(rv[k]) * static_cast<double>(v);
-    }
-  }
-  loc = block_sum(loc, s_wsum);
-  if (tid == 0) part_b[blockIdx.x] = loc;
-  if (!grid_barrier(bar, nblk, sc)) return;
-  double dn = block_sum_array(part_b, nblk, s_red);
-  double dlim = tol * dn;
-  if (resid_mem > 0.0 && resid_mem > dlim) dlim = resid_mem;
-  double beta = 0.0;
-  int it = 0;
-  for (; it < max_iter; ++it) {
-    if (dn <= dlim) break;  // linear_solver_pcg.hpp:141
-    HT* dold = (it & 1) ? d1 : d0;
-    HT* dnew = (it & 1) ? d0 : d1;
-    const HT bh = static_cast<HT>(beta);
-    // ---- phase A: d = s + beta d (formed on the fly from the previous
-    // iteration's s and d, both complete), q = A d for the owned block rows;
-    // one wavefront per block row, 64 / 6 lane groups over the row's blocks.
-    double dq_loc = 0;
-    for (int row0 = blockIdx.x * 4; row0 < nb; row0 += nblk * 4) {
-      const int row = row0 + wv;
-      const int grp = lane / BD, k = lane % BD;
-      HT acc = 0;
-      if (row < nb && grp < G) {
-        for (int p = F_ptr[row] + grp; p < F_ptr[row + 1]; p += G) {
-          const HT* V = F_val + static_cast<size_t>(p) * 36 + BD * k;
-          const size_t cj = BD * static_cast<size_t>(F_col[p]);
-          for (int c = 0; c < BD; ++c) {
-            const HT dv = (it == 0) ? dold[cj + c] : s[cj + c] + bh * dold[cj + c];
-            acc += V[c] * dv;
-          }
-        }
-      }
-      s_part[wv][lane] = acc;
-      __syncthreads();
-      if (row < nb && lane < BD) {
-        HT v = 0;
-        for (int g2 = 0; g2 < G; ++g2) v += s_part[wv][g2 * BD + lane];
-        const size_t o = BD * static_cast<size_t>(row) + lane;
-        const HT dv = (it == 0) ? dold[o] : s[o] + bh * dold[o];
-        dnew[o] = dv;
-        q[o] = v;
-        dq_loc += static_cast<double>(v) * static_cast<double>(dv);
-      }
-      __syncthreads();
-    }
-    dq_loc = block_sum(dq_loc, s_wsum);
-    if (tid == 0) part_a[blockIdx.x] = dq_loc;
-    if (!grid_barrier(bar, nblk, sc)) return;
-    // ---- phase B: a = dn / d.q; x += a d; r -= a q; s = M^-1 r; dn' = r . s
-    const double dq = block_sum_array(part_a, nblk, s_red);
-    const double alpha = dn / dq;
-    const HT ah = static_cast<HT>(alpha);
-    double dn_loc = 0;
-    for (int i = blockIdx.x * 256 + tid; i < nb; i += nblk * 256) {
-      HT rv[BD];
-      for (int k = 0; k < BD; ++k) {
-        const size_t o = BD * static_cast<size_t>(i) + k;
-        x[o] += ah * dnew[o];
-        rv[k] = r[o] - ah * q[o];
-        r[o] = rv[k];
-      }
-      const HT* Ji = J + static_cast<size_t>(i) * 36;
-      for (int k = 0; k < BD; ++k) {
-        HT v = 0;
-        for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
-        s[BD * static_cast<size_t>(i) + k] = v;
-        dn_loc += static_cast<double>(rv[k]) * static_cast<double>(v);
-      }
-    }
-    dn_loc = block_sum(dn_loc, s_wsum);
-    if (tid == 0) part_b[blockIdx.x] = dn_loc;
-    if (!grid_barrier(bar, nblk, sc)) return;
-    const double dn_new = block_sum_array(part_b, nblk, s_red);
-    beta = dn_new / dn;
-    dn = dn_new;
-  }
-  if (blockIdx.x == 0 && tid == 0) {
-    sc->dn[it & 1] = dn;
-    sc->d0 = dlim;
-    sc->iters = it;
-    sc->done = 1;
-  }
-}
-
-template <typename HT>
-int pcg_persistent_max_grid(int device) {
-  int ncu = 0, per = 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pcg_persistent<HT>, 256, 0) != hipSuccess) return 0;
-  return ncu * per;
-}
-template int pcg_persistent_max_grid<float>(int);
-template int pcg_persistent_max_grid<double>(int);
-
-template <typename HT>
-hipError_t launch_pcg_persistent(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* Jinv,
-                                 const double* rhs, HT* x, HT* r, HT* d0, HT* d1, HT* q, HT* s, double* part,
-                                 unsigned* bar, PcgScalars* sc, double tol, double resid_mem, int max_iter, int grid,
-                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, bar, 2);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  void* args[] = {&nb, &F_ptr, &F_col, &F_val, &Jinv, &rhs, &x, &r, &d0, &d1, &q, &s, &part, &bar, &sc, &tol,
-                  &resid_mem, &max_iter};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<HT>), dim3(grid), dim3(256),
-                                    args, 0, st);
-}
-template hipError_t launch_pcg_persistent<float>(int, const int*, const int*, const float*, const float*,
-                                                 const double*, float*, float*, float*, float*, float*, float*,
-                                                 double*, unsigned*, PcgScalars*, double, double, int, int,
-                                                 hipStream_t);
-template hipError_t launch_pcg_persistent<double>(int, const int*, const int*, const double*, const double*,
-                                                  const double*, double*, double*, double*, double*, double*,
-                                                  double*, double*, unsigned*, PcgScalars*, double, double, int, int,
-                                                  hipStream_t);
 
 template <typename HT>
 __global__ void k_to_double(int n, const HT* __restrict__ src, double* __restrict__ dst, double scale_add) {
