@@ -90,7 +90,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cams", type=int, default=1500)
     ap.add_argument("--points", type=int, default=500_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies (out = -in, fp32) of MIB MiB each: the known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters, then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the solver path is single-GPU (no multi-GPU run in this build)",
               file=sys.stderr)

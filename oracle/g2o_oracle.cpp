@@ -808,6 +808,155 @@ double ba_scale(const BA& p) {
   return s;
 }
 
+
+// ------------------------------------------- exact second derivatives (adjoint)
+// Hyper-dual numbers x = a + b e1 + sum_j (c_j e2_j + d_j e1 e2_j) with
+// e1^2 = e2_j e2_k = 0: one evaluation of a function gives its value, the
+// derivative along direction 1 (b), the gradient (c) and the row of mixed
+// second derivatives d^2 f / dv_1 dv_j (d) exactly, without truncation error.
+// The energy of one projection edge is differentiated this way (12 variables:
+// the left exp-map update [omega; upsilon] of the camera, the point update,
+// the measurement and the information weight), independently of the closed
+// form the HIP kernels use.
+constexpr int HDN = 12;
+struct HD {
+  double a = 0, b = 0;
+  double c[HDN] = {}, d[HDN] = {};
+};
+inline HD hd_const(double v) { HD r; r.a = v; return r; }
+inline HD operator+(const HD& x, const HD& y) {
+  HD r; r.a = x.a + y.a; r.b = x.b + y.b;
+  for (int j = 0; j < HDN; ++j) { r.c[j] = x.c[j] + y.c[j]; r.d[j] = x.d[j] + y.d[j]; }
+  return r;
+}
+inline HD operator-(const HD& x, const HD& y) {
+  HD r; r.a = x.a - y.a; r.b = x.b - y.b;
+  for (int j = 0; j < HDN; ++j) { r.c[j] = x.c[j] - y.c[j]; r.d[j] = x.d[j] - y.d[j]; }
+  return r;
+}
+inline HD operator*(const HD& x, const HD& y) {
+  HD r; r.a = x.a * y.a; r.b = x.a * y.b + x.b * y.a;
+  for (int j = 0; j < HDN; ++j) {
+    r.c[j] = x.a * y.c[j] + x.c[j] * y.a;
+    r.d[j] = x.a * y.d[j] + x.d[j] * y.a + x.b * y.c[j] + x.c[j] * y.b;
+  }
+  return r;
+}
+inline HD operator*(double s, const HD& x) {
+  HD r; r.a = s * x.a; r.b = s * x.b;
+  for (int j = 0; j < HDN; ++j) { r.c[j] = s * x.c[j]; r.d[j] = s * x.d[j]; }
+  return r;
+}
+// f(x) for a scalar function with value f0, derivatives f1, f2 at x.a
+inline HD hd_apply(const HD& x, double f0, double f1, double f2) {
+  HD r; r.a = f0; r.b = f1 * x.b;
+  for (int j = 0; j < HDN; ++j) { r.c[j] = f1 * x.c[j]; r.d[j] = f1 * x.d[j] + f2 * x.b * x.c[j]; }
+  return r;
+}
+inline HD hd_inv(const HD& x) { return hd_apply(x, 1 / x.a, -1 / (x.a * x.a), 2 / (x.a * x.a * x.a)); }
+inline HD hd_sqrt(const HD& x) {
+  double s = std::sqrt(x.a);
+  return hd_apply(x, s, 0.5 / s, -0.25 / (s * x.a));
+}
+
+// 0.5 * rho(w' |z' - proj(exp(omega, upsilon) T (X + dX))|^2) with w' = w + dw,
+// z' = z + dz; v = [omega(3) upsilon(3) dX(3) dz(2) dw(1)] evaluated at v = 0.
+// SE3Quat::exp (se3quat.h:202-230) enters through R = I + A W + B W^2 and
+// V = I + C W + D W^2; at omega = 0 the hyper-dual theta^2 has only an e1 e2
+// part, so the series A = 1 - th2/6, B = C = 1/2 - th2/24, D = 1/6 - th2/120
+// are exact through second order.
+HD edge_energy_hd(const SE3& T, const double X[3], const double z[2], double w, const double in[4], double delta,
+                  const HD v[12]) {
+  double R0[9];
+  q2R(T.q, R0);
+  const HD* om = v;
+  const HD* up = v + 3;
+  HD th2 = om[0] * om[0] + om[1] * om[1] + om[2] * om[2];
+  HD A = hd_const(1.0) - (1.0 / 6.0) * th2;
+  HD B = hd_const(0.5) - (1.0 / 24.0) * th2;
+  HD D = hd_const(1.0 / 6.0) - (1.0 / 120.0) * th2;
+  HD W[9], W2[9];
+  HD zero = hd_const(0.0);
+  W[0] = zero; W[1] = hd_const(0.0) - om[2]; W[2] = om[1];
+  W[3] = om[2]; W[4] = zero; W[5] = hd_const(0.0) - om[0];
+  W[6] = hd_const(0.0) - om[1]; W[7] = om[0]; W[8] = zero;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) W2[3 * i + j] = W[3 * i] * W[j] + W[3 * i + 1] * W[3 + j] + W[3 * i + 2] * W[6 + j];
+  HD Y[3];
+  for (int i = 0; i < 3; ++i) {
+    Y[i] = hd_const(T.t[i]);
+    for (int k = 0; k < 3; ++k) Y[i] = Y[i] + R0[3 * i + k] * (hd_const(X[k]) + v[6 + k]);
+  }
+  HD P[3];
+  for (int i = 0; i < 3; ++i) {
+    HD acc = Y[i] + up[i];
+    for (int k = 0; k < 3; ++k) acc = acc + (A * W[3 * i + k] + B * W2[3 * i + k]) * Y[k] + (B * W[3 * i + k] + D * W2[3 * i + k]) * up[k];
+    P[i] = acc;
+  }
+  HD iz = hd_inv(P[2]);
+  HD e0 = hd_const(z[0]) + v[9] - (in[0] * (P[0] * iz) + hd_const(in[2]));
+  HD e1 = hd_const(z[1]) + v[10] - (in[1] * (P[1] * iz) + hd_const(in[3]));
+  HD s = (hd_const(w) + v[11]) * (e0 * e0 + e1 * e1);
+  HD rho = s;
+  if (delta > 0 && s.a > delta * delta) rho = (2 * delta) * hd_sqrt(s) - hd_const(delta * delta);
+  return 0.5 * rho;
+}
+
+// Gradient g[12] and Hessian H[12*12] (row-major) of edge_energy_hd at v = 0.
+void edge_energy_derivs(const SE3& T, const double X[3], const double z[2], double w, const double in[4],
+                        double delta, double* E, double* g, double* H) {
+  for (int i = 0; i < HDN; ++i) {
+    HD v[12];
+    v[i].b = 1.0;
+    for (int j = 0; j < HDN; ++j) v[j].c[j] = 1.0;
+    HD f = edge_energy_hd(T, X, z, w, in, delta, v);
+    if (i == 0) {
+      if (E) *E = f.a;
+      for (int j = 0; j < HDN; ++j) g[j] = f.c[j];
+    }
+    for (int j = 0; j < HDN; ++j) H[HDN * i + j] = f.d[j];
+  }
+}
+
+// Exact Hessian of F/2 = sum 0.5 rho_e at the current state, assembled into the
+// solver's blocks (Hpp, Hll, Hpl as in ba_build_system). With eg != null the
+// per-edge mixed derivatives d^2 E_e / d state d (z, w) are kept (27 per edge:
+// rows of the 9 state variables, columns dz0 dz1 dw).
+void ba_build_exact(BA& p, vector<double>* mixed) {
+  p.Hpp.assign(static_cast<size_t>(p.Cf) * 36, 0.0);
+  p.Hll.assign(static_cast<size_t>(p.Pf) * 9, 0.0);
+  p.Hpl.assign(p.colCam.size() * 18, 0.0);
+  if (mixed) mixed->assign(static_cast<size_t>(p.E) * 27, 0.0);
+  const int n = static_cast<int>(p.active.size());
+  vector<double> He(static_cast<size_t>(n) * 144);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int a = 0; a < n; ++a) {
+    int k = p.active[a];
+    double g[12];
+    edge_energy_derivs(p.cams[p.ecam[k]], &p.pts[3 * p.ept[k]], &p.meas[2 * k], p.w[k], &p.intr[4 * p.ecam[k]],
+                       p.huber, nullptr, g, &He[144 * static_cast<size_t>(a)]);
+  }
+  for (int a = 0; a < n; ++a) {
+    int k = p.active[a];
+    const double* H = &He[144 * static_cast<size_t>(a)];
+    int ci = p.camH[p.ecam[k]], li = p.ptH[p.ept[k]];
+    if (ci >= 0)
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) p.Hpp[36 * static_cast<size_t>(ci) + 6 * r + c] += H[12 * r + c];
+    if (li >= 0)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) p.Hll[9 * static_cast<size_t>(li) + 3 * r + c] += H[12 * (6 + r) + 6 + c];
+    if (ci >= 0 && li >= 0) {
+      double* h = &p.Hpl[static_cast<size_t>(p.edgeBlk[k]) * 18];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 3; ++c) h[3 * r + c] += H[12 * r + 6 + c];
+    }
+    if (mixed)
+      for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 3; ++c) (*mixed)[27 * static_cast<size_t>(k) + 3 * r + c] = H[12 * r + 9 + c];
+  }
+}
+
 }  // namespace
 
 // =============================================================== C interface
@@ -986,6 +1135,113 @@ int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
   p->times[5] = now_s() - tall;
   if (times) std::copy(p->times, p->times + 6, times);
   return it;
+}
+
+// Replace the measurements / information weights (same structure).
+void orc_ba_set_meas(void* h, const double* meas, const double* w) {
+  BA* p = static_cast<BA*>(h);
+  std::copy(meas, meas + 2 * static_cast<size_t>(p->E), p->meas.begin());
+  if (w) std::copy(w, w + p->E, p->w.begin());
+}
+
+// ----------------------------------------------------------------- adjoint
+// Implicit differentiation of the converged solution theta*(z, w): the
+// stationarity G(theta*, z, w) = sum_e dE_e/dtheta = 0 gives
+// dL/dz = -(dG/dz)^T lambda with H lambda = dL/dtheta, H = dG/dtheta the exact
+// Hessian (second-order projection and Huber terms included), solved by the
+// Schur complement and a dense Cholesky of the reduced camera system.
+// dL_dcam[C*6] is the loss gradient in the left exp-map coordinates
+// [omega; upsilon] of each camera, dL_dpt[P*3] in point coordinates (entries of
+// fixed vertices are ignored). Outputs dL_dmeas[E*2], dL_dw[E] (zero for
+// inactive edges) and, if not null, lambda[6Cf+3Pf]. Returns 0, or 2 if H is
+// not positive definite.
+int orc_ba_adjoint(void* h, const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw,
+                   double* lambda_out) {
+  BA* p = static_cast<BA*>(h);
+  ba_chi2(*p);
+  vector<double> mixed;
+  ba_build_exact(*p, &mixed);
+  p->b.assign(6 * static_cast<size_t>(p->Cf) + 3 * static_cast<size_t>(p->Pf), 0.0);
+  for (int c = 0; c < p->C; ++c)
+    if (p->camH[c] >= 0)
+      for (int r = 0; r < 6; ++r) p->b[6 * static_cast<size_t>(p->camH[c]) + r] = dL_dcam[6 * static_cast<size_t>(c) + r];
+  for (int i = 0; i < p->P; ++i)
+    if (p->ptH[i] >= 0)
+      for (int r = 0; r < 3; ++r)
+        p->b[6 * static_cast<size_t>(p->Cf) + 3 * static_cast<size_t>(p->ptH[i]) + r] = dL_dpt[3 * static_cast<size_t>(i) + r];
+  const int saved_kind = p->solver_kind;
+  const double saved_lambda = p->lambda;
+  p->solver_kind = 0;
+  p->lambda = 0.0;
+  bool ok = ba_solve(*p);
+  p->solver_kind = saved_kind;
+  p->lambda = saved_lambda;
+  if (!ok) return 2;
+  for (int k = 0; k < p->E; ++k) {
+    double gz[3] = {0, 0, 0};
+    int ci = p->camH[p->ecam[k]], li = p->ptH[p->ept[k]];
+    const double* m = &mixed[27 * static_cast<size_t>(k)];
+    if (ci >= 0)
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 3; ++c) gz[c] -= m[3 * r + c] * p->x[6 * static_cast<size_t>(ci) + r];
+    if (li >= 0)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          gz[c] -= m[3 * (6 + r) + c] * p->x[6 * static_cast<size_t>(p->Cf) + 3 * static_cast<size_t>(li) + r];
+    dL_dmeas[2 * static_cast<size_t>(k)] = gz[0];
+    dL_dmeas[2 * static_cast<size_t>(k) + 1] = gz[1];
+    dL_dw[k] = gz[2];
+  }
+  if (lambda_out) std::copy(p->x.begin(), p->x.end(), lambda_out);
+  return 0;
+}
+
+// Newton steps with the exact Hessian (gradient from ba_build_system's b =
+// -grad(F/2)), undamped and always applied: drives a converged LM solution to
+// the stationary point to machine precision before it is differentiated.
+// Returns 0, or 2 if the Hessian is not positive definite.
+int orc_ba_newton(void* h, int steps) {
+  BA* p = static_cast<BA*>(h);
+  const int saved_kind = p->solver_kind;
+  const double saved_lambda = p->lambda;
+  p->solver_kind = 0;
+  p->lambda = 0.0;
+  int st = 0;
+  for (int it = 0; it < steps; ++it) {
+    ba_chi2(*p);
+    ba_build_system(*p);  // b
+    ba_build_exact(*p, nullptr);
+    if (!ba_solve(*p)) { st = 2; break; }
+    ba_update(*p, p->x.data());
+  }
+  p->solver_kind = saved_kind;
+  p->lambda = saved_lambda;
+  return st;
+}
+
+// Exact-Hessian blocks at the current state (as orc_ba_linearize lays out the
+// Gauss-Newton ones).
+void orc_ba_exact_hessian(void* h, double* Hpp, double* Hll, double* Hpl_edge) {
+  BA* p = static_cast<BA*>(h);
+  ba_chi2(*p);
+  ba_build_exact(*p, nullptr);
+  if (Hpp) std::copy(p->Hpp.begin(), p->Hpp.end(), Hpp);
+  if (Hll) std::copy(p->Hll.begin(), p->Hll.end(), Hll);
+  if (Hpl_edge)
+    for (int k = 0; k < p->E; ++k) {
+      double* o = &Hpl_edge[18 * static_cast<size_t>(k)];
+      if (p->edgeBlk[k] < 0) { std::fill(o, o + 18, 0.0); continue; }
+      std::copy(&p->Hpl[18 * static_cast<size_t>(p->edgeBlk[k])], &p->Hpl[18 * static_cast<size_t>(p->edgeBlk[k])] + 18, o);
+    }
+}
+
+// Energy 0.5 rho of one edge with its gradient g[12] and Hessian H[144] in
+// the variables [omega upsilon dX dz dw] at zero.
+double orc_edge_energy_derivs(const double* cam7, const double* X3, const double* z2, double w, const double* intr4,
+                              double huber_delta, double* g12, double* H144) {
+  double E = 0;
+  edge_energy_derivs(se3_from7(cam7), X3, z2, w, intr4, huber_delta, &E, g12, H144);
+  return E;
 }
 
 // -------------------------------------------------------- single functions

@@ -65,6 +65,14 @@ def lib():
                                      _dp]
         L.orc_chol_solve.restype = ct.c_int
         L.orc_chol_solve.argtypes = [ct.c_int, _dp, _dp, _dp]
+        L.orc_ba_set_meas.argtypes = [ct.c_void_p, _dp, _dp]
+        L.orc_ba_adjoint.restype = ct.c_int
+        L.orc_ba_adjoint.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_ba_newton.restype = ct.c_int
+        L.orc_ba_newton.argtypes = [ct.c_void_p, ct.c_int]
+        L.orc_ba_exact_hessian.argtypes = [ct.c_void_p, _dp, _dp, _dp]
+        L.orc_edge_energy_derivs.restype = ct.c_double
+        L.orc_edge_energy_derivs.argtypes = [_dp, _dp, _dp, ct.c_double, _dp, ct.c_double, _dp, _dp]
         L.orc_num_threads.restype = ct.c_int
         L.orc_set_num_threads.argtypes = [ct.c_int]
         _lib = L
@@ -154,6 +162,47 @@ class OracleBA:
         n = lib().orc_ba_optimize(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp),
                                   times.ctypes.data_as(_dp))
         return n, trace[:n], times
+
+
+    def set_meas(self, meas, weight):
+        m, pm = _d(meas)
+        w, pw = _d(weight)
+        lib().orc_ba_set_meas(self.h, pm, pw)
+
+    def adjoint(self, dL_dcam, dL_dpt):
+        """Implicit-function gradient of a loss of the converged state with
+        respect to the measurements and information weights.
+        Returns (status, dL_dmeas [E,2], dL_dw [E], lambda [6Cf+3Pf])."""
+        dc, pdc = _d(dL_dcam)
+        dp_, pdp = _d(dL_dpt)
+        dz = np.zeros((self.E, 2))
+        dw = np.zeros(self.E)
+        lam = np.zeros(6 * self.Cf + 3 * self.Pf)
+        st = lib().orc_ba_adjoint(self.h, pdc, pdp, dz.ctypes.data_as(_dp), dw.ctypes.data_as(_dp),
+                                  lam.ctypes.data_as(_dp))
+        return st, dz, dw, lam
+
+    def newton(self, steps: int) -> int:
+        """Undamped Newton steps with the exact Hessian (0 = ok, 2 = not PD)."""
+        return lib().orc_ba_newton(self.h, steps)
+
+    def exact_hessian(self):
+        Hpp = np.zeros((self.Cf, 6, 6))
+        Hll = np.zeros((self.Pf, 3, 3))
+        Hpl = np.zeros((self.E, 6, 3))
+        lib().orc_ba_exact_hessian(self.h, Hpp.ctypes.data_as(_dp), Hll.ctypes.data_as(_dp),
+                                   Hpl.ctypes.data_as(_dp))
+        return Hpp, Hll, Hpl
+
+
+def edge_energy_derivs(cam7, X, z, w, intr4, huber_delta):
+    """0.5 rho of one projection edge, its gradient [12] and Hessian [12,12] in
+    (omega, upsilon, dX, dz, dw) at zero (hyper-dual evaluation)."""
+    g = np.zeros(12)
+    H = np.zeros((12, 12))
+    E = lib().orc_edge_energy_derivs(_d(cam7)[1], _d(X)[1], _d(z)[1], float(w), _d(intr4)[1], float(huber_delta),
+                                     g.ctypes.data_as(_dp), H.ctypes.data_as(_dp))
+    return E, g, H
 
 
 def se3_exp(upd):
