@@ -25,7 +25,7 @@ EXPORTS = [
     "g2ohip_get_b", "g2ohip_update", "g2ohip_push", "g2ohip_pop", "g2ohip_discard_top", "g2ohip_lambda_init",
     "g2ohip_compute_scale", "g2ohip_optimize", "g2ohip_set_force_stop_flag", "g2ohip_ba_get_hessian",
     "g2ohip_ba_get_schur", "g2ohip_last_solve_info", "g2ohip_set_profiling", "g2ohip_get_profile",
-    "g2ohip_linear_solve_pcg", "g2ohip_ba_adjoint",
+    "g2ohip_linear_solve_pcg", "g2ohip_ba_adjoint", "g2ohip_ba_newton",
 ]
 K_LINEARIZE, K_SCHUR, K_PCG, K_REFINE, K_CHI2, K_UPDATE = range(6)
 KERNEL_NAMES = ["linearize", "schur", "pcg", "refine", "chi2", "update"]
@@ -99,6 +99,7 @@ def lib():
         "g2ohip_linear_solve_pcg": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _dp, _dp, _dp,
                                                ct.c_double, ct.c_int, _dp, _ip]),
         "g2ohip_ba_adjoint": (ct.c_int, [_vp, _dp, _dp, _dp, _dp, ct.c_double, ct.c_int, _ip]),
+        "g2ohip_ba_newton": (ct.c_int, [_vp, ct.c_int, ct.c_double, _ip]),
     }
     for name in EXPORTS:
         fn = getattr(L, name)

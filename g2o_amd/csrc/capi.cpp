@@ -260,7 +260,13 @@ int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int
 int g2ohip_ba_adjoint(g2ohip_ctx* ctx, const double* dL_dcam, const double* dL_dpt, double* dL_dmeas,
                       double* dL_dweight, double tol, int max_iter, int* iterations) {
   G2OHIP_ENTER(ctx);
-  return ctx->adjoint(dL_dcam, dL_dpt, dL_dmeas, dL_dweight, tol, max_iter, iterations);
+  return ctx->adjoint(dL_dcam, dL_dpt, dL_dmeas, dL_dweight, tol < 0 ? 1e-24 : tol, max_iter, iterations);
+}
+
+int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, int* ok) {
+  G2OHIP_ENTER(ctx);
+  if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "ok must not be NULL");
+  return ctx->newton(steps, tol < 0 ? 1e-24 : tol, ok);
 }
 
 }  // extern "C"

@@ -330,6 +330,7 @@ int g2ohip_ctx::build_system() {
   HK(launch_sum(d_chunk_chi.p, st.nlc, d_scal.p + 1, stream));
   CK(read_scalar(d_scal.p + 1, &chi2_lin));
   have_system = true;
+  have_exact = false;
   system_prec = prec;
   have_solution = false;
   return G2OHIP_OK;
@@ -651,7 +652,7 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
 
 // --------------------------------------------------------------- inspection
 int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
-  if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+  if (!have_system && !have_exact) return fail(G2OHIP_ERR_INVALID, "build_system first");
   HK(hipStreamSynchronize(stream));
   if (Hpp) HK(hipMemcpy(Hpp, d_Hpp.p, 36 * sizeof(double) * st.Cf, hipMemcpyDeviceToHost));
   if (Hll) {
@@ -663,7 +664,7 @@ int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
   }
   if (Hpl_edge) {
     std::vector<double> h(18 * static_cast<size_t>(st.nq));
-    if (system_prec == G2OHIP_FP32) {
+    if (system_prec == G2OHIP_FP32 && !have_exact) {
       std::vector<float> hf(h.size());
       if (st.nq) HK(hipMemcpy(hf.data(), f32.Hpl.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < h.size(); ++i) h[i] = hf[i];
@@ -780,6 +781,147 @@ int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, cons
   return G2OHIP_OK;
 }
 
-int g2ohip_ctx::adjoint(const double*, const double*, double*, double*, double, int, int*) {
-  return fail(G2OHIP_ERR_UNSUPPORTED, "adjoint not built yet");
+// ------------------------------------------------ exact Hessian: Newton, adjoint
+// Exact Hessian of F/2 in the solver's FP64 blocks (Hpp, Hll, Hpl); b is left
+// as it is.
+int g2ohip_ctx::exact_system() {
+  HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
+  HK(f64.Hpl.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
+  HK(launch_linearize_exact(g, d_camRt.p, d_pts.p, f64.Hpl.p, d_Hll.p, d_hpp_part.p, stream));
+  HK(launch_reduce_hpp(g, d_hpp_part.p, d_Hpp.p, nullptr, stream));
+  have_system = false;  // the blocks no longer hold the Gauss-Newton system
+  have_exact = true;
+  return G2OHIP_OK;
+}
+
+// x = (H + lam I)^-1 b for the current FP64 blocks: explicit Schur, PCG to the
+// relative tolerance tol (no residual memory), back-substitution from the
+// blocks. *ok = 0 if a point block or a preconditioner block is singular or
+// the solve produced non-finite values.
+int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, int* ok) {
+  PrecBufs<double>& B = f64;
+  const int nb = st.Cf;
+  *ok = 1;
+  *iters = 0;
+  HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
+  HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
+  CK(ensure_rows());
+  HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
+  HK(B.W.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
+  HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
+  HK(launch_point_dinv<double>(g, d_Hll.p, lam, B.Dinv.p, d_err.p, stream));
+  HK(launch_schur_w<double>(g, B.Hpl.p, B.Dinv.p, B.W.p, stream));
+  HK(launch_schur_rows<double>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lam, B.F_val.p, d_bschur.p, nullptr, 0,
+                               stream));
+  HK((launch_jacobi<double, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream)));
+  int err = 0;
+  CK(read_err(&err));
+  if (err) { *ok = 0; return G2OHIP_OK; }
+  PcgRun pr;
+  // default iteration cap 20 n: the exact reduced systems of long camera
+  // chains are ill-conditioned (cond ~1e11) and the solve is run to a tight
+  // tolerance
+  CK(run_pcg<double>(B, d_bschur.p, PcgControl{tol, -1.0, -1.0}, false, max_iter < 0 ? 120 * nb : max_iter, 0,
+                     &pr));
+  *iters = pr.iters;
+  HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 0.0, stream));
+  HK(launch_backsub_blocks<double>(g, B.Hpl.p, B.Dinv.p, d_b.p, d_x.p, stream));
+  const size_t nx = 6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);
+  std::vector<double> hx(nx);
+  HK(hipMemcpyAsync(hx.data(), d_x.p, nx * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  for (double v : hx)
+    if (!std::isfinite(v)) { *ok = 0; break; }
+  return G2OHIP_OK;
+}
+
+// Newton steps with the exact Hessian, damped as Levenberg-Marquardt on that
+// Hessian: a step is kept if the robust chi2 does not increase (within 1e-12
+// relative: rounding at the optimum), else the state is restored and the
+// damping mu (0 at first) raised to max(1e-15 * max diag, 10 mu); after an
+// accepted step mu drops by 10 (to 0 below 1e-15 * max diag). At most 30
+// trials per step; *ok = 0 if none was acceptable.
+int g2ohip_ctx::newton(int steps, double tol, int* ok) {
+  CK(require_device());
+  if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+  *ok = 1;
+  double mu = 0.0;
+  for (int it = 0; it < steps && *ok; ++it) {
+    double chi0 = 0;
+    CK(compute_active_errors(&chi0));
+    // b = -grad(F/2) from the Gauss-Newton linearisation, then the exact blocks
+    HK(f64.Hpl.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
+    HK(launch_linearize<double>(g, d_camRt.p, d_pts.p, f64.Hpl.p, d_Hll.p, d_b.p, d_hpp_part.p, d_chunk_chi.p,
+                                stream));
+    HK(launch_reduce_hpp(g, d_hpp_part.p, d_Hpp.p, d_b.p, stream));
+    CK(exact_system());
+    HK(launch_max_diag(g, d_Hpp.p, d_Hll.p, d_red.p, d_scal.p + 2, stream));
+    double mx = 0;
+    CK(read_scalar(d_scal.p + 2, &mx));
+    const double mu_min = 1e-15 * mx;
+    bool accepted = false;
+    for (int trial = 0; trial < 30 && !accepted; ++trial) {
+      int iters = 0, solved = 0;
+      CK(solve_exact(mu, tol, -1, &iters, &solved));
+      if (solved) {
+        CK(push());
+        HK(launch_update(g, d_x.p, d_cams.p, d_pts.p, stream));
+        double chi1 = 0;
+        CK(compute_active_errors(&chi1));
+        if (std::isfinite(chi1) && chi1 <= chi0 + 1e-12 * std::fabs(chi0)) {
+          accepted = true;
+          CK(discard_top());
+        } else {
+          CK(pop());
+        }
+      }
+      if (!accepted) mu = std::max(mu_min, 10 * mu);
+    }
+    if (!accepted) *ok = 0;
+    mu = (mu / 10 < mu_min) ? 0.0 : mu / 10;
+  }
+  HK(hipStreamSynchronize(stream));
+  have_system = false;
+  have_solution = false;
+  return G2OHIP_OK;
+}
+
+int g2ohip_ctx::adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,
+                        int max_iter, int* iterations) {
+  CK(require_device());
+  if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+  if (!dL_dcam || !dL_dmeas) return fail(G2OHIP_ERR_INVALID, "dL_dcam and dL_dmeas are required");
+  CK(exact_system());
+  // right-hand side dL/dtheta in the solver's ordering (free cameras, then free points)
+  const size_t nx = 6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);
+  std::vector<double> rhs(nx, 0.0);
+  for (int c = 0; c < st.C; ++c)
+    if (st.camH[c] >= 0)
+      for (int r = 0; r < 6; ++r) rhs[6 * static_cast<size_t>(st.camH[c]) + r] = dL_dcam[6 * static_cast<size_t>(c) + r];
+  if (dL_dpt)
+    for (int i = 0; i < st.P; ++i)
+      if (st.ptH[i] >= 0)
+        for (int r = 0; r < 3; ++r)
+          rhs[6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.ptH[i]) + r] = dL_dpt[3 * static_cast<size_t>(i) + r];
+  HK(hipMemcpyAsync(d_b.p, rhs.data(), nx * sizeof(double), hipMemcpyHostToDevice, stream));
+  int iters = 0, ok = 1;
+  CK(solve_exact(0.0, tol, max_iter, &iters, &ok));
+  if (iterations) *iterations = iters;
+  if (!ok) return fail(G2OHIP_ERR_NOT_PD, "exact Hessian is not positive definite at this state");
+  DBuf<double> out;
+  HK(out.alloc(3 * static_cast<size_t>(std::max(st.ns, 1))));
+  HK(launch_adjoint_out(g, d_camRt.p, d_pts.p, d_x.p, out.p, stream));
+  std::vector<double> h(3 * static_cast<size_t>(st.ns));
+  HK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  std::fill(dL_dmeas, dL_dmeas + 2 * static_cast<size_t>(st.E), 0.0);
+  if (dL_dw) std::fill(dL_dw, dL_dw + st.E, 0.0);
+  for (int s = 0; s < st.ns; ++s) {
+    const size_t k = static_cast<size_t>(st.slot_edge[s]);
+    dL_dmeas[2 * k] = h[3 * static_cast<size_t>(s)];
+    dL_dmeas[2 * k + 1] = h[3 * static_cast<size_t>(s) + 1];
+    if (dL_dw) dL_dw[k] = h[3 * static_cast<size_t>(s) + 2];
+  }
+  have_solution = false;
+  return G2OHIP_OK;
 }

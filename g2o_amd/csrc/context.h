@@ -79,6 +79,7 @@ struct g2ohip_ctx {
   bool have_system = false;   // build_system ran since the last state change
   int system_prec = 0;        // precision of the last build_system
   bool have_solution = false;
+  bool have_exact = false;    // the FP64 blocks hold the exact Hessian (adjoint / Newton)
   double lambda = 0.0;
   double pcg_residual_mem = -1.0;  // LinearSolverPCG::_residual
   double last_info[3] = {0, 0, -1};
@@ -144,6 +145,10 @@ struct g2ohip_ctx {
                        const double* b, double* x, double tol, int max_iter, double* residual_io, int* iterations);
   int adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,
               int max_iter, int* iterations);
+
+  int exact_system();
+  int solve_exact(double lam, double tol, int max_iter, int* iters, int* ok);
+  int newton(int steps, double tol, int* ok);
 
   template <typename HT>
   int solve_impl(int* ok);

@@ -131,7 +131,7 @@ __device__ inline void project_error(const double* Rt, const double* in, const d
 // (edge_project_xyz.cpp:57-93) in precision T: Jp = d e / d X (2x3 row-major),
 // Jc = d e / d [omega; upsilon] (2x6 row-major), from the camera-frame point.
 template <typename T>
-__device__ inline void project_jacobians(const T* R, T fx, T fy, T x, T y, T z, T* Jp, T* Jc) {
+__host__ __device__ inline void project_jacobians(const T* R, T fx, T fy, T x, T y, T z, T* Jp, T* Jc) {
   T iz = T(1) / z;
   T z2 = z * z;
   T t00 = fx, t02 = -x * iz * fx, t11 = fy, t12 = -y * iz * fy;
@@ -151,6 +151,72 @@ __device__ inline void project_jacobians(const T* R, T fx, T fy, T x, T y, T z, 
   Jc[9] = T(0);
   Jc[10] = -iz * fy;
   Jc[11] = y / z2 * fy;
+}
+
+// Exact Hessian of one projection edge's energy E = 0.5 rho(w |e|^2),
+// e = z - proj(exp(omega, upsilon) T (X + dX)), in the variables
+// v = [omega upsilon dX] at v = 0 (9x9, row-major):
+//   rho' w Je^T Je + 2 rho'' w^2 (Je^T e)(Je^T e)^T - JP^T M JP - S(g)
+// with Je = [Jc Jp] the edge Jacobian, Y = R X + t the camera-frame point,
+// JP = dY/dv = [-[Y]x, I, R], a = rho' w e, M = sum_k a_k d2proj_k/dY2,
+// g = dproj/dY^T a, and S(g) the Hessian of g . Y(v) through the second-order
+// terms of exp(omega, upsilon) Y = Y + omega x Y + upsilon + 1/2 omega x
+// (omega x Y) + 1/2 omega x upsilon and omega x (R dX):
+//   S_ww = 1/2 (g Y^T + Y g^T) - (g . Y) I, S_wu = -1/2 [g]x, S_wX = -[g]x R.
+// Also returns rho', rho'' and s for the mixed derivatives.
+__host__ __device__ inline void edge_exact_hessian(const double* R, double fx, double fy, const double* Y,
+                                                   const double* e, double w, double huber, double* H,
+                                                   double* rho_out) {
+  double rho[3];
+  const double s = w * (e[0] * e[0] + e[1] * e[1]);
+  huber_rho(s, huber, rho);
+  rho_out[0] = rho[1];
+  rho_out[1] = rho[2];
+  rho_out[2] = s;
+  double Jp[6], Jc[12];
+  project_jacobians<double>(R, fx, fy, Y[0], Y[1], Y[2], Jp, Jc);
+  double J[2][9];
+  for (int c = 0; c < 6; ++c) { J[0][c] = Jc[c]; J[1][c] = Jc[6 + c]; }
+  for (int c = 0; c < 3; ++c) { J[0][6 + c] = Jp[c]; J[1][6 + c] = Jp[3 + c]; }
+  double u[9];
+  for (int i = 0; i < 9; ++i) u[i] = J[0][i] * e[0] + J[1][i] * e[1];
+  const double gn = rho[1] * w, hb = 2.0 * rho[2] * w * w;
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) H[9 * i + j] = gn * (J[0][i] * J[0][j] + J[1][i] * J[1][j]) + hb * u[i] * u[j];
+  // second-order terms
+  const double x = Y[0], y = Y[1], z = Y[2], iz = 1.0 / z, iz2 = iz * iz, iz3 = iz2 * iz;
+  const double a0 = gn * e[0], a1 = gn * e[1];
+  const double g[3] = {a0 * fx * iz, a1 * fy * iz, -(a0 * fx * x + a1 * fy * y) * iz2};
+  double M[9] = {0, 0, -a0 * fx * iz2, 0, 0, -a1 * fy * iz2, -a0 * fx * iz2, -a1 * fy * iz2,
+                 2.0 * (a0 * fx * x + a1 * fy * y) * iz3};
+  // JP columns: omega k -> -[Y]x e_k = Y x e_k ... as a 3x9 matrix
+  double JP[3][9];
+  const double Yx[9] = {0, -z, y, z, 0, -x, -y, x, 0};  // [Y]x
+  for (int r = 0; r < 3; ++r) {
+    for (int k = 0; k < 3; ++k) {
+      JP[r][k] = -Yx[3 * r + k];
+      JP[r][3 + k] = (r == k) ? 1.0 : 0.0;
+      JP[r][6 + k] = R[3 * r + k];
+    }
+  }
+  double MJ[3][9];
+  for (int r = 0; r < 3; ++r)
+    for (int j = 0; j < 9; ++j) MJ[r][j] = M[3 * r] * JP[0][j] + M[3 * r + 1] * JP[1][j] + M[3 * r + 2] * JP[2][j];
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) H[9 * i + j] -= JP[0][i] * MJ[0][j] + JP[1][i] * MJ[1][j] + JP[2][i] * MJ[2][j];
+  const double gY = g[0] * x + g[1] * y + g[2] * z;
+  const double gx[9] = {0, -g[2], g[1], g[2], 0, -g[0], -g[1], g[0], 0};  // [g]x
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double sww = 0.5 * (g[i] * Y[j] + Y[i] * g[j]) - (i == j ? gY : 0.0);
+      H[9 * i + j] -= sww;
+      H[9 * i + 3 + j] -= -0.5 * gx[3 * i + j];  // S_wu
+      H[9 * (3 + j) + i] -= -0.5 * gx[3 * i + j];  // S_uw = S_wu^T
+      double swx = 0;
+      for (int k = 0; k < 3; ++k) swx -= gx[3 * i + k] * R[3 * k + j];
+      H[9 * i + 6 + j] -= swx;
+      H[9 * (6 + j) + i] -= swx;
+    }
 }
 
 // 3x3 inverse by cofactors; symmetric input given by its 6 unique entries

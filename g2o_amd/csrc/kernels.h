@@ -128,6 +128,14 @@ template <typename HT, int BD>
 hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
                              const double* part_dq, int n_dq, double* part_rs, const PcgScalars* sc, int it,
                              hipStream_t st);
+// ---- exact Hessian (adjoint, Newton polish), adjoint.hip
+hipError_t launch_linearize_exact(const DevGraph& g, const double* camRt, const double* pts, double* Hpl,
+                                  double* Hll, double* hpp_part, hipStream_t st);
+template <typename HT>
+hipError_t launch_backsub_blocks(const DevGraph& g, const HT* Hpl, const HT* Dinv, const double* b, double* x,
+                                 hipStream_t st);
+hipError_t launch_adjoint_out(const DevGraph& g, const double* camRt, const double* pts, const double* lam,
+                              double* out, hipStream_t st);
 template <typename HT>
 hipError_t launch_to_double(int n, const HT* src, double* dst, double scale_add, hipStream_t st);
 hipError_t launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t st);

@@ -284,7 +284,8 @@ template hipError_t launch_linearize<float>(const DevGraph&, const double*, cons
 template hipError_t launch_linearize<double>(const DevGraph&, const double*, const double*, double*, double*,
                                              double*, double*, double*, hipStream_t);
 
-// Hpp and b_p: sum of the chunk partials of each free camera, fixed order.
+// Hpp and b_p (b == null: Hpp only): sum of the chunk partials of each free
+// camera, fixed order.
 __global__ void k_reduce_hpp(DevGraph g, const double* __restrict__ hpp_part, double* __restrict__ Hpp,
                              double* __restrict__ b) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,7 +297,7 @@ __global__ void k_reduce_hpp(DevGraph g, const double* __restrict__ hpp_part, do
     const int r = c_up6_r[k], cc = c_up6_c[k];
     Hpp[36 * static_cast<size_t>(c) + 6 * r + cc] = v;
     Hpp[36 * static_cast<size_t>(c) + 6 * cc + r] = v;
-  } else {
+  } else if (b) {
     b[6 * static_cast<size_t>(c) + (k - 21)] = v;
   }
 }

@@ -281,3 +281,28 @@ def make_diff_ba(seed: int = 1004) -> BAProblem:
     """Differentiable-BA workload: 200 cameras, 50k points, ~300k observations,
     per-edge weights U(0.5, 2)."""
     return make_walk_ba(seed=seed, num_cams=200, num_pts=50_000, weights=(0.5, 2.0))
+
+
+def select_edges(prob: BAProblem, keep: np.ndarray, min_obs: int = 2) -> BAProblem:
+    """Sub-problem with the edges where ``keep`` is true, as after g2o's
+    outlier rejection (edges set to level 1 and left out of
+    initializeOptimization(0)); points left with fewer than ``min_obs`` edges
+    are dropped with their edges. info['edge_index'] / info['pt_index'] map
+    back to the original edges / points."""
+    keep = np.asarray(keep, dtype=bool).copy()
+    while True:
+        cnt = np.bincount(prob.edge_pt[keep], minlength=prob.num_pts)
+        drop = keep & (cnt[prob.edge_pt] < min_obs)
+        if not drop.any():
+            break
+        keep &= ~drop
+    used = np.flatnonzero(np.bincount(prob.edge_pt[keep], minlength=prob.num_pts) > 0)
+    remap = -np.ones(prob.num_pts, dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    eidx = np.flatnonzero(keep)
+    info = dict(prob.info)
+    info.update(edge_index=eidx, pt_index=used)
+    return BAProblem(prob.cams.copy(), prob.intr.copy(), prob.pts[used].copy(), prob.edge_cam[eidx].copy(),
+                     remap[prob.edge_pt[eidx]].astype(np.int32), prob.meas[eidx].copy(), prob.weight[eidx].copy(),
+                     prob.cam_fixed.copy(), prob.pt_fixed[used].copy(), prob.huber, name=prob.name + "_sel",
+                     gt_cams=prob.gt_cams, gt_pts=None if prob.gt_pts is None else prob.gt_pts[used], info=info)

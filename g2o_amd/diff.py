@@ -1,0 +1,126 @@
+"""Differentiable bundle adjustment: a torch.autograd.Function around the
+HIP solver.
+
+Forward: LM to convergence (any precision the BundleAdjuster is set to), then
+undamped exact-Hessian Newton steps in FP64 so the state is a stationary point
+of F = sum_e rho(w_e |e_e|^2). Outputs the camera centres C = -R^T t [C,3]
+and the points [P,3].
+
+Backward: one implicit-function solve at the optimum (g2ohip_ba_adjoint):
+H lambda = dL/dx with the exact Hessian, dL/dz = -(dG/dz)^T lambda,
+dL/dw = -(dG/dw)^T lambda. The gradients come back in the dtype and on the
+device of the measurement / weight tensors.
+
+There is no counterpart in g2o, which stops at the solution.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .datasets import BAProblem, quat_to_rot, select_edges
+from .solver import BundleAdjuster
+
+
+def camera_centres(cams7: np.ndarray) -> np.ndarray:
+    """C = -R^T t for world-to-camera poses [tx ty tz qx qy qz qw]."""
+    R = quat_to_rot(cams7[:, 3:7])
+    return -np.einsum("cji,cj->ci", R, cams7[:, :3])
+
+
+def centres_grad_to_tangent(cams7: np.ndarray, g_centres: np.ndarray) -> np.ndarray:
+    """Pull dL/dC back to the left exp-map update [omega, upsilon] of each
+    camera: C(exp(d) T) = C - R^T upsilon + O(|d|^2), so dL/domega = 0 and
+    dL/dupsilon = -R dL/dC."""
+    R = quat_to_rot(cams7[:, 3:7])
+    out = np.zeros((cams7.shape[0], 6))
+    out[:, 3:] = -np.einsum("cij,cj->ci", R, g_centres)
+    return out
+
+
+class BundleAdjustmentFunction(torch.autograd.Function):
+    """(meas [E,2], weight [E]) -> (centres [C,3], points [P,3])."""
+
+    @staticmethod
+    def forward(ctx, meas, weight, ba: BundleAdjuster, init_cams, init_pts, lm_iterations: int,
+                newton_steps: int):
+        m = meas.detach().to("cpu", torch.float64).numpy()
+        w = weight.detach().to("cpu", torch.float64).numpy()
+        prob = ba.prob
+        ba.set_measurements(m, w, prob.intr, prob.huber)
+        ba.set_state(init_cams, init_pts)
+        ba.optimize(lm_iterations)
+        if newton_steps > 0 and not ba.newton(newton_steps):
+            raise RuntimeError("exact Hessian not positive definite: the LM solution is not a strict minimum")
+        cams, pts = ba.state()
+        ctx.ba = ba
+        ctx.cams = cams
+        ctx.meta = (meas.dtype, meas.device, weight.dtype, weight.device)
+        centres = torch.from_numpy(camera_centres(cams)).to(meas.device, meas.dtype)
+        points = torch.from_numpy(pts).to(meas.device, meas.dtype)
+        return centres, points
+
+    @staticmethod
+    def backward(ctx, g_centres, g_points):
+        mdt, mdev, wdt, wdev = ctx.meta
+        cams = ctx.cams
+        gc = np.zeros((cams.shape[0], 3)) if g_centres is None else g_centres.detach().to("cpu", torch.float64).numpy()
+        gp = None if g_points is None else g_points.detach().to("cpu", torch.float64).numpy()
+        dz, dw, _ = ctx.ba.adjoint(centres_grad_to_tangent(cams, gc), gp)
+        return (torch.from_numpy(dz).to(mdev, mdt), torch.from_numpy(dw).to(wdev, wdt), None, None, None, None, None)
+
+
+def solve(meas: torch.Tensor, weight: torch.Tensor, ba: BundleAdjuster, init_cams: np.ndarray, init_pts: np.ndarray,
+          lm_iterations: int = 30, newton_steps: int = 3):
+    """Differentiable solve; see BundleAdjustmentFunction."""
+    return BundleAdjustmentFunction.apply(meas, weight, ba, init_cams, init_pts, lm_iterations, newton_steps)
+
+
+def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine_steps: int = 1,
+            lm_iterations: int = 30, chi2_max: float | None = None, min_obs: int = 3, max_cond: float = 1e6,
+            newton_steps: int = 4):
+    """Conditions a robust BA problem for implicit differentiation and solves it.
+
+    The implicit function theorem needs a strict stationary point with a
+    well-conditioned exact Hessian. Under a Huber kernel, uniform outliers and
+    landmarks seen by two cameras whose residuals both lie in the kernel's
+    linear range have indefinite or near-singular exact point blocks. So:
+    robust LM; edges with w |e|^2 > chi2_max (default 5.991 delta^2, the 95 %
+    chi-square bound scaled to the kernel) are dropped, as g2o workflows set
+    outlier edges to level 1; landmarks left with < min_obs edges are dropped;
+    LM again; landmarks whose exact 3x3 block has condition number > max_cond
+    are dropped; LM again and exact-Hessian Newton steps.
+
+    Returns (sub_problem, BundleAdjuster at the stationary point). The
+    sub-problem's info['edge_index'] maps its edges to the input's."""
+    ba = BundleAdjuster(prob, device=device, precision=precision, refine_steps=refine_steps)
+    ba.optimize(lm_iterations)
+    cams, pts = ba.state()
+    e = ba.errors()
+    chi = prob.weight * (e * e).sum(1)
+    if chi2_max is None:
+        chi2_max = 5.991 * prob.huber ** 2 if prob.huber > 0 else np.inf
+    sub = select_edges(prob, chi <= chi2_max, min_obs)
+    sub.cams, sub.pts = cams, pts[sub.info["pt_index"]]
+    ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps)
+    ba.optimize(lm_iterations)
+    ba.adjoint(np.zeros((sub.num_cams, 6)))  # evaluates the exact blocks at the state
+    _, Hll, _ = ba.hessian()
+    ev = np.linalg.eigvalsh(Hll)
+    free = np.flatnonzero(sub.pt_fixed == 0)
+    weak = np.zeros(sub.num_pts, dtype=bool)
+    weak[free] = ev[:, 0] <= ev[:, 2] / max_cond
+    if weak.any():
+        cams, pts = ba.state()
+        eidx = sub.info["edge_index"]
+        sub2 = select_edges(sub, ~weak[sub.edge_pt], min_obs)
+        sub2.info["edge_index"] = eidx[sub2.info["edge_index"]]
+        sub2.info["pt_index"] = sub.info["pt_index"][sub2.info["pt_index"]]
+        sub2.cams, sub2.pts = cams, pts[np.searchsorted(sub.info["pt_index"], sub2.info["pt_index"])]
+        sub = sub2
+        ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps)
+        ba.optimize(lm_iterations)
+    if not ba.newton(newton_steps):
+        raise RuntimeError("exact-Hessian Newton found no descent step")
+    return sub, ba

@@ -222,6 +222,27 @@ class BundleAdjuster:
             out[name] = (t.value, n.value)
         return out
 
+    # ---- exact Hessian: stationary point and implicit differentiation
+    def newton(self, steps: int = 3, tol: float = -1.0) -> bool:
+        """Undamped exact-Hessian Newton steps (drive an LM solution to the
+        stationary point). Returns False if the Hessian is not positive definite."""
+        ok = ct.c_int()
+        self.ctx.call("g2ohip_ba_newton", int(steps), float(tol), ct.byref(ok))
+        return bool(ok.value)
+
+    def adjoint(self, dL_dcam, dL_dpt=None, tol: float = -1.0, max_iter: int = -1):
+        """Implicit-function gradient of a loss of the converged state.
+        dL_dcam [C,6] in the left exp-map coordinates [omega, upsilon], dL_dpt [P,3].
+        Returns (dL_dmeas [E,2], dL_dweight [E], PCG iterations)."""
+        dc = np.ascontiguousarray(dL_dcam, dtype=np.float64).reshape(self.C, 6)
+        dp = None if dL_dpt is None else np.ascontiguousarray(dL_dpt, dtype=np.float64).reshape(self.P, 3)
+        dz = np.zeros((self.E, 2))
+        dw = np.zeros(self.E)
+        it = ct.c_int()
+        self.ctx.call("g2ohip_ba_adjoint", _dp(dc), None if dp is None else _dp(dp), _dp(dz), _dp(dw), float(tol),
+                      int(max_iter), ct.byref(it))
+        return dz, dw, it.value
+
     def last_solve_info(self):
         v = np.zeros(3)
         self.ctx.call("g2ohip_last_solve_info", _dp(v))

@@ -161,7 +161,9 @@ int g2ohip_set_force_stop_flag(g2ohip_ctx* ctx, const volatile int* flag);
 
 /* ------------------------------------------------- inspection (FP64 copies) */
 /* Hpp[Cf*36], Hll[Pf*9], Hpl per edge [E*18] (6x3 row-major; zero for edges
- * without a block), all row-major, of the last build_system. NULL skips. */
+ * without a block), all row-major, of the last build_system, or of the last
+ * exact-Hessian evaluation (g2ohip_ba_adjoint / g2ohip_ba_newton) if that came
+ * later. NULL skips. */
 int g2ohip_ba_get_hessian(g2ohip_ctx* ctx, double* Hpp, double* Hll, double* Hpl_edge);
 /* Hschur upper blocks of the last solve, including the damping: rows/cols
  * [nS], blocks [nS*36] row-major; bschur [6Cf]. */
@@ -201,15 +203,28 @@ int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int
 /* ---------------------------------------------- adjoint (differentiation) */
 /* Gradient of a scalar loss L(x*) at the converged state with respect to the
  * measurements and the per-edge information weights, by implicit
- * differentiation of the optimality condition grad E(x*, z, w) = 0:
- *   (d^2E/dx^2) lambda = dL/dx,  dL/dz = -(d^2E/dx dz)^T lambda,
- *   dL/dw = -(d^2E/dx dw)^T lambda,
- * with the exact second-order (Newton) Hessian of the projection and Huber
- * terms. dL_dcam[C*6] is the loss gradient with respect to the left-multiplied
- * camera perturbation [omega; upsilon] (zero rows for fixed cameras), dL_dpt[P*3].
- * Outputs dL_dmeas[E*2], dL_dweight[E]. No reference counterpart. */
+ * differentiation of the optimality condition G(x*, z, w) = grad(F/2) = 0,
+ * F = sum_e rho(w_e |e_e|^2):
+ *   H lambda = dL/dx,  dL/dz = -(dG/dz)^T lambda,  dL/dw = -(dG/dw)^T lambda,
+ * H = dG/dx the exact Hessian (Gauss-Newton term plus the second-order terms
+ * of the projection, of the exp-map update and of the Huber kernel), solved
+ * once by the explicit Schur complement and PCG in FP64 to the relative
+ * tolerance tol (<0 defaults to 1e-24 on r.M^-1 r; max_iter < 0: system size).
+ * dL_dcam[C*6] is the loss gradient with respect to the left-multiplied camera
+ * update [omega; upsilon] (rows of fixed cameras ignored), dL_dpt[P*3] (NULL =
+ * zero). Outputs dL_dmeas[E*2] and dL_dweight[E] (NULL skips; zero for
+ * inactive edges). Requires a stationary state (see g2ohip_ba_newton).
+ * No counterpart in the reference, which stops at the solution. */
 int g2ohip_ba_adjoint(g2ohip_ctx* ctx, const double* dL_dcam, const double* dL_dpt, double* dL_dmeas,
                       double* dL_dweight, double tol, int max_iter, int* iterations);
+/* Newton steps x <- x + (H + mu I)^-1 (-grad(F/2)) with the exact Hessian
+ * above (FP64, PCG tolerance tol as for g2ohip_ba_adjoint), damped as
+ * Levenberg-Marquardt on that Hessian (mu = 0 while the robust chi2 does not
+ * increase): drives an LM solution, whose accuracy LM's chi2-based step
+ * acceptance limits, to the stationary point that g2ohip_ba_adjoint
+ * differentiates. *ok = 0 if a step found no acceptable damping. No
+ * counterpart in the reference. */
+int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, int* ok);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,7 @@
 #include <limits>
 #include <unordered_map>
 #include <vector>
+#include <cstdio>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1197,22 +1198,53 @@ int orc_ba_adjoint(void* h, const double* dL_dcam, const double* dL_dpt, double*
 }
 
 // Newton steps with the exact Hessian (gradient from ba_build_system's b =
-// -grad(F/2)), undamped and always applied: drives a converged LM solution to
-// the stationary point to machine precision before it is differentiated.
-// Returns 0, or 2 if the Hessian is not positive definite.
+// -grad(F/2)), damped as Levenberg-Marquardt on that Hessian: a step is kept
+// if the robust chi2 does not increase, else the state is restored and the
+// damping mu (0 at first) raised to max(1e-9 * max diag, 10 mu) (an increase
+// within 1e-12 relative, rounding at the optimum, counts as none); after an
+// accepted step mu drops by 10 (below 1e-9 * max diag it is 0). Drives a
+// converged LM solution to the stationary point to machine precision before
+// it is differentiated. Returns 0, or 2 if no step was acceptable.
 int orc_ba_newton(void* h, int steps) {
   BA* p = static_cast<BA*>(h);
   const int saved_kind = p->solver_kind;
   const double saved_lambda = p->lambda;
   p->solver_kind = 0;
-  p->lambda = 0.0;
   int st = 0;
-  for (int it = 0; it < steps; ++it) {
-    ba_chi2(*p);
+  double mu = 0.0;
+  for (int it = 0; it < steps && st == 0; ++it) {
+    const double chi0 = ba_chi2(*p);
     ba_build_system(*p);  // b
     ba_build_exact(*p, nullptr);
-    if (!ba_solve(*p)) { st = 2; break; }
-    ba_update(*p, p->x.data());
+    double mx = 0;
+    for (int c = 0; c < p->Cf; ++c)
+      for (int k = 0; k < 6; ++k) mx = std::max(mx, std::fabs(p->Hpp[36 * static_cast<size_t>(c) + 7 * k]));
+    for (int l = 0; l < p->Pf; ++l)
+      for (int k = 0; k < 3; ++k) mx = std::max(mx, std::fabs(p->Hll[9 * static_cast<size_t>(l) + 4 * k]));
+    const double mu_min = 1e-15 * mx;
+    bool accepted = false;
+    for (int trial = 0; trial < 30 && !accepted; ++trial) {
+      p->lambda = mu;
+      bool solved = ba_solve(*p);
+
+      if (solved) {
+        vector<SE3> c0 = p->cams;
+        vector<double> x0 = p->pts;
+        ba_update(*p, p->x.data());
+        const double chi1 = ba_chi2(*p);
+
+        if (std::isfinite(chi1) && chi1 <= chi0 + 1e-12 * std::fabs(chi0)) {
+          accepted = true;
+        } else {
+          p->cams = c0;
+          p->pts = x0;
+        }
+      }
+      if (!accepted) mu = std::max(mu_min, 10 * mu);
+    }
+    if (!accepted) st = 2;
+
+    mu = (mu / 10 < mu_min) ? 0.0 : mu / 10;
   }
   p->solver_kind = saved_kind;
   p->lambda = saved_lambda;
