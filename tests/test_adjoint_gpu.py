@@ -40,7 +40,10 @@ def test_exact_hessian_blocks_match_oracle():
         Hpp_o, Hll_o, Hpl_o = ora.exact_hessian()
         ba = g2o_amd.BundleAdjuster(prob, device=0)
         ba.set_state(c, p)
-        ba.adjoint(np.zeros((prob.num_cams, 6)))
+        try:
+            ba.adjoint(np.zeros((prob.num_cams, 6)))
+        except g2o_amd.G2oHipError as ex:  # away from the optimum H may be indefinite
+            assert ex.status == g2o_amd._lib.ERR_NOT_PD
         Hpp, Hll, Hpl = ba.hessian()
         for a, b in ((Hpp, Hpp_o), (Hll, Hll_o), (Hpl, Hpl_o)):
             assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
