@@ -306,3 +306,115 @@ def select_edges(prob: BAProblem, keep: np.ndarray, min_obs: int = 2) -> BAProbl
                      remap[prob.edge_pt[eidx]].astype(np.int32), prob.meas[eidx].copy(), prob.weight[eidx].copy(),
                      prob.cam_fixed.copy(), prob.pt_fixed[used].copy(), prob.huber, name=prob.name + "_sel",
                      gt_cams=prob.gt_cams, gt_pts=None if prob.gt_pts is None else prob.gt_pts[used], info=info)
+
+
+# ------------------------------------------------------------ 3D pose graph
+@dataclass
+class PoseGraphProblem:
+    """EdgeSE3 / VertexSE3 pose graph. poses [N,7] = Isometry3 [tx ty tz qx qy
+    qz qw] (x_world = R x + t); meas [E,7] relative pose from -> to; info
+    [E,6,6] information matrices; fixed [N] 0/1."""
+    poses: np.ndarray
+    edge_from: np.ndarray
+    edge_to: np.ndarray
+    meas: np.ndarray
+    info: np.ndarray
+    fixed: np.ndarray
+    name: str = ""
+    gt_poses: np.ndarray | None = None
+    info_dict: dict = field(default_factory=dict)
+
+    @property
+    def num_poses(self) -> int:
+        return int(self.poses.shape[0])
+
+    @property
+    def num_edges(self) -> int:
+        return int(self.edge_from.shape[0])
+
+
+def _quat_mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    ax, ay, az, aw = np.moveaxis(a, -1, 0)
+    bx, by, bz, bw = np.moveaxis(b, -1, 0)
+    return np.stack([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz], axis=-1)
+
+
+def _relative(pa: np.ndarray, pb: np.ndarray) -> np.ndarray:
+    """pa^-1 pb for Isometry3 poses [.., 7]."""
+    Ra = quat_to_rot(pa[..., 3:])
+    t = np.einsum("...ji,...j->...i", Ra, pb[..., :3] - pa[..., :3])
+    qa_inv = pa[..., 3:] * np.array([-1, -1, -1, 1.0])
+    q = _quat_mul(qa_inv, pb[..., 3:])
+    q *= np.where(q[..., 3:] < 0, -1.0, 1.0)
+    return np.concatenate([t, q / np.linalg.norm(q, axis=-1, keepdims=True)], axis=-1)
+
+
+def _compose(pa: np.ndarray, pb: np.ndarray) -> np.ndarray:
+    """pa pb for Isometry3 poses [.., 7]."""
+    Ra = quat_to_rot(pa[..., 3:])
+    t = pa[..., :3] + np.einsum("...ij,...j->...i", Ra, pb[..., :3])
+    q = _quat_mul(pa[..., 3:], pb[..., 3:])
+    return np.concatenate([t, q / np.linalg.norm(q, axis=-1, keepdims=True)], axis=-1)
+
+
+def make_sphere_pose_graph(seed: int = 1001, nodes_per_level: int = 1000, laps: int = 10, radius: float = 5.0,
+                           sigma_t: float = 0.01, sigma_r: float = 0.005) -> PoseGraphProblem:
+    """Sphere trajectory as g2o's create_sphere example builds it
+    (examples/sphere/create_sphere.cpp): ``laps`` laps of ``nodes_per_level``
+    poses, azimuth -pi + 2 pi n / nodes_per_level, elevation rising from
+    -pi/2 to pi/2 over the whole run, position R (radius, 0, 0). Odometry
+    edges (i-1, i) and, from each pose of lap f-1, edges to the poses n-1,
+    n, n+1 of lap f (the adjacent laps are 2 r sin(pi / (2 laps)) apart: 1.56
+    m at r = 5). Measurements are the true relative poses with noise: the
+    translation plus N(0, sigma_t^2), the rotation right-multiplied by the
+    normalised quaternion (1 - |v|, v), v ~ N(0, sigma_r^2). Information =
+    diag(1/sigma_t^2 x3, 1/sigma_r^2 x3). The initial estimate chains the
+    noisy odometry from the first pose, which is fixed."""
+    rng = np.random.default_rng(seed)
+    N = nodes_per_level * laps
+    ids = np.arange(N)
+    n = ids % nodes_per_level
+    az = -np.pi + 2 * n * np.pi / nodes_per_level
+    el = -0.5 * np.pi + (ids + 1) * np.pi / N
+    # rot = Rz(az) Ry(el)
+    cz, sz, cy, sy = np.cos(az), np.sin(az), np.cos(el), np.sin(el)
+    R = np.zeros((N, 3, 3))
+    R[:, 0, 0] = cz * cy; R[:, 0, 1] = -sz; R[:, 0, 2] = cz * sy
+    R[:, 1, 0] = sz * cy; R[:, 1, 1] = cz; R[:, 1, 2] = sz * sy
+    R[:, 2, 0] = -sy; R[:, 2, 1] = 0; R[:, 2, 2] = cy
+    t = R[:, :, 0] * radius
+    gt = np.concatenate([t, rot_to_quat(R)], axis=1)
+    gt[:, 3:] *= np.where(gt[:, 6:7] < 0, -1.0, 1.0)
+    fr, to = [np.arange(N - 1)], [np.arange(1, N)]
+    for f in range(1, laps):
+        for d in (-1, 0, 1):
+            nn = np.arange(nodes_per_level)
+            if f == laps - 1 and d == 1:
+                continue
+            src = (f - 1) * nodes_per_level + nn
+            dst = f * nodes_per_level + nn + d
+            ok = (dst >= 0) & (dst < N)
+            fr.append(src[ok])
+            to.append(dst[ok])
+    fr = np.concatenate(fr).astype(np.int32)
+    to = np.concatenate(to).astype(np.int32)
+    E = fr.size
+    meas = _relative(gt[fr], gt[to])
+    v = rng.normal(0, sigma_r, size=(E, 3))
+    qw = np.maximum(1.0 - np.linalg.norm(v, axis=1), 0.0)
+    dq = np.concatenate([v, qw[:, None]], axis=1)
+    dq /= np.linalg.norm(dq, axis=1, keepdims=True)
+    meas[:, 3:] = _quat_mul(meas[:, 3:], dq)
+    meas[:, :3] += rng.normal(0, sigma_t, size=(E, 3))
+    info = np.zeros((E, 6, 6))
+    info[:, [0, 1, 2], [0, 1, 2]] = 1.0 / sigma_t ** 2
+    info[:, [3, 4, 5], [3, 4, 5]] = 1.0 / sigma_r ** 2
+    poses = np.empty_like(gt)
+    poses[0] = gt[0]
+    for i in range(1, N):  # odometry edges are the first N - 1
+        poses[i] = _compose(poses[i - 1], meas[i - 1])
+    fixed = np.zeros(N, dtype=np.uint8)
+    fixed[0] = 1
+    return PoseGraphProblem(poses, fr, to, meas, info, fixed, name=f"sphere_{laps}x{nodes_per_level}",
+                            gt_poses=gt, info_dict={"radius": radius})

@@ -958,6 +958,335 @@ void ba_build_exact(BA& p, vector<double>* mixed) {
   }
 }
 
+
+// ---------------------------------------------------------- 3D pose graph
+// VertexSE3 / EdgeSE3 (vertex_se3.h:99-108, edge_se3.cpp:60-88,
+// isometry3d_mappings.cpp:80-128). A pose is the Isometry3 x_world = R(q) x + t,
+// stored [tx ty tz qx qy qz qw]. Increment: X <- X * fromVectorMQT(d), d =
+// [dt; dqx dqy dqz], qw = sqrt(1 - |dq|^2) (identity rotation if negative).
+// Error: toVectorMQT(Z^-1 Xi^-1 Xj) = [t; vec(q)] with q normalised to w >= 0.
+// The Jacobians d e / d [d_i; d_j] at 0 come from dual-number evaluation of
+// that map (the c parts of the hyper-dual numbers above), not from a closed
+// form.
+template <class S>
+struct QT {
+  S x, y, z, w;
+};
+inline HD operator*(const HD& x, double s) { return s * x; }
+inline HD operator-(const HD& x) { return (-1.0) * x; }
+template <class S>
+QT<S> qt_mul(const QT<S>& a, const QT<S>& b) {
+  return {a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+          a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+template <class S>
+QT<S> qt_conj(const QT<S>& a) { return {-a.x, -a.y, -a.z, a.w}; }
+// v' = R(q) v for a unit quaternion
+template <class S>
+void qt_rot(const QT<S>& q, const S v[3], S out[3]) {
+  S c[3] = {q.y * v[2] - q.z * v[1], q.z * v[0] - q.x * v[2], q.x * v[1] - q.y * v[0]};
+  S c2[3] = {q.y * c[2] - q.z * c[1], q.z * c[0] - q.x * c[2], q.x * c[1] - q.y * c[0]};
+  for (int i = 0; i < 3; ++i) out[i] = v[i] + 2.0 * (q.w * c[i] + c2[i]);
+}
+
+struct PG {
+  int N = 0, E = 0;
+  vector<double> t;     // 3 per pose
+  vector<Quat> q;       // per pose
+  vector<int> from, to;
+  vector<double> mt;    // measurement translation, 3 per edge
+  vector<Quat> mq;      // measurement rotation
+  vector<double> info;  // 36 per edge
+  vector<uint8_t> fixed;
+  vector<int> H;        // free index or -1
+  int Nf = 0;
+  vector<int> active;
+  BlockSym S;           // upper blocks: diagonal per free pose, one per free pair
+  vector<int> edgeBlk;  // off-diagonal block of each edge or -1
+  vector<int> diagBlk;  // per free pose
+  vector<double> Hdiag; // undamped diagonal blocks
+  vector<double> b, x;
+  double lambda = 0;
+  PCGState pcg;
+  int solver_kind = 1;
+  vector<vector<double>> stackT;
+  vector<vector<Quat>> stackQ;
+};
+
+// error e[6] and, if J != null, J[6*12] = d e / d [d_i(6) d_j(6)] (row-major)
+void pg_edge(const PG& p, int k, double e[6], double* J) {
+  const int i = p.from[k], j = p.to[k];
+  HD v[12];
+  if (J)
+    for (int a = 0; a < 12; ++a) v[a].c[a] = 1.0;
+  auto incr = [&](int vtx, const HD* d, HD tout[3], QT<HD>& qout) {
+    const Quat& q0 = p.q[vtx];
+    QT<HD> qv{hd_const(q0.x), hd_const(q0.y), hd_const(q0.z), hd_const(q0.w)};
+    HD n2 = d[3] * d[3] + d[4] * d[4] + d[5] * d[5];
+    HD w = hd_sqrt(hd_const(1.0) - n2);  // n2 = 0 at the evaluation point
+    QT<HD> dq{d[3], d[4], d[5], w};
+    qout = qt_mul(qv, dq);
+    HD dt[3] = {d[0], d[1], d[2]}, rdt[3];
+    qt_rot(qv, dt, rdt);
+    for (int a = 0; a < 3; ++a) tout[a] = hd_const(p.t[3 * vtx + a]) + rdt[a];
+  };
+  HD ti[3], tj[3];
+  QT<HD> qi, qj;
+  incr(i, v, ti, qi);
+  incr(j, v + 6, tj, qj);
+  const Quat& m = p.mq[k];
+  QT<HD> qz{hd_const(m.x), hd_const(m.y), hd_const(m.z), hd_const(m.w)};
+  QT<HD> qa = qt_mul(qt_conj(qz), qt_mul(qt_conj(qi), qj));
+  HD d[3] = {tj[0] - ti[0], tj[1] - ti[1], tj[2] - ti[2]}, u[3], w3[3], ta[3];
+  qt_rot(qt_conj(qi), d, u);
+  for (int a = 0; a < 3; ++a) w3[a] = u[a] - hd_const(p.mt[3 * k + a]);
+  qt_rot(qt_conj(qz), w3, ta);
+  HD nq = hd_sqrt(qa.x * qa.x + qa.y * qa.y + qa.z * qa.z + qa.w * qa.w);
+  const double sgn = qa.w.a < 0 ? -1.0 : 1.0;
+  HD inv = sgn * hd_inv(nq);
+  HD ev[6] = {ta[0], ta[1], ta[2], qa.x * inv, qa.y * inv, qa.z * inv};
+  for (int r = 0; r < 6; ++r) {
+    e[r] = ev[r].a;
+    if (J)
+      for (int a = 0; a < 12; ++a) J[12 * r + a] = ev[r].c[a];
+  }
+}
+
+void pg_structure(PG& p) {
+  p.H.assign(p.N, -1);
+  p.Nf = 0;
+  for (int i = 0; i < p.N; ++i)
+    if (!p.fixed[i]) p.H[i] = p.Nf++;
+  p.active.clear();
+  for (int k = 0; k < p.E; ++k)
+    if (!(p.fixed[p.from[k]] && p.fixed[p.to[k]])) p.active.push_back(k);
+  p.S = BlockSym();
+  p.S.bd = 6;
+  p.S.nb = p.Nf;
+  p.S.diag.assign(p.Nf, -1);
+  p.diagBlk.assign(p.Nf, -1);
+  for (int i = 0; i < p.Nf; ++i) {
+    p.S.diag[i] = p.diagBlk[i] = static_cast<int>(p.S.row.size());
+    p.S.row.push_back(i);
+    p.S.col.push_back(i);
+  }
+  std::unordered_map<long long, int> pair;
+  p.edgeBlk.assign(p.E, -1);
+  for (int k : p.active) {
+    int a = p.H[p.from[k]], c = p.H[p.to[k]];
+    if (a < 0 || c < 0 || a == c) continue;
+    int r = std::min(a, c), cc = std::max(a, c);
+    long long key = static_cast<long long>(r) * p.Nf + cc;
+    auto it = pair.find(key);
+    if (it == pair.end()) {
+      it = pair.emplace(key, static_cast<int>(p.S.row.size())).first;
+      p.S.row.push_back(r);
+      p.S.col.push_back(cc);
+    }
+    p.edgeBlk[k] = it->second;
+  }
+  p.S.val.assign(p.S.row.size() * 36, 0.0);
+}
+
+double pg_chi2(const PG& p) {
+  double chi = 0;
+  const int n = static_cast<int>(p.active.size());
+#pragma omp parallel for reduction(+ : chi) schedule(static)
+  for (int a = 0; a < n; ++a) {
+    int k = p.active[a];
+    double e[6];
+    pg_edge(p, k, e, nullptr);
+    const double* O = &p.info[36 * static_cast<size_t>(k)];
+    double s = 0;
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) s += e[r] * O[6 * r + c] * e[c];
+    chi += s;
+  }
+  return chi;
+}
+
+// BlockSolver::buildSystem without Schur (block_solver.hpp:492-565, 333-343)
+void pg_build(PG& p) {
+  std::fill(p.S.val.begin(), p.S.val.end(), 0.0);
+  p.b.assign(6 * static_cast<size_t>(p.Nf), 0.0);
+  for (int k : p.active) {
+    double e[6], J[72];
+    pg_edge(p, k, e, J);
+    const double* O = &p.info[36 * static_cast<size_t>(k)];
+    int hv[2] = {p.H[p.from[k]], p.H[p.to[k]]};
+    double OJ[2][36], Oe[6];  // Omega J_v (6x6), Omega e
+    for (int r = 0; r < 6; ++r) {
+      Oe[r] = 0;
+      for (int c = 0; c < 6; ++c) Oe[r] += O[6 * r + c] * e[c];
+    }
+    for (int v = 0; v < 2; ++v)
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+          double s = 0;
+          for (int m = 0; m < 6; ++m) s += O[6 * r + m] * J[12 * m + 6 * v + c];
+          OJ[v][6 * r + c] = s;
+        }
+    for (int v = 0; v < 2; ++v) {
+      if (hv[v] < 0) continue;
+      double* bb = &p.b[6 * static_cast<size_t>(hv[v])];
+      double* D = &p.S.val[36 * static_cast<size_t>(p.diagBlk[hv[v]])];
+      for (int r = 0; r < 6; ++r) {
+        for (int m = 0; m < 6; ++m) bb[r] -= J[12 * m + 6 * v + r] * Oe[m];
+        for (int c = 0; c < 6; ++c) {
+          double s = 0;
+          for (int m = 0; m < 6; ++m) s += J[12 * m + 6 * v + r] * OJ[v][6 * m + c];
+          D[6 * r + c] += s;
+        }
+      }
+    }
+    if (p.edgeBlk[k] >= 0) {
+      // block (min, max): J_a^T Omega J_b with a the lower free index
+      int va = hv[0] < hv[1] ? 0 : 1, vb = 1 - va;
+      double* B = &p.S.val[36 * static_cast<size_t>(p.edgeBlk[k])];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+          double s = 0;
+          for (int m = 0; m < 6; ++m) s += J[12 * m + 6 * va + r] * OJ[vb][6 * m + c];
+          B[6 * r + c] += s;
+        }
+    }
+  }
+  p.Hdiag.resize(36 * static_cast<size_t>(p.Nf));
+  for (int i = 0; i < p.Nf; ++i)
+    std::copy(&p.S.val[36 * static_cast<size_t>(p.diagBlk[i])], &p.S.val[36 * static_cast<size_t>(p.diagBlk[i])] + 36,
+              &p.Hdiag[36 * static_cast<size_t>(i)]);
+}
+
+bool pg_solve(PG& p) {
+  for (int i = 0; i < p.Nf; ++i) {
+    double* D = &p.S.val[36 * static_cast<size_t>(p.diagBlk[i])];
+    for (int c = 0; c < 36; ++c) D[c] = p.Hdiag[36 * static_cast<size_t>(i) + c];
+    for (int r = 0; r < 6; ++r) D[7 * r] += p.lambda;
+  }
+  p.x.assign(6 * static_cast<size_t>(p.Nf), 0.0);
+  return p.solver_kind == 0 ? chol_solve_blocks(p.S, p.b.data(), p.x.data())
+                            : pcg_solve(p.S, p.b.data(), p.x.data(), p.pcg);
+}
+
+// VertexSE3::oplusImpl: X <- X * fromVectorMQT(d)
+void pg_update(PG& p, const double* x) {
+  for (int i = 0; i < p.N; ++i) {
+    int h = p.H[i];
+    if (h < 0) continue;
+    const double* d = &x[6 * static_cast<size_t>(h)];
+    double n2 = d[3] * d[3] + d[4] * d[4] + d[5] * d[5];
+    Quat dq = n2 > 1 ? Quat{0, 0, 0, 1} : Quat{d[3], d[4], d[5], std::sqrt(1 - n2)};
+    double rt[3];
+    qrot(p.q[i], d, rt);
+    for (int a = 0; a < 3; ++a) p.t[3 * i + a] += rt[a];
+    p.q[i] = qmul(p.q[i], dq);
+    qnormalize(p.q[i]);
+  }
+}
+
+// OptimizationAlgorithmLevenberg::solve (optimization_algorithm_levenberg.cpp:
+// 60-153) over a problem P providing chi2(), build() (linearise at the
+// state), lambda_init(), solve(lambda, &pcg_iters) -> ok, update(), scale()
+// (computeScale of the last solve), push(), pop(), discard() and times[6].
+// trace[6 * it + (0..5)] = chi2 before, chi2 after, lambda, trials, PCG
+// iterations, status (1 = Terminate). Returns the iterations run.
+template <class P>
+int lm_optimize(P& p, int iters, double* trace, double* times) {
+  for (int k = 0; k < 6; ++k) p.times[k] = 0;
+  double tall = now_s();
+  double lambda = 0, ni = 2;
+  int it;
+  for (it = 0; it < iters; ++it) {
+    double t0 = now_s();
+    double currentChi = p.chi2();
+    double t1 = now_s();
+    p.times[0] += t1 - t0;
+    p.build();
+    p.times[1] += now_s() - t1;
+    if (it == 0) { lambda = p.lambda_init(); ni = 2; }
+    double chi_before = currentChi;
+    double rho = 0;
+    int q = 0, pcgsum = 0;
+    do {
+      p.push();
+      int pcg_it = 0;
+      bool ok = p.solve(lambda, &pcg_it);
+      pcgsum += pcg_it;
+      double tu = now_s();
+      // A failed solve is rejected below and the state popped, so the update
+      // the reference still applies (levenberg.cpp:112) has no effect.
+      if (ok) p.update();
+      p.times[4] += now_s() - tu;
+      double tc = now_s();
+      double tempChi = p.chi2();
+      p.times[0] += now_s() - tc;
+      if (!ok) tempChi = std::numeric_limits<double>::max();
+      rho = currentChi - tempChi;
+      double scale = ok ? p.scale(lambda) + 1e-3 : 1;
+      rho /= scale;
+      if (rho > 0 && std::isfinite(tempChi) && ok) {
+        double alpha = 1. - std::pow((2 * rho - 1), 3);
+        alpha = std::min(alpha, 2. / 3.);
+        double sf = std::max(1. / 3., alpha);
+        lambda *= sf;
+        ni = 2;
+        currentChi = tempChi;
+        p.discard();
+      } else {
+        lambda *= ni;
+        ni *= 2;
+        p.pop();
+        if (!std::isfinite(lambda)) break;
+      }
+      q++;
+    } while (rho < 0 && q < 10);
+    int status = (q == 10 || rho == 0 || !std::isfinite(lambda)) ? 1 : 0;
+    if (trace) {
+      trace[6 * it + 0] = chi_before;
+      trace[6 * it + 1] = currentChi;
+      trace[6 * it + 2] = lambda;
+      trace[6 * it + 3] = q;
+      trace[6 * it + 4] = pcgsum;
+      trace[6 * it + 5] = status;
+    }
+    if (status) { ++it; break; }
+  }
+  p.times[5] = now_s() - tall;
+  if (times) std::copy(p.times, p.times + 6, times);
+  return it;
+}
+
+// LM adapter of the pose graph
+struct PGLM {
+  PG& p;
+  double times[6] = {0, 0, 0, 0, 0, 0};
+  double chi2() { return pg_chi2(p); }
+  void build() { pg_build(p); }
+  double lambda_init() {
+    double m = 0;
+    for (int i = 0; i < p.Nf; ++i)
+      for (int r = 0; r < 6; ++r) m = std::max(m, std::fabs(p.Hdiag[36 * static_cast<size_t>(i) + 7 * r]));
+    return 1e-5 * m;
+  }
+  bool solve(double lambda, int* pcg_it) {
+    p.lambda = lambda;
+    double t = now_s();
+    bool ok = pg_solve(p);
+    times[3] += now_s() - t;
+    *pcg_it = p.solver_kind == 1 ? p.pcg.last_iters : 0;
+    return ok;
+  }
+  void update() { pg_update(p, p.x.data()); }
+  double scale(double lambda) {
+    double s = 0;
+    for (size_t j = 0; j < p.x.size(); ++j) s += p.x[j] * (lambda * p.x[j] + p.b[j]);
+    return s;
+  }
+  void push() { p.stackT.push_back(p.t); p.stackQ.push_back(p.q); }
+  void pop() { p.t = p.stackT.back(); p.q = p.stackQ.back(); discard(); }
+  void discard() { p.stackT.pop_back(); p.stackQ.pop_back(); }
+};
+
 }  // namespace
 
 // =============================================================== C interface
@@ -1274,6 +1603,92 @@ double orc_edge_energy_derivs(const double* cam7, const double* X3, const double
   double E = 0;
   edge_energy_derivs(se3_from7(cam7), X3, z2, w, intr4, huber_delta, &E, g12, H144);
   return E;
+}
+
+// ------------------------------------------------------------ pose graph API
+// poses7[N*7] [tx ty tz qx qy qz qw], from/to[E], meas7[E*7], info36[E*36]
+// (row-major), fixed[N] (0/1).
+void* orc_pg_new(int N, int E, const double* poses7, const int* from, const int* to, const double* meas7,
+                 const double* info36, const uint8_t* fixed) {
+  PG* p = new PG();
+  p->N = N;
+  p->E = E;
+  p->t.resize(3 * static_cast<size_t>(N));
+  p->q.resize(N);
+  for (int i = 0; i < N; ++i) {
+    for (int a = 0; a < 3; ++a) p->t[3 * i + a] = poses7[7 * static_cast<size_t>(i) + a];
+    const double* qq = poses7 + 7 * static_cast<size_t>(i) + 3;
+    p->q[i] = {qq[0], qq[1], qq[2], qq[3]};
+    qnormalize(p->q[i]);
+  }
+  p->from.assign(from, from + E);
+  p->to.assign(to, to + E);
+  p->mt.resize(3 * static_cast<size_t>(E));
+  p->mq.resize(E);
+  for (int k = 0; k < E; ++k) {
+    for (int a = 0; a < 3; ++a) p->mt[3 * k + a] = meas7[7 * static_cast<size_t>(k) + a];
+    const double* qq = meas7 + 7 * static_cast<size_t>(k) + 3;
+    p->mq[k] = {qq[0], qq[1], qq[2], qq[3]};
+    qnormalize(p->mq[k]);
+  }
+  p->info.assign(info36, info36 + 36 * static_cast<size_t>(E));
+  p->fixed.assign(fixed, fixed + N);
+  pg_structure(*p);
+  return p;
+}
+
+void orc_pg_free(void* h) { delete static_cast<PG*>(h); }
+int orc_pg_nfree(void* h) { return static_cast<PG*>(h)->Nf; }
+int orc_pg_nblocks(void* h) { return static_cast<int>(static_cast<PG*>(h)->S.row.size()); }
+
+void orc_pg_get_state(void* h, double* poses7) {
+  PG* p = static_cast<PG*>(h);
+  for (int i = 0; i < p->N; ++i) {
+    for (int a = 0; a < 3; ++a) poses7[7 * static_cast<size_t>(i) + a] = p->t[3 * i + a];
+    const Quat& q = p->q[i];
+    double* o = poses7 + 7 * static_cast<size_t>(i) + 3;
+    o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+  }
+}
+
+void orc_pg_set_state(void* h, const double* poses7) {
+  PG* p = static_cast<PG*>(h);
+  for (int i = 0; i < p->N; ++i) {
+    for (int a = 0; a < 3; ++a) p->t[3 * i + a] = poses7[7 * static_cast<size_t>(i) + a];
+    const double* qq = poses7 + 7 * static_cast<size_t>(i) + 3;
+    p->q[i] = {qq[0], qq[1], qq[2], qq[3]};
+    qnormalize(p->q[i]);
+  }
+}
+
+double orc_pg_chi2(void* h) { return pg_chi2(*static_cast<PG*>(h)); }
+
+// error e6 and Jacobian J72 = d e / d [d_from(6) d_to(6)] (6x12 row-major) of edge k
+void orc_pg_edge(void* h, int k, double* e6, double* J72) { pg_edge(*static_cast<PG*>(h), k, e6, J72); }
+
+// Undamped system at the state: rows/cols/blocks (count orc_pg_nblocks, upper,
+// 36 row-major each) and b[6 Nf]. Returns chi2.
+double orc_pg_linearize(void* h, int* rows, int* cols, double* blocks, double* b) {
+  PG* p = static_cast<PG*>(h);
+  double chi = pg_chi2(*p);
+  pg_build(*p);
+  for (size_t k = 0; k < p->S.row.size(); ++k) {
+    if (rows) rows[k] = p->S.row[k];
+    if (cols) cols[k] = p->S.col[k];
+  }
+  if (blocks) std::copy(p->S.val.begin(), p->S.val.end(), blocks);
+  if (b) std::copy(p->b.begin(), p->b.end(), b);
+  return chi;
+}
+
+int orc_pg_optimize(void* h, int iters, int solver, double tol, int maxiter, double* trace, double* times) {
+  PG* p = static_cast<PG*>(h);
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  PGLM lm{*p};
+  return lm_optimize(lm, iters, trace, times);
 }
 
 // -------------------------------------------------------- single functions

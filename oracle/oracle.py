@@ -73,6 +73,22 @@ def lib():
         L.orc_ba_exact_hessian.argtypes = [ct.c_void_p, _dp, _dp, _dp]
         L.orc_edge_energy_derivs.restype = ct.c_double
         L.orc_edge_energy_derivs.argtypes = [_dp, _dp, _dp, ct.c_double, _dp, ct.c_double, _dp, _dp]
+        L.orc_pg_new.restype = ct.c_void_p
+        L.orc_pg_new.argtypes = [ct.c_int, ct.c_int, _dp, _ip, _ip, _dp, _dp, _up]
+        L.orc_pg_free.argtypes = [ct.c_void_p]
+        L.orc_pg_nfree.restype = ct.c_int
+        L.orc_pg_nfree.argtypes = [ct.c_void_p]
+        L.orc_pg_nblocks.restype = ct.c_int
+        L.orc_pg_nblocks.argtypes = [ct.c_void_p]
+        L.orc_pg_get_state.argtypes = [ct.c_void_p, _dp]
+        L.orc_pg_set_state.argtypes = [ct.c_void_p, _dp]
+        L.orc_pg_chi2.restype = ct.c_double
+        L.orc_pg_chi2.argtypes = [ct.c_void_p]
+        L.orc_pg_edge.argtypes = [ct.c_void_p, ct.c_int, _dp, _dp]
+        L.orc_pg_linearize.restype = ct.c_double
+        L.orc_pg_linearize.argtypes = [ct.c_void_p, _ip, _ip, _dp, _dp]
+        L.orc_pg_optimize.restype = ct.c_int
+        L.orc_pg_optimize.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp, _dp]
         L.orc_num_threads.restype = ct.c_int
         L.orc_set_num_threads.argtypes = [ct.c_int]
         _lib = L
@@ -193,6 +209,62 @@ class OracleBA:
         lib().orc_ba_exact_hessian(self.h, Hpp.ctypes.data_as(_dp), Hll.ctypes.data_as(_dp),
                                    Hpl.ctypes.data_as(_dp))
         return Hpp, Hll, Hpl
+
+
+class OraclePG:
+    """FP64 CPU restatement of g2o's VertexSE3 / EdgeSE3 pose graph with LM
+    over 6x6 blocks (no Schur)."""
+
+    def __init__(self, prob):
+        L = lib()
+        self._keep = [np.ascontiguousarray(a, dtype=d) for a, d in (
+            (prob.poses, np.float64), (prob.edge_from, np.int32), (prob.edge_to, np.int32),
+            (prob.meas, np.float64), (prob.info, np.float64), (prob.fixed, np.uint8))]
+        po, fr, to, me, inf, fx = self._keep
+        self.N, self.E = prob.num_poses, prob.num_edges
+        self.h = L.orc_pg_new(self.N, self.E, po.ctypes.data_as(_dp), fr.ctypes.data_as(_ip), to.ctypes.data_as(_ip),
+                              me.ctypes.data_as(_dp), inf.ctypes.data_as(_dp), fx.ctypes.data_as(_up))
+        self.Nf = L.orc_pg_nfree(self.h)
+        self.nblocks = L.orc_pg_nblocks(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_pg_free(self.h)
+            self.h = None
+
+    def state(self):
+        out = np.zeros((self.N, 7))
+        lib().orc_pg_get_state(self.h, out.ctypes.data_as(_dp))
+        return out
+
+    def set_state(self, poses):
+        a, p = _d(poses)
+        lib().orc_pg_set_state(self.h, p)
+
+    def chi2(self) -> float:
+        return lib().orc_pg_chi2(self.h)
+
+    def edge(self, k: int):
+        e = np.zeros(6)
+        J = np.zeros((6, 12))
+        lib().orc_pg_edge(self.h, int(k), e.ctypes.data_as(_dp), J.ctypes.data_as(_dp))
+        return e, J
+
+    def linearize(self):
+        rows = np.zeros(self.nblocks, dtype=np.int32)
+        cols = np.zeros(self.nblocks, dtype=np.int32)
+        blocks = np.zeros((self.nblocks, 6, 6))
+        b = np.zeros(6 * self.Nf)
+        chi = lib().orc_pg_linearize(self.h, rows.ctypes.data_as(_ip), cols.ctypes.data_as(_ip),
+                                     blocks.ctypes.data_as(_dp), b.ctypes.data_as(_dp))
+        return chi, rows, cols, blocks, b
+
+    def optimize(self, iters: int, solver: int = 1, tol: float = 1e-8, maxiter: int = -1):
+        trace = np.zeros((iters, 6))
+        times = np.zeros(6)
+        n = lib().orc_pg_optimize(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp),
+                                  times.ctypes.data_as(_dp))
+        return n, trace[:n], times
 
 
 def edge_energy_derivs(cam7, X, z, w, intr4, huber_delta):
