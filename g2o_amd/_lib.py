@@ -25,7 +25,8 @@ EXPORTS = [
     "g2ohip_get_b", "g2ohip_update", "g2ohip_push", "g2ohip_pop", "g2ohip_discard_top", "g2ohip_lambda_init",
     "g2ohip_compute_scale", "g2ohip_optimize", "g2ohip_set_force_stop_flag", "g2ohip_ba_get_hessian",
     "g2ohip_ba_get_schur", "g2ohip_last_solve_info", "g2ohip_set_profiling", "g2ohip_get_profile",
-    "g2ohip_linear_solve_pcg", "g2ohip_ba_adjoint", "g2ohip_ba_newton",
+    "g2ohip_linear_solve_pcg", "g2ohip_pg_set_structure", "g2ohip_pg_structure_info", "g2ohip_pg_set_measurements",
+    "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_ba_adjoint", "g2ohip_ba_newton",
 ]
 K_LINEARIZE, K_SCHUR, K_PCG, K_REFINE, K_CHI2, K_UPDATE = range(6)
 KERNEL_NAMES = ["linearize", "schur", "pcg", "refine", "chi2", "update"]
@@ -100,6 +101,12 @@ def lib():
                                                ct.c_double, ct.c_int, _dp, _ip]),
         "g2ohip_ba_adjoint": (ct.c_int, [_vp, _dp, _dp, _dp, _dp, ct.c_double, ct.c_int, _ip]),
         "g2ohip_ba_newton": (ct.c_int, [_vp, ct.c_int, ct.c_double, _ip]),
+        "g2ohip_pg_set_structure": (ct.c_int, [_vp, ct.c_int, ct.c_int, _ip, _ip, _up]),
+        "g2ohip_pg_structure_info": (ct.c_int, [_vp, ct.POINTER(ct.c_longlong)]),
+        "g2ohip_pg_set_measurements": (ct.c_int, [_vp, _dp, _dp]),
+        "g2ohip_pg_set_state": (ct.c_int, [_vp, _dp]),
+        "g2ohip_pg_get_state": (ct.c_int, [_vp, _dp]),
+        "g2ohip_pg_get_system": (ct.c_int, [_vp, _ip, _ip, _dp, _dp]),
     }
     for name in EXPORTS:
         fn = getattr(L, name)

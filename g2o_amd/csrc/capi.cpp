@@ -160,7 +160,7 @@ int g2ohip_solve(g2ohip_ctx* ctx, int* ok) {
 int g2ohip_get_x(g2ohip_ctx* ctx, double* x) {
   G2OHIP_ENTER(ctx);
   if (!x || !ctx->have_solution) return ctx->fail(G2OHIP_ERR_INVALID, "no solution");
-  size_t n = 6 * static_cast<size_t>(ctx->st.Cf) + 3 * static_cast<size_t>(ctx->st.Pf);
+  size_t n = ctx->system_size();
   if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
       hipMemcpy(x, ctx->d_x.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     return ctx->fail(G2OHIP_ERR_HIP, "copy of x failed");
@@ -170,7 +170,7 @@ int g2ohip_get_x(g2ohip_ctx* ctx, double* x) {
 int g2ohip_get_b(g2ohip_ctx* ctx, double* b) {
   G2OHIP_ENTER(ctx);
   if (!b || !ctx->have_system) return ctx->fail(G2OHIP_ERR_INVALID, "no system");
-  size_t n = 6 * static_cast<size_t>(ctx->st.Cf) + 3 * static_cast<size_t>(ctx->st.Pf);
+  size_t n = ctx->system_size();
   if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
       hipMemcpy(b, ctx->d_b.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     return ctx->fail(G2OHIP_ERR_HIP, "copy of b failed");
@@ -267,6 +267,46 @@ int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, int* ok) {
   G2OHIP_ENTER(ctx);
   if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "ok must not be NULL");
   return ctx->newton(steps, tol < 0 ? 1e-24 : tol, ok);
+}
+
+// ------------------------------------------------------------ pose graph
+int g2ohip_pg_set_structure(g2ohip_ctx* ctx, int num_poses, int num_edges, const int* edge_from, const int* edge_to,
+                            const uint8_t* fixed) {
+  G2OHIP_ENTER(ctx);
+  return ctx->pg_set_structure(num_poses, num_edges, edge_from, edge_to, fixed);
+}
+
+int g2ohip_pg_structure_info(g2ohip_ctx* ctx, long long* info) {
+  G2OHIP_ENTER(ctx);
+  if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  if (!ctx->have_structure || ctx->mode != 1) return ctx->fail(G2OHIP_ERR_INVALID, "no pose-graph structure");
+  info[0] = ctx->pgst.Nf;
+  info[1] = static_cast<long long>(ctx->pgst.act.size());
+  info[2] = ctx->pgst.nU;
+  info[3] = ctx->pgst.nF;
+  return G2OHIP_OK;
+}
+
+int g2ohip_pg_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* information) {
+  G2OHIP_ENTER(ctx);
+  return ctx->pg_set_measurements(meas, information);
+}
+
+int g2ohip_pg_set_state(g2ohip_ctx* ctx, const double* poses) {
+  G2OHIP_ENTER(ctx);
+  if (!poses) return ctx->fail(G2OHIP_ERR_INVALID, "null poses");
+  return ctx->pg_set_state(poses);
+}
+
+int g2ohip_pg_get_state(g2ohip_ctx* ctx, double* poses) {
+  G2OHIP_ENTER(ctx);
+  if (!poses) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+  return ctx->pg_get_state(poses);
+}
+
+int g2ohip_pg_get_system(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* b) {
+  G2OHIP_ENTER(ctx);
+  return ctx->pg_get_system(rows, cols, blocks, b);
 }
 
 }  // extern "C"

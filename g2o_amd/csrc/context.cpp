@@ -1,6 +1,7 @@
 // Device context: structure upload, the g2o Solver operations on the device
 // and the Levenberg-Marquardt loop (see context.h).
 #include "context.h"
+#include "pcg_drive.h"
 
 #include <algorithm>
 #include <chrono>
@@ -51,12 +52,6 @@ int g2ohip_ctx::hip(hipError_t e, const char* what) {
   return fail(G2OHIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define CK(...)                                  \
-  do {                                           \
-    int rc_ = (__VA_ARGS__);                     \
-    if (rc_ != G2OHIP_OK) return rc_;            \
-  } while (0)
-#define HK(...) CK(hip((__VA_ARGS__), #__VA_ARGS__))
 
 int g2ohip_ctx::read_scalar(const double* dptr, double* out) {
   HK(hipMemcpyAsync(h_pinned, dptr, sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -117,12 +112,6 @@ int g2ohip_ctx::get_profile(int kernel, double* total_ms, int* launches) {
   return G2OHIP_OK;
 }
 
-struct ProfScope {
-  g2ohip_ctx* c;
-  int k, b;
-  ProfScope(g2ohip_ctx* c_, int k_) : c(c_), k(k_), b(c_->prof_begin()) {}
-  ~ProfScope() { c->prof_end(k, b); }
-};
 
 // ------------------------------------------------------------- structure
 int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep, const uint8_t* cf,
@@ -137,7 +126,8 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
                 msg);
   }
   have_structure = true;
-  have_meas = have_state = have_system = have_solution = false;
+  mode = 0;
+  have_meas = have_state = have_system = have_solution = have_exact = false;
   rows_ready = false;
   pcg_residual_mem = -1.0;
   stack_depth = 0;
@@ -235,6 +225,7 @@ int g2ohip_ctx::ensure_rows() {
 }
 
 int g2ohip_ctx::set_measurements(const double* meas, const double* w, const double* intr, double huber) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
   if (!meas || !intr) return fail(G2OHIP_ERR_INVALID, "measurements and intrinsics are required");
   CK(require_device());
@@ -260,6 +251,7 @@ int g2ohip_ctx::set_measurements(const double* meas, const double* w, const doub
 }
 
 int g2ohip_ctx::set_state(const double* cams, const double* pts) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
   CK(require_device());
   std::vector<double> c(cams, cams + 7 * static_cast<size_t>(st.C));
@@ -281,6 +273,7 @@ int g2ohip_ctx::set_state(const double* cams, const double* pts) {
 }
 
 int g2ohip_ctx::get_state(double* cams, double* pts) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   if (!have_state) return fail(G2OHIP_ERR_INVALID, "no state");
   if (cams) HK(hipMemcpyAsync(cams, d_cams.p, 7 * sizeof(double) * st.C, hipMemcpyDeviceToHost, stream));
   if (pts && st.P > 0) HK(hipMemcpyAsync(pts, d_pts.p, 3 * sizeof(double) * st.P, hipMemcpyDeviceToHost, stream));
@@ -290,6 +283,10 @@ int g2ohip_ctx::get_state(double* cams, double* pts) {
 
 // ------------------------------------------------------------- residuals
 int g2ohip_ctx::compute_active_errors(double* chi2, double* err_out) {
+  if (mode == 1) {
+    if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+    return pg_chi2(chi2, err_out);
+  }
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
   DBuf<double> eslots;
@@ -313,6 +310,10 @@ int g2ohip_ctx::compute_active_errors(double* chi2, double* err_out) {
 }
 
 int g2ohip_ctx::build_system() {
+  if (mode == 1) {
+    if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+    return pg_build();
+  }
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
   const int prec = opt.precision;
@@ -337,6 +338,10 @@ int g2ohip_ctx::build_system() {
 }
 
 int g2ohip_ctx::lambda_init(double* out) {
+  if (mode == 1) {
+    if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+    return pg_lambda_init(out);
+  }
   if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
   if (opt.user_lambda_init > 0) { *out = opt.user_lambda_init; return G2OHIP_OK; }
   HK(launch_max_diag(g, d_Hpp.p, d_Hll.p, d_red.p, d_scal.p + 2, stream));
@@ -348,60 +353,12 @@ int g2ohip_ctx::lambda_init(double* out) {
 
 int g2ohip_ctx::compute_scale(double* out) {
   if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
-  const int n = 6 * st.Cf + 3 * st.Pf;
+  const int n = static_cast<int>(system_size());
   HK(launch_scale(n, d_x.p, d_b.p, lambda, d_red.p, d_scal.p + 3, stream));
   return read_scalar(d_scal.p + 3, out);
 }
 
 // ------------------------------------------------------------------- PCG
-namespace {
-
-// One LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) on the device:
-// init, then `op` (direction + operator) and the update kernel per
-// iteration. The stop test runs on the device and later launches return at
-// once, so the host only polls the scalars: first after `hint` iterations
-// (the previous solve's count), then every 4. If the loop reaches max_iter
-// the head runs once more to record the final residual.
-template <typename HT, int BD, typename OpFn>
-hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const double* rhs, const HT* J, HT* x, HT* r,
-                     HT* s, HT* const dirs[2], HT* q, double* part_rs, double* part_dq, int n_dq,
-                     g2ohip::PcgScalars* sc, int max_iter, int hint, OpFn op, g2ohip::PcgRun* out) {
-  using namespace g2ohip;
-  hipError_t e;
-  PcgScalars hs{};
-  auto poll = [&]() -> hipError_t {
-    hipError_t e2 = hipMemcpyAsync(h_pinned, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream);
-    if (e2 == hipSuccess) e2 = hipStreamSynchronize(stream);
-    if (e2 == hipSuccess) std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
-    return e2;
-  };
-  if ((e = hipMemsetAsync(sc, 0, sizeof(PcgScalars), stream)) != hipSuccess) return e;
-  if ((e = launch_pcg_init<HT, BD>(nb, rhs, J, x, r, s, part_rs, stream)) != hipSuccess) return e;
-  int next = std::max(4, hint);
-  bool done = false;
-  for (int it = 0; it < max_iter; ++it) {
-    if ((e = op(it, 0, dirs[(it + 1) & 1], dirs[it & 1])) != hipSuccess) return e;
-    if ((e = launch_pcg_update<HT, BD>(nb, J, x, r, dirs[it & 1], q, s, part_dq, n_dq, part_rs, sc, it, stream)) !=
-        hipSuccess)
-      return e;
-    if (it + 1 == next) {
-      if ((e = poll()) != hipSuccess) return e;
-      if (hs.done) { done = true; break; }
-      next += 4;
-    }
-  }
-  if (!done) {
-    if ((e = op(max_iter, 1, dirs[(max_iter + 1) & 1], dirs[max_iter & 1])) != hipSuccess) return e;
-    if ((e = poll()) != hipSuccess) return e;
-  }
-  out->iters = hs.iters;
-  out->dn_final = hs.dn[hs.iters & 1];
-  out->d0 = hs.d0;
-  out->dn0 = hs.dn0;
-  return hipSuccess;
-}
-
-}  // namespace
 
 template <typename HT>
 int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ctl, bool implicit, int max_iter,
@@ -522,6 +479,10 @@ int g2ohip_ctx::solve_impl(int* ok) {
 }
 
 int g2ohip_ctx::solve(int* ok) {
+  if (mode == 1) {
+    if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+    return pg_solve(ok);
+  }
   if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
   if (system_prec == G2OHIP_FP32) return solve_impl<float>(ok);
   return solve_impl<double>(ok);
@@ -529,6 +490,10 @@ int g2ohip_ctx::solve(int* ok) {
 
 // --------------------------------------------------------- update / stack
 int g2ohip_ctx::update() {
+  if (mode == 1) {
+    if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
+    return pg_update();
+  }
   if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
   {
     ProfScope ps(this, G2OHIP_K_UPDATE);
@@ -547,10 +512,10 @@ int g2ohip_ctx::push() {
   }
   DBuf<double>& c = *stack_c[stack_depth];
   DBuf<double>& p = *stack_p[stack_depth];
-  HK(c.alloc(7 * static_cast<size_t>(st.C)));
-  HK(p.alloc(3 * static_cast<size_t>(std::max(st.P, 1))));
-  HK(launch_copy(c.p, d_cams.p, 7 * sizeof(double) * st.C, stream));
-  if (st.P) HK(launch_copy(p.p, d_pts.p, 3 * sizeof(double) * st.P, stream));
+  HK(c.alloc(7 * state7()));
+  HK(p.alloc(3 * std::max<size_t>(state3(), 1)));
+  HK(launch_copy(c.p, d_cams.p, 7 * sizeof(double) * state7(), stream));
+  if (state3()) HK(launch_copy(p.p, d_pts.p, 3 * sizeof(double) * state3(), stream));
   stack_depth++;
   return G2OHIP_OK;
 }
@@ -558,8 +523,8 @@ int g2ohip_ctx::push() {
 int g2ohip_ctx::pop() {
   if (stack_depth <= 0) return fail(G2OHIP_ERR_INVALID, "state stack is empty");
   stack_depth--;
-  HK(launch_copy(d_cams.p, stack_c[stack_depth]->p, 7 * sizeof(double) * st.C, stream));
-  if (st.P) HK(launch_copy(d_pts.p, stack_p[stack_depth]->p, 3 * sizeof(double) * st.P, stream));
+  HK(launch_copy(d_cams.p, stack_c[stack_depth]->p, 7 * sizeof(double) * state7(), stream));
+  if (state3()) HK(launch_copy(d_pts.p, stack_p[stack_depth]->p, 3 * sizeof(double) * state3(), stream));
   return G2OHIP_OK;
 }
 
@@ -652,6 +617,7 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
 
 // --------------------------------------------------------------- inspection
 int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   if (!have_system && !have_exact) return fail(G2OHIP_ERR_INVALID, "build_system first");
   HK(hipStreamSynchronize(stream));
   if (Hpp) HK(hipMemcpy(Hpp, d_Hpp.p, 36 * sizeof(double) * st.Cf, hipMemcpyDeviceToHost));
@@ -682,6 +648,7 @@ int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
 }
 
 int g2ohip_ctx::get_schur(int* rows, int* cols, double* blocks, double* bschur) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
   if (opt.schur_mode != G2OHIP_SCHUR_EXPLICIT) return fail(G2OHIP_ERR_INVALID, "explicit Schur mode only");
   HK(hipStreamSynchronize(stream));
@@ -842,6 +809,7 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
 // accepted step mu drops by 10 (to 0 below 1e-15 * max diag). At most 30
 // trials per step; *ok = 0 if none was acceptable.
 int g2ohip_ctx::newton(int steps, double tol, int* ok) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   CK(require_device());
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   *ok = 1;
@@ -888,6 +856,7 @@ int g2ohip_ctx::newton(int steps, double tol, int* ok) {
 
 int g2ohip_ctx::adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,
                         int max_iter, int* iterations) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   CK(require_device());
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   if (!dL_dcam || !dL_dmeas) return fail(G2OHIP_ERR_INVALID, "dL_dcam and dL_dmeas are required");

@@ -75,6 +75,7 @@ struct g2ohip_ctx {
   const volatile int* force_stop = nullptr;
 
   g2ohip::BAStructure st;
+  int mode = 0;  // 0: bundle adjustment, 1: 3D pose graph (the last *_set_structure)
   bool have_structure = false, have_meas = false, have_state = false;
   bool have_system = false;   // build_system ran since the last state change
   int system_prec = 0;        // precision of the last build_system
@@ -107,6 +108,13 @@ struct g2ohip_ctx {
   g2ohip::PrecBufs<float> f32;
   g2ohip::PrecBufs<double> f64;
   double* h_pinned = nullptr;  // small pinned host buffer for scalars
+
+  // 3D pose graph (posegraph.cpp)
+  g2ohip::PGStructure pgst;
+  g2ohip::DevPG pg{};
+  g2ohip::DBuf<int> d_pg_H, d_pg_act, d_pg_from, d_pg_to, d_pg_swap, d_pg_end_ptr, d_pg_end_ref, d_pg_blk_ptr,
+      d_pg_blk_ref, d_pg_up_pos, d_pg_up_posT, d_pg_F_diag, d_pg_F_ptr, d_pg_F_col;
+  g2ohip::DBuf<double> d_pg_meas, d_pg_info, d_pg_rec, d_pg_F_val, d_pg_Hdiag, d_pg_part;
 
   // kernel timing (HIP events on `stream`)
   bool profiling = false;
@@ -146,6 +154,23 @@ struct g2ohip_ctx {
   int adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,
               int max_iter, int* iterations);
 
+  // state sizes saved by push / pop: 7 doubles per camera or pose, 3 per point
+  size_t state7() const { return mode == 1 ? static_cast<size_t>(pgst.N) : static_cast<size_t>(st.C); }
+  size_t state3() const { return mode == 1 ? 0 : static_cast<size_t>(st.P); }
+  size_t system_size() const {
+    return mode == 1 ? 6 * static_cast<size_t>(pgst.Nf) : 6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);
+  }
+  int pg_set_structure(int N, int E, const int* from, const int* to, const uint8_t* fixed);
+  int pg_set_measurements(const double* meas7, const double* info36);
+  int pg_set_state(const double* poses7);
+  int pg_get_state(double* poses7);
+  int pg_chi2(double* chi2, double* err_out);
+  int pg_build();
+  int pg_solve(int* ok);
+  int pg_update();
+  int pg_lambda_init(double* out);
+  int pg_get_system(int* rows, int* cols, double* blocks, double* b);
+
   int exact_system();
   int solve_exact(double lam, double tol, int max_iter, int* iters, int* ok);
   int newton(int steps, double tol, int* ok);
@@ -157,4 +182,20 @@ struct g2ohip_ctx {
               int hint, g2ohip::PcgRun* out);
   int read_scalar(const double* dptr, double* out);
   int read_err(int* out);
+};
+
+// status-propagating helpers of the context implementation files
+#define CK(...)                                  \
+  do {                                           \
+    int rc_ = (__VA_ARGS__);                     \
+    if (rc_ != G2OHIP_OK) return rc_;            \
+  } while (0)
+#define HK(...) CK(hip((__VA_ARGS__), #__VA_ARGS__))
+
+// HIP-event timing of one kernel group (no-op unless profiling is on)
+struct ProfScope {
+  g2ohip_ctx* c;
+  int k, b;
+  ProfScope(g2ohip_ctx* c_, int k_) : c(c_), k(k_), b(c_->prof_begin()) {}
+  ~ProfScope() { c->prof_end(k, b); }
 };

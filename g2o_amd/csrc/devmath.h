@@ -219,6 +219,72 @@ __host__ __device__ inline void edge_exact_hessian(const double* R, double fx, d
     }
 }
 
+// ------------------------------------------------------------ pose graph
+// EdgeSE3::computeError (edge_se3.cpp:60-66): e = toVectorMQT(Z^-1 Xi^-1 Xj)
+// = [t_E; s vec(q_E)] with s = sign(w_E) (quaternion normalised to w >= 0,
+// isometry3d_mappings.cpp:40-46, 80-85), and its Jacobians with respect to the
+// right-multiplied increments Xi fromVectorMQT(d_i), Xj fromVectorMQT(d_j)
+// (vertex_se3.h:99-108), d = [dt; dq], at d = 0. Poses are [t; q] with
+// q = (x, y, z, w). With u = Rz^T dq (to first order, Z^-1 (Xi D)^-1 Xj =
+// (I - 2[u]x, -Rz^T (dt + 2 dq x tz)) E and Xi^-1 (Xj D) = E D):
+//   J_j = [[R_E, 0], [0, s (w_E I + [v_E]x)]]
+//   J_i = [[-Rz^T, 2 ([t_E]x Rz^T + Rz^T [tz]x)], [0, s (-w_E I + [v_E]x) Rz^T]]
+// Ji, Jj row-major 6x6.
+__host__ __device__ inline void pg_quat_mul(const double* a, const double* b, double* o) {
+  o[0] = a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1];
+  o[1] = a[3] * b[1] - a[0] * b[2] + a[1] * b[3] + a[2] * b[0];
+  o[2] = a[3] * b[2] + a[0] * b[1] - a[1] * b[0] + a[2] * b[3];
+  o[3] = a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2];
+}
+__host__ __device__ inline void pg_quat_R(const double* q, double* R) {
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - z * w); R[2] = 2 * (x * z + y * w);
+  R[3] = 2 * (x * y + z * w); R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - x * w);
+  R[6] = 2 * (x * z - y * w); R[7] = 2 * (y * z + x * w); R[8] = 1 - 2 * (x * x + y * y);
+}
+__host__ __device__ inline void pg_edge_error(const double* Xi, const double* Xj, const double* Z, double* e,
+                                              double* Ji, double* Jj) {
+  const double qi_c[4] = {-Xi[3], -Xi[4], -Xi[5], Xi[6]};
+  const double qz_c[4] = {-Z[3], -Z[4], -Z[5], Z[6]};
+  double tmp[4], qe[4];
+  pg_quat_mul(qi_c, Xj + 3, tmp);
+  pg_quat_mul(qz_c, tmp, qe);
+  const double nq = sqrt(qe[0] * qe[0] + qe[1] * qe[1] + qe[2] * qe[2] + qe[3] * qe[3]);
+  for (int k = 0; k < 4; ++k) qe[k] /= nq;
+  double Ri[9], Rz[9];
+  pg_quat_R(Xi + 3, Ri);
+  pg_quat_R(Z + 3, Rz);
+  const double d[3] = {Xj[0] - Xi[0], Xj[1] - Xi[1], Xj[2] - Xi[2]};
+  double u[3], te[3];
+  for (int r = 0; r < 3; ++r) u[r] = Ri[r] * d[0] + Ri[3 + r] * d[1] + Ri[6 + r] * d[2] - Z[r];
+  for (int r = 0; r < 3; ++r) te[r] = Rz[r] * u[0] + Rz[3 + r] * u[1] + Rz[6 + r] * u[2];
+  const double s = qe[3] < 0 ? -1.0 : 1.0;
+  for (int r = 0; r < 3; ++r) { e[r] = te[r]; e[3 + r] = s * qe[r]; }
+  if (!Ji) return;
+  double Re[9];
+  pg_quat_R(qe, Re);
+  const double w = qe[3];
+  const double vx[9] = {0, -qe[2], qe[1], qe[2], 0, -qe[0], -qe[1], qe[0], 0};  // [v_E]x
+  const double tx[9] = {0, -te[2], te[1], te[2], 0, -te[0], -te[1], te[0], 0};  // [t_E]x
+  const double zx[9] = {0, -Z[2], Z[1], Z[2], 0, -Z[0], -Z[1], Z[0], 0};         // [tz]x
+  for (int k = 0; k < 36; ++k) { Ji[k] = 0; Jj[k] = 0; }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      Jj[6 * r + c] = Re[3 * r + c];
+      Jj[6 * (3 + r) + 3 + c] = s * ((r == c ? w : 0.0) + vx[3 * r + c]);
+      const double RzT = Rz[3 * c + r];
+      Ji[6 * r + c] = -RzT;
+      double a = 0, bq = 0, m = 0;
+      for (int k = 0; k < 3; ++k) {
+        a += tx[3 * r + k] * Rz[3 * c + k];  // [t_E]x Rz^T
+        bq += Rz[3 * k + r] * zx[3 * k + c];  // Rz^T [tz]x
+        m += ((r == k ? -w : 0.0) + vx[3 * r + k]) * Rz[3 * c + k];
+      }
+      Ji[6 * r + 3 + c] = 2.0 * (a + bq);
+      Ji[6 * (3 + r) + 3 + c] = s * m;
+    }
+}
+
 // 3x3 inverse by cofactors; symmetric input given by its 6 unique entries
 // [xx xy xz yy yz zz] plus lambda on the diagonal. Returns false if singular.
 __host__ __device__ inline bool inv3_sym(const double* h, double lambda, double* out) {

@@ -136,6 +136,23 @@ hipError_t launch_backsub_blocks(const DevGraph& g, const HT* Hpl, const HT* Din
                                  hipStream_t st);
 hipError_t launch_adjoint_out(const DevGraph& g, const double* camRt, const double* pts, const double* lam,
                               double* out, hipStream_t st);
+// ---- 3D pose graph, posegraph.hip
+constexpr int PG_REC = 120;  // doubles per active edge record
+struct DevPG {
+  int N, Nf, A, nU;
+  const int *H, *act, *from, *to, *edge_swap;
+  const int *end_ptr, *end_ref, *blk_ptr, *blk_ref, *up_pos, *up_posT, *F_diag;
+  const double *meas, *info;  // 7 and 36 per edge
+};
+int pg_edge_blocks(int A);
+hipError_t launch_pg_edges(const DevPG& pg, const double* poses, double* rec, double* part, hipStream_t st);
+hipError_t launch_pg_assemble(const DevPG& pg, const double* rec, double* F_val, double* Hdiag, double* b,
+                              hipStream_t st);
+hipError_t launch_pg_damp(const DevPG& pg, const double* Hdiag, double lambda, double* F_val, hipStream_t st);
+hipError_t launch_pg_update(const DevPG& pg, const double* x, double* poses, hipStream_t st);
+hipError_t launch_pg_chi2(const DevPG& pg, const double* poses, double* part, double* err_out, hipStream_t st);
+hipError_t launch_pg_max_diag(int Nf, const double* Hdiag, double* out, hipStream_t st);
+
 template <typename HT>
 hipError_t launch_to_double(int n, const HT* src, double* dst, double scale_add, hipStream_t st);
 hipError_t launch_axpy_d(int n, double a, const double* x, double* y, hipStream_t st);

@@ -305,4 +305,90 @@ std::string build_row_plan(BAStructure& st, int max_cols, int chunk, int num_xcd
   return "";
 }
 
+std::string build_pg_structure(PGStructure& st, int N, int E, const int* from, const int* to, const uint8_t* fixed) {
+  st = PGStructure();
+  if (N <= 0 || E < 0) return "invalid pose graph size";
+  st.N = N;
+  st.E = E;
+  st.H.assign(N, -1);
+  for (int i = 0; i < N; ++i)
+    if (!(fixed && fixed[i])) st.H[i] = st.Nf++;
+  if (st.Nf == 0) return "every pose is fixed";
+  for (int k = 0; k < E; ++k) {
+    if (from[k] < 0 || from[k] >= N || to[k] < 0 || to[k] >= N) return "edge vertex index out of range";
+    if (from[k] == to[k]) return "edge joins a pose to itself";
+    if (st.H[from[k]] >= 0 || st.H[to[k]] >= 0) st.act.push_back(k);
+  }
+  const int A = static_cast<int>(st.act.size());
+  // upper pattern: diagonal, then distinct pairs sorted by (row, col)
+  std::vector<std::pair<long long, int>> keys;  // (row * Nf + col, active position)
+  st.edge_blk.assign(A, -1);
+  st.edge_swap.assign(A, 0);
+  for (int a = 0; a < A; ++a) {
+    const int k = st.act[a];
+    const int hi = st.H[from[k]], hj = st.H[to[k]];
+    if (hi < 0 || hj < 0) continue;
+    st.edge_swap[a] = hi > hj ? 1 : 0;
+    const int r = std::min(hi, hj), c = std::max(hi, hj);
+    keys.push_back({static_cast<long long>(r) * st.Nf + c, a});
+  }
+  std::sort(keys.begin(), keys.end());
+  st.U_row.resize(st.Nf);
+  st.U_col.resize(st.Nf);
+  for (int i = 0; i < st.Nf; ++i) st.U_row[i] = st.U_col[i] = i;
+  st.blk_ptr.push_back(0);
+  for (size_t q = 0; q < keys.size(); ++q) {
+    if (q == 0 || keys[q].first != keys[q - 1].first) {
+      if (q) st.blk_ptr.push_back(static_cast<int>(st.blk_ref.size()));
+      st.U_row.push_back(static_cast<int>(keys[q].first / st.Nf));
+      st.U_col.push_back(static_cast<int>(keys[q].first % st.Nf));
+    }
+    st.edge_blk[keys[q].second] = static_cast<int>(st.U_row.size()) - 1;
+    st.blk_ref.push_back(keys[q].second);
+  }
+  st.blk_ptr.push_back(static_cast<int>(st.blk_ref.size()));
+  if (keys.empty()) st.blk_ptr.assign(1, 0);
+  st.nU = static_cast<int>(st.U_row.size());
+  // edge ends per free pose, in active-edge order
+  st.end_ptr.assign(st.Nf + 1, 0);
+  for (int a = 0; a < A; ++a) {
+    const int k = st.act[a];
+    if (st.H[from[k]] >= 0) st.end_ptr[st.H[from[k]] + 1]++;
+    if (st.H[to[k]] >= 0) st.end_ptr[st.H[to[k]] + 1]++;
+  }
+  for (int i = 0; i < st.Nf; ++i) st.end_ptr[i + 1] += st.end_ptr[i];
+  st.end_ref.resize(st.end_ptr[st.Nf]);
+  {
+    std::vector<int> fill(st.end_ptr.begin(), st.end_ptr.end() - 1);
+    for (int a = 0; a < A; ++a) {
+      const int k = st.act[a];
+      if (st.H[from[k]] >= 0) st.end_ref[fill[st.H[from[k]]]++] = 2 * a;
+      if (st.H[to[k]] >= 0) st.end_ref[fill[st.H[to[k]]]++] = 2 * a + 1;
+    }
+  }
+  // full block rows
+  std::vector<std::vector<std::pair<int, int>>> rows(st.Nf);  // (col, upper block, transposed?)
+  for (int k = 0; k < st.nU; ++k) {
+    rows[st.U_row[k]].push_back({st.U_col[k], 2 * k});
+    if (st.U_row[k] != st.U_col[k]) rows[st.U_col[k]].push_back({st.U_row[k], 2 * k + 1});
+  }
+  st.F_ptr.assign(st.Nf + 1, 0);
+  st.up_pos.assign(st.nU, -1);
+  st.up_posT.assign(st.nU, -1);
+  st.F_diag.assign(st.Nf, -1);
+  for (int i = 0; i < st.Nf; ++i) {
+    std::sort(rows[i].begin(), rows[i].end());
+    for (auto& e : rows[i]) {
+      const int pos = static_cast<int>(st.F_col.size());
+      st.F_col.push_back(e.first);
+      if (e.second & 1) st.up_posT[e.second >> 1] = pos;
+      else st.up_pos[e.second >> 1] = pos;
+      if (e.first == i) st.F_diag[i] = pos;
+    }
+    st.F_ptr[i + 1] = static_cast<int>(st.F_col.size());
+  }
+  st.nF = static_cast<int>(st.F_col.size());
+  return "";
+}
+
 }  // namespace g2ohip

@@ -98,4 +98,29 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
 // Builds st.rows; num_xcds spreads consecutive block rows over the same XCD.
 std::string build_row_plan(BAStructure& st, int max_cols, int chunk, int num_xcds);
 
+// 3D pose graph (VertexSE3 / EdgeSE3, 6x6 blocks, no Schur): free poses get
+// indices 0..Nf-1 in input order; the Hessian pattern is the diagonal plus one
+// upper block per distinct free pair joined by an edge (block_solver.hpp:
+// 110-269 without marginalised vertices). Per-edge contributions are summed
+// per block through fixed CSR lists, so the assembly is deterministic.
+struct PGStructure {
+  int N = 0, E = 0, Nf = 0;
+  std::vector<int> H;          // free index per pose or -1
+  std::vector<int> act;        // active edges (not both ends fixed)
+  int nU = 0;                  // upper blocks (diagonal first, then pairs)
+  std::vector<int> U_row, U_col;
+  std::vector<int> edge_blk;   // per active edge: its off-diagonal upper block or -1
+  std::vector<int> edge_swap;  // 1 if the edge's "from" pose has the higher free index
+  // sums: diagonal block i = sum over end_ptr[i]..end_ptr[i+1] of end_ref
+  // (2 * active position + 0 for "from", + 1 for "to"); off-diagonal block k
+  // (k >= Nf) = sum over blk_ptr[k - Nf].. of active edge positions
+  std::vector<int> end_ptr, end_ref, blk_ptr, blk_ref;
+  // full block rows (both triangles): F_ptr / F_col, position of each upper
+  // block and of its transpose (-1 on the diagonal), diagonal positions
+  int nF = 0;
+  std::vector<int> F_ptr, F_col, up_pos, up_posT, F_diag;
+};
+
+std::string build_pg_structure(PGStructure& st, int N, int E, const int* from, const int* to, const uint8_t* fixed);
+
 }  // namespace g2ohip

@@ -200,6 +200,32 @@ int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int
                             const double* blocks, const double* b, double* x, double tol, int max_iter,
                             double* residual_io, int* iterations);
 
+/* ------------------------------------------------------ 3D pose graph */
+/* SparseOptimizer::initializeOptimization + BlockSolver::buildStructure for
+ * EdgeSE3 edges between VertexSE3 poses (edge_se3.cpp, vertex_se3.h), 6x6
+ * blocks without Schur (block_solver.hpp:333-343). Replaces the context's
+ * problem: afterwards the generic Solver calls above (compute_active_errors,
+ * build_system, set_lambda, solve, get_x, get_b, update, push / pop /
+ * discard_top, lambda_init, compute_scale, optimize) act on the pose graph;
+ * the ba_* calls fail until a ba_set_structure. The solve is block-Jacobi
+ * PCG in FP64 (g2ohip_options.pcg_*). fixed[N] 0/1 (NULL = none). */
+int g2ohip_pg_set_structure(g2ohip_ctx* ctx, int num_poses, int num_edges, const int* edge_from, const int* edge_to,
+                            const uint8_t* fixed);
+/* info[0..3] = free poses, active edges, upper Hessian blocks, blocks in both triangles */
+int g2ohip_pg_structure_info(g2ohip_ctx* ctx, long long* info);
+/* EdgeSE3::setMeasurement / setInformation: meas[E*7] relative pose from -> to
+ * as [tx ty tz qx qy qz qw], information[E*36] row-major 6x6 (error order
+ * [t; qx qy qz] of toVectorMQT, isometry3d_mappings.cpp:80-85). */
+int g2ohip_pg_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* information);
+/* VertexSE3 estimates poses[N*7] = Isometry3 [tx ty tz qx qy qz qw]
+ * (x_world = R x + t); quaternions are normalised on entry. */
+int g2ohip_pg_set_state(g2ohip_ctx* ctx, const double* poses);
+int g2ohip_pg_get_state(g2ohip_ctx* ctx, double* poses);
+/* Undamped Hessian upper blocks of the last build_system (diagonal blocks
+ * first, then the pairs): rows/cols [info[2]], blocks [info[2]*36] row-major,
+ * b [6 * free poses]. NULL skips. */
+int g2ohip_pg_get_system(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* b);
+
 /* ---------------------------------------------- adjoint (differentiation) */
 /* Gradient of a scalar loss L(x*) at the converged state with respect to the
  * measurements and the per-edge information weights, by implicit

@@ -92,3 +92,27 @@ def test_fixed_points_and_cameras_are_excluded():
     # edges with both ends fixed are inactive
     both = (prob.cam_fixed[prob.edge_cam] == 1) & (prob.pt_fixed[prob.edge_pt] == 1)
     assert info[2] == prob.num_edges - int(both.sum())
+
+
+def test_pose_graph_structure_matches_reference_pattern():
+    prob = datasets.make_sphere_pose_graph(seed=3, nodes_per_level=30, laps=4)
+    pgo = g2o_amd.PoseGraphOptimizer(prob, device=-1)
+    info = pgo.structure_info()
+    free = prob.fixed == 0
+    H = np.where(free, np.cumsum(free) - 1, -1)
+    hi, hj = H[prob.edge_from], H[prob.edge_to]
+    both = (hi >= 0) & (hj >= 0)
+    pairs = {(min(a, b), max(a, b)) for a, b in zip(hi[both], hj[both])}
+    assert info[0] == free.sum()
+    assert info[1] == int(((hi >= 0) | (hj >= 0)).sum())
+    assert info[2] == free.sum() + len(pairs)
+    assert info[3] == free.sum() + 2 * len(pairs)
+
+
+def test_pose_graph_structure_rejects_bad_edges():
+    prob = datasets.make_sphere_pose_graph(seed=3, nodes_per_level=10, laps=2)
+    prob.edge_to = prob.edge_to.copy()
+    prob.edge_to[3] = prob.edge_from[3]
+    with pytest.raises(g2o_amd.G2oHipError) as ei:
+        g2o_amd.PoseGraphOptimizer(prob, device=-1)
+    assert ei.value.status == g2o_amd._lib.ERR_INVALID

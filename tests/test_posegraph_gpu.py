@@ -1,0 +1,98 @@
+"""3D pose graph (EdgeSE3 / VertexSE3, 6x6 blocks, PCG) on the GPU against
+the FP64 CPU oracle: errors, Hessian blocks, LM traces, the reference's
+two-vertex tests, and the 10k-pose sphere graph."""
+import time
+
+import numpy as np
+import pytest
+
+import g2o_amd
+from g2o_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(prob):
+    from oracle import oracle
+    return oracle.OraclePG(prob)
+
+
+def test_errors_and_system_match_oracle():
+    prob = datasets.make_sphere_pose_graph(seed=3, nodes_per_level=40, laps=4)
+    ora = _oracle(prob)
+    pgo = g2o_amd.PoseGraphOptimizer(prob, device=0)
+    chi_o, rows_o, cols_o, blocks_o, b_o = ora.linearize()
+    chi = pgo.compute_active_errors()
+    assert abs(chi - chi_o) <= 1e-11 * chi_o
+    pgo.build_system()
+    rows, cols, blocks, b = pgo.system()
+    key_o = {(r, c): k for k, (r, c) in enumerate(zip(rows_o, cols_o))}
+    assert len(key_o) == len(rows)
+    perm = np.array([key_o[(r, c)] for r, c in zip(rows, cols)])
+    scale = np.abs(blocks_o).max()
+    assert np.abs(blocks - blocks_o[perm]).max() <= 1e-11 * scale
+    assert np.abs(b - b_o).max() <= 1e-11 * np.abs(b_o).max()
+    e = pgo.errors()
+    e_o = np.stack([ora.edge(k)[0] for k in range(prob.num_edges)])
+    assert np.abs(e - e_o).max() <= 1e-12
+
+
+def test_lm_trace_matches_oracle():
+    prob = datasets.make_sphere_pose_graph(seed=4, nodes_per_level=50, laps=4)
+    ora = _oracle(prob)
+    n, trace, _ = ora.optimize(15, solver=1, tol=1e-8)
+    pgo = g2o_amd.PoseGraphOptimizer(prob, device=0, pcg_tolerance=1e-8)
+    stats = pgo.optimize(15)
+    assert len(stats) == n
+    for s, t in zip(stats, trace):
+        assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
+        assert s["levenberg_iterations"] == int(t[3])
+        assert abs(s["iterations_linear_solver"] - int(t[4])) <= 1
+    st, st_o = pgo.state(), ora.state()
+    assert np.abs(st[:, :3] - st_o[:, :3]).max() <= 1e-6
+
+
+@pytest.mark.parametrize("which", ["translation", "rotation"])
+def test_two_vertex_lm_like_reference(which):
+    """optimization_slam3d.cpp:62-138 on the device."""
+    I7 = np.array([0, 0, 0, 0, 0, 0, 1.0])
+    p2 = I7.copy()
+    if which == "translation":
+        p2[:3] = 10.0
+    else:
+        a = np.deg2rad(2)
+        p2[3:] = np.concatenate([np.sin(a / 2) * np.ones(3) / np.sqrt(3), [np.cos(a / 2)]])
+    prob = datasets.PoseGraphProblem(np.stack([I7, p2]), np.array([0], np.int32), np.array([1], np.int32),
+                                     I7[None], np.eye(6)[None], np.array([1, 0], np.uint8))
+    pgo = g2o_amd.PoseGraphOptimizer(prob, device=0, pcg_tolerance=1e-6)
+    assert pgo.compute_active_errors() > 0
+    stats = pgo.optimize(100)
+    assert len(stats) > 0
+    assert pgo.compute_active_errors() < 1e-6
+    st = pgo.state()
+    assert np.abs(st[1, :3]).max() < 1e-3
+    assert np.abs(np.diag(datasets.quat_to_rot(st[1, 3:])) - 1).max() < 1e-3
+
+
+def test_sphere_10k_poses_matches_oracle():
+    """10k poses on 10 laps, ~36k edges, PCG tolerance 1e-8, 20 LM
+    iterations from the odometry chain; first pose fixed."""
+    prob = datasets.make_sphere_pose_graph()
+    pgo = g2o_amd.PoseGraphOptimizer(prob, device=0, pcg_tolerance=1e-8)
+    pgo.optimize(1)
+    pgo.set_state(prob.poses)
+    t = time.perf_counter()
+    stats = pgo.optimize(20)
+    dt = time.perf_counter() - t
+    ora = _oracle(prob)
+    t = time.perf_counter()
+    n, trace, _ = ora.optimize(20, solver=1, tol=1e-8)
+    dto = time.perf_counter() - t
+    print(f"\nsphere 10k: GPU {len(stats)} LM iterations in {dt * 1e3:.1f} ms "
+          f"({len(stats) / dt:.1f} it/s), oracle {n} in {dto:.2f} s; chi2 {stats[-1]['chi2']:.6e} vs "
+          f"{trace[-1, 1]:.6e}; PCG iterations {[s['iterations_linear_solver'] for s in stats]}", flush=True)
+    assert len(stats) == n
+    assert abs(stats[-1]["chi2"] - trace[-1, 1]) <= 1e-6 * trace[-1, 1]
+    gt = prob.gt_poses
+    err = np.linalg.norm(pgo.state()[:, :3] - gt[:, :3], axis=1)
+    assert err.max() < 0.5
