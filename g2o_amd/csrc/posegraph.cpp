@@ -199,7 +199,6 @@ int g2ohip_ctx::pg_update() {
     HK(launch_pg_update(pg, d_x.p, d_cams.p, stream));
   }
   HK(hipStreamSynchronize(stream));
-  have_system = false;
   return G2OHIP_OK;
 }
 

@@ -306,6 +306,24 @@ class PoseGraphOptimizer:
     def build_system(self):
         self.ctx.call("g2ohip_build_system")
 
+    def set_lambda(self, lam: float):
+        self.ctx.call("g2ohip_set_lambda", float(lam))
+
+    def solve(self) -> bool:
+        ok = ct.c_int()
+        self.ctx.call("g2ohip_solve", ct.byref(ok))
+        return bool(ok.value)
+
+    def x(self) -> np.ndarray:
+        v = np.zeros(6 * self.Nf)
+        self.ctx.call("g2ohip_get_x", _dp(v))
+        return v
+
+    def lambda_init(self) -> float:
+        v = ct.c_double()
+        self.ctx.call("g2ohip_lambda_init", ct.byref(v))
+        return v.value
+
     def system(self):
         """(rows, cols, blocks [n,6,6], b) of the last build_system, undamped upper blocks."""
         rows = np.zeros(self.nblocks, dtype=np.int32)
