@@ -64,6 +64,7 @@ int g2ohip_set_options(g2ohip_ctx* ctx, const g2ohip_options* opt) {
   if (!(opt->pcg_tolerance >= 0) || opt->max_trials < 1 || opt->refine_steps < 0 || opt->refine_steps > 16)
     return ctx->fail(G2OHIP_ERR_INVALID, "invalid option value");
   ctx->opt = *opt;
+  ctx->pcg_residual_mem = -1.0;  // LinearSolverPCG::init
   return G2OHIP_OK;
 }
 

@@ -1,0 +1,217 @@
+// C++ host classes over the C ABI of include/g2o_hip.h, with the method names
+// and call contracts of the g2o interfaces they stand in for:
+//   BlockSolverHIP   - g2o::Solver (g2o/core/solver.h:44-153) as implemented
+//                      by BlockSolver<BlockSolverTraits<6,3>>
+//                      (g2o/core/block_solver.h:124, .hpp:110-606), plus the
+//                      SparseOptimizer / OptimizationAlgorithmLevenberg calls
+//                      that read or write vertex state
+//                      (sparse_optimizer.cpp:78-130, 457-468, 547-567;
+//                      optimization_algorithm_levenberg.cpp:155-176);
+//   LinearSolverHIP  - g2o::LinearSolver<MatrixType>::solve on a host
+//                      symmetric block matrix (linear_solver.h:47-145) with
+//                      LinearSolverPCG semantics (linear_solver_pcg.h:45-50);
+//   PoseGraphSolverHIP - the same Solver contract for VertexSE3 / EdgeSE3
+//                      graphs without Schur.
+// As in g2o, calls return bool and never throw; lastError() gives the reason.
+// The classes hold no Eigen or g2o types, so they build without either; the
+// adapter that derives from g2o::Solver and forwards to them is shown in the
+// integration notes.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/g2o_hip.h"
+
+namespace g2o_hip {
+
+class DeviceContext {
+ public:
+  explicit DeviceContext(int device) { _status = g2ohip_create(device, &_ctx); }
+  ~DeviceContext() { if (_ctx) g2ohip_destroy(_ctx); }
+  DeviceContext(const DeviceContext&) = delete;
+  DeviceContext& operator=(const DeviceContext&) = delete;
+  g2ohip_ctx* get() const { return _ctx; }
+  bool ok(int status) {
+    _status = status;
+    return status == G2OHIP_OK;
+  }
+  int status() const { return _status; }
+  std::string lastError() const { return _ctx ? g2ohip_last_error(_ctx) : "no context"; }
+
+ private:
+  g2ohip_ctx* _ctx = nullptr;
+  int _status = G2OHIP_OK;
+};
+
+class BlockSolverHIP {
+ public:
+  explicit BlockSolverHIP(int device = 0) : _dev(device) {
+    g2ohip_default_options(&_opt);
+    if (_dev.get()) _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
+  }
+
+  // ---- graph data (SparseOptimizer::initializeOptimization + the vertices' and edges' data)
+  bool setGraph(int numCams, int numPoints, int numEdges, const int* edgeCam, const int* edgePoint,
+                const uint8_t* camFixed, const uint8_t* pointFixed) {
+    if (!_dev.ok(g2ohip_ba_set_structure(_dev.get(), numCams, numPoints, numEdges, edgeCam, edgePoint, camFixed,
+                                         pointFixed)))
+      return false;
+    long long info[12];
+    if (!_dev.ok(g2ohip_ba_structure_info(_dev.get(), info))) return false;
+    _size = static_cast<size_t>(6 * info[0] + 3 * info[1]);
+    return true;
+  }
+  bool setMeasurements(const double* meas, const double* weight, const double* intrinsics, double huberDelta) {
+    return _dev.ok(g2ohip_ba_set_measurements(_dev.get(), meas, weight, intrinsics, huberDelta));
+  }
+  bool setEstimates(const double* cams, const double* points) {
+    return _dev.ok(g2ohip_ba_set_state(_dev.get(), cams, points));
+  }
+  bool getEstimates(double* cams, double* points) { return _dev.ok(g2ohip_ba_get_state(_dev.get(), cams, points)); }
+  bool setOptions(const g2ohip_options& opt) {
+    _opt = opt;
+    return _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
+  }
+  const g2ohip_options& options() const { return _opt; }
+
+  // ---- g2o::Solver
+  // init(): the linear solver forgets its PCG residual memory (LinearSolverPCG::init)
+  bool init(bool /*online*/ = false) { return _dev.ok(g2ohip_set_options(_dev.get(), &_opt)); }
+  // the block structure is built once by setGraph
+  bool buildStructure(bool /*zeroBlocks*/ = false) { return _size > 0; }
+  bool buildSystem() { return _dev.ok(g2ohip_build_system(_dev.get())); }
+  bool solve() {
+    int ok = 0;
+    return _dev.ok(g2ohip_solve(_dev.get(), &ok)) && ok != 0;
+  }
+  bool setLambda(double lambda, bool /*backup*/ = false) { return _dev.ok(g2ohip_set_lambda(_dev.get(), lambda)); }
+  void restoreDiagonal() { _dev.ok(g2ohip_restore_diagonal(_dev.get())); }
+  bool schur() const { return true; }
+  size_t vectorSize() const { return _size; }
+  // host copies of the update and right-hand side (Solver::x(), Solver::b())
+  const double* x() {
+    _x.resize(_size);
+    return _dev.ok(g2ohip_get_x(_dev.get(), _x.data())) ? _x.data() : nullptr;
+  }
+  const double* b() {
+    _b.resize(_size);
+    return _dev.ok(g2ohip_get_b(_dev.get(), _b.data())) ? _b.data() : nullptr;
+  }
+
+  // ---- state-side calls the LM algorithm makes, executed on the device
+  double computeActiveErrors() {  // computeActiveErrors + activeRobustChi2
+    double chi = -1;
+    return _dev.ok(g2ohip_compute_active_errors(_dev.get(), &chi)) ? chi : -1;
+  }
+  bool update() { return _dev.ok(g2ohip_update(_dev.get())); }  // SparseOptimizer::update(solver->x())
+  bool push() { return _dev.ok(g2ohip_push(_dev.get())); }
+  bool pop() { return _dev.ok(g2ohip_pop(_dev.get())); }
+  bool discardTop() { return _dev.ok(g2ohip_discard_top(_dev.get())); }
+  double computeLambdaInit() {
+    double l = -1;
+    return _dev.ok(g2ohip_lambda_init(_dev.get(), &l)) ? l : -1;
+  }
+  double computeScale() {
+    double s = 0;
+    return _dev.ok(g2ohip_compute_scale(_dev.get(), &s)) ? s : 0;
+  }
+  int iterationsLinearSolver() {
+    double info[3];
+    return _dev.ok(g2ohip_last_solve_info(_dev.get(), info)) ? static_cast<int>(info[0]) : -1;
+  }
+  // the whole SparseOptimizer::optimize loop on the device
+  int optimize(int iterations, std::vector<g2ohip_iter_stats>* stats = nullptr) {
+    std::vector<g2ohip_iter_stats> s(iterations > 0 ? iterations : 1);
+    int done = 0;
+    if (!_dev.ok(g2ohip_optimize(_dev.get(), iterations, s.data(), &done))) return -1;
+    if (stats) stats->assign(s.begin(), s.begin() + done);
+    return done;
+  }
+  std::string lastError() const { return _dev.lastError(); }
+  g2ohip_ctx* context() const { return _dev.get(); }
+
+ private:
+  DeviceContext _dev;
+  g2ohip_options _opt{};
+  size_t _size = 0;
+  std::vector<double> _x, _b;
+};
+
+// LinearSolver<Matrix{3,6}d>::solve for a host matrix given by its upper
+// blocks (the unit-test path: upload, then the device PCG).
+class LinearSolverHIP {
+ public:
+  LinearSolverHIP(int device, int blockDim) : _dev(device), _bd(blockDim) {}
+  bool init() {
+    _residual = -1.0;  // LinearSolverPCG::init
+    return true;
+  }
+  void setTolerance(double t) { _tol = t; }
+  void setMaxIterations(int m) { _maxIter = m; }
+  bool solve(int numBlocks, const std::vector<int>& rows, const std::vector<int>& cols,
+             const std::vector<double>& blocks, double* x, const double* b) {
+    return _dev.ok(g2ohip_linear_solve_pcg(_dev.get(), numBlocks, _bd, static_cast<int>(rows.size()), rows.data(),
+                                           cols.data(), blocks.data(), b, x, _tol, _maxIter, &_residual, &_iters));
+  }
+  int iterations() const { return _iters; }
+  std::string lastError() const { return _dev.lastError(); }
+
+ private:
+  DeviceContext _dev;
+  int _bd;
+  double _tol = 1e-6;
+  int _maxIter = -1;
+  double _residual = -1.0;
+  int _iters = 0;
+};
+
+// Solver contract for a VertexSE3 / EdgeSE3 pose graph (6x6 blocks, no Schur).
+class PoseGraphSolverHIP {
+ public:
+  explicit PoseGraphSolverHIP(int device = 0) : _dev(device) {
+    g2ohip_default_options(&_opt);
+    _opt.pcg_tolerance = 1e-8;
+    if (_dev.get()) _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
+  }
+  bool setGraph(int numPoses, int numEdges, const int* from, const int* to, const uint8_t* fixed) {
+    if (!_dev.ok(g2ohip_pg_set_structure(_dev.get(), numPoses, numEdges, from, to, fixed))) return false;
+    long long info[4];
+    if (!_dev.ok(g2ohip_pg_structure_info(_dev.get(), info))) return false;
+    _size = static_cast<size_t>(6 * info[0]);
+    return true;
+  }
+  bool setMeasurements(const double* meas7, const double* info36) {
+    return _dev.ok(g2ohip_pg_set_measurements(_dev.get(), meas7, info36));
+  }
+  bool setEstimates(const double* poses7) { return _dev.ok(g2ohip_pg_set_state(_dev.get(), poses7)); }
+  bool getEstimates(double* poses7) { return _dev.ok(g2ohip_pg_get_state(_dev.get(), poses7)); }
+  bool setOptions(const g2ohip_options& opt) {
+    _opt = opt;
+    return _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
+  }
+  bool buildSystem() { return _dev.ok(g2ohip_build_system(_dev.get())); }
+  bool solve() {
+    int ok = 0;
+    return _dev.ok(g2ohip_solve(_dev.get(), &ok)) && ok != 0;
+  }
+  bool setLambda(double lambda, bool = false) { return _dev.ok(g2ohip_set_lambda(_dev.get(), lambda)); }
+  void restoreDiagonal() { _dev.ok(g2ohip_restore_diagonal(_dev.get())); }
+  size_t vectorSize() const { return _size; }
+  int optimize(int iterations, std::vector<g2ohip_iter_stats>* stats = nullptr) {
+    std::vector<g2ohip_iter_stats> s(iterations > 0 ? iterations : 1);
+    int done = 0;
+    if (!_dev.ok(g2ohip_optimize(_dev.get(), iterations, s.data(), &done))) return -1;
+    if (stats) stats->assign(s.begin(), s.begin() + done);
+    return done;
+  }
+  std::string lastError() const { return _dev.lastError(); }
+
+ private:
+  DeviceContext _dev;
+  g2ohip_options _opt{};
+  size_t _size = 0;
+};
+
+}  // namespace g2o_hip

@@ -1,0 +1,127 @@
+// Runs Levenberg-Marquardt on a bundle-adjustment problem through
+// BlockSolverHIP in two ways and prints one line per iteration:
+//   solver   - the host loop of OptimizationAlgorithmLevenberg::solve
+//              (optimization_algorithm_levenberg.cpp:60-153) written against
+//              the g2o::Solver calls (buildSystem, setLambda, solve,
+//              restoreDiagonal, x) and the optimizer's state calls, as g2o
+//              drives a solver plugin;
+//   optimize - the library's own device-resident loop (g2ohip_optimize).
+// Usage: lm_drive <problem.bin> <solver|optimize> <iterations> <device>
+// problem.bin: int32 C, P, E, precision, refine_steps; f64 cams[C*7],
+// intr[C*4], pts[P*3]; int32 edge_cam[E], edge_pt[E]; f64 meas[E*2],
+// weight[E]; u8 cam_fixed[C], pt_fixed[P]; f64 huber.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "g2o_hip.hpp"
+
+namespace {
+
+template <typename T>
+bool read_vec(FILE* f, std::vector<T>& v, size_t n) {
+  v.resize(n);
+  return n == 0 || std::fread(v.data(), sizeof(T), n, f) == n;
+}
+
+int run_solver_loop(g2o_hip::BlockSolverHIP& s, int iterations) {
+  s.init();
+  double lambda = 0, ni = 2;
+  for (int i = 0; i < iterations; ++i) {
+    double currentChi = s.computeActiveErrors();
+    const double chiBefore = currentChi;
+    if (!s.buildSystem()) return 1;
+    if (i == 0) {
+      lambda = s.computeLambdaInit();
+      ni = 2;
+    }
+    double rho = 0;
+    int q = 0, pcg = 0;
+    do {
+      s.push();
+      s.setLambda(lambda, true);
+      const bool ok = s.solve();
+      pcg += s.iterationsLinearSolver();
+      s.restoreDiagonal();
+      double tempChi = std::numeric_limits<double>::max();
+      if (ok) {
+        s.update();
+        tempChi = s.computeActiveErrors();
+      }
+      rho = currentChi - tempChi;
+      const double scale = ok ? s.computeScale() + 1e-3 : 1.0;
+      rho /= scale;
+      if (rho > 0 && std::isfinite(tempChi) && ok) {
+        double alpha = 1. - std::pow((2 * rho - 1), 3);
+        alpha = std::min(alpha, 2. / 3.);
+        lambda *= std::max(1. / 3., alpha);
+        ni = 2;
+        currentChi = tempChi;
+        s.discardTop();
+      } else {
+        lambda *= ni;
+        ni *= 2;
+        s.pop();
+        if (!std::isfinite(lambda)) break;
+      }
+      q++;
+    } while (rho < 0 && q < s.options().max_trials);
+    std::printf("%d %.17g %.17g %.17g %d %d\n", i, chiBefore, currentChi, lambda, q, pcg);
+    if (q == s.options().max_trials || rho == 0 || !std::isfinite(lambda)) break;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 5) {
+    std::fprintf(stderr, "usage: %s problem.bin solver|optimize iterations device\n", argv[0]);
+    return 2;
+  }
+  FILE* f = std::fopen(argv[1], "rb");
+  if (!f) return 2;
+  int hdr[5];
+  if (std::fread(hdr, sizeof(int), 5, f) != 5) return 2;
+  const int C = hdr[0], P = hdr[1], E = hdr[2];
+  std::vector<double> cams, intr, pts, meas, w;
+  std::vector<int> ec, ep;
+  std::vector<uint8_t> cf, pf;
+  double huber = 0;
+  bool ok = read_vec(f, cams, 7 * static_cast<size_t>(C)) && read_vec(f, intr, 4 * static_cast<size_t>(C)) &&
+            read_vec(f, pts, 3 * static_cast<size_t>(P)) && read_vec(f, ec, E) && read_vec(f, ep, E) &&
+            read_vec(f, meas, 2 * static_cast<size_t>(E)) && read_vec(f, w, E) && read_vec(f, cf, C) &&
+            read_vec(f, pf, P) && std::fread(&huber, sizeof(double), 1, f) == 1;
+  std::fclose(f);
+  if (!ok) return 2;
+  g2o_hip::BlockSolverHIP s(std::atoi(argv[4]));
+  g2ohip_options opt = s.options();
+  opt.precision = hdr[3];
+  opt.refine_steps = hdr[4];
+  if (!s.setOptions(opt) || !s.setGraph(C, P, E, ec.data(), ep.data(), cf.data(), pf.data()) ||
+      !s.setMeasurements(meas.data(), w.data(), intr.data(), huber) || !s.setEstimates(cams.data(), pts.data())) {
+    std::fprintf(stderr, "setup failed: %s\n", s.lastError().c_str());
+    return 1;
+  }
+  const int iters = std::atoi(argv[3]);
+  if (std::string(argv[2]) == "solver") {
+    if (run_solver_loop(s, iters)) {
+      std::fprintf(stderr, "solver loop failed: %s\n", s.lastError().c_str());
+      return 1;
+    }
+  } else {
+    std::vector<g2ohip_iter_stats> st;
+    if (s.optimize(iters, &st) < 0) {
+      std::fprintf(stderr, "optimize failed: %s\n", s.lastError().c_str());
+      return 1;
+    }
+    for (size_t i = 0; i < st.size(); ++i)
+      std::printf("%zu %.17g %.17g %.17g %d %d\n", i, st[i].chi2_before, st[i].chi2, st[i].lambda,
+                  st[i].levenberg_iterations, st[i].iterations_linear_solver);
+  }
+  return 0;
+}

@@ -85,6 +85,8 @@ void g2ohip_default_options(g2ohip_options* opt);
 int g2ohip_create(int device, g2ohip_ctx** out);
 void g2ohip_destroy(g2ohip_ctx* ctx);
 const char* g2ohip_last_error(const g2ohip_ctx* ctx);
+/* Also resets the PCG residual memory, as LinearSolverPCG::init does when the
+ * algorithm is (re)initialised. */
 int g2ohip_set_options(g2ohip_ctx* ctx, const g2ohip_options* opt);
 int g2ohip_get_options(const g2ohip_ctx* ctx, g2ohip_options* opt);
 
