@@ -764,7 +764,8 @@ int g2ohip_ctx::exact_system() {
 // x = (H + lam I)^-1 b for the current FP64 blocks: explicit Schur, PCG to the
 // relative tolerance tol (no residual memory), back-substitution from the
 // blocks. *ok = 0 if a point block or a preconditioner block is singular or
-// the solve produced non-finite values.
+// the solve produced non-finite values, or the PCG met a direction of
+// non-positive curvature (the role of a failed Cholesky).
 int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, int* ok) {
   PrecBufs<double>& B = f64;
   const int nb = st.Cf;
@@ -791,6 +792,7 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
   CK(run_pcg<double>(B, d_bschur.p, PcgControl{tol, -1.0, -1.0}, false, max_iter < 0 ? 120 * nb : max_iter, 0,
                      &pr));
   *iters = pr.iters;
+  if (pr.breakdown) { *ok = 0; return G2OHIP_OK; }  // (H + lam I) is not positive definite
   HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 0.0, stream));
   HK(launch_backsub_blocks<double>(g, B.Hpl.p, B.Dinv.p, d_b.p, d_x.p, stream));
   const size_t nx = 6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);

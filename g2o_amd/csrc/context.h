@@ -59,6 +59,7 @@ struct PcgRun {
   double dn_final = 0;  // r . M^-1 r at exit
   double d0 = 0;        // the absolute stop threshold used
   double dn0 = 0;       // r . M^-1 r of the right-hand side
+  bool breakdown = false;  // d.q <= 0 or r.M^-1 r < 0 seen: operator or preconditioner not PD
 };
 
 struct RowPlanBufs {

@@ -126,7 +126,7 @@ hipError_t launch_pcg_dir(int nb, const HT* s, const HT* d_old, HT* d, const dou
                           const PcgControl& ctl, int it, int last, hipStream_t st);
 template <typename HT, int BD>
 hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
-                             const double* part_dq, int n_dq, double* part_rs, const PcgScalars* sc, int it,
+                             const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st);
 // ---- exact Hessian (adjoint, Newton polish), adjoint.hip
 hipError_t launch_linearize_exact(const DevGraph& g, const double* camRt, const double* pts, double* Hpl,

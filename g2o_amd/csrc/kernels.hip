@@ -1048,6 +1048,7 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
   if (first) {
     if (it == 0) { sc->d0 = d0; sc->dn0 = dn; }
     sc->dn[it & 1] = dn;  // the other slot still holds dn_old for this launch
+    if (!(dn >= 0.0)) sc->err = 1;  // indefinite preconditioner (reported, not acted on)
   }
   if (dn <= d0 || last) {
     if (first) { sc->done = 1; sc->iters = it; }
@@ -1128,10 +1129,14 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const HT* __restrict_
                                                    HT* __restrict__ r, const HT* __restrict__ d,
                                                    const HT* __restrict__ q, HT* __restrict__ s,
                                                    const double* __restrict__ part_dq, int n_dq,
-                                                   double* __restrict__ part_rs, const PcgScalars* sc, int it) {
+                                                   double* __restrict__ part_rs, PcgScalars* sc, int it) {
   if (sc->done) return;
   const double dq = wave_sum_array(part_dq, n_dq);
   const HT a = static_cast<HT>(sc->dn[it & 1] / dq);
+  // d . q <= 0: the operator is not positive definite along d (reported in
+  // sc->err for callers that need it; the iteration itself follows
+  // linear_solver_pcg.hpp, which does not check)
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !(dq > 0.0)) sc->err = 1;
   const int i = blockIdx.x * 64 + threadIdx.x;
   double dot = 0;
   if (i < nb) {
@@ -1184,7 +1189,7 @@ hipError_t launch_pcg_dir(int nb, const HT* s, const HT* d_old, HT* d, const dou
 
 template <typename HT, int BD>
 hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
-                             const double* part_dq, int n_dq, double* part_rs, const PcgScalars* sc, int it,
+                             const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st) {
   hipLaunchKernelGGL((k_pcg_update<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, Jinv, x, r, d, q, s,
                      part_dq, n_dq, part_rs, sc, it);
@@ -1200,7 +1205,7 @@ hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, 
   template hipError_t launch_pcg_dir<HT, BD>(int, const HT*, const HT*, HT*, const double*, PcgScalars*,          \
                                              const PcgControl&, int, int, hipStream_t);                          \
   template hipError_t launch_pcg_update<HT, BD>(int, const HT*, HT*, HT*, const HT*, const HT*, HT*,              \
-                                                const double*, int, double*, const PcgScalars*, int, hipStream_t);
+                                                const double*, int, double*, PcgScalars*, int, hipStream_t);
 G2OHIP_PCG_INST(float, 6)
 G2OHIP_PCG_INST(double, 6)
 G2OHIP_PCG_INST(double, 3)

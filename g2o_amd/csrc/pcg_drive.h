@@ -52,6 +52,7 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const do
   out->dn_final = hs.dn[hs.iters & 1];
   out->d0 = hs.d0;
   out->dn0 = hs.dn0;
+  out->breakdown = hs.err != 0;
   return hipSuccess;
 }
 

@@ -79,7 +79,7 @@ def solve(meas: torch.Tensor, weight: torch.Tensor, ba: BundleAdjuster, init_cam
 
 def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine_steps: int = 1,
             lm_iterations: int = 30, chi2_max: float | None = None, min_obs: int = 3, max_cond: float = 1e6,
-            newton_steps: int = 4):
+            newton_steps: int = 12, polish_iterations: int = 100, polish_tolerance: float = 1e-12):
     """Conditions a robust BA problem for implicit differentiation and solves it.
 
     The implicit function theorem needs a strict stationary point with a
@@ -90,7 +90,10 @@ def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine
     chi-square bound scaled to the kernel) are dropped, as g2o workflows set
     outlier edges to level 1; landmarks left with < min_obs edges are dropped;
     LM again; landmarks whose exact 3x3 block has condition number > max_cond
-    are dropped; LM again and exact-Hessian Newton steps.
+    are dropped; LM again (polish_iterations at PCG tolerance
+    polish_tolerance, so that Newton starts inside its quadratic region) and
+    exact-Hessian Newton steps until the gradient reaches its rounding floor
+    (info['gradient_max']).
 
     Returns (sub_problem, BundleAdjuster at the stationary point). The
     sub-problem's info['edge_index'] maps its edges to the input's."""
@@ -103,8 +106,9 @@ def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine
         chi2_max = 5.991 * prob.huber ** 2 if prob.huber > 0 else np.inf
     sub = select_edges(prob, chi <= chi2_max, min_obs)
     sub.cams, sub.pts = cams, pts[sub.info["pt_index"]]
-    ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps)
-    ba.optimize(lm_iterations)
+    ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps,
+                        pcg_tolerance=polish_tolerance)
+    ba.optimize(polish_iterations)
     ba.adjoint(np.zeros((sub.num_cams, 6)))  # evaluates the exact blocks at the state
     _, Hll, _ = ba.hessian()
     ev = np.linalg.eigvalsh(Hll)
@@ -119,8 +123,43 @@ def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine
         sub2.info["pt_index"] = sub.info["pt_index"][sub2.info["pt_index"]]
         sub2.cams, sub2.pts = cams, pts[np.searchsorted(sub.info["pt_index"], sub2.info["pt_index"])]
         sub = sub2
-        ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps)
-        ba.optimize(lm_iterations)
-    if not ba.newton(newton_steps):
-        raise RuntimeError("exact-Hessian Newton found no descent step")
+        ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps,
+                            pcg_tolerance=polish_tolerance)
+        ba.optimize(polish_iterations)
+    g = _newton_to_stationary(ba, newton_steps)
+    sub.info["gradient_max"] = g
     return sub, ba
+
+
+def _newton_to_stationary(ba: BundleAdjuster, max_steps: int, gtol: float = 1e-6) -> float:
+    """Exact-Hessian Newton steps until max |grad| stops falling below 1e-3 of
+    its previous value's rounding floor: at most max_steps steps, early exit
+    at max |grad| <= gtol * (1 + initial max |grad|). Returns the final max
+    |grad|; raises if no step is acceptable or the gradient stays large.
+    The gradient is read from an FP64 linearisation (an FP32 one carries
+    rounding far above the stationary point's)."""
+    prec = ba.ctx.set_options().precision
+    ba.set_options(precision=_lib.FP64)
+    try:
+        return _newton_loop(ba, max_steps, gtol)
+    finally:
+        ba.set_options(precision=prec)
+
+
+def _newton_loop(ba: BundleAdjuster, max_steps: int, gtol: float) -> float:
+    ba.build_system()
+    g0 = g = float(np.abs(ba.b()).max())
+    for _ in range(max_steps):
+        if g <= gtol * (1.0 + g0):
+            break
+        if not ba.newton(1):
+            raise RuntimeError("exact-Hessian Newton found no descent step")
+        ba.build_system()
+        g_new = float(np.abs(ba.b()).max())
+        if g_new > 0.5 * g and g_new <= 1e-4 * (1.0 + g0):
+            g = g_new
+            break  # at the rounding floor
+        g = g_new
+    if g > 1e-3 * (1.0 + g0):
+        raise RuntimeError(f"exact-Hessian Newton did not reach a stationary point (max |grad| {g:.3e})")
+    return g
