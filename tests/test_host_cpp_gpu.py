@@ -42,10 +42,15 @@ def test_solver_contract_loop_matches_device_optimize(tmp_path, precision, refin
     b = run(path, "optimize", 8)
     assert a.shape == b.shape and a.shape[0] >= 5
     # chi2 before / after and lambda: same decisions, values equal up to the
-    # summation order of the two chi2 passes
-    np.testing.assert_allclose(a[:, 1:4], b[:, 1:4], rtol=1e-10)
-    np.testing.assert_array_equal(a[:, 4:], b[:, 4:])
+    # summation order of the two chi2 passes (FP64); the FP32 Schur sums its
+    # pair products with float LDS atomics, so two FP32 runs differ in the
+    # last float digits and PCG counts may differ by an iteration or two
+    fp32 = precision == g2o_amd.FP32
+    rtol = 1e-5 if fp32 else 1e-10
+    np.testing.assert_allclose(a[:, 1:4], b[:, 1:4], rtol=rtol)
+    np.testing.assert_array_equal(a[:, 4], b[:, 4])
+    assert np.abs(a[:, 5] - b[:, 5]).max() <= (3 if fp32 else 0)
     # and the Python mirror of the same library
     ba = g2o_amd.BundleAdjuster(prob, device=0, precision=precision, refine_steps=refine)
     st = ba.optimize(8)
-    np.testing.assert_allclose([s["chi2"] for s in st], b[:, 2], rtol=1e-12)
+    np.testing.assert_allclose([s["chi2"] for s in st], b[:, 2], rtol=1e-5 if fp32 else 1e-12)
