@@ -100,7 +100,7 @@ def lib():
         "g2ohip_linear_solve_pcg": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _dp, _dp, _dp,
                                                ct.c_double, ct.c_int, _dp, _ip]),
         "g2ohip_ba_adjoint": (ct.c_int, [_vp, _dp, _dp, _dp, _dp, ct.c_double, ct.c_int, _ip]),
-        "g2ohip_ba_newton": (ct.c_int, [_vp, ct.c_int, ct.c_double, _ip]),
+        "g2ohip_ba_newton": (ct.c_int, [_vp, ct.c_int, ct.c_double, ct.c_double, _ip]),
         "g2ohip_pg_set_structure": (ct.c_int, [_vp, ct.c_int, ct.c_int, _ip, _ip, _up]),
         "g2ohip_pg_structure_info": (ct.c_int, [_vp, ct.POINTER(ct.c_longlong)]),
         "g2ohip_pg_set_measurements": (ct.c_int, [_vp, _dp, _dp]),

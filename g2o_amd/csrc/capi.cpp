@@ -264,10 +264,10 @@ int g2ohip_ba_adjoint(g2ohip_ctx* ctx, const double* dL_dcam, const double* dL_d
   return ctx->adjoint(dL_dcam, dL_dpt, dL_dmeas, dL_dweight, tol < 0 ? 1e-24 : tol, max_iter, iterations);
 }
 
-int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, int* ok) {
+int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, double max_step, int* ok) {
   G2OHIP_ENTER(ctx);
   if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "ok must not be NULL");
-  return ctx->newton(steps, tol < 0 ? 1e-24 : tol, ok);
+  return ctx->newton(steps, tol < 0 ? 1e-24 : tol, max_step, ok);
 }
 
 // ------------------------------------------------------------ pose graph

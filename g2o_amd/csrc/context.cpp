@@ -766,7 +766,7 @@ int g2ohip_ctx::exact_system() {
 // blocks. *ok = 0 if a point block or a preconditioner block is singular or
 // the solve produced non-finite values, or the PCG met a direction of
 // non-positive curvature (the role of a failed Cholesky).
-int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, int* ok) {
+int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, int* ok, double* max_abs) {
   PrecBufs<double>& B = f64;
   const int nb = st.Cf;
   *ok = 1;
@@ -799,8 +799,12 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
   std::vector<double> hx(nx);
   HK(hipMemcpyAsync(hx.data(), d_x.p, nx * sizeof(double), hipMemcpyDeviceToHost, stream));
   HK(hipStreamSynchronize(stream));
-  for (double v : hx)
+  double m = 0;
+  for (double v : hx) {
     if (!std::isfinite(v)) { *ok = 0; break; }
+    m = std::max(m, std::fabs(v));
+  }
+  if (max_abs) *max_abs = m;
   return G2OHIP_OK;
 }
 
@@ -808,9 +812,10 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
 // Hessian: a step is kept if the robust chi2 does not increase (within 1e-12
 // relative: rounding at the optimum), else the state is restored and the
 // damping mu (0 at first) raised to max(1e-15 * max diag, 10 mu); after an
-// accepted step mu drops by 10 (to 0 below 1e-15 * max diag). At most 30
+// accepted step mu drops by 10 (to 0 below 1e-15 * max diag). A step whose
+// largest component exceeds max_step (> 0) counts as rejected. At most 30
 // trials per step; *ok = 0 if none was acceptable.
-int g2ohip_ctx::newton(int steps, double tol, int* ok) {
+int g2ohip_ctx::newton(int steps, double tol, double max_step, int* ok) {
   if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
   CK(require_device());
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
@@ -832,7 +837,9 @@ int g2ohip_ctx::newton(int steps, double tol, int* ok) {
     bool accepted = false;
     for (int trial = 0; trial < 30 && !accepted; ++trial) {
       int iters = 0, solved = 0;
-      CK(solve_exact(mu, tol, -1, &iters, &solved));
+      double step = 0;
+      CK(solve_exact(mu, tol, -1, &iters, &solved, &step));
+      if (solved && max_step > 0 && step > max_step) solved = 0;  // outside the trust region
       if (solved) {
         CK(push());
         HK(launch_update(g, d_x.p, d_cams.p, d_pts.p, stream));
@@ -876,7 +883,7 @@ int g2ohip_ctx::adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_
           rhs[6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.ptH[i]) + r] = dL_dpt[3 * static_cast<size_t>(i) + r];
   HK(hipMemcpyAsync(d_b.p, rhs.data(), nx * sizeof(double), hipMemcpyHostToDevice, stream));
   int iters = 0, ok = 1;
-  CK(solve_exact(0.0, tol, max_iter, &iters, &ok));
+  CK(solve_exact(0.0, tol, max_iter, &iters, &ok, nullptr));
   if (iterations) *iterations = iters;
   if (!ok) return fail(G2OHIP_ERR_NOT_PD, "exact Hessian is not positive definite at this state");
   DBuf<double> out;

@@ -173,8 +173,8 @@ struct g2ohip_ctx {
   int pg_get_system(int* rows, int* cols, double* blocks, double* b);
 
   int exact_system();
-  int solve_exact(double lam, double tol, int max_iter, int* iters, int* ok);
-  int newton(int steps, double tol, int* ok);
+  int solve_exact(double lam, double tol, int max_iter, int* iters, int* ok, double* max_abs);
+  int newton(int steps, double tol, double max_step, int* ok);
 
   template <typename HT>
   int solve_impl(int* ok);

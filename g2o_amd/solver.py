@@ -223,11 +223,12 @@ class BundleAdjuster:
         return out
 
     # ---- exact Hessian: stationary point and implicit differentiation
-    def newton(self, steps: int = 3, tol: float = -1.0) -> bool:
-        """Undamped exact-Hessian Newton steps (drive an LM solution to the
-        stationary point). Returns False if the Hessian is not positive definite."""
+    def newton(self, steps: int = 3, tol: float = -1.0, max_step: float = 1.0) -> bool:
+        """Damped exact-Hessian Newton steps with a trust region of max_step
+        per component (drive an LM solution to the stationary point). Returns
+        False if a step found no acceptable damping."""
         ok = ct.c_int()
-        self.ctx.call("g2ohip_ba_newton", int(steps), float(tol), ct.byref(ok))
+        self.ctx.call("g2ohip_ba_newton", int(steps), float(tol), float(max_step), ct.byref(ok))
         return bool(ok.value)
 
     def adjoint(self, dL_dcam, dL_dpt=None, tol: float = -1.0, max_iter: int = -1):

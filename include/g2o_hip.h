@@ -248,11 +248,13 @@ int g2ohip_ba_adjoint(g2ohip_ctx* ctx, const double* dL_dcam, const double* dL_d
 /* Newton steps x <- x + (H + mu I)^-1 (-grad(F/2)) with the exact Hessian
  * above (FP64, PCG tolerance tol as for g2ohip_ba_adjoint), damped as
  * Levenberg-Marquardt on that Hessian (mu = 0 while the robust chi2 does not
- * increase): drives an LM solution, whose accuracy LM's chi2-based step
- * acceptance limits, to the stationary point that g2ohip_ba_adjoint
- * differentiates. *ok = 0 if a step found no acceptable damping. No
- * counterpart in the reference. */
-int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, int* ok);
+ * increase; a PCG that meets non-positive curvature counts as a failed
+ * factorisation and raises mu) with a trust region: a step whose largest
+ * component exceeds max_step (> 0; <= 0: unbounded) is rejected. Drives an LM
+ * solution, whose accuracy LM's chi2-based step acceptance limits, to the
+ * stationary point that g2ohip_ba_adjoint differentiates. *ok = 0 if a step
+ * found no acceptable damping. No counterpart in the reference. */
+int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, double max_step, int* ok);
 
 #ifdef __cplusplus
 }

@@ -1529,12 +1529,13 @@ int orc_ba_adjoint(void* h, const double* dL_dcam, const double* dL_dpt, double*
 // Newton steps with the exact Hessian (gradient from ba_build_system's b =
 // -grad(F/2)), damped as Levenberg-Marquardt on that Hessian: a step is kept
 // if the robust chi2 does not increase, else the state is restored and the
-// damping mu (0 at first) raised to max(1e-9 * max diag, 10 mu) (an increase
+// damping mu (0 at first) raised to max(1e-15 * max diag, 10 mu) (an increase
 // within 1e-12 relative, rounding at the optimum, counts as none); after an
-// accepted step mu drops by 10 (below 1e-9 * max diag it is 0). Drives a
+// accepted step mu drops by 10 (below 1e-15 * max diag it is 0); a step with
+// a component above max_step (> 0) counts as rejected. Drives a
 // converged LM solution to the stationary point to machine precision before
 // it is differentiated. Returns 0, or 2 if no step was acceptable.
-int orc_ba_newton(void* h, int steps) {
+int orc_ba_newton(void* h, int steps, double max_step) {
   BA* p = static_cast<BA*>(h);
   const int saved_kind = p->solver_kind;
   const double saved_lambda = p->lambda;
@@ -1555,6 +1556,11 @@ int orc_ba_newton(void* h, int steps) {
     for (int trial = 0; trial < 30 && !accepted; ++trial) {
       p->lambda = mu;
       bool solved = ba_solve(*p);
+      if (solved && max_step > 0) {  // trust region on the step
+        double m = 0;
+        for (double v : p->x) m = std::max(m, std::fabs(v));
+        if (m > max_step) solved = false;
+      }
 
       if (solved) {
         vector<SE3> c0 = p->cams;

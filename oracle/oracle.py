@@ -69,7 +69,7 @@ def lib():
         L.orc_ba_adjoint.restype = ct.c_int
         L.orc_ba_adjoint.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.orc_ba_newton.restype = ct.c_int
-        L.orc_ba_newton.argtypes = [ct.c_void_p, ct.c_int]
+        L.orc_ba_newton.argtypes = [ct.c_void_p, ct.c_int, ct.c_double]
         L.orc_ba_exact_hessian.argtypes = [ct.c_void_p, _dp, _dp, _dp]
         L.orc_edge_energy_derivs.restype = ct.c_double
         L.orc_edge_energy_derivs.argtypes = [_dp, _dp, _dp, ct.c_double, _dp, ct.c_double, _dp, _dp]
@@ -198,9 +198,10 @@ class OracleBA:
                                   lam.ctypes.data_as(_dp))
         return st, dz, dw, lam
 
-    def newton(self, steps: int) -> int:
-        """Undamped Newton steps with the exact Hessian (0 = ok, 2 = not PD)."""
-        return lib().orc_ba_newton(self.h, steps)
+    def newton(self, steps: int, max_step: float = 1.0) -> int:
+        """Damped exact-Hessian Newton steps with a trust region (0 = ok, 2 =
+        no acceptable step)."""
+        return lib().orc_ba_newton(self.h, steps, float(max_step))
 
     def exact_hessian(self):
         Hpp = np.zeros((self.Cf, 6, 6))
