@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 from .datasets import BAProblem, quat_to_rot, select_edges
-from .solver import BundleAdjuster
+from .solver import BundleAdjuster, G2oHipError
 
 
 def camera_centres(cams7: np.ndarray) -> np.ndarray:
@@ -109,7 +109,11 @@ def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine
     ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps,
                         pcg_tolerance=polish_tolerance)
     ba.optimize(polish_iterations)
-    ba.adjoint(np.zeros((sub.num_cams, 6)))  # evaluates the exact blocks at the state
+    try:
+        ba.adjoint(np.zeros((sub.num_cams, 6)))  # evaluates the exact blocks at the state
+    except G2oHipError as ex:  # blocks are there even where the reduced system is not yet PD
+        if ex.status != _lib.ERR_NOT_PD:
+            raise
     _, Hll, _ = ba.hessian()
     ev = np.linalg.eigvalsh(Hll)
     free = np.flatnonzero(sub.pt_fixed == 0)
