@@ -53,4 +53,4 @@ def test_solver_contract_loop_matches_device_optimize(tmp_path, precision, refin
     # and the Python mirror of the same library
     ba = g2o_amd.BundleAdjuster(prob, device=0, precision=precision, refine_steps=refine)
     st = ba.optimize(8)
-    np.testing.assert_allclose([s["chi2"] for s in st], b[:, 2], rtol=1e-5 if fp32 else 1e-12)
+    np.testing.assert_allclose([s["chi2"] for s in st], b[:, 2], rtol=1e-5 if fp32 else 1e-10)
