@@ -397,6 +397,10 @@ int g2ohip_ctx::solve_impl(int* ok) {
   const int nb = st.Cf;
   const bool implicit = opt.schur_mode == G2OHIP_SCHUR_IMPLICIT;
   *ok = 1;
+  // the FP64 matrix-free passes below re-linearise at the current state: the
+  // camera table must follow it (a rejected LM trial is popped after its chi2
+  // pass left the table at the trial state)
+  HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
   CK(ensure_rows());

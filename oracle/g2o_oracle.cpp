@@ -502,6 +502,7 @@ struct BA {
   BlockSym S;
   vector<double> bschur, Dinv, x, coeff;
   double lambda = 0;
+  double user_lambda = 0;  // "initialLambda" property (<= 0: tau * max diag)
   PCGState pcg;
   int solver_kind = 0;  // 0 Cholesky, 1 PCG
   double times[6] = {0, 0, 0, 0, 0, 0};
@@ -1414,7 +1415,7 @@ int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
     p->times[0] += t1 - t0;
     ba_build_system(*p);
     p->times[1] += now_s() - t1;
-    if (it == 0) { lambda = ba_lambda_init(*p); ni = 2; }
+    if (it == 0) { lambda = p->user_lambda > 0 ? p->user_lambda : ba_lambda_init(*p); ni = 2; }
     double chi_before = currentChi;
     double rho = 0;
     int q = 0, pcgsum = 0;
@@ -1466,6 +1467,9 @@ int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
   if (times) std::copy(p->times, p->times + 6, times);
   return it;
 }
+
+// OptimizationAlgorithmLevenberg's "initialLambda" property (<= 0: computeLambdaInit).
+void orc_ba_set_user_lambda(void* h, double v) { static_cast<BA*>(h)->user_lambda = v; }
 
 // Replace the measurements / information weights (same structure).
 void orc_ba_set_meas(void* h, const double* meas, const double* w) {

@@ -66,6 +66,7 @@ def lib():
         L.orc_chol_solve.restype = ct.c_int
         L.orc_chol_solve.argtypes = [ct.c_int, _dp, _dp, _dp]
         L.orc_ba_set_meas.argtypes = [ct.c_void_p, _dp, _dp]
+        L.orc_ba_set_user_lambda.argtypes = [ct.c_void_p, ct.c_double]
         L.orc_ba_adjoint.restype = ct.c_int
         L.orc_ba_adjoint.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.orc_ba_newton.restype = ct.c_int
@@ -179,6 +180,9 @@ class OracleBA:
                                   times.ctypes.data_as(_dp))
         return n, trace[:n], times
 
+
+    def set_user_lambda(self, lam: float):
+        lib().orc_ba_set_user_lambda(self.h, float(lam))
 
     def set_meas(self, meas, weight):
         m, pm = _d(meas)

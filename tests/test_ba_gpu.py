@@ -146,6 +146,28 @@ def test_lm_trace_fp64(case, request, oracle_lib):
     assert _rel(p, po) < 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lm_trace_with_rejected_trials(precision, oracle_lib):
+    """A tiny initial lambda makes the first steps overshoot, so trials are
+    rejected and popped: the next trial must linearise at the restored
+    state (regression: the device camera table once kept the rejected one)."""
+    import g2o_amd
+    prob = datasets.make_walk_ba(seed=8, num_cams=60, num_pts=6000)
+    prec = g2o_amd.FP64 if precision == "fp64" else g2o_amd.FP32
+    ba = _mk(prob, precision=prec, refine_steps=0 if precision == "fp64" else 1)
+    ba.set_options(user_lambda_init=1e-9)
+    ora = oracle_lib.OracleBA(prob)
+    ora.set_user_lambda(1e-9)
+    n_o, trace, _ = ora.optimize(8, solver=1, tol=1e-6)
+    stats = ba.optimize(8)
+    _say(f"{precision}: trials gpu {[s['levenberg_iterations'] for s in stats]} cpu {list(trace[:, 3].astype(int))}")
+    assert trace[:, 3].sum() > n_o  # some trials were rejected
+    assert len(stats) == n_o
+    for s, t in zip(stats, trace):
+        assert s["levenberg_iterations"] == int(t[3])
+        assert abs(s["chi2"] - t[1]) <= (1e-8 if precision == "fp64" else 1e-6) * t[1]
+
+
 @pytest.mark.parametrize("refine", [0, 1, 2])
 def test_lm_fp32_refine(refine, walk, oracle_lib):
     import g2o_amd
