@@ -148,16 +148,14 @@ def test_lm_trace_fp64(case, request, oracle_lib):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_lm_trace_with_rejected_trials(precision, oracle_lib):
-    """A tiny initial lambda makes the first steps overshoot, so trials are
+    """A start far from the optimum makes a step overshoot, so a trial is
     rejected and popped: the next trial must linearise at the restored
     state (regression: the device camera table once kept the rejected one)."""
     import g2o_amd
-    prob = datasets.make_walk_ba(seed=8, num_cams=60, num_pts=6000)
+    prob = datasets.make_walk_ba(seed=8, num_cams=60, num_pts=6000, sig_t=0.3, sig_r=0.01, sig_p=0.5)
     prec = g2o_amd.FP64 if precision == "fp64" else g2o_amd.FP32
     ba = _mk(prob, precision=prec, refine_steps=0 if precision == "fp64" else 1)
-    ba.set_options(user_lambda_init=1e-9)
     ora = oracle_lib.OracleBA(prob)
-    ora.set_user_lambda(1e-9)
     n_o, trace, _ = ora.optimize(8, solver=1, tol=1e-6)
     stats = ba.optimize(8)
     _say(f"{precision}: trials gpu {[s['levenberg_iterations'] for s in stats]} cpu {list(trace[:, 3].astype(int))}")
