@@ -44,7 +44,7 @@ class IterStats(ct.Structure):
                 ("refine_steps", ct.c_int), ("status", ct.c_int), ("time_residuals", ct.c_double),
                 ("time_quadratic_form", ct.c_double), ("time_schur_complement", ct.c_double),
                 ("time_linear_solver", ct.c_double), ("time_update", ct.c_double),
-                ("time_iteration", ct.c_double)]
+                ("time_iteration", ct.c_double), ("fp64_fallback", ct.c_int)]
 
 
 _dp = ct.POINTER(ct.c_double)

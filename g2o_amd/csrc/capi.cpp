@@ -65,6 +65,7 @@ int g2ohip_set_options(g2ohip_ctx* ctx, const g2ohip_options* opt) {
     return ctx->fail(G2OHIP_ERR_INVALID, "invalid option value");
   ctx->opt = *opt;
   ctx->pcg_residual_mem = -1.0;  // LinearSolverPCG::init
+  ctx->fp32_off = false;
   return G2OHIP_OK;
 }
 
@@ -237,7 +238,7 @@ int g2ohip_ba_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, d
 int g2ohip_last_solve_info(g2ohip_ctx* ctx, double* info) {
   G2OHIP_ENTER(ctx);
   if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  for (int i = 0; i < 3; ++i) info[i] = ctx->last_info[i];
+  for (int i = 0; i < 4; ++i) info[i] = ctx->last_info[i];
   return G2OHIP_OK;
 }
 

@@ -316,7 +316,7 @@ int g2ohip_ctx::build_system() {
   }
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
-  const int prec = opt.precision;
+  const int prec = (opt.precision == G2OHIP_FP32 && !fp32_off) ? G2OHIP_FP32 : G2OHIP_FP64;
   if (prec == G2OHIP_FP32) {
     HK(f32.Hpl.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
     ProfScope ps(this, G2OHIP_K_LINEARIZE);
@@ -366,11 +366,11 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
   const int nb = st.Cf;
   const int n = 6 * nb;
   HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.d2.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
-  HT* const dirs[2] = {B.d.p, B.d2.p};
+  double* const dirs[2] = {B.d.p, B.d2.p};
   ProfScope ps(this, G2OHIP_K_PCG);
   if (implicit) {
-    auto op = [&](int it, int last, const HT* d_old, HT* d) -> hipError_t {
-      hipError_t e = launch_pcg_dir<HT, 6>(nb, B.s.p, d_old, d, d_dn_part.p, d_sc.p, ctl, it, last, stream);
+    auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
+      hipError_t e = launch_pcg_dir<6>(nb, B.s.p, d_old, d, d_dn_part.p, d_sc.p, ctl, it, last, stream);
       if (e != hipSuccess || last) return e;
       e = launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, d, nullptr, d_cam_part.p, d_sc.p, it, stream);
       if (e != hipSuccess) return e;
@@ -379,7 +379,7 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
     HK((pcg_drive<HT, 6>(stream, h_pinned, nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p, d_dn_part.p,
                          d_dq_part.p, pcg_row_blocks(nb), d_sc.p, max_iter, hint, op, out)));
   } else {
-    auto op = [&](int it, int last, const HT* d_old, HT* d) -> hipError_t {
+    auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
       return launch_pcg_dir_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, d_old, d, B.q.p, d_dn_part.p,
                                         d_dq_part.p, d_sc.p, ctl, it, last, stream);
     };
@@ -414,25 +414,34 @@ int g2ohip_ctx::solve_impl(int* ok) {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
       HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lambda, B.F_val.p, d_bschur.p, nullptr, 0,
                                stream));
-    } else {
+    } else if (!std::is_same<HT, float>::value) {  // FP32 forms its diagonal blocks in FP64 below
       HK(B.Sdiag.alloc(36 * static_cast<size_t>(nb)));
       HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lambda, nullptr, nullptr, B.Sdiag.p, 1,
                                stream));
     }
   }
-  if (!implicit) {
-    HK(launch_jacobi<HT, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream));
-  } else {
-    // matrix-free reduced operator; block-Jacobi preconditioner from the
-    // diagonal blocks of the Schur complement
+  const bool fp32 = std::is_same<HT, float>::value;
+  if (implicit) {
+    // matrix-free reduced operator: right-hand side by one pass
     HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, nullptr, d_b.p, d_cam_part.p, nullptr, 0, stream));
     HK(launch_implicit_rhs_reduce(g, d_cam_part.p, d_b.p, d_bschur.p, stream));
+  }
+  if (fp32) {
+    // FP32 blocks: the diagonal Schur blocks (operator diagonal and
+    // block-Jacobi preconditioner) are formed in FP64 from the state
+    ProfScope ps(this, G2OHIP_K_SCHUR);
+    HK(launch_schur_diag64<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, lambda, d_hpp_part.p, implicit ? nullptr : B.F_val.p,
+                               B.Jinv.p, d_err.p, stream));
+  } else if (!implicit) {
+    HK(launch_jacobi<HT, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream));
+  } else {
     HK(launch_jacobi_damped<HT, 6>(nb, B.Sdiag.p, d_iota.p, lambda, B.Jinv.p, d_err.p, stream));
   }
   int err = 0;
   CK(read_err(&err));
   if (err) { *ok = 0; return G2OHIP_OK; }
   const bool refine = (which == 0) && opt.refine_steps > 0;
+  refine_stalled = false;
   const int max_iter = opt.pcg_max_iter < 0 ? 6 * nb : opt.pcg_max_iter;
   const double* xp64 = d_x.p;
   double* xl64 = d_x.p + 6 * static_cast<size_t>(nb);
@@ -444,13 +453,13 @@ int g2ohip_ctx::solve_impl(int* ok) {
   pcg_hint[0] = pr.iters;
   int total_it = pr.iters;
   double dn_final = pr.dn_final;
-  HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 0.0, stream));
+  HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 0.0, stream));
   if (refine) {
     // FP64 iterative refinement: r = g - S x evaluated in FP64 matrix-free from
     // the state (no FP32 block enters it), then an FP32 PCG correction S dx = r
     // that stops on the first solve's absolute threshold d0, so the refined x
     // meets the reference stop rule measured on the FP64 residual. A
-    // correction never runs longer than the first solve plus 10 iterations.
+    // correction runs at most twice the first solve's iterations plus 10.
     for (int step = 0; step < opt.refine_steps; ++step) {
       {
         ProfScope ps(this, G2OHIP_K_REFINE);
@@ -459,11 +468,18 @@ int g2ohip_ctx::solve_impl(int* ok) {
       HK(launch_refine_reduce(g, d_cam_part.p, xp64, lambda, d_res.p, stream));
       PcgRun pc;
       CK(run_pcg<HT>(B, d_res.p, PcgControl{opt.pcg_tolerance, -1.0, pr.d0}, implicit,
-                     std::min(max_iter, pr.iters + 10), pcg_hint[1], &pc));
+                     std::min(max_iter, 2 * pr.iters + 10), pcg_hint[1], &pc));
       pcg_hint[1] = pc.iters;
       total_it += pc.iters;
-      dn_final = pc.dn_final;
-      if (pc.iters > 0) HK(launch_to_double<HT>(6 * nb, B.x.p, d_x.p, 1.0, stream));
+      // the FP32-rounded system stops resolving the FP64 problem: its
+      // solution misses the FP64 threshold by more than 10x, or the
+      // correction stops above it
+      if (pc.dn0 > 10.0 * pr.d0 || pc.dn_final > pr.d0 || pc.breakdown) refine_stalled = true;
+      // residual memory as the FP64 solver leaves it: at most half the
+      // threshold the solve stopped on, also when a correction stopped at
+      // its iteration cap above it
+      dn_final = std::min(pc.dn_final, pr.d0);
+      if (pc.iters > 0) HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 1.0, stream));
     }
   }
   pcg_residual_mem = 0.5 * dn_final;
@@ -488,8 +504,19 @@ int g2ohip_ctx::solve(int* ok) {
     return pg_solve(ok);
   }
   if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
-  if (system_prec == G2OHIP_FP32) return solve_impl<float>(ok);
-  return solve_impl<double>(ok);
+  last_info[3] = 0;
+  if (system_prec != G2OHIP_FP32) return solve_impl<double>(ok);
+  const double mem = pcg_residual_mem;
+  CK(solve_impl<float>(ok));
+  if (!refine_stalled) return G2OHIP_OK;
+  // FP64 fallback at the same state (the linearisation point: the LM solves
+  // before it updates)
+  fp32_off = true;
+  pcg_residual_mem = mem;
+  CK(build_system());
+  CK(solve_impl<double>(ok));
+  last_info[3] = 1;
+  return G2OHIP_OK;
 }
 
 // --------------------------------------------------------- update / stack
@@ -546,6 +573,7 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
   *done = 0;
   pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
   pcg_hint[0] = pcg_hint[1] = 0;
+  fp32_off = false;
   double ni = 2.0;
   for (int i = 0; i < iterations; ++i) {
     if (force_stop && *force_stop) break;
@@ -572,6 +600,7 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
       s.time_linear_solver += now_s() - t;
       s.iterations_linear_solver += static_cast<int>(last_info[0]);
       s.refine_steps += static_cast<int>(last_info[1]);
+      s.fp64_fallback |= static_cast<int>(last_info[3]);
       double tempChi = std::numeric_limits<double>::max();
       if (ok) {
         t = now_s();

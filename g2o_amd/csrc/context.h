@@ -50,7 +50,7 @@ struct PrecBufs {
   DBuf<HT> Jinv;      // Cf * 36
   DBuf<HT> Dinv;      // Pf * 9, (Hll + lambda I)^-1 of the current solve
   DBuf<HT> Sdiag;     // Cf * 36 (implicit mode preconditioner)
-  DBuf<HT> x, r, d, d2, q, s;  // PCG vectors (d, d2: directions of odd / even iterations)
+  DBuf<double> x, r, d, d2, q, s;  // PCG vectors, FP64 (d, d2: directions of odd / even iterations)
 };
 
 // Outcome of one PCG solve (read back from PcgScalars).
@@ -84,7 +84,14 @@ struct g2ohip_ctx {
   bool have_exact = false;    // the FP64 blocks hold the exact Hessian (adjoint / Newton)
   double lambda = 0.0;
   double pcg_residual_mem = -1.0;  // LinearSolverPCG::_residual
-  double last_info[3] = {0, 0, -1};
+  double last_info[4] = {0, 0, -1, 0};
+  // FP32 blocks with FP64 refinement: once the FP32 solution's FP64 residual
+  // misses the first solve's threshold by more than 10x, or a correction
+  // stops above it (the FP32-rounded reduced system no longer resolves its
+  // weakest modes), the solve is redone in FP64 and FP64 is kept until the
+  // next optimize() / set_options()
+  bool fp32_off = false;
+  bool refine_stalled = false;
   double chi2_lin = 0.0;
 
   g2ohip::DevGraph g{};

@@ -88,11 +88,12 @@ hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* 
 template <typename HT>
 hipError_t launch_point_dinv(const DevGraph& g, const double* Hll, double lambda, HT* Dinv, int* err, hipStream_t st);
 template <typename HT>
-hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const HT* p, const double* b,
+hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const double* p, const double* b,
                                 double* cam_part, const PcgScalars* sc, int it, hipStream_t st);
 template <typename HT>
 hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, const double* Hpp, double lambda,
-                                  const HT* p, HT* q, double* dq_part, PcgScalars* sc, int it, hipStream_t st);
+                                  const double* p, double* q, double* dq_part, PcgScalars* sc, int it,
+                                  hipStream_t st);
 hipError_t launch_implicit_rhs_reduce(const DevGraph& g, const double* cam_part, const double* b, double* bschur,
                                       hipStream_t st);
 
@@ -108,25 +109,32 @@ hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, i
 template <typename HT, int BD>
 hipError_t launch_jacobi_damped(int nb, const HT* F_val, const int* F_diag, double lambda, HT* Jinv, int* err,
                                 hipStream_t st);
+// Schur diagonal blocks in FP64 from the state (no cancellation against Hpp):
+// F_val's diagonal blocks (if F_val) and the block-Jacobi inverses Jinv.
+// part21: 21 doubles per (chunk, camera) row.
+template <typename HT>
+hipError_t launch_schur_diag64(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
+                               double lambda, double* part21, HT* F_val, HT* Jinv, int* err, hipStream_t st);
 // Two launches per iteration: pcg_dir_spmv (or pcg_dir + the matrix-free
-// operator) then pcg_update. Partials: part_rs has pcg_row_blocks(nb)
+// operator) then pcg_update. Vectors are FP64, the matrix and the
+// preconditioner blocks MT. Partials: part_rs has pcg_row_blocks(nb)
 // entries, part_dq pcg_spmv_blocks(nb) (explicit) or pcg_row_blocks(Cf)
 // (implicit). d and d_old swap every iteration.
 int pcg_row_blocks(int nb);
 int pcg_spmv_blocks(int nb);
-template <typename HT, int BD>
-hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* s, double* part_rs,
-                           hipStream_t st);
-template <typename HT, int BD>
-hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* s,
-                               const HT* d_old, HT* d, HT* q, const double* part_rs, double* part_dq,
+template <typename MT, int BD>
+hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
+                           double* part_rs, hipStream_t st);
+template <typename MT, int BD>
+hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
+                               const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);
-template <typename HT, int BD>
-hipError_t launch_pcg_dir(int nb, const HT* s, const HT* d_old, HT* d, const double* part_rs, PcgScalars* sc,
-                          const PcgControl& ctl, int it, int last, hipStream_t st);
-template <typename HT, int BD>
-hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
-                             const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
+template <int BD>
+hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
+                          PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);
+template <typename MT, int BD>
+hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const double* d, const double* q,
+                             double* s, const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st);
 // ---- exact Hessian (adjoint, Newton polish), adjoint.hip
 hipError_t launch_linearize_exact(const DevGraph& g, const double* camRt, const double* pts, double* Hpl,

@@ -607,7 +607,8 @@ template hipError_t launch_point_dinv<double>(const DevGraph&, const double*, do
 // sc != null: PCG iteration `it` (skipped once the solve is over).
 template <typename HT>
 __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const HT* __restrict__ Hpl,
-                                                              const HT* __restrict__ Dinv, const HT* __restrict__ p,
+                                                              const HT* __restrict__ Dinv,
+                                                              const double* __restrict__ p,
                                                               const double* __restrict__ b,
                                                               double* __restrict__ cam_part, PcgScalars* sc, int it) {
   if (sc && sc->done) return;
@@ -628,12 +629,12 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
       const int pos = g.slot_pos[s];
       const int chh = g.camH[g.slot_cam[s]];
       if (g.pos_h[pos] < 0 || chh < 0) continue;
-      const HT* pc = p + 6 * static_cast<size_t>(chh);
+      const double* pc = p + 6 * static_cast<size_t>(chh);
       const HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]);
       for (int c = 0; c < 3; ++c) {
-        HT v = 0;
-        for (int r = 0; r < 6; ++r) v += h[3 * r + c] * pc[r];
-        atomicAdd(&s_v[3 * (pos - p0) + c], static_cast<double>(v));
+        double v = 0;
+        for (int r = 0; r < 6; ++r) v += static_cast<double>(h[3 * r + c]) * pc[r];
+        atomicAdd(&s_v[3 * (pos - p0) + c], v);
       }
     }
     __syncthreads();
@@ -659,12 +660,12 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
     const int chh = g.camH[g.slot_cam[s]];
     if (g.pos_h[pos] < 0 || chh < 0) continue;
     const double* y = s_v + 3 * (pos - p0);
-    const HT y0 = static_cast<HT>(y[0]), y1 = static_cast<HT>(y[1]), y2 = static_cast<HT>(y[2]);
     const int lc = lower_bound_i(s_cam, nc, chh);
     for (int r = 0; r < 6; ++r) {
       const HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]) + 3 * r;
-      HT v = h[0] * y0 + h[1] * y1 + h[2] * y2;
-      atomicAdd(&s_cacc[6 * lc + r], static_cast<double>(v));
+      const double v = static_cast<double>(h[0]) * y[0] + static_cast<double>(h[1]) * y[1] +
+                       static_cast<double>(h[2]) * y[2];
+      atomicAdd(&s_cacc[6 * lc + r], v);
     }
   }
   __syncthreads();
@@ -672,13 +673,13 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
 }
 
 template <typename HT>
-hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const HT* p, const double* b,
+hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const double* p, const double* b,
                                 double* cam_part, const PcgScalars* sc, int it, hipStream_t st) {
   hipLaunchKernelGGL(k_implicit_pass<HT>, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, Hpl, Dinv, p, b, cam_part,
                      const_cast<PcgScalars*>(sc), it);
   return hipGetLastError();
 }
-template hipError_t launch_implicit_pass<float>(const DevGraph&, const float*, const float*, const float*,
+template hipError_t launch_implicit_pass<float>(const DevGraph&, const float*, const float*, const double*,
                                                 const double*, double*, const PcgScalars*, int, hipStream_t);
 template hipError_t launch_implicit_pass<double>(const DevGraph&, const double*, const double*, const double*,
                                                  const double*, double*, const PcgScalars*, int, hipStream_t);
@@ -687,14 +688,14 @@ template hipError_t launch_implicit_pass<double>(const DevGraph&, const double*,
 template <typename HT>
 __global__ __launch_bounds__(64) void k_implicit_reduce(DevGraph g, const double* __restrict__ cam_part,
                                                         const double* __restrict__ Hpp, double lambda,
-                                                        const HT* __restrict__ p, HT* __restrict__ q,
+                                                        const double* __restrict__ p, double* __restrict__ q,
                                                         double* __restrict__ dq_part, const PcgScalars* sc) {
   if (sc->done) return;
   const int c = blockIdx.x * 64 + threadIdx.x;
   double dq = 0;
   if (c < g.Cf) {
     double pc[6], out[6];
-    for (int r = 0; r < 6; ++r) pc[r] = static_cast<double>(p[6 * static_cast<size_t>(c) + r]);
+    for (int r = 0; r < 6; ++r) pc[r] = p[6 * static_cast<size_t>(c) + r];
     for (int r = 0; r < 6; ++r) {
       double v = lambda * pc[r];
       for (int k = 0; k < 6; ++k) v += Hpp[36 * static_cast<size_t>(c) + 6 * r + k] * pc[k];
@@ -705,9 +706,8 @@ __global__ __launch_bounds__(64) void k_implicit_reduce(DevGraph g, const double
       for (int r = 0; r < 6; ++r) out[r] -= pp[r];
     }
     for (int r = 0; r < 6; ++r) {
-      HT v = static_cast<HT>(out[r]);
-      q[6 * static_cast<size_t>(c) + r] = v;
-      dq += static_cast<double>(v) * pc[r];
+      q[6 * static_cast<size_t>(c) + r] = out[r];
+      dq += out[r] * pc[r];
     }
   }
   dq = wave_sum(dq);
@@ -716,14 +716,15 @@ __global__ __launch_bounds__(64) void k_implicit_reduce(DevGraph g, const double
 
 template <typename HT>
 hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, const double* Hpp, double lambda,
-                                  const HT* p, HT* q, double* dq_part, PcgScalars* sc, int it, hipStream_t st) {
+                                  const double* p, double* q, double* dq_part, PcgScalars* sc, int it,
+                                  hipStream_t st) {
   (void)it;
   hipLaunchKernelGGL(k_implicit_reduce<HT>, dim3(nblocks(g.Cf, 64)), dim3(64), 0, st, g, cam_part, Hpp, lambda, p,
                      q, dq_part, sc);
   return hipGetLastError();
 }
-template hipError_t launch_implicit_reduce<float>(const DevGraph&, const double*, const double*, double, const float*,
-                                                  float*, double*, PcgScalars*, int, hipStream_t);
+template hipError_t launch_implicit_reduce<float>(const DevGraph&, const double*, const double*, double,
+                                                  const double*, double*, double*, PcgScalars*, int, hipStream_t);
 template hipError_t launch_implicit_reduce<double>(const DevGraph&, const double*, const double*, double,
                                                    const double*, double*, double*, PcgScalars*, int, hipStream_t);
 
@@ -969,6 +970,154 @@ template hipError_t launch_jacobi_damped<float, 6>(int, const float*, const int*
 template hipError_t launch_jacobi_damped<double, 6>(int, const double*, const int*, double, double*, int*,
                                                     hipStream_t);
 
+// ------------------------------------------ FP64 diagonal Schur blocks
+// S_ii = Hpp_ii + lambda I - sum_p W_ip Hpl_ip^T subtracts terms of the size
+// of Hpp (~1e9 here) to leave a block whose smallest eigenvalue can be ~1e2:
+// formed from FP32 blocks it loses positive definiteness. Per observation the
+// same block is omega Jc^T (I - omega Jp Dinv_p Jp^T) Jc, a sum of positive
+// semi-definite terms, so it is formed here in FP64, re-linearised from the
+// state (observations of fixed points have Dinv = 0; two observations of one
+// point by the same camera add their cross terms). Chunk partials, 21 per
+// (chunk, camera), reduced in a fixed order by k_schur_diag_finish.
+__device__ __forceinline__ void obs_jacobians64(const DevGraph& g, const double* __restrict__ camRt,
+                                                const double* __restrict__ pts, int s, double* Jp, double* Jc,
+                                                double* om) {
+  const int pos = g.slot_pos[s];
+  const int praw = g.pos_pt[pos];
+  const int cam = g.slot_cam[s];
+  const double* Rt = camRt + 12 * cam;
+  const double* in = g.intr + 4 * cam;
+  const double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+  const double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
+  const double w = g.slot_w[s];
+  double Xc[3], e[2], rho[3];
+  project_error(Rt, in, X, z, Xc, e);
+  huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+  *om = rho[1] * w;
+  project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+}
+
+__global__ __launch_bounds__(LIN_THREADS) void k_schur_diag64(DevGraph g, const double* __restrict__ camRt,
+                                                             const double* __restrict__ pts,
+                                                             const double* __restrict__ Hll, double lambda,
+                                                             double* __restrict__ part21, int* __restrict__ err) {
+  __shared__ int s_cam[LIN_CAMS];
+  __shared__ double s_cacc[LIN_CAMS * 21];
+  __shared__ double s_D[LIN_PTS * 6];
+  const int ch = blockIdx.x, tid = threadIdx.x;
+  const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
+  const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
+  const int c0 = g.lc_cam_ptr[ch], nc = g.lc_cam_ptr[ch + 1] - c0;
+  const int np = p1 - p0;
+  for (int i = tid; i < nc; i += LIN_THREADS) s_cam[i] = g.lc_cams[c0 + i];
+  for (int i = tid; i < nc * 21; i += LIN_THREADS) s_cacc[i] = 0.0;
+  for (int lp = tid; lp < np; lp += LIN_THREADS) {
+    const int ph = g.pos_h[p0 + lp];
+    double D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (ph >= 0 && !inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
+      atomicOr(err, 1);
+      for (int i = 0; i < 9; ++i) D[i] = 0;
+    }
+    double* o = s_D + 6 * lp;
+    o[0] = D[0]; o[1] = D[1]; o[2] = D[2]; o[3] = D[4]; o[4] = D[5]; o[5] = D[8];
+  }
+  __syncthreads();
+  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
+    const int chh = g.slot_camh[s];
+    if (chh < 0) continue;
+    const int pos = g.slot_pos[s];
+    const double* d6 = s_D + 6 * (pos - p0);
+    const double D[9] = {d6[0], d6[1], d6[2], d6[1], d6[3], d6[4], d6[2], d6[4], d6[5]};
+    double Jp[6], Jc[12], om;
+    obs_jacobians64(g, camRt, pts, s, Jp, Jc, &om);
+    // JD = Jp D (2x3); M = JD Jp^T (2x2)
+    double JD[6];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 3; ++c)
+        JD[3 * r + c] = Jp[3 * r] * D[c] + Jp[3 * r + 1] * D[3 + c] + Jp[3 * r + 2] * D[6 + c];
+    double M[4];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c)
+        M[2 * r + c] = JD[3 * r] * Jp[3 * c] + JD[3 * r + 1] * Jp[3 * c + 1] + JD[3 * r + 2] * Jp[3 * c + 2];
+    const double P00 = 1.0 - om * M[0], P01 = -om * 0.5 * (M[1] + M[2]), P11 = 1.0 - om * M[3];
+    double C[21];
+    int k = 0;
+    for (int r = 0; r < 6; ++r)
+      for (int c = r; c < 6; ++c) {
+        const double a0 = Jc[r], a1 = Jc[6 + r], b0 = Jc[c], b1 = Jc[6 + c];
+        C[k++] = om * (a0 * (P00 * b0 + P01 * b1) + a1 * (P01 * b0 + P11 * b1));
+      }
+    // another observation of the same point by the same camera: cross terms
+    // -om om2 Jc^T (Jp D Jp2^T) Jc2 (this slot's row side; the other slot adds
+    // the transposed side)
+    for (int s2 = g.pos_ptr[pos]; s2 < g.pos_ptr[pos + 1]; ++s2) {
+      if (s2 == s || g.slot_camh[s2] != chh) continue;
+      double Jp2[6], Jc2[12], om2;
+      obs_jacobians64(g, camRt, pts, s2, Jp2, Jc2, &om2);
+      double X2[4];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c)
+          X2[2 * r + c] = JD[3 * r] * Jp2[3 * c] + JD[3 * r + 1] * Jp2[3 * c + 1] + JD[3 * r + 2] * Jp2[3 * c + 2];
+      k = 0;
+      for (int r = 0; r < 6; ++r)
+        for (int c = r; c < 6; ++c) {
+          // symmetric part of Jc^T X2 Jc2 (the pair (s2, s) supplies the rest)
+          const double v1 = Jc[r] * (X2[0] * Jc2[c] + X2[1] * Jc2[6 + c]) +
+                            Jc[6 + r] * (X2[2] * Jc2[c] + X2[3] * Jc2[6 + c]);
+          const double v2 = Jc[c] * (X2[0] * Jc2[r] + X2[1] * Jc2[6 + r]) +
+                            Jc[6 + c] * (X2[2] * Jc2[r] + X2[3] * Jc2[6 + r]);
+          C[k++] -= om * om2 * 0.5 * (v1 + v2);
+        }
+    }
+    double* ca = s_cacc + 21 * lower_bound_i(s_cam, nc, chh);
+    for (int q = 0; q < 21; ++q) atomicAdd(&ca[q], C[q]);
+  }
+  __syncthreads();
+  for (int i = tid; i < nc * 21; i += LIN_THREADS) part21[static_cast<size_t>(c0) * 21 + i] = s_cacc[i];
+}
+
+// S_ii = lambda I + sum of the camera's partials (fixed order); writes the
+// block into F_val (if not null) and its inverse into Jinv.
+template <typename HT>
+__global__ void k_schur_diag_finish(DevGraph g, const double* __restrict__ part21, double lambda,
+                                    HT* __restrict__ F_val, HT* __restrict__ Jinv, int* __restrict__ err) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.Cf) return;
+  double m[36];
+  for (int k = 0; k < 21; ++k) {
+    double v = 0;
+    for (int q = g.camp_ptr[c]; q < g.camp_ptr[c + 1]; ++q) v += part21[21 * static_cast<size_t>(g.camp_idx[q]) + k];
+    const int r = c_up6_r[k], cc = c_up6_c[k];
+    m[6 * r + cc] = v;
+    m[6 * cc + r] = v;
+  }
+  for (int k = 0; k < 6; ++k) m[7 * k] += lambda;
+  if (F_val) {
+    HT* dst = F_val + 36 * static_cast<size_t>(g.F_diag[c]);
+    for (int k = 0; k < 36; ++k) dst[k] = static_cast<HT>(m[k]);
+  }
+  if (!inv_spd<6>(m)) {
+    atomicOr(err, 2);
+    for (int k = 0; k < 36; ++k) m[k] = 0;
+  }
+  for (int k = 0; k < 36; ++k) Jinv[36 * static_cast<size_t>(c) + k] = static_cast<HT>(m[k]);
+}
+
+template <typename HT>
+hipError_t launch_schur_diag64(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
+                               double lambda, double* part21, HT* F_val, HT* Jinv, int* err, hipStream_t st) {
+  if (g.nlc > 0)
+    hipLaunchKernelGGL(k_schur_diag64, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hll, lambda, part21,
+                       err);
+  hipLaunchKernelGGL(k_schur_diag_finish<HT>, dim3(nblocks(g.Cf, 64)), dim3(64), 0, st, g, part21, lambda, F_val,
+                     Jinv, err);
+  return hipGetLastError();
+}
+template hipError_t launch_schur_diag64<float>(const DevGraph&, const double*, const double*, const double*, double,
+                                               double*, float*, float*, int*, hipStream_t);
+template hipError_t launch_schur_diag64<double>(const DevGraph&, const double*, const double*, const double*, double,
+                                                double*, double*, double*, int*, hipStream_t);
+
 // ----------------------------------------------------- block-Jacobi PCG
 // LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) as two launches per
 // iteration. Direction update, convergence test and SpMV are fused: the new
@@ -993,31 +1142,36 @@ __device__ double wave_sum_array(const double* __restrict__ part, int n) {
   return s + c;
 }
 
-template <typename HT>
-__device__ __forceinline__ HT pcg_dir_val(HT s, HT d_old, HT beta, bool first) {
+__device__ __forceinline__ double pcg_dir_val(double s, double d_old, double beta, bool first) {
   return first ? s : fma(beta, d_old, s);
 }
 
+// Vectors (x, r, d, q, s) and all recurrences are FP64; the matrix F_val and
+// the block-Jacobi inverses J are stored in MT (FP32 or FP64). With MT =
+// float the iteration is FP64 PCG on the FP32-rounded operator: the Krylov
+// process keeps its FP64 orthogonality, and the bandwidth-heavy data (the
+// matrix) moves at half width.
+
 // x = 0, r = b, s = M^-1 r, part[block] = r . s   (linear_solver_pcg.hpp:120-130)
-template <typename HT, int BD>
-__global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restrict__ b, const HT* __restrict__ J,
-                                                 HT* __restrict__ x, HT* __restrict__ r, HT* __restrict__ s,
-                                                 double* __restrict__ part) {
+template <typename MT, int BD>
+__global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restrict__ b, const MT* __restrict__ J,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ s, double* __restrict__ part) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   double dot = 0;
   if (i < nb) {
-    HT rv[BD];
+    double rv[BD];
     for (int k = 0; k < BD; ++k) {
-      rv[k] = static_cast<HT>(b[BD * static_cast<size_t>(i) + k]);
+      rv[k] = b[BD * static_cast<size_t>(i) + k];
       r[BD * static_cast<size_t>(i) + k] = rv[k];
-      x[BD * static_cast<size_t>(i) + k] = HT(0);
+      x[BD * static_cast<size_t>(i) + k] = 0.0;
     }
-    const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
+    const MT* Ji = J + static_cast<size_t>(i) * BD * BD;
     for (int k = 0; k < BD; ++k) {
-      HT v = 0;
-      for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+      double v = 0;
+      for (int c = 0; c < BD; ++c) v += static_cast<double>(Ji[BD * k + c]) * rv[c];
       s[BD * static_cast<size_t>(i) + k] = v;
-      dot += static_cast<double>(rv[k]) * static_cast<double>(v);
+      dot += rv[k] * v;
     }
   }
   dot = wave_sum(dot);
@@ -1062,43 +1216,43 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
 // the full block rows (both triangles). One wavefront per block row; lanes
 // split into 64 / BD groups striding over the row's blocks, lane k of a group
 // owns output row k. part_dq[block] = sum over its rows of d . q.
-template <typename HT, int BD>
+template <typename MT, int BD>
 __global__ __launch_bounds__(256) void k_pcg_dir_spmv(int nb, const int* __restrict__ F_ptr,
-                                                      const int* __restrict__ F_col, const HT* __restrict__ F_val,
-                                                      const HT* __restrict__ s, const HT* __restrict__ d_old,
-                                                      HT* __restrict__ d, HT* __restrict__ q,
+                                                      const int* __restrict__ F_col, const MT* __restrict__ F_val,
+                                                      const double* __restrict__ s, const double* __restrict__ d_old,
+                                                      double* __restrict__ d, double* __restrict__ q,
                                                       const double* __restrict__ part_rs, int n_rs,
                                                       double* __restrict__ part_dq, PcgScalars* sc, double tol,
                                                       double resid_mem, double d0_abs, int it, int last) {
-  double beta_d;
-  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta_d)) return;
+  double beta;
+  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta)) return;
   const bool firstit = it == 0;
-  const HT beta = static_cast<HT>(beta_d);
   constexpr int G = 64 / BD;
-  __shared__ HT s_part[4][64];
+  __shared__ double s_part[4][64];
   __shared__ double s_dot[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int row = blockIdx.x * 4 + wv;
   const int grp = lane / BD, k = lane % BD;
-  HT acc = 0;
+  double acc = 0;
   if (row < nb && grp < G) {
     for (int p = F_ptr[row] + grp; p < F_ptr[row + 1]; p += G) {
-      const HT* V = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
+      const MT* V = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
       const size_t j = BD * static_cast<size_t>(F_col[p]);
-      for (int c = 0; c < BD; ++c) acc += V[c] * pcg_dir_val(s[j + c], firstit ? HT(0) : d_old[j + c], beta, firstit);
+      for (int c = 0; c < BD; ++c)
+        acc += static_cast<double>(V[c]) * pcg_dir_val(s[j + c], firstit ? 0.0 : d_old[j + c], beta, firstit);
     }
   }
   s_part[wv][lane] = acc;
   __syncthreads();
   double dot = 0;
   if (row < nb && lane < BD) {
-    HT v = 0;
+    double v = 0;
     for (int g2 = 0; g2 < G; ++g2) v += s_part[wv][g2 * BD + lane];
     const size_t o = BD * static_cast<size_t>(row) + lane;
-    const HT dv = pcg_dir_val(s[o], firstit ? HT(0) : d_old[o], beta, firstit);
+    const double dv = pcg_dir_val(s[o], firstit ? 0.0 : d_old[o], beta, firstit);
     q[o] = v;
     d[o] = dv;
-    dot = static_cast<double>(v) * static_cast<double>(dv);
+    dot = v * dv;
   }
   dot = wave_sum(dot);
   if (lane == 0) s_dot[wv] = dot;
@@ -1107,32 +1261,31 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv(int nb, const int* __restr
 }
 
 // Direction only (matrix-free operator): d = s + beta d_old, thread per row.
-template <typename HT, int BD>
-__global__ __launch_bounds__(64) void k_pcg_dir(int nb, const HT* __restrict__ s, const HT* __restrict__ d_old,
-                                                HT* __restrict__ d, const double* __restrict__ part_rs, int n_rs,
-                                                PcgScalars* sc, double tol, double resid_mem, double d0_abs, int it,
-                                                int last) {
-  double beta_d;
-  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta_d)) return;
-  const HT beta = static_cast<HT>(beta_d);
+template <int BD>
+__global__ __launch_bounds__(64) void k_pcg_dir(int nb, const double* __restrict__ s,
+                                                const double* __restrict__ d_old, double* __restrict__ d,
+                                                const double* __restrict__ part_rs, int n_rs, PcgScalars* sc,
+                                                double tol, double resid_mem, double d0_abs, int it, int last) {
+  double beta;
+  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta)) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= nb) return;
   for (int k = 0; k < BD; ++k) {
     const size_t o = BD * static_cast<size_t>(i) + k;
-    d[o] = pcg_dir_val(s[o], it == 0 ? HT(0) : d_old[o], beta, it == 0);
+    d[o] = pcg_dir_val(s[o], it == 0 ? 0.0 : d_old[o], beta, it == 0);
   }
 }
 
 // a = dn / d.q; x += a d; r -= a q; s = M^-1 r; part_rs[block] = r . s
-template <typename HT, int BD>
-__global__ __launch_bounds__(64) void k_pcg_update(int nb, const HT* __restrict__ J, HT* __restrict__ x,
-                                                   HT* __restrict__ r, const HT* __restrict__ d,
-                                                   const HT* __restrict__ q, HT* __restrict__ s,
+template <typename MT, int BD>
+__global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict__ J, double* __restrict__ x,
+                                                   double* __restrict__ r, const double* __restrict__ d,
+                                                   const double* __restrict__ q, double* __restrict__ s,
                                                    const double* __restrict__ part_dq, int n_dq,
                                                    double* __restrict__ part_rs, PcgScalars* sc, int it) {
   if (sc->done) return;
   const double dq = wave_sum_array(part_dq, n_dq);
-  const HT a = static_cast<HT>(sc->dn[it & 1] / dq);
+  const double a = sc->dn[it & 1] / dq;
   // d . q <= 0: the operator is not positive definite along d (reported in
   // sc->err for callers that need it; the iteration itself follows
   // linear_solver_pcg.hpp, which does not check)
@@ -1140,19 +1293,19 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const HT* __restrict_
   const int i = blockIdx.x * 64 + threadIdx.x;
   double dot = 0;
   if (i < nb) {
-    HT rv[BD];
+    double rv[BD];
     for (int k = 0; k < BD; ++k) {
       const size_t o = BD * static_cast<size_t>(i) + k;
-      x[o] += a * d[o];
-      rv[k] = r[o] - a * q[o];
+      x[o] = fma(a, d[o], x[o]);
+      rv[k] = fma(-a, q[o], r[o]);
       r[o] = rv[k];
     }
-    const HT* Ji = J + static_cast<size_t>(i) * BD * BD;
+    const MT* Ji = J + static_cast<size_t>(i) * BD * BD;
     for (int k = 0; k < BD; ++k) {
-      HT v = 0;
-      for (int c = 0; c < BD; ++c) v += Ji[BD * k + c] * rv[c];
+      double v = 0;
+      for (int c = 0; c < BD; ++c) v += static_cast<double>(Ji[BD * k + c]) * rv[c];
       s[BD * static_cast<size_t>(i) + k] = v;
-      dot += static_cast<double>(rv[k]) * static_cast<double>(v);
+      dot += rv[k] * v;
     }
   }
   dot = wave_sum(dot);
@@ -1162,50 +1315,52 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const HT* __restrict_
 int pcg_row_blocks(int nb) { return nblocks(nb, 64); }
 int pcg_spmv_blocks(int nb) { return nblocks(nb, 4); }
 
-template <typename HT, int BD>
-hipError_t launch_pcg_init(int nb, const double* b, const HT* Jinv, HT* x, HT* r, HT* s, double* part_rs,
-                           hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_init<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs);
+template <typename MT, int BD>
+hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
+                           double* part_rs, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs);
   return hipGetLastError();
 }
 
-template <typename HT, int BD>
-hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const HT* F_val, const HT* s,
-                               const HT* d_old, HT* d, HT* q, const double* part_rs, double* part_dq,
+template <typename MT, int BD>
+hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
+                               const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_dir_spmv<HT, BD>), dim3(nblocks(nb, 4)), dim3(256), 0, st, nb, F_ptr, F_col, F_val, s,
+  hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD>), dim3(nblocks(nb, 4)), dim3(256), 0, st, nb, F_ptr, F_col, F_val, s,
                      d_old, d, q, part_rs, nblocks(nb, 64), part_dq, sc, ctl.tol, ctl.resid_mem, ctl.d0_abs, it,
                      last);
   return hipGetLastError();
 }
 
-template <typename HT, int BD>
-hipError_t launch_pcg_dir(int nb, const HT* s, const HT* d_old, HT* d, const double* part_rs, PcgScalars* sc,
-                          const PcgControl& ctl, int it, int last, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_dir<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, s, d_old, d, part_rs,
+template <int BD>
+hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
+                          PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_dir<BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, s, d_old, d, part_rs,
                      nblocks(nb, 64), sc, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
   return hipGetLastError();
 }
+template hipError_t launch_pcg_dir<6>(int, const double*, const double*, double*, const double*, PcgScalars*,
+                                      const PcgControl&, int, int, hipStream_t);
 
-template <typename HT, int BD>
-hipError_t launch_pcg_update(int nb, const HT* Jinv, HT* x, HT* r, const HT* d, const HT* q, HT* s,
-                             const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
+template <typename MT, int BD>
+hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const double* d, const double* q,
+                             double* s, const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_update<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, Jinv, x, r, d, q, s,
+  hipLaunchKernelGGL((k_pcg_update<MT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, Jinv, x, r, d, q, s,
                      part_dq, n_dq, part_rs, sc, it);
   return hipGetLastError();
 }
 
-#define G2OHIP_PCG_INST(HT, BD)                                                                                     \
-  template hipError_t launch_jacobi<HT, BD>(int, const HT*, const int*, HT*, int*, hipStream_t);                   \
-  template hipError_t launch_pcg_init<HT, BD>(int, const double*, const HT*, HT*, HT*, HT*, double*, hipStream_t); \
-  template hipError_t launch_pcg_dir_spmv<HT, BD>(int, const int*, const int*, const HT*, const HT*, const HT*,   \
-                                                  HT*, HT*, const double*, double*, PcgScalars*,                 \
-                                                  const PcgControl&, int, int, hipStream_t);                     \
-  template hipError_t launch_pcg_dir<HT, BD>(int, const HT*, const HT*, HT*, const double*, PcgScalars*,          \
-                                             const PcgControl&, int, int, hipStream_t);                          \
-  template hipError_t launch_pcg_update<HT, BD>(int, const HT*, HT*, HT*, const HT*, const HT*, HT*,              \
-                                                const double*, int, double*, PcgScalars*, int, hipStream_t);
+#define G2OHIP_PCG_INST(MT, BD)                                                                                    \
+  template hipError_t launch_jacobi<MT, BD>(int, const MT*, const int*, MT*, int*, hipStream_t);                  \
+  template hipError_t launch_pcg_init<MT, BD>(int, const double*, const MT*, double*, double*, double*, double*, \
+                                              hipStream_t);                                                      \
+  template hipError_t launch_pcg_dir_spmv<MT, BD>(int, const int*, const int*, const MT*, const double*,         \
+                                                  const double*, double*, double*, const double*, double*,       \
+                                                  PcgScalars*, const PcgControl&, int, int, hipStream_t);        \
+  template hipError_t launch_pcg_update<MT, BD>(int, const MT*, double*, double*, const double*, const double*,  \
+                                                double*, const double*, int, double*, PcgScalars*, int,          \
+                                                hipStream_t);
 G2OHIP_PCG_INST(float, 6)
 G2OHIP_PCG_INST(double, 6)
 G2OHIP_PCG_INST(double, 3)

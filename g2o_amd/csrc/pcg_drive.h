@@ -17,10 +17,11 @@ namespace g2ohip {
 // once, so the host only polls the scalars: first after `hint` iterations
 // (the previous solve's count), then every 4. If the loop reaches max_iter
 // the head runs once more to record the final residual.
-template <typename HT, int BD, typename OpFn>
-inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const double* rhs, const HT* J, HT* x, HT* r,
-                     HT* s, HT* const dirs[2], HT* q, double* part_rs, double* part_dq, int n_dq,
-                     PcgScalars* sc, int max_iter, int hint, OpFn op, PcgRun* out) {
+template <typename MT, int BD, typename OpFn>
+inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const double* rhs, const MT* J, double* x,
+                            double* r, double* s, double* const dirs[2], double* q, double* part_rs,
+                            double* part_dq, int n_dq, PcgScalars* sc, int max_iter, int hint, OpFn op,
+                            PcgRun* out) {
   hipError_t e;
   PcgScalars hs{};
   auto poll = [&]() -> hipError_t {
@@ -30,12 +31,12 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const do
     return e2;
   };
   if ((e = hipMemsetAsync(sc, 0, sizeof(PcgScalars), stream)) != hipSuccess) return e;
-  if ((e = launch_pcg_init<HT, BD>(nb, rhs, J, x, r, s, part_rs, stream)) != hipSuccess) return e;
+  if ((e = launch_pcg_init<MT, BD>(nb, rhs, J, x, r, s, part_rs, stream)) != hipSuccess) return e;
   int next = std::max(4, hint);
   bool done = false;
   for (int it = 0; it < max_iter; ++it) {
     if ((e = op(it, 0, dirs[(it + 1) & 1], dirs[it & 1])) != hipSuccess) return e;
-    if ((e = launch_pcg_update<HT, BD>(nb, J, x, r, dirs[it & 1], q, s, part_dq, n_dq, part_rs, sc, it, stream)) !=
+    if ((e = launch_pcg_update<MT, BD>(nb, J, x, r, dirs[it & 1], q, s, part_dq, n_dq, part_rs, sc, it, stream)) !=
         hipSuccess)
       return e;
     if (it + 1 == next) {

@@ -118,7 +118,7 @@ class BlockSolverHIP {
     return _dev.ok(g2ohip_compute_scale(_dev.get(), &s)) ? s : 0;
   }
   int iterationsLinearSolver() {
-    double info[3];
+    double info[4];
     return _dev.ok(g2ohip_last_solve_info(_dev.get(), info)) ? static_cast<int>(info[0]) : -1;
   }
   // the whole SparseOptimizer::optimize loop on the device

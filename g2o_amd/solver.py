@@ -245,7 +245,8 @@ class BundleAdjuster:
         return dz, dw, it.value
 
     def last_solve_info(self):
-        v = np.zeros(3)
+        """[PCG iterations, refinement corrections, PCG residual memory, 1 if FP64 fallback]"""
+        v = np.zeros(4)
         self.ctx.call("g2ohip_last_solve_info", _dp(v))
         return v
 

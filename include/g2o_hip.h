@@ -52,7 +52,14 @@ typedef struct g2ohip_options {
   int schur_mode;         /* g2ohip_schur_mode */
   double pcg_tolerance;   /* LinearSolverPCG::setTolerance */
   int pcg_max_iter;       /* LinearSolverPCG::setMaxIterations, <0: n */
-  int refine_steps;       /* FP32 only: FP64 iterative-refinement corrections */
+  int refine_steps;       /* FP32 only: FP64 iterative-refinement corrections.
+                           * If the FP32 solution's FP64 residual misses the
+                           * first solve's threshold by more than 10x, or a
+                           * correction stops above it, the FP32 reduced
+                           * system no longer resolves its weakest modes: that
+                           * solve is redone in FP64 at the same state and the
+                           * rest of the optimize() call (or until the next
+                           * set_options) runs in FP64 */
   int max_trials;         /* OptimizationAlgorithmLevenberg maxTrialsAfterFailure */
   double user_lambda_init;/* "initialLambda" property, <=0: tau * max diag */
   double tau;             /* OptimizationAlgorithmLevenberg::_tau */
@@ -74,6 +81,7 @@ typedef struct g2ohip_iter_stats {
   double time_linear_solver;
   double time_update;
   double time_iteration;
+  int fp64_fallback;           /* 1 if an FP32 solve of this iteration was redone in FP64 */
 } g2ohip_iter_stats;
 
 void g2ohip_default_options(g2ohip_options* opt);
@@ -172,7 +180,8 @@ int g2ohip_ba_get_hessian(g2ohip_ctx* ctx, double* Hpp, double* Hll, double* Hpl
 int g2ohip_ba_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* bschur);
 /* statistics of the last g2ohip_solve: info[0] PCG iterations (summed over
  * refinement), info[1] refinement corrections, info[2] final PCG residual
- * memory (LinearSolverPCG::_residual) */
+ * memory (LinearSolverPCG::_residual), info[3] 1 if the solve fell back to
+ * FP64 (see g2ohip_options.precision) */
 int g2ohip_last_solve_info(g2ohip_ctx* ctx, double* info);
 
 /* ------------------------------------------------------- kernel timing */
