@@ -145,6 +145,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_slot_q.upload(st.slot_q, stream));
   HK(d_cam_qptr.upload(st.cam_qptr, stream));
   HK(d_q_ph.upload(st.q_ph, stream));
+  HK(d_q_slot.upload(st.q_slot, stream));
   HK(d_lc_pos0.upload(st.lc_pos0, stream));
   HK(d_lc_cam_ptr.upload(st.lc_cam_ptr, stream));
   HK(d_lc_cams.upload(st.lc_cams, stream));
@@ -412,6 +413,7 @@ int g2ohip_ctx::solve_impl(int* ok) {
     HK(launch_schur_w<HT>(g, B.Hpl.p, B.Dinv.p, B.W.p, stream));
     if (!implicit) {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
+      ProfScope pr(this, G2OHIP_K_SCHUR_ROWS);
       HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lambda, B.F_val.p, d_bschur.p, nullptr, 0,
                                stream));
     } else if (!std::is_same<HT, float>::value) {  // FP32 forms its diagonal blocks in FP64 below
@@ -430,8 +432,8 @@ int g2ohip_ctx::solve_impl(int* ok) {
     // FP32 blocks: the diagonal Schur blocks (operator diagonal and
     // block-Jacobi preconditioner) are formed in FP64 from the state
     ProfScope ps(this, G2OHIP_K_SCHUR);
-    HK(launch_schur_diag64<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, lambda, d_hpp_part.p, implicit ? nullptr : B.F_val.p,
-                               B.Jinv.p, d_err.p, stream));
+    HK(launch_schur_diag64<HT>(g, d_q_slot.p, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, lambda,
+                               implicit ? nullptr : B.F_val.p, B.Jinv.p, d_err.p, stream));
   } else if (!implicit) {
     HK(launch_jacobi<HT, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream));
   } else {

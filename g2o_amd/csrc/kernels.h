@@ -109,12 +109,14 @@ hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, i
 template <typename HT, int BD>
 hipError_t launch_jacobi_damped(int nb, const HT* F_val, const int* F_diag, double lambda, HT* Jinv, int* err,
                                 hipStream_t st);
-// Schur diagonal blocks in FP64 from the state (no cancellation against Hpp):
-// F_val's diagonal blocks (if F_val) and the block-Jacobi inverses Jinv.
-// part21: 21 doubles per (chunk, camera) row.
+// Schur diagonal blocks in FP64 (Hpp, Hll FP64; Hpl re-linearised in FP64 from
+// the state), one workgroup per free camera: F_val's diagonal blocks (if
+// F_val) and the block-Jacobi inverses Jinv. q_slot: slot of each
+// camera-major observation.
 template <typename HT>
-hipError_t launch_schur_diag64(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
-                               double lambda, double* part21, HT* F_val, HT* Jinv, int* err, hipStream_t st);
+hipError_t launch_schur_diag64(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
+                               const double* Hll, const double* Hpp, double lambda, HT* F_val, HT* Jinv, int* err,
+                               hipStream_t st);
 // Two launches per iteration: pcg_dir_spmv (or pcg_dir + the matrix-free
 // operator) then pcg_update. Vectors are FP64, the matrix and the
 // preconditioner blocks MT. Partials: part_rs has pcg_row_blocks(nb)

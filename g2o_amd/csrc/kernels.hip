@@ -184,8 +184,8 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 
 // ============================================================ linearisation
 // BlockSolver::buildSystem (block_solver.hpp:492-565) for EdgeSE3ProjectXYZ:
-// errors and Huber weights in FP64, Jacobians (edge_project_xyz.cpp:57-93)
-// and block products in HT, constructQuadraticForm
+// errors, Huber weights, Jacobians (edge_project_xyz.cpp:57-93) and the
+// Hll / Hpp / b sums in FP64, the stored Hpl blocks in HT; constructQuadraticForm
 // (base_fixed_sized_edge.hpp:48-130) with Omega' = rho' w I (robustInformation,
 // base_edge.h:156-162) and b -= ... as +J^T omega_r, omega_r = -rho' w e.
 // One workgroup per chunk of consecutive points: the slot's Hpl block goes to
@@ -227,34 +227,30 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
     chi += rho[0];
     const double om = rho[1] * w;
     const double wr0 = -om * e[0], wr1 = -om * e[1];
-    HT R[9];
-    for (int i = 0; i < 9; ++i) R[i] = static_cast<HT>(Rt[i]);
-    HT Jp[6], Jc[12];
-    project_jacobians<HT>(R, static_cast<HT>(in[0]), static_cast<HT>(in[1]), static_cast<HT>(Xc[0]),
-                          static_cast<HT>(Xc[1]), static_cast<HT>(Xc[2]), Jp, Jc);
-    const HT omh = static_cast<HT>(om);
+    // Jacobians and the Hll / Hpp / b sums in FP64 for either HT; only the
+    // stored Hpl block is rounded to HT (the FP32 path keeps Hpp and Hll
+    // accurate, which its FP64 Schur diagonal relies on)
+    double Jp[6], Jc[12];
+    project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
     if (ph >= 0) {
       double* pa = s_pacc + 9 * lp;
       int k = 0;
       for (int r = 0; r < 3; ++r)
-        for (int c = r; c < 3; ++c)
-          atomicAdd(&pa[k++], static_cast<double>(omh * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c])));
-      for (int r = 0; r < 3; ++r)
-        atomicAdd(&pa[6 + r], static_cast<double>(Jp[r]) * wr0 + static_cast<double>(Jp[3 + r]) * wr1);
+        for (int c = r; c < 3; ++c) atomicAdd(&pa[k++], om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]));
+      for (int r = 0; r < 3; ++r) atomicAdd(&pa[6 + r], Jp[r] * wr0 + Jp[3 + r] * wr1);
     }
     if (chh >= 0) {
       const int lc = lower_bound_i(s_cam, nc, chh);
       double* ca = s_cacc + 27 * lc;
       for (int k = 0; k < 21; ++k) {
         const int r = c_up6_r[k], c = c_up6_c[k];
-        atomicAdd(&ca[k], static_cast<double>(omh * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c])));
+        atomicAdd(&ca[k], om * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c]));
       }
-      for (int r = 0; r < 6; ++r)
-        atomicAdd(&ca[21 + r], static_cast<double>(Jc[r]) * wr0 + static_cast<double>(Jc[6 + r]) * wr1);
+      for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], Jc[r] * wr0 + Jc[6 + r] * wr1);
       if (ph >= 0) {
         HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]);
         for (int r = 0; r < 6; ++r)
-          for (int c = 0; c < 3; ++c) h[3 * r + c] = omh * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]);
+          for (int c = 0; c < 3; ++c) h[3 * r + c] = static_cast<HT>(om * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]));
       }
     }
   }
@@ -971,14 +967,16 @@ template hipError_t launch_jacobi_damped<double, 6>(int, const double*, const in
                                                     hipStream_t);
 
 // ------------------------------------------ FP64 diagonal Schur blocks
-// S_ii = Hpp_ii + lambda I - sum_p W_ip Hpl_ip^T subtracts terms of the size
-// of Hpp (~1e9 here) to leave a block whose smallest eigenvalue can be ~1e2:
-// formed from FP32 blocks it loses positive definiteness. Per observation the
-// same block is omega Jc^T (I - omega Jp Dinv_p Jp^T) Jc, a sum of positive
-// semi-definite terms, so it is formed here in FP64, re-linearised from the
-// state (observations of fixed points have Dinv = 0; two observations of one
-// point by the same camera add their cross terms). Chunk partials, 21 per
-// (chunk, camera), reduced in a fixed order by k_schur_diag_finish.
+// S_ii = Hpp_ii + lambda I - sum_p V_ip Dinv_p V_ip^T, V = omega Jc^T Jp (the
+// Hpl block), subtracts terms of the size of Hpp (~1e9 here) to leave a block
+// whose smallest eigenvalue can be ~1e2: from FP32 blocks it loses positive
+// definiteness. The FP32 path therefore forms these blocks in FP64: Hpp_ii
+// and Hll_p are FP64 sums already (k_linearize), V is re-linearised in FP64
+// from the state. One workgroup per free camera walks its observations in
+// camera-major order (two observations of one point by the camera are
+// adjacent and enter as one summed V), sums the 21 upper entries in
+// registers and writes the block into F_val (if not null) and its inverse
+// into Jinv. No atomics, fixed order.
 __device__ __forceinline__ void obs_jacobians64(const DevGraph& g, const double* __restrict__ camRt,
                                                 const double* __restrict__ pts, int s, double* Jp, double* Jc,
                                                 double* om) {
@@ -997,99 +995,56 @@ __device__ __forceinline__ void obs_jacobians64(const DevGraph& g, const double*
   project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
 }
 
-__global__ __launch_bounds__(LIN_THREADS) void k_schur_diag64(DevGraph g, const double* __restrict__ camRt,
-                                                             const double* __restrict__ pts,
-                                                             const double* __restrict__ Hll, double lambda,
-                                                             double* __restrict__ part21, int* __restrict__ err) {
-  __shared__ int s_cam[LIN_CAMS];
-  __shared__ double s_cacc[LIN_CAMS * 21];
-  __shared__ double s_D[LIN_PTS * 6];
-  const int ch = blockIdx.x, tid = threadIdx.x;
-  const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
-  const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
-  const int c0 = g.lc_cam_ptr[ch], nc = g.lc_cam_ptr[ch + 1] - c0;
-  const int np = p1 - p0;
-  for (int i = tid; i < nc; i += LIN_THREADS) s_cam[i] = g.lc_cams[c0 + i];
-  for (int i = tid; i < nc * 21; i += LIN_THREADS) s_cacc[i] = 0.0;
-  for (int lp = tid; lp < np; lp += LIN_THREADS) {
-    const int ph = g.pos_h[p0 + lp];
-    double D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (ph >= 0 && !inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
-      atomicOr(err, 1);
-      for (int i = 0; i < 9; ++i) D[i] = 0;
+template <typename HT>
+__global__ __launch_bounds__(256) void k_schur_diag64(DevGraph g, const int* __restrict__ q_slot,
+                                                      const double* __restrict__ camRt, const double* __restrict__ pts,
+                                                      const double* __restrict__ Hll, const double* __restrict__ Hpp,
+                                                      double lambda, HT* __restrict__ F_val, HT* __restrict__ Jinv,
+                                                      int* __restrict__ err) {
+  __shared__ double s_red[4][21];
+  const int c = blockIdx.x, tid = threadIdx.x;
+  const int qb = g.cam_qptr[c], qe = g.cam_qptr[c + 1];
+  double acc[21];
+  for (int k = 0; k < 21; ++k) acc[k] = 0;
+  for (int q = qb + tid; q < qe; q += 256) {
+    const int ph = g.q_ph[q];
+    if (q > qb && g.q_ph[q - 1] == ph) continue;  // summed by the run's first observation
+    double V[18];
+    for (int k = 0; k < 18; ++k) V[k] = 0;
+    for (int q2 = q; q2 < qe && g.q_ph[q2] == ph; ++q2) {
+      double Jp[6], Jc[12], om;
+      obs_jacobians64(g, camRt, pts, q_slot[q2], Jp, Jc, &om);
+      for (int r = 0; r < 6; ++r)
+        for (int cc = 0; cc < 3; ++cc) V[3 * r + cc] += om * (Jc[r] * Jp[cc] + Jc[6 + r] * Jp[3 + cc]);
     }
-    double* o = s_D + 6 * lp;
-    o[0] = D[0]; o[1] = D[1]; o[2] = D[2]; o[3] = D[4]; o[4] = D[5]; o[5] = D[8];
-  }
-  __syncthreads();
-  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
-    const int chh = g.slot_camh[s];
-    if (chh < 0) continue;
-    const int pos = g.slot_pos[s];
-    const double* d6 = s_D + 6 * (pos - p0);
-    const double D[9] = {d6[0], d6[1], d6[2], d6[1], d6[3], d6[4], d6[2], d6[4], d6[5]};
-    double Jp[6], Jc[12], om;
-    obs_jacobians64(g, camRt, pts, s, Jp, Jc, &om);
-    // JD = Jp D (2x3); M = JD Jp^T (2x2)
-    double JD[6];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 3; ++c)
-        JD[3 * r + c] = Jp[3 * r] * D[c] + Jp[3 * r + 1] * D[3 + c] + Jp[3 * r + 2] * D[6 + c];
-    double M[4];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c)
-        M[2 * r + c] = JD[3 * r] * Jp[3 * c] + JD[3 * r + 1] * Jp[3 * c + 1] + JD[3 * r + 2] * Jp[3 * c + 2];
-    const double P00 = 1.0 - om * M[0], P01 = -om * 0.5 * (M[1] + M[2]), P11 = 1.0 - om * M[3];
-    double C[21];
+    double D[9];
+    if (!inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
+      atomicOr(err, 1);
+      continue;
+    }
+    double VD[18];
+    for (int r = 0; r < 6; ++r)
+      for (int cc = 0; cc < 3; ++cc)
+        VD[3 * r + cc] = V[3 * r] * D[cc] + V[3 * r + 1] * D[3 + cc] + V[3 * r + 2] * D[6 + cc];
     int k = 0;
     for (int r = 0; r < 6; ++r)
-      for (int c = r; c < 6; ++c) {
-        const double a0 = Jc[r], a1 = Jc[6 + r], b0 = Jc[c], b1 = Jc[6 + c];
-        C[k++] = om * (a0 * (P00 * b0 + P01 * b1) + a1 * (P01 * b0 + P11 * b1));
-      }
-    // another observation of the same point by the same camera: cross terms
-    // -om om2 Jc^T (Jp D Jp2^T) Jc2 (this slot's row side; the other slot adds
-    // the transposed side)
-    for (int s2 = g.pos_ptr[pos]; s2 < g.pos_ptr[pos + 1]; ++s2) {
-      if (s2 == s || g.slot_camh[s2] != chh) continue;
-      double Jp2[6], Jc2[12], om2;
-      obs_jacobians64(g, camRt, pts, s2, Jp2, Jc2, &om2);
-      double X2[4];
-      for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c)
-          X2[2 * r + c] = JD[3 * r] * Jp2[3 * c] + JD[3 * r + 1] * Jp2[3 * c + 1] + JD[3 * r + 2] * Jp2[3 * c + 2];
-      k = 0;
-      for (int r = 0; r < 6; ++r)
-        for (int c = r; c < 6; ++c) {
-          // symmetric part of Jc^T X2 Jc2 (the pair (s2, s) supplies the rest)
-          const double v1 = Jc[r] * (X2[0] * Jc2[c] + X2[1] * Jc2[6 + c]) +
-                            Jc[6 + r] * (X2[2] * Jc2[c] + X2[3] * Jc2[6 + c]);
-          const double v2 = Jc[c] * (X2[0] * Jc2[r] + X2[1] * Jc2[6 + r]) +
-                            Jc[6 + c] * (X2[2] * Jc2[r] + X2[3] * Jc2[6 + r]);
-          C[k++] -= om * om2 * 0.5 * (v1 + v2);
-        }
-    }
-    double* ca = s_cacc + 21 * lower_bound_i(s_cam, nc, chh);
-    for (int q = 0; q < 21; ++q) atomicAdd(&ca[q], C[q]);
+      for (int cc = r; cc < 6; ++cc)
+        acc[k++] += VD[3 * r] * V[3 * cc] + VD[3 * r + 1] * V[3 * cc + 1] + VD[3 * r + 2] * V[3 * cc + 2];
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int k = 0; k < 21; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) s_red[wv][k] = v;
   }
   __syncthreads();
-  for (int i = tid; i < nc * 21; i += LIN_THREADS) part21[static_cast<size_t>(c0) * 21 + i] = s_cacc[i];
-}
-
-// S_ii = lambda I + sum of the camera's partials (fixed order); writes the
-// block into F_val (if not null) and its inverse into Jinv.
-template <typename HT>
-__global__ void k_schur_diag_finish(DevGraph g, const double* __restrict__ part21, double lambda,
-                                    HT* __restrict__ F_val, HT* __restrict__ Jinv, int* __restrict__ err) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.Cf) return;
+  if (tid != 0) return;
   double m[36];
   for (int k = 0; k < 21; ++k) {
-    double v = 0;
-    for (int q = g.camp_ptr[c]; q < g.camp_ptr[c + 1]; ++q) v += part21[21 * static_cast<size_t>(g.camp_idx[q]) + k];
+    const double v = ((s_red[0][k] + s_red[1][k]) + s_red[2][k]) + s_red[3][k];
     const int r = c_up6_r[k], cc = c_up6_c[k];
-    m[6 * r + cc] = v;
-    m[6 * cc + r] = v;
+    const double h = Hpp[36 * static_cast<size_t>(c) + 6 * r + cc] - v;
+    m[6 * r + cc] = h;
+    m[6 * cc + r] = h;
   }
   for (int k = 0; k < 6; ++k) m[7 * k] += lambda;
   if (F_val) {
@@ -1104,19 +1059,20 @@ __global__ void k_schur_diag_finish(DevGraph g, const double* __restrict__ part2
 }
 
 template <typename HT>
-hipError_t launch_schur_diag64(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
-                               double lambda, double* part21, HT* F_val, HT* Jinv, int* err, hipStream_t st) {
-  if (g.nlc > 0)
-    hipLaunchKernelGGL(k_schur_diag64, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hll, lambda, part21,
-                       err);
-  hipLaunchKernelGGL(k_schur_diag_finish<HT>, dim3(nblocks(g.Cf, 64)), dim3(64), 0, st, g, part21, lambda, F_val,
+hipError_t launch_schur_diag64(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
+                               const double* Hll, const double* Hpp, double lambda, HT* F_val, HT* Jinv, int* err,
+                               hipStream_t st) {
+  if (g.Cf == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_schur_diag64<HT>, dim3(g.Cf), dim3(256), 0, st, g, q_slot, camRt, pts, Hll, Hpp, lambda, F_val,
                      Jinv, err);
   return hipGetLastError();
 }
-template hipError_t launch_schur_diag64<float>(const DevGraph&, const double*, const double*, const double*, double,
-                                               double*, float*, float*, int*, hipStream_t);
-template hipError_t launch_schur_diag64<double>(const DevGraph&, const double*, const double*, const double*, double,
-                                                double*, double*, double*, int*, hipStream_t);
+template hipError_t launch_schur_diag64<float>(const DevGraph&, const int*, const double*, const double*,
+                                               const double*, const double*, double, float*, float*, int*,
+                                               hipStream_t);
+template hipError_t launch_schur_diag64<double>(const DevGraph&, const int*, const double*, const double*,
+                                                const double*, const double*, double, double*, double*, int*,
+                                                hipStream_t);
 
 // ----------------------------------------------------- block-Jacobi PCG
 // LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) as two launches per

@@ -196,7 +196,8 @@ enum g2ohip_kernel {
   G2OHIP_K_REFINE = 3,
   G2OHIP_K_CHI2 = 4,
   G2OHIP_K_UPDATE = 5,
-  G2OHIP_K_COUNT = 6
+  G2OHIP_K_SCHUR_ROWS = 6, /* the Hschur block-row kernel alone (within G2OHIP_K_SCHUR) */
+  G2OHIP_K_COUNT = 7
 };
 int g2ohip_set_profiling(g2ohip_ctx* ctx, int on);
 int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launches);
