@@ -30,17 +30,22 @@ def log(msg):
 
 def algorithmic_bytes(kernel, info, prec_bytes, lin_cam_rows):
     """Compulsory HBM bytes of one launch (every array touched once).
-    info = structure_info(): [Cf, Pf, E, nS, nlc, tiles, pairs, maxrow, maxslots, nF, lin rows, npos]."""
+    info = structure_info(): [Cf, Pf, E, nS, nlc, windows, od pairs, od partials, maxslots, nF, lin rows, npos, nq]."""
     Cf, Pf, E, nF = int(info[0]), int(info[1]), int(info[2]), int(info[9])
+    pairs, slots, nq = int(info[6]), int(info[7]), int(info[12])
     hb = 18 * prec_bytes  # one 6x3 Hpl block
     if kernel == "linearize":
         # slot: cam 4 + pos 4 + meas 16 + w 8 + Hpl write; point: ids 12 + X 24 + Hll 48 + b_l 24;
         # chunk camera partials 27 f64
         return E * (32 + hb) + Pf * 108 + lin_cam_rows * 216
-    if kernel == "schur":
-        # slot: row list 4 + slot_pos 4 + slot_cam 4 + Hpl read; point: ids 12 + Hll 48 + b_l 24;
-        # Hschur write (both triangles) + bschur
-        return E * (12 + hb) + Pf * 84 + nF * 36 * prec_bytes + Cf * 48
+    if kernel == "schur_windows":
+        # observation: Hpl block + q index 4 + point index 4; point: Hll 48 (6 unique f64);
+        # pair list 4 per pair product; partial: 36 values written + slot index 4
+        return nq * (hb + 8) + Pf * 48 + pairs * 4 + slots * (36 * prec_bytes + 4)
+    if kernel == "pcg_spmv":
+        # matrix blocks (both triangles) + column indices + row pointers; vectors s, d_old read and d, q
+        # written once (6 Cf doubles each); per-row dot partials
+        return nF * (36 * prec_bytes + 4) + (Cf + 1) * 4 + 4 * 6 * Cf * 8 + Cf * 8
     if kernel == "refine":
         # slot: cam 4 + pos 4 + meas 16 + w 8; point: ids 12 + X 24 + x_l write 24; chunk camera partials 6 f64
         return E * 32 + Pf * 60 + lin_cam_rows * 48
@@ -49,20 +54,48 @@ def algorithmic_bytes(kernel, info, prec_bytes, lin_cam_rows):
     return None
 
 
-def run_gpu(prob, precision, refine, steps, warmup, profile):
+KERNEL_SYMBOL = {"schur_windows": "k_schur_win<float>", "pcg_spmv": "k_pcg_dir_spmv<float, 6>",
+                 "linearize": "k_linearize<float>", "refine": "k_refine", "chi2": "k_chi2"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the FP32-path kernel from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE + WRITE_SIZE, each calibrated
+    against a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_ba_1500cam_500kpt.md")
+    key = f"traffic_bytes_per_launch[{KERNEL_SYMBOL.get(kernel)}]:"
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def run_gpu(prob, precision, refine, steps, warmup):
+    """Times `steps` LM iterations (profiling off), then repeats them from the
+    same start with per-kernel HIP-event timing for the breakdown."""
     import torch
     import g2o_amd
     ba = g2o_amd.BundleAdjuster(prob, device=0, precision=precision, refine_steps=refine)
     if warmup > 0:
         ba.optimize(warmup)
         ba.set_state(prob.cams, prob.pts)
-    ba.set_profiling(profile)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = ba.optimize(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = ba.profile() if profile else None
+    ba.set_state(prob.cams, prob.pts)
+    ba.set_profiling(True)
+    pstats = ba.optimize(steps)
+    prof = ba.profile()
+    ba.set_profiling(False)
+    # full PCG iterations of the profiled run (the SpMV slot also holds the
+    # cheap launches that only find the solve finished)
+    prof["pcg_spmv_iterations"] = sum(st["iterations_linear_solver"] for st in pstats)
     return ba, stats, t1 - t0, prof
 
 
@@ -117,10 +150,10 @@ def main():
     prob = datasets.make_walk_ba(seed=1002, num_cams=args.cams, num_pts=args.points)
     log(f"workload: C={prob.num_cams} P={prob.num_pts} E={prob.num_edges} ({time.perf_counter() - t:.1f} s)")
 
-    ba32, s32, dt32, prof32 = run_gpu(prob, g2o_amd.FP32, args.refine, args.steps, args.warmup, True)
+    ba32, s32, dt32, prof32 = run_gpu(prob, g2o_amd.FP32, args.refine, args.steps, args.warmup)
     log(f"fp32+refine: {len(s32)} iters in {dt32 * 1e3:.1f} ms; chi2 {s32[-1]['chi2']:.9e}")
     log(f"profile fp32: {prof32}")
-    ba64, s64, dt64, prof64 = run_gpu(prob, g2o_amd.FP64, 0, args.steps, args.warmup, True)
+    ba64, s64, dt64, prof64 = run_gpu(prob, g2o_amd.FP64, 0, args.steps, args.warmup)
     log(f"fp64: {len(s64)} iters in {dt64 * 1e3:.1f} ms; chi2 {s64[-1]['chi2']:.9e}")
     log(f"profile fp64: {prof64}")
     n32 = len(s32)
@@ -129,21 +162,23 @@ def main():
 
     info = ba32.structure_info()
     lin_cam_rows = int(info[10])
-    # dominant single-kernel class in the timed region
-    best = None
-    for k in ("linearize", "schur", "refine", "chi2"):
+    # per-kernel rooflines of the FP32 path; `roofline` is the kernel with the
+    # largest total time
+    roofs = {}
+    for k in ("pcg_spmv", "schur_windows", "linearize", "refine", "chi2"):
         tot, cnt = prof32[k]
-        if cnt and (best is None or tot > best[1]):
-            best = (k, tot, cnt)
-    roof = None
-    if best:
-        k, tot, cnt = best
+        if k == "pcg_spmv":
+            cnt = prof32["pcg_spmv_iterations"]
+        if not cnt:
+            continue
         avg_s = tot / cnt / 1e3
         bytes_ = algorithmic_bytes(k, info, 4, lin_cam_rows)
         ach = bytes_ / avg_s
-        roof = {"bound": "hbm", "kernel": k, "achieved": round(ach / 1e9, 1), "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": round(ach / PEAK_HBM, 4), "traffic": None,
-                "bytes_per_launch": int(bytes_), "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt}
+        roofs[k] = {"bound": "hbm", "kernel": KERNEL_SYMBOL[k], "achieved": round(ach / 1e9, 1),
+                    "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": round(ach / PEAK_HBM, 4),
+                    "traffic": measured_traffic(k), "bytes_per_launch": int(bytes_),
+                    "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt, "total_ms": round(tot, 3)}
+    roof = max(roofs.values(), key=lambda r: r["total_ms"]) if roofs else None
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -169,11 +204,15 @@ def main():
             "final_chi2_fp64": s64[-1]["chi2"],
             "lm_trials": [s["levenberg_iterations"] for s in s32],
             "pcg_iterations": [s["iterations_linear_solver"] for s in s32],
-            "kernel_ms_fp32": {k: round(v[0], 3) for k, v in prof32.items()},
-            "kernel_ms_fp64": {k: round(v[0], 3) for k, v in prof64.items()},
+            "kernel_ms_fp32": {k: round(v[0], 3) for k, v in prof32.items() if k in g2o_amd.KERNEL_NAMES},
+            "kernel_ms_fp64": {k: round(v[0], 3) for k, v in prof64.items() if k in g2o_amd.KERNEL_NAMES},
             "hschur_blocks": int(info[3]),
+            "schur_windows": int(info[5]),
+            "schur_offdiag_pairs": int(info[6]),
+            "schur_partials": int(info[7]),
         },
         "roofline": roof,
+        "roofline_by_kernel": roofs,
     }
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -28,8 +28,8 @@ EXPORTS = [
     "g2ohip_linear_solve_pcg", "g2ohip_pg_set_structure", "g2ohip_pg_structure_info", "g2ohip_pg_set_measurements",
     "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_ba_adjoint", "g2ohip_ba_newton",
 ]
-K_LINEARIZE, K_SCHUR, K_PCG, K_REFINE, K_CHI2, K_UPDATE, K_SCHUR_ROWS = range(7)
-KERNEL_NAMES = ["linearize", "schur", "pcg", "refine", "chi2", "update", "schur_rows"]
+K_LINEARIZE, K_SCHUR, K_PCG, K_REFINE, K_CHI2, K_UPDATE, K_SCHUR_WINDOWS, K_PCG_SPMV = range(8)
+KERNEL_NAMES = ["linearize", "schur", "pcg", "refine", "chi2", "update", "schur_windows", "pcg_spmv"]
 
 
 class Options(ct.Structure):

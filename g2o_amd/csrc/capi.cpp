@@ -37,6 +37,19 @@ int g2ohip_create(int device, g2ohip_ctx** out) {
       return G2OHIP_ERR_HIP;
     }
     c->device = device;
+    // host-mapped progress word of the PCG (optional: without it the PCG
+    // polls the stream)
+    void* hp = nullptr;
+    void* dp = nullptr;
+    if (hipHostMalloc(&hp, 64, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
+      if (hipHostGetDevicePointer(&dp, hp, 0) == hipSuccess && dp) {
+        c->h_prog = static_cast<g2ohip::PcgProgress*>(hp);
+        c->d_prog = static_cast<int*>(dp);
+      } else {
+        (void)hipHostFree(hp);
+      }
+    }
+    (void)hipGetLastError();
   }
   *out = c;
   return G2OHIP_OK;
@@ -84,7 +97,7 @@ int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_
 int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
   G2OHIP_ENTER(ctx);
   if (!ctx->have_structure || !info) return ctx->fail(G2OHIP_ERR_INVALID, "no structure");
-  int rc = ctx->ensure_rows();
+  int rc = ctx->ensure_win();
   if (rc) return rc;
   const auto& st = ctx->st;
   int maxn = 0;
@@ -94,14 +107,15 @@ int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
   info[2] = st.ns;
   info[3] = st.nS;
   info[4] = st.nlc;
-  const auto& rp = st.rows;
-  info[5] = static_cast<long long>(rp.tile_row.size());
-  info[6] = rp.pair_instances;
-  info[7] = rp.max_row_cols;
+  const auto& wp = st.win;
+  info[5] = static_cast<long long>(wp.win_obs0.size()) - 1;
+  info[6] = wp.pairs;
+  info[7] = wp.od_ptr.back();
   info[8] = maxn;
   info[9] = st.nF;
   info[10] = static_cast<long long>(st.lc_cams.size());
   info[11] = st.npos;
+  info[12] = st.nq;
   return G2OHIP_OK;
 }
 

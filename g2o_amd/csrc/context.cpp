@@ -33,6 +33,7 @@ g2ohip_ctx::~g2ohip_ctx() {
   for (auto* b : stack_c) delete b;
   for (auto* b : stack_p) delete b;
   if (h_pinned) (void)hipHostFree(h_pinned);
+  if (h_prog) (void)hipHostFree(h_prog);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -128,7 +129,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   have_structure = true;
   mode = 0;
   have_meas = have_state = have_system = have_solution = have_exact = false;
-  rows_ready = false;
+  win_ready = false;
   pcg_residual_mem = -1.0;
   stack_depth = 0;
   if (device < 0) return G2OHIP_OK;
@@ -146,6 +147,8 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_cam_qptr.upload(st.cam_qptr, stream));
   HK(d_q_ph.upload(st.q_ph, stream));
   HK(d_q_slot.upload(st.q_slot, stream));
+  HK(d_q_pt.upload(st.q_pt, stream));
+  HK(d_cam_raw.upload(st.free_cam_raw, stream));
   HK(d_lc_pos0.upload(st.lc_pos0, stream));
   HK(d_lc_cam_ptr.upload(st.lc_cam_ptr, stream));
   HK(d_lc_cams.upload(st.lc_cams, stream));
@@ -191,6 +194,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   g.pos_pt = d_pos_pt.p; g.pos_h = d_pos_h.p; g.pos_ptr = d_pos_ptr.p; g.pos_nfree = d_pos_nfree.p;
   g.slot_cam = d_slot_cam.p; g.slot_pos = d_slot_pos.p; g.slot_camh = d_slot_camh.p; g.slot_ph = d_slot_ph.p;
   g.nq = st.nq; g.slot_q = d_slot_q.p; g.cam_qptr = d_cam_qptr.p; g.q_ph = d_q_ph.p;
+  g.q_pt = d_q_pt.p; g.cam_raw = d_cam_raw.p;
   g.lc_pos0 = d_lc_pos0.p; g.lc_cam_ptr = d_lc_cam_ptr.p; g.lc_cams = d_lc_cams.p;
   g.camp_ptr = d_camp_ptr.p; g.camp_idx = d_camp_idx.p;
   g.S_row = d_S_row.p; g.S_col = d_S_col.p; g.S_diag = d_S_diag.p; g.up_pos = d_up_pos.p; g.up_posT = d_up_posT.p;
@@ -199,29 +203,31 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   return G2OHIP_OK;
 }
 
-int g2ohip_ctx::ensure_rows() {
-  if (rows_ready) return G2OHIP_OK;
-  if (st.rows.tile_row.empty()) {
-    std::string msg = build_row_plan(st, SCHUR_MAXC, SCHUR_CHUNK, 8);
+int g2ohip_ctx::ensure_win() {
+  if (win_ready) return G2OHIP_OK;
+  if (st.win.win_obs0.empty()) {
+    std::string msg = build_win_plan(st, SCHUR_WIN_OBS, SCHUR_WIN_PCH);
     if (!msg.empty()) return fail(G2OHIP_ERR_UNSUPPORTED, msg);
   }
   if (device < 0) return G2OHIP_OK;
-  const RowPlan& rp = st.rows;
-  RowPlanBufs& B = row_bufs;
-  HK(B.tile_row.upload(rp.tile_row, stream));
-  HK(B.tile_k0.upload(rp.tile_k0, stream));
-  HK(B.tile_k1.upload(rp.tile_k1, stream));
-  HK(B.tile_order.upload(rp.tile_order, stream));
-  HK(B.S_rowptr.upload(st.S_rowptr, stream));
-  HK(B.blk_pair_ptr.upload(rp.blk_pair_ptr, stream));
-  HK(B.pair_qa.upload(rp.pair_qa, stream));
-  HK(B.pair_qb.upload(rp.pair_qb, stream));
-  drows.ntiles = static_cast<int>(rp.tile_row.size());
-  drows.tile_row = B.tile_row.p; drows.tile_k0 = B.tile_k0.p; drows.tile_k1 = B.tile_k1.p;
-  drows.tile_order = B.tile_order.p; drows.S_rowptr = B.S_rowptr.p;
-  drows.blk_pair_ptr = B.blk_pair_ptr.p; drows.pair_qa = B.pair_qa.p; drows.pair_qb = B.pair_qb.p;
+  const WinPlan& wp = st.win;
+  WinPlanBufs& B = win_bufs;
+  HK(B.win_obs0.upload(wp.win_obs0, stream));
+  HK(B.win_q.upload(wp.win_q, stream));
+  HK(B.win_grp0.upload(wp.win_grp0, stream));
+  HK(B.grp_len.upload(wp.grp_len, stream));
+  HK(B.grp_row0.upload(wp.grp_row0, stream));
+  HK(B.grp_slot.upload(wp.grp_slot, stream));
+  HK(B.od_blk.upload(wp.od_blk, stream));
+  HK(B.od_ptr.upload(wp.od_ptr, stream));
+  HK(B.pair_rows.upload(wp.pair_rows, stream));
+  dwin.nwin = static_cast<int>(wp.win_obs0.size()) - 1;
+  dwin.nod = static_cast<int>(wp.od_blk.size());
+  dwin.win_obs0 = B.win_obs0.p; dwin.win_q = B.win_q.p; dwin.win_grp0 = B.win_grp0.p;
+  dwin.grp_len = B.grp_len.p; dwin.grp_row0 = B.grp_row0.p; dwin.grp_slot = B.grp_slot.p;
+  dwin.od_blk = B.od_blk.p; dwin.od_ptr = B.od_ptr.p; dwin.pair_rows = B.pair_rows.p;
   HK(hipStreamSynchronize(stream));
-  rows_ready = true;
+  win_ready = true;
   return G2OHIP_OK;
 }
 
@@ -362,9 +368,11 @@ int g2ohip_ctx::compute_scale(double* out) {
 // ------------------------------------------------------------------- PCG
 
 template <typename HT>
-int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ctl, bool implicit, int max_iter,
-                        int hint, PcgRun* out) {
+int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ctl0, bool implicit, int max_iter,
+                        PcgRun* out) {
   const int nb = st.Cf;
+  const PcgControl ctl = with_progress(ctl0);
+  bool prog_failed = false;
   const int n = 6 * nb;
   HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.d2.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
   double* const dirs[2] = {B.d.p, B.d2.p};
@@ -377,16 +385,21 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
       if (e != hipSuccess) return e;
       return launch_implicit_reduce<HT>(g, d_cam_part.p, d_Hpp.p, lambda, d, B.q.p, d_dq_part.p, d_sc.p, it, stream);
     };
-    HK((pcg_drive<HT, 6>(stream, h_pinned, nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p, d_dn_part.p,
-                         d_dq_part.p, pcg_row_blocks(nb), d_sc.p, max_iter, hint, op, out)));
+    HK((pcg_drive<HT, 6>(stream, h_pinned, progress(), nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
+                         d_dn_part.p, d_dq_part.p, implicit_reduce_blocks(nb), d_sc.p, max_iter, op, out,
+                         &prog_failed)));
   } else {
     auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
-      return launch_pcg_dir_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, d_old, d, B.q.p, d_dn_part.p,
-                                        d_dq_part.p, d_sc.p, ctl, it, last, stream);
+      const int pb = prof_begin();
+      const hipError_t e = launch_pcg_dir_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, d_old, d, B.q.p,
+                                                      d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream);
+      prof_end(G2OHIP_K_PCG_SPMV, pb);
+      return e;
     };
-    HK((pcg_drive<HT, 6>(stream, h_pinned, nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p, d_dn_part.p,
-                         d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, hint, op, out)));
+    HK((pcg_drive<HT, 6>(stream, h_pinned, progress(), nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
+                         d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, out, &prog_failed)));
   }
+  if (prog_failed) prog_ok = false;
   return G2OHIP_OK;
 }
 
@@ -404,40 +417,34 @@ int g2ohip_ctx::solve_impl(int* ok) {
   HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
-  CK(ensure_rows());
-  HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
-  HK(B.W.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
+  CK(ensure_win());
   {
     ProfScope ps(this, G2OHIP_K_SCHUR);
-    HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
-    HK(launch_schur_w<HT>(g, B.Hpl.p, B.Dinv.p, B.W.p, stream));
-    if (!implicit) {
+    if (implicit) {
+      HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
+      HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
+    } else {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
-      ProfScope pr(this, G2OHIP_K_SCHUR_ROWS);
-      HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lambda, B.F_val.p, d_bschur.p, nullptr, 0,
-                               stream));
-    } else if (!std::is_same<HT, float>::value) {  // FP32 forms its diagonal blocks in FP64 below
-      HK(B.Sdiag.alloc(36 * static_cast<size_t>(nb)));
-      HK(launch_schur_rows<HT>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lambda, nullptr, nullptr, B.Sdiag.p, 1,
-                               stream));
+      HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win.od_ptr.back(), 1))));
+      {
+        ProfScope pr(this, G2OHIP_K_SCHUR_WINDOWS);
+        HK(launch_schur_win<HT>(g, dwin, B.Hpl.p, d_Hll.p, lambda, B.part.p, d_err.p, stream));
+      }
+      HK(launch_schur_offdiag<HT>(g, dwin, B.part.p, B.F_val.p, stream));
     }
+    // diagonal blocks, block-Jacobi inverses and (explicit) the reduced
+    // right-hand side in FP64: from the FP64 blocks, or for FP32 blocks
+    // (too coarse for the cancellation in Hpp - sum V Dinv V^T)
+    // re-linearised in FP64 from the state
+    const HT* hpl_diag = std::is_same<HT, float>::value ? nullptr : B.Hpl.p;
+    HK(launch_schur_diag<HT>(g, d_q_slot.p, d_camRt.p, d_pts.p, hpl_diag, d_Hll.p, d_Hpp.p, d_b.p, lambda,
+                             implicit ? nullptr : B.F_val.p, B.Jinv.p, implicit ? nullptr : d_bschur.p, d_err.p,
+                             stream));
   }
-  const bool fp32 = std::is_same<HT, float>::value;
   if (implicit) {
     // matrix-free reduced operator: right-hand side by one pass
     HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, nullptr, d_b.p, d_cam_part.p, nullptr, 0, stream));
     HK(launch_implicit_rhs_reduce(g, d_cam_part.p, d_b.p, d_bschur.p, stream));
-  }
-  if (fp32) {
-    // FP32 blocks: the diagonal Schur blocks (operator diagonal and
-    // block-Jacobi preconditioner) are formed in FP64 from the state
-    ProfScope ps(this, G2OHIP_K_SCHUR);
-    HK(launch_schur_diag64<HT>(g, d_q_slot.p, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, lambda,
-                               implicit ? nullptr : B.F_val.p, B.Jinv.p, d_err.p, stream));
-  } else if (!implicit) {
-    HK(launch_jacobi<HT, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream));
-  } else {
-    HK(launch_jacobi_damped<HT, 6>(nb, B.Sdiag.p, d_iota.p, lambda, B.Jinv.p, d_err.p, stream));
   }
   int err = 0;
   CK(read_err(&err));
@@ -450,9 +457,7 @@ int g2ohip_ctx::solve_impl(int* ok) {
   // reduced system on the Schur right-hand side, with the residual memory of
   // the previous solve (linear_solver_pcg.hpp:128-136)
   PcgRun pr;
-  CK(run_pcg<HT>(B, d_bschur.p, PcgControl{opt.pcg_tolerance, pcg_residual_mem, -1.0}, implicit, max_iter,
-                 pcg_hint[0], &pr));
-  pcg_hint[0] = pr.iters;
+  CK(run_pcg<HT>(B, d_bschur.p, PcgControl{opt.pcg_tolerance, pcg_residual_mem, -1.0}, implicit, max_iter, &pr));
   int total_it = pr.iters;
   double dn_final = pr.dn_final;
   HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 0.0, stream));
@@ -470,8 +475,7 @@ int g2ohip_ctx::solve_impl(int* ok) {
       HK(launch_refine_reduce(g, d_cam_part.p, xp64, lambda, d_res.p, stream));
       PcgRun pc;
       CK(run_pcg<HT>(B, d_res.p, PcgControl{opt.pcg_tolerance, -1.0, pr.d0}, implicit,
-                     std::min(max_iter, 2 * pr.iters + 10), pcg_hint[1], &pc));
-      pcg_hint[1] = pc.iters;
+                     std::min(max_iter, 2 * pr.iters + 10), &pc));
       total_it += pc.iters;
       // the FP32-rounded system stops resolving the FP64 problem: its
       // solution misses the FP64 threshold by more than 10x, or the
@@ -574,7 +578,6 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   *done = 0;
   pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
-  pcg_hint[0] = pcg_hint[1] = 0;
   fp32_off = false;
   double ni = 2.0;
   for (int i = 0; i < iterations; ++i) {
@@ -757,7 +760,7 @@ int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, cons
   HK(dq.alloc(n)); HK(ds.alloc(n)); HK(ddq.alloc(pcg_spmv_blocks(nb))); HK(ddn.alloc(pcg_row_blocks(nb)));
   HK(derr.alloc(1)); HK(dsc.alloc(1));
   HK(hipMemsetAsync(derr.p, 0, sizeof(int), stream));
-  const PcgControl ctl{tol, residual_io ? *residual_io : -1.0, -1.0};
+  const PcgControl ctl = with_progress(PcgControl{tol, residual_io ? *residual_io : -1.0, -1.0});
   double* const dirs[2] = {dd.p, dd2.p};
   auto run = [&](auto bdc) -> int {
     constexpr int BD = decltype(bdc)::value;
@@ -768,8 +771,10 @@ int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, cons
                                              dsc.p, ctl, it, last, stream);
     };
     PcgRun pr;
-    HK((pcg_drive<double, BD>(stream, h_pinned, nb, db.p, dJ.p, dx.p, dr.p, ds.p, dirs, dq.p, ddn.p, ddq.p,
-                              pcg_spmv_blocks(nb), dsc.p, mi, 0, op, &pr)));
+    bool prog_failed = false;
+    HK((pcg_drive<double, BD>(stream, h_pinned, progress(), nb, db.p, dJ.p, dx.p, dr.p, ds.p, dirs, dq.p, ddn.p,
+                              ddq.p, pcg_spmv_blocks(nb), dsc.p, mi, op, &pr, &prog_failed)));
+    if (prog_failed) prog_ok = false;
     if (iterations) *iterations = pr.iters;
     if (residual_io) *residual_io = 0.5 * pr.dn_final;
     return G2OHIP_OK;
@@ -808,15 +813,15 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
   *iters = 0;
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
-  CK(ensure_rows());
+  CK(ensure_win());
   HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
-  HK(B.W.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
   HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
+  HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win.od_ptr.back(), 1))));
   HK(launch_point_dinv<double>(g, d_Hll.p, lam, B.Dinv.p, d_err.p, stream));
-  HK(launch_schur_w<double>(g, B.Hpl.p, B.Dinv.p, B.W.p, stream));
-  HK(launch_schur_rows<double>(g, drows, B.Hpl.p, B.W.p, d_b.p, d_Hpp.p, lam, B.F_val.p, d_bschur.p, nullptr, 0,
-                               stream));
-  HK((launch_jacobi<double, 6>(nb, B.F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream)));
+  HK(launch_schur_win<double>(g, dwin, B.Hpl.p, d_Hll.p, lam, B.part.p, d_err.p, stream));
+  HK(launch_schur_offdiag<double>(g, dwin, B.part.p, B.F_val.p, stream));
+  HK(launch_schur_diag<double>(g, d_q_slot.p, d_camRt.p, d_pts.p, B.Hpl.p, d_Hll.p, d_Hpp.p, d_b.p, lam, B.F_val.p,
+                               B.Jinv.p, d_bschur.p, d_err.p, stream));
   int err = 0;
   CK(read_err(&err));
   if (err) { *ok = 0; return G2OHIP_OK; }
@@ -824,8 +829,7 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
   // default iteration cap 20 n: the exact reduced systems of long camera
   // chains are ill-conditioned (cond ~1e11) and the solve is run to a tight
   // tolerance
-  CK(run_pcg<double>(B, d_bschur.p, PcgControl{tol, -1.0, -1.0}, false, max_iter < 0 ? 120 * nb : max_iter, 0,
-                     &pr));
+  CK(run_pcg<double>(B, d_bschur.p, PcgControl{tol, -1.0, -1.0}, false, max_iter < 0 ? 120 * nb : max_iter, &pr));
   *iters = pr.iters;
   if (pr.breakdown) { *ok = 0; return G2OHIP_OK; }  // (H + lam I) is not positive definite
   HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 0.0, stream));

@@ -45,11 +45,10 @@ struct DBuf {
 template <typename HT>
 struct PrecBufs {
   DBuf<HT> Hpl;       // 18 * nq, one row-major 6x3 block per observation, camera-major
-  DBuf<HT> W;         // 18 * nq, Hpl Dinv of the current solve
   DBuf<HT> F_val;     // nF * 36
   DBuf<HT> Jinv;      // Cf * 36
-  DBuf<HT> Dinv;      // Pf * 9, (Hll + lambda I)^-1 of the current solve
-  DBuf<HT> Sdiag;     // Cf * 36 (implicit mode preconditioner)
+  DBuf<HT> Dinv;      // Pf * 9, (Hll + lambda I)^-1 (implicit operator, exact back-substitution)
+  DBuf<HT> part;      // 36 per window-plan slot: off-diagonal Schur partials
   DBuf<double> x, r, d, d2, q, s;  // PCG vectors, FP64 (d, d2: directions of odd / even iterations)
 };
 
@@ -62,8 +61,9 @@ struct PcgRun {
   bool breakdown = false;  // d.q <= 0 or r.M^-1 r < 0 seen: operator or preconditioner not PD
 };
 
-struct RowPlanBufs {
-  DBuf<int> tile_row, tile_k0, tile_k1, tile_order, S_rowptr, blk_pair_ptr, pair_qa, pair_qb;
+struct WinPlanBufs {
+  DBuf<int> win_obs0, win_q, win_grp0, grp_len, grp_row0, grp_slot, od_blk, od_ptr;
+  DBuf<unsigned> pair_rows;
 };
 
 }  // namespace g2ohip
@@ -95,13 +95,13 @@ struct g2ohip_ctx {
   double chi2_lin = 0.0;
 
   g2ohip::DevGraph g{};
-  g2ohip::DBuf<int> d_camH, d_ptH, d_pos_pt, d_pos_h, d_pos_ptr, d_pos_nfree, d_slot_cam, d_slot_pos, d_slot_camh, d_slot_ph, d_slot_q, d_cam_qptr, d_q_ph, d_q_slot, d_lc_pos0,
+  g2ohip::DBuf<int> d_camH, d_ptH, d_pos_pt, d_pos_h, d_pos_ptr, d_pos_nfree, d_slot_cam, d_slot_pos, d_slot_camh, d_slot_ph, d_slot_q, d_cam_qptr, d_q_ph, d_q_slot, d_q_pt, d_cam_raw, d_lc_pos0,
       d_lc_cam_ptr, d_lc_cams, d_camp_ptr, d_camp_idx, d_S_row, d_S_col, d_S_diag, d_up_pos, d_up_posT, d_F_ptr,
       d_F_col, d_F_diag, d_iota;
   g2ohip::DBuf<double> d_meas, d_w, d_intr;
-  g2ohip::RowPlanBufs row_bufs;
-  g2ohip::DevRowPlan drows{};
-  bool rows_ready = false;
+  g2ohip::WinPlanBufs win_bufs;
+  g2ohip::DevWinPlan dwin{};
+  bool win_ready = false;
 
   g2ohip::DBuf<double> d_cams, d_pts, d_camRt;
   std::vector<g2ohip::DBuf<double>*> stack_c, stack_p;
@@ -111,7 +111,16 @@ struct g2ohip_ctx {
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;
   g2ohip::DBuf<int> d_err;
-  int pcg_hint[2] = {0, 0};  // iterations of the previous main / correction solve (first host poll)
+  // host-mapped PCG progress word (PcgControl::prog); prog_ok drops to false
+  // if it ever fails to reach the host, and the PCG then polls the stream
+  g2ohip::PcgProgress* h_prog = nullptr;
+  int* d_prog = nullptr;
+  bool prog_ok = true;
+  volatile g2ohip::PcgProgress* progress() const { return prog_ok && h_prog ? h_prog : nullptr; }
+  g2ohip::PcgControl with_progress(g2ohip::PcgControl c) const {
+    c.prog = prog_ok && h_prog ? d_prog : nullptr;
+    return c;
+  }
   g2ohip::DBuf<g2ohip::PcgScalars> d_sc;
   g2ohip::PrecBufs<float> f32;
   g2ohip::PrecBufs<double> f64;
@@ -144,7 +153,7 @@ struct g2ohip_ctx {
   int set_measurements(const double* meas, const double* w, const double* intr, double huber);
   int set_state(const double* cams, const double* pts);
   int get_state(double* cams, double* pts);
-  int ensure_rows();
+  int ensure_win();
   int compute_active_errors(double* chi2, double* err_out = nullptr);
   int build_system();
   int solve(int* ok);
@@ -186,8 +195,8 @@ struct g2ohip_ctx {
   template <typename HT>
   int solve_impl(int* ok);
   template <typename HT>
-  int run_pcg(g2ohip::PrecBufs<HT>& B, const double* rhs, const g2ohip::PcgControl& ctl, bool implicit, int max_iter,
-              int hint, g2ohip::PcgRun* out);
+  int run_pcg(g2ohip::PrecBufs<HT>& B, const double* rhs, const g2ohip::PcgControl& ctl0, bool implicit, int max_iter,
+              g2ohip::PcgRun* out);
   int read_scalar(const double* dptr, double* out);
   int read_err(int* out);
 };

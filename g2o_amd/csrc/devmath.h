@@ -301,6 +301,31 @@ __host__ __device__ inline bool inv3_sym(const double* h, double lambda, double*
   return true;
 }
 
+// Cholesky factor L L^T = A + lambda I of a symmetric 3x3 A given by its six
+// unique entries [xx xy xz yy yz zz]; writes the lower-triangular L^-1 as
+// [i00 i10 i11 i20 i21 i22]. Returns false unless A + lambda I is positive
+// definite (and finite).
+__host__ __device__ inline bool chol3_inv(const double* h, double lambda, double* m) {
+  const double a00 = h[0] + lambda;
+  if (!(a00 > 0.0) || !isfinite(a00)) return false;
+  const double l00 = sqrt(a00), l10 = h[1] / l00, l20 = h[2] / l00;
+  const double d1 = h[3] + lambda - l10 * l10;
+  if (!(d1 > 0.0) || !isfinite(d1)) return false;
+  const double l11 = sqrt(d1), l21 = (h[4] - l20 * l10) / l11;
+  const double d2 = h[5] + lambda - l20 * l20 - l21 * l21;
+  if (!(d2 > 0.0) || !isfinite(d2)) return false;
+  const double l22 = sqrt(d2);
+  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+  const double i10 = -l10 * i00 * i11;
+  m[0] = i00;
+  m[1] = i10;
+  m[2] = i11;
+  m[3] = -(l20 * i00 + l21 * i10) * i22;
+  m[4] = -l21 * i11 * i22;
+  m[5] = i22;
+  return true;
+}
+
 // 6x6 inverse (row-major, in place) by Gauss-Jordan with partial pivoting.
 __device__ inline bool inv6(double* m) {
   double a[6][12];

@@ -19,17 +19,19 @@ struct DevGraph {
   const int *slot_cam, *slot_pos, *slot_camh, *slot_ph;
   int nq;                                    // observations with a free camera and a free point
   const int *slot_q, *cam_qptr, *q_ph;       // camera-major order of Hpl
+  const int *q_pt, *cam_raw;                 // raw point per q, raw camera per free camera
   const double *slot_meas, *slot_w;
   const int *lc_pos0, *lc_cam_ptr, *lc_cams, *camp_ptr, *camp_idx;
   const int *S_row, *S_col, *S_diag, *up_pos, *up_posT, *F_ptr, *F_col, *F_diag;
   const double* intr;
 };
 
-struct DevRowPlan {
-  int ntiles;
-  const int *tile_row, *tile_k0, *tile_k1, *tile_order;
-  const int* S_rowptr;
-  const int *blk_pair_ptr, *pair_qa, *pair_qb;
+// Device view of the window plan of the off-diagonal Schur blocks
+// (structure.h, WinPlan).
+struct DevWinPlan {
+  int nwin, nod;
+  const int *win_obs0, *win_q, *win_grp0, *grp_len, *grp_row0, *grp_slot, *od_blk, *od_ptr;
+  const unsigned* pair_rows;
 };
 
 // Scalars of one PCG solve, kept on the device so iterations need no host
@@ -46,10 +48,19 @@ struct PcgScalars {
 
 // Stop rule of one solve: d0 = tol * dn0 floored by resid_mem (the previous
 // solve's residual memory, ignored when <= 0), or exactly d0_abs when >= 0.
+// prog (device address of host-mapped memory, or null): the iteration heads
+// store the iteration reached in prog[0] and 1 in prog[1] once done.
 struct PcgControl {
   double tol;
   double resid_mem;
   double d0_abs;
+  int* prog = nullptr;
+};
+
+// Host view of PcgControl::prog.
+struct PcgProgress {
+  int it;
+  int done;
 };
 
 // ---- state / residuals
@@ -70,19 +81,18 @@ hipError_t launch_max_diag(const DevGraph& g, const double* Hpp, const double* H
 hipError_t launch_scale(int n, const double* x, const double* b, double lambda, double* partial, double* out,
                         hipStream_t st);
 
-// ---- explicit Schur complement (row-owner). diag_only: only the diagonal
-// blocks (no damping on Hpp) into Sdiag[Cf*36] - the implicit-mode
-// preconditioner; otherwise all row blocks into both triangles of F_val and
-// bschur.
-constexpr int SCHUR_MAXC = 64;      // Hschur blocks per row tile
-constexpr int SCHUR_CHUNK = 256;    // pair products per work item
-// W = Hpl Dinv per observation, camera-major (Dinv from launch_point_dinv).
+// ---- explicit Schur complement
+constexpr int SCHUR_WIN_OBS = 512;  // observations per window (LDS-staged G blocks)
+constexpr int SCHUR_WIN_PCH = 32;   // pair products per partial
+// Off-diagonal partials: part[slot*36] = sum of G_a G_b^T over the slot's
+// pairs, G = Hpl L^-T with L L^T = Hll + lambda I (err |= 1 if not PD).
 template <typename HT>
-hipError_t launch_schur_w(const DevGraph& g, const HT* Hpl, const HT* Dinv, HT* W, hipStream_t st);
+hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* Hpl, const double* Hll, double lambda,
+                            HT* part, int* err, hipStream_t st);
+// Sums every off-diagonal upper block's partials (fixed order, FP64) and
+// writes -sum and its transpose into both triangles of F_val.
 template <typename HT>
-hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const HT* W, const double* b,
-                             const double* Hpp, double lambda, HT* F_val, double* bschur, HT* Sdiag, int diag_only,
-                             hipStream_t st);
+hipError_t launch_schur_offdiag(const DevGraph& g, const DevWinPlan& wp, const HT* part, HT* F_val, hipStream_t st);
 
 // ---- implicit Schur operator
 template <typename HT>
@@ -106,24 +116,24 @@ hipError_t launch_refine_reduce(const DevGraph& g, const double* cam_part, const
 // ---- block-Jacobi PCG (LinearSolverPCG semantics), block size BD
 template <typename HT, int BD>
 hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, int* err, hipStream_t st);
-template <typename HT, int BD>
-hipError_t launch_jacobi_damped(int nb, const HT* F_val, const int* F_diag, double lambda, HT* Jinv, int* err,
-                                hipStream_t st);
-// Schur diagonal blocks in FP64 (Hpp, Hll FP64; Hpl re-linearised in FP64 from
-// the state), one workgroup per free camera: F_val's diagonal blocks (if
-// F_val) and the block-Jacobi inverses Jinv. q_slot: slot of each
-// camera-major observation.
+// Schur diagonal blocks in FP64, one workgroup per free camera: S_ii =
+// Hpp_ii + lambda I - sum_p V_ip Dinv_p V_ip^T with V the Hpl block,
+// re-linearised in FP64 from the state (Hpl == nullptr) or read from Hpl.
+// Writes F_val's diagonal blocks (if F_val), their inverses Jinv (err |= 2
+// if not PD) and, if bschur, bschur_i = b_i - sum_p V_ip Dinv_p b_p.
+// q_slot: slot of each camera-major observation.
 template <typename HT>
-hipError_t launch_schur_diag64(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
-                               const double* Hll, const double* Hpp, double lambda, HT* F_val, HT* Jinv, int* err,
-                               hipStream_t st);
+hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
+                             const HT* Hpl, const double* Hll, const double* Hpp, const double* b, double lambda,
+                             HT* F_val, HT* Jinv, double* bschur, int* err, hipStream_t st);
 // Two launches per iteration: pcg_dir_spmv (or pcg_dir + the matrix-free
 // operator) then pcg_update. Vectors are FP64, the matrix and the
 // preconditioner blocks MT. Partials: part_rs has pcg_row_blocks(nb)
-// entries, part_dq pcg_spmv_blocks(nb) (explicit) or pcg_row_blocks(Cf)
-// (implicit). d and d_old swap every iteration.
+// entries, part_dq pcg_spmv_blocks(nb) (explicit) or
+// implicit_reduce_blocks(Cf) (implicit). d and d_old swap every iteration.
 int pcg_row_blocks(int nb);
 int pcg_spmv_blocks(int nb);
+int implicit_reduce_blocks(int Cf);
 template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
                            double* part_rs, hipStream_t st);

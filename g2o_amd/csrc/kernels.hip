@@ -370,210 +370,193 @@ hipError_t launch_scale(int n, const double* x, const double* b, double lambda, 
 }
 
 // ========================================================== explicit Schur
-// BlockSolver::solve, Schur part (block_solver.hpp:345-433), row-owner form.
+// BlockSolver::solve, Schur part (block_solver.hpp:345-433):
 // Hschur(i, j) = Hpp(i, i) [i == j] + lambda I [i == j]
 //              - sum over free points p seen by i and j of Hpl_ip Dinv_p Hpl_jp^T,
-// Dinv_p = (Hll_p + lambda I)^-1 (f64), and bschur_i = b_i - sum_p Hpl_ip Dinv_p b_p.
-// One workgroup per tile of block row i (camera i): each thread takes
-// observations (point p, slot a of camera i) of that camera, forms
-// W = Hpl_a Dinv_p once and the products W Hpl_b^T for the point's slots b with
-// camera >= i. Off-diagonal products go to the row's LDS accumulator; the
-// diagonal block and the coefficient stay in registers and are reduced over
-// the workgroup at the end. Every Hschur block is written once, by its row's
-// tile, to both triangles of the block-CSR copy; nothing is accumulated in
-// global memory.
-// DPP move within 16-lane rows (quad_perm / row_ror controls).
-template <int CTRL>
-__device__ __forceinline__ float dpp_mov(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-  const long long x = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(x), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(x >> 32), CTRL, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-// Sum over the 16 lanes of a DPP row, result in every lane of the row.
-template <typename T>
-__device__ __forceinline__ T row16_sum(T v) {
-  v += dpp_mov<0xB1>(v);   // quad_perm(1, 0, 3, 2)
-  v += dpp_mov<0x4E>(v);   // quad_perm(2, 3, 0, 1)
-  v += dpp_mov<0x124>(v);  // row_ror:4
-  v += dpp_mov<0x128>(v);  // row_ror:8
-  return v;
-}
-
-// W_q = Hpl_q Dinv_p for every observation q (camera-major), Dinv_p =
-// (Hll_p + lambda I)^-1 of the observation's point.
+// Dinv_p = (Hll_p + lambda I)^-1, and bschur_i = b_i - sum_p Hpl_ip Dinv_p b_p.
+// Off-diagonal blocks (i < j) come from point windows: with L_p L_p^T =
+// Hll_p + lambda I and G = Hpl L_p^-T, the point's term is G_ip G_jp^T. A
+// workgroup stages its window's G blocks in LDS once (every observation is
+// read from HBM once per solve) and one lane forms each partial sum of up
+// to SCHUR_WIN_PCH pair products in registers, no atomics; a second pass sums
+// each block's partials in a fixed order. Diagonal blocks and bschur are
+// formed per camera in FP64 (k_schur_diag).
 template <typename HT>
-__global__ __launch_bounds__(256) void k_schur_w(int nq, const int* __restrict__ q_ph, const HT* __restrict__ Hpl,
-                                                 const HT* __restrict__ Dinv, HT* __restrict__ W) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
-  const HT* D = Dinv + 9 * static_cast<size_t>(q_ph[q]);
-  const HT* H = Hpl + 18 * static_cast<size_t>(q);
-  HT d[9], h[18];
-  for (int k = 0; k < 9; ++k) d[k] = D[k];
-  for (int k = 0; k < 18; ++k) h[k] = H[k];
-  HT* o = W + 18 * static_cast<size_t>(q);
-  for (int r = 0; r < 6; ++r)
-    for (int c = 0; c < 3; ++c) o[3 * r + c] = h[3 * r] * d[c] + h[3 * r + 1] * d[3 + c] + h[3 * r + 2] * d[6 + c];
-}
+struct GStride;
+template <>
+struct GStride<float> { static constexpr int v = 20; };  // 80-byte rows: 16-byte aligned
+template <>
+struct GStride<double> { static constexpr int v = 18; };
 
-template <typename HT>
-hipError_t launch_schur_w(const DevGraph& g, const HT* Hpl, const HT* Dinv, HT* W, hipStream_t st) {
-  if (g.nq == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_schur_w<HT>, dim3(nblocks(g.nq, 256)), dim3(256), 0, st, g.nq, g.q_ph, Hpl, Dinv, W);
-  return hipGetLastError();
-}
-template hipError_t launch_schur_w<float>(const DevGraph&, const float*, const float*, float*, hipStream_t);
-template hipError_t launch_schur_w<double>(const DevGraph&, const double*, const double*, double*, hipStream_t);
-
-// Hschur block row i, row-owner form. Block (i, j) = sum over the points p
-// seen by both of W_ip Hpl_jp^T, where W_ip and Hpl_jp sit at positions qa
-// and qb of the camera-major lists (runs of consecutive points give
-// consecutive q on both sides). The tile's blocks are cut into work items of
-// at most CH pair products; a 16-lane group takes an item, its lanes stride
-// over the pairs accumulating in registers, the group sums them with DPP row
-// operations and adds the item into the block's LDS accumulator (one atomic
-// add per element per item). The first tile of a row also forms the
-// right-hand-side coefficient sum_p W_ip b_p.
-template <typename HT, int MAXC, int CH>
-__global__ __launch_bounds__(256) void k_schur_rows(DevGraph g, DevRowPlan rp, const HT* __restrict__ Hpl,
-                                                    const HT* __restrict__ W, const double* __restrict__ b,
-                                                    const double* __restrict__ Hpp, double lambda,
-                                                    HT* __restrict__ F_val, double* __restrict__ bschur,
-                                                    HT* __restrict__ Sdiag, int diag_only) {
-  __shared__ HT s_acc[MAXC * 36];
-  __shared__ int s_item[MAXC + 1];
-  __shared__ double s_red[4 * 8];
-  const int tile = rp.tile_order[blockIdx.x];
-  const int row = rp.tile_row[tile];
-  const int k0 = rp.tile_k0[tile], k1 = diag_only ? rp.tile_k0[tile] + 1 : rp.tile_k1[tile];
-  const int ncol = k1 - k0;
-  const bool diag_tile = (k0 == rp.S_rowptr[row]);
-  if (diag_only && !diag_tile) return;
-  const int tid = threadIdx.x;
-  const int grp = tid >> 4, gl = tid & 15;  // 16 groups of 16 lanes (one DPP row each)
-  for (int i = tid; i < ncol * 36; i += 256) s_acc[i] = HT(0);
-  if (tid == 0) {
-    int acc = 0;
-    for (int c = 0; c < ncol; ++c) {
-      s_item[c] = acc;
-      acc += (rp.blk_pair_ptr[k0 + c + 1] - rp.blk_pair_ptr[k0 + c] + CH - 1) / CH;
-    }
-    s_item[ncol] = acc;
+__device__ __forceinline__ void lds_load18(const float* p, float* o) {
+  const float4* p4 = reinterpret_cast<const float4*>(p);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float4 v = p4[k];
+    o[4 * k] = v.x; o[4 * k + 1] = v.y; o[4 * k + 2] = v.z; o[4 * k + 3] = v.w;
   }
-  // ---- right-hand-side coefficient over the camera's observations
-  double coef[6] = {0, 0, 0, 0, 0, 0};
-  if (diag_tile && !diag_only) {
-    const double* bl_base = b + 6 * static_cast<size_t>(g.Cf);
-    for (int q = g.cam_qptr[row] + tid; q < g.cam_qptr[row + 1]; q += 256) {
-      const HT* w = W + 18 * static_cast<size_t>(q);
-      const double* bl = bl_base + 3 * static_cast<size_t>(g.q_ph[q]);
-      for (int r = 0; r < 6; ++r)
-        coef[r] += static_cast<double>(w[3 * r]) * bl[0] + static_cast<double>(w[3 * r + 1]) * bl[1] +
-                   static_cast<double>(w[3 * r + 2]) * bl[2];
+  const float2 v = reinterpret_cast<const float2*>(p)[8];
+  o[16] = v.x; o[17] = v.y;
+}
+__device__ __forceinline__ void lds_load18(const double* p, double* o) {
+  const double2* p2 = reinterpret_cast<const double2*>(p);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const double2 v = p2[k];
+    o[2 * k] = v.x; o[2 * k + 1] = v.y;
+  }
+}
+__device__ __forceinline__ void store36(float* o, const float* a) {
+  float4* o4 = reinterpret_cast<float4*>(o);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o4[k] = make_float4(a[4 * k], a[4 * k + 1], a[4 * k + 2], a[4 * k + 3]);
+}
+__device__ __forceinline__ void store36(double* o, const double* a) {
+  double2* o2 = reinterpret_cast<double2*>(o);
+#pragma unroll
+  for (int k = 0; k < 18; ++k) o2[k] = make_double2(a[2 * k], a[2 * k + 1]);
+}
+
+// Vector chunk of an Hpl block for the staging copy: 9 chunks per block.
+template <typename HT>
+struct Chunk;
+template <>
+struct Chunk<float> { using T = float2; static constexpr int per_block = 9; };   // 72-byte block
+template <>
+struct Chunk<double> { using T = double2; static constexpr int per_block = 9; };  // 144-byte block
+
+// Multiply-accumulate of one pair: acc += Ga Gb^T (6x3 times 3x6).
+template <typename HT>
+__device__ __forceinline__ void pair_mac(const HT* __restrict__ sG, unsigned pr, HT* acc) {
+  constexpr int GS = GStride<HT>::v;
+  HT Ga[18], Gb[18];
+  lds_load18(sG + (pr & 0xFFFFu) * GS, Ga);
+  lds_load18(sG + (pr >> 16) * GS, Gb);
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      acc[6 * i + j] += Ga[3 * i] * Gb[3 * j] + Ga[3 * i + 1] * Gb[3 * j + 1] + Ga[3 * i + 2] * Gb[3 * j + 2];
+}
+
+template <typename HT>
+__global__ __launch_bounds__(256) void k_schur_win(DevGraph g, DevWinPlan wp, const HT* __restrict__ Hpl,
+                                                  const double* __restrict__ Hll, double lambda,
+                                                  HT* __restrict__ part, int* __restrict__ err) {
+  constexpr int GS = GStride<HT>::v;
+  using CT = typename Chunk<HT>::T;
+  constexpr int NCH = Chunk<HT>::per_block;
+  __shared__ __align__(16) HT s_G[SCHUR_WIN_OBS * GS];
+  // q of each window observation: in the padding of the FP32 rows (keeps the
+  // FP32 footprint at 40 KiB, four workgroups per CU), else its own array
+  constexpr bool QPAD = GS * sizeof(HT) >= 18 * sizeof(HT) + sizeof(int);
+  __shared__ int s_qa[QPAD ? 1 : SCHUR_WIN_OBS];
+  auto s_q = [&](int l) -> int& {
+    if constexpr (QPAD) return *reinterpret_cast<int*>(s_G + l * GS + 18);
+    else return s_qa[l];
+  };
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int o0 = wp.win_obs0[w], nob = wp.win_obs0[w + 1] - o0;
+  for (int l = tid; l < nob; l += 256) s_q(l) = wp.win_q[o0 + l];
+  __syncthreads();
+  // the window's Hpl blocks in 8/16-byte chunks; its observations ascend in
+  // q, so runs of one camera are contiguous in HBM. Independent loads: the
+  // unrolled loop keeps several in flight per thread.
+#pragma unroll 6
+  for (int e = tid; e < nob * NCH; e += 256) {
+    const int l = e / NCH, k = e - NCH * l;
+    const CT v = reinterpret_cast<const CT*>(Hpl + 18 * static_cast<size_t>(s_q(l)))[k];
+    reinterpret_cast<CT*>(s_G + l * GS)[k] = v;
+  }
+  __syncthreads();
+  // G = Hpl L^-T, L L^T = Hll + lambda I of the observation's point
+  for (int l = tid; l < nob; l += 256) {
+    double m[6];
+    if (!chol3_inv(Hll + 6 * static_cast<size_t>(g.q_ph[s_q(l)]), lambda, m)) {
+      atomicOr(err, 1);
+      for (int k = 0; k < 6; ++k) m[k] = 0.0;
+    }
+    HT* G = s_G + l * GS;
+    for (int r = 0; r < 6; ++r) {
+      const double h0 = G[3 * r], h1 = G[3 * r + 1], h2 = G[3 * r + 2];
+      G[3 * r] = static_cast<HT>(h0 * m[0]);
+      G[3 * r + 1] = static_cast<HT>(h0 * m[1] + h1 * m[2]);
+      G[3 * r + 2] = static_cast<HT>(h0 * m[3] + h1 * m[4] + h2 * m[5]);
     }
   }
   __syncthreads();
-  const int nitems = s_item[ncol];
-  for (int it = grp; it < nitems; it += 16) {
-    int c = 0;  // last column with s_item[c] <= it
-    {
-      int lo = 0, hi = ncol;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_item[mid] <= it) lo = mid;
-        else hi = mid;
-      }
-      c = lo;
-    }
-    const int pbeg = rp.blk_pair_ptr[k0 + c] + (it - s_item[c]) * CH;
-    const int pend = min(pbeg + CH, rp.blk_pair_ptr[k0 + c + 1]);
-    const bool diag_col = diag_tile && c == 0;
+  const int wv = tid >> 6, lane = tid & 63;
+  const int ge = wp.win_grp0[w + 1];
+  constexpr unsigned NONE = 0xFFFFFFFFu;
+  for (int grp = wp.win_grp0[w] + wv; grp < ge; grp += 4) {
+    const int len = wp.grp_len[grp];
+    const unsigned* rows = wp.pair_rows + 64 * static_cast<size_t>(wp.grp_row0[grp]) + lane;
+    const int slot = wp.grp_slot[64 * static_cast<size_t>(grp) + lane];
     HT acc[36];
-    for (int q = 0; q < 36; ++q) acc[q] = HT(0);
-    for (int p = pbeg + gl; p < pend; p += 16) {
-      const int qa = rp.pair_qa[p], qb = rp.pair_qb[p];
-      const HT* w = W + 18 * static_cast<size_t>(qa);
-      const HT* h = Hpl + 18 * static_cast<size_t>(qb);
-      HT Wa[18], Hb[18];
-      for (int k = 0; k < 18; ++k) { Wa[k] = w[k]; Hb[k] = h[k]; }
-      // a second edge of the point to camera i (qb != qa on the diagonal
-      // block) adds the product and its transpose: g2o merges such edges into
-      // one Hpl block
-      const bool sym = diag_col && qa != qb;
-      for (int r = 0; r < 6; ++r)
-        for (int cc = 0; cc < 6; ++cc) {
-          HT x = Wa[3 * r] * Hb[3 * cc] + Wa[3 * r + 1] * Hb[3 * cc + 1] + Wa[3 * r + 2] * Hb[3 * cc + 2];
-          if (sym) x += Wa[3 * cc] * Hb[3 * r] + Wa[3 * cc + 1] * Hb[3 * r + 1] + Wa[3 * cc + 2] * Hb[3 * r + 2];
-          acc[6 * r + cc] += x;
-        }
+#pragma unroll
+    for (int k = 0; k < 36; ++k) acc[k] = HT(0);
+    // pair entries four rows at a time, the next four in flight while the
+    // current ones are multiplied
+    unsigned cur[4], nxt[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cur[j] = j < len ? rows[64 * static_cast<size_t>(j)] : NONE;
+    for (int r0 = 0; r0 < len; r0 += 4) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) nxt[j] = r0 + 4 + j < len ? rows[64 * static_cast<size_t>(r0 + 4 + j)] : NONE;
+      if (cur[0] == NONE) break;  // this lane's list has ended
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (cur[j] != NONE) pair_mac<HT>(s_G, cur[j], acc);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
     }
-    HT* dst = s_acc + 36 * c;
-    for (int q = 0; q < 36; ++q) {
-      const HT sum = row16_sum(acc[q]);
-      if ((q & 15) == gl) atomicAdd(&dst[q], -sum);
-    }
-  }
-  // workgroup reduction of the coefficient (diagonal tile only)
-  if (diag_tile) {
-    const int lane = tid & 63, wv = tid >> 6;
-    for (int k = 0; k < 6; ++k) {
-      const double v = wave_sum(coef[k]);
-      if (lane == 0) s_red[8 * wv + k] = v;
-    }
-  }
-  __syncthreads();
-  // diagonal block (column 0 of the diagonal tile): Hpp + lambda I minus the
-  // accumulated products, symmetrised (the float products W Hb^T of one block
-  // are symmetric only up to rounding)
-  double hv = 0.0;
-  if (diag_tile && tid < 36) {
-    const int r = tid / 6, c = tid % 6;
-    hv = Hpp[36 * static_cast<size_t>(row) + tid] +
-         0.5 * (static_cast<double>(s_acc[6 * r + c]) + static_cast<double>(s_acc[6 * c + r]));
-    if (!diag_only && r == c) hv += lambda;
-  }
-  __syncthreads();
-  if (diag_tile && tid < 36) s_acc[tid] = static_cast<HT>(hv);
-  if (diag_tile && !diag_only && tid >= 64 && tid < 70) {
-    const int r = tid - 64;
-    const double v = s_red[r] + s_red[8 + r] + s_red[16 + r] + s_red[24 + r];
-    bschur[6 * static_cast<size_t>(row) + r] = b[6 * static_cast<size_t>(row) + r] - v;
-  }
-  __syncthreads();
-  if (diag_only) {
-    for (int i = tid; i < 36; i += 256) Sdiag[36 * static_cast<size_t>(row) + i] = s_acc[i];
-    return;
-  }
-  for (int i = tid; i < ncol * 36; i += 256) {
-    const int lc = i / 36, e = i % 36;
-    const int k = k0 + lc;
-    const HT v = s_acc[i];
-    F_val[36 * static_cast<size_t>(g.up_pos[k]) + e] = v;
-    const int pT = g.up_posT[k];
-    if (pT >= 0) F_val[36 * static_cast<size_t>(pT) + (e % 6) * 6 + e / 6] = v;
+    if (slot >= 0) store36(part + 36 * static_cast<size_t>(slot), acc);
   }
 }
 
 template <typename HT>
-hipError_t launch_schur_rows(const DevGraph& g, const DevRowPlan& rp, const HT* Hpl, const HT* W, const double* b,
-                             const double* Hpp, double lambda, HT* F_val, double* bschur, HT* Sdiag, int diag_only,
-                             hipStream_t st) {
-  if (rp.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_schur_rows<HT, SCHUR_MAXC, SCHUR_CHUNK>), dim3(rp.ntiles), dim3(256), 0, st, g, rp, Hpl, W, b,
-                     Hpp, lambda, F_val, bschur, Sdiag, diag_only);
+__global__ __launch_bounds__(256) void k_schur_offdiag(DevGraph g, DevWinPlan wp, const HT* __restrict__ part,
+                                                      HT* __restrict__ F_val) {
+  const long long t = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  const int u = static_cast<int>(t / 36), e = static_cast<int>(t % 36);
+  if (u >= wp.nod) return;
+  // four independent running sums keep four loads in flight; their fixed
+  // combination order keeps the result deterministic
+  double s4[4] = {0.0, 0.0, 0.0, 0.0};
+  const int s0 = wp.od_ptr[u], s1 = wp.od_ptr[u + 1];
+  int sl = s0;
+  for (; sl + 4 <= s1; sl += 4)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s4[j] += static_cast<double>(part[36 * static_cast<size_t>(sl + j) + e]);
+  for (int j = 0; sl < s1; ++sl, ++j) s4[j] += static_cast<double>(part[36 * static_cast<size_t>(sl) + e]);
+  const HT v = static_cast<HT>(-((s4[0] + s4[1]) + (s4[2] + s4[3])));
+  const int k = wp.od_blk[u];
+  F_val[36 * static_cast<size_t>(g.up_pos[k]) + e] = v;
+  F_val[36 * static_cast<size_t>(g.up_posT[k]) + (e % 6) * 6 + e / 6] = v;
+}
+
+template <typename HT>
+hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* Hpl, const double* Hll, double lambda,
+                            HT* part, int* err, hipStream_t st) {
+  if (wp.nwin == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_schur_win<HT>, dim3(wp.nwin), dim3(256), 0, st, g, wp, Hpl, Hll, lambda, part, err);
   return hipGetLastError();
 }
-template hipError_t launch_schur_rows<float>(const DevGraph&, const DevRowPlan&, const float*, const float*,
-                                             const double*, const double*, double, float*, double*, float*, int,
-                                             hipStream_t);
-template hipError_t launch_schur_rows<double>(const DevGraph&, const DevRowPlan&, const double*, const double*,
-                                              const double*, const double*, double, double*, double*, double*, int,
-                                              hipStream_t);
+template <typename HT>
+hipError_t launch_schur_offdiag(const DevGraph& g, const DevWinPlan& wp, const HT* part, HT* F_val, hipStream_t st) {
+  if (wp.nod == 0) return hipSuccess;
+  const long long n = 36LL * wp.nod;
+  hipLaunchKernelGGL(k_schur_offdiag<HT>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, g, wp,
+                     part, F_val);
+  return hipGetLastError();
+}
+template hipError_t launch_schur_win<float>(const DevGraph&, const DevWinPlan&, const float*, const double*, double,
+                                            float*, int*, hipStream_t);
+template hipError_t launch_schur_win<double>(const DevGraph&, const DevWinPlan&, const double*, const double*,
+                                             double, double*, int*, hipStream_t);
+template hipError_t launch_schur_offdiag<float>(const DevGraph&, const DevWinPlan&, const float*, float*, hipStream_t);
+template hipError_t launch_schur_offdiag<double>(const DevGraph&, const DevWinPlan&, const double*, double*,
+                                                 hipStream_t);
 
 // ========================================================== implicit Schur
 template <typename HT>
@@ -715,8 +698,8 @@ hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, con
                                   const double* p, double* q, double* dq_part, PcgScalars* sc, int it,
                                   hipStream_t st) {
   (void)it;
-  hipLaunchKernelGGL(k_implicit_reduce<HT>, dim3(nblocks(g.Cf, 64)), dim3(64), 0, st, g, cam_part, Hpp, lambda, p,
-                     q, dq_part, sc);
+  hipLaunchKernelGGL(k_implicit_reduce<HT>, dim3(implicit_reduce_blocks(g.Cf)), dim3(64), 0, st, g, cam_part, Hpp,
+                     lambda, p, q, dq_part, sc);
   return hipGetLastError();
 }
 template hipError_t launch_implicit_reduce<float>(const DevGraph&, const double*, const double*, double,
@@ -931,15 +914,13 @@ __device__ __forceinline__ bool inv_spd(double* m) {
 
 template <typename HT, int BD>
 __global__ void k_jacobi(int nb, const HT* __restrict__ F_val, const int* __restrict__ F_diag, HT* __restrict__ Jinv,
-                         double lambda, int* __restrict__ err) {
+                         int* __restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   double m[BD * BD];
   const HT* src = F_val + static_cast<size_t>(F_diag[i]) * BD * BD;
 #pragma unroll
   for (int k = 0; k < BD * BD; ++k) m[k] = static_cast<double>(src[k]);
-#pragma unroll
-  for (int k = 0; k < BD; ++k) m[k * (BD + 1)] += lambda;
   if (!inv_spd<BD>(m)) {
     atomicOr(err, 2);
 #pragma unroll
@@ -951,71 +932,74 @@ __global__ void k_jacobi(int nb, const HT* __restrict__ F_val, const int* __rest
 
 template <typename HT, int BD>
 hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, int* err, hipStream_t st) {
-  hipLaunchKernelGGL((k_jacobi<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, F_val, F_diag, Jinv, 0.0, err);
+  hipLaunchKernelGGL((k_jacobi<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, F_val, F_diag, Jinv, err);
   return hipGetLastError();
 }
-
-template <typename HT, int BD>
-hipError_t launch_jacobi_damped(int nb, const HT* F_val, const int* F_diag, double lambda, HT* Jinv, int* err,
-                                hipStream_t st) {
-  hipLaunchKernelGGL((k_jacobi<HT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, F_val, F_diag, Jinv, lambda,
-                     err);
-  return hipGetLastError();
-}
-template hipError_t launch_jacobi_damped<float, 6>(int, const float*, const int*, double, float*, int*, hipStream_t);
-template hipError_t launch_jacobi_damped<double, 6>(int, const double*, const int*, double, double*, int*,
-                                                    hipStream_t);
 
 // ------------------------------------------ FP64 diagonal Schur blocks
-// S_ii = Hpp_ii + lambda I - sum_p V_ip Dinv_p V_ip^T, V = omega Jc^T Jp (the
-// Hpl block), subtracts terms of the size of Hpp (~1e9 here) to leave a block
-// whose smallest eigenvalue can be ~1e2: from FP32 blocks it loses positive
-// definiteness. The FP32 path therefore forms these blocks in FP64: Hpp_ii
-// and Hll_p are FP64 sums already (k_linearize), V is re-linearised in FP64
-// from the state. One workgroup per free camera walks its observations in
-// camera-major order (two observations of one point by the camera are
-// adjacent and enter as one summed V), sums the 21 upper entries in
-// registers and writes the block into F_val (if not null) and its inverse
-// into Jinv. No atomics, fixed order.
-__device__ __forceinline__ void obs_jacobians64(const DevGraph& g, const double* __restrict__ camRt,
-                                                const double* __restrict__ pts, int s, double* Jp, double* Jc,
-                                                double* om) {
-  const int pos = g.slot_pos[s];
-  const int praw = g.pos_pt[pos];
-  const int cam = g.slot_cam[s];
-  const double* Rt = camRt + 12 * cam;
-  const double* in = g.intr + 4 * cam;
-  const double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
-  const double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
-  const double w = g.slot_w[s];
-  double Xc[3], e[2], rho[3];
+// S_ii = Hpp_ii + lambda I - sum_p V_ip Dinv_p V_ip^T with V the Hpl block
+// subtracts terms of the size of Hpp (~1e9 on long trajectories) to leave a
+// block whose smallest eigenvalue can be ~1e2, so it is formed in FP64 for
+// every precision: Hpp_ii and Hll_p are FP64 sums (k_linearize), and V is
+// re-linearised in FP64 from the state, V = omega Jc^T Jp
+// (edge_project_xyz.cpp:57-93 via base_fixed_sized_edge.hpp:49-140), or read from
+// the given blocks (the exact Hessian). One workgroup per free camera walks
+// its observations in camera-major order (two observations of one point by
+// the camera are adjacent and enter as one summed V), sums the 21 upper
+// entries and the 6 right-hand-side terms V Dinv b_p in registers and
+// reduces them over the workgroup: no atomics, fixed order. Writes the block
+// into F_val (if not null), its inverse into Jinv and b_i - sum V Dinv b_p
+// into bschur (if not null).
+// V += omega Jc^T Jp of the observation (slot, raw point) by the camera
+// whose [R | t] and intrinsics are Rt, in.
+__device__ __forceinline__ void obs_V64(const DevGraph& g, const double* __restrict__ Rt,
+                                       const double* __restrict__ in, const double* __restrict__ pts, int slot,
+                                       int praw, double* V) {
+  const double X[3] = {pts[3 * static_cast<size_t>(praw)], pts[3 * static_cast<size_t>(praw) + 1],
+                       pts[3 * static_cast<size_t>(praw) + 2]};
+  const double z[2] = {g.slot_meas[2 * static_cast<size_t>(slot)], g.slot_meas[2 * static_cast<size_t>(slot) + 1]};
+  const double w = g.slot_w[slot];
+  double Xc[3], e[2], rho[3], Jp[6], Jc[12];
   project_error(Rt, in, X, z, Xc, e);
   huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
-  *om = rho[1] * w;
+  const double om = rho[1] * w;
   project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+  for (int r = 0; r < 6; ++r)
+    for (int cc = 0; cc < 3; ++cc) V[3 * r + cc] += om * (Jc[r] * Jp[cc] + Jc[6 + r] * Jp[3 + cc]);
 }
 
 template <typename HT>
-__global__ __launch_bounds__(256) void k_schur_diag64(DevGraph g, const int* __restrict__ q_slot,
-                                                      const double* __restrict__ camRt, const double* __restrict__ pts,
-                                                      const double* __restrict__ Hll, const double* __restrict__ Hpp,
-                                                      double lambda, HT* __restrict__ F_val, HT* __restrict__ Jinv,
-                                                      int* __restrict__ err) {
-  __shared__ double s_red[4][21];
+__global__ __launch_bounds__(256) void k_schur_diag(DevGraph g, const int* __restrict__ q_slot,
+                                                    const double* __restrict__ camRt, const double* __restrict__ pts,
+                                                    const HT* __restrict__ Hpl, const double* __restrict__ Hll,
+                                                    const double* __restrict__ Hpp, const double* __restrict__ b,
+                                                    double lambda, HT* __restrict__ F_val, HT* __restrict__ Jinv,
+                                                    double* __restrict__ bschur, int* __restrict__ err) {
+  __shared__ double s_red[4][27];
   const int c = blockIdx.x, tid = threadIdx.x;
   const int qb = g.cam_qptr[c], qe = g.cam_qptr[c + 1];
-  double acc[21];
-  for (int k = 0; k < 21; ++k) acc[k] = 0;
+  const int craw = g.cam_raw[c];
+  const double* Rt = camRt + 12 * static_cast<size_t>(craw);
+  const double* in = g.intr + 4 * static_cast<size_t>(craw);
+  const double* bl = b ? b + 6 * static_cast<size_t>(g.Cf) : nullptr;
+  double acc[27];
+  for (int k = 0; k < 27; ++k) acc[k] = 0;
   for (int q = qb + tid; q < qe; q += 256) {
+    // indices of this observation and of its predecessor, loaded together
     const int ph = g.q_ph[q];
-    if (q > qb && g.q_ph[q - 1] == ph) continue;  // summed by the run's first observation
+    const int ph_prev = q > qb ? g.q_ph[q - 1] : -1;
+    const int slot = q_slot[q], praw = g.q_pt[q];
+    if (ph_prev == ph) continue;  // summed by the run's first observation
     double V[18];
     for (int k = 0; k < 18; ++k) V[k] = 0;
-    for (int q2 = q; q2 < qe && g.q_ph[q2] == ph; ++q2) {
-      double Jp[6], Jc[12], om;
-      obs_jacobians64(g, camRt, pts, q_slot[q2], Jp, Jc, &om);
-      for (int r = 0; r < 6; ++r)
-        for (int cc = 0; cc < 3; ++cc) V[3 * r + cc] += om * (Jc[r] * Jp[cc] + Jc[6 + r] * Jp[3 + cc]);
+    if (Hpl) {
+      for (int q2 = q; q2 < qe && g.q_ph[q2] == ph; ++q2) {
+        const HT* h = Hpl + 18 * static_cast<size_t>(q2);
+        for (int k = 0; k < 18; ++k) V[k] += static_cast<double>(h[k]);
+      }
+    } else {
+      obs_V64(g, Rt, in, pts, slot, praw, V);
+      for (int q2 = q + 1; q2 < qe && g.q_ph[q2] == ph; ++q2) obs_V64(g, Rt, in, pts, q_slot[q2], g.q_pt[q2], V);
     }
     double D[9];
     if (!inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
@@ -1030,13 +1014,22 @@ __global__ __launch_bounds__(256) void k_schur_diag64(DevGraph g, const int* __r
     for (int r = 0; r < 6; ++r)
       for (int cc = r; cc < 6; ++cc)
         acc[k++] += VD[3 * r] * V[3 * cc] + VD[3 * r + 1] * V[3 * cc + 1] + VD[3 * r + 2] * V[3 * cc + 2];
+    if (bl) {
+      const double* y = bl + 3 * static_cast<size_t>(ph);
+      for (int r = 0; r < 6; ++r) acc[21 + r] += VD[3 * r] * y[0] + VD[3 * r + 1] * y[1] + VD[3 * r + 2] * y[2];
+    }
   }
   const int lane = tid & 63, wv = tid >> 6;
-  for (int k = 0; k < 21; ++k) {
+  for (int k = 0; k < 27; ++k) {
     const double v = wave_sum(acc[k]);
     if (lane == 0) s_red[wv][k] = v;
   }
   __syncthreads();
+  if (bl && bschur && tid >= 64 && tid < 70) {
+    const int r = tid - 64;
+    const double v = ((s_red[0][21 + r] + s_red[1][21 + r]) + s_red[2][21 + r]) + s_red[3][21 + r];
+    bschur[6 * static_cast<size_t>(c) + r] = b[6 * static_cast<size_t>(c) + r] - v;
+  }
   if (tid != 0) return;
   double m[36];
   for (int k = 0; k < 21; ++k) {
@@ -1059,20 +1052,20 @@ __global__ __launch_bounds__(256) void k_schur_diag64(DevGraph g, const int* __r
 }
 
 template <typename HT>
-hipError_t launch_schur_diag64(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
-                               const double* Hll, const double* Hpp, double lambda, HT* F_val, HT* Jinv, int* err,
-                               hipStream_t st) {
+hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
+                             const HT* Hpl, const double* Hll, const double* Hpp, const double* b, double lambda,
+                             HT* F_val, HT* Jinv, double* bschur, int* err, hipStream_t st) {
   if (g.Cf == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_schur_diag64<HT>, dim3(g.Cf), dim3(256), 0, st, g, q_slot, camRt, pts, Hll, Hpp, lambda, F_val,
-                     Jinv, err);
+  hipLaunchKernelGGL(k_schur_diag<HT>, dim3(g.Cf), dim3(256), 0, st, g, q_slot, camRt, pts, Hpl, Hll, Hpp, b, lambda,
+                     F_val, Jinv, bschur, err);
   return hipGetLastError();
 }
-template hipError_t launch_schur_diag64<float>(const DevGraph&, const int*, const double*, const double*,
-                                               const double*, const double*, double, float*, float*, int*,
-                                               hipStream_t);
-template hipError_t launch_schur_diag64<double>(const DevGraph&, const int*, const double*, const double*,
-                                                const double*, const double*, double, double*, double*, int*,
-                                                hipStream_t);
+template hipError_t launch_schur_diag<float>(const DevGraph&, const int*, const double*, const double*, const float*,
+                                             const double*, const double*, const double*, double, float*, float*,
+                                             double*, int*, hipStream_t);
+template hipError_t launch_schur_diag<double>(const DevGraph&, const int*, const double*, const double*,
+                                              const double*, const double*, const double*, const double*, double,
+                                              double*, double*, double*, int*, hipStream_t);
 
 // ----------------------------------------------------- block-Jacobi PCG
 // LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) as two launches per
@@ -1087,7 +1080,14 @@ template hipError_t launch_schur_diag64<double>(const DevGraph&, const int*, con
 __device__ double wave_sum_array(const double* __restrict__ part, int n) {
   const int lane = threadIdx.x & 63;
   double s = 0, c = 0;
-  for (int i = lane; i < n; i += 64) neumaier_add(s, c, part[i]);
+  // eight loads in flight per lane, summed in index order
+  for (int i0 = lane; i0 < n; i0 += 64 * 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = i0 + 64 * j < n ? part[i0 + 64 * j] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) neumaier_add(s, c, v[j]);
+  }
   for (int off = 32; off > 0; off >>= 1) {
     const double os = __shfl_xor(s, off, 64), oc = __shfl_xor(c, off, 64);
     const double t = s + os;
@@ -1108,39 +1108,55 @@ __device__ __forceinline__ double pcg_dir_val(double s, double d_old, double bet
 // process keeps its FP64 orthogonality, and the bandwidth-heavy data (the
 // matrix) moves at half width.
 
+// Row-parallel vector kernels (init, update) give each block row 8 lanes
+// (lane k < BD owns component k), 8 rows per 64-lane workgroup; the
+// preconditioner product gathers the row's r components by lane shuffles.
+constexpr int PCG_RPW = 8;  // block rows per 64-lane workgroup
+
+// Stores of the solve's progress into host-mapped memory (system scope), so
+// the host can follow the iteration without synchronising the stream.
+__device__ __forceinline__ void progress_store(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // x = 0, r = b, s = M^-1 r, part[block] = r . s   (linear_solver_pcg.hpp:120-130)
 template <typename MT, int BD>
 __global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restrict__ b, const MT* __restrict__ J,
                                                  double* __restrict__ x, double* __restrict__ r,
                                                  double* __restrict__ s, double* __restrict__ part) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  double dot = 0;
-  if (i < nb) {
-    double rv[BD];
-    for (int k = 0; k < BD; ++k) {
-      rv[k] = b[BD * static_cast<size_t>(i) + k];
-      r[BD * static_cast<size_t>(i) + k] = rv[k];
-      x[BD * static_cast<size_t>(i) + k] = 0.0;
-    }
-    const MT* Ji = J + static_cast<size_t>(i) * BD * BD;
-    for (int k = 0; k < BD; ++k) {
-      double v = 0;
-      for (int c = 0; c < BD; ++c) v += static_cast<double>(Ji[BD * k + c]) * rv[c];
-      s[BD * static_cast<size_t>(i) + k] = v;
-      dot += rv[k] * v;
-    }
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * PCG_RPW + (lane >> 3), k = lane & 7;
+  const bool on = i < nb && k < BD;
+  const size_t o = BD * static_cast<size_t>(i) + k;
+  double rv = 0.0;
+  if (on) {
+    rv = b[o];
+    r[o] = rv;
+    x[o] = 0.0;
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int c = 0; c < BD; ++c) {
+    const double rc = __shfl(rv, (lane & ~7) + c, 64);
+    if (on) v += static_cast<double>(J[static_cast<size_t>(i) * BD * BD + BD * k + c]) * rc;
+  }
+  double dot = 0.0;
+  if (on) {
+    s[o] = v;
+    dot = rv * v;
   }
   dot = wave_sum(dot);
-  if (threadIdx.x == 0) part[blockIdx.x] = dot;
+  if (lane == 0) part[blockIdx.x] = dot;
 }
 
 // Head of iteration `it`: dn = r . s from the previous update's partials; the
 // stop test `dn <= d0` of linear_solver_pcg.hpp:141 (d0 fixed at it == 0 as
 // tol * dn0 floored by the residual memory, or d0_abs when >= 0); then
 // beta = dn / dn_old. Returns false (and records the iteration count) when
-// the solve is over. Every workgroup evaluates this identically.
-__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, double tol, double resid_mem,
-                         double d0_abs, int it, int last, double* beta) {
+// the solve is over. Every workgroup evaluates this identically; workgroup 0
+// publishes the progress (iteration reached, done) to host-mapped memory.
+__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, int* prog, double tol,
+                         double resid_mem, double d0_abs, int it, int last, double* beta) {
   if (sc->done) return false;
   const double dn = wave_sum_array(part_rs, n_rs);
   double d0;
@@ -1161,59 +1177,104 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
     if (!(dn >= 0.0)) sc->err = 1;  // indefinite preconditioner (reported, not acted on)
   }
   if (dn <= d0 || last) {
-    if (first) { sc->done = 1; sc->iters = it; }
+    if (first) {
+      sc->done = 1;
+      sc->iters = it;
+      if (prog) progress_store(prog + 1, 1);
+    }
     return false;
   }
+  if (first && prog) progress_store(prog, it);
   *beta = it == 0 ? 0.0 : dn / sc->dn[(it + 1) & 1];
   return true;
 }
 
 // Explicit operator: d = s + beta d_old for the owned rows and q = A d over
-// the full block rows (both triangles). One wavefront per block row; lanes
-// split into 64 / BD groups striding over the row's blocks, lane k of a group
-// owns output row k. part_dq[block] = sum over its rows of d . q.
+// the full block rows (both triangles). One workgroup of PCG_SPMV_WAVES
+// wavefronts per block row (the whole grid is resident at once on 256 CUs):
+// its lanes form groups of BD lanes striding over the row's blocks (lane k of
+// a group owns output row k). Each group's first PCG_SPMV_HOIST blocks - the
+// matrix entries and the gathered s, d_old, none of which depend on beta -
+// are loaded before the iteration head, so the dependency chains of the
+// matrix and of the scalars are in flight together.
+// part_dq[row] = d . q over the row.
+constexpr int PCG_SPMV_WAVES = 2;
+constexpr int PCG_SPMV_HOIST = 3;
 template <typename MT, int BD>
-__global__ __launch_bounds__(256) void k_pcg_dir_spmv(int nb, const int* __restrict__ F_ptr,
-                                                      const int* __restrict__ F_col, const MT* __restrict__ F_val,
-                                                      const double* __restrict__ s, const double* __restrict__ d_old,
-                                                      double* __restrict__ d, double* __restrict__ q,
-                                                      const double* __restrict__ part_rs, int n_rs,
-                                                      double* __restrict__ part_dq, PcgScalars* sc, double tol,
-                                                      double resid_mem, double d0_abs, int it, int last) {
-  double beta;
-  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta)) return;
+__global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
+    int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
+    const double* __restrict__ s, const double* __restrict__ d_old, double* __restrict__ d, double* __restrict__ q,
+    const double* __restrict__ part_rs, int n_rs, double* __restrict__ part_dq, PcgScalars* sc, int* prog, double tol,
+    double resid_mem, double d0_abs, int it, int last) {
   const bool firstit = it == 0;
-  constexpr int G = 64 / BD;
-  __shared__ double s_part[4][64];
-  __shared__ double s_dot[4];
+  constexpr int GPW = 64 / BD;  // groups per wavefront
+  constexpr int NG = PCG_SPMV_WAVES * GPW;
+  constexpr int H = PCG_SPMV_HOIST;
+  __shared__ double s_part[PCG_SPMV_WAVES][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int row = blockIdx.x * 4 + wv;
-  const int grp = lane / BD, k = lane % BD;
+  const int row = blockIdx.x;
+  const int gw = lane / BD, k = lane % BD;
+  const int grp = wv * GPW + gw;
+  const int p0 = F_ptr[row], p1 = F_ptr[row + 1];
+  MT v[H][BD];
+  double sv[H][BD], dv[H][BD];
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const int p = p0 + grp + h * NG;
+    const bool on = gw < GPW && p < p1;
+    const size_t j = on ? BD * static_cast<size_t>(F_col[p]) : 0;
+    const MT* V = F_val + static_cast<size_t>(on ? p : 0) * BD * BD + BD * k;
+#pragma unroll
+    for (int c = 0; c < BD; ++c) {
+      v[h][c] = on ? V[c] : MT(0);
+      sv[h][c] = s[j + c];
+      dv[h][c] = firstit ? 0.0 : d_old[j + c];
+    }
+  }
+  double beta;
+  if (!pcg_head(part_rs, n_rs, sc, prog, tol, resid_mem, d0_abs, it, last, &beta)) return;
   double acc = 0;
-  if (row < nb && grp < G) {
-    for (int p = F_ptr[row] + grp; p < F_ptr[row + 1]; p += G) {
-      const MT* V = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
-      const size_t j = BD * static_cast<size_t>(F_col[p]);
-      for (int c = 0; c < BD; ++c)
-        acc += static_cast<double>(V[c]) * pcg_dir_val(s[j + c], firstit ? 0.0 : d_old[j + c], beta, firstit);
+#pragma unroll
+  for (int h = 0; h < H; ++h)
+#pragma unroll
+    for (int c = 0; c < BD; ++c) acc += static_cast<double>(v[h][c]) * pcg_dir_val(sv[h][c], dv[h][c], beta, firstit);
+  if (gw < GPW) {
+    // blocks past the hoisted ones (rows longer than H * NG blocks), two at a
+    // time
+    for (int p = p0 + grp + H * NG; p < p1; p += 2 * NG) {
+      const int pb = p + NG;
+      const bool hb = pb < p1;
+      const size_t ja = BD * static_cast<size_t>(F_col[p]);
+      const size_t jb = hb ? BD * static_cast<size_t>(F_col[pb]) : ja;
+      const MT* Va = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
+      const MT* Vb = F_val + static_cast<size_t>(hb ? pb : p) * BD * BD + BD * k;
+      double ta = 0.0, tb = 0.0;
+#pragma unroll
+      for (int c = 0; c < BD; ++c) {
+        ta += static_cast<double>(Va[c]) * pcg_dir_val(s[ja + c], firstit ? 0.0 : d_old[ja + c], beta, firstit);
+        tb += static_cast<double>(Vb[c]) * pcg_dir_val(s[jb + c], firstit ? 0.0 : d_old[jb + c], beta, firstit);
+      }
+      acc += ta;
+      if (hb) acc += tb;
     }
   }
   s_part[wv][lane] = acc;
   __syncthreads();
-  double dot = 0;
-  if (row < nb && lane < BD) {
-    double v = 0;
-    for (int g2 = 0; g2 < G; ++g2) v += s_part[wv][g2 * BD + lane];
-    const size_t o = BD * static_cast<size_t>(row) + lane;
-    const double dv = pcg_dir_val(s[o], firstit ? 0.0 : d_old[o], beta, firstit);
-    q[o] = v;
-    d[o] = dv;
-    dot = v * dv;
+  if (threadIdx.x < 64) {
+    double dot = 0.0;
+    if (lane < BD) {
+      double vs = 0;
+      for (int w2 = 0; w2 < PCG_SPMV_WAVES; ++w2)
+        for (int g2 = 0; g2 < GPW; ++g2) vs += s_part[w2][g2 * BD + lane];
+      const size_t o = BD * static_cast<size_t>(row) + lane;
+      const double dd = pcg_dir_val(s[o], firstit ? 0.0 : d_old[o], beta, firstit);
+      q[o] = vs;
+      d[o] = dd;
+      dot = vs * dd;
+    }
+    dot = wave_sum(dot);
+    if (lane == 0) part_dq[row] = dot;
   }
-  dot = wave_sum(dot);
-  if (lane == 0) s_dot[wv] = dot;
-  __syncthreads();
-  if (threadIdx.x == 0) part_dq[blockIdx.x] = ((s_dot[0] + s_dot[1]) + s_dot[2]) + s_dot[3];
 }
 
 // Direction only (matrix-free operator): d = s + beta d_old, thread per row.
@@ -1221,9 +1282,10 @@ template <int BD>
 __global__ __launch_bounds__(64) void k_pcg_dir(int nb, const double* __restrict__ s,
                                                 const double* __restrict__ d_old, double* __restrict__ d,
                                                 const double* __restrict__ part_rs, int n_rs, PcgScalars* sc,
-                                                double tol, double resid_mem, double d0_abs, int it, int last) {
+                                                int* prog, double tol, double resid_mem, double d0_abs, int it,
+                                                int last) {
   double beta;
-  if (!pcg_head(part_rs, n_rs, sc, tol, resid_mem, d0_abs, it, last, &beta)) return;
+  if (!pcg_head(part_rs, n_rs, sc, prog, tol, resid_mem, d0_abs, it, last, &beta)) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= nb) return;
   for (int k = 0; k < BD; ++k) {
@@ -1239,6 +1301,17 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
                                                    const double* __restrict__ q, double* __restrict__ s,
                                                    const double* __restrict__ part_dq, int n_dq,
                                                    double* __restrict__ part_rs, PcgScalars* sc, int it) {
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * PCG_RPW + (lane >> 3), k = lane & 7;
+  const bool on = i < nb && k < BD;
+  const size_t o = BD * static_cast<size_t>(i) + k;
+  // operands first (independent of alpha), then the reduction of d . q
+  double dv = 0.0, qv = 0.0, rv0 = 0.0, xv = 0.0, jr[BD];
+  if (on) {
+    dv = d[o]; qv = q[o]; rv0 = r[o]; xv = x[o];
+  }
+#pragma unroll
+  for (int c = 0; c < BD; ++c) jr[c] = on ? static_cast<double>(J[static_cast<size_t>(i) * BD * BD + BD * k + c]) : 0.0;
   if (sc->done) return;
   const double dq = wave_sum_array(part_dq, n_dq);
   const double a = sc->dn[it & 1] / dq;
@@ -1246,35 +1319,32 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
   // sc->err for callers that need it; the iteration itself follows
   // linear_solver_pcg.hpp, which does not check)
   if (blockIdx.x == 0 && threadIdx.x == 0 && !(dq > 0.0)) sc->err = 1;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  double dot = 0;
-  if (i < nb) {
-    double rv[BD];
-    for (int k = 0; k < BD; ++k) {
-      const size_t o = BD * static_cast<size_t>(i) + k;
-      x[o] = fma(a, d[o], x[o]);
-      rv[k] = fma(-a, q[o], r[o]);
-      r[o] = rv[k];
-    }
-    const MT* Ji = J + static_cast<size_t>(i) * BD * BD;
-    for (int k = 0; k < BD; ++k) {
-      double v = 0;
-      for (int c = 0; c < BD; ++c) v += static_cast<double>(Ji[BD * k + c]) * rv[c];
-      s[BD * static_cast<size_t>(i) + k] = v;
-      dot += rv[k] * v;
-    }
+  double rv = 0.0;
+  if (on) {
+    x[o] = fma(a, dv, xv);
+    rv = fma(-a, qv, rv0);
+    r[o] = rv;
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int c = 0; c < BD; ++c) v += jr[c] * __shfl(rv, (lane & ~7) + c, 64);
+  double dot = 0.0;
+  if (on) {
+    s[o] = v;
+    dot = rv * v;
   }
   dot = wave_sum(dot);
-  if (threadIdx.x == 0) part_rs[blockIdx.x] = dot;
+  if (lane == 0) part_rs[blockIdx.x] = dot;
 }
 
-int pcg_row_blocks(int nb) { return nblocks(nb, 64); }
-int pcg_spmv_blocks(int nb) { return nblocks(nb, 4); }
+int pcg_row_blocks(int nb) { return nblocks(nb, PCG_RPW); }
+int pcg_spmv_blocks(int nb) { return nb; }
+int implicit_reduce_blocks(int Cf) { return nblocks(Cf, 64); }
 
 template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
                            double* part_rs, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs);
+  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs);
   return hipGetLastError();
 }
 
@@ -1282,9 +1352,10 @@ template <typename MT, int BD>
 hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD>), dim3(nblocks(nb, 4)), dim3(256), 0, st, nb, F_ptr, F_col, F_val, s,
-                     d_old, d, q, part_rs, nblocks(nb, 64), part_dq, sc, ctl.tol, ctl.resid_mem, ctl.d0_abs, it,
-                     last);
+  hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, st, nb, F_ptr,
+                     F_col, F_val,
+                     s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog, ctl.tol, ctl.resid_mem,
+                     ctl.d0_abs, it, last);
   return hipGetLastError();
 }
 
@@ -1292,7 +1363,7 @@ template <int BD>
 hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
                           PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
   hipLaunchKernelGGL((k_pcg_dir<BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, s, d_old, d, part_rs,
-                     nblocks(nb, 64), sc, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
+                     pcg_row_blocks(nb), sc, ctl.prog, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
   return hipGetLastError();
 }
 template hipError_t launch_pcg_dir<6>(int, const double*, const double*, double*, const double*, PcgScalars*,
@@ -1302,7 +1373,7 @@ template <typename MT, int BD>
 hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const double* d, const double* q,
                              double* s, const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_update<MT, BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, Jinv, x, r, d, q, s,
+  hipLaunchKernelGGL((k_pcg_update<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(64), 0, st, nb, Jinv, x, r, d, q, s,
                      part_dq, n_dq, part_rs, sc, it);
   return hipGetLastError();
 }

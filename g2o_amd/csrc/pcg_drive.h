@@ -2,6 +2,7 @@
 // pose-graph solvers).
 #pragma once
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include <hip/hip_runtime.h>
@@ -14,14 +15,24 @@ namespace g2ohip {
 // One LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) on the device:
 // init, then `op` (direction + operator) and the update kernel per
 // iteration. The stop test runs on the device and later launches return at
-// once, so the host only polls the scalars: first after `hint` iterations
-// (the previous solve's count), then every 4. If the loop reaches max_iter
-// the head runs once more to record the final residual.
+// once. With `prog` (host-mapped PcgProgress, written by the iteration
+// heads) the host never waits on the stream inside the loop: it launches
+// until it is PCG_LAG iterations ahead of the device's progress, then spins
+// on the progress word, and stops launching once the device reports done -
+// so the queue does not drain while the host looks, and at most PCG_LAG
+// iterations are launched past the end. If the word does not advance for
+// PCG_STALL_MS the host synchronises the stream instead (which always ends
+// the wait) and reports prog_failed. Without `prog` the host polls the
+// device scalars every 4 iterations. If the loop reaches max_iter the head
+// runs once more to record the final residual.
+constexpr int PCG_LAG = 3;
+constexpr double PCG_STALL_MS = 50.0;
+
 template <typename MT, int BD, typename OpFn>
-inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const double* rhs, const MT* J, double* x,
-                            double* r, double* s, double* const dirs[2], double* q, double* part_rs,
-                            double* part_dq, int n_dq, PcgScalars* sc, int max_iter, int hint, OpFn op,
-                            PcgRun* out) {
+inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProgress* prog, int nb,
+                            const double* rhs, const MT* J, double* x, double* r, double* s, double* const dirs[2],
+                            double* q, double* part_rs, double* part_dq, int n_dq, PcgScalars* sc, int max_iter,
+                            OpFn op, PcgRun* out, bool* prog_failed) {
   hipError_t e;
   PcgScalars hs{};
   auto poll = [&]() -> hipError_t {
@@ -30,25 +41,53 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const do
     if (e2 == hipSuccess) std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
     return e2;
   };
+  if (prog) {
+    prog->it = -1;
+    prog->done = 0;
+  }
   if ((e = hipMemsetAsync(sc, 0, sizeof(PcgScalars), stream)) != hipSuccess) return e;
   if ((e = launch_pcg_init<MT, BD>(nb, rhs, J, x, r, s, part_rs, stream)) != hipSuccess) return e;
-  int next = std::max(4, hint);
+  int next = 4;
   bool done = false;
+  bool have_final = false;  // hs already holds the final scalars
   for (int it = 0; it < max_iter; ++it) {
     if ((e = op(it, 0, dirs[(it + 1) & 1], dirs[it & 1])) != hipSuccess) return e;
     if ((e = launch_pcg_update<MT, BD>(nb, J, x, r, dirs[it & 1], q, s, part_dq, n_dq, part_rs, sc, it, stream)) !=
         hipSuccess)
       return e;
-    if (it + 1 == next) {
+    if (prog) {
+      if (prog->done) { done = true; break; }
+      if (it - prog->it < PCG_LAG) continue;
+      const auto t0 = std::chrono::steady_clock::now();
+      int seen = prog->it;
+      auto last_move = t0;
+      while (!prog->done && it - prog->it >= PCG_LAG) {
+        const auto now = std::chrono::steady_clock::now();
+        if (prog->it != seen) {
+          seen = prog->it;
+          last_move = now;
+        } else if (std::chrono::duration<double, std::milli>(now - last_move).count() > PCG_STALL_MS) {
+          // the progress word is not reaching the host: synchronise instead
+          if (prog_failed) *prog_failed = true;
+          prog = nullptr;
+          if ((e = poll()) != hipSuccess) return e;
+          if (hs.done) done = have_final = true;
+          next = it + 1 + 4;
+          break;
+        }
+      }
+      if (done) break;
+      if (prog && prog->done) { done = true; break; }
+    } else if (it + 1 == next) {
       if ((e = poll()) != hipSuccess) return e;
-      if (hs.done) { done = true; break; }
+      if (hs.done) { done = have_final = true; break; }
       next += 4;
     }
   }
   if (!done) {
     if ((e = op(max_iter, 1, dirs[(max_iter + 1) & 1], dirs[max_iter & 1])) != hipSuccess) return e;
-    if ((e = poll()) != hipSuccess) return e;
   }
+  if (!have_final && (e = poll()) != hipSuccess) return e;
   out->iters = hs.iters;
   out->dn_final = hs.dn[hs.iters & 1];
   out->d0 = hs.d0;
@@ -56,6 +95,5 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, int nb, const do
   out->breakdown = hs.err != 0;
   return hipSuccess;
 }
-
 
 }  // namespace g2ohip

@@ -171,7 +171,7 @@ int g2ohip_ctx::pg_solve(int* ok) {
   CK(read_err(&err));
   if (err) { *ok = 0; return G2OHIP_OK; }
   const int max_iter = opt.pcg_max_iter < 0 ? n : opt.pcg_max_iter;
-  const PcgControl ctl{opt.pcg_tolerance, pcg_residual_mem, -1.0};
+  const PcgControl ctl = with_progress(PcgControl{opt.pcg_tolerance, pcg_residual_mem, -1.0});
   double* const dirs[2] = {B.d.p, B.d2.p};
   auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
     return launch_pcg_dir_spmv<double, 6>(nb, d_pg_F_ptr.p, d_pg_F_col.p, d_pg_F_val.p, B.s.p, d_old, d, B.q.p,
@@ -180,10 +180,12 @@ int g2ohip_ctx::pg_solve(int* ok) {
   PcgRun pr;
   {
     ProfScope ps(this, G2OHIP_K_PCG);
-    HK((pcg_drive<double, 6>(stream, h_pinned, nb, d_b.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p, d_dn_part.p,
-                             d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, pcg_hint[0], op, &pr)));
+    bool prog_failed = false;
+    HK((pcg_drive<double, 6>(stream, h_pinned, progress(), nb, d_b.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
+                             d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, &pr,
+                             &prog_failed)));
+    if (prog_failed) prog_ok = false;
   }
-  pcg_hint[0] = pr.iters;
   pcg_residual_mem = 0.5 * pr.dn_final;
   HK(launch_to_double<double>(n, B.x.p, d_x.p, 0.0, stream));
   last_info[0] = pr.iters;

@@ -119,6 +119,7 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   st.nq = st.cam_qptr[st.Cf];
   st.q_slot.resize(st.nq);
   st.q_ph.resize(st.nq);
+  st.q_pt.resize(st.nq);
   st.slot_q.assign(st.ns, -1);
   {
     std::vector<int> fill(st.cam_qptr.begin(), st.cam_qptr.end() - 1);
@@ -128,6 +129,7 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
       const int d = fill[c]++;
       st.q_slot[d] = q;
       st.q_ph[d] = st.slot_ph[q];
+      st.q_pt[d] = st.pos_pt[st.slot_pos[q]];
       st.slot_q[q] = d;
     }
   }
@@ -239,69 +241,113 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   return "";
 }
 
-std::string build_row_plan(BAStructure& st, int max_cols, int chunk, int num_xcds) {
-  RowPlan& rp = st.rows;
-  rp = RowPlan();
-  rp.max_cols = max_cols;
-  rp.chunk = chunk;
+std::string build_win_plan(BAStructure& st, int obs_max, int pch) {
+  WinPlan& wp = st.win;
+  wp = WinPlan();
+  if (obs_max < 1 || obs_max > 65535 || pch < 1) return "invalid window plan parameters";
+  wp.obs_max = obs_max;
+  wp.pch = pch;
   const int Cf = st.Cf;
-  for (int i = 0; i < st.npos; ++i)
-    if (st.pos_h[i] >= 0) rp.pair_instances += static_cast<long long>(st.pos_nfree[i]) * (st.pos_nfree[i] + 1) / 2;
-  if (rp.pair_instances >= (1LL << 31)) return "more than 2^31 Schur pair products";
   for (int r = 0; r < Cf; ++r) {
     const int k0 = st.S_rowptr[r], k1 = st.S_rowptr[r + 1];
     if (k1 <= k0 || st.S_col[k0] != r) return "Hschur row without its diagonal block";
-    rp.max_row_cols = std::max(rp.max_row_cols, k1 - k0);
-    for (int k = k0; k < k1; k += max_cols) {
-      rp.tile_row.push_back(r);
-      rp.tile_k0.push_back(k);
-      rp.tile_k1.push_back(std::min(k1, k + max_cols));
-    }
   }
-  // pair products grouped by upper block (counting sort by block)
-  rp.blk_pair_ptr.assign(st.nS + 1, 0);
-  std::vector<int> pair_blk;
-  pair_blk.reserve(static_cast<size_t>(rp.pair_instances));
-  for (int i = 0; i < st.npos; ++i) {
-    if (st.pos_h[i] < 0) continue;
-    const int a0 = st.pos_ptr[i], nf = st.pos_nfree[i];
-    for (int a = 0; a < nf; ++a) {
-      int k = st.S_rowptr[st.slot_camh[a0 + a]];
-      for (int bb = a; bb < nf; ++bb) {
-        const int cb = st.slot_camh[a0 + bb];
-        while (st.S_col[k] < cb) ++k;  // cameras ascend within the point
-        pair_blk.push_back(k);
-        rp.blk_pair_ptr[k + 1]++;
+  std::vector<int> od_of(st.nS, -1);
+  for (int k = 0; k < st.nS; ++k)
+    if (st.S_row[k] != st.S_col[k]) {
+      od_of[k] = static_cast<int>(wp.od_blk.size());
+      wp.od_blk.push_back(k);
+    }
+  const int nod = static_cast<int>(wp.od_blk.size());
+  std::vector<int> od_cnt(nod, 0);
+  // partial id -> (off-diagonal block, rank within the block); converted to
+  // block-major slots once every block's count is known
+  std::vector<int> part_od, part_rank;
+  auto nobs = [&](int pos) { return st.pos_h[pos] >= 0 ? st.pos_nfree[pos] : 0; };
+  wp.win_obs0.push_back(0);
+  wp.win_grp0.push_back(0);
+  wp.grp_row0.push_back(0);
+  std::vector<int> qs;
+  std::vector<std::pair<int, uint32_t>> lst;  // (off-diagonal block, pair)
+  struct Part { int od, beg, n; };
+  std::vector<Part> parts;
+  int pos = 0;
+  while (pos < st.npos) {
+    int pe = pos, cur = 0;
+    while (pe < st.npos) {
+      const int n = nobs(pe);
+      if (n > obs_max)
+        return "a point has more than " + std::to_string(obs_max) + " observations with a free camera";
+      if (cur > 0 && cur + n > obs_max) break;
+      cur += n;
+      ++pe;
+    }
+    qs.clear();
+    for (int p = pos; p < pe; ++p)
+      for (int k = 0; k < nobs(p); ++k) qs.push_back(st.slot_q[st.pos_ptr[p] + k]);
+    std::sort(qs.begin(), qs.end());
+    wp.win_q.insert(wp.win_q.end(), qs.begin(), qs.end());
+    wp.win_obs0.push_back(static_cast<int>(wp.win_q.size()));
+    auto local = [&](int q) {
+      return static_cast<uint32_t>(std::lower_bound(qs.begin(), qs.end(), q) - qs.begin());
+    };
+    lst.clear();
+    for (int p = pos; p < pe; ++p) {
+      const int s0 = st.pos_ptr[p], nf = nobs(p);
+      for (int a = 0; a < nf; ++a) {
+        const int ca = st.slot_camh[s0 + a];
+        const uint32_t la = local(st.slot_q[s0 + a]);
+        const int* cb0 = st.S_col.data() + st.S_rowptr[ca];
+        const int* cb1 = st.S_col.data() + st.S_rowptr[ca + 1];
+        for (int bb = a + 1; bb < nf; ++bb) {
+          const int cb = st.slot_camh[s0 + bb];
+          if (cb == ca) continue;  // a second edge to the same camera: diagonal block
+          const int k = static_cast<int>(std::lower_bound(cb0, cb1, cb) - st.S_col.data());
+          if (k >= st.S_rowptr[ca + 1] || st.S_col[k] != cb) return "Schur pattern misses a co-observed pair";
+          lst.emplace_back(od_of[k], la | (local(st.slot_q[s0 + bb]) << 16));
+        }
       }
     }
-  }
-  for (int k = 0; k < st.nS; ++k) rp.blk_pair_ptr[k + 1] += rp.blk_pair_ptr[k];
-  rp.pair_qa.resize(static_cast<size_t>(rp.pair_instances));
-  rp.pair_qb.resize(static_cast<size_t>(rp.pair_instances));
-  {
-    std::vector<int> fill(rp.blk_pair_ptr.begin(), rp.blk_pair_ptr.end() - 1);
-    size_t q = 0;
-    for (int i = 0; i < st.npos; ++i) {
-      if (st.pos_h[i] < 0) continue;
-      const int a0 = st.pos_ptr[i], nf = st.pos_nfree[i];
-      for (int a = 0; a < nf; ++a)
-        for (int bb = a; bb < nf; ++bb) {
-          const int dst = fill[pair_blk[q++]]++;
-          rp.pair_qa[dst] = st.slot_q[a0 + a];
-          rp.pair_qb[dst] = st.slot_q[a0 + bb];
-        }
+    wp.pairs += static_cast<long long>(lst.size());
+    std::stable_sort(lst.begin(), lst.end(),
+                     [](const std::pair<int, uint32_t>& x, const std::pair<int, uint32_t>& y) { return x.first < y.first; });
+    parts.clear();
+    for (size_t i = 0; i < lst.size();) {
+      size_t j = i;
+      while (j < lst.size() && lst[j].first == lst[i].first) ++j;
+      for (size_t c = i; c < j; c += pch)
+        parts.push_back({lst[i].first, static_cast<int>(c), static_cast<int>(std::min<size_t>(pch, j - c))});
+      i = j;
     }
+    std::stable_sort(parts.begin(), parts.end(), [](const Part& x, const Part& y) { return x.n > y.n; });
+    for (size_t g0 = 0; g0 < parts.size(); g0 += 64) {
+      const int len = parts[g0].n;
+      wp.grp_len.push_back(len);
+      for (int r = 0; r < len; ++r)
+        for (int lane = 0; lane < 64; ++lane) {
+          const size_t i = g0 + lane;
+          wp.pair_rows.push_back(i < parts.size() && r < parts[i].n ? lst[parts[i].beg + r].second : 0xFFFFFFFFu);
+        }
+      wp.grp_row0.push_back(wp.grp_row0.back() + len);
+      for (int lane = 0; lane < 64; ++lane) {
+        const size_t i = g0 + lane;
+        if (i < parts.size()) {
+          wp.grp_slot.push_back(static_cast<int>(part_od.size()));
+          part_od.push_back(parts[i].od);
+          part_rank.push_back(od_cnt[parts[i].od]++);
+        } else {
+          wp.grp_slot.push_back(-1);
+        }
+      }
+    }
+    wp.win_grp0.push_back(static_cast<int>(wp.grp_len.size()));
+    pos = pe;
   }
-  // Workgroups are dealt to the XCDs round-robin by blockIdx; map blockIdx so
-  // that each XCD walks a contiguous range of rows (neighbouring cameras share
-  // points, so their Hpl blocks meet in that XCD's L2). blockIdx blk runs on
-  // XCD blk % X as its (blk / X)-th workgroup.
-  const int T = static_cast<int>(rp.tile_row.size());
-  const int X = std::max(1, num_xcds);
-  std::vector<int> start(X + 1, 0);
-  for (int x = 0; x < X; ++x) start[x + 1] = start[x] + (T > x ? (T - x + X - 1) / X : 0);
-  rp.tile_order.assign(T, 0);
-  for (int blk = 0; blk < T; ++blk) rp.tile_order[blk] = start[blk % X] + blk / X;
+  if (static_cast<long long>(wp.pair_rows.size()) >= (1LL << 31)) return "more than 2^31 Schur pair entries";
+  wp.od_ptr.assign(nod + 1, 0);
+  for (int u = 0; u < nod; ++u) wp.od_ptr[u + 1] = wp.od_ptr[u] + od_cnt[u];
+  for (int& sl : wp.grp_slot)
+    if (sl >= 0) sl = wp.od_ptr[part_od[sl]] + part_rank[sl];
   return "";
 }
 

@@ -28,24 +28,33 @@ struct LinPlanCaps {
   int max_pts = 256;
 };
 
-// Row-owner plan of the explicit Schur complement: one workgroup per tile of
-// one Hschur block row (a free camera). The tile walks the camera's
-// observations of free points and accumulates the row's blocks
-// [k0, k1) (indices into the upper block list) in LDS; the first tile of a row
-// also owns the diagonal block and the right-hand-side coefficient.
-struct RowPlan {
-  int max_cols = 64;                 // Hschur blocks per row tile
-  int chunk = 256;                   // pair products per work item
-  std::vector<int> tile_row, tile_k0, tile_k1;
-  std::vector<int> tile_order;       // blockIdx -> tile, XCD-aware
-  // pair products of each upper Hschur block (i, j): positions (qa, qb) of
-  // the point's observations in camera i's and camera j's observation lists
-  // (the camera-major order of Hpl), grouped by block, points in processing
-  // order; CSR over the nS upper blocks
-  std::vector<int> blk_pair_ptr;     // nS+1 (pair count < 2^31)
-  std::vector<int> pair_qa, pair_qb;
-  long long pair_instances = 0;
-  int max_row_cols = 0;
+// Window plan of the off-diagonal blocks of the explicit Schur complement.
+// The points, in processing order, are cut into windows of at most obs_max
+// observations with a free camera and a free point. One workgroup per window
+// stages the window's Hpl blocks in LDS as G = Hpl L^-T (L L^T = Hll +
+// lambda I of the observation's point), so that the point's contribution to
+// Hschur block (i, j) is G_a G_b^T for its observations a by camera i and b by
+// camera j. Every off-diagonal upper block (i < j) the window touches gets one
+// partial sum per run of at most pch such pairs; one lane forms one partial.
+// Partials are grouped 64 to a lane group, longest first, and their pair
+// lists are stored lane-interleaved (row r holds pair r of the group's 64
+// partials). Each partial owns a slot; slots are block-major (window order
+// within a block) so that a second pass sums every block's slots in a fixed
+// order. Diagonal blocks are formed per camera in FP64 and are not part of
+// the plan.
+struct WinPlan {
+  int obs_max = 512;
+  int pch = 32;
+  long long pairs = 0;               // off-diagonal pair products
+  std::vector<int> win_obs0;         // nwin+1: ranges of win_q
+  std::vector<int> win_q;            // nq: each window's observations (Hpl index q), ascending q
+  std::vector<int> win_grp0;         // nwin+1: lane groups of each window
+  std::vector<int> grp_len;          // ngrp: pair rows of the group (its longest list)
+  std::vector<int> grp_row0;         // ngrp+1: first pair row of the group
+  std::vector<uint32_t> pair_rows;   // 64 per row: a | b << 16 (window-local), 0xFFFFFFFF = none
+  std::vector<int> grp_slot;         // 64 per group: partial slot of the lane, -1 = none
+  std::vector<int> od_blk;           // off-diagonal upper blocks (i < j)
+  std::vector<int> od_ptr;           // their slot ranges (size + 1)
 };
 
 struct BAStructure {
@@ -70,6 +79,7 @@ struct BAStructure {
   int nq = 0;
   std::vector<int> cam_qptr;         // Cf+1
   std::vector<int> q_slot, q_ph;     // nq
+  std::vector<int> q_pt;             // nq: raw point of each observation
   std::vector<int> slot_q;           // ns -> q or -1
   std::vector<int> edge_slot;        // E -> slot or -1 (inactive edge)
   // linearisation chunks
@@ -87,7 +97,7 @@ struct BAStructure {
   int nF = 0;
   std::vector<int> F_ptr, F_col, F_up, F_trans;
   std::vector<int> up_pos, up_posT;  // upper block -> F rows (posT = -1 on diag)
-  RowPlan rows;
+  WinPlan win;
 };
 
 // Builds everything except the Schur plans. Returns an empty string on
@@ -95,8 +105,9 @@ struct BAStructure {
 std::string build_structure(BAStructure& st, int C, int P, int E, const int* edge_cam, const int* edge_pt,
                             const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur);
 
-// Builds st.rows; num_xcds spreads consecutive block rows over the same XCD.
-std::string build_row_plan(BAStructure& st, int max_cols, int chunk, int num_xcds);
+// Builds st.win (windows of at most obs_max observations, partials of at most
+// pch pairs; obs_max <= 65535).
+std::string build_win_plan(BAStructure& st, int obs_max, int pch);
 
 // 3D pose graph (VertexSE3 / EdgeSE3, 6x6 blocks, no Schur): free poses get
 // indices 0..Nf-1 in input order; the Hessian pattern is the diagonal plus one

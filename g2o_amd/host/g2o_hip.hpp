@@ -58,7 +58,7 @@ class BlockSolverHIP {
     if (!_dev.ok(g2ohip_ba_set_structure(_dev.get(), numCams, numPoints, numEdges, edgeCam, edgePoint, camFixed,
                                          pointFixed)))
       return false;
-    long long info[12];
+    long long info[13];
     if (!_dev.ok(g2ohip_ba_structure_info(_dev.get(), info))) return false;
     _size = static_cast<size_t>(6 * info[0] + 3 * info[1]);
     return true;

@@ -105,7 +105,7 @@ class BundleAdjuster:
         return self.ctx.set_options(**kw)
 
     def structure_info(self) -> np.ndarray:
-        info = (ct.c_longlong * 12)()
+        info = (ct.c_longlong * 13)()
         self.ctx.check(self.ctx._lib.g2ohip_ba_structure_info(self.ctx.h, info))
         return np.array(info[:], dtype=np.int64)
 

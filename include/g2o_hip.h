@@ -108,11 +108,12 @@ int g2ohip_get_options(const g2ohip_ctx* ctx, g2ohip_options* opt);
 int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_edges, const int* edge_cam,
                             const int* edge_pt, const uint8_t* cam_fixed, const uint8_t* pt_fixed);
 
-/* Structure summary: info[0..11] = free cameras, free points, active edges,
- * Hschur upper blocks, linearisation chunks, Schur row tiles, Schur pair
- * instances, longest Hschur block row, max slots per point, Hschur blocks in
- * both triangles, (chunk, camera) partial rows of the linearisation, points
- * with active edges. */
+/* Structure summary: info[0..12] = free cameras, free points, active edges,
+ * Hschur upper blocks, linearisation chunks, Schur point windows,
+ * off-diagonal Schur pair products, off-diagonal Schur partial sums, max
+ * slots per point, Hschur blocks in both triangles, (chunk, camera) partial
+ * rows of the linearisation, points with active edges, observations with a
+ * free camera and a free point (Hpl blocks). */
 int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info);
 
 /* Measurements of EdgeSE3ProjectXYZ (edge_project_xyz.cpp:44-49): meas[E*2]
@@ -196,8 +197,9 @@ enum g2ohip_kernel {
   G2OHIP_K_REFINE = 3,
   G2OHIP_K_CHI2 = 4,
   G2OHIP_K_UPDATE = 5,
-  G2OHIP_K_SCHUR_ROWS = 6, /* the Hschur block-row kernel alone (within G2OHIP_K_SCHUR) */
-  G2OHIP_K_COUNT = 7
+  G2OHIP_K_SCHUR_WINDOWS = 6, /* the off-diagonal Schur window kernel alone (within G2OHIP_K_SCHUR) */
+  G2OHIP_K_PCG_SPMV = 7,      /* the explicit-operator direction + SpMV kernel alone (within G2OHIP_K_PCG) */
+  G2OHIP_K_COUNT = 8
 };
 int g2ohip_set_profiling(g2ohip_ctx* ctx, int on);
 int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launches);

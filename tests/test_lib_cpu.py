@@ -46,16 +46,15 @@ def _pairs_numpy(prob):
     for c, p in zip(prob.edge_cam, prob.edge_pt):
         if free[c] >= 0 and prob.pt_fixed[p] == 0:
             by_pt.setdefault(int(p), []).append(int(free[c]))
-    keys = set(range(0, 0))
     pairs = 0
     Cf = int((prob.cam_fixed == 0).sum())
     keys = {(i, i) for i in range(Cf)}
     for cams in by_pt.values():
         cams = sorted(cams)
-        pairs += len(cams) * (len(cams) + 1) // 2
         for a in range(len(cams)):
             for b in range(a, len(cams)):
                 keys.add((cams[a], cams[b]))
+                pairs += cams[a] != cams[b]
     return pairs, len(keys)
 
 
@@ -67,10 +66,14 @@ def test_structure_matches_reference_pattern(seed, cams, pts):
     Cf = int((prob.cam_fixed == 0).sum())
     assert info[0] == Cf and info[1] == prob.num_pts and info[2] == prob.num_edges
     pairs, nS = _pairs_numpy(prob)
-    assert info[6] == pairs      # Schur pair instances
+    assert info[6] == pairs      # off-diagonal Schur pair products
     assert info[3] == nS         # upper Hschur blocks (block_solver.hpp:237-266)
     assert info[9] == 2 * nS - Cf  # both triangles
-    assert info[5] >= Cf         # one or more tiles per block row
+    # every off-diagonal block has at least one partial sum; windows hold at
+    # most 512 observations each
+    assert info[7] >= nS - Cf
+    nq = int(((prob.cam_fixed[prob.edge_cam] == 0) & (prob.pt_fixed[prob.edge_pt] == 0)).sum())
+    assert (nq + 511) // 512 <= info[5] <= nq
 
 
 def test_structure_rejects_bad_indices():
