@@ -68,6 +68,18 @@ int g2ohip_ctx::read_err(int* out) {
   return G2OHIP_OK;
 }
 
+// d_scal[0..3] (chi2, linearisation chi2, max diagonal, scale) and the error
+// word in one round trip.
+int g2ohip_ctx::read_trial_scalars(double* sc4, int* err) {
+  char* hp = reinterpret_cast<char*>(h_pinned);
+  HK(hipMemcpyAsync(hp, d_scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HK(hipMemcpyAsync(hp + 64, d_err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  std::memcpy(sc4, hp, 4 * sizeof(double));
+  std::memcpy(err, hp + 64, sizeof(int));
+  return G2OHIP_OK;
+}
+
 // ---------------------------------------------------------------- timing
 int g2ohip_ctx::prof_begin() {
   if (!profiling) return -1;
@@ -316,7 +328,7 @@ int g2ohip_ctx::compute_active_errors(double* chi2, double* err_out) {
   return G2OHIP_OK;
 }
 
-int g2ohip_ctx::build_system() {
+int g2ohip_ctx::build_system(bool read_chi) {
   if (mode == 1) {
     if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
     return pg_build();
@@ -336,7 +348,7 @@ int g2ohip_ctx::build_system() {
   }
   HK(launch_reduce_hpp(g, d_hpp_part.p, d_Hpp.p, d_b.p, stream));
   HK(launch_sum(d_chunk_chi.p, st.nlc, d_scal.p + 1, stream));
-  CK(read_scalar(d_scal.p + 1, &chi2_lin));
+  if (read_chi) CK(read_scalar(d_scal.p + 1, &chi2_lin));
   have_system = true;
   have_exact = false;
   system_prec = prec;
@@ -387,7 +399,7 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
     };
     HK((pcg_drive<HT, 6>(stream, h_pinned, progress(), nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
                          d_dn_part.p, d_dq_part.p, implicit_reduce_blocks(nb), d_sc.p, max_iter, op, out,
-                         &prog_failed)));
+                         &prog_failed, ctl.abort)));
   } else {
     auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
       const int pb = prof_begin();
@@ -397,15 +409,18 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
       return e;
     };
     HK((pcg_drive<HT, 6>(stream, h_pinned, progress(), nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
-                         d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, out, &prog_failed)));
+                         d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, out, &prog_failed,
+                         ctl.abort)));
   }
   if (prog_failed) prog_ok = false;
   return G2OHIP_OK;
 }
 
 // BlockSolver::solve with Schur (block_solver.hpp:331-477).
+// defer_err: the final error word (back-substitution) is left for the
+// caller to read with its next scalars (d_err).
 template <typename HT>
-int g2ohip_ctx::solve_impl(int* ok) {
+int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   const int which = std::is_same<HT, float>::value ? 0 : 1;
   PrecBufs<HT>& B = bufs_of<HT>(*this);
   const int nb = st.Cf;
@@ -446,9 +461,8 @@ int g2ohip_ctx::solve_impl(int* ok) {
     HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, nullptr, d_b.p, d_cam_part.p, nullptr, 0, stream));
     HK(launch_implicit_rhs_reduce(g, d_cam_part.p, d_b.p, d_bschur.p, stream));
   }
-  int err = 0;
-  CK(read_err(&err));
-  if (err) { *ok = 0; return G2OHIP_OK; }
+  // a failed point or camera block (d_err) stops the PCG at its first head
+  // and is read back with the PCG's own scalars: no separate round trip
   const bool refine = (which == 0) && opt.refine_steps > 0;
   refine_stalled = false;
   const int max_iter = opt.pcg_max_iter < 0 ? 6 * nb : opt.pcg_max_iter;
@@ -456,8 +470,11 @@ int g2ohip_ctx::solve_impl(int* ok) {
   double* xl64 = d_x.p + 6 * static_cast<size_t>(nb);
   // reduced system on the Schur right-hand side, with the residual memory of
   // the previous solve (linear_solver_pcg.hpp:128-136)
+  PcgControl ctl_main{opt.pcg_tolerance, pcg_residual_mem, -1.0};
+  ctl_main.abort = d_err.p;
   PcgRun pr;
-  CK(run_pcg<HT>(B, d_bschur.p, PcgControl{opt.pcg_tolerance, pcg_residual_mem, -1.0}, implicit, max_iter, &pr));
+  CK(run_pcg<HT>(B, d_bschur.p, ctl_main, implicit, max_iter, &pr));
+  if (pr.err_word) { *ok = 0; return G2OHIP_OK; }
   int total_it = pr.iters;
   double dn_final = pr.dn_final;
   HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 0.0, stream));
@@ -473,9 +490,11 @@ int g2ohip_ctx::solve_impl(int* ok) {
         HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, xp64, xl64, d_cam_part.p, d_err.p, stream));
       }
       HK(launch_refine_reduce(g, d_cam_part.p, xp64, lambda, d_res.p, stream));
+      PcgControl ctl_corr{opt.pcg_tolerance, -1.0, pr.d0};
+      ctl_corr.abort = d_err.p;
       PcgRun pc;
-      CK(run_pcg<HT>(B, d_res.p, PcgControl{opt.pcg_tolerance, -1.0, pr.d0}, implicit,
-                     std::min(max_iter, 2 * pr.iters + 10), &pc));
+      CK(run_pcg<HT>(B, d_res.p, ctl_corr, implicit, std::min(max_iter, 2 * pr.iters + 10), &pc));
+      if (pc.err_word) { *ok = 0; return G2OHIP_OK; }
       total_it += pc.iters;
       // the FP32-rounded system stops resolving the FP64 problem: its
       // solution misses the FP64 threshold by more than 10x, or the
@@ -496,31 +515,34 @@ int g2ohip_ctx::solve_impl(int* ok) {
     HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, xp64, xl64, nullptr, d_err.p, stream));
   }
   last_info[1] = refine ? opt.refine_steps : 0;
-  CK(read_err(&err));
-  if (err) *ok = 0;
+  if (!defer_err) {
+    int err = 0;
+    CK(read_err(&err));
+    if (err) *ok = 0;
+  }
   last_info[0] = total_it;
   last_info[2] = pcg_residual_mem;
   have_solution = true;
   return G2OHIP_OK;
 }
 
-int g2ohip_ctx::solve(int* ok) {
+int g2ohip_ctx::solve(int* ok, bool defer_err) {
   if (mode == 1) {
     if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
     return pg_solve(ok);
   }
   if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
   last_info[3] = 0;
-  if (system_prec != G2OHIP_FP32) return solve_impl<double>(ok);
+  if (system_prec != G2OHIP_FP32) return solve_impl<double>(ok, defer_err);
   const double mem = pcg_residual_mem;
-  CK(solve_impl<float>(ok));
+  CK(solve_impl<float>(ok, defer_err));
   if (!refine_stalled) return G2OHIP_OK;
   // FP64 fallback at the same state (the linearisation point: the LM solves
   // before it updates)
   fp32_off = true;
   pcg_residual_mem = mem;
   CK(build_system());
-  CK(solve_impl<double>(ok));
+  CK(solve_impl<double>(ok, defer_err));
   last_info[3] = 1;
   return G2OHIP_OK;
 }
@@ -586,42 +608,64 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
     const double t_it = now_s();
     double t = now_s();
     // computeActiveErrors + activeRobustChi2 + buildSystem, fused: the
-    // linearisation pass returns the chi2 of the same state.
-    CK(build_system());
-    const double currentChi0 = chi2_lin;
-    double currentChi = currentChi0;
+    // linearisation pass returns the chi2 of the same state (read back with
+    // the first trial's scalars unless lambda_init reads first).
+    CK(build_system(i == 0 || mode != 0));
+    double currentChi0 = chi2_lin;
+    bool have_chi0 = i == 0 || mode != 0;
     s.time_quadratic_form = now_s() - t;
     if (i == 0) {
       CK(lambda_init(&lambda));
       ni = 2.0;
     }
+    double currentChi = currentChi0;
     double rho = 0;
     int q = 0;
     do {
       CK(push());
       t = now_s();
       int ok = 0;
-      CK(solve(&ok));
+      CK(solve(&ok, true));
       s.time_linear_solver += now_s() - t;
       s.iterations_linear_solver += static_cast<int>(last_info[0]);
       s.refine_steps += static_cast<int>(last_info[1]);
       s.fp64_fallback |= static_cast<int>(last_info[3]);
       double tempChi = std::numeric_limits<double>::max();
-      if (ok) {
-        t = now_s();
-        CK(update());
-        s.time_update += now_s() - t;
-        t = now_s();
-        CK(compute_active_errors(&tempChi));
-        s.time_residuals += now_s() - t;
-      }
-      rho = currentChi - tempChi;
       double scale = 1.0;
-      if (ok) {
-        CK(compute_scale(&scale));
-        scale += 1e-3;
+      // update, new chi2 and computeScale enqueued together; one read of
+      // chi2, the linearisation chi2, the scale and the error word
+      t = now_s();
+      if (ok && mode == 0) {
+        HK(launch_scale(static_cast<int>(system_size()), d_x.p, d_b.p, lambda, d_red.p, d_scal.p + 3, stream));
+        {
+          ProfScope ps(this, G2OHIP_K_UPDATE);
+          HK(launch_update(g, d_x.p, d_cams.p, d_pts.p, stream));
+        }
+        HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
+        ProfScope ps(this, G2OHIP_K_CHI2);
+        HK(launch_chi2(g, d_camRt.p, d_pts.p, d_red.p, d_scal.p, nullptr, stream));
       }
-      rho /= scale;
+      double sc4[4] = {0, 0, 0, 0};
+      int err = 0;
+      if (mode == 0) {
+        CK(read_trial_scalars(sc4, &err));
+      } else if (ok) {  // pose graph: its own state-side calls
+        CK(update());
+        CK(compute_active_errors(&sc4[0]));
+        CK(compute_scale(&sc4[3]));
+      }
+      s.time_residuals += now_s() - t;
+      if (!have_chi0) {
+        chi2_lin = currentChi0 = currentChi = sc4[1];
+        have_chi0 = true;
+      }
+      if (err) ok = 0;
+      if (ok) {
+        tempChi = sc4[0];
+        scale = sc4[3] + 1e-3;
+        have_solution = true;
+      }
+      rho = (currentChi - tempChi) / scale;
       if (rho > 0 && std::isfinite(tempChi) && ok) {
         double alpha = 1. - std::pow((2 * rho - 1), 3);
         alpha = std::min(alpha, 2. / 3.);

@@ -59,6 +59,7 @@ struct PcgRun {
   double d0 = 0;        // the absolute stop threshold used
   double dn0 = 0;       // r . M^-1 r of the right-hand side
   bool breakdown = false;  // d.q <= 0 or r.M^-1 r < 0 seen: operator or preconditioner not PD
+  int err_word = 0;        // the PcgControl::abort word at the end (0 without one)
 };
 
 struct WinPlanBufs {
@@ -155,8 +156,8 @@ struct g2ohip_ctx {
   int get_state(double* cams, double* pts);
   int ensure_win();
   int compute_active_errors(double* chi2, double* err_out = nullptr);
-  int build_system();
-  int solve(int* ok);
+  int build_system(bool read_chi = true);
+  int solve(int* ok, bool defer_err = false);
   int update();
   int push();
   int pop();
@@ -193,11 +194,12 @@ struct g2ohip_ctx {
   int newton(int steps, double tol, double max_step, int* ok);
 
   template <typename HT>
-  int solve_impl(int* ok);
+  int solve_impl(int* ok, bool defer_err);
   template <typename HT>
   int run_pcg(g2ohip::PrecBufs<HT>& B, const double* rhs, const g2ohip::PcgControl& ctl0, bool implicit, int max_iter,
               g2ohip::PcgRun* out);
   int read_scalar(const double* dptr, double* out);
+  int read_trial_scalars(double* sc4, int* err);
   int read_err(int* out);
 };
 

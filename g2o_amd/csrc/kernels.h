@@ -50,11 +50,15 @@ struct PcgScalars {
 // solve's residual memory, ignored when <= 0), or exactly d0_abs when >= 0.
 // prog (device address of host-mapped memory, or null): the iteration heads
 // store the iteration reached in prog[0] and 1 in prog[1] once done.
+// abort (device address, or null): a nonzero word stops the solve at its
+// next iteration head (an earlier kernel of the same solve reported a
+// failure).
 struct PcgControl {
   double tol;
   double resid_mem;
   double d0_abs;
   int* prog = nullptr;
+  const int* abort = nullptr;
 };
 
 // Host view of PcgControl::prog.

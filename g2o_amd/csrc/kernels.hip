@@ -1155,9 +1155,18 @@ __global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restric
 // beta = dn / dn_old. Returns false (and records the iteration count) when
 // the solve is over. Every workgroup evaluates this identically; workgroup 0
 // publishes the progress (iteration reached, done) to host-mapped memory.
-__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, int* prog, double tol,
-                         double resid_mem, double d0_abs, int it, int last, double* beta) {
+__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, int* prog,
+                         const int* abort, double tol, double resid_mem, double d0_abs, int it, int last,
+                         double* beta) {
   if (sc->done) return false;
+  if (abort && *abort) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = 1;
+      sc->iters = it;
+      if (prog) progress_store(prog + 1, 1);
+    }
+    return false;
+  }
   const double dn = wave_sum_array(part_rs, n_rs);
   double d0;
   if (it == 0) {
@@ -1204,8 +1213,8 @@ template <typename MT, int BD>
 __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
     int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
     const double* __restrict__ s, const double* __restrict__ d_old, double* __restrict__ d, double* __restrict__ q,
-    const double* __restrict__ part_rs, int n_rs, double* __restrict__ part_dq, PcgScalars* sc, int* prog, double tol,
-    double resid_mem, double d0_abs, int it, int last) {
+    const double* __restrict__ part_rs, int n_rs, double* __restrict__ part_dq, PcgScalars* sc, int* prog,
+    const int* abort, double tol, double resid_mem, double d0_abs, int it, int last) {
   const bool firstit = it == 0;
   constexpr int GPW = 64 / BD;  // groups per wavefront
   constexpr int NG = PCG_SPMV_WAVES * GPW;
@@ -1232,7 +1241,7 @@ __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
     }
   }
   double beta;
-  if (!pcg_head(part_rs, n_rs, sc, prog, tol, resid_mem, d0_abs, it, last, &beta)) return;
+  if (!pcg_head(part_rs, n_rs, sc, prog, abort, tol, resid_mem, d0_abs, it, last, &beta)) return;
   double acc = 0;
 #pragma unroll
   for (int h = 0; h < H; ++h)
@@ -1282,10 +1291,10 @@ template <int BD>
 __global__ __launch_bounds__(64) void k_pcg_dir(int nb, const double* __restrict__ s,
                                                 const double* __restrict__ d_old, double* __restrict__ d,
                                                 const double* __restrict__ part_rs, int n_rs, PcgScalars* sc,
-                                                int* prog, double tol, double resid_mem, double d0_abs, int it,
-                                                int last) {
+                                                int* prog, const int* abort, double tol, double resid_mem,
+                                                double d0_abs, int it, int last) {
   double beta;
-  if (!pcg_head(part_rs, n_rs, sc, prog, tol, resid_mem, d0_abs, it, last, &beta)) return;
+  if (!pcg_head(part_rs, n_rs, sc, prog, abort, tol, resid_mem, d0_abs, it, last, &beta)) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= nb) return;
   for (int k = 0; k < BD; ++k) {
@@ -1354,8 +1363,8 @@ hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
   hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, st, nb, F_ptr,
                      F_col, F_val,
-                     s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog, ctl.tol, ctl.resid_mem,
-                     ctl.d0_abs, it, last);
+                     s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog, ctl.abort, ctl.tol,
+                     ctl.resid_mem, ctl.d0_abs, it, last);
   return hipGetLastError();
 }
 
@@ -1363,7 +1372,7 @@ template <int BD>
 hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
                           PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
   hipLaunchKernelGGL((k_pcg_dir<BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, s, d_old, d, part_rs,
-                     pcg_row_blocks(nb), sc, ctl.prog, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
+                     pcg_row_blocks(nb), sc, ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
   return hipGetLastError();
 }
 template hipError_t launch_pcg_dir<6>(int, const double*, const double*, double*, const double*, PcgScalars*,

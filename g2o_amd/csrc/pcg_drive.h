@@ -32,13 +32,20 @@ template <typename MT, int BD, typename OpFn>
 inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProgress* prog, int nb,
                             const double* rhs, const MT* J, double* x, double* r, double* s, double* const dirs[2],
                             double* q, double* part_rs, double* part_dq, int n_dq, PcgScalars* sc, int max_iter,
-                            OpFn op, PcgRun* out, bool* prog_failed) {
+                            OpFn op, PcgRun* out, bool* prog_failed, const int* abort = nullptr) {
   hipError_t e;
   PcgScalars hs{};
+  int hw = 0;
+  static_assert(sizeof(PcgScalars) <= 64, "scalars and the abort word share the pinned buffer");
+  char* hp = static_cast<char*>(h_pinned);
   auto poll = [&]() -> hipError_t {
-    hipError_t e2 = hipMemcpyAsync(h_pinned, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream);
+    hipError_t e2 = hipMemcpyAsync(hp, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream);
+    if (e2 == hipSuccess && abort) e2 = hipMemcpyAsync(hp + 64, abort, sizeof(int), hipMemcpyDeviceToHost, stream);
     if (e2 == hipSuccess) e2 = hipStreamSynchronize(stream);
-    if (e2 == hipSuccess) std::memcpy(&hs, h_pinned, sizeof(PcgScalars));
+    if (e2 == hipSuccess) {
+      std::memcpy(&hs, hp, sizeof(PcgScalars));
+      if (abort) std::memcpy(&hw, hp + 64, sizeof(int));
+    }
     return e2;
   };
   if (prog) {
@@ -93,6 +100,7 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProg
   out->d0 = hs.d0;
   out->dn0 = hs.dn0;
   out->breakdown = hs.err != 0;
+  out->err_word = hw;
   return hipSuccess;
 }
 
