@@ -186,6 +186,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_x.alloc(nx));
   HK(d_Hpp.alloc(36 * static_cast<size_t>(st.Cf)));
   HK(d_hpp_part.alloc(27 * std::max<size_t>(st.lc_cams.size(), 1)));
+  HK(d_diag_part.alloc(27 * std::max<size_t>(st.lc_cams.size(), 1)));
   HK(d_cam_part.alloc(6 * std::max<size_t>(st.lc_cams.size(), 1)));
   HK(d_chunk_chi.alloc(std::max(st.nlc, 1)));
   HK(d_bschur.alloc(6 * static_cast<size_t>(st.Cf)));
@@ -448,13 +449,11 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
       HK(launch_schur_offdiag<HT>(g, dwin, B.part.p, B.F_val.p, stream));
     }
     // diagonal blocks, block-Jacobi inverses and (explicit) the reduced
-    // right-hand side in FP64: from the FP64 blocks, or for FP32 blocks
-    // (too coarse for the cancellation in Hpp - sum V Dinv V^T)
-    // re-linearised in FP64 from the state
-    const HT* hpl_diag = std::is_same<HT, float>::value ? nullptr : B.Hpl.p;
-    HK(launch_schur_diag<HT>(g, d_q_slot.p, d_camRt.p, d_pts.p, hpl_diag, d_Hll.p, d_Hpp.p, d_b.p, lambda,
-                             implicit ? nullptr : B.F_val.p, B.Jinv.p, implicit ? nullptr : d_bschur.p, d_err.p,
-                             stream));
+    // right-hand side in FP64, re-linearised from the state point-major (FP32
+    // blocks are too coarse for the cancellation in Hpp - sum V Dinv V^T)
+    HK(launch_schur_diag_pm<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, implicit ? nullptr : d_b.p, lambda,
+                                d_diag_part.p, implicit ? nullptr : B.F_val.p, B.Jinv.p,
+                                implicit ? nullptr : d_bschur.p, d_err.p, stream));
   }
   if (implicit) {
     // matrix-free reduced operator: right-hand side by one pass

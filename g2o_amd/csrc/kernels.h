@@ -126,6 +126,13 @@ hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, i
 // Writes F_val's diagonal blocks (if F_val), their inverses Jinv (err |= 2
 // if not PD) and, if bschur, bschur_i = b_i - sum_p V_ip Dinv_p b_p.
 // q_slot: slot of each camera-major observation.
+// The same for the LM path, point-major (V always re-linearised in FP64 from
+// the state): chunk partials into part (27 per linearisation chunk camera
+// row), then per camera the block, its inverse and bschur.
+template <typename HT>
+hipError_t launch_schur_diag_pm(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
+                                const double* Hpp, const double* b, double lambda, double* part, HT* F_val, HT* Jinv,
+                                double* bschur, int* err, hipStream_t st);
 template <typename HT>
 hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
                              const HT* Hpl, const double* Hll, const double* Hpp, const double* b, double lambda,

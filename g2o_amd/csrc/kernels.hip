@@ -280,6 +280,21 @@ template hipError_t launch_linearize<float>(const DevGraph&, const double*, cons
 template hipError_t launch_linearize<double>(const DevGraph&, const double*, const double*, double*, double*,
                                              double*, double*, double*, hipStream_t);
 
+// Sum over free camera c's chunk partial rows of entry k (rows of `stride`
+// doubles): four running sums keep four loads in flight; their fixed
+// combination keeps the result deterministic.
+__device__ __forceinline__ double camp_sum(const DevGraph& g, const double* __restrict__ part, int stride, int c,
+                                           int k) {
+  double v4[4] = {0.0, 0.0, 0.0, 0.0};
+  const int q0 = g.camp_ptr[c], q1 = g.camp_ptr[c + 1];
+  int q = q0;
+  for (; q + 4 <= q1; q += 4)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v4[j] += part[static_cast<size_t>(g.camp_idx[q + j]) * stride + k];
+  for (int j = 0; q < q1; ++q, ++j) v4[j] += part[static_cast<size_t>(g.camp_idx[q]) * stride + k];
+  return (v4[0] + v4[1]) + (v4[2] + v4[3]);
+}
+
 // Hpp and b_p (b == null: Hpp only): sum of the chunk partials of each free
 // camera, fixed order.
 __global__ void k_reduce_hpp(DevGraph g, const double* __restrict__ hpp_part, double* __restrict__ Hpp,
@@ -287,8 +302,7 @@ __global__ void k_reduce_hpp(DevGraph g, const double* __restrict__ hpp_part, do
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.Cf * 27) return;
   const int c = t / 27, k = t % 27;
-  double v = 0;
-  for (int q = g.camp_ptr[c]; q < g.camp_ptr[c + 1]; ++q) v += hpp_part[static_cast<size_t>(g.camp_idx[q]) * 27 + k];
+  const double v = camp_sum(g, hpp_part, 27, c, k);
   if (k < 21) {
     const int r = c_up6_r[k], cc = c_up6_c[k];
     Hpp[36 * static_cast<size_t>(c) + 6 * r + cc] = v;
@@ -838,9 +852,7 @@ __global__ void k_refine_reduce(DevGraph g, const double* __restrict__ cam_part,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 6 * g.Cf) return;
   const int c = t / 6, k = t % 6;
-  double v = xp ? -lambda * xp[t] : 0.0;
-  for (int q = g.camp_ptr[c]; q < g.camp_ptr[c + 1]; ++q) v += cam_part[6 * static_cast<size_t>(g.camp_idx[q]) + k];
-  r[t] = v;
+  r[t] = (xp ? -lambda * xp[t] : 0.0) + camp_sum(g, cam_part, 6, c, k);
 }
 
 hipError_t launch_refine_reduce(const DevGraph& g, const double* cam_part, const double* xp, double lambda, double* r,
@@ -1050,6 +1062,131 @@ __global__ __launch_bounds__(256) void k_schur_diag(DevGraph g, const int* __res
   }
   for (int k = 0; k < 36; ++k) Jinv[36 * static_cast<size_t>(c) + k] = static_cast<HT>(m[k]);
 }
+
+// The same diagonal terms point-major (the LM path): one workgroup per
+// linearisation chunk, whose slots, points and measurements are contiguous.
+// Per point D = (Hll + lambda I)^-1 and y = D b_p go to LDS; per observation
+// run (one point, one camera) V is re-linearised in FP64 from the state and
+// the 21 upper entries of V D V^T and the 6 entries of V y are added to the
+// chunk camera's LDS accumulator (f64 atomics); the chunk writes one partial
+// row per camera in the layout of the linearisation's Hpp partials.
+__global__ __launch_bounds__(LIN_THREADS) void k_schur_diag_pm(DevGraph g, const double* __restrict__ camRt,
+                                                               const double* __restrict__ pts,
+                                                               const double* __restrict__ Hll,
+                                                               const double* __restrict__ b, double lambda,
+                                                               double* __restrict__ part, int* __restrict__ err) {
+  __shared__ int s_cam[LIN_CAMS];
+  __shared__ double s_cacc[LIN_CAMS * 27];
+  __shared__ double s_pd[LIN_PTS * 9];
+  const int ch = blockIdx.x, tid = threadIdx.x;
+  const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
+  const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
+  const int c0 = g.lc_cam_ptr[ch], nc = g.lc_cam_ptr[ch + 1] - c0;
+  const int np = p1 - p0;
+  const double* bl = b ? b + 6 * static_cast<size_t>(g.Cf) : nullptr;
+  for (int i = tid; i < nc; i += LIN_THREADS) s_cam[i] = g.lc_cams[c0 + i];
+  for (int i = tid; i < nc * 27; i += LIN_THREADS) s_cacc[i] = 0.0;
+  for (int lp = tid; lp < np; lp += LIN_THREADS) {
+    const int ph = g.pos_h[p0 + lp];
+    double* pd = s_pd + 9 * lp;
+    if (ph < 0) continue;
+    double D[9];
+    if (!inv3_sym(Hll + 6 * static_cast<size_t>(ph), lambda, D)) {
+      atomicOr(err, 1);
+      for (int k = 0; k < 9; ++k) D[k] = 0.0;
+    }
+    pd[0] = D[0]; pd[1] = D[1]; pd[2] = D[2]; pd[3] = D[4]; pd[4] = D[5]; pd[5] = D[8];
+    for (int r = 0; r < 3; ++r) {
+      const double* y = bl ? bl + 3 * static_cast<size_t>(ph) : nullptr;
+      pd[6 + r] = y ? D[3 * r] * y[0] + D[3 * r + 1] * y[1] + D[3 * r + 2] * y[2] : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
+    const int pos = g.slot_pos[s], cam = g.slot_cam[s];
+    const int ph = g.pos_h[pos], chh = g.camH[cam];
+    if (ph < 0 || chh < 0) continue;
+    if (s > s0 && g.slot_pos[s - 1] == pos && g.slot_cam[s - 1] == cam) continue;  // the run's first slot sums it
+    const int praw = g.pos_pt[pos];
+    const double* Rt = camRt + 12 * static_cast<size_t>(cam);
+    const double* in = g.intr + 4 * static_cast<size_t>(cam);
+    double V[18];
+    for (int k = 0; k < 18; ++k) V[k] = 0.0;
+    obs_V64(g, Rt, in, pts, s, praw, V);
+    for (int s2 = s + 1; s2 < s1 && g.slot_pos[s2] == pos && g.slot_cam[s2] == cam; ++s2)
+      obs_V64(g, Rt, in, pts, s2, praw, V);
+    const double* pd = s_pd + 9 * (pos - p0);
+    const double D[9] = {pd[0], pd[1], pd[2], pd[1], pd[3], pd[4], pd[2], pd[4], pd[5]};
+    double VD[18];
+    for (int r = 0; r < 6; ++r)
+      for (int cc = 0; cc < 3; ++cc)
+        VD[3 * r + cc] = V[3 * r] * D[cc] + V[3 * r + 1] * D[3 + cc] + V[3 * r + 2] * D[6 + cc];
+    double* ca = s_cacc + 27 * lower_bound_i(s_cam, nc, chh);
+    int k = 0;
+    for (int r = 0; r < 6; ++r)
+      for (int cc = r; cc < 6; ++cc)
+        atomicAdd(&ca[k++], VD[3 * r] * V[3 * cc] + VD[3 * r + 1] * V[3 * cc + 1] + VD[3 * r + 2] * V[3 * cc + 2]);
+    if (bl)
+      for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], V[3 * r] * pd[6] + V[3 * r + 1] * pd[7] + V[3 * r + 2] * pd[8]);
+  }
+  __syncthreads();
+  for (int i = tid; i < nc * 27; i += LIN_THREADS) part[static_cast<size_t>(c0) * 27 + i] = s_cacc[i];
+}
+
+// Per free camera (64 lanes): the chunk partials summed in a fixed order
+// (lanes 0..26), then S_ii = Hpp_ii + lambda I - sum into F_val (if not
+// null), its inverse into Jinv (err |= 2 if not PD) and bschur_i = b_i - sum
+// (if bschur).
+template <typename HT>
+__global__ __launch_bounds__(64) void k_schur_diag_finish(DevGraph g, const double* __restrict__ part,
+                                                          const double* __restrict__ Hpp,
+                                                          const double* __restrict__ b, double lambda,
+                                                          HT* __restrict__ F_val, HT* __restrict__ Jinv,
+                                                          double* __restrict__ bschur, int* __restrict__ err) {
+  __shared__ double s_v[27];
+  const int c = blockIdx.x, lane = threadIdx.x;
+  if (lane < 27) s_v[lane] = camp_sum(g, part, 27, c, lane);
+  __syncthreads();
+  if (bschur && lane >= 21 && lane < 27)
+    bschur[6 * static_cast<size_t>(c) + lane - 21] = b[6 * static_cast<size_t>(c) + lane - 21] - s_v[lane];
+  if (lane != 0) return;
+  double m[36];
+  for (int k = 0; k < 21; ++k) {
+    const int r = c_up6_r[k], cc = c_up6_c[k];
+    const double h = Hpp[36 * static_cast<size_t>(c) + 6 * r + cc] - s_v[k];
+    m[6 * r + cc] = h;
+    m[6 * cc + r] = h;
+  }
+  for (int k = 0; k < 6; ++k) m[7 * k] += lambda;
+  if (F_val) {
+    HT* dst = F_val + 36 * static_cast<size_t>(g.F_diag[c]);
+    for (int k = 0; k < 36; ++k) dst[k] = static_cast<HT>(m[k]);
+  }
+  if (!inv_spd<6>(m)) {
+    atomicOr(err, 2);
+    for (int k = 0; k < 36; ++k) m[k] = 0;
+  }
+  for (int k = 0; k < 36; ++k) Jinv[36 * static_cast<size_t>(c) + k] = static_cast<HT>(m[k]);
+}
+
+template <typename HT>
+hipError_t launch_schur_diag_pm(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
+                                const double* Hpp, const double* b, double lambda, double* part, HT* F_val, HT* Jinv,
+                                double* bschur, int* err, hipStream_t st) {
+  if (g.Cf == 0) return hipSuccess;
+  if (g.nlc > 0)
+    hipLaunchKernelGGL(k_schur_diag_pm, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hll, b, lambda, part,
+                       err);
+  hipLaunchKernelGGL(k_schur_diag_finish<HT>, dim3(g.Cf), dim3(64), 0, st, g, part, Hpp, b, lambda, F_val, Jinv,
+                     bschur, err);
+  return hipGetLastError();
+}
+template hipError_t launch_schur_diag_pm<float>(const DevGraph&, const double*, const double*, const double*,
+                                                const double*, const double*, double, double*, float*, float*,
+                                                double*, int*, hipStream_t);
+template hipError_t launch_schur_diag_pm<double>(const DevGraph&, const double*, const double*, const double*,
+                                                 const double*, const double*, double, double*, double*, double*,
+                                                 double*, int*, hipStream_t);
 
 template <typename HT>
 hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
