@@ -1295,8 +1295,15 @@ __global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restric
 __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, int* prog,
                          const int* abort, double tol, double resid_mem, double d0_abs, int it, int last,
                          double* beta) {
-  if (sc->done) return false;
-  if (abort && *abort) {
+  // every load of the head is issued before the first branch: the scalars,
+  // the abort word and the partials are one dependency level, not four
+  const int done0 = sc->done;
+  const int ab = abort ? *abort : 0;
+  const double d0_prev = sc->d0;
+  const double dn_old = sc->dn[(it + 1) & 1];
+  const double dn = wave_sum_array(part_rs, n_rs);
+  if (done0) return false;
+  if (ab) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->done = 1;
       sc->iters = it;
@@ -1304,7 +1311,6 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
     }
     return false;
   }
-  const double dn = wave_sum_array(part_rs, n_rs);
   double d0;
   if (it == 0) {
     if (d0_abs >= 0.0) {
@@ -1314,7 +1320,7 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
       if (resid_mem > 0.0 && resid_mem > d0) d0 = resid_mem;
     }
   } else {
-    d0 = sc->d0;
+    d0 = d0_prev;
   }
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (first) {
@@ -1331,7 +1337,7 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
     return false;
   }
   if (first && prog) progress_store(prog, it);
-  *beta = it == 0 ? 0.0 : dn / sc->dn[(it + 1) & 1];
+  *beta = it == 0 ? 0.0 : dn / dn_old;
   return true;
 }
 
@@ -1458,9 +1464,11 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
   }
 #pragma unroll
   for (int c = 0; c < BD; ++c) jr[c] = on ? static_cast<double>(J[static_cast<size_t>(i) * BD * BD + BD * k + c]) : 0.0;
-  if (sc->done) return;
+  const int done0 = sc->done;
+  const double dn = sc->dn[it & 1];
   const double dq = wave_sum_array(part_dq, n_dq);
-  const double a = sc->dn[it & 1] / dq;
+  if (done0) return;
+  const double a = dn / dq;
   // d . q <= 0: the operator is not positive definite along d (reported in
   // sc->err for callers that need it; the iteration itself follows
   // linear_solver_pcg.hpp, which does not check)
