@@ -442,9 +442,11 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
     } else {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
       HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win.od_ptr.back(), 1))));
+      HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
+      HK(launch_point_linv(g, d_Hll.p, lambda, d_linv.p, d_err.p, stream));
       {
         ProfScope pr(this, G2OHIP_K_SCHUR_WINDOWS);
-        HK(launch_schur_win<HT>(g, dwin, B.Hpl.p, d_Hll.p, lambda, B.part.p, d_err.p, stream));
+        HK(launch_schur_win<HT>(g, dwin, B.Hpl.p, d_linv.p, B.part.p, stream));
       }
       HK(launch_schur_offdiag<HT>(g, dwin, B.part.p, B.F_val.p, stream));
     }
@@ -861,7 +863,9 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
   HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
   HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win.od_ptr.back(), 1))));
   HK(launch_point_dinv<double>(g, d_Hll.p, lam, B.Dinv.p, d_err.p, stream));
-  HK(launch_schur_win<double>(g, dwin, B.Hpl.p, d_Hll.p, lam, B.part.p, d_err.p, stream));
+  HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
+  HK(launch_point_linv(g, d_Hll.p, lam, d_linv.p, d_err.p, stream));
+  HK(launch_schur_win<double>(g, dwin, B.Hpl.p, d_linv.p, B.part.p, stream));
   HK(launch_schur_offdiag<double>(g, dwin, B.part.p, B.F_val.p, stream));
   HK(launch_schur_diag<double>(g, d_q_slot.p, d_camRt.p, d_pts.p, B.Hpl.p, d_Hll.p, d_Hpp.p, d_b.p, lam, B.F_val.p,
                                B.Jinv.p, d_bschur.p, d_err.p, stream));

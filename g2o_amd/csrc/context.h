@@ -108,6 +108,7 @@ struct g2ohip_ctx {
   std::vector<g2ohip::DBuf<double>*> stack_c, stack_p;
   int stack_depth = 0;
 
+  g2ohip::DBuf<double> d_linv;  // L^-1 of the point blocks (Schur windows)
   g2ohip::DBuf<double> d_Hll, d_b, d_Hpp, d_hpp_part, d_diag_part, d_chunk_chi, d_coef_part, d_bschur, d_x, d_red, d_scal,
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;

@@ -88,11 +88,15 @@ hipError_t launch_scale(int n, const double* x, const double* b, double lambda, 
 // ---- explicit Schur complement
 constexpr int SCHUR_WIN_OBS = 512;  // observations per window (LDS-staged G blocks)
 constexpr int SCHUR_WIN_PCH = 32;   // pair products per partial
+// L^-1 of L L^T = Hll + lambda I per free point (6 doubles, lower triangle
+// row by row); err |= 1 if not positive definite.
+hipError_t launch_point_linv(const DevGraph& g, const double* Hll, double lambda, double* Linv, int* err,
+                             hipStream_t st);
 // Off-diagonal partials: part[slot*36] = sum of G_a G_b^T over the slot's
-// pairs, G = Hpl L^-T with L L^T = Hll + lambda I (err |= 1 if not PD).
+// pairs, G = Hpl L^-T with the point's L^-1 from launch_point_linv.
 template <typename HT>
-hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* Hpl, const double* Hll, double lambda,
-                            HT* part, int* err, hipStream_t st);
+hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* Hpl, const double* Linv, HT* part,
+                            hipStream_t st);
 // Sums every off-diagonal upper block's partials (fixed order, FP64) and
 // writes -sum and its transpose into both triangles of F_val.
 template <typename HT>

@@ -453,10 +453,30 @@ __device__ __forceinline__ void pair_mac(const HT* __restrict__ sG, unsigned pr,
       acc[6 * i + j] += Ga[3 * i] * Gb[3 * j] + Ga[3 * i + 1] * Gb[3 * j + 1] + Ga[3 * i + 2] * Gb[3 * j + 2];
 }
 
+// L^-1 (lower triangular, [i00 i10 i11 i20 i21 i22]) of L L^T = Hll + lambda I
+// per free point; err |= 1 unless positive definite.
+__global__ __launch_bounds__(256) void k_point_linv(int Pf, const double* __restrict__ Hll, double lambda,
+                                                    double* __restrict__ Linv, int* __restrict__ err) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= Pf) return;
+  double m[6];
+  if (!chol3_inv(Hll + 6 * static_cast<size_t>(p), lambda, m)) {
+    atomicOr(err, 1);
+    for (int k = 0; k < 6; ++k) m[k] = 0.0;
+  }
+  for (int k = 0; k < 6; ++k) Linv[6 * static_cast<size_t>(p) + k] = m[k];
+}
+
+hipError_t launch_point_linv(const DevGraph& g, const double* Hll, double lambda, double* Linv, int* err,
+                             hipStream_t st) {
+  if (g.Pf == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_point_linv, dim3(nblocks(g.Pf, 256)), dim3(256), 0, st, g.Pf, Hll, lambda, Linv, err);
+  return hipGetLastError();
+}
+
 template <typename HT>
 __global__ __launch_bounds__(256) void k_schur_win(DevGraph g, DevWinPlan wp, const HT* __restrict__ Hpl,
-                                                  const double* __restrict__ Hll, double lambda,
-                                                  HT* __restrict__ part, int* __restrict__ err) {
+                                                  const double* __restrict__ Linv, HT* __restrict__ part) {
   constexpr int GS = GStride<HT>::v;
   using CT = typename Chunk<HT>::T;
   constexpr int NCH = Chunk<HT>::per_block;
@@ -476,20 +496,24 @@ __global__ __launch_bounds__(256) void k_schur_win(DevGraph g, DevWinPlan wp, co
   // the window's Hpl blocks in 8/16-byte chunks; its observations ascend in
   // q, so runs of one camera are contiguous in HBM. Independent loads: the
   // unrolled loop keeps several in flight per thread.
+  // chunk e = (l, k) walked incrementally: 256 = 28 NCH + 4 for NCH = 9
+  static_assert(NCH == 9, "the staging walk assumes 9 chunks per block");
+  {
+    int l = tid / NCH, k = tid - NCH * (tid / NCH);
 #pragma unroll 6
-  for (int e = tid; e < nob * NCH; e += 256) {
-    const int l = e / NCH, k = e - NCH * l;
-    const CT v = reinterpret_cast<const CT*>(Hpl + 18 * static_cast<size_t>(s_q(l)))[k];
-    reinterpret_cast<CT*>(s_G + l * GS)[k] = v;
+    for (int e = tid; e < nob * NCH; e += 256) {
+      const CT v = reinterpret_cast<const CT*>(Hpl + 18 * static_cast<size_t>(s_q(l)))[k];
+      reinterpret_cast<CT*>(s_G + l * GS)[k] = v;
+      l += 28;
+      k += 4;
+      if (k >= NCH) { k -= NCH; ++l; }
+    }
   }
   __syncthreads();
-  // G = Hpl L^-T, L L^T = Hll + lambda I of the observation's point
+  // G = Hpl L^-T with the point's L^-1 (k_point_linv)
   for (int l = tid; l < nob; l += 256) {
-    double m[6];
-    if (!chol3_inv(Hll + 6 * static_cast<size_t>(g.q_ph[s_q(l)]), lambda, m)) {
-      atomicOr(err, 1);
-      for (int k = 0; k < 6; ++k) m[k] = 0.0;
-    }
+    const double* mp = Linv + 6 * static_cast<size_t>(g.q_ph[s_q(l)]);
+    const double m[6] = {mp[0], mp[1], mp[2], mp[3], mp[4], mp[5]};
     HT* G = s_G + l * GS;
     for (int r = 0; r < 6; ++r) {
       const double h0 = G[3 * r], h1 = G[3 * r + 1], h2 = G[3 * r + 2];
@@ -550,10 +574,10 @@ __global__ __launch_bounds__(256) void k_schur_offdiag(DevGraph g, DevWinPlan wp
 }
 
 template <typename HT>
-hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* Hpl, const double* Hll, double lambda,
-                            HT* part, int* err, hipStream_t st) {
+hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* Hpl, const double* Linv, HT* part,
+                            hipStream_t st) {
   if (wp.nwin == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_schur_win<HT>, dim3(wp.nwin), dim3(256), 0, st, g, wp, Hpl, Hll, lambda, part, err);
+  hipLaunchKernelGGL(k_schur_win<HT>, dim3(wp.nwin), dim3(256), 0, st, g, wp, Hpl, Linv, part);
   return hipGetLastError();
 }
 template <typename HT>
@@ -564,10 +588,10 @@ hipError_t launch_schur_offdiag(const DevGraph& g, const DevWinPlan& wp, const H
                      part, F_val);
   return hipGetLastError();
 }
-template hipError_t launch_schur_win<float>(const DevGraph&, const DevWinPlan&, const float*, const double*, double,
-                                            float*, int*, hipStream_t);
+template hipError_t launch_schur_win<float>(const DevGraph&, const DevWinPlan&, const float*, const double*, float*,
+                                            hipStream_t);
 template hipError_t launch_schur_win<double>(const DevGraph&, const DevWinPlan&, const double*, const double*,
-                                             double, double*, int*, hipStream_t);
+                                             double*, hipStream_t);
 template hipError_t launch_schur_offdiag<float>(const DevGraph&, const DevWinPlan&, const float*, float*, hipStream_t);
 template hipError_t launch_schur_offdiag<double>(const DevGraph&, const DevWinPlan&, const double*, double*,
                                                  hipStream_t);

@@ -28,11 +28,12 @@ namespace g2ohip {
 constexpr int PCG_LAG = 3;
 constexpr double PCG_STALL_MS = 50.0;
 
-template <typename MT, int BD, typename OpFn>
-inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProgress* prog, int nb,
-                            const double* rhs, const MT* J, double* x, double* r, double* s, double* const dirs[2],
-                            double* q, double* part_rs, double* part_dq, int n_dq, PcgScalars* sc, int max_iter,
-                            OpFn op, PcgRun* out, bool* prog_failed, const int* abort = nullptr) {
+// Host loop shared by both arrangements: init() once, then step(it, last)
+// per iteration (step(max_iter, 1) only runs the final head).
+template <typename InitFn, typename StepFn>
+inline hipError_t pcg_loop(hipStream_t stream, void* h_pinned, volatile PcgProgress* prog, PcgScalars* sc,
+                           int max_iter, InitFn init, StepFn step, PcgRun* out, bool* prog_failed,
+                           const int* abort) {
   hipError_t e;
   PcgScalars hs{};
   int hw = 0;
@@ -53,15 +54,12 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProg
     prog->done = 0;
   }
   if ((e = hipMemsetAsync(sc, 0, sizeof(PcgScalars), stream)) != hipSuccess) return e;
-  if ((e = launch_pcg_init<MT, BD>(nb, rhs, J, x, r, s, part_rs, stream)) != hipSuccess) return e;
+  if ((e = init()) != hipSuccess) return e;
   int next = 4;
   bool done = false;
   bool have_final = false;  // hs already holds the final scalars
   for (int it = 0; it < max_iter; ++it) {
-    if ((e = op(it, 0, dirs[(it + 1) & 1], dirs[it & 1])) != hipSuccess) return e;
-    if ((e = launch_pcg_update<MT, BD>(nb, J, x, r, dirs[it & 1], q, s, part_dq, n_dq, part_rs, sc, it, stream)) !=
-        hipSuccess)
-      return e;
+    if ((e = step(it, 0)) != hipSuccess) return e;
     if (prog) {
       if (prog->done) { done = true; break; }
       if (it - prog->it < PCG_LAG) continue;
@@ -92,7 +90,7 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProg
     }
   }
   if (!done) {
-    if ((e = op(max_iter, 1, dirs[(max_iter + 1) & 1], dirs[max_iter & 1])) != hipSuccess) return e;
+    if ((e = step(max_iter, 1)) != hipSuccess) return e;
   }
   if (!have_final && (e = poll()) != hipSuccess) return e;
   out->iters = hs.iters;
@@ -102,6 +100,21 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProg
   out->breakdown = hs.err != 0;
   out->err_word = hw;
   return hipSuccess;
+}
+
+// Two-launch arrangement: `op` (direction + operator) and the update kernel.
+template <typename MT, int BD, typename OpFn>
+inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProgress* prog, int nb,
+                            const double* rhs, const MT* J, double* x, double* r, double* s, double* const dirs[2],
+                            double* q, double* part_rs, double* part_dq, int n_dq, PcgScalars* sc, int max_iter,
+                            OpFn op, PcgRun* out, bool* prog_failed, const int* abort = nullptr) {
+  auto init = [&]() { return launch_pcg_init<MT, BD>(nb, rhs, J, x, r, s, part_rs, stream); };
+  auto step = [&](int it, int last) -> hipError_t {
+    hipError_t e = op(it, last, dirs[(it + 1) & 1], dirs[it & 1]);
+    if (e != hipSuccess || last) return e;
+    return launch_pcg_update<MT, BD>(nb, J, x, r, dirs[it & 1], q, s, part_dq, n_dq, part_rs, sc, it, stream);
+  };
+  return pcg_loop(stream, h_pinned, prog, sc, max_iter, init, step, out, prog_failed, abort);
 }
 
 }  // namespace g2ohip
