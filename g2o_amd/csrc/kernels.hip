@@ -1376,13 +1376,14 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
 // part_dq[row] = d . q over the row.
 constexpr int PCG_SPMV_WAVES = 2;
 constexpr int PCG_SPMV_HOIST = 3;
-template <typename MT, int BD>
+// FIRST: iteration 0 (d = s; no d_old, no beta), a separate instantiation so
+// the other iterations carry no run-time branch on it.
+template <typename MT, int BD, bool FIRST>
 __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
     int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
     const double* __restrict__ s, const double* __restrict__ d_old, double* __restrict__ d, double* __restrict__ q,
     const double* __restrict__ part_rs, int n_rs, double* __restrict__ part_dq, PcgScalars* sc, int* prog,
     const int* abort, double tol, double resid_mem, double d0_abs, int it, int last) {
-  const bool firstit = it == 0;
   constexpr int GPW = 64 / BD;  // groups per wavefront
   constexpr int NG = PCG_SPMV_WAVES * GPW;
   constexpr int H = PCG_SPMV_HOIST;
@@ -1404,7 +1405,7 @@ __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
     for (int c = 0; c < BD; ++c) {
       v[h][c] = on ? V[c] : MT(0);
       sv[h][c] = s[j + c];
-      dv[h][c] = firstit ? 0.0 : d_old[j + c];
+      if (!FIRST) dv[h][c] = d_old[j + c];
     }
   }
   double beta;
@@ -1413,25 +1414,16 @@ __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
 #pragma unroll
   for (int h = 0; h < H; ++h)
 #pragma unroll
-    for (int c = 0; c < BD; ++c) acc += static_cast<double>(v[h][c]) * pcg_dir_val(sv[h][c], dv[h][c], beta, firstit);
+    for (int c = 0; c < BD; ++c)
+      acc += static_cast<double>(v[h][c]) * (FIRST ? sv[h][c] : fma(beta, dv[h][c], sv[h][c]));
   if (gw < GPW) {
-    // blocks past the hoisted ones (rows longer than H * NG blocks), two at a
-    // time
-    for (int p = p0 + grp + H * NG; p < p1; p += 2 * NG) {
-      const int pb = p + NG;
-      const bool hb = pb < p1;
-      const size_t ja = BD * static_cast<size_t>(F_col[p]);
-      const size_t jb = hb ? BD * static_cast<size_t>(F_col[pb]) : ja;
-      const MT* Va = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
-      const MT* Vb = F_val + static_cast<size_t>(hb ? pb : p) * BD * BD + BD * k;
-      double ta = 0.0, tb = 0.0;
+    // blocks past the hoisted ones (rows longer than H * NG blocks)
+    for (int p = p0 + grp + H * NG; p < p1; p += NG) {
+      const MT* V = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
+      const size_t j = BD * static_cast<size_t>(F_col[p]);
 #pragma unroll
-      for (int c = 0; c < BD; ++c) {
-        ta += static_cast<double>(Va[c]) * pcg_dir_val(s[ja + c], firstit ? 0.0 : d_old[ja + c], beta, firstit);
-        tb += static_cast<double>(Vb[c]) * pcg_dir_val(s[jb + c], firstit ? 0.0 : d_old[jb + c], beta, firstit);
-      }
-      acc += ta;
-      if (hb) acc += tb;
+      for (int c = 0; c < BD; ++c)
+        acc += static_cast<double>(V[c]) * (FIRST ? s[j + c] : fma(beta, d_old[j + c], s[j + c]));
     }
   }
   s_part[wv][lane] = acc;
@@ -1443,7 +1435,7 @@ __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
       for (int w2 = 0; w2 < PCG_SPMV_WAVES; ++w2)
         for (int g2 = 0; g2 < GPW; ++g2) vs += s_part[w2][g2 * BD + lane];
       const size_t o = BD * static_cast<size_t>(row) + lane;
-      const double dd = pcg_dir_val(s[o], firstit ? 0.0 : d_old[o], beta, firstit);
+      const double dd = FIRST ? s[o] : fma(beta, d_old[o], s[o]);
       q[o] = vs;
       d[o] = dd;
       dot = vs * dd;
@@ -1530,10 +1522,14 @@ template <typename MT, int BD>
 hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, st, nb, F_ptr,
-                     F_col, F_val,
-                     s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog, ctl.abort, ctl.tol,
-                     ctl.resid_mem, ctl.d0_abs, it, last);
+  if (it == 0)
+    hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, true>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, st,
+                       nb, F_ptr, F_col, F_val, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog,
+                       ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
+  else
+    hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, false>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, st,
+                       nb, F_ptr, F_col, F_val, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog,
+                       ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
   return hipGetLastError();
 }
 
