@@ -471,6 +471,7 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   double* xl64 = d_x.p + 6 * static_cast<size_t>(nb);
   // reduced system on the Schur right-hand side, with the residual memory of
   // the previous solve (linear_solver_pcg.hpp:128-136)
+  bool xl_current = false;  // x_l matches the current x_p
   PcgControl ctl_main{opt.pcg_tolerance, pcg_residual_mem, -1.0};
   ctl_main.abort = d_err.p;
   PcgRun pr;
@@ -486,6 +487,7 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
     // meets the reference stop rule measured on the FP64 residual. A
     // correction runs at most twice the first solve's iterations plus 10.
     for (int step = 0; step < opt.refine_steps; ++step) {
+      xl_current = true;  // the residual pass writes x_l for the current x_p
       {
         ProfScope ps(this, G2OHIP_K_REFINE);
         HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, xp64, xl64, d_cam_part.p, d_err.p, stream));
@@ -505,13 +507,17 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
       // threshold the solve stopped on, also when a correction stopped at
       // its iteration cap above it
       dn_final = std::min(pc.dn_final, pr.d0);
-      if (pc.iters > 0) HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 1.0, stream));
+      if (pc.iters > 0) {
+        HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 1.0, stream));
+        xl_current = false;
+      }
     }
   }
   pcg_residual_mem = 0.5 * dn_final;
   // back-substitution x_l = Dinv (b_l - Hpl^T x_p), re-linearised in FP64
-  // (block_solver.hpp:452-476)
-  {
+  // (block_solver.hpp:452-476); already done by the last residual pass if
+  // no correction changed x_p after it
+  if (!xl_current) {
     ProfScope ps(this, G2OHIP_K_REFINE);
     HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, xp64, xl64, nullptr, d_err.p, stream));
   }

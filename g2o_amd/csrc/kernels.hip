@@ -1238,16 +1238,18 @@ template hipError_t launch_schur_diag<double>(const DevGraph&, const int*, const
 
 // Compensated sum of part[0..n); every lane of the calling wavefront returns
 // the same value (pairs of lanes combine with the same commutative ops).
+// B loads in flight per lane (one batch covers 64 B entries), summed in
+// index order.
+template <int B = 8>
 __device__ double wave_sum_array(const double* __restrict__ part, int n) {
   const int lane = threadIdx.x & 63;
   double s = 0, c = 0;
-  // eight loads in flight per lane, summed in index order
-  for (int i0 = lane; i0 < n; i0 += 64 * 8) {
-    double v[8];
+  for (int i0 = lane; i0 < n; i0 += 64 * B) {
+    double v[B];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = i0 + 64 * j < n ? part[i0 + 64 * j] : 0.0;
+    for (int j = 0; j < B; ++j) v[j] = i0 + 64 * j < n ? part[i0 + 64 * j] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) neumaier_add(s, c, v[j]);
+    for (int j = 0; j < B; ++j) neumaier_add(s, c, v[j]);
   }
   for (int off = 32; off > 0; off >>= 1) {
     const double os = __shfl_xor(s, off, 64), oc = __shfl_xor(c, off, 64);
@@ -1325,7 +1327,7 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
   const int ab = abort ? *abort : 0;
   const double d0_prev = sc->d0;
   const double dn_old = sc->dn[(it + 1) & 1];
-  const double dn = wave_sum_array(part_rs, n_rs);
+  const double dn = wave_sum_array<4>(part_rs, n_rs);
   if (done0) return false;
   if (ab) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1482,7 +1484,7 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
   for (int c = 0; c < BD; ++c) jr[c] = on ? static_cast<double>(J[static_cast<size_t>(i) * BD * BD + BD * k + c]) : 0.0;
   const int done0 = sc->done;
   const double dn = sc->dn[it & 1];
-  const double dq = wave_sum_array(part_dq, n_dq);
+  const double dq = wave_sum_array<24>(part_dq, n_dq);  // one batch up to 1536 rows
   if (done0) return;
   const double a = dn / dq;
   // d . q <= 0: the operator is not positive definite along d (reported in

@@ -25,7 +25,7 @@ namespace g2ohip {
 // the wait) and reports prog_failed. Without `prog` the host polls the
 // device scalars every 4 iterations. If the loop reaches max_iter the head
 // runs once more to record the final residual.
-constexpr int PCG_LAG = 3;
+constexpr int PCG_LAG = 2;
 constexpr double PCG_STALL_MS = 50.0;
 
 // Host loop shared by both arrangements: init() once, then step(it, last)
