@@ -97,7 +97,7 @@ int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_
 int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
   G2OHIP_ENTER(ctx);
   if (!ctx->have_structure || !info) return ctx->fail(G2OHIP_ERR_INVALID, "no structure");
-  int rc = ctx->ensure_win();
+  int rc = ctx->ensure_win(0);
   if (rc) return rc;
   const auto& st = ctx->st;
   int maxn = 0;
@@ -107,7 +107,7 @@ int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
   info[2] = st.ns;
   info[3] = st.nS;
   info[4] = st.nlc;
-  const auto& wp = st.win;
+  const auto& wp = st.win[0];
   info[5] = static_cast<long long>(wp.win_obs0.size()) - 1;
   info[6] = wp.pairs;
   info[7] = wp.od_ptr.back();

@@ -141,7 +141,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   have_structure = true;
   mode = 0;
   have_meas = have_state = have_system = have_solution = have_exact = false;
-  win_ready = false;
+  win_ready[0] = win_ready[1] = false;
   pcg_residual_mem = -1.0;
   stack_depth = 0;
   if (device < 0) return G2OHIP_OK;
@@ -216,15 +216,18 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   return G2OHIP_OK;
 }
 
-int g2ohip_ctx::ensure_win() {
-  if (win_ready) return G2OHIP_OK;
-  if (st.win.win_obs0.empty()) {
-    std::string msg = build_win_plan(st, SCHUR_WIN_OBS, SCHUR_WIN_PCH);
+// which = 0: the plan of the FP32 blocks, 1: of the FP64 blocks
+int g2ohip_ctx::ensure_win(int which) {
+  if (win_ready[which]) return G2OHIP_OK;
+  WinPlan& wp = st.win[which];
+  if (wp.win_obs0.empty()) {
+    const int obs = which == 0 || max_point_obs(st) > SCHUR_WIN_OBS64 ? SCHUR_WIN_OBS32 : SCHUR_WIN_OBS64;
+    std::string msg = build_win_plan(st, wp, obs, SCHUR_WIN_PCH);
     if (!msg.empty()) return fail(G2OHIP_ERR_UNSUPPORTED, msg);
   }
   if (device < 0) return G2OHIP_OK;
-  const WinPlan& wp = st.win;
-  WinPlanBufs& B = win_bufs;
+  WinPlanBufs& B = win_bufs[which];
+  DevWinPlan& dwin = dwins[which];
   HK(B.win_obs0.upload(wp.win_obs0, stream));
   HK(B.win_q.upload(wp.win_q, stream));
   HK(B.win_grp0.upload(wp.win_grp0, stream));
@@ -236,11 +239,12 @@ int g2ohip_ctx::ensure_win() {
   HK(B.pair_rows.upload(wp.pair_rows, stream));
   dwin.nwin = static_cast<int>(wp.win_obs0.size()) - 1;
   dwin.nod = static_cast<int>(wp.od_blk.size());
+  dwin.obs_max = wp.obs_max;
   dwin.win_obs0 = B.win_obs0.p; dwin.win_q = B.win_q.p; dwin.win_grp0 = B.win_grp0.p;
   dwin.grp_len = B.grp_len.p; dwin.grp_row0 = B.grp_row0.p; dwin.grp_slot = B.grp_slot.p;
   dwin.od_blk = B.od_blk.p; dwin.od_ptr = B.od_ptr.p; dwin.pair_rows = B.pair_rows.p;
   HK(hipStreamSynchronize(stream));
-  win_ready = true;
+  win_ready[which] = true;
   return G2OHIP_OK;
 }
 
@@ -433,7 +437,8 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
-  CK(ensure_win());
+  const int wpi = std::is_same<HT, float>::value ? 0 : 1;
+  if (!implicit) CK(ensure_win(wpi));
   {
     ProfScope ps(this, G2OHIP_K_SCHUR);
     if (implicit) {
@@ -441,14 +446,14 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
       HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
     } else {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
-      HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win.od_ptr.back(), 1))));
+      HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win[wpi].od_ptr.back(), 1))));
       HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
       HK(launch_point_linv(g, d_Hll.p, lambda, d_linv.p, d_err.p, stream));
       {
         ProfScope pr(this, G2OHIP_K_SCHUR_WINDOWS);
-        HK(launch_schur_win<HT>(g, dwin, B.Hpl.p, d_linv.p, B.part.p, stream));
+        HK(launch_schur_win<HT>(g, dwins[wpi], B.Hpl.p, d_linv.p, B.part.p, stream));
       }
-      HK(launch_schur_offdiag<HT>(g, dwin, B.part.p, B.F_val.p, stream));
+      HK(launch_schur_offdiag<HT>(g, dwins[wpi], B.part.p, B.F_val.p, stream));
     }
     // diagonal blocks, block-Jacobi inverses and (explicit) the reduced
     // right-hand side in FP64, re-linearised from the state point-major (FP32
@@ -864,15 +869,15 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
   *iters = 0;
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
-  CK(ensure_win());
+  CK(ensure_win(1));
   HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
   HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
-  HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win.od_ptr.back(), 1))));
+  HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win[1].od_ptr.back(), 1))));
   HK(launch_point_dinv<double>(g, d_Hll.p, lam, B.Dinv.p, d_err.p, stream));
   HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
   HK(launch_point_linv(g, d_Hll.p, lam, d_linv.p, d_err.p, stream));
-  HK(launch_schur_win<double>(g, dwin, B.Hpl.p, d_linv.p, B.part.p, stream));
-  HK(launch_schur_offdiag<double>(g, dwin, B.part.p, B.F_val.p, stream));
+  HK(launch_schur_win<double>(g, dwins[1], B.Hpl.p, d_linv.p, B.part.p, stream));
+  HK(launch_schur_offdiag<double>(g, dwins[1], B.part.p, B.F_val.p, stream));
   HK(launch_schur_diag<double>(g, d_q_slot.p, d_camRt.p, d_pts.p, B.Hpl.p, d_Hll.p, d_Hpp.p, d_b.p, lam, B.F_val.p,
                                B.Jinv.p, d_bschur.p, d_err.p, stream));
   int err = 0;

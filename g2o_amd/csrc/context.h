@@ -100,9 +100,9 @@ struct g2ohip_ctx {
       d_lc_cam_ptr, d_lc_cams, d_camp_ptr, d_camp_idx, d_S_row, d_S_col, d_S_diag, d_up_pos, d_up_posT, d_F_ptr,
       d_F_col, d_F_diag, d_iota;
   g2ohip::DBuf<double> d_meas, d_w, d_intr;
-  g2ohip::WinPlanBufs win_bufs;
-  g2ohip::DevWinPlan dwin{};
-  bool win_ready = false;
+  g2ohip::WinPlanBufs win_bufs[2];
+  g2ohip::DevWinPlan dwins[2]{};
+  bool win_ready[2] = {false, false};
 
   g2ohip::DBuf<double> d_cams, d_pts, d_camRt;
   std::vector<g2ohip::DBuf<double>*> stack_c, stack_p;
@@ -155,7 +155,7 @@ struct g2ohip_ctx {
   int set_measurements(const double* meas, const double* w, const double* intr, double huber);
   int set_state(const double* cams, const double* pts);
   int get_state(double* cams, double* pts);
-  int ensure_win();
+  int ensure_win(int which);
   int compute_active_errors(double* chi2, double* err_out = nullptr);
   int build_system(bool read_chi = true);
   int solve(int* ok, bool defer_err = false);

@@ -29,7 +29,7 @@ struct DevGraph {
 // Device view of the window plan of the off-diagonal Schur blocks
 // (structure.h, WinPlan).
 struct DevWinPlan {
-  int nwin, nod;
+  int nwin, nod, obs_max;
   const int *win_obs0, *win_q, *win_grp0, *grp_len, *grp_row0, *grp_slot, *od_blk, *od_ptr;
   const unsigned* pair_rows;
 };
@@ -86,7 +86,11 @@ hipError_t launch_scale(int n, const double* x, const double* b, double lambda, 
                         hipStream_t st);
 
 // ---- explicit Schur complement
-constexpr int SCHUR_WIN_OBS = 512;  // observations per window (LDS-staged G blocks)
+// observations per window (LDS-staged G blocks): FP32 blocks 512 (40 KiB of
+// LDS); FP64 blocks 256 (36 KiB, four workgroups per CU) unless a point has
+// more, then 512
+constexpr int SCHUR_WIN_OBS32 = 512;
+constexpr int SCHUR_WIN_OBS64 = 256;
 constexpr int SCHUR_WIN_PCH = 32;   // pair products per partial
 // L^-1 of L L^T = Hll + lambda I per free point (6 doubles, lower triangle
 // row by row); err |= 1 if not positive definite.

@@ -474,17 +474,17 @@ hipError_t launch_point_linv(const DevGraph& g, const double* Hll, double lambda
   return hipGetLastError();
 }
 
-template <typename HT>
+template <typename HT, int OBS>
 __global__ __launch_bounds__(256) void k_schur_win(DevGraph g, DevWinPlan wp, const HT* __restrict__ Hpl,
                                                   const double* __restrict__ Linv, HT* __restrict__ part) {
   constexpr int GS = GStride<HT>::v;
   using CT = typename Chunk<HT>::T;
   constexpr int NCH = Chunk<HT>::per_block;
-  __shared__ __align__(16) HT s_G[SCHUR_WIN_OBS * GS];
+  __shared__ __align__(16) HT s_G[OBS * GS];
   // q of each window observation: in the padding of the FP32 rows (keeps the
   // FP32 footprint at 40 KiB, four workgroups per CU), else its own array
   constexpr bool QPAD = GS * sizeof(HT) >= 18 * sizeof(HT) + sizeof(int);
-  __shared__ int s_qa[QPAD ? 1 : SCHUR_WIN_OBS];
+  __shared__ int s_qa[QPAD ? 1 : OBS];
   auto s_q = [&](int l) -> int& {
     if constexpr (QPAD) return *reinterpret_cast<int*>(s_G + l * GS + 18);
     else return s_qa[l];
@@ -577,7 +577,12 @@ template <typename HT>
 hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* Hpl, const double* Linv, HT* part,
                             hipStream_t st) {
   if (wp.nwin == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_schur_win<HT>, dim3(wp.nwin), dim3(256), 0, st, g, wp, Hpl, Linv, part);
+  if (wp.obs_max <= 256)
+    hipLaunchKernelGGL((k_schur_win<HT, 256>), dim3(wp.nwin), dim3(256), 0, st, g, wp, Hpl, Linv, part);
+  else if (wp.obs_max <= 512)
+    hipLaunchKernelGGL((k_schur_win<HT, 512>), dim3(wp.nwin), dim3(256), 0, st, g, wp, Hpl, Linv, part);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 template <typename HT>

@@ -241,8 +241,14 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   return "";
 }
 
-std::string build_win_plan(BAStructure& st, int obs_max, int pch) {
-  WinPlan& wp = st.win;
+int max_point_obs(const BAStructure& st) {
+  int m = 0;
+  for (int p = 0; p < st.npos; ++p)
+    if (st.pos_h[p] >= 0) m = std::max(m, st.pos_nfree[p]);
+  return m;
+}
+
+std::string build_win_plan(const BAStructure& st, WinPlan& wp, int obs_max, int pch) {
   wp = WinPlan();
   if (obs_max < 1 || obs_max > 65535 || pch < 1) return "invalid window plan parameters";
   wp.obs_max = obs_max;

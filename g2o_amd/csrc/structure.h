@@ -97,7 +97,7 @@ struct BAStructure {
   int nF = 0;
   std::vector<int> F_ptr, F_col, F_up, F_trans;
   std::vector<int> up_pos, up_posT;  // upper block -> F rows (posT = -1 on diag)
-  WinPlan win;
+  WinPlan win[2];  // Schur window plans: [0] FP32 blocks, [1] FP64 blocks
 };
 
 // Builds everything except the Schur plans. Returns an empty string on
@@ -105,9 +105,11 @@ struct BAStructure {
 std::string build_structure(BAStructure& st, int C, int P, int E, const int* edge_cam, const int* edge_pt,
                             const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur);
 
-// Builds st.win (windows of at most obs_max observations, partials of at most
-// pch pairs; obs_max <= 65535).
-std::string build_win_plan(BAStructure& st, int obs_max, int pch);
+// Builds a Schur window plan (windows of at most obs_max observations,
+// partials of at most pch pairs; obs_max <= 65535).
+std::string build_win_plan(const BAStructure& st, WinPlan& wp, int obs_max, int pch);
+// Largest number of observations with a free camera of one free point.
+int max_point_obs(const BAStructure& st);
 
 // 3D pose graph (VertexSE3 / EdgeSE3, 6x6 blocks, no Schur): free poses get
 // indices 0..Nf-1 in input order; the Hessian pattern is the diagonal plus one
