@@ -54,7 +54,7 @@ def algorithmic_bytes(kernel, info, prec_bytes, lin_cam_rows):
     return None
 
 
-KERNEL_SYMBOL = {"schur_windows": "k_schur_win<float>", "pcg_spmv": "k_pcg_dir_spmv<float, 6>",
+KERNEL_SYMBOL = {"schur_windows": "k_schur_win<float, 512>", "pcg_spmv": "k_pcg_dir_spmv<float, 6, false>",
                  "linearize": "k_linearize<float>", "refine": "k_refine", "chi2": "k_chi2"}
 
 
