@@ -160,6 +160,15 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_q_ph.upload(st.q_ph, stream));
   HK(d_q_slot.upload(st.q_slot, stream));
   HK(d_q_pt.upload(st.q_pt, stream));
+  {
+    std::vector<int4> rec(st.ns);
+    for (int s = 0; s < st.ns; ++s) {
+      const int pos = st.slot_pos[s], cam = st.slot_cam[s];
+      rec[s] = make_int4(pos, st.pos_pt[pos], cam, st.camH[cam]);
+    }
+    HK(d_slot_rec.upload(rec, stream));
+    HK(hipStreamSynchronize(stream));  // the host vectors go out of scope
+  }
   HK(d_cam_raw.upload(st.free_cam_raw, stream));
   HK(d_lc_pos0.upload(st.lc_pos0, stream));
   HK(d_lc_cam_ptr.upload(st.lc_cam_ptr, stream));
@@ -208,6 +217,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   g.slot_cam = d_slot_cam.p; g.slot_pos = d_slot_pos.p; g.slot_camh = d_slot_camh.p; g.slot_ph = d_slot_ph.p;
   g.nq = st.nq; g.slot_q = d_slot_q.p; g.cam_qptr = d_cam_qptr.p; g.q_ph = d_q_ph.p;
   g.q_pt = d_q_pt.p; g.cam_raw = d_cam_raw.p;
+  g.slot_rec = d_slot_rec.p;
   g.lc_pos0 = d_lc_pos0.p; g.lc_cam_ptr = d_lc_cam_ptr.p; g.lc_cams = d_lc_cams.p;
   g.camp_ptr = d_camp_ptr.p; g.camp_idx = d_camp_idx.p;
   g.S_row = d_S_row.p; g.S_col = d_S_col.p; g.S_diag = d_S_diag.p; g.up_pos = d_up_pos.p; g.up_posT = d_up_posT.p;

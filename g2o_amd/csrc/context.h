@@ -99,6 +99,7 @@ struct g2ohip_ctx {
   g2ohip::DBuf<int> d_camH, d_ptH, d_pos_pt, d_pos_h, d_pos_ptr, d_pos_nfree, d_slot_cam, d_slot_pos, d_slot_camh, d_slot_ph, d_slot_q, d_cam_qptr, d_q_ph, d_q_slot, d_q_pt, d_cam_raw, d_lc_pos0,
       d_lc_cam_ptr, d_lc_cams, d_camp_ptr, d_camp_idx, d_S_row, d_S_col, d_S_diag, d_up_pos, d_up_posT, d_F_ptr,
       d_F_col, d_F_diag, d_iota;
+  g2ohip::DBuf<int4> d_slot_rec;
   g2ohip::DBuf<double> d_meas, d_w, d_intr;
   g2ohip::WinPlanBufs win_bufs[2];
   g2ohip::DevWinPlan dwins[2]{};

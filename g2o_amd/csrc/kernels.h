@@ -20,6 +20,9 @@ struct DevGraph {
   int nq;                                    // observations with a free camera and a free point
   const int *slot_q, *cam_qptr, *q_ph;       // camera-major order of Hpl
   const int *q_pt, *cam_raw;                 // raw point per q, raw camera per free camera
+  // per slot in one load: {position, raw point, raw camera, free camera}
+  // (the indices above, without their dependent lookups; used by k_chi2)
+  const int4* slot_rec;
   const double *slot_meas, *slot_w;
   const int *lc_pos0, *lc_cam_ptr, *lc_cams, *camp_ptr, *camp_idx;
   const int *S_row, *S_col, *S_diag, *up_pos, *up_posT, *F_ptr, *F_col, *F_diag;

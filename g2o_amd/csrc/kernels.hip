@@ -116,8 +116,8 @@ __global__ __launch_bounds__(256) void k_chi2(DevGraph g, const double* __restri
   __shared__ double s_red[4];
   double chi = 0;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.ns; s += gridDim.x * blockDim.x) {
-    const int cam = g.slot_cam[s];
-    const int praw = g.pos_pt[g.slot_pos[s]];
+    const int4 rec = g.slot_rec[s];
+    const int cam = rec.z, praw = rec.y;
     double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
     double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
     double Xc[3], e[2];
