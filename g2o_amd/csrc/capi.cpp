@@ -37,14 +37,17 @@ int g2ohip_create(int device, g2ohip_ctx** out) {
       return G2OHIP_ERR_HIP;
     }
     c->device = device;
-    // host-mapped progress word of the PCG (optional: without it the PCG
-    // polls the stream)
+    // host-mapped PCG progress record and LM trial scalars (optional:
+    // without them the host copies and synchronises the stream)
+    static_assert(sizeof(g2ohip::PcgProgress) <= 128 && sizeof(g2ohip::TrialScalars) <= 128, "mapped layout");
     void* hp = nullptr;
     void* dp = nullptr;
-    if (hipHostMalloc(&hp, 64, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
+    if (hipHostMalloc(&hp, 256, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
       if (hipHostGetDevicePointer(&dp, hp, 0) == hipSuccess && dp) {
         c->h_prog = static_cast<g2ohip::PcgProgress*>(hp);
-        c->d_prog = static_cast<int*>(dp);
+        c->d_prog = static_cast<g2ohip::PcgProgress*>(dp);
+        c->h_trial = reinterpret_cast<g2ohip::TrialScalars*>(static_cast<char*>(hp) + 128);
+        c->d_trial = reinterpret_cast<g2ohip::TrialScalars*>(static_cast<char*>(dp) + 128);
       } else {
         (void)hipHostFree(hp);
       }

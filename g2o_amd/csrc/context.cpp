@@ -69,8 +69,33 @@ int g2ohip_ctx::read_err(int* out) {
 }
 
 // d_scal[0..3] (chi2, linearisation chi2, max diagonal, scale) and the error
-// word in one round trip.
+// word in one round trip: published by a kernel into host-mapped memory
+// (the host spins on the sequence number, no stream synchronisation), or
+// copied after a synchronisation if that is unavailable or stalls.
 int g2ohip_ctx::read_trial_scalars(double* sc4, int* err) {
+  if (h_trial && prog_ok) {
+    const int seq = ++trial_seq;
+    HK(launch_publish_scalars(d_scal.p, d_err.p, d_trial, seq, stream));
+    volatile TrialScalars* t = h_trial;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool have = false;
+    for (int k = 0;; ++k) {
+      if (t->seq == seq) { have = true; break; }
+      if ((k & 255) == 255 &&
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() > 200.0)
+        break;
+    }
+    if (!have) {
+      HK(hipStreamSynchronize(stream));
+      have = t->seq == seq;
+    }
+    if (have) {
+      for (int k = 0; k < 4; ++k) sc4[k] = t->v[k];
+      *err = t->err;
+      return G2OHIP_OK;
+    }
+    prog_ok = false;  // the mapped memory does not reach the host
+  }
   char* hp = reinterpret_cast<char*>(h_pinned);
   HK(hipMemcpyAsync(hp, d_scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
   HK(hipMemcpyAsync(hp + 64, d_err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -444,8 +469,7 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   // the FP64 matrix-free passes below re-linearise at the current state: the
   // camera table must follow it (a rejected LM trial is popped after its chi2
   // pass left the table at the trial state)
-  HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream));
-  HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
+  HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream, d_err.p));  // and d_err[0..3] = 0
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
   const int wpi = std::is_same<HT, float>::value ? 0 : 1;
   if (!implicit) CK(ensure_win(wpi));
@@ -704,7 +728,6 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
       }
       q++;
     } while (rho < 0 && q < opt.max_trials && !(force_stop && *force_stop));
-    HK(hipStreamSynchronize(stream));
     s.chi2_before = currentChi0;
     s.chi2 = currentChi;
     s.lambda = lambda;
@@ -716,6 +739,8 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
     have_system = false;
     if (s.status) break;
   }
+  // the last pop / copies and any asynchronous error
+  HK(hipStreamSynchronize(stream));
   return G2OHIP_OK;
 }
 
@@ -848,6 +873,7 @@ int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, cons
   if (bd == 3) CK(run(std::integral_constant<int, 3>()));
   else CK(run(std::integral_constant<int, 6>()));
   int err = 0;
+  HK(hipStreamSynchronize(stream));
   HK(hipMemcpy(&err, derr.p, sizeof(int), hipMemcpyDeviceToHost));
   if (err) return fail(G2OHIP_ERR_NOT_PD, "singular diagonal block");
   HK(hipMemcpy(x, dx.p, n * sizeof(double), hipMemcpyDeviceToHost));

@@ -114,11 +114,16 @@ struct g2ohip_ctx {
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;
   g2ohip::DBuf<int> d_err;
-  // host-mapped PCG progress word (PcgControl::prog); prog_ok drops to false
-  // if it ever fails to reach the host, and the PCG then polls the stream
+  // host-mapped PCG progress record (PcgControl::prog); prog_ok drops to
+  // false if it ever fails to reach the host, and the PCG then polls the
+  // stream. h_trial / d_trial: the LM trial scalars (launch_publish_scalars),
+  // tagged with trial_seq.
   g2ohip::PcgProgress* h_prog = nullptr;
-  int* d_prog = nullptr;
+  g2ohip::PcgProgress* d_prog = nullptr;
   bool prog_ok = true;
+  g2ohip::TrialScalars* h_trial = nullptr;
+  g2ohip::TrialScalars* d_trial = nullptr;
+  int trial_seq = 0;
   volatile g2ohip::PcgProgress* progress() const { return prog_ok && h_prog ? h_prog : nullptr; }
   g2ohip::PcgControl with_progress(g2ohip::PcgControl c) const {
     c.prog = prog_ok && h_prog ? d_prog : nullptr;

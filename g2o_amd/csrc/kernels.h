@@ -49,10 +49,25 @@ struct PcgScalars {
   int pad;
 };
 
+// Host-mapped record of one solve: the iteration heads store the iteration
+// reached in `it`; the head that ends the solve stores the final values and
+// then `done` = 1 with release semantics at system scope, so a host that
+// reads done == 1 (acquire) reads the final values without a copy.
+struct PcgProgress {
+  int it;
+  int done;
+  int iters;   // PcgScalars::iters
+  int err;     // PcgScalars::err
+  int abort;   // the abort word seen by the final head
+  int pad;
+  double dn;   // r.M^-1 r at exit
+  double d0;
+  double dn0;
+};
+
 // Stop rule of one solve: d0 = tol * dn0 floored by resid_mem (the previous
 // solve's residual memory, ignored when <= 0), or exactly d0_abs when >= 0.
-// prog (device address of host-mapped memory, or null): the iteration heads
-// store the iteration reached in prog[0] and 1 in prog[1] once done.
+// prog (device address of a host-mapped PcgProgress, or null).
 // abort (device address, or null): a nonzero word stops the solve at its
 // next iteration head (an earlier kernel of the same solve reported a
 // failure).
@@ -60,18 +75,24 @@ struct PcgControl {
   double tol;
   double resid_mem;
   double d0_abs;
-  int* prog = nullptr;
+  PcgProgress* prog = nullptr;
   const int* abort = nullptr;
 };
 
-// Host view of PcgControl::prog.
-struct PcgProgress {
-  int it;
-  int done;
+// Host-mapped copy of the LM trial's scalars (launch_publish_scalars):
+// v = d_scal[0..3] (chi2, linearisation chi2, max diagonal, scale), err the
+// error word; seq stored last (release, system scope).
+struct TrialScalars {
+  double v[4];
+  int err;
+  int seq;
 };
+hipError_t launch_publish_scalars(const double* scal, const int* err, TrialScalars* out, int seq, hipStream_t st);
 
 // ---- state / residuals
-hipError_t launch_cam_table(const DevGraph& g, const double* cams7, double* camRt, hipStream_t st);
+// err4 (optional): four error words zeroed by the same launch.
+hipError_t launch_cam_table(const DevGraph& g, const double* cams7, double* camRt, hipStream_t st,
+                            int* err4 = nullptr);
 hipError_t launch_chi2(const DevGraph& g, const double* camRt, const double* pts, double* partial, double* out,
                        double* err_out, hipStream_t st);
 hipError_t launch_update(const DevGraph& g, const double* dx, double* cams7, double* pts, hipStream_t st);
@@ -158,7 +179,7 @@ int pcg_spmv_blocks(int nb);
 int implicit_reduce_blocks(int Cf);
 template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
-                           double* part_rs, hipStream_t st);
+                           double* part_rs, PcgScalars* sc, hipStream_t st);
 template <typename MT, int BD>
 hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,

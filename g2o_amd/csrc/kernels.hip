@@ -91,8 +91,10 @@ inline int nblocks(long long n, int t) { return static_cast<int>((n + t - 1) / t
 }  // namespace
 
 // =================================================================== state
-__global__ void k_cam_table(int C, const double* __restrict__ cams7, double* __restrict__ camRt) {
+__global__ void k_cam_table(int C, const double* __restrict__ cams7, double* __restrict__ camRt,
+                            int* __restrict__ err4) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (err4 && c < 4) err4[c] = 0;
   if (c >= C) return;
   const double* p = cams7 + 7 * c;
   double q[4] = {p[3], p[4], p[5], p[6]};
@@ -103,8 +105,8 @@ __global__ void k_cam_table(int C, const double* __restrict__ cams7, double* __r
   o[9] = p[0]; o[10] = p[1]; o[11] = p[2];
 }
 
-hipError_t launch_cam_table(const DevGraph& g, const double* cams7, double* camRt, hipStream_t st) {
-  hipLaunchKernelGGL(k_cam_table, dim3(nblocks(g.C, 256)), dim3(256), 0, st, g.C, cams7, camRt);
+hipError_t launch_cam_table(const DevGraph& g, const double* cams7, double* camRt, hipStream_t st, int* err4) {
+  hipLaunchKernelGGL(k_cam_table, dim3(nblocks(g.C > 4 ? g.C : 4, 256)), dim3(256), 0, st, g.C, cams7, camRt, err4);
   return hipGetLastError();
 }
 
@@ -1286,13 +1288,40 @@ constexpr int PCG_RPW = 8;  // block rows per 64-lane workgroup
 __device__ __forceinline__ void progress_store(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+// The final values, then done = 1 (release: the values are visible first).
+__device__ __forceinline__ void progress_done(PcgProgress* p, int iters, int err, int ab, double dn, double d0,
+                                              double dn0) {
+  p->iters = iters;
+  p->err = err;
+  p->abort = ab;
+  p->dn = dn;
+  p->d0 = d0;
+  p->dn0 = dn0;
+  __hip_atomic_store(&p->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
-// x = 0, r = b, s = M^-1 r, part[block] = r . s   (linear_solver_pcg.hpp:120-130)
+__global__ void k_publish_scalars(const double* __restrict__ scal, const int* __restrict__ err,
+                                  TrialScalars* __restrict__ out, int seq) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < 4; ++k) out->v[k] = scal[k];
+  out->err = *err;
+  __hip_atomic_store(&out->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_publish_scalars(const double* scal, const int* err, TrialScalars* out, int seq, hipStream_t st) {
+  hipLaunchKernelGGL(k_publish_scalars, dim3(1), dim3(64), 0, st, scal, err, out, seq);
+  return hipGetLastError();
+}
+
+// x = 0, r = b, s = M^-1 r, part[block] = r . s   (linear_solver_pcg.hpp:120-130);
+// workgroup 0 also resets the solve's scalars.
 template <typename MT, int BD>
 __global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restrict__ b, const MT* __restrict__ J,
                                                  double* __restrict__ x, double* __restrict__ r,
-                                                 double* __restrict__ s, double* __restrict__ part) {
+                                                 double* __restrict__ s, double* __restrict__ part,
+                                                 PcgScalars* __restrict__ sc) {
   const int lane = threadIdx.x;
+  if (blockIdx.x == 0 && lane == 0) *sc = PcgScalars{};
   const int i = blockIdx.x * PCG_RPW + (lane >> 3), k = lane & 7;
   const bool on = i < nb && k < BD;
   const size_t o = BD * static_cast<size_t>(i) + k;
@@ -1323,14 +1352,15 @@ __global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restric
 // beta = dn / dn_old. Returns false (and records the iteration count) when
 // the solve is over. Every workgroup evaluates this identically; workgroup 0
 // publishes the progress (iteration reached, done) to host-mapped memory.
-__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, int* prog,
+__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, PcgProgress* prog,
                          const int* abort, double tol, double resid_mem, double d0_abs, int it, int last,
                          double* beta) {
   // every load of the head is issued before the first branch: the scalars,
   // the abort word and the partials are one dependency level, not four
   const int done0 = sc->done;
   const int ab = abort ? *abort : 0;
-  const double d0_prev = sc->d0;
+  const int err0 = sc->err;
+  const double d0_prev = sc->d0, dn0_prev = sc->dn0;
   const double dn_old = sc->dn[(it + 1) & 1];
   const double dn = wave_sum_array<4>(part_rs, n_rs);
   if (done0) return false;
@@ -1338,7 +1368,7 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->done = 1;
       sc->iters = it;
-      if (prog) progress_store(prog + 1, 1);
+      if (prog) progress_done(prog, it, err0, ab, dn, d0_prev, dn0_prev);
     }
     return false;
   }
@@ -1363,11 +1393,12 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
     if (first) {
       sc->done = 1;
       sc->iters = it;
-      if (prog) progress_store(prog + 1, 1);
+      if (prog)
+        progress_done(prog, it, err0 | (dn >= 0.0 ? 0 : 1), 0, dn, d0, it == 0 ? dn : dn0_prev);
     }
     return false;
   }
-  if (first && prog) progress_store(prog, it);
+  if (first && prog) progress_store(&prog->it, it);
   *beta = it == 0 ? 0.0 : dn / dn_old;
   return true;
 }
@@ -1389,7 +1420,7 @@ template <typename MT, int BD, bool FIRST>
 __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
     int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
     const double* __restrict__ s, const double* __restrict__ d_old, double* __restrict__ d, double* __restrict__ q,
-    const double* __restrict__ part_rs, int n_rs, double* __restrict__ part_dq, PcgScalars* sc, int* prog,
+    const double* __restrict__ part_rs, int n_rs, double* __restrict__ part_dq, PcgScalars* sc, PcgProgress* prog,
     const int* abort, double tol, double resid_mem, double d0_abs, int it, int last) {
   constexpr int GPW = 64 / BD;  // groups per wavefront
   constexpr int NG = PCG_SPMV_WAVES * GPW;
@@ -1457,7 +1488,7 @@ template <int BD>
 __global__ __launch_bounds__(64) void k_pcg_dir(int nb, const double* __restrict__ s,
                                                 const double* __restrict__ d_old, double* __restrict__ d,
                                                 const double* __restrict__ part_rs, int n_rs, PcgScalars* sc,
-                                                int* prog, const int* abort, double tol, double resid_mem,
+                                                PcgProgress* prog, const int* abort, double tol, double resid_mem,
                                                 double d0_abs, int it, int last) {
   double beta;
   if (!pcg_head(part_rs, n_rs, sc, prog, abort, tol, resid_mem, d0_abs, it, last, &beta)) return;
@@ -1520,8 +1551,9 @@ int implicit_reduce_blocks(int Cf) { return nblocks(Cf, 64); }
 
 template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
-                           double* part_rs, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs);
+                           double* part_rs, PcgScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs,
+                     sc);
   return hipGetLastError();
 }
 
@@ -1562,7 +1594,7 @@ hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const
 #define G2OHIP_PCG_INST(MT, BD)                                                                                    \
   template hipError_t launch_jacobi<MT, BD>(int, const MT*, const int*, MT*, int*, hipStream_t);                  \
   template hipError_t launch_pcg_init<MT, BD>(int, const double*, const MT*, double*, double*, double*, double*, \
-                                              hipStream_t);                                                      \
+                                              PcgScalars*, hipStream_t);                                         \
   template hipError_t launch_pcg_dir_spmv<MT, BD>(int, const int*, const int*, const MT*, const double*,         \
                                                   const double*, double*, double*, const double*, double*,       \
                                                   PcgScalars*, const PcgControl&, int, int, hipStream_t);        \

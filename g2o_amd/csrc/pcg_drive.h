@@ -2,6 +2,7 @@
 // pose-graph solvers).
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 
@@ -13,18 +14,19 @@
 namespace g2ohip {
 
 // One LinearSolverPCG::solve (linear_solver_pcg.hpp:112-155) on the device:
-// init, then `op` (direction + operator) and the update kernel per
-// iteration. The stop test runs on the device and later launches return at
-// once. With `prog` (host-mapped PcgProgress, written by the iteration
-// heads) the host never waits on the stream inside the loop: it launches
-// until it is PCG_LAG iterations ahead of the device's progress, then spins
-// on the progress word, and stops launching once the device reports done -
-// so the queue does not drain while the host looks, and at most PCG_LAG
-// iterations are launched past the end. If the word does not advance for
-// PCG_STALL_MS the host synchronises the stream instead (which always ends
-// the wait) and reports prog_failed. Without `prog` the host polls the
-// device scalars every 4 iterations. If the loop reaches max_iter the head
-// runs once more to record the final residual.
+// init (which also resets the scalars), then `op` (direction + operator)
+// and the update kernel per iteration. The stop test runs on the device and
+// later launches return at once. With `prog` (host-mapped PcgProgress,
+// written by the iteration heads) the host never waits on the stream: it
+// launches until it is PCG_LAG iterations ahead of the device's progress,
+// then spins on the progress word, stops launching once the device reports
+// done - so the queue does not drain while the host looks, and at most
+// PCG_LAG iterations are launched past the end - and reads the final values
+// from the same record. If the word does not advance for PCG_STALL_MS the
+// host synchronises the stream instead (which always ends the wait) and
+// reports prog_failed. Without `prog` the host polls the device scalars
+// every 4 iterations. If the loop reaches max_iter the head runs once more
+// to record the final residual.
 constexpr int PCG_LAG = 2;
 constexpr double PCG_STALL_MS = 50.0;
 
@@ -49,11 +51,29 @@ inline hipError_t pcg_loop(hipStream_t stream, void* h_pinned, volatile PcgProgr
     }
     return e2;
   };
+  // spin until the device reports done; false if the record stalled (then
+  // the stream has been synchronised and hs holds the scalars)
+  auto wait_done = [&](bool* stalled) -> hipError_t {
+    *stalled = false;
+    int seen = prog->it;
+    auto last_move = std::chrono::steady_clock::now();
+    for (int k = 0; !prog->done; ++k) {
+      if ((k & 63) != 63) continue;
+      const auto now = std::chrono::steady_clock::now();
+      if (prog->it != seen) {
+        seen = prog->it;
+        last_move = now;
+      } else if (std::chrono::duration<double, std::milli>(now - last_move).count() > PCG_STALL_MS) {
+        *stalled = true;
+        return poll();
+      }
+    }
+    return hipSuccess;
+  };
   if (prog) {
     prog->it = -1;
     prog->done = 0;
   }
-  if ((e = hipMemsetAsync(sc, 0, sizeof(PcgScalars), stream)) != hipSuccess) return e;
   if ((e = init()) != hipSuccess) return e;
   int next = 4;
   bool done = false;
@@ -92,6 +112,24 @@ inline hipError_t pcg_loop(hipStream_t stream, void* h_pinned, volatile PcgProgr
   if (!done) {
     if ((e = step(max_iter, 1)) != hipSuccess) return e;
   }
+  if (!have_final && prog) {
+    bool stalled = false;
+    if ((e = wait_done(&stalled)) != hipSuccess) return e;
+    if (stalled) {
+      if (prog_failed) *prog_failed = true;
+      have_final = true;
+    } else {
+      // acquire: the record's final values were stored before done
+      std::atomic_thread_fence(std::memory_order_acquire);
+      out->iters = prog->iters;
+      out->dn_final = prog->dn;
+      out->d0 = prog->d0;
+      out->dn0 = prog->dn0;
+      out->breakdown = prog->err != 0;
+      out->err_word = prog->abort;
+      return hipSuccess;
+    }
+  }
   if (!have_final && (e = poll()) != hipSuccess) return e;
   out->iters = hs.iters;
   out->dn_final = hs.dn[hs.iters & 1];
@@ -108,7 +146,7 @@ inline hipError_t pcg_drive(hipStream_t stream, void* h_pinned, volatile PcgProg
                             const double* rhs, const MT* J, double* x, double* r, double* s, double* const dirs[2],
                             double* q, double* part_rs, double* part_dq, int n_dq, PcgScalars* sc, int max_iter,
                             OpFn op, PcgRun* out, bool* prog_failed, const int* abort = nullptr) {
-  auto init = [&]() { return launch_pcg_init<MT, BD>(nb, rhs, J, x, r, s, part_rs, stream); };
+  auto init = [&]() { return launch_pcg_init<MT, BD>(nb, rhs, J, x, r, s, part_rs, sc, stream); };
   auto step = [&](int it, int last) -> hipError_t {
     hipError_t e = op(it, last, dirs[(it + 1) & 1], dirs[it & 1]);
     if (e != hipSuccess || last) return e;
