@@ -1252,9 +1252,13 @@ __device__ double wave_sum_array(const double* __restrict__ part, int n) {
   const int lane = threadIdx.x & 63;
   double s = 0, c = 0;
   for (int i0 = lane; i0 < n; i0 += 64 * B) {
+    // loads from clamped indices, then a select: a load under a lane
+    // condition becomes a branch that waits for it before the next one
     double v[B];
 #pragma unroll
-    for (int j = 0; j < B; ++j) v[j] = i0 + 64 * j < n ? part[i0 + 64 * j] : 0.0;
+    for (int j = 0; j < B; ++j) v[j] = part[min(i0 + 64 * j, n - 1)];
+#pragma unroll
+    for (int j = 0; j < B; ++j) v[j] = i0 + 64 * j < n ? v[j] : 0.0;
 #pragma unroll
     for (int j = 0; j < B; ++j) neumaier_add(s, c, v[j]);
   }
@@ -1324,19 +1328,21 @@ __global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restric
   if (blockIdx.x == 0 && lane == 0) *sc = PcgScalars{};
   const int i = blockIdx.x * PCG_RPW + (lane >> 3), k = lane & 7;
   const bool on = i < nb && k < BD;
-  const size_t o = BD * static_cast<size_t>(i) + k;
-  double rv = 0.0;
+  // loads from clamped indices (all in flight together), values selected
+  const int ic = on ? i : 0, kc = on ? k : 0;
+  const size_t o = BD * static_cast<size_t>(ic) + kc;
+  double jr[BD];
+#pragma unroll
+  for (int c = 0; c < BD; ++c) jr[c] = static_cast<double>(J[static_cast<size_t>(ic) * BD * BD + BD * kc + c]);
+  const double bv = b[o];
+  const double rv = on ? bv : 0.0;
   if (on) {
-    rv = b[o];
     r[o] = rv;
     x[o] = 0.0;
   }
   double v = 0.0;
 #pragma unroll
-  for (int c = 0; c < BD; ++c) {
-    const double rc = __shfl(rv, (lane & ~7) + c, 64);
-    if (on) v += static_cast<double>(J[static_cast<size_t>(i) * BD * BD + BD * k + c]) * rc;
-  }
+  for (int c = 0; c < BD; ++c) v = fma(jr[c], __shfl(rv, (lane & ~7) + c, 64), v);
   double dot = 0.0;
   if (on) {
     s[o] = v;
@@ -1437,11 +1443,16 @@ __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
   for (int h = 0; h < H; ++h) {
     const int p = p0 + grp + h * NG;
     const bool on = gw < GPW && p < p1;
-    const size_t j = on ? BD * static_cast<size_t>(F_col[p]) : 0;
-    const MT* V = F_val + static_cast<size_t>(on ? p : 0) * BD * BD + BD * k;
+    // every load unconditional from a clamped (valid) index, values
+    // selected afterwards, so all of them are in flight together
+    const int pc = on ? p : p0;
+    const int col = F_col[pc];
+    const size_t j = on ? BD * static_cast<size_t>(col) : 0;
+    const MT* V = F_val + static_cast<size_t>(pc) * BD * BD + BD * k;
 #pragma unroll
     for (int c = 0; c < BD; ++c) {
-      v[h][c] = on ? V[c] : MT(0);
+      const MT vv = V[c];
+      v[h][c] = on ? vv : MT(0);
       sv[h][c] = s[j + c];
       if (!FIRST) dv[h][c] = d_old[j + c];
     }
@@ -1510,14 +1521,14 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
   const int lane = threadIdx.x;
   const int i = blockIdx.x * PCG_RPW + (lane >> 3), k = lane & 7;
   const bool on = i < nb && k < BD;
-  const size_t o = BD * static_cast<size_t>(i) + k;
-  // operands first (independent of alpha), then the reduction of d . q
-  double dv = 0.0, qv = 0.0, rv0 = 0.0, xv = 0.0, jr[BD];
-  if (on) {
-    dv = d[o]; qv = q[o]; rv0 = r[o]; xv = x[o];
-  }
+  // operands first (independent of alpha), then the reduction of d . q;
+  // loads from clamped indices so they are all in flight together
+  const int ic = on ? i : 0, kc = on ? k : 0;
+  const size_t o = BD * static_cast<size_t>(ic) + kc;
+  const double dv = d[o], qv = q[o], rv0 = r[o], xv = x[o];
+  double jr[BD];
 #pragma unroll
-  for (int c = 0; c < BD; ++c) jr[c] = on ? static_cast<double>(J[static_cast<size_t>(i) * BD * BD + BD * k + c]) : 0.0;
+  for (int c = 0; c < BD; ++c) jr[c] = static_cast<double>(J[static_cast<size_t>(ic) * BD * BD + BD * kc + c]);
   const int done0 = sc->done;
   const double dn = sc->dn[it & 1];
   const double dq = wave_sum_array<24>(part_dq, n_dq);  // one batch up to 1536 rows
@@ -1535,7 +1546,7 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
   }
   double v = 0.0;
 #pragma unroll
-  for (int c = 0; c < BD; ++c) v += jr[c] * __shfl(rv, (lane & ~7) + c, 64);
+  for (int c = 0; c < BD; ++c) v += (on ? jr[c] : 0.0) * __shfl(rv, (lane & ~7) + c, 64);
   double dot = 0.0;
   if (on) {
     s[o] = v;
