@@ -185,6 +185,40 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 }
 
 // ============================================================ linearisation
+
+// Operands of one observation slot: its indices and everything its error and
+// Jacobians need. Every load is unconditional (callers pass a valid slot),
+// so a thread can have the operands of several slots in flight at once
+// instead of walking one slot's chain of dependent loads
+// (slot -> point / camera -> coordinates) at a time.
+struct SlotOps {
+  int pos, cam, ph, chh, praw, q;
+  double Rt[12], in[4], X[3], z[2], w;
+};
+__device__ __forceinline__ void load_slot_ops(const DevGraph& g, const double* __restrict__ camRt,
+                                              const double* __restrict__ pts, int s, SlotOps& o) {
+  o.pos = g.slot_pos[s];
+  o.cam = g.slot_cam[s];
+  o.q = g.slot_q[s];
+  o.z[0] = g.slot_meas[2 * static_cast<size_t>(s)];
+  o.z[1] = g.slot_meas[2 * static_cast<size_t>(s) + 1];
+  o.w = g.slot_w[s];
+  o.praw = g.pos_pt[o.pos];
+  o.ph = g.pos_h[o.pos];
+  o.chh = g.camH[o.cam];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) o.Rt[k] = camRt[12 * static_cast<size_t>(o.cam) + k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o.in[k] = g.intr[4 * static_cast<size_t>(o.cam) + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) o.X[k] = pts[3 * static_cast<size_t>(o.praw) + k];
+}
+// Slots whose operands one thread of a linearisation chunk loads together:
+// two for the linearisation; one (all of a slot's loads before its first
+// branch) where more would cost occupancy (the residual and Schur-diagonal
+// passes hold more registers per slot).
+constexpr int LIN_PF = 2;
+constexpr int LIN_PF1 = 1;
 // BlockSolver::buildSystem (block_solver.hpp:492-565) for EdgeSE3ProjectXYZ:
 // errors, Huber weights, Jacobians (edge_project_xyz.cpp:57-93) and the
 // Hll / Hpp / b sums in FP64, the stored Hpl blocks in HT; constructQuadraticForm
@@ -213,46 +247,50 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
   for (int i = tid; i < np * 9; i += LIN_THREADS) s_pacc[i] = 0.0;
   __syncthreads();
   double chi = 0;
-  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
-    const int pos = g.slot_pos[s], lp = pos - p0;
-    const int praw = g.pos_pt[pos], ph = g.pos_h[pos];
-    const int cam = g.slot_cam[s], chh = g.camH[cam];
-    const double* Rt = camRt + 12 * cam;
-    const double* in = g.intr + 4 * cam;
-    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
-    double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
-    const double w = g.slot_w[s];
-    double Xc[3], e[2];
-    project_error(Rt, in, X, z, Xc, e);
-    double rho[3];
-    huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
-    chi += rho[0];
-    const double om = rho[1] * w;
-    const double wr0 = -om * e[0], wr1 = -om * e[1];
-    // Jacobians and the Hll / Hpp / b sums in FP64 for either HT; only the
-    // stored Hpl block is rounded to HT (the FP32 path keeps Hpp and Hll
-    // accurate, which its FP64 Schur diagonal relies on)
-    double Jp[6], Jc[12];
-    project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
-    if (ph >= 0) {
-      double* pa = s_pacc + 9 * lp;
-      int k = 0;
-      for (int r = 0; r < 3; ++r)
-        for (int c = r; c < 3; ++c) atomicAdd(&pa[k++], om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]));
-      for (int r = 0; r < 3; ++r) atomicAdd(&pa[6 + r], Jp[r] * wr0 + Jp[3 + r] * wr1);
-    }
-    if (chh >= 0) {
-      const int lc = lower_bound_i(s_cam, nc, chh);
-      double* ca = s_cacc + 27 * lc;
-      for (int k = 0; k < 21; ++k) {
-        const int r = c_up6_r[k], c = c_up6_c[k];
-        atomicAdd(&ca[k], om * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c]));
-      }
-      for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], Jc[r] * wr0 + Jc[6 + r] * wr1);
+  for (int sb = s0 + tid; sb < s1; sb += LIN_PF * LIN_THREADS) {
+    SlotOps op[LIN_PF];
+#pragma unroll
+    for (int u = 0; u < LIN_PF; ++u) load_slot_ops(g, camRt, pts, min(sb + u * LIN_THREADS, s1 - 1), op[u]);
+#pragma unroll
+    for (int u = 0; u < LIN_PF; ++u) {
+      if (sb + u * LIN_THREADS >= s1) break;
+      const SlotOps& o = op[u];
+      const int lp = o.pos - p0, ph = o.ph, chh = o.chh;
+      const double* Rt = o.Rt;
+      const double* in = o.in;
+      const double w = o.w;
+      double Xc[3], e[2];
+      project_error(Rt, in, o.X, o.z, Xc, e);
+      double rho[3];
+      huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+      chi += rho[0];
+      const double om = rho[1] * w;
+      const double wr0 = -om * e[0], wr1 = -om * e[1];
+      // Jacobians and the Hll / Hpp / b sums in FP64 for either HT; only the
+      // stored Hpl block is rounded to HT (the FP32 path keeps Hpp and Hll
+      // accurate, which its FP64 Schur diagonal relies on)
+      double Jp[6], Jc[12];
+      project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
       if (ph >= 0) {
-        HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]);
-        for (int r = 0; r < 6; ++r)
-          for (int c = 0; c < 3; ++c) h[3 * r + c] = static_cast<HT>(om * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]));
+        double* pa = s_pacc + 9 * lp;
+        int k = 0;
+        for (int r = 0; r < 3; ++r)
+          for (int c = r; c < 3; ++c) atomicAdd(&pa[k++], om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]));
+        for (int r = 0; r < 3; ++r) atomicAdd(&pa[6 + r], Jp[r] * wr0 + Jp[3 + r] * wr1);
+      }
+      if (chh >= 0) {
+        const int lc = lower_bound_i(s_cam, nc, chh);
+        double* ca = s_cacc + 27 * lc;
+        for (int k = 0; k < 21; ++k) {
+          const int r = c_up6_r[k], c = c_up6_c[k];
+          atomicAdd(&ca[k], om * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c]));
+        }
+        for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], Jc[r] * wr0 + Jc[6 + r] * wr1);
+        if (ph >= 0) {
+          HT* h = Hpl + 18 * static_cast<size_t>(o.q);
+          for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 3; ++c) h[3 * r + c] = static_cast<HT>(om * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]));
+        }
       }
     }
   }
@@ -493,14 +531,28 @@ __global__ __launch_bounds__(256) void k_schur_win(DevGraph g, DevWinPlan wp, co
   };
   const int w = blockIdx.x, tid = threadIdx.x;
   const int o0 = wp.win_obs0[w], nob = wp.win_obs0[w + 1] - o0;
-  for (int l = tid; l < nob; l += 256) s_q(l) = wp.win_q[o0 + l];
-  __syncthreads();
-  // the window's Hpl blocks in 8/16-byte chunks; its observations ascend in
-  // q, so runs of one camera are contiguous in HBM. Independent loads: the
-  // unrolled loop keeps several in flight per thread.
-  // chunk e = (l, k) walked incrementally: 256 = 28 NCH + 4 for NCH = 9
-  static_assert(NCH == 9, "the staging walk assumes 9 chunks per block");
+  if (nob <= 0) return;
+  // Each staging phase issues all of a thread's loads before its first LDS
+  // write, from clamped (valid) indices: the q indices share the LDS array
+  // with the rows being written, so a load-then-write loop would wait for
+  // every load in turn.
+  constexpr int UQ = OBS / 256;  // observations per thread
   {
+    int qv[UQ];
+#pragma unroll
+    for (int u = 0; u < UQ; ++u) qv[u] = wp.win_q[o0 + min(tid + 256 * u, nob - 1)];
+#pragma unroll
+    for (int u = 0; u < UQ; ++u)
+      if (tid + 256 * u < nob) s_q(tid + 256 * u) = qv[u];
+  }
+  __syncthreads();
+  // the window's Hpl blocks in 8/16-byte chunks (chunk e = (l, k) = (e / NCH,
+  // e % NCH)); its observations ascend in q, so runs of one camera are
+  // contiguous in HBM
+  if constexpr (!QPAD) {
+    // q in its own array: the unrolled loop keeps several loads in flight
+    // (chunk e walked incrementally: 256 = 28 NCH + 4 for NCH = 9)
+    static_assert(NCH == 9, "the staging walk assumes 9 chunks per block");
     int l = tid / NCH, k = tid - NCH * (tid / NCH);
 #pragma unroll 6
     for (int e = tid; e < nob * NCH; e += 256) {
@@ -510,18 +562,46 @@ __global__ __launch_bounds__(256) void k_schur_win(DevGraph g, DevWinPlan wp, co
       k += 4;
       if (k >= NCH) { k -= NCH; ++l; }
     }
+  } else {
+    constexpr int UE = (OBS * NCH + 255) / 256;  // chunks per thread
+    const int ne = nob * NCH;
+    int qv[UE];
+#pragma unroll
+    for (int u = 0; u < UE; ++u) qv[u] = s_q(min(tid + 256 * u, ne - 1) / NCH);
+    CT v[UE];
+#pragma unroll
+    for (int u = 0; u < UE; ++u) {
+      const int e = min(tid + 256 * u, ne - 1);
+      v[u] = reinterpret_cast<const CT*>(Hpl + 18 * static_cast<size_t>(qv[u]))[e - NCH * (e / NCH)];
+    }
+#pragma unroll
+    for (int u = 0; u < UE; ++u) {
+      const int e = tid + 256 * u;
+      if (e < ne) reinterpret_cast<CT*>(s_G + (e / NCH) * GS)[e - NCH * (e / NCH)] = v[u];
+    }
   }
   __syncthreads();
   // G = Hpl L^-T with the point's L^-1 (k_point_linv)
-  for (int l = tid; l < nob; l += 256) {
-    const double* mp = Linv + 6 * static_cast<size_t>(g.q_ph[s_q(l)]);
-    const double m[6] = {mp[0], mp[1], mp[2], mp[3], mp[4], mp[5]};
-    HT* G = s_G + l * GS;
-    for (int r = 0; r < 6; ++r) {
-      const double h0 = G[3 * r], h1 = G[3 * r + 1], h2 = G[3 * r + 2];
-      G[3 * r] = static_cast<HT>(h0 * m[0]);
-      G[3 * r + 1] = static_cast<HT>(h0 * m[1] + h1 * m[2]);
-      G[3 * r + 2] = static_cast<HT>(h0 * m[3] + h1 * m[4] + h2 * m[5]);
+  {
+    int ph[UQ];
+#pragma unroll
+    for (int u = 0; u < UQ; ++u) ph[u] = g.q_ph[s_q(min(tid + 256 * u, nob - 1))];
+    double m[UQ][6];
+#pragma unroll
+    for (int u = 0; u < UQ; ++u)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) m[u][k] = Linv[6 * static_cast<size_t>(ph[u]) + k];
+#pragma unroll
+    for (int u = 0; u < UQ; ++u) {
+      const int l = tid + 256 * u;
+      if (l >= nob) continue;
+      HT* G = s_G + l * GS;
+      for (int r = 0; r < 6; ++r) {
+        const double h0 = G[3 * r], h1 = G[3 * r + 1], h2 = G[3 * r + 2];
+        G[3 * r] = static_cast<HT>(h0 * m[u][0]);
+        G[3 * r + 1] = static_cast<HT>(h0 * m[u][1] + h1 * m[u][2]);
+        G[3 * r + 2] = static_cast<HT>(h0 * m[u][3] + h1 * m[u][4] + h2 * m[u][5]);
+      }
     }
   }
   __syncthreads();
@@ -792,34 +872,36 @@ __global__ __launch_bounds__(LIN_THREADS) void k_refine(DevGraph g, const double
   for (int i = tid; i < nc * 6; i += LIN_THREADS) s_cacc[i] = 0.0;
   for (int i = tid; i < np * 9; i += LIN_THREADS) s_pacc[i] = 0.0;
   __syncthreads();
-  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
-    const int pos = g.slot_pos[s];
-    const int ph = g.pos_h[pos];
-    if (ph < 0) continue;
-    const int praw = g.pos_pt[pos];
-    const int cam = g.slot_cam[s], chh = g.camH[cam];
-    const double* Rt = camRt + 12 * cam;
-    const double* in = g.intr + 4 * cam;
-    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
-    double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
-    const double w = g.slot_w[s];
-    double Xc[3], e[2];
-    project_error(Rt, in, X, z, Xc, e);
-    double rho[3];
-    huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
-    const double om = rho[1] * w;
-    double Jp[6], Jc[12];
-    project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
-    double t0 = e[0], t1 = e[1];
-    if (chh >= 0 && xp) {
-      const double* x = xp + 6 * static_cast<size_t>(chh);
-      for (int k = 0; k < 6; ++k) { t0 += Jc[k] * x[k]; t1 += Jc[6 + k] * x[k]; }
+  for (int sb = s0 + tid; sb < s1; sb += LIN_PF1 * LIN_THREADS) {
+    SlotOps op[LIN_PF1];
+    double xv[LIN_PF1][6];
+#pragma unroll
+    for (int u = 0; u < LIN_PF1; ++u) {
+      load_slot_ops(g, camRt, pts, min(sb + u * LIN_THREADS, s1 - 1), op[u]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) xv[u][k] = xp ? xp[6 * static_cast<size_t>(max(op[u].chh, 0)) + k] : 0.0;
     }
-    double* pa = s_pacc + 9 * (pos - p0);
-    int k = 0;
-    for (int r = 0; r < 3; ++r)
-      for (int c = r; c < 3; ++c) atomicAdd(&pa[k++], om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]));
-    for (int r = 0; r < 3; ++r) atomicAdd(&pa[6 + r], -om * (Jp[r] * t0 + Jp[3 + r] * t1));
+#pragma unroll
+    for (int u = 0; u < LIN_PF1; ++u) {
+      if (sb + u * LIN_THREADS >= s1) break;
+      const SlotOps& o = op[u];
+      if (o.ph < 0) continue;
+      double Xc[3], e[2];
+      project_error(o.Rt, o.in, o.X, o.z, Xc, e);
+      double rho[3];
+      huber_rho(o.w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+      const double om = rho[1] * o.w;
+      double Jp[6], Jc[12];
+      project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+      double t0 = e[0], t1 = e[1];
+      if (o.chh >= 0 && xp)
+        for (int k = 0; k < 6; ++k) { t0 += Jc[k] * xv[u][k]; t1 += Jc[6 + k] * xv[u][k]; }
+      double* pa = s_pacc + 9 * (o.pos - p0);
+      int k = 0;
+      for (int r = 0; r < 3; ++r)
+        for (int c = r; c < 3; ++c) atomicAdd(&pa[k++], om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]));
+      for (int r = 0; r < 3; ++r) atomicAdd(&pa[6 + r], -om * (Jp[r] * t0 + Jp[3 + r] * t1));
+    }
   }
   __syncthreads();
   for (int lp = tid; lp < np; lp += LIN_THREADS) {
@@ -840,32 +922,35 @@ __global__ __launch_bounds__(LIN_THREADS) void k_refine(DevGraph g, const double
   }
   if (!cam_part) return;
   __syncthreads();
-  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
-    const int cam = g.slot_cam[s], chh = g.camH[cam];
-    if (chh < 0) continue;
-    const int pos = g.slot_pos[s];
-    const int praw = g.pos_pt[pos];
-    const double* Rt = camRt + 12 * cam;
-    const double* in = g.intr + 4 * cam;
-    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
-    double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
-    const double w = g.slot_w[s];
-    double Xc[3], e[2];
-    project_error(Rt, in, X, z, Xc, e);
-    double rho[3];
-    huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
-    const double om = rho[1] * w;
-    double Jp[6], Jc[12];
-    project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
-    double t0 = e[0], t1 = e[1];
-    if (xp) {
-      const double* x = xp + 6 * static_cast<size_t>(chh);
-      for (int k = 0; k < 6; ++k) { t0 += Jc[k] * x[k]; t1 += Jc[6 + k] * x[k]; }
+  for (int sb = s0 + tid; sb < s1; sb += LIN_PF1 * LIN_THREADS) {
+    SlotOps op[LIN_PF1];
+    double xv[LIN_PF1][6];
+#pragma unroll
+    for (int u = 0; u < LIN_PF1; ++u) {
+      load_slot_ops(g, camRt, pts, min(sb + u * LIN_THREADS, s1 - 1), op[u]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) xv[u][k] = xp ? xp[6 * static_cast<size_t>(max(op[u].chh, 0)) + k] : 0.0;
     }
-    const double* u = s_pacc + 9 * (pos - p0) + 6;
-    for (int k = 0; k < 3; ++k) { t0 += Jp[k] * u[k]; t1 += Jp[3 + k] * u[k]; }
-    const int lc = lower_bound_i(s_cam, nc, chh);
-    for (int r = 0; r < 6; ++r) atomicAdd(&s_cacc[6 * lc + r], -om * (Jc[r] * t0 + Jc[6 + r] * t1));
+#pragma unroll
+    for (int u = 0; u < LIN_PF1; ++u) {
+      if (sb + u * LIN_THREADS >= s1) break;
+      const SlotOps& o = op[u];
+      if (o.chh < 0) continue;
+      double Xc[3], e[2];
+      project_error(o.Rt, o.in, o.X, o.z, Xc, e);
+      double rho[3];
+      huber_rho(o.w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+      const double om = rho[1] * o.w;
+      double Jp[6], Jc[12];
+      project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+      double t0 = e[0], t1 = e[1];
+      if (xp)
+        for (int k = 0; k < 6; ++k) { t0 += Jc[k] * xv[u][k]; t1 += Jc[6 + k] * xv[u][k]; }
+      const double* uu = s_pacc + 9 * (o.pos - p0) + 6;
+      for (int k = 0; k < 3; ++k) { t0 += Jp[k] * uu[k]; t1 += Jp[3 + k] * uu[k]; }
+      const int lc = lower_bound_i(s_cam, nc, o.chh);
+      for (int r = 0; r < 6; ++r) atomicAdd(&s_cacc[6 * lc + r], -om * (Jc[r] * t0 + Jc[6 + r] * t1));
+    }
   }
   __syncthreads();
   for (int i = tid; i < nc * 6; i += LIN_THREADS) cam_part[static_cast<size_t>(c0) * 6 + i] = s_cacc[i];
@@ -1011,6 +1096,17 @@ __device__ __forceinline__ void obs_V64(const DevGraph& g, const double* __restr
     for (int cc = 0; cc < 3; ++cc) V[3 * r + cc] += om * (Jc[r] * Jp[cc] + Jc[6 + r] * Jp[3 + cc]);
 }
 
+// The same from operands already loaded (load_slot_ops).
+__device__ __forceinline__ void obs_V64_ops(const DevGraph& g, const SlotOps& o, double* V) {
+  double Xc[3], e[2], rho[3], Jp[6], Jc[12];
+  project_error(o.Rt, o.in, o.X, o.z, Xc, e);
+  huber_rho(o.w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+  const double om = rho[1] * o.w;
+  project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+  for (int r = 0; r < 6; ++r)
+    for (int cc = 0; cc < 3; ++cc) V[3 * r + cc] += om * (Jc[r] * Jp[cc] + Jc[6 + r] * Jp[3 + cc]);
+}
+
 template <typename HT>
 __global__ __launch_bounds__(256) void k_schur_diag(DevGraph g, const int* __restrict__ q_slot,
                                                     const double* __restrict__ camRt, const double* __restrict__ pts,
@@ -1133,32 +1229,52 @@ __global__ __launch_bounds__(LIN_THREADS) void k_schur_diag_pm(DevGraph g, const
     }
   }
   __syncthreads();
-  for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
-    const int pos = g.slot_pos[s], cam = g.slot_cam[s];
-    const int ph = g.pos_h[pos], chh = g.camH[cam];
-    if (ph < 0 || chh < 0) continue;
-    if (s > s0 && g.slot_pos[s - 1] == pos && g.slot_cam[s - 1] == cam) continue;  // the run's first slot sums it
-    const int praw = g.pos_pt[pos];
-    const double* Rt = camRt + 12 * static_cast<size_t>(cam);
-    const double* in = g.intr + 4 * static_cast<size_t>(cam);
-    double V[18];
-    for (int k = 0; k < 18; ++k) V[k] = 0.0;
-    obs_V64(g, Rt, in, pts, s, praw, V);
-    for (int s2 = s + 1; s2 < s1 && g.slot_pos[s2] == pos && g.slot_cam[s2] == cam; ++s2)
-      obs_V64(g, Rt, in, pts, s2, praw, V);
-    const double* pd = s_pd + 9 * (pos - p0);
-    const double D[9] = {pd[0], pd[1], pd[2], pd[1], pd[3], pd[4], pd[2], pd[4], pd[5]};
-    double VD[18];
-    for (int r = 0; r < 6; ++r)
-      for (int cc = 0; cc < 3; ++cc)
-        VD[3 * r + cc] = V[3 * r] * D[cc] + V[3 * r + 1] * D[3 + cc] + V[3 * r + 2] * D[6 + cc];
-    double* ca = s_cacc + 27 * lower_bound_i(s_cam, nc, chh);
-    int k = 0;
-    for (int r = 0; r < 6; ++r)
-      for (int cc = r; cc < 6; ++cc)
-        atomicAdd(&ca[k++], VD[3 * r] * V[3 * cc] + VD[3 * r + 1] * V[3 * cc + 1] + VD[3 * r + 2] * V[3 * cc + 2]);
-    if (bl)
-      for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], V[3 * r] * pd[6] + V[3 * r + 1] * pd[7] + V[3 * r + 2] * pd[8]);
+  for (int sb = s0 + tid; sb < s1; sb += LIN_PF1 * LIN_THREADS) {
+    // operands of this thread's next slots, and the neighbouring slots'
+    // indices that delimit runs of one (point, camera) pair, all in flight
+    SlotOps op[LIN_PF1];
+    int nb_pos[LIN_PF1][2], nb_cam[LIN_PF1][2];
+#pragma unroll
+    for (int u = 0; u < LIN_PF1; ++u) {
+      const int sc = min(sb + u * LIN_THREADS, s1 - 1);
+      load_slot_ops(g, camRt, pts, sc, op[u]);
+      const int sp = max(sc - 1, s0), sn = min(sc + 1, s1 - 1);
+      nb_pos[u][0] = g.slot_pos[sp];
+      nb_cam[u][0] = g.slot_cam[sp];
+      nb_pos[u][1] = g.slot_pos[sn];
+      nb_cam[u][1] = g.slot_cam[sn];
+    }
+#pragma unroll
+    for (int u = 0; u < LIN_PF1; ++u) {
+      const int s = sb + u * LIN_THREADS;
+      if (s >= s1) break;
+      const SlotOps& o = op[u];
+      const int pos = o.pos, cam = o.cam, chh = o.chh;
+      if (o.ph < 0 || chh < 0) continue;
+      if (s > s0 && nb_pos[u][0] == pos && nb_cam[u][0] == cam) continue;  // the run's first slot sums it
+      const int praw = o.praw;
+      const double* Rt = o.Rt;
+      const double* in = o.in;
+      double V[18];
+      for (int k = 0; k < 18; ++k) V[k] = 0.0;
+      obs_V64_ops(g, o, V);
+      if (s + 1 < s1 && nb_pos[u][1] == pos && nb_cam[u][1] == cam)
+        for (int s2 = s + 1; s2 < s1 && g.slot_pos[s2] == pos && g.slot_cam[s2] == cam; ++s2)
+          obs_V64(g, Rt, in, pts, s2, praw, V);
+      const double* pd = s_pd + 9 * (pos - p0);
+      const double D[9] = {pd[0], pd[1], pd[2], pd[1], pd[3], pd[4], pd[2], pd[4], pd[5]};
+      double VD[18];
+      for (int r = 0; r < 6; ++r)
+        for (int cc = 0; cc < 3; ++cc)
+          VD[3 * r + cc] = V[3 * r] * D[cc] + V[3 * r + 1] * D[3 + cc] + V[3 * r + 2] * D[6 + cc];
+      double* ca = s_cacc + 27 * lower_bound_i(s_cam, nc, chh);
+      int k = 0;
+      for (int r = 0; r < 6; ++r)
+        for (int cc = r; cc < 6; ++cc)
+          atomicAdd(&ca[k++], VD[3 * r] * V[3 * cc] + VD[3 * r + 1] * V[3 * cc + 1] + VD[3 * r + 2] * V[3 * cc + 2]);
+      if (bl)
+        for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], V[3 * r] * pd[6] + V[3 * r + 1] * pd[7] + V[3 * r + 2] * pd[8]);
+    }
   }
   __syncthreads();
   for (int i = tid; i < nc * 27; i += LIN_THREADS) part[static_cast<size_t>(c0) * 27 + i] = s_cacc[i];
