@@ -486,11 +486,16 @@ __device__ __forceinline__ void pair_mac(const HT* __restrict__ sG, unsigned pr,
   HT Ga[18], Gb[18];
   lds_load18(sG + (pr & 0xFFFFu) * GS, Ga);
   lds_load18(sG + (pr >> 16) * GS, Gb);
+  // three fused multiply-adds into the accumulator per entry (108 per pair)
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j)
-      acc[6 * i + j] += Ga[3 * i] * Gb[3 * j] + Ga[3 * i + 1] * Gb[3 * j + 1] + Ga[3 * i + 2] * Gb[3 * j + 2];
+    for (int j = 0; j < 6; ++j) {
+      HT a = acc[6 * i + j];
+      a = fma(Ga[3 * i], Gb[3 * j], a);
+      a = fma(Ga[3 * i + 1], Gb[3 * j + 1], a);
+      acc[6 * i + j] = fma(Ga[3 * i + 2], Gb[3 * j + 2], a);
+    }
 }
 
 // L^-1 (lower triangular, [i00 i10 i11 i20 i21 i22]) of L L^T = Hll + lambda I
