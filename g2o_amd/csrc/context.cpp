@@ -236,6 +236,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(hipMemsetAsync(d_x.p, 0, nx * sizeof(double), stream));
   g = DevGraph{};
   g.C = st.C; g.P = st.P; g.Cf = st.Cf; g.Pf = st.Pf; g.npos = st.npos; g.ns = st.ns; g.nlc = st.nlc;
+  g.lc_maxc = std::max(st.lc_maxc, 1);
   g.nS = st.nS; g.nF = st.nF;
   g.camH = d_camH.p; g.ptH = d_ptH.p;
   g.pos_pt = d_pos_pt.p; g.pos_h = d_pos_h.p; g.pos_ptr = d_pos_ptr.p; g.pos_nfree = d_pos_nfree.p;

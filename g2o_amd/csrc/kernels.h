@@ -13,6 +13,7 @@ constexpr int RED_BLOCKS = 512;  // fixed grid of the grid-stride reductions
 // Device view of the structure (all arrays device-resident, read-only).
 struct DevGraph {
   int C, P, Cf, Pf, npos, ns, nlc, nS, nF;
+  int lc_maxc;  // most free cameras in one linearisation chunk
   double huber;
   const int *camH, *ptH;
   const int *pos_pt, *pos_h, *pos_ptr, *pos_nfree;

@@ -234,9 +234,13 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
                                                           double* __restrict__ hpp_part,
                                                           double* __restrict__ chunk_chi) {
   __shared__ int s_cam[LIN_CAMS];
-  __shared__ double s_cacc[LIN_CAMS * 27];
-  __shared__ double s_pacc[LIN_PTS * 9];
   __shared__ double s_red[LIN_THREADS / 64];
+  // camera accumulators sized for the plan's largest chunk (27 per camera),
+  // then the point accumulators: a smaller footprint keeps more workgroups
+  // resident per CU
+  extern __shared__ double s_dyn[];
+  double* const s_cacc = s_dyn;
+  double* const s_pacc = s_dyn + 27 * g.lc_maxc;
   const int ch = blockIdx.x, tid = threadIdx.x;
   const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
   const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
@@ -311,7 +315,8 @@ template <typename HT>
 hipError_t launch_linearize(const DevGraph& g, const double* camRt, const double* pts, HT* Hpl, double* Hll,
                             double* b, double* hpp_part, double* chunk_chi, hipStream_t st) {
   if (g.nlc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_linearize<HT>, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hpl, Hll, b, hpp_part,
+  const size_t lds = sizeof(double) * (27 * static_cast<size_t>(g.lc_maxc) + 9 * LIN_PTS);
+  hipLaunchKernelGGL(k_linearize<HT>, dim3(g.nlc), dim3(LIN_THREADS), lds, st, g, camRt, pts, Hpl, Hll, b, hpp_part,
                      chunk_chi);
   return hipGetLastError();
 }

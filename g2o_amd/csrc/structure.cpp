@@ -143,6 +143,7 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   int cur_slots = 0, cur_pts = 0;
   auto close_chunk = [&](int end_pos) {
     std::sort(cams.begin(), cams.end());
+    st.lc_maxc = std::max(st.lc_maxc, static_cast<int>(cams.size()));
     st.lc_cams.insert(st.lc_cams.end(), cams.begin(), cams.end());
     st.lc_pos0.push_back(end_pos);
     st.lc_cam_ptr.push_back(static_cast<int>(st.lc_cams.size()));

@@ -85,6 +85,7 @@ struct BAStructure {
   // linearisation chunks
   LinPlanCaps lcaps;
   int nlc = 0;
+  int lc_maxc = 0;  // most free cameras in one chunk
   std::vector<int> lc_pos0;          // nlc+1
   std::vector<int> lc_cam_ptr;       // nlc+1
   std::vector<int> lc_cams;          // free cameras per chunk, sorted
