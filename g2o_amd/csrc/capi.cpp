@@ -1,5 +1,6 @@
 // extern "C" entry points of include/g2o_hip.h. Each wraps one context
 // operation; errors become status codes plus the context's last-error text.
+#include <cstdlib>
 #include <new>
 #include <string>
 
@@ -38,11 +39,14 @@ int g2ohip_create(int device, g2ohip_ctx** out) {
     }
     c->device = device;
     // host-mapped PCG progress record and LM trial scalars (optional:
-    // without them the host copies and synchronises the stream)
+    // without them, or with G2OHIP_HOST_MAPPED=0 in the environment, the host
+    // copies and synchronises the stream)
     static_assert(sizeof(g2ohip::PcgProgress) <= 128 && sizeof(g2ohip::TrialScalars) <= 128, "mapped layout");
+    const char* mapped_env = std::getenv("G2OHIP_HOST_MAPPED");
+    const bool mapped = !(mapped_env && mapped_env[0] == '0');
     void* hp = nullptr;
     void* dp = nullptr;
-    if (hipHostMalloc(&hp, 256, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
+    if (mapped && hipHostMalloc(&hp, 256, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
       if (hipHostGetDevicePointer(&dp, hp, 0) == hipSuccess && dp) {
         c->h_prog = static_cast<g2ohip::PcgProgress*>(hp);
         c->d_prog = static_cast<g2ohip::PcgProgress*>(dp);

@@ -147,6 +147,32 @@ def test_lm_trace_fp64(case, request, oracle_lib):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_host_mapped_and_synchronised_paths_agree(precision, walk, monkeypatch):
+    """The PCG's progress and final scalars and the LM trial scalars reach
+    the host through host-mapped memory; with G2OHIP_HOST_MAPPED=0 the host
+    copies them after a stream synchronisation instead. Both run the same
+    kernels, so the LM traces agree to the run-to-run rounding of the LDS sums
+    (and take the same trials)."""
+    import g2o_amd
+    prec = g2o_amd.FP64 if precision == "fp64" else g2o_amd.FP32
+    runs = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("G2OHIP_HOST_MAPPED", env)
+        ba = _mk(walk, precision=prec, refine_steps=0 if precision == "fp64" else 1)
+        runs.append((ba.optimize(6), ba.state()))
+        del ba
+    (s1, (c1, p1)), (s0, (c0, p0)) = runs
+    _say(f"{precision}: mapped {[s['chi2'] for s in s1]}")
+    _say(f"{precision}: synced {[s['chi2'] for s in s0]}")
+    assert len(s1) == len(s0)
+    for a, b in zip(s1, s0):
+        assert a["levenberg_iterations"] == b["levenberg_iterations"]
+        assert abs(a["iterations_linear_solver"] - b["iterations_linear_solver"]) <= 1
+        assert abs(a["chi2"] - b["chi2"]) <= 1e-8 * b["chi2"]
+    assert _rel(p1, p0) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_lm_trace_with_rejected_trials(precision, oracle_lib):
     """A start far from the optimum makes a step overshoot, so a trial is
     rejected and popped: the next trial must linearise at the restored
