@@ -4,8 +4,8 @@ Reads the data (numbers only) held by g2o's unit-test helper
 unit_test/solver/sparse_system_helper.cpp: the serialized 12 x 12 block
 matrix with 3x3 blocks (sparseMatrixString, :52-148), its dense inverse
 (denseInverseMatrixString), and the vectors b (createTestVectorB, :255-294)
-and x (createTestVectorX, :296-337). Run here, where the reference is mounted;
-the GPU box only reads the JSON it writes.
+and x (createTestVectorX, :296-337). Run where the reference is mounted;
+the tests only read the JSON it writes.
 """
 import json
 import os
