@@ -291,9 +291,14 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
         }
         for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], Jc[r] * wr0 + Jc[6 + r] * wr1);
         if (ph >= 0) {
-          HT* h = Hpl + 18 * static_cast<size_t>(o.q);
+          // the 18 entries as 9 two-element stores (a block is 8-byte aligned)
+          HT v[18];
           for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 3; ++c) h[3 * r + c] = static_cast<HT>(om * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]));
+            for (int c = 0; c < 3; ++c) v[3 * r + c] = static_cast<HT>(om * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]));
+          using HT2 = typename std::conditional<sizeof(HT) == 4, float2, double2>::type;
+          HT2* h2 = reinterpret_cast<HT2*>(Hpl + 18 * static_cast<size_t>(o.q));
+#pragma unroll
+          for (int k = 0; k < 9; ++k) h2[k] = HT2{v[2 * k], v[2 * k + 1]};
         }
       }
     }
