@@ -172,6 +172,28 @@ def test_host_mapped_and_synchronised_paths_agree(precision, walk, monkeypatch):
     assert _rel(p1, p0) < 1e-6
 
 
+def test_fp32_falls_back_to_fp64_when_refinement_stalls(walk):
+    """At a tolerance the FP32-rounded reduced system cannot reach (1e-18 of
+    the initial r.M^-1 r: the FP64 residual of an FP32 solution is about
+    (6e-8)^2 of it), the FP64 residual misses the threshold by more than 10x,
+    so the solve is redone in FP64 at the same state and the rest of
+    optimize() stays FP64: the trace is then the FP64 path's."""
+    import g2o_amd
+    kw = dict(pcg_tolerance=1e-18, pcg_max_iter=300)
+    b32 = _mk(walk, precision=g2o_amd.FP32, refine_steps=1, **kw)
+    s32 = b32.optimize(3)
+    b64 = _mk(walk, precision=g2o_amd.FP64, refine_steps=0, **kw)
+    s64 = b64.optimize(3)
+    _say(f"fallback flags {[s['fp64_fallback'] for s in s32]}")
+    _say(f"fp32 {[s['chi2'] for s in s32]}")
+    _say(f"fp64 {[s['chi2'] for s in s64]}")
+    assert s32[0]["fp64_fallback"] == 1
+    assert len(s32) == len(s64)
+    for a, b in zip(s32, s64):
+        assert a["levenberg_iterations"] == b["levenberg_iterations"]
+        assert abs(a["chi2"] - b["chi2"]) <= 1e-8 * b["chi2"]
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_lm_trace_with_rejected_trials(precision, oracle_lib):
     """A start far from the optimum makes a step overshoot, so a trial is
