@@ -100,6 +100,8 @@ def run_gpu(prob, precision, refine, steps, warmup):
 
 
 def cpu_baseline(prob, iters):
+    """The oracle's LM run from the same start: (baseline record, its chi2
+    after each iteration)."""
     from oracle import oracle
     n_threads = oracle.lib().orc_num_threads()
     t = time.perf_counter()
@@ -108,9 +110,11 @@ def cpu_baseline(prob, iters):
     t = time.perf_counter()
     n, trace, times = ora.optimize(iters, solver=1, tol=1e-6)
     dt = time.perf_counter() - t
-    return {"value": n / dt, "unit": "LM iter/s", "cores": int(n_threads), "kind": "port",
-            "sample": f"{n} LM iterations (FP64 explicit Schur + block-Jacobi PCG, tol 1e-6) of the same "
-                      f"workload on the host; structure setup {setup:.1f} s excluded; chi2 after: {trace[-1][1]:.6e}"}
+    return ({"value": n / dt, "unit": "LM iter/s", "cores": int(n_threads), "kind": "port",
+             "sample": f"{n} LM iterations (FP64 explicit Schur + block-Jacobi PCG, tol 1e-6) of the same "
+                       f"workload from the same start on the host; structure setup {setup:.1f} s excluded; "
+                       f"chi2 after: {trace[-1][1]:.6e}"},
+            [float(c) for c in trace[:n, 1]])
 
 
 def main():
@@ -119,7 +123,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--refine", type=int, default=1)
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=None, help="oracle LM iterations (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cams", type=int, default=1500)
     ap.add_argument("--points", type=int, default=500_000)
@@ -220,7 +224,13 @@ def main():
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_iters)
+        out["cpu_baseline"], chi_o = cpu_baseline(prob, args.cpu_iters or args.steps)
+        # the same LM iterations on the oracle: chi2 error of the FP32+refine
+        # path against the FP64 CPU oracle, per iteration
+        k = min(len(chi_o), len(s32))
+        if k:
+            out["config"]["rel_chi2_diff_vs_oracle_per_iter"] = [abs(s32[i]["chi2"] - chi_o[i]) / chi_o[i]
+                                                                 for i in range(k)]
     print(json.dumps(out), flush=True)
 
 
