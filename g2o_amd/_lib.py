@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "csrc", "libg2o_hip.so")
 
-OK, ERR_INVALID, ERR_NOT_PD, ERR_NAN, ERR_OOM, ERR_HIP, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(8)
+OK, ERR_INVALID, ERR_NOT_PD, ERR_NAN, ERR_OOM, ERR_HIP, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_IO = range(9)
 FP64, FP32 = 0, 1
 SCHUR_EXPLICIT, SCHUR_IMPLICIT = 0, 1
 
@@ -26,7 +26,11 @@ EXPORTS = [
     "g2ohip_compute_scale", "g2ohip_optimize", "g2ohip_set_force_stop_flag", "g2ohip_ba_get_hessian",
     "g2ohip_ba_get_schur", "g2ohip_last_solve_info", "g2ohip_set_profiling", "g2ohip_get_profile",
     "g2ohip_linear_solve_pcg", "g2ohip_pg_set_structure", "g2ohip_pg_structure_info", "g2ohip_pg_set_measurements",
-    "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_ba_adjoint", "g2ohip_ba_newton",
+    "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_graph_create",
+    "g2ohip_graph_destroy", "g2ohip_graph_last_error", "g2ohip_graph_log", "g2ohip_graph_load", "g2ohip_graph_info",
+    "g2ohip_graph_get_ba", "g2ohip_graph_get_pg", "g2ohip_graph_set_ba_state", "g2ohip_graph_set_pg_state",
+    "g2ohip_graph_add_ba", "g2ohip_graph_add_pg", "g2ohip_graph_save", "g2ohip_graph_load_bal", "g2ohip_graph_get_bal",
+    "g2ohip_ba_adjoint", "g2ohip_ba_newton",
 ]
 K_LINEARIZE, K_SCHUR, K_PCG, K_REFINE, K_CHI2, K_UPDATE, K_SCHUR_WINDOWS, K_PCG_SPMV = range(8)
 KERNEL_NAMES = ["linearize", "schur", "pcg", "refine", "chi2", "update", "schur_windows", "pcg_spmv"]
@@ -107,6 +111,22 @@ def lib():
         "g2ohip_pg_set_state": (ct.c_int, [_vp, _dp]),
         "g2ohip_pg_get_state": (ct.c_int, [_vp, _dp]),
         "g2ohip_pg_get_system": (ct.c_int, [_vp, _ip, _ip, _dp, _dp]),
+        "g2ohip_graph_create": (ct.c_int, [ct.POINTER(_vp)]),
+        "g2ohip_graph_destroy": (None, [_vp]),
+        "g2ohip_graph_last_error": (ct.c_char_p, [_vp]),
+        "g2ohip_graph_log": (ct.c_char_p, [_vp]),
+        "g2ohip_graph_load": (ct.c_int, [_vp, ct.c_char_p, ct.c_int]),
+        "g2ohip_graph_info": (ct.c_int, [_vp, ct.POINTER(ct.c_longlong)]),
+        "g2ohip_graph_get_ba": (ct.c_int, [_vp, _ip, _dp, _up, _ip, _dp, _up, _ip, _ip, _dp, _dp]),
+        "g2ohip_graph_get_pg": (ct.c_int, [_vp, _ip, _dp, _up, _ip, _ip, _dp, _dp]),
+        "g2ohip_graph_set_ba_state": (ct.c_int, [_vp, _dp, _dp]),
+        "g2ohip_graph_set_pg_state": (ct.c_int, [_vp, _dp]),
+        "g2ohip_graph_add_ba": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _ip, _dp, _up, _ip, _dp, _up, _ip, _ip,
+                                           _dp, _dp]),
+        "g2ohip_graph_add_pg": (ct.c_int, [_vp, ct.c_int, ct.c_int, _ip, _dp, _up, _ip, _ip, _dp, _dp]),
+        "g2ohip_graph_save": (ct.c_int, [_vp, ct.c_char_p, ct.c_int]),
+        "g2ohip_graph_load_bal": (ct.c_int, [_vp, ct.c_char_p, ct.c_int]),
+        "g2ohip_graph_get_bal": (ct.c_int, [_vp, ct.POINTER(ct.c_longlong), _dp, _dp, _ip, _ip, _dp]),
     }
     for name in EXPORTS:
         fn = getattr(L, name)

@@ -418,3 +418,44 @@ def make_sphere_pose_graph(seed: int = 1001, nodes_per_level: int = 1000, laps: 
     fixed[0] = 1
     return PoseGraphProblem(poses, fr, to, meas, info, fixed, name=f"sphere_{laps}x{nodes_per_level}",
                             gt_poses=gt, info_dict={"radius": radius})
+
+
+# ------------------------------------------------------------------------ BAL
+@dataclass
+class BALProblem:
+    """BAL data (g2o/examples/bal/bal_example.cpp:70-207): cams [C, 9] =
+    angle-axis rotation, translation, f, k1, k2 (world to camera, P = R X + t,
+    the camera looking down its -z axis); pts [P, 3]; observations in file
+    order, meas [E, 2] pixels relative to the image centre."""
+    cams: np.ndarray
+    pts: np.ndarray
+    edge_cam: np.ndarray
+    edge_pt: np.ndarray
+    meas: np.ndarray
+
+
+def make_bal(seed: int = 11, num_cams: int = 8, num_pts: int = 300, k=(0.0, 0.0), noise: float = 0.5) -> BALProblem:
+    """Seeded BAL problem: cameras near the origin looking down their -z axis
+    at a point cloud 12 units away, focal lengths 500 + 10 c, radial
+    distortion k, Gaussian pixel noise; each point seen by each camera with
+    probability 0.7."""
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(-3, 3, size=(num_pts, 3))
+    cams, obs = [], []
+    for c in range(num_cams):
+        w = rng.normal(size=3) * 0.3
+        R = axis_angle_to_rot(w)
+        t = np.array([0.0, 0.0, -12.0]) + rng.normal(size=3) * 0.2
+        f = 500.0 + 10 * c
+        cams.append(np.r_[w, t, f, k[0], k[1]])
+        P = pts @ R.T + t
+        p = -P[:, :2] / P[:, 2:3]
+        r2 = np.sum(p * p, axis=1)
+        r = 1 + k[0] * r2 + k[1] * r2 * r2
+        u = f * r[:, None] * p + rng.normal(size=(num_pts, 2)) * noise
+        seen = rng.random(num_pts) < 0.7
+        for j in np.flatnonzero(seen):
+            obs.append((c, j, u[j, 0], u[j, 1]))
+    obs = np.array(obs)
+    return BALProblem(np.array(cams), pts, obs[:, 0].astype(np.int32), obs[:, 1].astype(np.int32),
+                      obs[:, 2:4].copy())

@@ -11,7 +11,10 @@
 //                      symmetric block matrix (linear_solver.h:47-145) with
 //                      LinearSolverPCG semantics (linear_solver_pcg.h:45-50);
 //   PoseGraphSolverHIP - the same Solver contract for VertexSE3 / EdgeSE3
-//                      graphs without Schur.
+//                      graphs without Schur;
+//   GraphIO          - OptimizableGraph::load / save (optimizable_graph.cpp:
+//                      344-567) for the path's vertex and edge types, and the
+//                      BAL parser of bal_example.cpp:265-335.
 // As in g2o, calls return bool and never throw; lastError() gives the reason.
 // The classes hold no Eigen or g2o types, so they build without either; the
 // adapter that derives from g2o::Solver and forwards to them is shown in the
@@ -212,6 +215,75 @@ class PoseGraphSolverHIP {
   DeviceContext _dev;
   g2ohip_options _opt{};
   size_t _size = 0;
+};
+
+// g2o text files and BAL files through the library's native graph container.
+// Arrays come out in the layout BlockSolverHIP / PoseGraphSolverHIP take:
+// vertices of one type sorted by id, edges in file order.
+class GraphIO {
+ public:
+  struct BA {
+    std::vector<int> camId, pointId, edgeCam, edgePoint;
+    std::vector<double> cams, points, meas, omega;  // omega: O00 O01 O11 per edge
+    std::vector<uint8_t> camFixed, pointFixed;
+  };
+  struct PoseGraph {
+    std::vector<int> poseId, edgeFrom, edgeTo;
+    std::vector<double> poses, meas, information;
+    std::vector<uint8_t> fixed;
+  };
+
+  GraphIO() { _status = g2ohip_graph_create(&_g); }
+  ~GraphIO() { g2ohip_graph_destroy(_g); }
+  GraphIO(const GraphIO&) = delete;
+  GraphIO& operator=(const GraphIO&) = delete;
+
+  bool load(const char* filename, int nthreads = 0) { return ok(g2ohip_graph_load(_g, filename, nthreads)); }
+  bool loadBAL(const char* filename, int nthreads = 0) { return ok(g2ohip_graph_load_bal(_g, filename, nthreads)); }
+  // precision 6 prints as g2o's default ostream; 17 round-trips exactly
+  bool save(const char* filename, int precision = 6) { return ok(g2ohip_graph_save(_g, filename, precision)); }
+  std::string log() const { return _g ? g2ohip_graph_log(_g) : ""; }
+  std::string lastError() const { return _g ? g2ohip_graph_last_error(_g) : "no graph"; }
+  int status() const { return _status; }
+
+  // [cameras, points, projection edges, poses, pose edges, fixed, lines,
+  //  unknown-tag lines, skipped elements, BAL observations]
+  std::vector<long long> info() const {
+    std::vector<long long> v(10, 0);
+    if (_g) g2ohip_graph_info(_g, v.data());
+    return v;
+  }
+
+  bool getBA(BA& o) {
+    const std::vector<long long> n = info();
+    const size_t C = n[0], P = n[1], E = n[2];
+    o.camId.resize(C); o.cams.resize(7 * C); o.camFixed.resize(C);
+    o.pointId.resize(P); o.points.resize(3 * P); o.pointFixed.resize(P);
+    o.edgeCam.resize(E); o.edgePoint.resize(E); o.meas.resize(2 * E); o.omega.resize(3 * E);
+    return ok(g2ohip_graph_get_ba(_g, o.camId.data(), o.cams.data(), o.camFixed.data(), o.pointId.data(),
+                                  o.points.data(), o.pointFixed.data(), o.edgeCam.data(), o.edgePoint.data(),
+                                  o.meas.data(), o.omega.data()));
+  }
+  bool getPoseGraph(PoseGraph& o) {
+    const std::vector<long long> n = info();
+    const size_t N = n[3], E = n[4];
+    o.poseId.resize(N); o.poses.resize(7 * N); o.fixed.resize(N);
+    o.edgeFrom.resize(E); o.edgeTo.resize(E); o.meas.resize(7 * E); o.information.resize(36 * E);
+    return ok(g2ohip_graph_get_pg(_g, o.poseId.data(), o.poses.data(), o.fixed.data(), o.edgeFrom.data(),
+                                  o.edgeTo.data(), o.meas.data(), o.information.data()));
+  }
+  bool setBAEstimates(const double* cams, const double* points) {
+    return ok(g2ohip_graph_set_ba_state(_g, cams, points));
+  }
+  bool setPoseEstimates(const double* poses) { return ok(g2ohip_graph_set_pg_state(_g, poses)); }
+
+ private:
+  bool ok(int status) {
+    _status = status;
+    return status == G2OHIP_OK;
+  }
+  g2ohip_graph* _g = nullptr;
+  int _status = G2OHIP_OK;
 };
 
 }  // namespace g2o_hip

@@ -10,6 +10,10 @@
 // problem.bin: int32 C, P, E, precision, refine_steps; f64 cams[C*7],
 // intr[C*4], pts[P*3]; int32 edge_cam[E], edge_pt[E]; f64 meas[E*2],
 // weight[E]; u8 cam_fixed[C], pt_fixed[P]; f64 huber.
+//   or: lm_drive <graph.g2o> <solver|optimize> <iterations> <device>
+//                fx fy cx cy [out.g2o]
+// reads a g2o file (the intrinsics are not part of the format), runs FP64 LM
+// and, as g2o's command-line tool does with -o, saves the optimised graph.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -76,6 +80,68 @@ int run_solver_loop(g2o_hip::BlockSolverHIP& s, int iterations) {
   return 0;
 }
 
+int run(g2o_hip::BlockSolverHIP& s, const char* mode, int iters) {
+  if (std::string(mode) == "solver") {
+    if (run_solver_loop(s, iters)) {
+      std::fprintf(stderr, "solver loop failed: %s\n", s.lastError().c_str());
+      return 1;
+    }
+  } else {
+    std::vector<g2ohip_iter_stats> st;
+    if (s.optimize(iters, &st) < 0) {
+      std::fprintf(stderr, "optimize failed: %s\n", s.lastError().c_str());
+      return 1;
+    }
+    for (size_t i = 0; i < st.size(); ++i)
+      std::printf("%zu %.17g %.17g %.17g %d %d\n", i, st[i].chi2_before, st[i].chi2, st[i].lambda,
+                  st[i].levenberg_iterations, st[i].iterations_linear_solver);
+  }
+  return 0;
+}
+
+int run_g2o_file(int argc, char** argv) {
+  if (argc < 9) {
+    std::fprintf(stderr, "usage: %s graph.g2o solver|optimize iterations device fx fy cx cy [out.g2o]\n", argv[0]);
+    return 2;
+  }
+  g2o_hip::GraphIO io;
+  g2o_hip::GraphIO::BA ba;
+  if (!io.load(argv[1]) || !io.getBA(ba)) {
+    std::fprintf(stderr, "load failed: %s\n", io.lastError().c_str());
+    return 1;
+  }
+  const std::string log = io.log();
+  if (!log.empty()) std::fprintf(stderr, "%s", log.c_str());
+  const int C = (int)ba.camId.size(), P = (int)ba.pointId.size(), E = (int)ba.edgeCam.size();
+  std::vector<double> w(E), intr(4 * static_cast<size_t>(C));
+  for (int e = 0; e < E; ++e) {
+    const double* o = &ba.omega[3 * static_cast<size_t>(e)];
+    if (o[1] != 0.0 || o[0] != o[2]) {
+      std::fprintf(stderr, "edge %d: information matrix is not w * I\n", e);
+      return 1;
+    }
+    w[e] = o[0];
+  }
+  for (int c = 0; c < C; ++c)
+    for (int k = 0; k < 4; ++k) intr[4 * static_cast<size_t>(c) + k] = std::atof(argv[5 + k]);
+  g2o_hip::BlockSolverHIP s(std::atoi(argv[4]));
+  if (!s.setGraph(C, P, E, ba.edgeCam.data(), ba.edgePoint.data(), ba.camFixed.data(), ba.pointFixed.data()) ||
+      !s.setMeasurements(ba.meas.data(), w.data(), intr.data(), 0.0) ||
+      !s.setEstimates(ba.cams.data(), ba.points.data())) {
+    std::fprintf(stderr, "setup failed: %s\n", s.lastError().c_str());
+    return 1;
+  }
+  if (run(s, argv[2], std::atoi(argv[3]))) return 1;
+  if (argc > 9) {
+    if (!s.getEstimates(ba.cams.data(), ba.points.data()) || !io.setBAEstimates(ba.cams.data(), ba.points.data()) ||
+        !io.save(argv[9], 17)) {
+      std::fprintf(stderr, "save failed: %s %s\n", s.lastError().c_str(), io.lastError().c_str());
+      return 1;
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -83,6 +149,8 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "usage: %s problem.bin solver|optimize iterations device\n", argv[0]);
     return 2;
   }
+  const std::string in(argv[1]);
+  if (in.size() > 4 && in.compare(in.size() - 4, 4, ".g2o") == 0) return run_g2o_file(argc, argv);
   FILE* f = std::fopen(argv[1], "rb");
   if (!f) return 2;
   int hdr[5];
@@ -107,21 +175,5 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "setup failed: %s\n", s.lastError().c_str());
     return 1;
   }
-  const int iters = std::atoi(argv[3]);
-  if (std::string(argv[2]) == "solver") {
-    if (run_solver_loop(s, iters)) {
-      std::fprintf(stderr, "solver loop failed: %s\n", s.lastError().c_str());
-      return 1;
-    }
-  } else {
-    std::vector<g2ohip_iter_stats> st;
-    if (s.optimize(iters, &st) < 0) {
-      std::fprintf(stderr, "optimize failed: %s\n", s.lastError().c_str());
-      return 1;
-    }
-    for (size_t i = 0; i < st.size(); ++i)
-      std::printf("%zu %.17g %.17g %.17g %d %d\n", i, st[i].chi2_before, st[i].chi2, st[i].lambda,
-                  st[i].levenberg_iterations, st[i].iterations_linear_solver);
-  }
-  return 0;
+  return run(s, argv[2], std::atoi(argv[3]));
 }

@@ -33,7 +33,8 @@ enum g2ohip_status {
   G2OHIP_ERR_OOM = 4,          /* device allocation failed */
   G2OHIP_ERR_HIP = 5,          /* HIP runtime error */
   G2OHIP_ERR_UNSUPPORTED = 6,  /* structure outside the supported limits */
-  G2OHIP_ERR_NO_DEVICE = 7     /* context created without a device */
+  G2OHIP_ERR_NO_DEVICE = 7,    /* context created without a device */
+  G2OHIP_ERR_IO = 8            /* file cannot be opened, read or written */
 };
 
 /* precision of the Hessian blocks, the Schur reduction and the PCG SpMV */
@@ -239,6 +240,96 @@ int g2ohip_pg_get_state(g2ohip_ctx* ctx, double* poses);
  * first, then the pairs): rows/cols [info[2]], blocks [info[2]*36] row-major,
  * b [6 * free poses]. NULL skips. */
 int g2ohip_pg_get_system(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* b);
+
+/* ------------------------------------------------------- g2o file format */
+/* Host-side graph container for the path's vertex and edge types, read from
+ * and written to g2o's text format (OptimizableGraph::load / save,
+ * optimizable_graph.cpp:344-567, 736-780). One line per element: a tag, the
+ * vertex id or the edge's vertex ids, then the values:
+ *   VERTEX_SE3:EXPMAP id  tx ty tz qx qy qz qw   camera-to-world pose (the file
+ *                         holds the inverse of the world-to-camera estimate,
+ *                         vertex_se3_expmap.cpp:35-44)
+ *   VERTEX_TRACKXYZ id  x y z                    (vertex_pointxyz.cpp:40-46)
+ *   EDGE_SE3_PROJECT_XYZ:EXPMAP point camera  u v  O00 O01 O11
+ *                         (edge_project_xyz.cpp:34-42; the information matrix
+ *                         as its upper triangle, base_edge.h:165-180)
+ *   VERTEX_SE3:QUAT id  tx ty tz qx qy qz qw    (vertex_se3.cpp:46-55)
+ *   EDGE_SE3:QUAT from to  tx ty tz qx qy qz qw  21 upper-triangle values
+ *                         (edge_se3.cpp:43-58; the quaternion is normalised)
+ *   FIX id ...            fixes vertices defined earlier in the file
+ * Empty lines and lines whose first token starts with '#' are skipped. As in
+ * the reference, a vertex with a negative or already used id, an edge whose
+ * vertices are not defined yet or repeat, and an unknown tag are reported in
+ * the graph's log and skipped. Stricter than the reference: a line with too
+ * few or malformed values, an edge joining vertices of the wrong types or
+ * naming an unassigned (-1) vertex is skipped too (g2o keeps such elements
+ * partly read). EdgeSE3ProjectXYZ's intrinsics (fx fy cx cy) are not part of
+ * the format; the caller supplies them. */
+typedef struct g2ohip_graph g2ohip_graph;
+
+int g2ohip_graph_create(g2ohip_graph** out);
+void g2ohip_graph_destroy(g2ohip_graph* g);
+/* message of the graph's last failed call */
+const char* g2ohip_graph_last_error(const g2ohip_graph* g);
+/* reports of skipped lines and elements, one per line, as the reference logs
+ * them (G2O_ERROR / G2O_WARN in optimizable_graph.cpp:344-521) */
+const char* g2ohip_graph_log(const g2ohip_graph* g);
+
+/* OptimizableGraph::load(const char*) (optimizable_graph.cpp:344-530): appends
+ * the file's elements to the graph. The text is parsed by nthreads host
+ * threads (<= 0: hardware concurrency, at most 16) and applied in line
+ * order, so the result does not depend on nthreads. */
+int g2ohip_graph_load(g2ohip_graph* g, const char* filename, int nthreads);
+
+/* info[0..9] = cameras (VERTEX_SE3:EXPMAP), points (VERTEX_TRACKXYZ),
+ * projection edges, poses (VERTEX_SE3:QUAT), pose edges, fixed vertices, lines
+ * read, lines with an unknown tag, skipped elements, BAL observations */
+int g2ohip_graph_info(const g2ohip_graph* g, long long* info);
+
+/* Bundle-adjustment part in the layout g2ohip_ba_set_structure /
+ * set_measurements / set_state take: cameras and points each sorted by vertex
+ * id (the order of SparseOptimizer's active vertices,
+ * sparse_optimizer.cpp:175-200), edges in file order with indices into those
+ * arrays. cams[C*7] world-to-camera SE3Quat [t q]; omega[E*3] = O00 O01 O11.
+ * NULL skips an output. */
+int g2ohip_graph_get_ba(const g2ohip_graph* g, int* cam_id, double* cams, uint8_t* cam_fixed, int* pt_id,
+                        double* pts, uint8_t* pt_fixed, int* edge_cam, int* edge_pt, double* meas, double* omega);
+/* Pose-graph part: poses sorted by id, poses[N*7] Isometry3 [t q], edges in
+ * file order, meas[E*7], info[E*36] full symmetric. NULL skips. */
+int g2ohip_graph_get_pg(const g2ohip_graph* g, int* pose_id, double* poses, uint8_t* fixed, int* edge_from,
+                        int* edge_to, double* meas, double* info);
+/* Replace the estimates (same order as the getters), as
+ * SparseOptimizer::optimize leaves them in the vertices before save. */
+int g2ohip_graph_set_ba_state(g2ohip_graph* g, const double* cams, const double* pts);
+int g2ohip_graph_set_pg_state(g2ohip_graph* g, const double* poses);
+
+/* Add elements from arrays (OptimizableGraph::addVertex / addEdge,
+ * optimizable_graph.cpp:179-246), e.g. to write a problem built in memory.
+ * cam_id / pt_id NULL: ids C0.., then points after the cameras, where C0 is
+ * one past the graph's largest id (0 for an empty graph); omega NULL:
+ * identity. A duplicate id fails the call and adds nothing. */
+int g2ohip_graph_add_ba(g2ohip_graph* g, int num_cams, int num_pts, int num_edges, const int* cam_id,
+                        const double* cams, const uint8_t* cam_fixed, const int* pt_id, const double* pts,
+                        const uint8_t* pt_fixed, const int* edge_cam, const int* edge_pt, const double* meas,
+                        const double* omega);
+/* pose_id NULL: ids C0..; info NULL: identity */
+int g2ohip_graph_add_pg(g2ohip_graph* g, int num_poses, int num_edges, const int* pose_id, const double* poses,
+                        const uint8_t* fixed, const int* edge_from, const int* edge_to, const double* meas,
+                        const double* info);
+
+/* OptimizableGraph::save(const char*, level 0) (optimizable_graph.cpp:531-567):
+ * the vertices of the edges, sorted by id, each followed by "FIX id" if
+ * fixed, then the edges in insertion order; each value printed as an ostream
+ * prints a double at the given precision (1..17; g2o's streams use 6). */
+int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision);
+
+/* BAL text format as bal_example.cpp:265-335 parses it: "C P E", E lines
+ * "camera point x y", C cameras of 9 values (angle-axis rotation, translation,
+ * f, k1, k2), P points of 3 values. Replaces the graph's BAL part. */
+int g2ohip_graph_load_bal(g2ohip_graph* g, const char* filename, int nthreads);
+/* info: [0] cameras, [1] points, [2] observations; NULL skips an output */
+int g2ohip_graph_get_bal(const g2ohip_graph* g, long long* info, double* cams9, double* pts, int* edge_cam,
+                         int* edge_pt, double* meas);
 
 /* ---------------------------------------------- adjoint (differentiation) */
 /* Gradient of a scalar loss L(x*) at the converged state with respect to the

@@ -54,3 +54,28 @@ def test_solver_contract_loop_matches_device_optimize(tmp_path, precision, refin
     ba = g2o_amd.BundleAdjuster(prob, device=0, precision=precision, refine_steps=refine)
     st = ba.optimize(8)
     np.testing.assert_allclose([s["chi2"] for s in st], b[:, 2], rtol=1e-5 if fp32 else 1e-10)
+
+
+def test_g2o_file_through_the_cpp_driver(tmp_path):
+    """load (GraphIO) -> BlockSolverHIP LM -> save, as g2o's command-line tool
+    does with a file: the trace equals the Python mirror's on the same file and
+    the saved graph holds the optimised state."""
+    from g2o_amd import graphio as gio
+    prob = datasets.make_small_ba(seed=1000)
+    fx, fy, cx, cy = (float(v) for v in prob.intr[0])
+    assert np.all(prob.intr == prob.intr[0])
+    src = str(tmp_path / "small.g2o")
+    dst = str(tmp_path / "small_opt.g2o")
+    gio.save_ba_problem(src, prob)
+    out = subprocess.run([DRIVER, src, "optimize", "6", "0", repr(fx), repr(fy), repr(cx), repr(cy), dst],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    a = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()])
+    loaded = gio.G2oGraph.load(src).to_ba_problem(prob.intr)
+    ba = g2o_amd.BundleAdjuster(loaded, device=0, precision=g2o_amd.FP64)
+    st = ba.optimize(6)
+    np.testing.assert_allclose(a[:, 2], [s["chi2"] for s in st], rtol=1e-10)
+    cams, pts = ba.state()
+    saved = gio.G2oGraph.load(dst).to_ba_problem(prob.intr)
+    assert np.max(np.abs(saved.cams - cams)) < 1e-12
+    assert np.max(np.abs(saved.pts - pts)) < 1e-9 * np.max(np.abs(pts))
