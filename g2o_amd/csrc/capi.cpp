@@ -240,6 +240,14 @@ int g2ohip_optimize(g2ohip_ctx* ctx, int iterations, g2ohip_iter_stats* stats, i
   return rc;
 }
 
+int g2ohip_set_algorithm(g2ohip_ctx* ctx, int algorithm) {
+  G2OHIP_ENTER(ctx);
+  if (algorithm != G2OHIP_LEVENBERG && algorithm != G2OHIP_GAUSS_NEWTON)
+    return ctx->fail(G2OHIP_ERR_INVALID, "unknown algorithm");
+  ctx->algorithm = algorithm;
+  return G2OHIP_OK;
+}
+
 int g2ohip_set_force_stop_flag(g2ohip_ctx* ctx, const volatile int* flag) {
   G2OHIP_ENTER(ctx);
   ctx->force_stop = flag;

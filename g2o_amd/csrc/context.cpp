@@ -645,6 +645,7 @@ int g2ohip_ctx::discard_top() {
 // OptimizationAlgorithmLevenberg::solve (optimization_algorithm_levenberg.cpp:60-153).
 int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+  if (algorithm == G2OHIP_GAUSS_NEWTON) return optimize_gauss_newton(iterations, stats, done);
   *done = 0;
   pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
   fp32_off = false;
@@ -741,6 +742,47 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
     if (s.status) break;
   }
   // the last pop / copies and any asynchronous error
+  HK(hipStreamSynchronize(stream));
+  return G2OHIP_OK;
+}
+
+// OptimizationAlgorithmGaussNewton::solve (optimization_algorithm_gauss_newton.cpp:
+// 47-90) under SparseOptimizer::optimize: computeActiveErrors, buildSystem,
+// the undamped solve, update; the loop ends after the first failed solve.
+int g2ohip_ctx::optimize_gauss_newton(int iterations, g2ohip_iter_stats* stats, int* done) {
+  *done = 0;
+  pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
+  fp32_off = false;
+  for (int i = 0; i < iterations; ++i) {
+    if (force_stop && *force_stop) break;
+    g2ohip_iter_stats s{};
+    const double t_it = now_s();
+    double t = now_s();
+    CK(build_system(true));
+    s.chi2_before = chi2_lin;
+    s.time_quadratic_form = now_s() - t;
+    lambda = 0.0;
+    t = now_s();
+    int ok = 0;
+    CK(solve(&ok));
+    s.time_linear_solver = now_s() - t;
+    s.iterations_linear_solver = static_cast<int>(last_info[0]);
+    s.refine_steps = static_cast<int>(last_info[1]);
+    s.fp64_fallback = static_cast<int>(last_info[3]);
+    t = now_s();
+    if (ok) CK(update());
+    s.time_update = now_s() - t;
+    // the chi2 SparseOptimizer records after the iteration (batch statistics)
+    t = now_s();
+    CK(compute_active_errors(&s.chi2));
+    s.time_residuals = now_s() - t;
+    s.status = ok ? 0 : 1;
+    s.time_iteration = now_s() - t_it;
+    if (stats) stats[i] = s;
+    *done = i + 1;
+    have_system = false;
+    if (!ok) break;
+  }
   HK(hipStreamSynchronize(stream));
   return G2OHIP_OK;
 }

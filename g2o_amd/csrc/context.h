@@ -84,6 +84,7 @@ struct g2ohip_ctx {
   bool have_solution = false;
   bool have_exact = false;    // the FP64 blocks hold the exact Hessian (adjoint / Newton)
   double lambda = 0.0;
+  int algorithm = 0;  // g2ohip_algorithm
   double pcg_residual_mem = -1.0;  // LinearSolverPCG::_residual
   double last_info[4] = {0, 0, -1, 0};
   // FP32 blocks with FP64 refinement: once the FP32 solution's FP64 residual
@@ -172,6 +173,7 @@ struct g2ohip_ctx {
   int lambda_init(double* out);
   int compute_scale(double* out);
   int optimize(int iterations, g2ohip_iter_stats* stats, int* done);
+  int optimize_gauss_newton(int iterations, g2ohip_iter_stats* stats, int* done);
   int get_hessian(double* Hpp, double* Hll, double* Hpl_edge);
   int get_schur(int* rows, int* cols, double* blocks, double* bschur);
   int linear_solve_pcg(int nb, int bd, int nblk, const int* brow, const int* bcol, const double* blocks,

@@ -78,6 +78,8 @@ class BlockSolverHIP {
     return _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
   }
   const g2ohip_options& options() const { return _opt; }
+  // SparseOptimizer::setAlgorithm: G2OHIP_LEVENBERG or G2OHIP_GAUSS_NEWTON, used by optimize()
+  bool setAlgorithm(int algorithm) { return _dev.ok(g2ohip_set_algorithm(_dev.get(), algorithm)); }
 
   // ---- g2o::Solver
   // init(): the linear solver forgets its PCG residual memory (LinearSolverPCG::init)
@@ -190,6 +192,8 @@ class PoseGraphSolverHIP {
   }
   bool setEstimates(const double* poses7) { return _dev.ok(g2ohip_pg_set_state(_dev.get(), poses7)); }
   bool getEstimates(double* poses7) { return _dev.ok(g2ohip_pg_get_state(_dev.get(), poses7)); }
+  // SparseOptimizer::setAlgorithm: G2OHIP_LEVENBERG or G2OHIP_GAUSS_NEWTON, used by optimize()
+  bool setAlgorithm(int algorithm) { return _dev.ok(g2ohip_set_algorithm(_dev.get(), algorithm)); }
   bool setOptions(const g2ohip_options& opt) {
     _opt = opt;
     return _dev.ok(g2ohip_set_options(_dev.get(), &_opt));

@@ -104,6 +104,10 @@ class BundleAdjuster:
     def set_options(self, **kw):
         return self.ctx.set_options(**kw)
 
+    def set_algorithm(self, algorithm: int):
+        """SparseOptimizer::setAlgorithm: L.LEVENBERG or L.GAUSS_NEWTON."""
+        self.ctx.call("g2ohip_set_algorithm", int(algorithm))
+
     def structure_info(self) -> np.ndarray:
         info = (ct.c_longlong * 13)()
         self.ctx.check(self.ctx._lib.g2ohip_ba_structure_info(self.ctx.h, info))
@@ -275,6 +279,10 @@ class PoseGraphOptimizer:
 
     def set_options(self, **kw):
         return self.ctx.set_options(**kw)
+
+    def set_algorithm(self, algorithm: int):
+        """SparseOptimizer::setAlgorithm: L.LEVENBERG or L.GAUSS_NEWTON."""
+        self.ctx.call("g2ohip_set_algorithm", int(algorithm))
 
     def structure_info(self) -> np.ndarray:
         info = (ct.c_longlong * 4)()

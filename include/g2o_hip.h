@@ -163,10 +163,19 @@ int g2ohip_discard_top(g2ohip_ctx* ctx);
 int g2ohip_lambda_init(g2ohip_ctx* ctx, double* lambda);
 int g2ohip_compute_scale(g2ohip_ctx* ctx, double* scale);
 
-/* SparseOptimizer::optimize(iterations) with OptimizationAlgorithmLevenberg
- * (sparse_optimizer.cpp:393-455, optimization_algorithm_levenberg.cpp:60-153).
- * stats[iterations] may be NULL. *done = iterations run. */
+/* SparseOptimizer::optimize(iterations) (sparse_optimizer.cpp:393-455) with
+ * the context's algorithm: OptimizationAlgorithmLevenberg
+ * (optimization_algorithm_levenberg.cpp:60-153, the default) or
+ * OptimizationAlgorithmGaussNewton (optimization_algorithm_gauss_newton.cpp:
+ * 47-90). stats[iterations] may be NULL. *done = iterations run. */
 int g2ohip_optimize(g2ohip_ctx* ctx, int iterations, g2ohip_iter_stats* stats, int* done);
+/* SparseOptimizer::setAlgorithm for the two algorithms above. Gauss-Newton
+ * builds and solves the undamped system each iteration (lambda 0; stats
+ * report levenberg_iterations 0), updates the state and stops after the first
+ * failed solve (status 1). A failed solve leaves the state unchanged, where
+ * the reference applies whatever the solver left in x. */
+enum g2ohip_algorithm { G2OHIP_LEVENBERG = 0, G2OHIP_GAUSS_NEWTON = 1 };
+int g2ohip_set_algorithm(g2ohip_ctx* ctx, int algorithm);
 /* SparseOptimizer::setForceStopFlag (sparse_optimizer.h:179-183): the flag is
  * read between LM trials; NULL clears it. */
 int g2ohip_set_force_stop_flag(g2ohip_ctx* ctx, const volatile int* flag);

@@ -1291,6 +1291,46 @@ struct PGLM {
 }  // namespace
 
 // =============================================================== C interface
+// SparseOptimizer::optimize(iters) with OptimizationAlgorithmGaussNewton
+// (optimization_algorithm_gauss_newton.cpp:47-90, sparse_optimizer.cpp:
+// 393-453): computeActiveErrors, buildSystem, solve without damping, update,
+// and stop after the first failed solve. A failed solve leaves the state
+// unchanged here (the reference applies whatever x the solver left).
+// trace[iters*4]: chi2 before, chi2 after, pcg iterations, status (1 = Fail).
+template <typename P>
+int gn_optimize(P& p, int iters, double* trace) {
+  int it;
+  for (it = 0; it < iters; ++it) {
+    const double before = p.chi2();
+    p.build();
+    int pcg = 0;
+    const bool ok = p.solve(0.0, &pcg);
+    if (ok) p.update();
+    if (trace) {
+      trace[4 * it + 0] = before;
+      trace[4 * it + 1] = p.chi2();
+      trace[4 * it + 2] = pcg;
+      trace[4 * it + 3] = ok ? 0 : 1;
+    }
+    if (!ok) { ++it; break; }
+  }
+  return it;
+}
+
+struct BAGN {
+  BA& p;
+  double chi2() { return ba_chi2(p); }
+  void build() { ba_build_system(p); }
+  bool solve(double lambda, int* pcg_it) {
+    p.lambda = lambda;
+    const bool ok = ba_solve(p);
+    *pcg_it = p.pcg.last_iters;
+    return ok;
+  }
+  void update() { ba_update(p, p.x.data()); }
+};
+
+
 extern "C" {
 
 void* orc_ba_new(int C, int P, int E, const double* cam7, const double* intr4, const double* pts3,
@@ -1470,6 +1510,16 @@ int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
 
 // OptimizationAlgorithmLevenberg's "initialLambda" property (<= 0: computeLambdaInit).
 void orc_ba_set_user_lambda(void* h, double v) { static_cast<BA*>(h)->user_lambda = v; }
+
+int orc_ba_gauss_newton(void* h, int iters, int solver, double tol, int maxiter, double* trace) {
+  BA* p = static_cast<BA*>(h);
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  BAGN gn{*p};
+  return gn_optimize(gn, iters, trace);
+}
 
 // Replace the measurements / information weights (same structure).
 void orc_ba_set_meas(void* h, const double* meas, const double* w) {
@@ -1699,6 +1749,16 @@ int orc_pg_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
   p->pcg.max_iter = maxiter;
   PGLM lm{*p};
   return lm_optimize(lm, iters, trace, times);
+}
+
+int orc_pg_gauss_newton(void* h, int iters, int solver, double tol, int maxiter, double* trace) {
+  PG* p = static_cast<PG*>(h);
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  PGLM lm{*p};
+  return gn_optimize(lm, iters, trace);
 }
 
 // -------------------------------------------------------- single functions

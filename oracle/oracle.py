@@ -53,6 +53,8 @@ def lib():
         L.orc_ba_solve.argtypes = [ct.c_void_p, ct.c_double, ct.c_int, ct.c_double, ct.c_int, _dp, _ip]
         L.orc_ba_optimize.restype = ct.c_int
         L.orc_ba_optimize.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp, _dp]
+        L.orc_ba_gauss_newton.restype = ct.c_int
+        L.orc_ba_gauss_newton.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp]
         L.orc_se3_exp.argtypes = [_dp, _dp]
         L.orc_se3_log.argtypes = [_dp, _dp]
         L.orc_se3_mul.argtypes = [_dp, _dp, _dp]
@@ -90,6 +92,8 @@ def lib():
         L.orc_pg_linearize.argtypes = [ct.c_void_p, _ip, _ip, _dp, _dp]
         L.orc_pg_optimize.restype = ct.c_int
         L.orc_pg_optimize.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp, _dp]
+        L.orc_pg_gauss_newton.restype = ct.c_int
+        L.orc_pg_gauss_newton.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp]
         L.orc_num_threads.restype = ct.c_int
         L.orc_set_num_threads.argtypes = [ct.c_int]
         _lib = L
@@ -180,6 +184,11 @@ class OracleBA:
                                   times.ctypes.data_as(_dp))
         return n, trace[:n], times
 
+    def gauss_newton(self, iters: int, solver: int = 1, tol: float = 1e-6, maxiter: int = -1):
+        """Rows: chi2 before, chi2 after, pcg iterations, status (1 = Fail)."""
+        trace = np.zeros((iters, 4))
+        n = lib().orc_ba_gauss_newton(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp))
+        return n, trace[:n]
 
     def set_user_lambda(self, lam: float):
         lib().orc_ba_set_user_lambda(self.h, float(lam))
@@ -270,6 +279,12 @@ class OraclePG:
         n = lib().orc_pg_optimize(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp),
                                   times.ctypes.data_as(_dp))
         return n, trace[:n], times
+
+    def gauss_newton(self, iters: int, solver: int = 1, tol: float = 1e-8, maxiter: int = -1):
+        """Rows: chi2 before, chi2 after, pcg iterations, status (1 = Fail)."""
+        trace = np.zeros((iters, 4))
+        n = lib().orc_pg_gauss_newton(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp))
+        return n, trace[:n]
 
 
 def edge_energy_derivs(cam7, X, z, w, intr4, huber_delta):

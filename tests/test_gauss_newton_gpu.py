@@ -1,0 +1,78 @@
+"""OptimizationAlgorithmGaussNewton on the device (g2ohip_set_algorithm)
+against the oracle's Gauss-Newton loop (optimization_algorithm_gauss_newton.cpp:
+47-90): same chi2 before / after per iteration and the same PCG iteration
+counts.
+
+Tolerances: FP64 chi2 relative 1e-8 per iteration (inexact PCG at 1e-6 lets
+rounding differences grow slowly, as for the LM traces); FP32 with FP64
+refinement: final chi2 relative 1e-6 of the FP64 device run.
+"""
+import numpy as np
+import pytest
+
+from g2o_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("case", ["small", "walk"])
+def test_gauss_newton_ba_matches_oracle(case, oracle_lib):
+    import g2o_amd
+    prob = (datasets.make_small_ba(seed=1000) if case == "small"
+            else datasets.make_walk_ba(seed=7, num_cams=60, num_pts=6000))
+    _say(f"GN {case}: C={prob.num_cams} P={prob.num_pts} E={prob.num_edges}")
+    ba = g2o_amd.BundleAdjuster(prob, device=0, precision=g2o_amd.FP64)
+    ba.set_algorithm(g2o_amd.GAUSS_NEWTON)
+    st = ba.optimize(6)
+    n, tr = oracle_lib.OracleBA(prob).gauss_newton(6, solver=1, tol=1e-6)
+    _say(f"gpu {[s['chi2'] for s in st]}")
+    _say(f"cpu {list(tr[:, 1])}")
+    if case == "small":
+        assert len(st) == n
+        common = n
+    else:
+        # Undamped GN on the walk meets a nearly singular reduced system after
+        # two steps: from step 3 on, the PCG solution (tolerance 1e-6) and
+        # whether a preconditioner block's Cholesky pivot stays positive vary
+        # with rounding, between oracle runs (OpenMP sums) as between devices.
+        # Only the first two steps are reproducible, so only they are compared.
+        common = 2
+        assert len(st) >= 2 and n >= 2
+    for s, t in zip(st[:common], tr[:common]):
+        assert abs(s["chi2_before"] - t[0]) <= 1e-8 * t[0]
+        assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
+        assert s["iterations_linear_solver"] == int(t[2])
+        assert s["status"] == int(t[3]) and s["levenberg_iterations"] == 0 and s["lambda_"] == 0.0
+    # FP32 blocks with FP64 refinement reach the same chi2
+    b32 = g2o_amd.BundleAdjuster(prob, device=0, precision=g2o_amd.FP32, refine_steps=1)
+    b32.set_algorithm(g2o_amd.GAUSS_NEWTON)
+    s32 = b32.optimize(common)
+    assert abs(s32[-1]["chi2"] - st[common - 1]["chi2"]) <= 1e-6 * st[common - 1]["chi2"]
+    # switching back to Levenberg-Marquardt on the same context
+    ba.set_algorithm(g2o_amd.LEVENBERG)
+    lm = ba.optimize(2)
+    assert lm[0]["levenberg_iterations"] >= 1 and lm[-1]["chi2"] <= lm[0]["chi2_before"]
+
+
+def test_gauss_newton_pose_graph_matches_oracle(oracle_lib):
+    import g2o_amd
+    pg = datasets.make_sphere_pose_graph(nodes_per_level=100, laps=5)
+    _say(f"GN pose graph: N={pg.num_poses} E={pg.num_edges}")
+    # A PCG tolerance of 1e-8 on r.M^-1 r leaves relative residuals near
+    # 1e-4 on the sphere, so the GN paths would drift apart by rounding (as
+    # the pose-graph LM traces do); solved to 1e-24 both follow the exact
+    # Gauss-Newton iterates.
+    opt = g2o_amd.PoseGraphOptimizer(pg, device=0, pcg_tolerance=1e-24)
+    opt.set_algorithm(g2o_amd.GAUSS_NEWTON)
+    st = opt.optimize(5)
+    n, tr = oracle_lib.OraclePG(pg).gauss_newton(5, solver=1, tol=1e-24)
+    _say(f"gpu {[s['chi2'] for s in st]}")
+    _say(f"cpu {list(tr[:, 1])}")
+    assert len(st) == n
+    for s, t in zip(st, tr):
+        assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
+        assert s["status"] == int(t[3]) == 0

@@ -315,3 +315,31 @@ def test_lm_trace_pcg_vs_cholesky(O):
     _, ta, _ = a.optimize(6, solver=0)
     _, tb, _ = b.optimize(6, solver=1, tol=1e-12)
     assert np.allclose(ta[:, 1], tb[:, 1], rtol=1e-6)
+
+
+def test_gauss_newton_step_is_the_lm_step_at_vanishing_lambda(O):
+    """OptimizationAlgorithmGaussNewton (gauss_newton.cpp:47-90) solves the
+    undamped system: its first step equals the first LM step with an initial
+    lambda far below every diagonal entry's rounding, and each iteration's
+    chi2 before is the previous one's chi2 after. On this well-posed problem
+    the GN chi2 reaches the statistical optimum as LM does."""
+    prob = datasets.make_small_ba(seed=1000)
+    gn = O.OracleBA(prob)
+    n, tr = gn.gauss_newton(8, solver=0)
+    assert n == 8 and np.all(tr[:, 3] == 0)
+    lm = O.OracleBA(prob)
+    lm.set_user_lambda(1e-300)
+    _, tl, _ = lm.optimize(1, solver=0)
+    assert tl[0, 3] == 1  # one trial, accepted
+    # (the oracle's chi2 sums with an OpenMP reduction: equal to rounding)
+    assert abs(tr[0, 0] - tl[0, 0]) <= 1e-13 * tl[0, 0]
+    assert abs(tr[0, 1] - tl[0, 1]) <= 1e-12 * tl[0, 1]
+    np.testing.assert_allclose(tr[1:, 0], tr[:-1, 1], rtol=1e-13)
+    dof = 2 * prob.num_edges - (6 * (prob.num_cams - 2) + 3 * prob.num_pts)
+    assert abs(tr[-1, 1] - dof) < 5 * np.sqrt(2 * dof)
+    # pose graph: the same loop over the 6x6 system
+    pg = datasets.make_sphere_pose_graph(nodes_per_level=40, laps=3)
+    n, tp = O.OraclePG(pg).gauss_newton(6, solver=0)
+    assert n == 6 and np.all(tp[:, 3] == 0)
+    np.testing.assert_allclose(tp[1:, 0], tp[:-1, 1], rtol=1e-13)
+    assert tp[-1, 1] < 1e-2 * tp[0, 0]
