@@ -240,6 +240,12 @@ int g2ohip_optimize(g2ohip_ctx* ctx, int iterations, g2ohip_iter_stats* stats, i
   return rc;
 }
 
+int g2ohip_compute_marginals(g2ohip_ctx* ctx, int nblocks, const int* rows, const int* cols, double* blocks,
+                             double tol) {
+  G2OHIP_ENTER(ctx);
+  return ctx->marginals(nblocks, rows, cols, blocks, tol);
+}
+
 int g2ohip_set_algorithm(g2ohip_ctx* ctx, int algorithm) {
   G2OHIP_ENTER(ctx);
   if (algorithm != G2OHIP_LEVENBERG && algorithm != G2OHIP_GAUSS_NEWTON)

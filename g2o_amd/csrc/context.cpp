@@ -165,7 +165,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   }
   have_structure = true;
   mode = 0;
-  have_meas = have_state = have_system = have_solution = have_exact = false;
+  have_meas = have_state = have_system = have_solution = have_exact = have_lin = false;
   win_ready[0] = win_ready[1] = false;
   pcg_residual_mem = -1.0;
   stack_depth = 0;
@@ -306,7 +306,7 @@ int g2ohip_ctx::set_measurements(const double* meas, const double* w, const doub
   g.intr = d_intr.p;
   g.huber = huber;
   have_meas = true;
-  have_system = false;
+  have_system = have_lin = false;
   return G2OHIP_OK;
 }
 
@@ -390,7 +390,7 @@ int g2ohip_ctx::build_system(bool read_chi) {
   HK(launch_reduce_hpp(g, d_hpp_part.p, d_Hpp.p, d_b.p, stream));
   HK(launch_sum(d_chunk_chi.p, st.nlc, d_scal.p + 1, stream));
   if (read_chi) CK(read_scalar(d_scal.p + 1, &chi2_lin));
-  have_system = true;
+  have_system = have_lin = true;
   have_exact = false;
   system_prec = prec;
   have_solution = false;
@@ -787,6 +787,44 @@ int g2ohip_ctx::optimize_gauss_newton(int iterations, g2ohip_iter_stats* stats, 
   return G2OHIP_OK;
 }
 
+// BlockSolver::computeMarginals (block_solver.hpp:479-490): the requested
+// blocks of Hpp^-1, Hpp as the last buildSystem left it (undamped; after
+// optimize that is the last iteration's linearisation). Projection edges join
+// one camera and one point, so Hpp is block diagonal: a diagonal block is the
+// inverse of that camera's 6x6 block and every other block is zero. Pose
+// graphs: Hpp is the whole Hessian (pg_marginals).
+int g2ohip_ctx::marginals(int nblk, const int* rows, const int* cols, double* out, double tol) {
+  if (!have_lin) return fail(G2OHIP_ERR_INVALID, "no linearised system: build_system or optimize first");
+  const int nb = mode == 0 ? st.Cf : pgst.Nf;
+  if (nblk < 0 || (nblk > 0 && (!rows || !cols || !out))) return fail(G2OHIP_ERR_INVALID, "bad block list");
+  for (int k = 0; k < nblk; ++k)
+    if (rows[k] < 0 || rows[k] >= nb || cols[k] < 0 || cols[k] >= nb)
+      return fail(G2OHIP_ERR_INVALID, "block index out of range");
+  if (nblk == 0) return G2OHIP_OK;
+  if (mode == 1) return pg_marginals(nblk, rows, cols, out, tol);
+  g2ohip::DBuf<int> idx;
+  g2ohip::DBuf<double> inv;
+  std::vector<int> id(nb);
+  for (int i = 0; i < nb; ++i) id[i] = i;
+  HK(idx.upload(id, stream));
+  HK(inv.alloc(36 * static_cast<size_t>(nb)));
+  HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
+  HK((launch_jacobi<double, 6>(nb, d_Hpp.p, idx.p, inv.p, d_err.p, stream)));
+  int err = 0;
+  CK(read_err(&err));
+  if (err) return fail(G2OHIP_ERR_NOT_PD, "a camera block of Hpp is not positive definite");
+  std::vector<double> h(36 * static_cast<size_t>(nb));
+  HK(hipMemcpy(h.data(), inv.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < nblk; ++k) {
+    double* o = out + 36 * static_cast<size_t>(k);
+    if (rows[k] == cols[k])
+      std::memcpy(o, &h[36 * static_cast<size_t>(rows[k])], 36 * sizeof(double));
+    else
+      std::fill(o, o + 36, 0.0);
+  }
+  return G2OHIP_OK;
+}
+
 // --------------------------------------------------------------- inspection
 int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
   if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
@@ -931,7 +969,7 @@ int g2ohip_ctx::exact_system() {
   HK(f64.Hpl.alloc(18 * static_cast<size_t>(std::max(st.nq, 1))));
   HK(launch_linearize_exact(g, d_camRt.p, d_pts.p, f64.Hpl.p, d_Hll.p, d_hpp_part.p, stream));
   HK(launch_reduce_hpp(g, d_hpp_part.p, d_Hpp.p, nullptr, stream));
-  have_system = false;  // the blocks no longer hold the Gauss-Newton system
+  have_system = have_lin = false;  // the blocks no longer hold the Gauss-Newton system
   have_exact = true;
   return G2OHIP_OK;
 }
@@ -1034,7 +1072,7 @@ int g2ohip_ctx::newton(int steps, double tol, double max_step, int* ok) {
     mu = (mu / 10 < mu_min) ? 0.0 : mu / 10;
   }
   HK(hipStreamSynchronize(stream));
-  have_system = false;
+  have_system = have_lin = false;
   have_solution = false;
   return G2OHIP_OK;
 }

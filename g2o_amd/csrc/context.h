@@ -80,6 +80,8 @@ struct g2ohip_ctx {
   int mode = 0;  // 0: bundle adjustment, 1: 3D pose graph (the last *_set_structure)
   bool have_structure = false, have_meas = false, have_state = false;
   bool have_system = false;   // build_system ran since the last state change
+  bool have_lin = false;      // the blocks hold a Gauss-Newton system (kept across state updates,
+                              // as the reference's solver keeps its last Hpp after optimize)
   int system_prec = 0;        // precision of the last build_system
   bool have_solution = false;
   bool have_exact = false;    // the FP64 blocks hold the exact Hessian (adjoint / Newton)
@@ -194,6 +196,8 @@ struct g2ohip_ctx {
   int pg_chi2(double* chi2, double* err_out);
   int pg_build();
   int pg_solve(int* ok);
+  int pg_marginals(int nblk, const int* rows, const int* cols, double* out, double tol);
+  int marginals(int nblk, const int* rows, const int* cols, double* out, double tol);
   int pg_update();
   int pg_lambda_init(double* out);
   int pg_get_system(int* rows, int* cols, double* blocks, double* b);

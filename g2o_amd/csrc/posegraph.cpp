@@ -20,7 +20,7 @@ int g2ohip_ctx::pg_set_structure(int N, int E, const int* from, const int* to, c
   }
   mode = 1;
   have_structure = true;
-  have_meas = have_state = have_system = have_solution = have_exact = false;
+  have_meas = have_state = have_system = have_solution = have_exact = have_lin = false;
   pcg_residual_mem = -1.0;
   stack_depth = 0;
   if (device < 0) return G2OHIP_OK;
@@ -139,7 +139,7 @@ int g2ohip_ctx::pg_build() {
   }
   HK(launch_sum(d_pg_part.p, pg_edge_blocks(A), d_scal.p + 1, stream));
   CK(read_scalar(d_scal.p + 1, &chi2_lin));
-  have_system = true;
+  have_system = have_lin = true;
   have_exact = false;
   system_prec = G2OHIP_FP64;
   have_solution = false;
@@ -192,6 +192,62 @@ int g2ohip_ctx::pg_solve(int* ok) {
   last_info[1] = 0;
   last_info[2] = pcg_residual_mem;
   have_solution = true;
+  return G2OHIP_OK;
+}
+
+// Blocks of H^-1 for a pose graph, whose Hpp is the whole Hessian
+// (block_solver.hpp:479-490; the reference's Cholesky-based linear solvers
+// compute them from the factor, marginal_covariance_cholesky.h:42-80). Here
+// each distinct block column c is found by six block-Jacobi PCG solves
+// H X = e_(6c+j) on the undamped system, stopped at r.M^-1 r <= tol * r0.M^-1 r0;
+// the requested rows of X are the blocks.
+int g2ohip_ctx::pg_marginals(int nblk, const int* rows, const int* cols, double* out, double tol) {
+  const int nb = pgst.Nf;
+  const int n = 6 * nb;
+  PrecBufs<double>& B = f64;
+  HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
+  HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
+  HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.d2.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
+  HK(launch_pg_damp(pg, d_pg_Hdiag.p, 0.0, d_pg_F_val.p, stream));  // the next solve damps again
+  HK((launch_jacobi<double, 6>(nb, d_pg_F_val.p, d_pg_F_diag.p, B.Jinv.p, d_err.p, stream)));
+  int err = 0;
+  CK(read_err(&err));
+  if (err) return fail(G2OHIP_ERR_NOT_PD, "a diagonal block of H is not positive definite");
+  const PcgControl ctl = with_progress(PcgControl{tol > 0 ? tol : 1e-24, -1.0, -1.0});
+  double* const dirs[2] = {B.d.p, B.d2.p};
+  auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
+    return launch_pcg_dir_spmv<double, 6>(nb, d_pg_F_ptr.p, d_pg_F_col.p, d_pg_F_val.p, B.s.p, d_old, d, B.q.p,
+                                          d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream);
+  };
+  g2ohip::DBuf<double> rhs;
+  HK(rhs.alloc(n));
+  std::vector<double> e(n, 0.0), x(n);
+  std::vector<int> ucols(cols, cols + nblk);
+  std::sort(ucols.begin(), ucols.end());
+  ucols.erase(std::unique(ucols.begin(), ucols.end()), ucols.end());
+  for (int c : ucols) {
+    for (int j = 0; j < 6; ++j) {
+      e[6 * static_cast<size_t>(c) + j] = 1.0;
+      HK(hipMemcpyAsync(rhs.p, e.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
+      e[6 * static_cast<size_t>(c) + j] = 0.0;
+      PcgRun pr;
+      {
+        ProfScope ps(this, G2OHIP_K_PCG);
+        bool prog_failed = false;
+        HK((pcg_drive<double, 6>(stream, h_pinned, progress(), nb, rhs.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
+                                 d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, n, op, &pr, &prog_failed)));
+        if (prog_failed) prog_ok = false;
+      }
+      HK(hipMemcpyAsync(x.data(), B.x.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HK(hipStreamSynchronize(stream));
+      for (int k = 0; k < nblk; ++k) {
+        if (cols[k] != c) continue;
+        for (int i = 0; i < 6; ++i)
+          out[36 * static_cast<size_t>(k) + 6 * i + j] = x[6 * static_cast<size_t>(rows[k]) + i];
+      }
+    }
+  }
+  have_solution = false;
   return G2OHIP_OK;
 }
 

@@ -80,6 +80,10 @@ class BlockSolverHIP {
   const g2ohip_options& options() const { return _opt; }
   // SparseOptimizer::setAlgorithm: G2OHIP_LEVENBERG or G2OHIP_GAUSS_NEWTON, used by optimize()
   bool setAlgorithm(int algorithm) { return _dev.ok(g2ohip_set_algorithm(_dev.get(), algorithm)); }
+  // Solver::computeMarginals: blocks (rows[k], cols[k]) of Hpp^-1 into blocks[36 * n]
+  bool computeMarginals(int n, const int* rows, const int* cols, double* blocks, double tol = 0.0) {
+    return _dev.ok(g2ohip_compute_marginals(_dev.get(), n, rows, cols, blocks, tol));
+  }
 
   // ---- g2o::Solver
   // init(): the linear solver forgets its PCG residual memory (LinearSolverPCG::init)
@@ -194,6 +198,10 @@ class PoseGraphSolverHIP {
   bool getEstimates(double* poses7) { return _dev.ok(g2ohip_pg_get_state(_dev.get(), poses7)); }
   // SparseOptimizer::setAlgorithm: G2OHIP_LEVENBERG or G2OHIP_GAUSS_NEWTON, used by optimize()
   bool setAlgorithm(int algorithm) { return _dev.ok(g2ohip_set_algorithm(_dev.get(), algorithm)); }
+  // Solver::computeMarginals: blocks (rows[k], cols[k]) of Hpp^-1 into blocks[36 * n]
+  bool computeMarginals(int n, const int* rows, const int* cols, double* blocks, double tol = 0.0) {
+    return _dev.ok(g2ohip_compute_marginals(_dev.get(), n, rows, cols, blocks, tol));
+  }
   bool setOptions(const g2ohip_options& opt) {
     _opt = opt;
     return _dev.ok(g2ohip_set_options(_dev.get(), &_opt));

@@ -77,6 +77,17 @@ class Context:
         return o
 
 
+def _marginals(ctx, blocks, tol):
+    """SparseOptimizer::computeMarginals for the (row, col) pairs in blocks
+    (free-vertex indices); returns float64[n, 6, 6]."""
+    bl = np.asarray(blocks, dtype=np.int32).reshape(-1, 2)
+    r = np.ascontiguousarray(bl[:, 0])
+    c = np.ascontiguousarray(bl[:, 1])
+    out = np.zeros((bl.shape[0], 6, 6))
+    ctx.call("g2ohip_compute_marginals", int(bl.shape[0]), _ip(r), _ip(c), _dp(out), float(tol))
+    return out
+
+
 class BundleAdjuster:
     """SparseOptimizer with OptimizationAlgorithmLevenberg and a BlockSolverHIP
     for one bundle-adjustment problem (``g2o_amd.datasets.BAProblem``)."""
@@ -107,6 +118,10 @@ class BundleAdjuster:
     def set_algorithm(self, algorithm: int):
         """SparseOptimizer::setAlgorithm: L.LEVENBERG or L.GAUSS_NEWTON."""
         self.ctx.call("g2ohip_set_algorithm", int(algorithm))
+
+    def compute_marginals(self, blocks, tol: float = 0.0) -> np.ndarray:
+        """Blocks of Hpp^-1 (see g2ohip_compute_marginals)."""
+        return _marginals(self.ctx, blocks, tol)
 
     def structure_info(self) -> np.ndarray:
         info = (ct.c_longlong * 13)()
@@ -283,6 +298,10 @@ class PoseGraphOptimizer:
     def set_algorithm(self, algorithm: int):
         """SparseOptimizer::setAlgorithm: L.LEVENBERG or L.GAUSS_NEWTON."""
         self.ctx.call("g2ohip_set_algorithm", int(algorithm))
+
+    def compute_marginals(self, blocks, tol: float = 0.0) -> np.ndarray:
+        """Blocks of Hpp^-1 (see g2ohip_compute_marginals)."""
+        return _marginals(self.ctx, blocks, tol)
 
     def structure_info(self) -> np.ndarray:
         info = (ct.c_longlong * 4)()

@@ -180,6 +180,23 @@ int g2ohip_set_algorithm(g2ohip_ctx* ctx, int algorithm);
  * read between LM trials; NULL clears it. */
 int g2ohip_set_force_stop_flag(g2ohip_ctx* ctx, const volatile int* flag);
 
+/* SparseOptimizer::computeMarginals / BlockSolver::computeMarginals
+ * (block_solver.hpp:479-490, g2o.cpp:620-649): the requested blocks
+ * (rows[k], cols[k]) of Hpp^-1, in free-vertex (hessianIndex) numbering, into
+ * blocks[nblocks*36] (row-major 6x6), from the system the last build_system
+ * or optimize linearised (undamped). Bundle adjustment: Hpp is the camera
+ * block of the Hessian, block diagonal for projection edges, so a diagonal
+ * block is the inverse of that camera's Hpp block (exact, FP64) and every
+ * other block is zero; this is the reference's result, not the covariance of
+ * the reduced camera system. Pose graphs: Hpp is the whole Hessian, and each
+ * requested block column is found by six FP64 block-Jacobi PCG solves
+ * stopped at r.M^-1 r <= tol * r0.M^-1 r0 (tol <= 0: 1e-24), where the
+ * reference factorises (marginal_covariance_cholesky.h:42-80). The
+ * reference's LinearSolverPCG itself provides no marginals
+ * (linear_solver.h:82-90 returns false). */
+int g2ohip_compute_marginals(g2ohip_ctx* ctx, int nblocks, const int* rows, const int* cols, double* blocks,
+                             double tol);
+
 /* ------------------------------------------------- inspection (FP64 copies) */
 /* Hpp[Cf*36], Hll[Pf*9], Hpl per edge [E*18] (6x3 row-major; zero for edges
  * without a block), all row-major, of the last build_system, or of the last
