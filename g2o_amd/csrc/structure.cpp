@@ -31,14 +31,19 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
     if (edge_cam[k] < 0 || edge_cam[k] >= C) return "edge camera index out of range";
     if (edge_pt[k] < 0 || edge_pt[k] >= P) return "edge point index out of range";
   }
-  // hessianIndex: free vertices in input order (sparse_optimizer.cpp:175-200)
+  // hessianIndex: the free vertices among the active ones, in input order
+  // (sparse_optimizer.cpp:175-200). A vertex is active if one of its edges is
+  // (sparse_optimizer.cpp:217-285: an edge is active unless all its vertices
+  // are fixed), so a free vertex is active if it has any edge.
+  std::vector<uint8_t> cam_used(C, 0), pt_used(P, 0);
+  for (int k = 0; k < E; ++k) cam_used[edge_cam[k]] = pt_used[edge_pt[k]] = 1;
   st.camH.assign(C, -1);
   st.ptH.assign(P, -1);
   for (int c = 0; c < C; ++c)
-    if (!(cam_fixed && cam_fixed[c])) { st.camH[c] = st.Cf++; st.free_cam_raw.push_back(c); }
+    if (!(cam_fixed && cam_fixed[c]) && cam_used[c]) { st.camH[c] = st.Cf++; st.free_cam_raw.push_back(c); }
   for (int p = 0; p < P; ++p)
-    if (!(pt_fixed && pt_fixed[p])) st.ptH[p] = st.Pf++;
-  if (st.Cf == 0) return "all cameras are fixed";
+    if (!(pt_fixed && pt_fixed[p]) && pt_used[p]) st.ptH[p] = st.Pf++;
+  if (st.Cf == 0) return "no free camera has an edge";
   if (static_cast<long long>(st.Cf) * st.Cf >= (1LL << 31)) return "too many free cameras for 32-bit block keys";
 
   // active edges grouped by point
@@ -363,15 +368,19 @@ std::string build_pg_structure(PGStructure& st, int N, int E, const int* from, c
   if (N <= 0 || E < 0) return "invalid pose graph size";
   st.N = N;
   st.E = E;
-  st.H.assign(N, -1);
-  for (int i = 0; i < N; ++i)
-    if (!(fixed && fixed[i])) st.H[i] = st.Nf++;
-  if (st.Nf == 0) return "every pose is fixed";
+  // free poses with an edge are the active ones (sparse_optimizer.cpp:217-285)
+  std::vector<uint8_t> used(N, 0);
   for (int k = 0; k < E; ++k) {
     if (from[k] < 0 || from[k] >= N || to[k] < 0 || to[k] >= N) return "edge vertex index out of range";
     if (from[k] == to[k]) return "edge joins a pose to itself";
-    if (st.H[from[k]] >= 0 || st.H[to[k]] >= 0) st.act.push_back(k);
+    used[from[k]] = used[to[k]] = 1;
   }
+  st.H.assign(N, -1);
+  for (int i = 0; i < N; ++i)
+    if (!(fixed && fixed[i]) && used[i]) st.H[i] = st.Nf++;
+  if (st.Nf == 0) return "no free pose has an edge";
+  for (int k = 0; k < E; ++k)
+    if (st.H[from[k]] >= 0 || st.H[to[k]] >= 0) st.act.push_back(k);
   const int A = static_cast<int>(st.act.size());
   // upper pattern: diagonal, then distinct pairs sorted by (row, col)
   std::vector<std::pair<long long, int>> keys;  // (row * Nf + col, active position)

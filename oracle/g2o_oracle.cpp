@@ -512,8 +512,12 @@ void ba_structure(BA& p) {
   p.camH.assign(p.C, -1);
   p.ptH.assign(p.P, -1);
   p.Cf = p.Pf = 0;
-  for (int c = 0; c < p.C; ++c) if (!p.cfix[c]) p.camH[c] = p.Cf++;
-  for (int i = 0; i < p.P; ++i) if (!p.pfix[i]) p.ptH[i] = p.Pf++;
+  // active vertices: those with an active edge (sparse_optimizer.cpp:217-285);
+  // every edge of a free vertex is active
+  vector<char> cu(p.C, 0), pu(p.P, 0);
+  for (int k = 0; k < p.E; ++k) { cu[p.ecam[k]] = 1; pu[p.ept[k]] = 1; }
+  for (int c = 0; c < p.C; ++c) if (!p.cfix[c] && cu[c]) p.camH[c] = p.Cf++;
+  for (int i = 0; i < p.P; ++i) if (!p.pfix[i] && pu[i]) p.ptH[i] = p.Pf++;
   p.active.clear();
   for (int k = 0; k < p.E; ++k)
     if (!(p.cfix[p.ecam[k]] && p.pfix[p.ept[k]])) p.active.push_back(k);
@@ -1056,8 +1060,10 @@ void pg_edge(const PG& p, int k, double e[6], double* J) {
 void pg_structure(PG& p) {
   p.H.assign(p.N, -1);
   p.Nf = 0;
+  vector<char> used(p.N, 0);
+  for (int k = 0; k < p.E; ++k) used[p.from[k]] = used[p.to[k]] = 1;
   for (int i = 0; i < p.N; ++i)
-    if (!p.fixed[i]) p.H[i] = p.Nf++;
+    if (!p.fixed[i] && used[i]) p.H[i] = p.Nf++;
   p.active.clear();
   for (int k = 0; k < p.E; ++k)
     if (!(p.fixed[p.from[k]] && p.fixed[p.to[k]])) p.active.push_back(k);

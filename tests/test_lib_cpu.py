@@ -97,6 +97,34 @@ def test_fixed_points_and_cameras_are_excluded():
     assert info[2] == prob.num_edges - int(both.sum())
 
 
+def with_isolated_vertices(prob):
+    """The problem plus one free camera and one free point that no edge uses."""
+    return datasets.BAProblem(np.vstack([prob.cams, prob.cams[-1:]]), np.vstack([prob.intr, prob.intr[-1:]]),
+                              np.vstack([prob.pts, prob.pts[-1:]]), prob.edge_cam, prob.edge_pt, prob.meas,
+                              prob.weight, np.r_[prob.cam_fixed, 0].astype(np.uint8),
+                              np.r_[prob.pt_fixed, 0].astype(np.uint8), prob.huber)
+
+
+def test_vertices_without_edges_are_not_active():
+    """initializeOptimization(level) activates a vertex only through an
+    active edge (sparse_optimizer.cpp:217-285), so a free vertex without edges
+    gets no Hessian index; with no edge at all there is nothing to optimise."""
+    prob = datasets.make_small_ba(seed=1)
+    base = g2o_amd.BundleAdjuster(prob, device=-1).structure_info()
+    iso = g2o_amd.BundleAdjuster(with_isolated_vertices(prob), device=-1).structure_info()
+    assert iso[0] == base[0] and iso[1] == base[1] and iso[2] == base[2]
+    empty = datasets.BAProblem(prob.cams, prob.intr, prob.pts, prob.edge_cam[:0], prob.edge_pt[:0],
+                               prob.meas[:0], prob.weight[:0], prob.cam_fixed, prob.pt_fixed, 0.0)
+    with pytest.raises(g2o_amd.G2oHipError):
+        g2o_amd.BundleAdjuster(empty, device=-1)
+    pg = datasets.make_sphere_pose_graph(nodes_per_level=20, laps=2)
+    pgi = datasets.PoseGraphProblem(np.vstack([pg.poses, pg.poses[-1:]]), pg.edge_from, pg.edge_to, pg.meas,
+                                    pg.info, np.r_[pg.fixed, 0].astype(np.uint8))
+    a = g2o_amd.PoseGraphOptimizer(pg, device=-1).structure_info()
+    b = g2o_amd.PoseGraphOptimizer(pgi, device=-1).structure_info()
+    assert np.array_equal(a, b)
+
+
 def test_pose_graph_structure_matches_reference_pattern():
     prob = datasets.make_sphere_pose_graph(seed=3, nodes_per_level=30, laps=4)
     pgo = g2o_amd.PoseGraphOptimizer(prob, device=-1)
