@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "csrc", "libg2o_hip.so")
 OK, ERR_INVALID, ERR_NOT_PD, ERR_NAN, ERR_OOM, ERR_HIP, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_IO = range(9)
 FP64, FP32 = 0, 1
 SCHUR_EXPLICIT, SCHUR_IMPLICIT = 0, 1
-LEVENBERG, GAUSS_NEWTON = 0, 1
+LEVENBERG, GAUSS_NEWTON, DOGLEG = 0, 1, 2
 
 # exported symbols, in the order include/g2o_hip.h declares them
 EXPORTS = [

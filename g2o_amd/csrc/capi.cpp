@@ -248,7 +248,7 @@ int g2ohip_compute_marginals(g2ohip_ctx* ctx, int nblocks, const int* rows, cons
 
 int g2ohip_set_algorithm(g2ohip_ctx* ctx, int algorithm) {
   G2OHIP_ENTER(ctx);
-  if (algorithm != G2OHIP_LEVENBERG && algorithm != G2OHIP_GAUSS_NEWTON)
+  if (algorithm != G2OHIP_LEVENBERG && algorithm != G2OHIP_GAUSS_NEWTON && algorithm != G2OHIP_DOGLEG)
     return ctx->fail(G2OHIP_ERR_INVALID, "unknown algorithm");
   ctx->algorithm = algorithm;
   return G2OHIP_OK;

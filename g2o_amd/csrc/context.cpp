@@ -646,6 +646,7 @@ int g2ohip_ctx::discard_top() {
 int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   if (algorithm == G2OHIP_GAUSS_NEWTON) return optimize_gauss_newton(iterations, stats, done);
+  if (algorithm == G2OHIP_DOGLEG) return optimize_dogleg(iterations, stats, done);
   *done = 0;
   pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
   fp32_off = false;
@@ -782,6 +783,155 @@ int g2ohip_ctx::optimize_gauss_newton(int iterations, g2ohip_iter_stats* stats, 
     *done = i + 1;
     have_system = false;
     if (!ok) break;
+  }
+  HK(hipStreamSynchronize(stream));
+  return G2OHIP_OK;
+}
+
+// OptimizationAlgorithmDogleg::solve (optimization_algorithm_dogleg.cpp:62-213)
+// at its default properties (initialDelta 1e4, maxTrialsAfterFailure 100,
+// initialLambda 1e-7, lambdaFactor 10). Linearisation, the (damped) solve,
+// the state update and chi2 run on the device; the step algebra over the
+// n-vectors b, h_gn, h_sd runs here in FP64. multiplyHessian is BlockSolver's
+// (block_solver.h:149-151), the product with Hpp alone: block diagonal for
+// bundle adjustment, with a zero landmark part as in the reference; the whole
+// Hessian for pose graphs. The Hpp blocks are copied once per iteration, right
+// after the linearisation (undamped).
+int g2ohip_ctx::optimize_dogleg(int iterations, g2ohip_iter_stats* stats, int* done) {
+  *done = 0;
+  pcg_residual_mem = -1.0;
+  fp32_off = false;
+  double delta = 1e4, lam = 1e-7;
+  const double lf = 10.0;
+  const int max_tries = 100;
+  bool wasPD = true;
+  const size_t n = system_size();
+  std::vector<double> b(n), aux(n), hsd(n), hdl(n), hgn(n), H;
+  auto dot = [n](const std::vector<double>& x, const std::vector<double>& y) {
+    double t = 0;
+    for (size_t i = 0; i < n; ++i) t += x[i] * y[i];
+    return t;
+  };
+  auto hess_mul = [&](const std::vector<double>& v, std::vector<double>& out) {
+    std::fill(out.begin(), out.end(), 0.0);
+    if (mode == 0) {
+      for (int c = 0; c < st.Cf; ++c)
+        for (int r = 0; r < 6; ++r) {
+          double t = 0;
+          for (int k = 0; k < 6; ++k) t += H[36 * static_cast<size_t>(c) + 6 * r + k] * v[6 * static_cast<size_t>(c) + k];
+          out[6 * static_cast<size_t>(c) + r] = t;
+        }
+    } else {
+      for (int r = 0; r < pgst.Nf; ++r)
+        for (int k = pgst.F_ptr[r]; k < pgst.F_ptr[r + 1]; ++k) {
+          const int c = pgst.F_col[k];
+          const double* B = &H[36 * static_cast<size_t>(k)];
+          for (int i = 0; i < 6; ++i) {
+            double t = 0;
+            for (int j = 0; j < 6; ++j) t += B[6 * i + j] * v[6 * static_cast<size_t>(c) + j];
+            out[6 * static_cast<size_t>(r) + i] += t;
+          }
+        }
+    }
+  };
+  for (int it = 0; it < iterations; ++it) {
+    if (force_stop && *force_stop) break;
+    g2ohip_iter_stats s{};
+    const double t_it = now_s();
+    double t = now_s();
+    CK(build_system(true));
+    const double currentChi = chi2_lin;
+    s.time_quadratic_form = now_s() - t;
+    HK(hipMemcpy(b.data(), d_b.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (mode == 0) {
+      H.resize(36 * static_cast<size_t>(st.Cf));
+      HK(hipMemcpy(H.data(), d_Hpp.p, H.size() * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      H.resize(36 * static_cast<size_t>(pgst.nF));
+      HK(hipMemcpy(H.data(), d_pg_F_val.p, H.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    hess_mul(b, aux);
+    const double alpha = dot(b, b) / dot(aux, b);
+    for (size_t i = 0; i < n; ++i) hsd[i] = alpha * b[i];
+    const double hsdNorm = std::sqrt(dot(hsd, hsd));
+    double hgnNorm = -1.0, chiAfter = currentChi;
+    bool solved = false, good = false;
+    int tries = 0, status = 0;
+    do {
+      ++tries;
+      if (!solved) {
+        solved = true;
+        int ok = 0;
+        while (!ok) {
+          lambda = wasPD ? 0.0 : lam;
+          t = now_s();
+          CK(solve(&ok));
+          s.time_linear_solver += now_s() - t;
+          s.iterations_linear_solver += static_cast<int>(last_info[0]);
+          s.refine_steps += static_cast<int>(last_info[1]);
+          s.fp64_fallback |= static_cast<int>(last_info[3]);
+          wasPD = wasPD && ok;
+          if (!wasPD) {
+            if (ok) {
+              lam = std::max(1e-12, lam / (0.5 * lf));
+            } else {
+              lam *= lf;
+              if (lam > 1e3) { lam = 1e3; status = 2; break; }
+            }
+          }
+        }
+        if (status == 2) break;
+        HK(hipMemcpy(hgn.data(), d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        hgnNorm = std::sqrt(dot(hgn, hgn));
+      }
+      if (hgnNorm < delta) {
+        hdl = hgn;
+      } else if (hsdNorm > delta) {
+        for (size_t i = 0; i < n; ++i) hdl[i] = delta / hsdNorm * hsd[i];
+      } else {
+        for (size_t i = 0; i < n; ++i) aux[i] = hgn[i] - hsd[i];
+        const double c = dot(hsd, aux), bma = dot(aux, aux), hsd2 = dot(hsd, hsd);
+        const double disc = std::sqrt(c * c + bma * (delta * delta - hsd2));
+        const double beta = c <= 0 ? (-c + disc) / bma : (delta * delta - hsd2) / (c + disc);
+        for (size_t i = 0; i < n; ++i) hdl[i] = hsd[i] + beta * (hgn[i] - hsd[i]);
+      }
+      hess_mul(hdl, aux);
+      double lin = -dot(aux, hdl) + 2 * dot(b, hdl);
+      t = now_s();
+      CK(push());
+      HK(hipMemcpy(d_x.p, hdl.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      have_solution = true;
+      CK(update());
+      s.time_update += now_s() - t;
+      t = now_s();
+      double newChi = 0;
+      CK(compute_active_errors(&newChi));
+      s.time_residuals += now_s() - t;
+      if (std::fabs(lin) < 1e-12) lin = 1e-12;
+      const double rho = (currentChi - newChi) / lin;
+      if (rho > 0) {
+        CK(discard_top());
+        good = true;
+        chiAfter = newChi;
+      } else {
+        CK(pop());
+      }
+      if (rho > 0.75)
+        delta = std::max(delta, 3 * std::sqrt(dot(hdl, hdl)));
+      else if (rho < 0.25)
+        delta *= 0.5;
+    } while (!good && tries < max_tries && !(force_stop && *force_stop));
+    if (status == 0 && !good) status = 1;
+    s.chi2_before = currentChi;
+    s.chi2 = chiAfter;
+    s.lambda = delta;  // the trust-region radius after the iteration
+    s.levenberg_iterations = tries;
+    s.status = status;
+    s.time_iteration = now_s() - t_it;
+    if (stats) stats[it] = s;
+    *done = it + 1;
+    have_system = false;
+    if (status) break;
   }
   HK(hipStreamSynchronize(stream));
   return G2OHIP_OK;

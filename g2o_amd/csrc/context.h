@@ -176,6 +176,7 @@ struct g2ohip_ctx {
   int compute_scale(double* out);
   int optimize(int iterations, g2ohip_iter_stats* stats, int* done);
   int optimize_gauss_newton(int iterations, g2ohip_iter_stats* stats, int* done);
+  int optimize_dogleg(int iterations, g2ohip_iter_stats* stats, int* done);
   int get_hessian(double* Hpp, double* Hll, double* Hpl_edge);
   int get_schur(int* rows, int* cols, double* blocks, double* bschur);
   int linear_solve_pcg(int nb, int bd, int nblk, const int* brow, const int* bcol, const double* blocks,

@@ -75,7 +75,7 @@ typedef struct g2ohip_iter_stats {
   int levenberg_iterations;    /* inner trials */
   int iterations_linear_solver;/* PCG iterations summed over trials */
   int refine_steps;            /* FP64 refinement corrections summed */
-  int status;                  /* 0 OK, 1 Terminate (levenberg.cpp:149-152) */
+  int status;                  /* 0 OK, 1 Terminate (levenberg.cpp:149-152), 2 Fail */
   double time_residuals;
   double time_quadratic_form;
   double time_schur_complement;
@@ -169,12 +169,19 @@ int g2ohip_compute_scale(g2ohip_ctx* ctx, double* scale);
  * OptimizationAlgorithmGaussNewton (optimization_algorithm_gauss_newton.cpp:
  * 47-90). stats[iterations] may be NULL. *done = iterations run. */
 int g2ohip_optimize(g2ohip_ctx* ctx, int iterations, g2ohip_iter_stats* stats, int* done);
-/* SparseOptimizer::setAlgorithm for the two algorithms above. Gauss-Newton
- * builds and solves the undamped system each iteration (lambda 0; stats
- * report levenberg_iterations 0), updates the state and stops after the first
- * failed solve (status 1). A failed solve leaves the state unchanged, where
- * the reference applies whatever the solver left in x. */
-enum g2ohip_algorithm { G2OHIP_LEVENBERG = 0, G2OHIP_GAUSS_NEWTON = 1 };
+/* SparseOptimizer::setAlgorithm. Gauss-Newton builds and solves the
+ * undamped system each iteration (lambda 0; stats report
+ * levenberg_iterations 0), updates the state and stops after the first failed
+ * solve (status 1). A failed solve leaves the state unchanged, where the
+ * reference applies whatever the solver left in x. Dogleg
+ * (optimization_algorithm_dogleg.cpp:62-213, default properties: initialDelta
+ * 1e4, maxTrialsAfterFailure 100, initialLambda 1e-7, lambdaFactor 10) uses
+ * BlockSolver::multiplyHessian's product with Hpp alone (block_solver.h:
+ * 149-151), whose landmark part is zero for bundle adjustment, as the
+ * reference does; its stats report the trust-region radius in lambda, the
+ * tries in levenberg_iterations, and status 1 Terminate / 2 Fail (damping
+ * limit). */
+enum g2ohip_algorithm { G2OHIP_LEVENBERG = 0, G2OHIP_GAUSS_NEWTON = 1, G2OHIP_DOGLEG = 2 };
 int g2ohip_set_algorithm(g2ohip_ctx* ctx, int algorithm);
 /* SparseOptimizer::setForceStopFlag (sparse_optimizer.h:179-183): the flag is
  * read between LM trials; NULL clears it. */

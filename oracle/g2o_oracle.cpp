@@ -1337,6 +1337,160 @@ struct BAGN {
 };
 
 
+// SparseOptimizer::optimize(iters) with OptimizationAlgorithmDogleg
+// (optimization_algorithm_dogleg.cpp:62-213) at its default properties
+// (initialDelta 1e4, maxTrialsAfterFailure 100, initialLambda 1e-7,
+// lambdaFactor 10). multiplyHessian is BlockSolver's (block_solver.h:149-151):
+// the product with Hpp only, so for bundle adjustment the landmark part of
+// H v is zero, as in the reference.
+// trace[iters*6]: chi2 before, chi2 after, delta after, tries, pcg iterations,
+// status (0 OK, 1 Terminate, 2 Fail).
+template <typename P>
+int dl_optimize(P& p, int iters, double* trace) {
+  double delta = 1e4, lam = 1e-7;
+  const double lf = 10.0;
+  const int max_tries = 100;
+  bool wasPD = true;
+  auto dot = [](const vector<double>& a, const vector<double>& b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+  };
+  int it;
+  for (it = 0; it < iters; ++it) {
+    const double currentChi = p.chi2();
+    p.build();
+    const vector<double> b = p.b();
+    const size_t n = b.size();
+    vector<double> aux(n, 0.0), hsd(n), hdl(n), hgn;
+    p.hess_mul(b, aux);
+    const double alpha = dot(b, b) / dot(aux, b);
+    for (size_t i = 0; i < n; ++i) hsd[i] = alpha * b[i];
+    const double hsdNorm = std::sqrt(dot(hsd, hsd));
+    double hgnNorm = -1.0, chiAfter = currentChi;
+    bool solved = false, good = false;
+    int tries = 0, pcg = 0, status = 0;
+    do {
+      ++tries;
+      if (!solved) {
+        solved = true;
+        bool ok = false;
+        while (!ok) {
+          int pit = 0;
+          ok = p.solve(wasPD ? 0.0 : lam, &pit);
+          pcg += pit;
+          wasPD = wasPD && ok;
+          if (!wasPD) {
+            if (ok) {
+              lam = std::max(1e-12, lam / (0.5 * lf));
+            } else {
+              lam *= lf;
+              if (lam > 1e3) { lam = 1e3; status = 2; break; }
+            }
+          }
+        }
+        if (status == 2) break;
+        hgn = p.x();
+        hgnNorm = std::sqrt(dot(hgn, hgn));
+      }
+      if (hgnNorm < delta) {
+        hdl = hgn;
+      } else if (hsdNorm > delta) {
+        for (size_t i = 0; i < n; ++i) hdl[i] = delta / hsdNorm * hsd[i];
+      } else {
+        for (size_t i = 0; i < n; ++i) aux[i] = hgn[i] - hsd[i];
+        const double c = dot(hsd, aux), bma = dot(aux, aux), hsd2 = dot(hsd, hsd);
+        const double disc = std::sqrt(c * c + bma * (delta * delta - hsd2));
+        const double beta = c <= 0 ? (-c + disc) / bma : (delta * delta - hsd2) / (c + disc);
+        for (size_t i = 0; i < n; ++i) hdl[i] = hsd[i] + beta * (hgn[i] - hsd[i]);
+      }
+      std::fill(aux.begin(), aux.end(), 0.0);
+      p.hess_mul(hdl, aux);
+      double lin = -dot(aux, hdl) + 2 * dot(b, hdl);
+      p.push();
+      p.update(hdl.data());
+      const double newChi = p.chi2();
+      if (std::fabs(lin) < 1e-12) lin = 1e-12;
+      const double rho = (currentChi - newChi) / lin;
+      if (rho > 0) {
+        p.discard();
+        good = true;
+        chiAfter = newChi;
+      } else {
+        p.pop();
+      }
+      if (rho > 0.75)
+        delta = std::max(delta, 3 * std::sqrt(dot(hdl, hdl)));
+      else if (rho < 0.25)
+        delta *= 0.5;
+    } while (!good && tries < max_tries);
+    if (status == 0 && !good) status = 1;
+    if (trace) {
+      double* t = trace + 6 * it;
+      t[0] = currentChi; t[1] = chiAfter; t[2] = delta; t[3] = tries; t[4] = pcg; t[5] = status;
+    }
+    if (status) { ++it; break; }
+  }
+  return it;
+}
+
+struct BADL {
+  BA& p;
+  double chi2() { return ba_chi2(p); }
+  void build() { ba_build_system(p); }
+  vector<double> b() { return p.b; }
+  vector<double> x() { return p.x; }
+  // Hpp v in the pose part (Hpp is block diagonal), zero in the landmark part
+  void hess_mul(const vector<double>& v, vector<double>& out) {
+    for (int c = 0; c < p.Cf; ++c)
+      for (int r = 0; r < 6; ++r) {
+        double s = 0;
+        for (int k = 0; k < 6; ++k) s += p.Hpp[36 * static_cast<size_t>(c) + 6 * r + k] * v[6 * static_cast<size_t>(c) + k];
+        out[6 * static_cast<size_t>(c) + r] = s;
+      }
+  }
+  bool solve(double lambda, int* pcg_it) {
+    p.lambda = lambda;
+    const bool ok = ba_solve(p);
+    *pcg_it = p.pcg.last_iters;
+    return ok;
+  }
+  void update(const double* v) { ba_update(p, v); }
+  void push() { ba_push(p); }
+  void pop() { ba_pop(p); }
+  void discard() { ba_discard(p); }
+};
+
+struct PGDL {
+  PG& p;
+  double chi2() { return pg_chi2(p); }
+  void build() { pg_build(p); }
+  vector<double> b() { return p.b; }
+  vector<double> x() { return p.x; }
+  // H v over the upper blocks and their transposes, with the undamped diagonal
+  void hess_mul(const vector<double>& v, vector<double>& out) {
+    for (size_t k = 0; k < p.S.row.size(); ++k) {
+      const int r = p.S.row[k], c = p.S.col[k];
+      const double* B = r == c ? &p.Hdiag[36 * static_cast<size_t>(r)] : &p.S.val[36 * k];
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+          out[6 * static_cast<size_t>(r) + i] += B[6 * i + j] * v[6 * static_cast<size_t>(c) + j];
+          if (r != c) out[6 * static_cast<size_t>(c) + j] += B[6 * i + j] * v[6 * static_cast<size_t>(r) + i];
+        }
+    }
+  }
+  bool solve(double lambda, int* pcg_it) {
+    p.lambda = lambda;
+    const bool ok = pg_solve(p);
+    *pcg_it = p.solver_kind == 1 ? p.pcg.last_iters : 0;
+    return ok;
+  }
+  void update(const double* v) { pg_update(p, v); }
+  void push() { p.stackT.push_back(p.t); p.stackQ.push_back(p.q); }
+  void pop() { p.t = p.stackT.back(); p.q = p.stackQ.back(); discard(); }
+  void discard() { p.stackT.pop_back(); p.stackQ.pop_back(); }
+};
+
 extern "C" {
 
 void* orc_ba_new(int C, int P, int E, const double* cam7, const double* intr4, const double* pts3,
@@ -1826,6 +1980,26 @@ void orc_set_num_threads(int n) {
 #else
   (void)n;
 #endif
+}
+
+int orc_ba_dogleg(void* h, int iters, int solver, double tol, int maxiter, double* trace) {
+  BA* p = static_cast<BA*>(h);
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  BADL dl{*p};
+  return dl_optimize(dl, iters, trace);
+}
+
+int orc_pg_dogleg(void* h, int iters, int solver, double tol, int maxiter, double* trace) {
+  PG* p = static_cast<PG*>(h);
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  PGDL dl{*p};
+  return dl_optimize(dl, iters, trace);
 }
 
 }  // extern "C"

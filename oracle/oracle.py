@@ -92,6 +92,10 @@ def lib():
         L.orc_pg_linearize.argtypes = [ct.c_void_p, _ip, _ip, _dp, _dp]
         L.orc_pg_optimize.restype = ct.c_int
         L.orc_pg_optimize.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp, _dp]
+        L.orc_ba_dogleg.restype = ct.c_int
+        L.orc_ba_dogleg.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp]
+        L.orc_pg_dogleg.restype = ct.c_int
+        L.orc_pg_dogleg.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp]
         L.orc_pg_gauss_newton.restype = ct.c_int
         L.orc_pg_gauss_newton.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp]
         L.orc_num_threads.restype = ct.c_int
@@ -190,6 +194,12 @@ class OracleBA:
         n = lib().orc_ba_gauss_newton(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp))
         return n, trace[:n]
 
+    def dogleg(self, iters: int, solver: int = 1, tol: float = 1e-6, maxiter: int = -1):
+        """Rows: chi2 before, chi2 after, delta, tries, pcg iterations, status."""
+        trace = np.zeros((iters, 6))
+        n = lib().orc_ba_dogleg(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp))
+        return n, trace[:n]
+
     def set_user_lambda(self, lam: float):
         lib().orc_ba_set_user_lambda(self.h, float(lam))
 
@@ -284,6 +294,12 @@ class OraclePG:
         """Rows: chi2 before, chi2 after, pcg iterations, status (1 = Fail)."""
         trace = np.zeros((iters, 4))
         n = lib().orc_pg_gauss_newton(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp))
+        return n, trace[:n]
+
+    def dogleg(self, iters: int, solver: int = 1, tol: float = 1e-8, maxiter: int = -1):
+        """Rows: chi2 before, chi2 after, delta, tries, pcg iterations, status."""
+        trace = np.zeros((iters, 6))
+        n = lib().orc_pg_dogleg(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp))
         return n, trace[:n]
 
 

@@ -1,7 +1,8 @@
-"""OptimizationAlgorithmGaussNewton on the device (g2ohip_set_algorithm)
-against the oracle's Gauss-Newton loop (optimization_algorithm_gauss_newton.cpp:
-47-90): same chi2 before / after per iteration and the same PCG iteration
-counts.
+"""OptimizationAlgorithmGaussNewton and OptimizationAlgorithmDogleg on the
+device (g2ohip_set_algorithm) against the oracle's loops
+(optimization_algorithm_gauss_newton.cpp:47-90, optimization_algorithm_dogleg.cpp:
+62-213): same chi2 before / after per iteration, the same PCG iteration
+counts, and for Dogleg the same trust-region radius and tries.
 
 Tolerances: FP64 chi2 relative 1e-8 per iteration (inexact PCG at 1e-6 lets
 rounding differences grow slowly, as for the LM traces); FP32 with FP64
@@ -76,3 +77,39 @@ def test_gauss_newton_pose_graph_matches_oracle(oracle_lib):
     for s, t in zip(st, tr):
         assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
         assert s["status"] == int(t[3]) == 0
+
+
+def test_dogleg_ba_matches_oracle(oracle_lib):
+    """OptimizationAlgorithmDogleg on the device against the oracle's: same
+    chi2, trust-region radius and tries per iteration."""
+    import g2o_amd
+    prob = datasets.make_small_ba(seed=1000)
+    ba = g2o_amd.BundleAdjuster(prob, device=0, precision=g2o_amd.FP64)
+    ba.set_algorithm(g2o_amd.DOGLEG)
+    st = ba.optimize(8)
+    n, tr = oracle_lib.OracleBA(prob).dogleg(8, solver=1, tol=1e-6)
+    _say(f"gpu chi2 {[s['chi2'] for s in st]} delta {[s['lambda_'] for s in st]} tries "
+         f"{[s['levenberg_iterations'] for s in st]}")
+    _say(f"cpu chi2 {list(tr[:, 1])} delta {list(tr[:, 2])} tries {list(tr[:, 3])}")
+    assert len(st) == n
+    for s, t in zip(st, tr):
+        assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
+        assert abs(s["lambda_"] - t[2]) <= 1e-8 * t[2]
+        assert s["levenberg_iterations"] == int(t[3])
+        assert s["iterations_linear_solver"] == int(t[4])
+        assert s["status"] == int(t[5])
+
+
+def test_dogleg_pose_graph_matches_oracle(oracle_lib):
+    import g2o_amd
+    pg = datasets.make_sphere_pose_graph(nodes_per_level=100, laps=5)
+    opt = g2o_amd.PoseGraphOptimizer(pg, device=0, pcg_tolerance=1e-24)
+    opt.set_algorithm(g2o_amd.DOGLEG)
+    st = opt.optimize(5)
+    n, tr = oracle_lib.OraclePG(pg).dogleg(5, solver=1, tol=1e-24)
+    _say(f"gpu {[s['chi2'] for s in st]}")
+    _say(f"cpu {list(tr[:, 1])}")
+    assert len(st) == n
+    for s, t in zip(st, tr):
+        assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
+        assert s["levenberg_iterations"] == int(t[3]) and s["status"] == int(t[5])

@@ -343,3 +343,31 @@ def test_gauss_newton_step_is_the_lm_step_at_vanishing_lambda(O):
     assert n == 6 and np.all(tp[:, 3] == 0)
     np.testing.assert_allclose(tp[1:, 0], tp[:-1, 1], rtol=1e-13)
     assert tp[-1, 1] < 1e-2 * tp[0, 0]
+
+
+def test_dogleg_oracle(O):
+    """OptimizationAlgorithmDogleg (dogleg.cpp:62-213): while the Gauss-Newton
+    step lies inside the trust region (initialDelta 1e4) it is the step taken,
+    so the first iterations equal Gauss-Newton's; a rejected try halves the
+    radius; accepted iterations never raise chi2."""
+    prob = datasets.make_small_ba(seed=1000)
+    n, dl = O.OracleBA(prob).dogleg(8, solver=0)
+    _, gn = O.OracleBA(prob).gauss_newton(2, solver=0)
+    assert n == 8
+    np.testing.assert_allclose(dl[:2, 1], gn[:, 1], rtol=1e-12)
+    assert np.all(dl[:2, 2] == 1e4) and np.all(dl[:2, 3] == 1)
+    assert np.all(dl[:, 1] <= dl[:, 0] * (1 + 1e-12))
+    # each of the k - 1 rejected tries of an iteration halves the radius; the
+    # accepted one halves it, keeps it or sets it to 3 |h_dl| <= 3 delta
+    many = np.flatnonzero(dl[:, 3] > 1)
+    assert many.size > 0
+    for i in many:
+        prev = dl[i - 1, 2] if i > 0 else 1e4
+        k = dl[i, 3]
+        assert prev * 0.5 ** k * (1 - 1e-12) <= dl[i, 2] <= prev * 0.5 ** (k - 1) * 3.0 * (1 + 1e-12)
+    pg = datasets.make_sphere_pose_graph(nodes_per_level=40, laps=3)
+    n, tp = O.OraclePG(pg).dogleg(6, solver=0)
+    _, tg = O.OraclePG(pg).gauss_newton(6, solver=0)
+    assert n == 6
+    # the pose-graph GN steps stay inside the radius: Dogleg is Gauss-Newton here
+    np.testing.assert_allclose(tp[:, 1], tg[:, 1], rtol=1e-10)
