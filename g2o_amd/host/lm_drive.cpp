@@ -10,10 +10,13 @@
 // problem.bin: int32 C, P, E, precision, refine_steps; f64 cams[C*7],
 // intr[C*4], pts[P*3]; int32 edge_cam[E], edge_pt[E]; f64 meas[E*2],
 // weight[E]; u8 cam_fixed[C], pt_fixed[P]; f64 huber.
-//   or: lm_drive <graph.g2o> <solver|optimize> <iterations> <device>
-//                fx fy cx cy [out.g2o]
-// reads a g2o file (the intrinsics are not part of the format), runs FP64 LM
-// and, as g2o's command-line tool does with -o, saves the optimised graph.
+//   or: lm_drive <graph.g2o> <solver|optimize|gn|dogleg> <iterations> <device>
+//                [fx fy cx cy] [out.g2o]
+// reads a g2o file, runs FP64 Levenberg-Marquardt (solver: the host loop;
+// optimize: on the device), Gauss-Newton or Dogleg and, as g2o's
+// command-line tool does with -o, saves the optimised graph. A bundle-
+// adjustment file needs the intrinsics (the format holds none); a file of
+// VERTEX_SE3:QUAT / EDGE_SE3:QUAT runs the pose-graph solver.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -80,6 +83,16 @@ int run_solver_loop(g2o_hip::BlockSolverHIP& s, int iterations) {
   return 0;
 }
 
+void print_stats(const std::vector<g2ohip_iter_stats>& st) {
+  for (size_t i = 0; i < st.size(); ++i)
+    std::printf("%zu %.17g %.17g %.17g %d %d\n", i, st[i].chi2_before, st[i].chi2, st[i].lambda,
+                st[i].levenberg_iterations, st[i].iterations_linear_solver);
+}
+
+int algorithm_of(const std::string& mode) {
+  return mode == "gn" ? G2OHIP_GAUSS_NEWTON : mode == "dogleg" ? G2OHIP_DOGLEG : G2OHIP_LEVENBERG;
+}
+
 int run(g2o_hip::BlockSolverHIP& s, const char* mode, int iters) {
   if (std::string(mode) == "solver") {
     if (run_solver_loop(s, iters)) {
@@ -88,30 +101,60 @@ int run(g2o_hip::BlockSolverHIP& s, const char* mode, int iters) {
     }
   } else {
     std::vector<g2ohip_iter_stats> st;
-    if (s.optimize(iters, &st) < 0) {
+    if (!s.setAlgorithm(algorithm_of(mode)) || s.optimize(iters, &st) < 0) {
       std::fprintf(stderr, "optimize failed: %s\n", s.lastError().c_str());
       return 1;
     }
-    for (size_t i = 0; i < st.size(); ++i)
-      std::printf("%zu %.17g %.17g %.17g %d %d\n", i, st[i].chi2_before, st[i].chi2, st[i].lambda,
-                  st[i].levenberg_iterations, st[i].iterations_linear_solver);
+    print_stats(st);
+  }
+  return 0;
+}
+
+int run_pose_graph_file(g2o_hip::GraphIO& io, int argc, char** argv) {
+  g2o_hip::GraphIO::PoseGraph pg;
+  if (!io.getPoseGraph(pg)) {
+    std::fprintf(stderr, "load failed: %s\n", io.lastError().c_str());
+    return 1;
+  }
+  g2o_hip::PoseGraphSolverHIP s(std::atoi(argv[4]));
+  const int N = static_cast<int>(pg.poseId.size()), E = static_cast<int>(pg.edgeFrom.size());
+  std::vector<g2ohip_iter_stats> st;
+  if (!s.setGraph(N, E, pg.edgeFrom.data(), pg.edgeTo.data(), pg.fixed.data()) ||
+      !s.setMeasurements(pg.meas.data(), pg.information.data()) || !s.setEstimates(pg.poses.data()) ||
+      !s.setAlgorithm(algorithm_of(argv[2])) || s.optimize(std::atoi(argv[3]), &st) < 0) {
+    std::fprintf(stderr, "pose graph failed: %s\n", s.lastError().c_str());
+    return 1;
+  }
+  print_stats(st);
+  if (argc > 5) {
+    if (!s.getEstimates(pg.poses.data()) || !io.setPoseEstimates(pg.poses.data()) || !io.save(argv[5], 17)) {
+      std::fprintf(stderr, "save failed: %s %s\n", s.lastError().c_str(), io.lastError().c_str());
+      return 1;
+    }
   }
   return 0;
 }
 
 int run_g2o_file(int argc, char** argv) {
-  if (argc < 9) {
-    std::fprintf(stderr, "usage: %s graph.g2o solver|optimize iterations device fx fy cx cy [out.g2o]\n", argv[0]);
-    return 2;
-  }
   g2o_hip::GraphIO io;
-  g2o_hip::GraphIO::BA ba;
-  if (!io.load(argv[1]) || !io.getBA(ba)) {
+  if (!io.load(argv[1])) {
     std::fprintf(stderr, "load failed: %s\n", io.lastError().c_str());
     return 1;
   }
   const std::string log = io.log();
   if (!log.empty()) std::fprintf(stderr, "%s", log.c_str());
+  const std::vector<long long> info = io.info();
+  if (info[2] == 0 && info[4] > 0) return run_pose_graph_file(io, argc, argv);
+  if (argc < 9) {
+    std::fprintf(stderr, "usage: %s graph.g2o solver|optimize|gn|dogleg iterations device fx fy cx cy [out.g2o]\n",
+                 argv[0]);
+    return 2;
+  }
+  g2o_hip::GraphIO::BA ba;
+  if (!io.getBA(ba)) {
+    std::fprintf(stderr, "load failed: %s\n", io.lastError().c_str());
+    return 1;
+  }
   const int C = (int)ba.camId.size(), P = (int)ba.pointId.size(), E = (int)ba.edgeCam.size();
   std::vector<double> w(E), intr(4 * static_cast<size_t>(C));
   for (int e = 0; e < E; ++e) {

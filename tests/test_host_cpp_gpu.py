@@ -79,3 +79,28 @@ def test_g2o_file_through_the_cpp_driver(tmp_path):
     saved = gio.G2oGraph.load(dst).to_ba_problem(prob.intr)
     assert np.max(np.abs(saved.cams - cams)) < 1e-12
     assert np.max(np.abs(saved.pts - pts)) < 1e-9 * np.max(np.abs(pts))
+
+
+@pytest.mark.parametrize("mode,algorithm", [("optimize", g2o_amd.LEVENBERG), ("gn", g2o_amd.GAUSS_NEWTON),
+                                            ("dogleg", g2o_amd.DOGLEG)])
+def test_pose_graph_file_through_the_cpp_driver(tmp_path, mode, algorithm):
+    """A VERTEX_SE3:QUAT / EDGE_SE3:QUAT file through GraphIO and
+    PoseGraphSolverHIP with each algorithm: the trace equals the Python
+    mirror's on the same file, and the saved graph holds the final poses."""
+    from g2o_amd import graphio as gio
+    pg = datasets.make_sphere_pose_graph(nodes_per_level=60, laps=4)
+    src = str(tmp_path / "sphere.g2o")
+    dst = str(tmp_path / "sphere_opt.g2o")
+    gio.save_pose_graph(src, pg)
+    out = subprocess.run([DRIVER, src, mode, "5", "0", dst], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    a = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()])
+    loaded = gio.G2oGraph.load(src).to_pose_graph()
+    opt = g2o_amd.PoseGraphOptimizer(loaded, device=0)  # PCG tolerance 1e-8, as PoseGraphSolverHIP
+    opt.set_algorithm(algorithm)
+    st = opt.optimize(5)
+    assert a.shape[0] == len(st)
+    np.testing.assert_allclose(a[:, 2], [s["chi2"] for s in st], rtol=1e-10)
+    saved = gio.G2oGraph.load(dst).to_pose_graph()
+    poses = opt.state()
+    assert np.max(np.abs(saved.poses[:, :3] - poses[:, :3])) < 1e-9 * max(1.0, np.max(np.abs(poses[:, :3])))
