@@ -40,7 +40,8 @@ struct g2ohip_graph {
   // BAL part
   std::vector<double> bal_cam, bal_pt, bal_meas;
   std::vector<int> bal_cam_idx, bal_pt_idx;
-  std::string log, err;
+  std::string log;
+  mutable std::string err;  // last failure, also of calls on a const graph (save)
   long long log_count = 0;
 };
 
@@ -54,7 +55,7 @@ const char* const kTag[] = {"", "VERTEX_SE3:EXPMAP", "VERTEX_TRACKXYZ", "VERTEX_
 const int kVals[] = {0, 7, 3, 7, 5, 28, 0, 0};
 const int kMaxLog = 1000;
 
-int fail(g2ohip_graph* g, int code, const std::string& msg) {
+int fail(const g2ohip_graph* g, int code, const std::string& msg) {
   g->err = msg;
   return code;
 }
@@ -641,8 +642,7 @@ int g2ohip_graph_add_pg(g2ohip_graph* g, int num_poses, int num_edges, const int
   return G2OHIP_OK;
 }
 
-int g2ohip_graph_save(const g2ohip_graph* gc, const char* filename, int precision) {
-  g2ohip_graph* g = const_cast<g2ohip_graph*>(gc);
+int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision) {
   if (!g || !filename) return G2OHIP_ERR_INVALID;
   if (precision < 1 || precision > 17) return fail(g, G2OHIP_ERR_INVALID, "graph_save: precision must be 1..17");
   // vertices of the edges, sorted by id (optimizable_graph.cpp:541-554)
