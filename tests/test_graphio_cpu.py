@@ -292,3 +292,19 @@ def test_bal_undistorted_mapping_of_exact_data(tmp_path):
     prob = gio.bal_to_ba(b, undistort=True)
     # every undistorted measurement is the pinhole projection of its point
     assert OracleBA(prob).chi2() < 1e-16 * len(b.meas) * 500.0 ** 2
+
+
+def test_whitespace_signs_and_exponents(tmp_path):
+    """CRLF line ends, tabs, leading '+' and exponents parse as an ostream
+    reads them (std::istream >> double accepts all of these)."""
+    path = tmp_path / "ws.g2o"
+    path.write_bytes(b"VERTEX_TRACKXYZ\t3\t+1.5e0  -2E-1 3.\r\n"
+                     b"  VERTEX_SE3:EXPMAP 7 0 0 0 0 0 0 +1\r\n"
+                     b"EDGE_SE3_PROJECT_XYZ:EXPMAP 3 7 1e2 2.5e+2 4 0 4\r\n"
+                     b"\r\n# trailing comment\r\n")
+    g = gio.G2oGraph.load(str(path))
+    n = g.info()
+    assert (n["points"], n["cameras"], n["projection_edges"], n["skipped"], n["lines"]) == (1, 1, 1, 0, 5)
+    a = g.ba_arrays()
+    assert a["pts"].tolist() == [[1.5, -0.2, 3.0]]
+    assert a["meas"].tolist() == [[100.0, 250.0]] and a["omega"].tolist() == [[4.0, 0.0, 4.0]]
