@@ -29,7 +29,7 @@ EXPORTS = [
     "g2ohip_linear_solve_pcg", "g2ohip_pg_set_structure", "g2ohip_pg_structure_info", "g2ohip_pg_set_measurements",
     "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_graph_create",
     "g2ohip_graph_destroy", "g2ohip_graph_last_error", "g2ohip_graph_log", "g2ohip_graph_load", "g2ohip_graph_info",
-    "g2ohip_graph_get_ba", "g2ohip_graph_get_pg", "g2ohip_graph_set_ba_state", "g2ohip_graph_set_pg_state",
+    "g2ohip_graph_get_ba", "g2ohip_graph_get_ba_params", "g2ohip_graph_get_pg", "g2ohip_graph_set_ba_state", "g2ohip_graph_set_pg_state",
     "g2ohip_graph_add_ba", "g2ohip_graph_add_pg", "g2ohip_graph_save", "g2ohip_graph_load_bal", "g2ohip_graph_get_bal",
     "g2ohip_ba_adjoint", "g2ohip_ba_newton",
 ]
@@ -121,6 +121,7 @@ def lib():
         "g2ohip_graph_load": (ct.c_int, [_vp, ct.c_char_p, ct.c_int]),
         "g2ohip_graph_info": (ct.c_int, [_vp, ct.POINTER(ct.c_longlong)]),
         "g2ohip_graph_get_ba": (ct.c_int, [_vp, _ip, _dp, _up, _ip, _dp, _up, _ip, _ip, _dp, _dp]),
+        "g2ohip_graph_get_ba_params": (ct.c_int, [_vp, _ip, _dp, _ip]),
         "g2ohip_graph_get_pg": (ct.c_int, [_vp, _ip, _dp, _up, _ip, _ip, _dp, _dp]),
         "g2ohip_graph_set_ba_state": (ct.c_int, [_vp, _dp, _dp]),
         "g2ohip_graph_set_pg_state": (ct.c_int, [_vp, _dp]),

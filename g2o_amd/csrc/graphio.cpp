@@ -28,8 +28,13 @@ struct g2ohip_graph {
   std::vector<int> cam_id, pt_id, pose_id;
   std::vector<double> cam, pt, pose;  // cam: world-to-camera [t q]; pose: Isometry3 [t q]
   std::vector<uint8_t> cam_fix, pt_fix, pose_fix;
-  // projection edges (indices into the camera / point arrays)
-  std::vector<int> pe_cam, pe_pt;
+  // projection edges (indices into the camera / point arrays); pe_par: the
+  // CameraParameters of an EDGE_PROJECT_XYZ2UV edge, -1 for EDGE_SE3_PROJECT_XYZ
+  std::vector<int> pe_cam, pe_pt, pe_par;
+  // PARAMS_CAMERAPARAMETERS: id -> index; focal length, principal point, baseline
+  std::unordered_map<int, int> pmap;
+  std::vector<int> par_id;
+  std::vector<double> par;
   std::vector<double> pe_meas, pe_omega;  // 2, 3 (upper triangle)
   // pose edges
   std::vector<int> se_from, se_to;
@@ -47,12 +52,23 @@ struct g2ohip_graph {
 
 namespace {
 
-enum Kind { K_NONE = 0, K_CAM = 1, K_PT = 2, K_POSE = 3, K_PROJ = 4, K_PEDGE = 5, K_FIX = 6, K_UNKNOWN = 7 };
+enum Kind {
+  K_NONE = 0, K_CAM = 1, K_PT = 2, K_POSE = 3, K_PROJ = 4, K_PEDGE = 5, K_FIX = 6, K_PARAM = 7, K_PROJ2 = 8,
+  K_UNKNOWN = 9
+};
 
-const char* const kTag[] = {"", "VERTEX_SE3:EXPMAP", "VERTEX_TRACKXYZ", "VERTEX_SE3:QUAT",
-                            "EDGE_SE3_PROJECT_XYZ:EXPMAP", "EDGE_SE3:QUAT", "FIX", ""};
+const char* const kTag[] = {"",
+                            "VERTEX_SE3:EXPMAP",
+                            "VERTEX_TRACKXYZ",
+                            "VERTEX_SE3:QUAT",
+                            "EDGE_SE3_PROJECT_XYZ:EXPMAP",
+                            "EDGE_SE3:QUAT",
+                            "FIX",
+                            "PARAMS_CAMERAPARAMETERS",   // parameter_cameraparameters.cpp:43-55
+                            "EDGE_PROJECT_XYZ2UV:EXPMAP",  // edge_project_xyz2uv.cpp:37-47
+                            ""};
 // values after the ids
-const int kVals[] = {0, 7, 3, 7, 5, 28, 0, 0};
+const int kVals[] = {0, 7, 3, 7, 5, 28, 0, 4, 5, 0};
 const int kMaxLog = 1000;
 
 int fail(const g2ohip_graph* g, int code, const std::string& msg) {
@@ -76,7 +92,8 @@ struct Rec {
   long long line;  // line number within the piece (0-based)
   int kind;
   int bad;         // too few or malformed values
-  int a, b;        // vertex id, or the edge's two vertex ids
+  int a, b;        // vertex / parameter id, or the edge's two vertex ids
+  int c;           // EDGE_PROJECT_XYZ2UV: its parameter id
   long long off;   // into vals (or ints for FIX, or names for K_UNKNOWN)
   int n;
 };
@@ -116,7 +133,7 @@ inline bool parse_int(const char*& p, const char* e, int& v) {
 }
 
 int tag_kind(const char* t, size_t n) {
-  for (int k = 1; k <= K_FIX; ++k)
+  for (int k = 1; k < K_UNKNOWN; ++k)
     if (std::strlen(kTag[k]) == n && std::memcmp(kTag[k], t, n) == 0) return k;
   return K_UNKNOWN;
 }
@@ -131,7 +148,7 @@ void parse_piece(const char* b, const char* e, Piece& out) {
     if (q < le && *q != '#') {
       const char* t0 = q;
       while (q < le && !is_space(*q)) ++q;
-      Rec r{line, tag_kind(t0, q - t0), 0, -1, -1, 0, 0};
+      Rec r{line, tag_kind(t0, q - t0), 0, -1, -1, -1, 0, 0};
       if (r.kind == K_UNKNOWN) {
         r.off = (long long)out.names.size();
         out.names.emplace_back(t0, q - t0);
@@ -142,8 +159,10 @@ void parse_piece(const char* b, const char* e, Piece& out) {
         while (parse_int(q, le, id)) out.ints.push_back(id);
         r.n = (int)(out.ints.size() - r.off);
       } else {
-        const bool edge = r.kind == K_PROJ || r.kind == K_PEDGE;
+        const bool edge = r.kind == K_PROJ || r.kind == K_PEDGE || r.kind == K_PROJ2;
         if (!parse_int(q, le, r.a) || (edge && !parse_int(q, le, r.b))) r.bad = 1;
+        // readParamIds (base_edge.h:187-194) precedes the measurement
+        if (!r.bad && r.kind == K_PROJ2 && !parse_int(q, le, r.c)) r.bad = 1;
         r.off = (long long)out.vals.size();
         r.n = kVals[r.kind];
         out.vals.resize(r.off + r.n, 0.0);
@@ -312,6 +331,25 @@ void apply_piece(g2ohip_graph* g, const Piece& pc, long long line0, std::set<std
           (k == K_CAM ? g->cam_fix : k == K_PT ? g->pt_fix : g->pose_fix)[j] = 1;
         }
         break;
+      case K_PARAM: {
+        if (r.bad) {
+          ++g->skipped;
+          std::snprintf(buf, sizeof buf, "reading data %d for parameter %s at line %lld", r.a, kTag[K_PARAM], line);
+          note(g, buf);
+          break;
+        }
+        if (g->pmap.count(r.a)) {
+          ++g->skipped;
+          std::snprintf(buf, sizeof buf, "Parameter of type: %s id: %d already defined at line %lld", kTag[K_PARAM],
+                        r.a, line);
+          note(g, buf);
+          break;
+        }
+        g->pmap[r.a] = (int)g->par_id.size();
+        g->par_id.push_back(r.a);
+        g->par.insert(g->par.end(), v, v + 4);
+        break;
+      }
       case K_CAM:
       case K_PT:
       case K_POSE: {
@@ -359,6 +397,7 @@ void apply_piece(g2ohip_graph* g, const Piece& pc, long long line0, std::set<std
         break;
       }
       case K_PROJ:
+      case K_PROJ2:
       case K_PEDGE: {
         auto ia = g->vmap.find(r.a), ib = g->vmap.find(r.b);
         if (!r.bad && (ia == g->vmap.end() || ib == g->vmap.end())) {
@@ -368,9 +407,14 @@ void apply_piece(g2ohip_graph* g, const Piece& pc, long long line0, std::set<std
           note(g, buf);
           break;
         }
-        const bool types_ok = !r.bad && (r.kind == K_PROJ ? (ia->second.first == K_PT && ib->second.first == K_CAM)
-                                                          : (ia->second.first == K_POSE && ib->second.first == K_POSE &&
-                                                             r.a != r.b));
+        const bool proj = r.kind == K_PROJ || r.kind == K_PROJ2;
+        // an XYZ2UV edge resolves its CameraParameters when it is added
+        // (optimizable_graph.cpp:232-236): the parameter must come first
+        auto ip = g->pmap.find(r.c);
+        const bool types_ok = !r.bad &&
+                              (proj ? (ia->second.first == K_PT && ib->second.first == K_CAM)
+                                    : (ia->second.first == K_POSE && ib->second.first == K_POSE && r.a != r.b)) &&
+                              (r.kind != K_PROJ2 || ip != g->pmap.end());
         if (!types_ok) {
           ++g->skipped;
           std::snprintf(buf, sizeof buf, "Unable to add edge %s at line %lld IDs: %d %d", kTag[r.kind], line, r.a,
@@ -378,10 +422,11 @@ void apply_piece(g2ohip_graph* g, const Piece& pc, long long line0, std::set<std
           note(g, buf);
           break;
         }
-        if (r.kind == K_PROJ) {
+        if (proj) {
           g->edge_order.push_back({K_PROJ, (int)g->pe_cam.size()});
           g->pe_pt.push_back(ia->second.second);
           g->pe_cam.push_back(ib->second.second);
+          g->pe_par.push_back(r.kind == K_PROJ2 ? ip->second : -1);
           g->pe_meas.insert(g->pe_meas.end(), v, v + 2);
           g->pe_omega.insert(g->pe_omega.end(), v + 2, v + 5);
         } else {
@@ -476,9 +521,12 @@ int g2ohip_graph_info(const g2ohip_graph* g, long long* info) {
   for (auto f : g->cam_fix) nfix += f;
   for (auto f : g->pt_fix) nfix += f;
   for (auto f : g->pose_fix) nfix += f;
-  const long long v[10] = {(long long)g->cam_id.size(), (long long)g->pt_id.size(), (long long)g->pe_cam.size(),
+  long long n2uv = 0;
+  for (int p : g->pe_par) n2uv += p >= 0;
+  const long long v[12] = {(long long)g->cam_id.size(), (long long)g->pt_id.size(), (long long)g->pe_cam.size(),
                            (long long)g->pose_id.size(), (long long)g->se_from.size(), nfix, g->lines,
-                           g->unknown_lines, g->skipped, (long long)g->bal_cam_idx.size()};
+                           g->unknown_lines, g->skipped, (long long)g->bal_cam_idx.size(), n2uv,
+                           (long long)g->par_id.size()};
   std::memcpy(info, v, sizeof v);
   return G2OHIP_OK;
 }
@@ -505,6 +553,19 @@ int g2ohip_graph_get_ba(const g2ohip_graph* g, int* cam_id, double* cams, uint8_
   }
   if (meas && E) std::memcpy(meas, g->pe_meas.data(), 2 * E * sizeof(double));
   if (omega && E) std::memcpy(omega, g->pe_omega.data(), 3 * E * sizeof(double));
+  return G2OHIP_OK;
+}
+
+int g2ohip_graph_get_ba_params(const g2ohip_graph* g, int* param_id, double* params, int* edge_param) {
+  if (!g) return G2OHIP_ERR_INVALID;
+  const std::vector<int> o = order_by_id(g->par_id);
+  const std::vector<int> rk = rank_of(o);
+  for (size_t i = 0; i < o.size(); ++i) {
+    if (param_id) param_id[i] = g->par_id[o[i]];
+    if (params) std::memcpy(params + 4 * i, &g->par[4 * (size_t)o[i]], 4 * sizeof(double));
+  }
+  if (edge_param)
+    for (size_t e = 0; e < g->pe_par.size(); ++e) edge_param[e] = g->pe_par[e] < 0 ? -1 : rk[g->pe_par[e]];
   return G2OHIP_OK;
 }
 
@@ -596,6 +657,7 @@ int g2ohip_graph_add_ba(g2ohip_graph* g, int num_cams, int num_pts, int num_edge
     g->edge_order.push_back({K_PROJ, (int)g->pe_cam.size()});
     g->pe_cam.push_back(c0 + edge_cam[e]);
     g->pe_pt.push_back(p0 + edge_pt[e]);
+    g->pe_par.push_back(-1);
     g->pe_meas.insert(g->pe_meas.end(), meas + 2 * (size_t)e, meas + 2 * (size_t)e + 2);
     const double I[3] = {1.0, 0.0, 1.0};
     const double* o = omega ? omega + 3 * (size_t)e : I;
@@ -659,6 +721,18 @@ int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision
       if (uq[i]) vs.push_back({g->pose_id[i], {K_POSE, (int)i}});
     std::sort(vs.begin(), vs.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
   }
+  // the parameters first, by id (optimizable_graph.cpp:538-539)
+  std::string head;
+  {
+    const std::vector<int> po = order_by_id(g->par_id);
+    char b[64];
+    for (int i : po) {
+      std::snprintf(b, sizeof b, "%s %d ", kTag[K_PARAM], g->par_id[i]);
+      head += b;
+      put_vals(head, &g->par[4 * (size_t)i], 4, precision);
+      head += '\n';
+    }
+  }
   // item k < nv is vertex vs[k], else edge edge_order[k - nv]
   const size_t nv = vs.size(), ni = nv + g->edge_order.size();
   auto format = [&](size_t k, std::string& s) {
@@ -695,7 +769,12 @@ int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision
     const auto& eo = g->edge_order[k - nv];
     const int i = eo.second;
     if (eo.first == K_PROJ) {
-      std::snprintf(b, sizeof b, "%s %d %d ", kTag[K_PROJ], g->pt_id[g->pe_pt[i]], g->cam_id[g->pe_cam[i]]);
+      const int pi = g->pe_par[i];
+      if (pi < 0)
+        std::snprintf(b, sizeof b, "%s %d %d ", kTag[K_PROJ], g->pt_id[g->pe_pt[i]], g->cam_id[g->pe_cam[i]]);
+      else
+        std::snprintf(b, sizeof b, "%s %d %d %d ", kTag[K_PROJ2], g->pt_id[g->pe_pt[i]], g->cam_id[g->pe_cam[i]],
+                      g->par_id[pi]);
       s += b;
       put_vals(s, &g->pe_meas[2 * (size_t)i], 2, precision);
       put_vals(s, &g->pe_omega[3 * (size_t)i], 3, precision);
@@ -717,7 +796,7 @@ int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision
   // blocks in order.
   const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
   const size_t blk = 1 << 16;
-  bool ok = true;
+  bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
   std::vector<std::string> out(nt);
   for (size_t k0 = 0; k0 < ni && ok; k0 += blk * nt) {
     run_threads(nt, [&](int t) {

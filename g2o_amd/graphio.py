@@ -81,10 +81,10 @@ class G2oGraph:
         return self._lib.g2ohip_graph_log(self._h).decode()
 
     def info(self) -> dict:
-        v = (ct.c_longlong * 10)()
+        v = (ct.c_longlong * 12)()
         self._check(self._lib.g2ohip_graph_info(self._h, v))
         keys = ["cameras", "points", "projection_edges", "poses", "pose_edges", "fixed", "lines", "unknown_lines",
-                "skipped", "bal_observations"]
+                "skipped", "bal_observations", "xyz2uv_edges", "camera_parameters"]
         return dict(zip(keys, (int(x) for x in v)))
 
     # ----------------------------------------------------- bundle adjustment
@@ -103,11 +103,50 @@ class G2oGraph:
             _ptr(o["omega"], ct.c_double)))
         return out
 
-    def to_ba_problem(self, intr, huber: float = 0.0, name: str = "") -> BAProblem:
+    def ba_params(self) -> dict:
+        """PARAMS_CAMERAPARAMETERS sorted by id (f cx cy baseline) and each
+        projection edge's parameter index (-1 for EDGE_SE3_PROJECT_XYZ)."""
+        n = self.info()
+        o = dict(param_id=np.empty(n["camera_parameters"], np.int32), params=np.empty((n["camera_parameters"], 4)),
+                 edge_param=np.empty(n["projection_edges"], np.int32))
+        self._check(self._lib.g2ohip_graph_get_ba_params(self._h, _ptr(o["param_id"], ct.c_int),
+                                                         _ptr(o["params"], ct.c_double),
+                                                         _ptr(o["edge_param"], ct.c_int)))
+        return o
+
+    def camera_intrinsics(self) -> np.ndarray:
+        """Per-camera fx fy cx cy from the CameraParameters of the cameras'
+        EDGE_PROJECT_XYZ2UV edges (fx = fy = f, cam_map in
+        parameter_cameraparameters.cpp:59-65). Every camera needs such edges,
+        all with one parameter set, since the path keeps intrinsics per camera."""
+        a = self.ba_arrays()
+        pr = self.ba_params()
+        C = a["cams"].shape[0]
+        intr = np.full((C, 4), np.nan)
+        for e in range(a["edge_cam"].shape[0]):
+            k = pr["edge_param"][e]
+            if k < 0:
+                raise G2oIOError(f"edge {e} is an EDGE_SE3_PROJECT_XYZ: its intrinsics are not in the file")
+            f, cx, cy, _ = pr["params"][k]
+            row = np.array([f, f, cx, cy])
+            c = a["edge_cam"][e]
+            if np.isnan(intr[c, 0]):
+                intr[c] = row
+            elif not np.array_equal(intr[c], row):
+                raise G2oIOError(f"camera {c} is observed through two different camera parameters")
+        if np.isnan(intr).any():
+            raise G2oIOError("a camera has no projection edge to take its intrinsics from")
+        return intr
+
+    def to_ba_problem(self, intr=None, huber: float = 0.0, name: str = "") -> BAProblem:
         """The graph's bundle-adjustment part as a BAProblem. ``intr`` is
-        fx fy cx cy, one row for every camera or a single row for all (the
-        format does not hold EdgeSE3ProjectXYZ's intrinsics). The path takes
-        scalar information per edge, so every Omega must be w * I."""
+        fx fy cx cy, one row for every camera or a single row for all; None
+        takes them from the file's camera parameters (EDGE_PROJECT_XYZ2UV
+        edges), since EDGE_SE3_PROJECT_XYZ keeps its intrinsics out of the
+        format. The path takes scalar information per edge, so every Omega must
+        be w * I."""
+        if intr is None:
+            intr = self.camera_intrinsics()
         a = self.ba_arrays()
         om = a["omega"]
         if om.size and (np.any(om[:, 1] != 0.0) or np.any(om[:, 0] != om[:, 2])):

@@ -259,9 +259,10 @@ class GraphIO {
   int status() const { return _status; }
 
   // [cameras, points, projection edges, poses, pose edges, fixed, lines,
-  //  unknown-tag lines, skipped elements, BAL observations]
+  //  unknown-tag lines, skipped elements, BAL observations, XYZ2UV edges,
+  //  camera parameters]
   std::vector<long long> info() const {
-    std::vector<long long> v(10, 0);
+    std::vector<long long> v(12, 0);
     if (_g) g2ohip_graph_info(_g, v.data());
     return v;
   }
@@ -283,6 +284,15 @@ class GraphIO {
     o.edgeFrom.resize(E); o.edgeTo.resize(E); o.meas.resize(7 * E); o.information.resize(36 * E);
     return ok(g2ohip_graph_get_pg(_g, o.poseId.data(), o.poses.data(), o.fixed.data(), o.edgeFrom.data(),
                                   o.edgeTo.data(), o.meas.data(), o.information.data()));
+  }
+  // camera parameters sorted by id (f cx cy baseline) and each projection
+  // edge's parameter index, -1 for EDGE_SE3_PROJECT_XYZ
+  bool getCameraParameters(std::vector<int>& ids, std::vector<double>& params, std::vector<int>& edgeParam) {
+    const std::vector<long long> n = info();
+    ids.resize(n[11]);
+    params.resize(4 * n[11]);
+    edgeParam.resize(n[2]);
+    return ok(g2ohip_graph_get_ba_params(_g, ids.data(), params.data(), edgeParam.data()));
   }
   bool setBAEstimates(const double* cams, const double* points) {
     return ok(g2ohip_graph_set_ba_state(_g, cams, points));

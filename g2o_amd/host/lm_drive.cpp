@@ -12,6 +12,8 @@
 // weight[E]; u8 cam_fixed[C], pt_fixed[P]; f64 huber.
 //   or: lm_drive <graph.g2o> <solver|optimize|gn|dogleg> <iterations> <device>
 //                [fx fy cx cy] [out.g2o]
+// (no intrinsics when every projection edge is an EDGE_PROJECT_XYZ2UV naming
+// its PARAMS_CAMERAPARAMETERS)
 // reads a g2o file, runs FP64 Levenberg-Marquardt (solver: the host loop;
 // optimize: on the device), Gauss-Newton or Dogleg and, as g2o's
 // command-line tool does with -o, saves the optimised graph. A bundle-
@@ -145,11 +147,14 @@ int run_g2o_file(int argc, char** argv) {
   if (!log.empty()) std::fprintf(stderr, "%s", log.c_str());
   const std::vector<long long> info = io.info();
   if (info[2] == 0 && info[4] > 0) return run_pose_graph_file(io, argc, argv);
-  if (argc < 9) {
+  // a file whose projection edges all name camera parameters carries its intrinsics
+  const bool from_params = info[2] > 0 && info[10] == info[2];
+  if (!from_params && argc < 9) {
     std::fprintf(stderr, "usage: %s graph.g2o solver|optimize|gn|dogleg iterations device fx fy cx cy [out.g2o]\n",
                  argv[0]);
     return 2;
   }
+  const int out_arg = from_params ? 5 : 9;
   g2o_hip::GraphIO::BA ba;
   if (!io.getBA(ba)) {
     std::fprintf(stderr, "load failed: %s\n", io.lastError().c_str());
@@ -165,8 +170,22 @@ int run_g2o_file(int argc, char** argv) {
     }
     w[e] = o[0];
   }
-  for (int c = 0; c < C; ++c)
-    for (int k = 0; k < 4; ++k) intr[4 * static_cast<size_t>(c) + k] = std::atof(argv[5 + k]);
+  if (from_params) {
+    std::vector<int> ids, edgeParam;
+    std::vector<double> params;
+    if (!io.getCameraParameters(ids, params, edgeParam)) return 1;
+    // EdgeProjectXYZ2UV: fx = fy = f, principal point (cx, cy)
+    for (int e = 0; e < E; ++e) {
+      const double* p = &params[4 * static_cast<size_t>(edgeParam[e])];
+      double* in = &intr[4 * static_cast<size_t>(ba.edgeCam[e])];
+      in[0] = in[1] = p[0];
+      in[2] = p[1];
+      in[3] = p[2];
+    }
+  } else {
+    for (int c = 0; c < C; ++c)
+      for (int k = 0; k < 4; ++k) intr[4 * static_cast<size_t>(c) + k] = std::atof(argv[5 + k]);
+  }
   g2o_hip::BlockSolverHIP s(std::atoi(argv[4]));
   if (!s.setGraph(C, P, E, ba.edgeCam.data(), ba.edgePoint.data(), ba.camFixed.data(), ba.pointFixed.data()) ||
       !s.setMeasurements(ba.meas.data(), w.data(), intr.data(), 0.0) ||
@@ -175,9 +194,9 @@ int run_g2o_file(int argc, char** argv) {
     return 1;
   }
   if (run(s, argv[2], std::atoi(argv[3]))) return 1;
-  if (argc > 9) {
+  if (argc > out_arg) {
     if (!s.getEstimates(ba.cams.data(), ba.points.data()) || !io.setBAEstimates(ba.cams.data(), ba.points.data()) ||
-        !io.save(argv[9], 17)) {
+        !io.save(argv[out_arg], 17)) {
       std::fprintf(stderr, "save failed: %s %s\n", s.lastError().c_str(), io.lastError().c_str());
       return 1;
     }

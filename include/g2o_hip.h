@@ -289,6 +289,12 @@ int g2ohip_pg_get_system(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, 
  *   VERTEX_SE3:QUAT id  tx ty tz qx qy qz qw    (vertex_se3.cpp:46-55)
  *   EDGE_SE3:QUAT from to  tx ty tz qx qy qz qw  21 upper-triangle values
  *                         (edge_se3.cpp:43-58; the quaternion is normalised)
+ *   PARAMS_CAMERAPARAMETERS id  f cx cy baseline
+ *                         (parameter_cameraparameters.cpp:43-55)
+ *   EDGE_PROJECT_XYZ2UV:EXPMAP point camera param  u v  O00 O01 O11
+ *                         (edge_project_xyz2uv.cpp:37-47: the projection
+ *                         of EDGE_SE3_PROJECT_XYZ with fx = fy = f and (cx, cy)
+ *                         from the parameter, which must precede the edge)
  *   FIX id ...            fixes vertices defined earlier in the file
  * Empty lines and lines whose first token starts with '#' are skipped. As in
  * the reference, a vertex with a negative or already used id, an edge whose
@@ -314,9 +320,10 @@ const char* g2ohip_graph_log(const g2ohip_graph* g);
  * order, so the result does not depend on nthreads. */
 int g2ohip_graph_load(g2ohip_graph* g, const char* filename, int nthreads);
 
-/* info[0..9] = cameras (VERTEX_SE3:EXPMAP), points (VERTEX_TRACKXYZ),
- * projection edges, poses (VERTEX_SE3:QUAT), pose edges, fixed vertices, lines
- * read, lines with an unknown tag, skipped elements, BAL observations */
+/* info[0..11] = cameras (VERTEX_SE3:EXPMAP), points (VERTEX_TRACKXYZ),
+ * projection edges (both tags), poses (VERTEX_SE3:QUAT), pose edges, fixed
+ * vertices, lines read, lines with an unknown tag, skipped elements, BAL
+ * observations, EDGE_PROJECT_XYZ2UV edges, camera parameters */
 int g2ohip_graph_info(const g2ohip_graph* g, long long* info);
 
 /* Bundle-adjustment part in the layout g2ohip_ba_set_structure /
@@ -327,6 +334,10 @@ int g2ohip_graph_info(const g2ohip_graph* g, long long* info);
  * NULL skips an output. */
 int g2ohip_graph_get_ba(const g2ohip_graph* g, int* cam_id, double* cams, uint8_t* cam_fixed, int* pt_id,
                         double* pts, uint8_t* pt_fixed, int* edge_cam, int* edge_pt, double* meas, double* omega);
+/* Camera parameters sorted by id: param_id[np], params[np*4] = f cx cy
+ * baseline; edge_param[E] = each projection edge's parameter (index into
+ * that order) or -1 for EDGE_SE3_PROJECT_XYZ. NULL skips. */
+int g2ohip_graph_get_ba_params(const g2ohip_graph* g, int* param_id, double* params, int* edge_param);
 /* Pose-graph part: poses sorted by id, poses[N*7] Isometry3 [t q], edges in
  * file order, meas[E*7], info[E*36] full symmetric. NULL skips. */
 int g2ohip_graph_get_pg(const g2ohip_graph* g, int* pose_id, double* poses, uint8_t* fixed, int* edge_from,

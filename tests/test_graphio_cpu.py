@@ -308,3 +308,62 @@ def test_whitespace_signs_and_exponents(tmp_path):
     a = g.ba_arrays()
     assert a["pts"].tolist() == [[1.5, -0.2, 3.0]]
     assert a["meas"].tolist() == [[100.0, 250.0]] and a["omega"].tolist() == [[4.0, 0.0, 4.0]]
+
+
+def xyz2uv_text(prob, tmp_path, f=500.0, cx=320.0, cy=240.0):
+    """The problem as g2o's ba_demo writes it: one PARAMS_CAMERAPARAMETERS and
+    EDGE_PROJECT_XYZ2UV:EXPMAP edges naming it (edge_project_xyz2uv.cpp:37-47)."""
+    path = tmp_path / "plain.g2o"
+    gio.save_ba_problem(str(path), prob)
+    out = [f"PARAMS_CAMERAPARAMETERS 0 {f!r} {cx!r} {cy!r} 0.5"]
+    for ln in path.read_text().splitlines():
+        if ln.startswith("EDGE_SE3_PROJECT_XYZ:EXPMAP "):
+            t = ln.split(" ", 3)
+            ln = f"EDGE_PROJECT_XYZ2UV:EXPMAP {t[1]} {t[2]} 0 {t[3]}"
+        out.append(ln)
+    return "\n".join(out) + "\n"
+
+
+def test_camera_parameters_and_xyz2uv_edges(tmp_path):
+    """EdgeProjectXYZ2UV is EdgeSE3ProjectXYZ with fx = fy = f and (cx, cy)
+    taken from its CameraParameters (cam_map, parameter_cameraparameters.cpp:
+    59-65; the Jacobians of edge_project_xyz2uv.cpp:58-97 are those of
+    edge_project_xyz.cpp:57-93 with fx = fy = f), so a ba_demo-style file runs
+    on the path with the intrinsics the file holds."""
+    p = datasets.make_small_ba(seed=1000)
+    path = tmp_path / "xyz2uv.g2o"
+    path.write_text(xyz2uv_text(p, tmp_path))
+    g = gio.G2oGraph.load(str(path))
+    n = g.info()
+    assert n["xyz2uv_edges"] == n["projection_edges"] == p.num_edges and n["camera_parameters"] == 1
+    assert n["skipped"] == 0
+    q = g.to_ba_problem()
+    assert np.array_equal(q.intr, np.tile([500.0, 500.0, 320.0, 240.0], (p.num_cams, 1)))
+    for k in ("pts", "edge_cam", "edge_pt", "meas", "weight", "cam_fixed"):
+        assert np.array_equal(getattr(q, k), getattr(p, k)), k
+    assert abs(OracleBA(q).chi2() - OracleBA(p).chi2()) <= 1e-12 * OracleBA(p).chi2()
+    # saved back: the parameter first, the edges keep their tag and parameter id
+    out = tmp_path / "xyz2uv_out.g2o"
+    g.save(str(out), precision=17)
+    lines = out.read_text().splitlines()
+    assert lines[0] == "PARAMS_CAMERAPARAMETERS 0 500 320 240 0.5 "
+    assert sum(ln.startswith("EDGE_PROJECT_XYZ2UV:EXPMAP ") for ln in lines) == p.num_edges
+    r = gio.G2oGraph.load(str(out)).to_ba_problem()
+    assert np.array_equal(r.intr, q.intr) and np.array_equal(r.meas, q.meas)
+
+
+def test_camera_parameter_rules(tmp_path):
+    path = tmp_path / "prules.g2o"
+    path.write_text("VERTEX_SE3:EXPMAP 0 0 0 0 0 0 0 1\n"
+                    "VERTEX_TRACKXYZ 1 0 0 5\n"
+                    "EDGE_PROJECT_XYZ2UV:EXPMAP 1 0 3 1 2 1 0 1\n"   # parameter 3 not defined yet
+                    "PARAMS_CAMERAPARAMETERS 3 400 300 200 0.1\n"
+                    "PARAMS_CAMERAPARAMETERS 3 999 0 0 0\n"         # duplicate id
+                    "EDGE_PROJECT_XYZ2UV:EXPMAP 1 0 3 1 2 1 0 1\n")
+    g = gio.G2oGraph.load(str(path))
+    n = g.info()
+    assert (n["projection_edges"], n["camera_parameters"], n["skipped"]) == (1, 1, 2)
+    log = g.log.splitlines()
+    assert "Unable to add edge EDGE_PROJECT_XYZ2UV:EXPMAP at line 3 IDs: 1 0" in log
+    assert "Parameter of type: PARAMS_CAMERAPARAMETERS id: 3 already defined at line 5" in log
+    assert g.to_ba_problem().intr.tolist() == [[400.0, 400.0, 300.0, 200.0]]

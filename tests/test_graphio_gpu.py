@@ -87,3 +87,28 @@ def test_bal_file_chi2_and_lm(tmp_path, oracle_lib):
     for x, t in zip(s, trace):
         assert abs(x["chi2"] - t[1]) <= 1e-8 * t[1]
     assert s[-1]["chi2"] < want
+
+
+def test_xyz2uv_file_runs_like_the_plain_edges(tmp_path, oracle_lib):
+    """A ba_demo-style file (PARAMS_CAMERAPARAMETERS + EDGE_PROJECT_XYZ2UV)
+    takes its intrinsics from the file and runs as EDGE_SE3_PROJECT_XYZ edges
+    with fx = fy = f."""
+    import g2o_amd
+    p = datasets.make_small_ba(seed=1000)
+    plain = tmp_path / "plain.g2o"
+    gio.save_ba_problem(str(plain), p)
+    lines = ["PARAMS_CAMERAPARAMETERS 0 500 320 240 0.5"]
+    for ln in plain.read_text().splitlines():
+        if ln.startswith("EDGE_SE3_PROJECT_XYZ:EXPMAP "):
+            t = ln.split(" ", 3)
+            ln = f"EDGE_PROJECT_XYZ2UV:EXPMAP {t[1]} {t[2]} 0 {t[3]}"
+        lines.append(ln)
+    path = tmp_path / "xyz2uv.g2o"
+    path.write_text("\n".join(lines) + "\n")
+    q = gio.G2oGraph.load(str(path)).to_ba_problem()
+    a = g2o_amd.BundleAdjuster(q, device=0, precision=g2o_amd.FP64)
+    b = g2o_amd.BundleAdjuster(p, device=0, precision=g2o_amd.FP64)
+    sa, sb = a.optimize(4), b.optimize(4)
+    assert len(sa) == len(sb)
+    for x, y in zip(sa, sb):
+        assert abs(x["chi2"] - y["chi2"]) <= 1e-10 * y["chi2"]

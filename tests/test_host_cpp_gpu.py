@@ -104,3 +104,28 @@ def test_pose_graph_file_through_the_cpp_driver(tmp_path, mode, algorithm):
     saved = gio.G2oGraph.load(dst).to_pose_graph()
     poses = opt.state()
     assert np.max(np.abs(saved.poses[:, :3] - poses[:, :3])) < 1e-9 * max(1.0, np.max(np.abs(poses[:, :3])))
+
+
+def test_xyz2uv_file_through_the_cpp_driver(tmp_path):
+    """A ba_demo-style file carries its intrinsics: the driver needs none."""
+    from g2o_amd import graphio as gio
+    prob = datasets.make_small_ba(seed=1000)
+    plain = tmp_path / "plain.g2o"
+    gio.save_ba_problem(str(plain), prob)
+    lines = ["PARAMS_CAMERAPARAMETERS 0 500 320 240 0.5"]
+    for ln in plain.read_text().splitlines():
+        if ln.startswith("EDGE_SE3_PROJECT_XYZ:EXPMAP "):
+            t = ln.split(" ", 3)
+            ln = f"EDGE_PROJECT_XYZ2UV:EXPMAP {t[1]} {t[2]} 0 {t[3]}"
+        lines.append(ln)
+    src = tmp_path / "xyz2uv.g2o"
+    src.write_text("\n".join(lines) + "\n")
+    dst = str(tmp_path / "xyz2uv_opt.g2o")
+    out = subprocess.run([DRIVER, str(src), "optimize", "5", "0", dst], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    a = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()])
+    ba = g2o_amd.BundleAdjuster(prob, device=0, precision=g2o_amd.FP64)
+    st = ba.optimize(5)
+    np.testing.assert_allclose(a[:, 2], [s["chi2"] for s in st], rtol=1e-10)
+    saved = gio.G2oGraph.load(dst)
+    assert saved.info()["xyz2uv_edges"] == prob.num_edges
