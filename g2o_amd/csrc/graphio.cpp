@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <set>
 #include <string>
 #include <thread>
@@ -456,10 +457,23 @@ void apply_piece(g2ohip_graph* g, const Piece& pc, long long line0, std::set<std
 
 template <typename F>
 void run_threads(int n, F f) {
+  // an exception may not leave a worker thread (std::terminate): each worker
+  // records a failed allocation, and the caller sees it as std::bad_alloc
+  // after every thread has joined
+  std::vector<char> failed(n, 0);
+  auto guarded = [&](int i) {
+    try {
+      f(i);
+    } catch (const std::bad_alloc&) {
+      failed[i] = 1;
+    }
+  };
   std::vector<std::thread> th;
-  for (int i = 1; i < n; ++i) th.emplace_back(f, i);
-  f(0);
+  for (int i = 1; i < n; ++i) th.emplace_back(guarded, i);
+  guarded(0);
   for (auto& t : th) t.join();
+  for (char x : failed)
+    if (x) throw std::bad_alloc();
 }
 
 // Each value as "%.{prec}g" followed by a space: what writeVector
@@ -505,11 +519,16 @@ int g2ohip_graph_load(g2ohip_graph* g, const char* filename, int nthreads) {
   buf.shrink_to_fit();
   std::set<std::string> warned;
   long long line0 = g->lines;
-  for (int i = 0; i < nt; ++i) {
-    apply_piece(g, pieces[i], line0, warned);
-    line0 += pieces[i].lines;
-    std::vector<Rec>().swap(pieces[i].recs);
-    std::vector<double>().swap(pieces[i].vals);
+  try {
+    for (int i = 0; i < nt; ++i) {
+      apply_piece(g, pieces[i], line0, warned);
+      line0 += pieces[i].lines;
+      std::vector<Rec>().swap(pieces[i].recs);
+      std::vector<double>().swap(pieces[i].vals);
+    }
+  } catch (const std::bad_alloc&) {
+    g->lines = line0;
+    return fail(g, G2OHIP_ERR_OOM, "out of host memory while loading (the graph holds the lines before it)");
   }
   g->lines = line0;
   return G2OHIP_OK;
@@ -798,13 +817,18 @@ int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision
   const size_t blk = 1 << 16;
   bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
   std::vector<std::string> out(nt);
-  for (size_t k0 = 0; k0 < ni && ok; k0 += blk * nt) {
-    run_threads(nt, [&](int t) {
-      out[t].clear();
-      const size_t a = std::min(ni, k0 + blk * t), b = std::min(ni, a + blk);
-      for (size_t k = a; k < b; ++k) format(k, out[t]);
-    });
-    for (int t = 0; t < nt && ok; ++t) ok = std::fwrite(out[t].data(), 1, out[t].size(), f) == out[t].size();
+  try {
+    for (size_t k0 = 0; k0 < ni && ok; k0 += blk * nt) {
+      run_threads(nt, [&](int t) {
+        out[t].clear();
+        const size_t a = std::min(ni, k0 + blk * t), b = std::min(ni, a + blk);
+        for (size_t k = a; k < b; ++k) format(k, out[t]);
+      });
+      for (int t = 0; t < nt && ok; ++t) ok = std::fwrite(out[t].data(), 1, out[t].size(), f) == out[t].size();
+    }
+  } catch (const std::bad_alloc&) {
+    std::fclose(f);
+    return fail(g, G2OHIP_ERR_OOM, "out of host memory while writing");
   }
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) return fail(g, G2OHIP_ERR_IO, std::string("Unable to write file ") + filename);
@@ -845,10 +869,15 @@ int g2ohip_graph_load_bal(g2ohip_graph* g, const char* filename, int nthreads) {
     return fail(g, G2OHIP_ERR_OOM, "out of host memory while parsing");
   }
   std::vector<double> all;
-  for (int i = 0; i < nt; ++i) {
+  for (int i = 0; i < nt; ++i)
     if (bad[i]) return fail(g, G2OHIP_ERR_INVALID, std::string("malformed number in BAL file ") + filename);
-    all.insert(all.end(), tok[i].begin(), tok[i].end());
-    std::vector<double>().swap(tok[i]);
+  try {
+    for (int i = 0; i < nt; ++i) {
+      all.insert(all.end(), tok[i].begin(), tok[i].end());
+      std::vector<double>().swap(tok[i]);
+    }
+  } catch (const std::bad_alloc&) {
+    return fail(g, G2OHIP_ERR_OOM, "out of host memory while parsing");
   }
   auto integral = [](double v) { return v == std::floor(v) && v >= 0 && v < 2147483648.0; };
   if (all.size() < 3 || !integral(all[0]) || !integral(all[1]) || !integral(all[2]))
