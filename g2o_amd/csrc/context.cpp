@@ -226,7 +226,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_bschur.alloc(6 * static_cast<size_t>(st.Cf)));
   HK(d_res.alloc(6 * static_cast<size_t>(st.Cf)));
   HK(d_xp.alloc(6 * static_cast<size_t>(st.Cf)));
-  HK(d_red.alloc(RED_BLOCKS));
+  HK(d_red.alloc(std::max(RED_BLOCKS, CHI2_BLOCKS)));
   HK(d_scal.alloc(8));
   HK(d_dq_part.alloc(std::max(st.Cf, 1)));
   HK(d_dn_part.alloc(std::max(st.Cf, 1)));

@@ -9,6 +9,9 @@ constexpr int LIN_THREADS = 256;
 constexpr int LIN_CAMS = 128;  // free cameras per linearisation chunk
 constexpr int LIN_PTS = 256;   // points per linearisation chunk
 constexpr int RED_BLOCKS = 512;  // fixed grid of the grid-stride reductions
+// fixed grid of the chi2 pass: each thread's slots form a chain of dependent
+// gathers, so the pass wants more waves in flight than the other reductions
+constexpr int CHI2_BLOCKS = 2048;
 
 // Device view of the structure (all arrays device-resident, read-only).
 struct DevGraph {

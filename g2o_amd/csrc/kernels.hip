@@ -146,8 +146,8 @@ hipError_t launch_sum(const double* partial, int n, double* out, hipStream_t st)
 
 hipError_t launch_chi2(const DevGraph& g, const double* camRt, const double* pts, double* partial, double* out,
                        double* err_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_chi2, dim3(RED_BLOCKS), dim3(256), 0, st, g, camRt, pts, partial, err_out);
-  return launch_sum(partial, RED_BLOCKS, out, st);
+  hipLaunchKernelGGL(k_chi2, dim3(CHI2_BLOCKS), dim3(256), 0, st, g, camRt, pts, partial, err_out);
+  return launch_sum(partial, CHI2_BLOCKS, out, st);
 }
 
 // SparseOptimizer::update (sparse_optimizer.cpp:457-468): exp-map left update
