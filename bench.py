@@ -122,7 +122,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=4)
     ap.add_argument("--cpu-iters", type=int, default=None, help="oracle LM iterations (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cams", type=int, default=1500)

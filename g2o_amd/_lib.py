@@ -21,6 +21,7 @@ LEVENBERG, GAUSS_NEWTON, DOGLEG = 0, 1, 2
 EXPORTS = [
     "g2ohip_default_options", "g2ohip_create", "g2ohip_destroy", "g2ohip_last_error", "g2ohip_set_options",
     "g2ohip_get_options", "g2ohip_ba_set_structure", "g2ohip_ba_structure_info", "g2ohip_ba_set_measurements",
+    "g2ohip_ba_set_information",
     "g2ohip_ba_set_state", "g2ohip_ba_get_state", "g2ohip_compute_active_errors", "g2ohip_get_errors",
     "g2ohip_build_system", "g2ohip_set_lambda", "g2ohip_restore_diagonal", "g2ohip_solve", "g2ohip_get_x",
     "g2ohip_get_b", "g2ohip_update", "g2ohip_push", "g2ohip_pop", "g2ohip_discard_top", "g2ohip_lambda_init",
@@ -79,6 +80,7 @@ def lib():
         "g2ohip_ba_set_structure": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _up, _up]),
         "g2ohip_ba_structure_info": (ct.c_int, [_vp, ct.POINTER(ct.c_longlong)]),
         "g2ohip_ba_set_measurements": (ct.c_int, [_vp, _dp, _dp, _dp, ct.c_double]),
+        "g2ohip_ba_set_information": (ct.c_int, [_vp, _dp]),
         "g2ohip_ba_set_state": (ct.c_int, [_vp, _dp, _dp]),
         "g2ohip_ba_get_state": (ct.c_int, [_vp, _dp, _dp]),
         "g2ohip_compute_active_errors": (ct.c_int, [_vp, _dp]),

@@ -55,7 +55,8 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize_exact(DevGraph g, con
     const double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
     double Y[3], e[2], H[81], rho[3];
     project_error(Rt, in, X, z, Y, e);
-    edge_exact_hessian(Rt, in[0], in[1], Y, e, g.slot_w[s], g.huber, H, rho);
+    const double* S = g.slot_S ? g.slot_S + 3 * static_cast<size_t>(s) : nullptr;
+    edge_exact_hessian(Rt, in[0], in[1], Y, e, g.slot_w[s], g.huber, S, H, rho);
     if (ph >= 0) {
       double* pa = s_pacc + 6 * lp;
       int k = 0;
@@ -126,8 +127,10 @@ template hipError_t launch_backsub_blocks<double>(const DevGraph&, const double*
                                                   double*, hipStream_t);
 
 // Adjoint outputs per slot from lam = H^-1 dL/dtheta: with v = Je lam (the
-// edge's camera and point parts; zero for fixed vertices),
-//   dL/dz = -(rho' w v + 2 rho'' w^2 e (e . v)),  dL/dw = -(rho' + rho'' s)(e . v)
+// edge's camera and point parts; zero for fixed vertices) and the edge's
+// information Omega (identity unless set),
+//   dL/dz = -(rho' w Omega v + 2 rho'' w^2 Omega e (Omega e . v)),
+//   dL/dw = -(rho' + rho'' s)(Omega e . v),  s = w e^T Omega e
 // (the mixed derivatives d grad / dz and d grad / dw of the edge energy).
 __global__ __launch_bounds__(256) void k_adjoint_out(DevGraph g, const double* __restrict__ camRt,
                                                      const double* __restrict__ pts, const double* __restrict__ lam,
@@ -155,13 +158,17 @@ __global__ __launch_bounds__(256) void k_adjoint_out(DevGraph g, const double* _
     const double* ll = lam + 6 * static_cast<size_t>(g.Cf) + 3 * static_cast<size_t>(ph);
     for (int k = 0; k < 3; ++k) { v[0] += Jp[k] * ll[k]; v[1] += Jp[3 + k] * ll[k]; }
   }
+  double om[3] = {1.0, 0.0, 1.0};
+  if (g.slot_S) info_of(g.slot_S + 3 * static_cast<size_t>(s), om);
+  const double oe[2] = {om[0] * e[0] + om[1] * e[1], om[1] * e[0] + om[2] * e[1]};
+  const double ov[2] = {om[0] * v[0] + om[1] * v[1], om[1] * v[0] + om[2] * v[1]};
   double rho[3];
-  const double sq = w * (e[0] * e[0] + e[1] * e[1]);
+  const double sq = w * (e[0] * oe[0] + e[1] * oe[1]);
   huber_rho(sq, g.huber, rho);
-  const double ev = e[0] * v[0] + e[1] * v[1];
+  const double ev = oe[0] * v[0] + oe[1] * v[1];
   const double hb = 2.0 * rho[2] * w * w * ev;
-  out[3 * static_cast<size_t>(s)] = -(rho[1] * w * v[0] + hb * e[0]);
-  out[3 * static_cast<size_t>(s) + 1] = -(rho[1] * w * v[1] + hb * e[1]);
+  out[3 * static_cast<size_t>(s)] = -(rho[1] * w * ov[0] + hb * oe[0]);
+  out[3 * static_cast<size_t>(s) + 1] = -(rho[1] * w * ov[1] + hb * oe[1]);
   out[3 * static_cast<size_t>(s) + 2] = -(rho[1] + rho[2] * sq) * ev;
 }
 

@@ -6,6 +6,28 @@
 
 #include "context.h"
 
+namespace {
+
+// Runs one entry point's body with the context's device current. No C++
+// exception leaves the ABI: a host allocation failure (std::bad_alloc from a
+// container of the structure builder, the window planner or a host copy)
+// becomes G2OHIP_ERR_OOM and anything else G2OHIP_ERR_INVALID, each with a
+// last-error message.
+template <typename F>
+int guarded(g2ohip_ctx* ctx, F&& body) noexcept {
+  if (!ctx) return G2OHIP_ERR_INVALID;
+  if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return ctx->fail_nothrow(G2OHIP_ERR_OOM, "out of host memory");
+  } catch (...) {
+    return ctx->fail_nothrow(G2OHIP_ERR_INVALID, "unexpected host exception");
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 void g2ohip_default_options(g2ohip_options* opt) {
@@ -14,13 +36,13 @@ void g2ohip_default_options(g2ohip_options* opt) {
   opt->schur_mode = G2OHIP_SCHUR_EXPLICIT;
   opt->pcg_tolerance = 1e-6;
   opt->pcg_max_iter = -1;
-  opt->refine_steps = 1;
+  opt->refine_steps = 4;
   opt->max_trials = 10;
   opt->user_lambda_init = 0.0;
   opt->tau = 1e-5;
 }
 
-int g2ohip_create(int device, g2ohip_ctx** out) {
+int g2ohip_create(int device, g2ohip_ctx** out) try {
   if (!out) return G2OHIP_ERR_INVALID;
   *out = nullptr;
   g2ohip_ctx* c = new (std::nothrow) g2ohip_ctx();
@@ -60,6 +82,8 @@ int g2ohip_create(int device, g2ohip_ctx** out) {
   }
   *out = c;
   return G2OHIP_OK;
+} catch (...) {
+  return G2OHIP_ERR_OOM;
 }
 
 void g2ohip_destroy(g2ohip_ctx* ctx) {
@@ -70,23 +94,20 @@ void g2ohip_destroy(g2ohip_ctx* ctx) {
 
 const char* g2ohip_last_error(const g2ohip_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
 
-#define G2OHIP_ENTER(ctx)                                   \
-  if (!(ctx)) return G2OHIP_ERR_INVALID;                    \
-  if ((ctx)->device >= 0) (void)hipSetDevice((ctx)->device);
-
 int g2ohip_set_options(g2ohip_ctx* ctx, const g2ohip_options* opt) {
-  G2OHIP_ENTER(ctx);
-  if (!opt) return ctx->fail(G2OHIP_ERR_INVALID, "null options");
-  if (opt->precision != G2OHIP_FP64 && opt->precision != G2OHIP_FP32)
-    return ctx->fail(G2OHIP_ERR_INVALID, "unknown precision");
-  if (opt->schur_mode != G2OHIP_SCHUR_EXPLICIT && opt->schur_mode != G2OHIP_SCHUR_IMPLICIT)
-    return ctx->fail(G2OHIP_ERR_INVALID, "unknown Schur mode");
-  if (!(opt->pcg_tolerance >= 0) || opt->max_trials < 1 || opt->refine_steps < 0 || opt->refine_steps > 16)
-    return ctx->fail(G2OHIP_ERR_INVALID, "invalid option value");
-  ctx->opt = *opt;
-  ctx->pcg_residual_mem = -1.0;  // LinearSolverPCG::init
-  ctx->fp32_off = false;
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!opt) return ctx->fail(G2OHIP_ERR_INVALID, "null options");
+    if (opt->precision != G2OHIP_FP64 && opt->precision != G2OHIP_FP32)
+      return ctx->fail(G2OHIP_ERR_INVALID, "unknown precision");
+    if (opt->schur_mode != G2OHIP_SCHUR_EXPLICIT && opt->schur_mode != G2OHIP_SCHUR_IMPLICIT)
+      return ctx->fail(G2OHIP_ERR_INVALID, "unknown Schur mode");
+    if (!(opt->pcg_tolerance >= 0) || opt->max_trials < 1 || opt->refine_steps < 0 || opt->refine_steps > 16)
+      return ctx->fail(G2OHIP_ERR_INVALID, "invalid option value");
+    ctx->opt = *opt;
+    ctx->pcg_residual_mem = -1.0;  // LinearSolverPCG::init
+    ctx->fp32_off = false;
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_get_options(const g2ohip_ctx* ctx, g2ohip_options* opt) {
@@ -97,253 +118,295 @@ int g2ohip_get_options(const g2ohip_ctx* ctx, g2ohip_options* opt) {
 
 int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_edges, const int* edge_cam,
                             const int* edge_pt, const uint8_t* cam_fixed, const uint8_t* pt_fixed) {
-  G2OHIP_ENTER(ctx);
-  return ctx->set_structure(num_cams, num_pts, num_edges, edge_cam, edge_pt, cam_fixed, pt_fixed);
+  return guarded(ctx, [&]() -> int {
+    return ctx->set_structure(num_cams, num_pts, num_edges, edge_cam, edge_pt, cam_fixed, pt_fixed);
+  });
 }
 
 int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
-  G2OHIP_ENTER(ctx);
-  if (!ctx->have_structure || !info) return ctx->fail(G2OHIP_ERR_INVALID, "no structure");
-  int rc = ctx->ensure_win(0);
-  if (rc) return rc;
-  const auto& st = ctx->st;
-  int maxn = 0;
-  for (int i = 0; i < st.npos; ++i) maxn = std::max(maxn, st.pos_ptr[i + 1] - st.pos_ptr[i]);
-  info[0] = st.Cf;
-  info[1] = st.Pf;
-  info[2] = st.ns;
-  info[3] = st.nS;
-  info[4] = st.nlc;
-  const auto& wp = st.win[0];
-  info[5] = static_cast<long long>(wp.win_obs0.size()) - 1;
-  info[6] = wp.pairs;
-  info[7] = wp.od_ptr.back();
-  info[8] = maxn;
-  info[9] = st.nF;
-  info[10] = static_cast<long long>(st.lc_cams.size());
-  info[11] = st.npos;
-  info[12] = st.nq;
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx->have_structure || !info) return ctx->fail(G2OHIP_ERR_INVALID, "no structure");
+    if (ctx->mode != 0) return ctx->fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+    int rc = ctx->ensure_win(0);
+    if (rc) return rc;
+    const auto& st = ctx->st;
+    int maxn = 0;
+    for (int i = 0; i < st.npos; ++i) maxn = std::max(maxn, st.pos_ptr[i + 1] - st.pos_ptr[i]);
+    info[0] = st.Cf;
+    info[1] = st.Pf;
+    info[2] = st.ns;
+    info[3] = st.nS;
+    info[4] = st.nlc;
+    const auto& wp = st.win[0];
+    info[5] = static_cast<long long>(wp.win_obs0.size()) - 1;
+    info[6] = wp.pairs;
+    info[7] = wp.od_ptr.back();
+    info[8] = maxn;
+    info[9] = st.nF;
+    info[10] = static_cast<long long>(st.lc_cams.size());
+    info[11] = st.npos;
+    info[12] = st.nq;
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_ba_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* weight, const double* intr,
                                double huber_delta) {
-  G2OHIP_ENTER(ctx);
-  return ctx->set_measurements(meas, weight, intr, huber_delta);
+  return guarded(ctx, [&]() -> int {
+    return ctx->set_measurements(meas, weight, intr, huber_delta);
+  });
+}
+
+int g2ohip_ba_set_information(g2ohip_ctx* ctx, const double* omega) {
+  return guarded(ctx, [&]() -> int { return ctx->set_information(omega); });
 }
 
 int g2ohip_ba_set_state(g2ohip_ctx* ctx, const double* cams, const double* pts) {
-  G2OHIP_ENTER(ctx);
-  if (!cams || (!pts && ctx->st.P > 0)) return ctx->fail(G2OHIP_ERR_INVALID, "null state");
-  return ctx->set_state(cams, pts);
+  return guarded(ctx, [&]() -> int {
+    if (!cams || (!pts && ctx->st.P > 0)) return ctx->fail(G2OHIP_ERR_INVALID, "null state");
+    return ctx->set_state(cams, pts);
+  });
 }
 
 int g2ohip_ba_get_state(g2ohip_ctx* ctx, double* cams, double* pts) {
-  G2OHIP_ENTER(ctx);
-  return ctx->get_state(cams, pts);
+  return guarded(ctx, [&]() -> int {
+    return ctx->get_state(cams, pts);
+  });
 }
 
 int g2ohip_compute_active_errors(g2ohip_ctx* ctx, double* chi2) {
-  G2OHIP_ENTER(ctx);
-  if (!chi2) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  return ctx->compute_active_errors(chi2);
+  return guarded(ctx, [&]() -> int {
+    if (!chi2) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    return ctx->compute_active_errors(chi2);
+  });
 }
 
 int g2ohip_get_errors(g2ohip_ctx* ctx, double* err) {
-  G2OHIP_ENTER(ctx);
-  if (!err) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  double chi;
-  return ctx->compute_active_errors(&chi, err);
+  return guarded(ctx, [&]() -> int {
+    if (!err) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    double chi;
+    return ctx->compute_active_errors(&chi, err);
+  });
 }
 
 int g2ohip_build_system(g2ohip_ctx* ctx) {
-  G2OHIP_ENTER(ctx);
-  return ctx->build_system();
+  return guarded(ctx, [&]() -> int {
+    return ctx->build_system();
+  });
 }
 
 int g2ohip_set_lambda(g2ohip_ctx* ctx, double lambda) {
-  G2OHIP_ENTER(ctx);
-  if (!(lambda >= 0) && !(lambda < 0)) return ctx->fail(G2OHIP_ERR_INVALID, "lambda is NaN");
-  ctx->lambda = lambda;
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!(lambda >= 0) && !(lambda < 0)) return ctx->fail(G2OHIP_ERR_INVALID, "lambda is NaN");
+    ctx->lambda = lambda;
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_restore_diagonal(g2ohip_ctx* ctx) {
-  G2OHIP_ENTER(ctx);
-  // the damping lives only inside the solve kernels; nothing to restore
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    // the damping lives only inside the solve kernels; nothing to restore
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_solve(g2ohip_ctx* ctx, int* ok) {
-  G2OHIP_ENTER(ctx);
-  if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  return ctx->solve(ok);
+  return guarded(ctx, [&]() -> int {
+    if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    return ctx->solve(ok);
+  });
 }
 
 int g2ohip_get_x(g2ohip_ctx* ctx, double* x) {
-  G2OHIP_ENTER(ctx);
-  if (!x || !ctx->have_solution) return ctx->fail(G2OHIP_ERR_INVALID, "no solution");
-  size_t n = ctx->system_size();
-  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
-      hipMemcpy(x, ctx->d_x.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-    return ctx->fail(G2OHIP_ERR_HIP, "copy of x failed");
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!x || !ctx->have_solution) return ctx->fail(G2OHIP_ERR_INVALID, "no solution");
+    size_t n = ctx->system_size();
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(x, ctx->d_x.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return ctx->fail(G2OHIP_ERR_HIP, "copy of x failed");
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_get_b(g2ohip_ctx* ctx, double* b) {
-  G2OHIP_ENTER(ctx);
-  if (!b || !ctx->have_system) return ctx->fail(G2OHIP_ERR_INVALID, "no system");
-  size_t n = ctx->system_size();
-  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
-      hipMemcpy(b, ctx->d_b.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-    return ctx->fail(G2OHIP_ERR_HIP, "copy of b failed");
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!b || !ctx->have_system) return ctx->fail(G2OHIP_ERR_INVALID, "no system");
+    size_t n = ctx->system_size();
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(b, ctx->d_b.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return ctx->fail(G2OHIP_ERR_HIP, "copy of b failed");
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_update(g2ohip_ctx* ctx) {
-  G2OHIP_ENTER(ctx);
-  return ctx->update();
+  return guarded(ctx, [&]() -> int {
+    return ctx->update();
+  });
 }
 
 int g2ohip_push(g2ohip_ctx* ctx) {
-  G2OHIP_ENTER(ctx);
-  return ctx->push();
+  return guarded(ctx, [&]() -> int {
+    return ctx->push();
+  });
 }
 
 int g2ohip_pop(g2ohip_ctx* ctx) {
-  G2OHIP_ENTER(ctx);
-  return ctx->pop();
+  return guarded(ctx, [&]() -> int {
+    return ctx->pop();
+  });
 }
 
 int g2ohip_discard_top(g2ohip_ctx* ctx) {
-  G2OHIP_ENTER(ctx);
-  return ctx->discard_top();
+  return guarded(ctx, [&]() -> int {
+    return ctx->discard_top();
+  });
 }
 
 int g2ohip_lambda_init(g2ohip_ctx* ctx, double* lambda) {
-  G2OHIP_ENTER(ctx);
-  if (!lambda) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  return ctx->lambda_init(lambda);
+  return guarded(ctx, [&]() -> int {
+    if (!lambda) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    return ctx->lambda_init(lambda);
+  });
 }
 
 int g2ohip_compute_scale(g2ohip_ctx* ctx, double* scale) {
-  G2OHIP_ENTER(ctx);
-  if (!scale) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  return ctx->compute_scale(scale);
+  return guarded(ctx, [&]() -> int {
+    if (!scale) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    return ctx->compute_scale(scale);
+  });
 }
 
 int g2ohip_optimize(g2ohip_ctx* ctx, int iterations, g2ohip_iter_stats* stats, int* done) {
-  G2OHIP_ENTER(ctx);
-  int d = 0;
-  int rc = ctx->optimize(iterations, stats, &d);
-  if (done) *done = d;
-  return rc;
+  return guarded(ctx, [&]() -> int {
+    int d = 0;
+    int rc = ctx->optimize(iterations, stats, &d);
+    if (done) *done = d;
+    return rc;
+  });
 }
 
 int g2ohip_compute_marginals(g2ohip_ctx* ctx, int nblocks, const int* rows, const int* cols, double* blocks,
                              double tol) {
-  G2OHIP_ENTER(ctx);
-  return ctx->marginals(nblocks, rows, cols, blocks, tol);
+  return guarded(ctx, [&]() -> int {
+    return ctx->marginals(nblocks, rows, cols, blocks, tol);
+  });
 }
 
 int g2ohip_set_algorithm(g2ohip_ctx* ctx, int algorithm) {
-  G2OHIP_ENTER(ctx);
-  if (algorithm != G2OHIP_LEVENBERG && algorithm != G2OHIP_GAUSS_NEWTON && algorithm != G2OHIP_DOGLEG)
-    return ctx->fail(G2OHIP_ERR_INVALID, "unknown algorithm");
-  ctx->algorithm = algorithm;
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (algorithm != G2OHIP_LEVENBERG && algorithm != G2OHIP_GAUSS_NEWTON && algorithm != G2OHIP_DOGLEG)
+      return ctx->fail(G2OHIP_ERR_INVALID, "unknown algorithm");
+    ctx->algorithm = algorithm;
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_set_force_stop_flag(g2ohip_ctx* ctx, const volatile int* flag) {
-  G2OHIP_ENTER(ctx);
-  ctx->force_stop = flag;
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    ctx->force_stop = flag;
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_ba_get_hessian(g2ohip_ctx* ctx, double* Hpp, double* Hll, double* Hpl_edge) {
-  G2OHIP_ENTER(ctx);
-  return ctx->get_hessian(Hpp, Hll, Hpl_edge);
+  return guarded(ctx, [&]() -> int {
+    return ctx->get_hessian(Hpp, Hll, Hpl_edge);
+  });
 }
 
 int g2ohip_ba_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* bschur) {
-  G2OHIP_ENTER(ctx);
-  return ctx->get_schur(rows, cols, blocks, bschur);
+  return guarded(ctx, [&]() -> int {
+    return ctx->get_schur(rows, cols, blocks, bschur);
+  });
 }
 
 int g2ohip_last_solve_info(g2ohip_ctx* ctx, double* info) {
-  G2OHIP_ENTER(ctx);
-  if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  for (int i = 0; i < 4; ++i) info[i] = ctx->last_info[i];
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    for (int i = 0; i < 4; ++i) info[i] = ctx->last_info[i];
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_set_profiling(g2ohip_ctx* ctx, int on) {
-  G2OHIP_ENTER(ctx);
-  return ctx->set_profiling(on);
+  return guarded(ctx, [&]() -> int {
+    return ctx->set_profiling(on);
+  });
 }
 
 int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launches) {
-  G2OHIP_ENTER(ctx);
-  return ctx->get_profile(kernel, total_ms, launches);
+  return guarded(ctx, [&]() -> int {
+    return ctx->get_profile(kernel, total_ms, launches);
+  });
 }
 
 int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int* brow, const int* bcol,
                             const double* blocks, const double* b, double* x, double tol, int max_iter,
                             double* residual_io, int* iterations) {
-  G2OHIP_ENTER(ctx);
-  return ctx->linear_solve_pcg(nb, bd, nblk, brow, bcol, blocks, b, x, tol, max_iter, residual_io, iterations);
+  return guarded(ctx, [&]() -> int {
+    return ctx->linear_solve_pcg(nb, bd, nblk, brow, bcol, blocks, b, x, tol, max_iter, residual_io, iterations);
+  });
 }
 
 int g2ohip_ba_adjoint(g2ohip_ctx* ctx, const double* dL_dcam, const double* dL_dpt, double* dL_dmeas,
                       double* dL_dweight, double tol, int max_iter, int* iterations) {
-  G2OHIP_ENTER(ctx);
-  return ctx->adjoint(dL_dcam, dL_dpt, dL_dmeas, dL_dweight, tol < 0 ? 1e-24 : tol, max_iter, iterations);
+  return guarded(ctx, [&]() -> int {
+    return ctx->adjoint(dL_dcam, dL_dpt, dL_dmeas, dL_dweight, tol < 0 ? 1e-24 : tol, max_iter, iterations);
+  });
 }
 
 int g2ohip_ba_newton(g2ohip_ctx* ctx, int steps, double tol, double max_step, int* ok) {
-  G2OHIP_ENTER(ctx);
-  if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "ok must not be NULL");
-  return ctx->newton(steps, tol < 0 ? 1e-24 : tol, max_step, ok);
+  return guarded(ctx, [&]() -> int {
+    if (!ok) return ctx->fail(G2OHIP_ERR_INVALID, "ok must not be NULL");
+    return ctx->newton(steps, tol < 0 ? 1e-24 : tol, max_step, ok);
+  });
 }
 
 // ------------------------------------------------------------ pose graph
 int g2ohip_pg_set_structure(g2ohip_ctx* ctx, int num_poses, int num_edges, const int* edge_from, const int* edge_to,
                             const uint8_t* fixed) {
-  G2OHIP_ENTER(ctx);
-  return ctx->pg_set_structure(num_poses, num_edges, edge_from, edge_to, fixed);
+  return guarded(ctx, [&]() -> int {
+    return ctx->pg_set_structure(num_poses, num_edges, edge_from, edge_to, fixed);
+  });
 }
 
 int g2ohip_pg_structure_info(g2ohip_ctx* ctx, long long* info) {
-  G2OHIP_ENTER(ctx);
-  if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  if (!ctx->have_structure || ctx->mode != 1) return ctx->fail(G2OHIP_ERR_INVALID, "no pose-graph structure");
-  info[0] = ctx->pgst.Nf;
-  info[1] = static_cast<long long>(ctx->pgst.act.size());
-  info[2] = ctx->pgst.nU;
-  info[3] = ctx->pgst.nF;
-  return G2OHIP_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    if (!ctx->have_structure || ctx->mode != 1) return ctx->fail(G2OHIP_ERR_INVALID, "no pose-graph structure");
+    info[0] = ctx->pgst.Nf;
+    info[1] = static_cast<long long>(ctx->pgst.act.size());
+    info[2] = ctx->pgst.nU;
+    info[3] = ctx->pgst.nF;
+    return G2OHIP_OK;
+  });
 }
 
 int g2ohip_pg_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* information) {
-  G2OHIP_ENTER(ctx);
-  return ctx->pg_set_measurements(meas, information);
+  return guarded(ctx, [&]() -> int {
+    return ctx->pg_set_measurements(meas, information);
+  });
 }
 
 int g2ohip_pg_set_state(g2ohip_ctx* ctx, const double* poses) {
-  G2OHIP_ENTER(ctx);
-  if (!poses) return ctx->fail(G2OHIP_ERR_INVALID, "null poses");
-  return ctx->pg_set_state(poses);
+  return guarded(ctx, [&]() -> int {
+    if (!poses) return ctx->fail(G2OHIP_ERR_INVALID, "null poses");
+    return ctx->pg_set_state(poses);
+  });
 }
 
 int g2ohip_pg_get_state(g2ohip_ctx* ctx, double* poses) {
-  G2OHIP_ENTER(ctx);
-  if (!poses) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
-  return ctx->pg_get_state(poses);
+  return guarded(ctx, [&]() -> int {
+    if (!poses) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    return ctx->pg_get_state(poses);
+  });
 }
 
 int g2ohip_pg_get_system(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* b) {
-  G2OHIP_ENTER(ctx);
-  return ctx->pg_get_system(rows, cols, blocks, b);
+  return guarded(ctx, [&]() -> int {
+    return ctx->pg_get_system(rows, cols, blocks, b);
+  });
 }
 
 }  // extern "C"

@@ -38,7 +38,15 @@ g2ohip_ctx::~g2ohip_ctx() {
 }
 
 int g2ohip_ctx::fail(int code, const std::string& msg) {
-  last_error = msg;
+  return fail_nothrow(code, msg.c_str());
+}
+
+int g2ohip_ctx::fail_nothrow(int code, const char* msg) noexcept {
+  try {
+    last_error.assign(msg);
+  } catch (...) {
+    last_error.clear();  // no memory for the message: the code still reports the failure
+  }
   return code;
 }
 
@@ -224,6 +232,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_cam_part.alloc(6 * std::max<size_t>(st.lc_cams.size(), 1)));
   HK(d_chunk_chi.alloc(std::max(st.nlc, 1)));
   HK(d_bschur.alloc(6 * static_cast<size_t>(st.Cf)));
+  HK(d_Sdiag.alloc(36 * static_cast<size_t>(std::max(st.Cf, 1))));
   HK(d_res.alloc(6 * static_cast<size_t>(st.Cf)));
   HK(d_xp.alloc(6 * static_cast<size_t>(st.Cf)));
   HK(d_red.alloc(std::max(RED_BLOCKS, CHI2_BLOCKS)));
@@ -248,6 +257,8 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   g.camp_ptr = d_camp_ptr.p; g.camp_idx = d_camp_idx.p;
   g.S_row = d_S_row.p; g.S_col = d_S_col.p; g.S_diag = d_S_diag.p; g.up_pos = d_up_pos.p; g.up_posT = d_up_posT.p;
   g.F_ptr = d_F_ptr.p; g.F_col = d_F_col.p; g.F_diag = d_F_diag.p;
+  d_S.release();  // information back to identity
+  g.slot_S = nullptr;
   HK(hipStreamSynchronize(stream));
   return G2OHIP_OK;
 }
@@ -306,6 +317,45 @@ int g2ohip_ctx::set_measurements(const double* meas, const double* w, const doub
   g.intr = d_intr.p;
   g.huber = huber;
   have_meas = true;
+  have_system = have_lin = false;
+  return G2OHIP_OK;
+}
+
+// BaseEdge::setInformation for every projection edge (base_edge.h:165-180):
+// omega[E*3] = [O00 O01 O11], the upper triangle of each edge's 2x2
+// information matrix, as the g2o file format stores it. The kernels take the
+// factor S (Omega = S^T S, upper triangular) per slot; all-identity input, or
+// omega == nullptr, drops it (the unwhitened fast path).
+int g2ohip_ctx::set_information(const double* omega) {
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
+  CK(require_device());
+  std::vector<double> S;
+  bool identity = true;
+  if (omega) {
+    S.resize(3 * static_cast<size_t>(st.ns));
+    for (int s = 0; s < st.ns; ++s) {
+      const double* o = omega + 3 * static_cast<size_t>(st.slot_edge[s]);
+      if (!(o[0] == 1.0 && o[1] == 0.0 && o[2] == 1.0)) identity = false;
+      const double s0 = std::sqrt(o[0]);
+      const double s1 = o[1] / s0;
+      const double d = o[2] - s1 * s1;
+      if (!(o[0] > 0.0) || !(d > 0.0) || !std::isfinite(o[0]) || !std::isfinite(o[1]) || !std::isfinite(o[2]))
+        return fail(G2OHIP_ERR_INVALID, "information matrix of edge " + std::to_string(st.slot_edge[s]) +
+                                            " is not positive definite");
+      S[3 * static_cast<size_t>(s)] = s0;
+      S[3 * static_cast<size_t>(s) + 1] = s1;
+      S[3 * static_cast<size_t>(s) + 2] = std::sqrt(d);
+    }
+  }
+  if (identity) {
+    d_S.release();
+    g.slot_S = nullptr;
+  } else {
+    HK(d_S.upload(S, stream));
+    HK(hipStreamSynchronize(stream));
+    g.slot_S = d_S.p;
+  }
   have_system = have_lin = false;
   return G2OHIP_OK;
 }
@@ -445,7 +495,8 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
     auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
       const int pb = prof_begin();
       const hipError_t e = launch_pcg_dir_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, d_old, d, B.q.p,
-                                                      d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream);
+                                                      d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream,
+                                                      std::is_same<HT, float>::value ? d_Sdiag.p : nullptr);
       prof_end(G2OHIP_K_PCG_SPMV, pb);
       return e;
     };
@@ -493,9 +544,11 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
     // diagonal blocks, block-Jacobi inverses and (explicit) the reduced
     // right-hand side in FP64, re-linearised from the state point-major (FP32
     // blocks are too coarse for the cancellation in Hpp - sum V Dinv V^T)
+    // FP32 blocks: the diagonal blocks stay FP64 (d_Sdiag) for the SpMV
+    double* sdiag = (!implicit && std::is_same<HT, float>::value) ? d_Sdiag.p : nullptr;
     HK(launch_schur_diag_pm<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, implicit ? nullptr : d_b.p, lambda,
                                 d_diag_part.p, implicit ? nullptr : B.F_val.p, B.Jinv.p,
-                                implicit ? nullptr : d_bschur.p, d_err.p, stream));
+                                implicit ? nullptr : d_bschur.p, sdiag, d_err.p, stream));
   }
   if (implicit) {
     // matrix-free reduced operator: right-hand side by one pass
@@ -520,13 +573,18 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   int total_it = pr.iters;
   double dn_final = pr.dn_final;
   HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 0.0, stream));
+  int steps_run = 0;
   if (refine) {
     // FP64 iterative refinement: r = g - S x evaluated in FP64 matrix-free from
     // the state (no FP32 block enters it), then an FP32 PCG correction S dx = r
-    // that stops on the first solve's absolute threshold d0, so the refined x
-    // meets the reference stop rule measured on the FP64 residual. A
-    // correction runs at most twice the first solve's iterations plus 10.
+    // that stops on the first solve's absolute threshold d0. Corrections are
+    // repeated until one finds the FP64 residual already at the threshold
+    // (zero iterations), at most refine_steps times: the refined x then meets
+    // the reference stop rule measured on the FP64 residual, not only on the
+    // FP32 recurrence. A correction runs at most twice the first solve's
+    // iterations plus 10.
     for (int step = 0; step < opt.refine_steps; ++step) {
+      ++steps_run;
       xl_current = true;  // the residual pass writes x_l for the current x_p
       {
         ProfScope ps(this, G2OHIP_K_REFINE);
@@ -547,10 +605,9 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
       // threshold the solve stopped on, also when a correction stopped at
       // its iteration cap above it
       dn_final = std::min(pc.dn_final, pr.d0);
-      if (pc.iters > 0) {
-        HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 1.0, stream));
-        xl_current = false;
-      }
+      if (pc.iters == 0) break;  // the FP64 residual meets the threshold
+      HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 1.0, stream));
+      xl_current = false;
     }
   }
   pcg_residual_mem = 0.5 * dn_final;
@@ -561,7 +618,7 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
     ProfScope ps(this, G2OHIP_K_REFINE);
     HK(launch_refine(g, d_camRt.p, d_pts.p, lambda, xp64, xl64, nullptr, d_err.p, stream));
   }
-  last_info[1] = refine ? opt.refine_steps : 0;
+  last_info[1] = steps_run;
   if (!defer_err) {
     int err = 0;
     CK(read_err(&err));
@@ -1022,6 +1079,12 @@ int g2ohip_ctx::get_schur(int* rows, int* cols, double* blocks, double* bschur) 
       std::vector<float> Ff(F.size());
       HK(hipMemcpy(Ff.data(), f32.F_val.p, Ff.size() * sizeof(float), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < F.size(); ++i) F[i] = Ff[i];
+      // the FP32 operator keeps its diagonal blocks in FP64
+      std::vector<double> D(36 * static_cast<size_t>(st.Cf));
+      HK(hipMemcpy(D.data(), d_Sdiag.p, D.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int c = 0; c < st.Cf; ++c)
+        std::copy(&D[36 * static_cast<size_t>(c)], &D[36 * static_cast<size_t>(c)] + 36,
+                  &F[36 * static_cast<size_t>(st.up_pos[st.S_diag[c]])]);
     } else {
       HK(hipMemcpy(F.data(), f64.F_val.p, F.size() * sizeof(double), hipMemcpyDeviceToHost));
     }

@@ -104,6 +104,7 @@ struct g2ohip_ctx {
       d_F_col, d_F_diag, d_iota;
   g2ohip::DBuf<int4> d_slot_rec;
   g2ohip::DBuf<double> d_meas, d_w, d_intr;
+  g2ohip::DBuf<double> d_S;  // per-slot information factors (empty: identity)
   g2ohip::WinPlanBufs win_bufs[2];
   g2ohip::DevWinPlan dwins[2]{};
   bool win_ready[2] = {false, false};
@@ -113,6 +114,7 @@ struct g2ohip_ctx {
   int stack_depth = 0;
 
   g2ohip::DBuf<double> d_linv;  // L^-1 of the point blocks (Schur windows)
+  g2ohip::DBuf<double> d_Sdiag;  // FP64 diagonal blocks of the FP32 reduced system [Cf][36]
   g2ohip::DBuf<double> d_Hll, d_b, d_Hpp, d_hpp_part, d_diag_part, d_chunk_chi, d_coef_part, d_bschur, d_x, d_red, d_scal,
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;
@@ -156,6 +158,7 @@ struct g2ohip_ctx {
 
   ~g2ohip_ctx();
   int fail(int code, const std::string& msg);
+  int fail_nothrow(int code, const char* msg) noexcept;
   int require_device();
   int hip(hipError_t e, const char* what);
 
@@ -163,6 +166,7 @@ struct g2ohip_ctx {
   int set_structure(int C, int P, int E, const int* ec, const int* ep, const uint8_t* cf, const uint8_t* pf);
   int set_measurements(const double* meas, const double* w, const double* intr, double huber);
   int set_state(const double* cams, const double* pts);
+  int set_information(const double* omega);
   int get_state(double* cams, double* pts);
   int ensure_win(int which);
   int compute_active_errors(double* chi2, double* err_out = nullptr);

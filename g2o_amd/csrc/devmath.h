@@ -127,6 +127,31 @@ __device__ inline void project_error(const double* Rt, const double* in, const d
   e[1] = z[1] - (Xc[1] / Xc[2] * in[1] + in[3]);
 }
 
+// An edge's 2x2 information Omega (BaseEdge::information, base_edge.h:165-180)
+// enters through its upper-triangular factor S = [[s0, s1], [0, s2]],
+// Omega = S^T S: with e' = S e and J' = S J, e^T Omega e = |e'|^2,
+// J^T Omega J = J'^T J' and J^T Omega e = J'^T e', so every Gauss-Newton
+// kernel works on whitened errors and Jacobians. S == nullptr: Omega = I.
+__host__ __device__ inline void whiten_vec(const double* S, double* e) {
+  e[0] = S[0] * e[0] + S[1] * e[1];
+  e[1] = S[2] * e[1];
+}
+// rows of a 2xN row-major Jacobian
+template <int N>
+__host__ __device__ inline void whiten_rows(const double* S, double* J) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    J[c] = S[0] * J[c] + S[1] * J[N + c];
+    J[N + c] = S[2] * J[N + c];
+  }
+}
+// Omega = S^T S as its upper triangle [o00 o01 o11]
+__host__ __device__ inline void info_of(const double* S, double* om) {
+  om[0] = S[0] * S[0];
+  om[1] = S[0] * S[1];
+  om[2] = S[1] * S[1] + S[2] * S[2];
+}
+
 // Analytic Jacobians of EdgeSE3ProjectXYZ::linearizeOplus
 // (edge_project_xyz.cpp:57-93) in precision T: Jp = d e / d X (2x3 row-major),
 // Jc = d e / d [omega; upsilon] (2x6 row-major), from the camera-frame point.
@@ -164,11 +189,17 @@ __host__ __device__ inline void project_jacobians(const T* R, T fx, T fy, T x, T
 // (omega x Y) + 1/2 omega x upsilon and omega x (R dX):
 //   S_ww = 1/2 (g Y^T + Y g^T) - (g . Y) I, S_wu = -1/2 [g]x, S_wX = -[g]x R.
 // Also returns rho', rho'' and s for the mixed derivatives.
+// With an information factor S (Omega = S^T S, see whiten_vec; null = I) the
+// energy is 0.5 rho(w e^T Omega e): every e above reads Omega e where it is
+// contracted, and Je^T Je reads Je^T Omega Je.
 __host__ __device__ inline void edge_exact_hessian(const double* R, double fx, double fy, const double* Y,
-                                                   const double* e, double w, double huber, double* H,
-                                                   double* rho_out) {
+                                                   const double* e, double w, double huber, const double* S,
+                                                   double* H, double* rho_out) {
+  double om[3] = {1.0, 0.0, 1.0};
+  if (S) info_of(S, om);
+  const double oe[2] = {om[0] * e[0] + om[1] * e[1], om[1] * e[0] + om[2] * e[1]};  // Omega e
   double rho[3];
-  const double s = w * (e[0] * e[0] + e[1] * e[1]);
+  const double s = w * (e[0] * oe[0] + e[1] * oe[1]);
   huber_rho(s, huber, rho);
   rho_out[0] = rho[1];
   rho_out[1] = rho[2];
@@ -178,14 +209,18 @@ __host__ __device__ inline void edge_exact_hessian(const double* R, double fx, d
   double J[2][9];
   for (int c = 0; c < 6; ++c) { J[0][c] = Jc[c]; J[1][c] = Jc[6 + c]; }
   for (int c = 0; c < 3; ++c) { J[0][6 + c] = Jp[c]; J[1][6 + c] = Jp[3 + c]; }
-  double u[9];
-  for (int i = 0; i < 9; ++i) u[i] = J[0][i] * e[0] + J[1][i] * e[1];
+  double u[9], OJ[2][9];
+  for (int i = 0; i < 9; ++i) {
+    u[i] = J[0][i] * oe[0] + J[1][i] * oe[1];
+    OJ[0][i] = om[0] * J[0][i] + om[1] * J[1][i];
+    OJ[1][i] = om[1] * J[0][i] + om[2] * J[1][i];
+  }
   const double gn = rho[1] * w, hb = 2.0 * rho[2] * w * w;
   for (int i = 0; i < 9; ++i)
-    for (int j = 0; j < 9; ++j) H[9 * i + j] = gn * (J[0][i] * J[0][j] + J[1][i] * J[1][j]) + hb * u[i] * u[j];
+    for (int j = 0; j < 9; ++j) H[9 * i + j] = gn * (J[0][i] * OJ[0][j] + J[1][i] * OJ[1][j]) + hb * u[i] * u[j];
   // second-order terms
   const double x = Y[0], y = Y[1], z = Y[2], iz = 1.0 / z, iz2 = iz * iz, iz3 = iz2 * iz;
-  const double a0 = gn * e[0], a1 = gn * e[1];
+  const double a0 = gn * oe[0], a1 = gn * oe[1];
   const double g[3] = {a0 * fx * iz, a1 * fy * iz, -(a0 * fx * x + a1 * fy * y) * iz2};
   double M[9] = {0, 0, -a0 * fx * iz2, 0, 0, -a1 * fy * iz2, -a0 * fx * iz2, -a1 * fy * iz2,
                  2.0 * (a0 * fx * x + a1 * fy * y) * iz3};

@@ -77,6 +77,18 @@ int fail(const g2ohip_graph* g, int code, const std::string& msg) {
   return code;
 }
 
+// status of an exception caught at the ABI: no exception leaves an entry point
+int fail_caught(const g2ohip_graph* g, bool oom) noexcept {
+  const int code = oom ? G2OHIP_ERR_OOM : G2OHIP_ERR_INVALID;
+  if (!g) return code;
+  try {
+    g->err = oom ? "out of host memory" : "unexpected host exception";
+  } catch (...) {
+    g->err.clear();
+  }
+  return code;
+}
+
 void note(g2ohip_graph* g, const std::string& msg) {
   if (g->log_count++ < kMaxLog) {
     g->log += msg;
@@ -503,7 +515,7 @@ const char* g2ohip_graph_last_error(const g2ohip_graph* g) { return g ? g->err.c
 
 const char* g2ohip_graph_log(const g2ohip_graph* g) { return g ? g->log.c_str() : ""; }
 
-int g2ohip_graph_load(g2ohip_graph* g, const char* filename, int nthreads) {
+int g2ohip_graph_load(g2ohip_graph* g, const char* filename, int nthreads) try {
   if (!g || !filename) return G2OHIP_ERR_INVALID;
   std::vector<char> buf;
   if (!read_file(filename, buf, g->err)) return G2OHIP_ERR_IO;
@@ -532,9 +544,13 @@ int g2ohip_graph_load(g2ohip_graph* g, const char* filename, int nthreads) {
   }
   g->lines = line0;
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
-int g2ohip_graph_info(const g2ohip_graph* g, long long* info) {
+int g2ohip_graph_info(const g2ohip_graph* g, long long* info) try {
   if (!g || !info) return G2OHIP_ERR_INVALID;
   long long nfix = 0;
   for (auto f : g->cam_fix) nfix += f;
@@ -548,10 +564,14 @@ int g2ohip_graph_info(const g2ohip_graph* g, long long* info) {
                            (long long)g->par_id.size()};
   std::memcpy(info, v, sizeof v);
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
 int g2ohip_graph_get_ba(const g2ohip_graph* g, int* cam_id, double* cams, uint8_t* cam_fixed, int* pt_id,
-                        double* pts, uint8_t* pt_fixed, int* edge_cam, int* edge_pt, double* meas, double* omega) {
+                        double* pts, uint8_t* pt_fixed, int* edge_cam, int* edge_pt, double* meas, double* omega) try {
   if (!g) return G2OHIP_ERR_INVALID;
   const std::vector<int> oc = order_by_id(g->cam_id), op = order_by_id(g->pt_id);
   for (size_t i = 0; i < oc.size(); ++i) {
@@ -573,9 +593,13 @@ int g2ohip_graph_get_ba(const g2ohip_graph* g, int* cam_id, double* cams, uint8_
   if (meas && E) std::memcpy(meas, g->pe_meas.data(), 2 * E * sizeof(double));
   if (omega && E) std::memcpy(omega, g->pe_omega.data(), 3 * E * sizeof(double));
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
-int g2ohip_graph_get_ba_params(const g2ohip_graph* g, int* param_id, double* params, int* edge_param) {
+int g2ohip_graph_get_ba_params(const g2ohip_graph* g, int* param_id, double* params, int* edge_param) try {
   if (!g) return G2OHIP_ERR_INVALID;
   const std::vector<int> o = order_by_id(g->par_id);
   const std::vector<int> rk = rank_of(o);
@@ -586,10 +610,14 @@ int g2ohip_graph_get_ba_params(const g2ohip_graph* g, int* param_id, double* par
   if (edge_param)
     for (size_t e = 0; e < g->pe_par.size(); ++e) edge_param[e] = g->pe_par[e] < 0 ? -1 : rk[g->pe_par[e]];
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
 int g2ohip_graph_get_pg(const g2ohip_graph* g, int* pose_id, double* poses, uint8_t* fixed, int* edge_from,
-                        int* edge_to, double* meas, double* info) {
+                        int* edge_to, double* meas, double* info) try {
   if (!g) return G2OHIP_ERR_INVALID;
   const std::vector<int> o = order_by_id(g->pose_id);
   for (size_t i = 0; i < o.size(); ++i) {
@@ -606,9 +634,13 @@ int g2ohip_graph_get_pg(const g2ohip_graph* g, int* pose_id, double* poses, uint
   if (meas && E) std::memcpy(meas, g->se_meas.data(), 7 * E * sizeof(double));
   if (info && E) std::memcpy(info, g->se_info.data(), 36 * E * sizeof(double));
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
-int g2ohip_graph_set_ba_state(g2ohip_graph* g, const double* cams, const double* pts) {
+int g2ohip_graph_set_ba_state(g2ohip_graph* g, const double* cams, const double* pts) try {
   if (!g) return G2OHIP_ERR_INVALID;
   if (cams) {
     const std::vector<int> o = order_by_id(g->cam_id);
@@ -623,21 +655,29 @@ int g2ohip_graph_set_ba_state(g2ohip_graph* g, const double* cams, const double*
     for (size_t i = 0; i < o.size(); ++i) std::memcpy(&g->pt[3 * (size_t)o[i]], pts + 3 * i, 3 * sizeof(double));
   }
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
-int g2ohip_graph_set_pg_state(g2ohip_graph* g, const double* poses) {
+int g2ohip_graph_set_pg_state(g2ohip_graph* g, const double* poses) try {
   if (!g) return G2OHIP_ERR_INVALID;
   if (poses) {
     const std::vector<int> o = order_by_id(g->pose_id);
     for (size_t i = 0; i < o.size(); ++i) std::memcpy(&g->pose[7 * (size_t)o[i]], poses + 7 * i, 7 * sizeof(double));
   }
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
 int g2ohip_graph_add_ba(g2ohip_graph* g, int num_cams, int num_pts, int num_edges, const int* cam_id,
                         const double* cams, const uint8_t* cam_fixed, const int* pt_id, const double* pts,
                         const uint8_t* pt_fixed, const int* edge_cam, const int* edge_pt, const double* meas,
-                        const double* omega) {
+                        const double* omega) try {
   if (!g) return G2OHIP_ERR_INVALID;
   if (num_cams < 0 || num_pts < 0 || num_edges < 0 || (num_cams && !cams) || (num_pts && !pts) ||
       (num_edges && (!edge_cam || !edge_pt || !meas)))
@@ -683,11 +723,15 @@ int g2ohip_graph_add_ba(g2ohip_graph* g, int num_cams, int num_pts, int num_edge
     g->pe_omega.insert(g->pe_omega.end(), o, o + 3);
   }
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
 int g2ohip_graph_add_pg(g2ohip_graph* g, int num_poses, int num_edges, const int* pose_id, const double* poses,
                         const uint8_t* fixed, const int* edge_from, const int* edge_to, const double* meas,
-                        const double* info) {
+                        const double* info) try {
   if (!g) return G2OHIP_ERR_INVALID;
   if (num_poses < 0 || num_edges < 0 || (num_poses && !poses) || (num_edges && (!edge_from || !edge_to || !meas)))
     return fail(g, G2OHIP_ERR_INVALID, "graph_add_pg: negative size or missing array");
@@ -721,9 +765,13 @@ int g2ohip_graph_add_pg(g2ohip_graph* g, int num_poses, int num_edges, const int
     g->se_info.insert(g->se_info.end(), o, o + 36);
   }
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
-int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision) {
+int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision) try {
   if (!g || !filename) return G2OHIP_ERR_INVALID;
   if (precision < 1 || precision > 17) return fail(g, G2OHIP_ERR_INVALID, "graph_save: precision must be 1..17");
   // vertices of the edges, sorted by id (optimizable_graph.cpp:541-554)
@@ -833,9 +881,13 @@ int g2ohip_graph_save(const g2ohip_graph* g, const char* filename, int precision
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) return fail(g, G2OHIP_ERR_IO, std::string("Unable to write file ") + filename);
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
-int g2ohip_graph_load_bal(g2ohip_graph* g, const char* filename, int nthreads) {
+int g2ohip_graph_load_bal(g2ohip_graph* g, const char* filename, int nthreads) try {
   if (!g || !filename) return G2OHIP_ERR_INVALID;
   std::vector<char> buf;
   if (!read_file(filename, buf, g->err)) return G2OHIP_ERR_IO;
@@ -905,10 +957,14 @@ int g2ohip_graph_load_bal(g2ohip_graph* g, const char* filename, int nthreads) {
   g->bal_pt_idx.swap(ep);
   g->bal_meas.swap(meas);
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
 int g2ohip_graph_get_bal(const g2ohip_graph* g, long long* info, double* cams9, double* pts, int* edge_cam,
-                         int* edge_pt, double* meas) {
+                         int* edge_pt, double* meas) try {
   if (!g) return G2OHIP_ERR_INVALID;
   const size_t C = g->bal_cam.size() / 9, P = g->bal_pt.size() / 3, E = g->bal_cam_idx.size();
   if (info) {
@@ -922,6 +978,10 @@ int g2ohip_graph_get_bal(const g2ohip_graph* g, long long* info, double* cams9, 
   if (edge_pt && E) std::memcpy(edge_pt, g->bal_pt_idx.data(), E * sizeof(int));
   if (meas && E) std::memcpy(meas, g->bal_meas.data(), 2 * E * sizeof(double));
   return G2OHIP_OK;
+} catch (const std::bad_alloc&) {
+  return fail_caught(g, true);
+} catch (...) {
+  return fail_caught(g, false);
 }
 
 }  // extern "C"

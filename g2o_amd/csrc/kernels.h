@@ -28,6 +28,9 @@ struct DevGraph {
   // (the indices above, without their dependent lookups; used by k_chi2)
   const int4* slot_rec;
   const double *slot_meas, *slot_w;
+  // per slot the upper-triangular factor [s0 s1 s2] of the edge's information
+  // Omega = S^T S (devmath.h whiten_vec); null: Omega = I for every edge
+  const double* slot_S;
   const int *lc_pos0, *lc_cam_ptr, *lc_cams, *camp_ptr, *camp_idx;
   const int *S_row, *S_col, *S_diag, *up_pos, *up_posT, *F_ptr, *F_col, *F_diag;
   const double* intr;
@@ -165,10 +168,12 @@ hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, i
 // The same for the LM path, point-major (V always re-linearised in FP64 from
 // the state): chunk partials into part (27 per linearisation chunk camera
 // row), then per camera the block, its inverse and bschur.
+// Sdiag (optional): the blocks S_ii in FP64 [Cf][36]; F_val's diagonal
+// blocks are then zeroed and the SpMV takes them from Sdiag.
 template <typename HT>
 hipError_t launch_schur_diag_pm(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
                                 const double* Hpp, const double* b, double lambda, double* part, HT* F_val, HT* Jinv,
-                                double* bschur, int* err, hipStream_t st);
+                                double* bschur, double* Sdiag, int* err, hipStream_t st);
 template <typename HT>
 hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
                              const HT* Hpl, const double* Hll, const double* Hpp, const double* b, double lambda,
@@ -184,10 +189,13 @@ int implicit_reduce_blocks(int Cf);
 template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
                            double* part_rs, PcgScalars* sc, hipStream_t st);
+// Ddiag (optional): FP64 diagonal blocks [nb][BD*BD] applied in place of
+// F_val's (zeroed) diagonal blocks.
 template <typename MT, int BD>
 hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
-                               PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);
+                               PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st,
+                               const double* Ddiag = nullptr);
 template <int BD>
 hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
                           PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);

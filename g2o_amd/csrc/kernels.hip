@@ -124,10 +124,11 @@ __global__ __launch_bounds__(256) void k_chi2(DevGraph g, const double* __restri
     double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
     double Xc[3], e[2];
     project_error(camRt + 12 * cam, g.intr + 4 * cam, X, z, Xc, e);
+    if (err_out) { err_out[2 * s] = e[0]; err_out[2 * s + 1] = e[1]; }
+    if (g.slot_S) whiten_vec(g.slot_S + 3 * static_cast<size_t>(s), e);
     double rho[3];
     huber_rho(g.slot_w[s] * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
     chi += rho[0];
-    if (err_out) { err_out[2 * s] = e[0]; err_out[2 * s + 1] = e[1]; }
   }
   chi = block_sum(chi, s_red);
   if (threadIdx.x == 0) partial[blockIdx.x] = chi;
@@ -194,7 +195,15 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 struct SlotOps {
   int pos, cam, ph, chh, praw, q;
   double Rt[12], in[4], X[3], z[2], w;
+  double S[3];  // information factor (identity without g.slot_S)
 };
+// e and the Jacobians of one observation, whitened by its information factor
+__device__ __forceinline__ void whiten_obs(const DevGraph& g, const double* S, double* e, double* Jp, double* Jc) {
+  if (!g.slot_S) return;
+  whiten_vec(S, e);
+  whiten_rows<3>(S, Jp);
+  whiten_rows<6>(S, Jc);
+}
 __device__ __forceinline__ void load_slot_ops(const DevGraph& g, const double* __restrict__ camRt,
                                               const double* __restrict__ pts, int s, SlotOps& o) {
   o.pos = g.slot_pos[s];
@@ -203,6 +212,12 @@ __device__ __forceinline__ void load_slot_ops(const DevGraph& g, const double* _
   o.z[0] = g.slot_meas[2 * static_cast<size_t>(s)];
   o.z[1] = g.slot_meas[2 * static_cast<size_t>(s) + 1];
   o.w = g.slot_w[s];
+  if (g.slot_S) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.S[k] = g.slot_S[3 * static_cast<size_t>(s) + k];
+  } else {
+    o.S[0] = 1.0; o.S[1] = 0.0; o.S[2] = 1.0;
+  }
   o.praw = g.pos_pt[o.pos];
   o.ph = g.pos_h[o.pos];
   o.chh = g.camH[o.cam];
@@ -265,16 +280,17 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
       const double w = o.w;
       double Xc[3], e[2];
       project_error(Rt, in, o.X, o.z, Xc, e);
-      double rho[3];
-      huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
-      chi += rho[0];
-      const double om = rho[1] * w;
-      const double wr0 = -om * e[0], wr1 = -om * e[1];
       // Jacobians and the Hll / Hpp / b sums in FP64 for either HT; only the
       // stored Hpl block is rounded to HT (the FP32 path keeps Hpp and Hll
       // accurate, which its FP64 Schur diagonal relies on)
       double Jp[6], Jc[12];
       project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+      whiten_obs(g, o.S, e, Jp, Jc);
+      double rho[3];
+      huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
+      chi += rho[0];
+      const double om = rho[1] * w;
+      const double wr0 = -om * e[0], wr1 = -om * e[1];
       if (ph >= 0) {
         double* pa = s_pacc + 9 * lp;
         int k = 0;
@@ -903,11 +919,12 @@ __global__ __launch_bounds__(LIN_THREADS) void k_refine(DevGraph g, const double
       if (o.ph < 0) continue;
       double Xc[3], e[2];
       project_error(o.Rt, o.in, o.X, o.z, Xc, e);
+      double Jp[6], Jc[12];
+      project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+      whiten_obs(g, o.S, e, Jp, Jc);
       double rho[3];
       huber_rho(o.w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
       const double om = rho[1] * o.w;
-      double Jp[6], Jc[12];
-      project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
       double t0 = e[0], t1 = e[1];
       if (o.chh >= 0 && xp)
         for (int k = 0; k < 6; ++k) { t0 += Jc[k] * xv[u][k]; t1 += Jc[6 + k] * xv[u][k]; }
@@ -953,11 +970,12 @@ __global__ __launch_bounds__(LIN_THREADS) void k_refine(DevGraph g, const double
       if (o.chh < 0) continue;
       double Xc[3], e[2];
       project_error(o.Rt, o.in, o.X, o.z, Xc, e);
+      double Jp[6], Jc[12];
+      project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+      whiten_obs(g, o.S, e, Jp, Jc);
       double rho[3];
       huber_rho(o.w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
       const double om = rho[1] * o.w;
-      double Jp[6], Jc[12];
-      project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
       double t0 = e[0], t1 = e[1];
       if (xp)
         for (int k = 0; k < 6; ++k) { t0 += Jc[k] * xv[u][k]; t1 += Jc[6 + k] * xv[u][k]; }
@@ -1104,9 +1122,10 @@ __device__ __forceinline__ void obs_V64(const DevGraph& g, const double* __restr
   const double w = g.slot_w[slot];
   double Xc[3], e[2], rho[3], Jp[6], Jc[12];
   project_error(Rt, in, X, z, Xc, e);
+  project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+  if (g.slot_S) whiten_obs(g, g.slot_S + 3 * static_cast<size_t>(slot), e, Jp, Jc);
   huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
   const double om = rho[1] * w;
-  project_jacobians<double>(Rt, in[0], in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
   for (int r = 0; r < 6; ++r)
     for (int cc = 0; cc < 3; ++cc) V[3 * r + cc] += om * (Jc[r] * Jp[cc] + Jc[6 + r] * Jp[3 + cc]);
 }
@@ -1115,9 +1134,10 @@ __device__ __forceinline__ void obs_V64(const DevGraph& g, const double* __restr
 __device__ __forceinline__ void obs_V64_ops(const DevGraph& g, const SlotOps& o, double* V) {
   double Xc[3], e[2], rho[3], Jp[6], Jc[12];
   project_error(o.Rt, o.in, o.X, o.z, Xc, e);
+  project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
+  whiten_obs(g, o.S, e, Jp, Jc);
   huber_rho(o.w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
   const double om = rho[1] * o.w;
-  project_jacobians<double>(o.Rt, o.in[0], o.in[1], Xc[0], Xc[1], Xc[2], Jp, Jc);
   for (int r = 0; r < 6; ++r)
     for (int cc = 0; cc < 3; ++cc) V[3 * r + cc] += om * (Jc[r] * Jp[cc] + Jc[6 + r] * Jp[3 + cc]);
 }
@@ -1299,12 +1319,15 @@ __global__ __launch_bounds__(LIN_THREADS) void k_schur_diag_pm(DevGraph g, const
 // (lanes 0..26), then S_ii = Hpp_ii + lambda I - sum into F_val (if not
 // null), its inverse into Jinv (err |= 2 if not PD) and bschur_i = b_i - sum
 // (if bschur).
+// With Sdiag the block goes there in FP64 and F_val's diagonal block is
+// zeroed (the SpMV applies it from Sdiag).
 template <typename HT>
 __global__ __launch_bounds__(64) void k_schur_diag_finish(DevGraph g, const double* __restrict__ part,
                                                           const double* __restrict__ Hpp,
                                                           const double* __restrict__ b, double lambda,
                                                           HT* __restrict__ F_val, HT* __restrict__ Jinv,
-                                                          double* __restrict__ bschur, int* __restrict__ err) {
+                                                          double* __restrict__ bschur, double* __restrict__ Sdiag,
+                                                          int* __restrict__ err) {
   __shared__ double s_v[27];
   const int c = blockIdx.x, lane = threadIdx.x;
   if (lane < 27) s_v[lane] = camp_sum(g, part, 27, c, lane);
@@ -1322,8 +1345,10 @@ __global__ __launch_bounds__(64) void k_schur_diag_finish(DevGraph g, const doub
   for (int k = 0; k < 6; ++k) m[7 * k] += lambda;
   if (F_val) {
     HT* dst = F_val + 36 * static_cast<size_t>(g.F_diag[c]);
-    for (int k = 0; k < 36; ++k) dst[k] = static_cast<HT>(m[k]);
+    for (int k = 0; k < 36; ++k) dst[k] = Sdiag ? HT(0) : static_cast<HT>(m[k]);
   }
+  if (Sdiag)
+    for (int k = 0; k < 36; ++k) Sdiag[36 * static_cast<size_t>(c) + k] = m[k];
   if (!inv_spd<6>(m)) {
     atomicOr(err, 2);
     for (int k = 0; k < 36; ++k) m[k] = 0;
@@ -1334,21 +1359,21 @@ __global__ __launch_bounds__(64) void k_schur_diag_finish(DevGraph g, const doub
 template <typename HT>
 hipError_t launch_schur_diag_pm(const DevGraph& g, const double* camRt, const double* pts, const double* Hll,
                                 const double* Hpp, const double* b, double lambda, double* part, HT* F_val, HT* Jinv,
-                                double* bschur, int* err, hipStream_t st) {
+                                double* bschur, double* Sdiag, int* err, hipStream_t st) {
   if (g.Cf == 0) return hipSuccess;
   if (g.nlc > 0)
     hipLaunchKernelGGL(k_schur_diag_pm, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hll, b, lambda, part,
                        err);
   hipLaunchKernelGGL(k_schur_diag_finish<HT>, dim3(g.Cf), dim3(64), 0, st, g, part, Hpp, b, lambda, F_val, Jinv,
-                     bschur, err);
+                     bschur, Sdiag, err);
   return hipGetLastError();
 }
 template hipError_t launch_schur_diag_pm<float>(const DevGraph&, const double*, const double*, const double*,
                                                 const double*, const double*, double, double*, float*, float*,
-                                                double*, int*, hipStream_t);
+                                                double*, double*, int*, hipStream_t);
 template hipError_t launch_schur_diag_pm<double>(const DevGraph&, const double*, const double*, const double*,
                                                  const double*, const double*, double, double*, double*, double*,
-                                                 double*, int*, hipStream_t);
+                                                 double*, double*, int*, hipStream_t);
 
 template <typename HT>
 hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double* camRt, const double* pts,
@@ -1553,12 +1578,17 @@ constexpr int PCG_SPMV_WAVES = 2;
 constexpr int PCG_SPMV_HOIST = 3;
 // FIRST: iteration 0 (d = s; no d_old, no beta), a separate instantiation so
 // the other iterations carry no run-time branch on it.
-template <typename MT, int BD, bool FIRST>
+// D64: the diagonal blocks are applied from Ddiag (FP64, BD*BD per row) and
+// F_val's diagonal blocks hold zeros. The FP32 operator keeps its diagonal in
+// FP64 this way: Hpp_ii - sum V Dinv V^T is the largest entry of a row, and
+// rounding it to FP32 is the largest perturbation of the reduced system.
+template <typename MT, int BD, bool FIRST, bool D64>
 __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
     int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
-    const double* __restrict__ s, const double* __restrict__ d_old, double* __restrict__ d, double* __restrict__ q,
-    const double* __restrict__ part_rs, int n_rs, double* __restrict__ part_dq, PcgScalars* sc, PcgProgress* prog,
-    const int* abort, double tol, double resid_mem, double d0_abs, int it, int last) {
+    const double* __restrict__ Ddiag, const double* __restrict__ s, const double* __restrict__ d_old,
+    double* __restrict__ d, double* __restrict__ q, const double* __restrict__ part_rs, int n_rs,
+    double* __restrict__ part_dq, PcgScalars* sc, PcgProgress* prog, const int* abort, double tol, double resid_mem,
+    double d0_abs, int it, int last) {
   constexpr int GPW = 64 / BD;  // groups per wavefront
   constexpr int NG = PCG_SPMV_WAVES * GPW;
   constexpr int H = PCG_SPMV_HOIST;
@@ -1567,6 +1597,17 @@ __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
   const int row = blockIdx.x;
   const int gw = lane / BD, k = lane % BD;
   const int grp = wv * GPW + gw;
+  // own-row operands of the final stage (lane kk of wavefront 0 owns
+  // component kk), issued with the matrix loads
+  const int kk = lane < BD ? lane : 0;
+  const size_t o_own = BD * static_cast<size_t>(row) + kk;
+  const double s_own = s[o_own];
+  const double dold_own = FIRST ? 0.0 : d_old[o_own];
+  double Dg[BD];
+  if (D64) {
+#pragma unroll
+    for (int c = 0; c < BD; ++c) Dg[c] = Ddiag[static_cast<size_t>(row) * BD * BD + BD * kk + c];
+  }
   const int p0 = F_ptr[row], p1 = F_ptr[row + 1];
   MT v[H][BD];
   double sv[H][BD], dv[H][BD];
@@ -1610,14 +1651,21 @@ __global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
   __syncthreads();
   if (threadIdx.x < 64) {
     double dot = 0.0;
-    if (lane < BD) {
-      double vs = 0;
+    const double dd = FIRST ? s_own : fma(beta, dold_own, s_own);
+    double vs = 0;
+    if (lane < BD)
       for (int w2 = 0; w2 < PCG_SPMV_WAVES; ++w2)
         for (int g2 = 0; g2 < GPW; ++g2) vs += s_part[w2][g2 * BD + lane];
-      const size_t o = BD * static_cast<size_t>(row) + lane;
-      const double dd = FIRST ? s[o] : fma(beta, d_old[o], s[o]);
-      q[o] = vs;
-      d[o] = dd;
+    if (D64) {
+      // every lane of the wavefront takes part in the shuffles
+      double dg = 0.0;
+#pragma unroll
+      for (int c = 0; c < BD; ++c) dg = fma(Dg[c], __shfl(dd, c, 64), dg);
+      vs += dg;
+    }
+    if (lane < BD) {
+      q[o_own] = vs;
+      d[o_own] = dd;
       dot = vs * dd;
     }
     dot = wave_sum(dot);
@@ -1702,15 +1750,20 @@ hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, d
 template <typename MT, int BD>
 hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
-                               PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
-  if (it == 0)
-    hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, true>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, st,
-                       nb, F_ptr, F_col, F_val, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog,
-                       ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
-  else
-    hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, false>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, st,
-                       nb, F_ptr, F_col, F_val, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc, ctl.prog,
-                       ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
+                               PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st,
+                               const double* Ddiag) {
+#define G2OHIP_SPMV_LAUNCH(FIRST, D64)                                                                            \
+  hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, FIRST, D64>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, \
+                     st, nb, F_ptr, F_col, F_val, Ddiag, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc,   \
+                     ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last)
+  if (Ddiag) {
+    if (it == 0) G2OHIP_SPMV_LAUNCH(true, true);
+    else G2OHIP_SPMV_LAUNCH(false, true);
+  } else {
+    if (it == 0) G2OHIP_SPMV_LAUNCH(true, false);
+    else G2OHIP_SPMV_LAUNCH(false, false);
+  }
+#undef G2OHIP_SPMV_LAUNCH
   return hipGetLastError();
 }
 
@@ -1739,7 +1792,8 @@ hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const
                                               PcgScalars*, hipStream_t);                                         \
   template hipError_t launch_pcg_dir_spmv<MT, BD>(int, const int*, const int*, const MT*, const double*,         \
                                                   const double*, double*, double*, const double*, double*,       \
-                                                  PcgScalars*, const PcgControl&, int, int, hipStream_t);        \
+                                                  PcgScalars*, const PcgControl&, int, int, hipStream_t,         \
+                                                  const double*);                                                 \
   template hipError_t launch_pcg_update<MT, BD>(int, const MT*, double*, double*, const double*, const double*,  \
                                                 double*, const double*, int, double*, PcgScalars*, int,          \
                                                 hipStream_t);

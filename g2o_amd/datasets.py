@@ -40,6 +40,9 @@ class BAProblem:
     gt_cams: np.ndarray | None = None
     gt_pts: np.ndarray | None = None
     info: dict = field(default_factory=dict)
+    # per-edge 2x2 information as its upper triangle [O00, O01, O11] (E x 3);
+    # None: identity (the information is weight * Omega)
+    omega: np.ndarray | None = None
 
     @property
     def num_cams(self) -> int:
@@ -459,3 +462,14 @@ def make_bal(seed: int = 11, num_cams: int = 8, num_pts: int = 300, k=(0.0, 0.0)
     obs = np.array(obs)
     return BALProblem(np.array(cams), pts, obs[:, 0].astype(np.int32), obs[:, 1].astype(np.int32),
                       obs[:, 2:4].copy())
+
+
+def random_information(num_edges: int, seed: int, lo: float = 0.3, hi: float = 3.0) -> np.ndarray:
+    """Per-edge positive-definite 2x2 information matrices as their upper
+    triangles [O00, O01, O11] (num_edges x 3): eigenvalues U(lo, hi) and a
+    uniformly random orientation (anisotropic pixel noise)."""
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0, np.pi, num_edges)
+    l1, l2 = rng.uniform(lo, hi, num_edges), rng.uniform(lo, hi, num_edges)
+    c, s = np.cos(th), np.sin(th)
+    return np.stack([l1 * c * c + l2 * s * s, (l1 - l2) * c * s, l1 * s * s + l2 * c * c], 1)

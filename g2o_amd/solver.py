@@ -93,7 +93,7 @@ class BundleAdjuster:
     for one bundle-adjustment problem (``g2o_amd.datasets.BAProblem``)."""
 
     def __init__(self, prob, device: int = 0, precision: int = L.FP64, schur_mode: int = L.SCHUR_EXPLICIT,
-                 pcg_tolerance: float = 1e-6, pcg_max_iter: int = -1, refine_steps: int = 1, max_trials: int = 10):
+                 pcg_tolerance: float = 1e-6, pcg_max_iter: int = -1, refine_steps: int = 4, max_trials: int = 10):
         self.ctx = Context(device)
         self.prob = prob
         self.set_options(precision=precision, schur_mode=schur_mode, pcg_tolerance=pcg_tolerance,
@@ -109,6 +109,8 @@ class BundleAdjuster:
         self.Cf, self.Pf = int(info[0]), int(info[1])
         if device >= 0:
             self.set_measurements(prob.meas, prob.weight, prob.intr, prob.huber)
+            if getattr(prob, "omega", None) is not None:
+                self.set_information(prob.omega)
             self.set_state(prob.cams, prob.pts)
 
     # ---- setup
@@ -133,6 +135,17 @@ class BundleAdjuster:
         w = np.ascontiguousarray(weight, dtype=np.float64)
         k = np.ascontiguousarray(intr, dtype=np.float64)
         self.ctx.call("g2ohip_ba_set_measurements", _dp(m), _dp(w), _dp(k), float(huber))
+
+    def set_information(self, omega):
+        """BaseEdge::setInformation for every edge: omega[E, 3] = the upper
+        triangle [O00, O01, O11] of each 2x2 information matrix (None: identity)."""
+        if omega is None:
+            self.ctx.call("g2ohip_ba_set_information", None)
+            return
+        o = np.ascontiguousarray(omega, dtype=np.float64).reshape(-1, 3)
+        if o.shape[0] != self.E:
+            raise ValueError(f"omega needs {self.E} rows of 3")
+        self.ctx.call("g2ohip_ba_set_information", _dp(o))
 
     def set_state(self, cams, pts):
         c = np.ascontiguousarray(cams, dtype=np.float64)

@@ -53,7 +53,12 @@ typedef struct g2ohip_options {
   int schur_mode;         /* g2ohip_schur_mode */
   double pcg_tolerance;   /* LinearSolverPCG::setTolerance */
   int pcg_max_iter;       /* LinearSolverPCG::setMaxIterations, <0: n */
-  int refine_steps;       /* FP32 only: FP64 iterative-refinement corrections.
+  int refine_steps;       /* FP32 only: at most this many FP64 iterative-
+                           * refinement corrections (default 4); they stop at
+                           * the first one that finds the FP64 residual of the
+                           * reduced system already within the solve's
+                           * threshold. The FP32 operator keeps its diagonal
+                           * blocks in FP64.
                            * If the FP32 solution's FP64 residual misses the
                            * first solve's threshold by more than 10x, or a
                            * correction stops above it, the FP32 reduced
@@ -123,6 +128,15 @@ int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info);
  * RobustKernelHuber(delta) to every edge (robust_kernel_impl.cpp:60-74). */
 int g2ohip_ba_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* weight, const double* intr,
                                double huber_delta);
+
+/* BaseEdge::setInformation (g2o/core/base_edge.h:165-180) for every edge:
+ * omega[E*3] = [O00 O01 O11], the upper triangle of the edge's 2x2 information
+ * matrix (the three values EdgeSE3ProjectXYZ::read takes,
+ * edge_project_xyz.cpp:34-42). Each must be positive definite. The edge's
+ * information is weight * Omega (the weight of set_measurements scales it).
+ * NULL (or all identity) resets every edge to Omega = I, which is also the
+ * state after set_structure. */
+int g2ohip_ba_set_information(g2ohip_ctx* ctx, const double* omega);
 
 /* Vertex estimates: cams[C*7] = SE3Quat [tx ty tz qx qy qz qw] (world to
  * camera, se3quat.h), pts[P*3]. set_state normalizes quaternions as the

@@ -477,6 +477,7 @@ struct BA {
   vector<double> pts;    // 3 per point
   vector<int> ecam, ept;
   vector<double> meas, w;
+  vector<double> om;     // 3 per edge: upper triangle [O00 O01 O11] of Omega (empty: I)
   vector<uint8_t> cfix, pfix;
   double huber = 0;
 
@@ -572,6 +573,20 @@ void ba_structure(BA& p) {
   p.err.assign(2 * static_cast<size_t>(p.E), 0.0);
 }
 
+// The edge's 2x2 information Omega (base_edge.h:165-180) as a full matrix
+// [o00 o01; o01 o11]; identity unless set.
+inline void edge_info(const BA& p, int k, double o[4]) {
+  if (p.om.empty()) { o[0] = 1; o[1] = 0; o[2] = 0; o[3] = 1; return; }
+  const double* v = &p.om[3 * static_cast<size_t>(k)];
+  o[0] = v[0]; o[1] = v[1]; o[2] = v[1]; o[3] = v[2];
+}
+// e^T Omega e (base_edge.h:99, chi2 = error.dot(information() * error))
+inline double edge_quad(const BA& p, int k, const double e[2]) {
+  double o[4];
+  edge_info(p, k, o);
+  return e[0] * (o[0] * e[0] + o[1] * e[1]) + e[1] * (o[2] * e[0] + o[3] * e[1]);
+}
+
 // computeActiveErrors + activeRobustChi2 (sparse_optimizer.cpp:78-130)
 double ba_chi2(BA& p) {
   double chi = 0;
@@ -586,7 +601,7 @@ double ba_chi2(BA& p) {
     project_edge(T, R, &p.intr[4 * p.ecam[k]], &p.pts[3 * p.ept[k]], &p.meas[2 * k], e, nullptr, nullptr);
     p.err[2 * k] = e[0];
     p.err[2 * k + 1] = e[1];
-    double c2 = p.w[k] * (e[0] * e[0] + e[1] * e[1]);
+    double c2 = p.w[k] * edge_quad(p, k, e);
     if (p.huber > 0) {
       double rho[3];
       huber(c2, p.huber, rho);
@@ -638,25 +653,33 @@ void ba_build_system(BA& p) {
       double e[2] = {p.err[2 * k], p.err[2 * k + 1]}, etmp[2];
       double Jp[6], Jc[12];
       project_edge(T, R, &p.intr[4 * p.ecam[k]], &p.pts[3 * p.ept[k]], &p.meas[2 * k], etmp, Jp, Jc);
-      double om = p.w[k];  // Omega = w I
+      double O[4];
+      edge_info(p, k, O);  // Omega (w I when unset: Omega = I)
       double wr_scale = 1.0;
       if (p.huber > 0) {
         double rho[3];
-        huber(om * (e[0] * e[0] + e[1] * e[1]), p.huber, rho);
+        huber(p.w[k] * edge_quad(p, k, e), p.huber, rho);
         wr_scale = rho[1];
       }
-      double omp_ = wr_scale * om;                     // robustInformation
-      double wr[2] = {-omp_ * e[0], -omp_ * e[1]};     // omega_r = -rho' Omega e
+      // robustInformation: Omega' = rho' w Omega (base_edge.h:156-162)
+      double Op[4];
+      for (int i = 0; i < 4; ++i) Op[i] = wr_scale * p.w[k] * O[i];
+      // omega_r = -Omega' e
+      double wr[2] = {-(Op[0] * e[0] + Op[1] * e[1]), -(Op[2] * e[0] + Op[3] * e[1])};
+      // A^T Omega' B for 2xN Jacobians A (column i) and B (column j)
+      auto q = [&](const double* A, int na, int i, const double* B, int nb, int j) {
+        return A[i] * (Op[0] * B[j] + Op[1] * B[nb + j]) + A[na + i] * (Op[2] * B[j] + Op[3] * B[nb + j]);
+      };
       if (li >= 0) {
         for (int r = 0; r < 3; ++r) {
           bl[3 * li + r] += Jp[r] * wr[0] + Jp[3 + r] * wr[1];
-          for (int c = 0; c < 3; ++c) hll[9 * li + 3 * r + c] += omp_ * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]);
+          for (int c = 0; c < 3; ++c) hll[9 * li + 3 * r + c] += q(Jp, 3, r, Jp, 3, c);
         }
       }
       if (ci >= 0) {
         for (int r = 0; r < 6; ++r) {
           bp[6 * ci + r] += Jc[r] * wr[0] + Jc[6 + r] * wr[1];
-          for (int c = 0; c < 6; ++c) hpp[36 * ci + 6 * r + c] += omp_ * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c]);
+          for (int c = 0; c < 6; ++c) hpp[36 * ci + 6 * r + c] += q(Jc, 6, r, Jc, 6, c);
         }
       }
       if (ci >= 0 && li >= 0) {
@@ -664,7 +687,7 @@ void ba_build_system(BA& p) {
         // because the point is vertex 0 (block_solver.hpp:210-216).
         double* h = &p.Hpl[static_cast<size_t>(p.edgeBlk[k]) * 18];
         for (int r = 0; r < 6; ++r)
-          for (int c = 0; c < 3; ++c) h[3 * r + c] += omp_ * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]);
+          for (int c = 0; c < 3; ++c) h[3 * r + c] += q(Jc, 6, r, Jp, 3, c);
       }
     }
   }
@@ -872,7 +895,7 @@ inline HD hd_sqrt(const HD& x) {
 // part, so the series A = 1 - th2/6, B = C = 1/2 - th2/24, D = 1/6 - th2/120
 // are exact through second order.
 HD edge_energy_hd(const SE3& T, const double X[3], const double z[2], double w, const double in[4], double delta,
-                  const HD v[12]) {
+                  const HD v[12], const double* info = nullptr) {
   double R0[9];
   q2R(T.q, R0);
   const HD* om = v;
@@ -902,7 +925,9 @@ HD edge_energy_hd(const SE3& T, const double X[3], const double z[2], double w, 
   HD iz = hd_inv(P[2]);
   HD e0 = hd_const(z[0]) + v[9] - (in[0] * (P[0] * iz) + hd_const(in[2]));
   HD e1 = hd_const(z[1]) + v[10] - (in[1] * (P[1] * iz) + hd_const(in[3]));
-  HD s = (hd_const(w) + v[11]) * (e0 * e0 + e1 * e1);
+  // e^T Omega e with Omega = [o00 o01; o01 o11] (identity when info is null)
+  const double o00 = info ? info[0] : 1.0, o01 = info ? info[1] : 0.0, o11 = info ? info[2] : 1.0;
+  HD s = (hd_const(w) + v[11]) * (o00 * (e0 * e0) + (2.0 * o01) * (e0 * e1) + o11 * (e1 * e1));
   HD rho = s;
   if (delta > 0 && s.a > delta * delta) rho = (2 * delta) * hd_sqrt(s) - hd_const(delta * delta);
   return 0.5 * rho;
@@ -910,12 +935,12 @@ HD edge_energy_hd(const SE3& T, const double X[3], const double z[2], double w, 
 
 // Gradient g[12] and Hessian H[12*12] (row-major) of edge_energy_hd at v = 0.
 void edge_energy_derivs(const SE3& T, const double X[3], const double z[2], double w, const double in[4],
-                        double delta, double* E, double* g, double* H) {
+                        double delta, double* E, double* g, double* H, const double* om = nullptr) {
   for (int i = 0; i < HDN; ++i) {
     HD v[12];
     v[i].b = 1.0;
     for (int j = 0; j < HDN; ++j) v[j].c[j] = 1.0;
-    HD f = edge_energy_hd(T, X, z, w, in, delta, v);
+    HD f = edge_energy_hd(T, X, z, w, in, delta, v, om);
     if (i == 0) {
       if (E) *E = f.a;
       for (int j = 0; j < HDN; ++j) g[j] = f.c[j];
@@ -940,7 +965,8 @@ void ba_build_exact(BA& p, vector<double>* mixed) {
     int k = p.active[a];
     double g[12];
     edge_energy_derivs(p.cams[p.ecam[k]], &p.pts[3 * p.ept[k]], &p.meas[2 * k], p.w[k], &p.intr[4 * p.ecam[k]],
-                       p.huber, nullptr, g, &He[144 * static_cast<size_t>(a)]);
+                       p.huber, nullptr, g, &He[144 * static_cast<size_t>(a)],
+                       p.om.empty() ? nullptr : &p.om[3 * static_cast<size_t>(k)]);
   }
   for (int a = 0; a < n; ++a) {
     int k = p.active[a];
@@ -1682,6 +1708,13 @@ int orc_ba_gauss_newton(void* h, int iters, int solver, double tol, int maxiter,
 }
 
 // Replace the measurements / information weights (same structure).
+// BaseEdge::setInformation per edge: om[E*3] = [O00 O01 O11]; null: identity
+void orc_ba_set_info(void* h, const double* om) {
+  BA* p = static_cast<BA*>(h);
+  if (om) p->om.assign(om, om + 3 * static_cast<size_t>(p->E));
+  else p->om.clear();
+}
+
 void orc_ba_set_meas(void* h, const double* meas, const double* w) {
   BA* p = static_cast<BA*>(h);
   std::copy(meas, meas + 2 * static_cast<size_t>(p->E), p->meas.begin());

@@ -68,6 +68,7 @@ def lib():
         L.orc_chol_solve.restype = ct.c_int
         L.orc_chol_solve.argtypes = [ct.c_int, _dp, _dp, _dp]
         L.orc_ba_set_meas.argtypes = [ct.c_void_p, _dp, _dp]
+        L.orc_ba_set_info.argtypes = [ct.c_void_p, _dp]
         L.orc_ba_set_user_lambda.argtypes = [ct.c_void_p, ct.c_double]
         L.orc_ba_adjoint.restype = ct.c_int
         L.orc_ba_adjoint.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp, _dp]
@@ -137,6 +138,16 @@ class OracleBA:
         cf, pf, ns = ct.c_int(), ct.c_int(), ct.c_int()
         L.orc_ba_dims(self.h, ct.byref(cf), ct.byref(pf), ct.byref(ns))
         self.Cf, self.Pf, self.nschur = cf.value, pf.value, ns.value
+        if getattr(prob, "omega", None) is not None:
+            self.set_info(prob.omega)
+
+    def set_info(self, omega):
+        """Per-edge information [O00, O01, O11] (E x 3); None: identity."""
+        if omega is None:
+            lib().orc_ba_set_info(self.h, None)
+            return
+        o, po = _d(np.asarray(omega, dtype=np.float64).reshape(-1, 3))
+        lib().orc_ba_set_info(self.h, po)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -37,6 +37,21 @@ def walk():
     return datasets.make_walk_ba(seed=7, num_cams=60, num_pts=6000)
 
 
+@pytest.fixture(scope="module")
+def small_info(small_huber):
+    """Full 2x2 information per edge (EdgeSE3ProjectXYZ::setInformation)."""
+    p = small_huber.copy()
+    p.omega = datasets.random_information(p.num_edges, 21)
+    return p
+
+
+@pytest.fixture(scope="module")
+def walk_info(walk):
+    p = walk.copy()
+    p.omega = datasets.random_information(p.num_edges, 22)
+    return p
+
+
 def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
@@ -46,7 +61,7 @@ def _mk(prob, **kw):
     return g2o_amd.BundleAdjuster(prob, device=0, **kw)
 
 
-@pytest.mark.parametrize("case", ["small", "small_huber", "walk"])
+@pytest.mark.parametrize("case", ["small", "small_huber", "walk", "small_info", "walk_info"])
 def test_chi2_and_hessian_fp64(case, request, oracle_lib):
     prob = request.getfixturevalue(case)
     _say(f"case {case}: C={prob.num_cams} P={prob.num_pts} E={prob.num_edges}")
@@ -84,7 +99,7 @@ def test_hessian_fp32(case, request, oracle_lib):
     assert _rel(ba.b(), b_o) < 2e-5
 
 
-@pytest.mark.parametrize("case", ["small", "small_huber", "walk"])
+@pytest.mark.parametrize("case", ["small", "small_huber", "walk", "small_info"])
 def test_schur_fp64(case, request, oracle_lib):
     prob = request.getfixturevalue(case)
     import g2o_amd
@@ -125,7 +140,7 @@ def test_pcg_iterations_match_reference_rule(case, request, oracle_lib):
     assert _rel(ba.x(), dx_o) < 1e-9
 
 
-@pytest.mark.parametrize("case", ["small", "small_huber", "walk"])
+@pytest.mark.parametrize("case", ["small", "small_huber", "walk", "walk_info"])
 def test_lm_trace_fp64(case, request, oracle_lib):
     prob = request.getfixturevalue(case)
     import g2o_amd
@@ -247,13 +262,17 @@ def test_linear_solver_pcg_fixture():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_implicit_matches_explicit(prec, walk):
+def test_implicit_matches_explicit(prec, walk, oracle_lib):
     """Implicit (matrix-free) Schur inside the PCG applies the same reduced
     operator as the explicit Hschur: run to a tight tolerance both give the
-    same step."""
+    step of the oracle's dense Cholesky solve of (H + lambda I) x = b."""
     import g2o_amd
     p = g2o_amd.FP64 if prec == "fp64" else g2o_amd.FP32
     tol = 1e-24 if prec == "fp64" else 1e-12
+    ora = oracle_lib.OracleBA(walk)
+    ora.linearize()
+    st, dx_o, _ = ora.solve(2.5, solver=0)
+    assert st == 0
     xs = []
     for mode in (g2o_amd.SCHUR_EXPLICIT, g2o_amd.SCHUR_IMPLICIT):
         ba = _mk(walk, precision=p, schur_mode=mode, pcg_tolerance=tol, pcg_max_iter=600, refine_steps=0)
@@ -262,8 +281,12 @@ def test_implicit_matches_explicit(prec, walk):
         assert ba.solve()
         xs.append(ba.x())
     bound = 1e-9 if prec == "fp64" else 2e-4
-    _say(f"implicit vs explicit {prec}: {_rel(xs[1], xs[0]):.3e}")
+    _say(f"implicit vs explicit {prec}: {_rel(xs[1], xs[0]):.3e}; vs dense: {_rel(xs[0], dx_o):.3e} "
+         f"{_rel(xs[1], dx_o):.3e}")
     assert _rel(xs[1], xs[0]) < bound
+    dense_bound = 1e-8 if prec == "fp64" else 2e-4
+    assert _rel(xs[0], dx_o) < dense_bound
+    assert _rel(xs[1], dx_o) < dense_bound
 
 
 def test_profile_counts(walk):

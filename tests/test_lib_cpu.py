@@ -147,3 +147,36 @@ def test_pose_graph_structure_rejects_bad_edges():
     with pytest.raises(g2o_amd.G2oHipError) as ei:
         g2o_amd.PoseGraphOptimizer(prob, device=-1)
     assert ei.value.status == g2o_amd._lib.ERR_INVALID
+
+
+_OOM_CHILD = r"""
+import ctypes as ct, resource, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import g2o_amd
+L = g2o_amd.lib()
+ctx = g2o_amd.Context(-1)
+E = 20_000_000
+ec = np.zeros(E, dtype=np.int32)
+ep = np.arange(E, dtype=np.int32)
+# cap the address space a little above what the process maps now: the
+# structure builder's host vectors for 20M edges cannot fit
+with open("/proc/self/status") as f:
+    vm = [int(l.split()[1]) for l in f if l.startswith("VmSize:")][0] * 1024
+resource.setrlimit(resource.RLIMIT_AS, (vm + (64 << 20), vm + (64 << 20)))
+st = L.g2ohip_ba_set_structure(ctx.h, 1, E, E, ec.ctypes.data_as(ct.POINTER(ct.c_int)),
+                               ep.ctypes.data_as(ct.POINTER(ct.c_int)), None, None)
+print("status", st, L.g2ohip_last_error(ctx.h).decode(), flush=True)
+"""
+
+
+def test_host_allocation_failure_returns_oom_status():
+    """A std::bad_alloc inside an entry point (here the structure builder's host
+    vectors) comes back as G2OHIP_ERR_OOM, not as a terminated process: no
+    exception crosses the C ABI."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", _OOM_CHILD, ROOT], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    line = [l for l in r.stdout.splitlines() if l.startswith("status")][0]
+    assert line.split()[1] == str(g2o_amd._lib.ERR_OOM), line

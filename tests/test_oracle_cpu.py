@@ -199,12 +199,15 @@ def _tiny_ba(huber):
     return p
 
 
-@pytest.mark.parametrize("huber", [0.0, 2.0])
-def test_quadratic_form_vs_dense_numeric_jacobian(O, huber):
+@pytest.mark.parametrize("huber,info", [(0.0, False), (2.0, False), (0.0, True), (2.0, True)])
+def test_quadratic_form_vs_dense_numeric_jacobian(O, huber, info):
     """H = J^T W J and b = -J^T W e with J the numeric Jacobian of all
     residuals with respect to all free parameters (oplus perturbations) and
-    W the per-edge Huber weights rho'(e^T Omega e) * Omega."""
+    W the per-edge blocks rho'(e^T Omega e) * Omega (base_edge.h:156-162),
+    for Omega = I and for a random full 2x2 information per edge."""
     prob = _tiny_ba(huber)
+    if info:
+        prob.omega = datasets.random_information(prob.num_edges, 5)
     ora = O.OracleBA(prob)
     chi, Hpp, Hll, Hpl, b = ora.linearize()
     cams0, pts0 = ora.state()
@@ -237,13 +240,19 @@ def test_quadratic_form_vs_dense_numeric_jacobian(O, huber):
             pp[p, d] += h
             pm[p, d] -= h
             J[:, 6 * Cf + 3 * p + d] = (residuals(cams0, pp) - residuals(cams0, pm)) / (2 * h)
+    om = prob.omega if info else np.tile([1.0, 0.0, 1.0], (E, 1))
+    Om = np.zeros((E, 2, 2))
+    Om[:, 0, 0], Om[:, 0, 1], Om[:, 1, 0], Om[:, 1, 1] = om[:, 0], om[:, 1], om[:, 1], om[:, 2]
+    ee = e0.reshape(-1, 2)
+    s = np.einsum("ki,kij,kj->k", ee, Om, ee)
     w = np.ones(E)
     if huber:
-        s = (e0.reshape(-1, 2) ** 2).sum(1)
         w = np.where(s <= huber ** 2, 1.0, huber / np.sqrt(s))
-    Wd = np.repeat(w, 2)
-    H = J.T @ (Wd[:, None] * J)
-    bb = -J.T @ (Wd * e0)
+    W = np.zeros((2 * E, 2 * E))
+    for k in range(E):
+        W[2 * k:2 * k + 2, 2 * k:2 * k + 2] = w[k] * Om[k]
+    H = J.T @ W @ J
+    bb = -J.T @ (W @ e0)
     Hd = np.zeros((n, n))
     for i in range(Cf):
         Hd[6 * i:6 * i + 6, 6 * i:6 * i + 6] = Hpp[i]
@@ -259,7 +268,6 @@ def test_quadratic_form_vs_dense_numeric_jacobian(O, huber):
     assert np.abs(Hd - H).max() <= 2e-5 * np.abs(H).max()
     assert np.abs(b - bb).max() <= 2e-5 * np.abs(bb).max()
     # robust chi2 of the same state
-    s = (e0.reshape(-1, 2) ** 2).sum(1)
     rho0 = s if not huber else np.where(s <= huber ** 2, s, 2 * huber * np.sqrt(s) - huber ** 2)
     assert np.isclose(chi, rho0.sum(), rtol=1e-12)
 
@@ -351,8 +359,15 @@ def test_dogleg_oracle(O):
     so the first iterations equal Gauss-Newton's; a rejected try halves the
     radius; accepted iterations never raise chi2."""
     prob = datasets.make_small_ba(seed=1000)
-    n, dl = O.OracleBA(prob).dogleg(8, solver=0)
-    _, gn = O.OracleBA(prob).gauss_newton(2, solver=0)
+    # the iteration where the radius first shrinks depends on the summation
+    # order of the OpenMP sums: fix the thread count for this trace
+    saved = O.lib().orc_num_threads()
+    O.lib().orc_set_num_threads(4)
+    try:
+        n, dl = O.OracleBA(prob).dogleg(8, solver=0)
+        _, gn = O.OracleBA(prob).gauss_newton(2, solver=0)
+    finally:
+        O.lib().orc_set_num_threads(saved)
     # near the optimum the radius shrinks to rounding level and a run may end
     # early with Terminate (100 rejected tries); where depends on the oracle's
     # OpenMP summation order
