@@ -60,6 +60,11 @@ int g2ohip_create(int device, g2ohip_ctx** out) try {
       return G2OHIP_ERR_HIP;
     }
     c->device = device;
+    // the no-allocation entry point's scratch array: its address on this device
+    if (g2ohip::noalloc_init_device(device) != hipSuccess) {
+      delete c;
+      return G2OHIP_ERR_HIP;
+    }
     // host-mapped PCG progress record and LM trial scalars (optional:
     // without them, or with G2OHIP_HOST_MAPPED=0 in the environment, the host
     // copies and synchronises the stream)
@@ -346,6 +351,15 @@ int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int
                             double* residual_io, int* iterations) {
   return guarded(ctx, [&]() -> int {
     return ctx->linear_solve_pcg(nb, bd, nblk, brow, bcol, blocks, b, x, tol, max_iter, residual_io, iterations);
+  });
+}
+
+int g2ohip_linear_solve_pcg_device(g2ohip_ctx* ctx, int nb, int bd, const int* F_ptr, const int* F_col,
+                                   const double* F_val, const int* F_diag, const double* b, double* x, double tol,
+                                   int max_iter, double* residual_io, int* iterations) {
+  return guarded(ctx, [&]() -> int {
+    return ctx->linear_solve_pcg_device(nb, bd, F_ptr, F_col, F_val, F_diag, b, x, tol, max_iter, residual_io,
+                                        iterations);
   });
 }
 

@@ -67,6 +67,10 @@ struct WinPlanBufs {
   DBuf<unsigned> pair_rows;
 };
 
+// noalloc.hip: the per-device scratch array of g2ohip_linear_solve_pcg_device
+hipError_t noalloc_init_device(int device);
+size_t noalloc_bytes(int nb, int bd);
+
 }  // namespace g2ohip
 
 struct g2ohip_ctx {
@@ -183,6 +187,9 @@ struct g2ohip_ctx {
   int optimize_dogleg(int iterations, g2ohip_iter_stats* stats, int* done);
   int get_hessian(double* Hpp, double* Hll, double* Hpl_edge);
   int get_schur(int* rows, int* cols, double* blocks, double* bschur);
+  int linear_solve_pcg_device(int nb, int bd, const int* F_ptr, const int* F_col, const double* F_val,
+                              const int* F_diag, const double* b, double* x, double tol, int max_iter,
+                              double* residual_io, int* iterations);
   int linear_solve_pcg(int nb, int bd, int nblk, const int* brow, const int* bcol, const double* blocks,
                        const double* b, double* x, double tol, int max_iter, double* residual_io, int* iterations);
   int adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,

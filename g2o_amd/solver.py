@@ -407,3 +407,42 @@ def linear_solve_pcg(ctx: Context, nb: int, bd: int, rows, cols, blocks, b, tol=
     ctx.call("g2ohip_linear_solve_pcg", nb, bd, len(rows), _ip(rows), _ip(cols), _dp(blocks), _dp(b), _dp(x),
              float(tol), int(max_iter), ct.byref(r), ct.byref(it))
     return x, it.value, r.value
+
+
+def full_block_rows(nb: int, bd: int, rows, cols, blocks):
+    """Upper blocks (row <= col) -> full block rows of the symmetric matrix
+    (both triangles, columns sorted within a row): (F_ptr, F_col, F_val,
+    F_diag), the layout g2ohip_linear_solve_pcg_device takes."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    blocks = np.asarray(blocks, dtype=np.float64).reshape(-1, bd, bd)
+    off = rows != cols
+    r = np.concatenate([rows, cols[off]])
+    c = np.concatenate([cols, rows[off]])
+    v = np.concatenate([blocks, blocks[off].transpose(0, 2, 1)])
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    F_ptr = np.zeros(nb + 1, dtype=np.int32)
+    np.add.at(F_ptr, r + 1, 1)
+    F_ptr = np.cumsum(F_ptr).astype(np.int32)
+    diag = np.flatnonzero(r == c)
+    if diag.size != nb:
+        raise ValueError("every block row needs its diagonal block")
+    F_diag = diag.astype(np.int32)
+    return F_ptr, c.astype(np.int32), np.ascontiguousarray(v), F_diag
+
+
+def linear_solve_pcg_device(ctx: Context, nb: int, bd: int, F_ptr, F_col, F_val, F_diag, b, x, tol=1e-6,
+                            max_iter=-1, residual=-1.0):
+    """LinearSolverPCG::solve on a device matrix, allocating nothing
+    (g2ohip_linear_solve_pcg_device). F_ptr / F_col / F_diag (int32),
+    F_val / b / x (float64) are device addresses (ints) or objects with a
+    data_ptr() (e.g. contiguous CUDA tensors of those dtypes); x is
+    overwritten. Returns (iterations, residual memory)."""
+    def addr(a):
+        return int(a.data_ptr()) if hasattr(a, "data_ptr") else int(a)
+    r = ct.c_double(residual)
+    it = ct.c_int()
+    ctx.call("g2ohip_linear_solve_pcg_device", nb, bd, addr(F_ptr), addr(F_col), addr(F_val), addr(F_diag),
+             addr(b), addr(x), float(tol), int(max_iter), ct.byref(r), ct.byref(it))
+    return it.value, r.value

@@ -262,6 +262,23 @@ int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int
                             const double* blocks, const double* b, double* x, double tol, int max_iter,
                             double* residual_io, int* iterations);
 
+/* The same solve on a matrix already in device memory, allocating nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). All pointers
+ * are device pointers: full block rows of the symmetric matrix (both
+ * triangles) F_ptr[nb+1], F_col[nnz] (sorted within a row), F_val[nnz*bd*bd]
+ * row-major blocks, F_diag[nb] the position of block (i, i) in row i; b and x
+ * [nb*bd]. The scratch (preconditioner, PCG vectors, partial sums) is a
+ * per-GPU static array of 6 MiB held until the process ends, so
+ * nb * bd <= G2OHIP_NOALLOC_MAX_N (larger systems: G2OHIP_ERR_UNSUPPORTED).
+ * Calls that overlap on one GPU share that array and race: serialise them.
+ * Same stop rule, residual memory and iteration count as
+ * g2ohip_linear_solve_pcg; x is written on the device. */
+#define G2OHIP_NOALLOC_MAX_N 65536
+int g2ohip_linear_solve_pcg_device(g2ohip_ctx* ctx, int nb, int bd, const int* F_ptr, const int* F_col,
+                                   const double* F_val, const int* F_diag, const double* b, double* x, double tol,
+                                   int max_iter, double* residual_io, int* iterations);
+
 /* ------------------------------------------------------ 3D pose graph */
 /* SparseOptimizer::initializeOptimization + BlockSolver::buildStructure for
  * EdgeSE3 edges between VertexSE3 poses (edge_se3.cpp, vertex_se3.h), 6x6
