@@ -9,6 +9,10 @@ one OptimizationAlgorithmLevenberg::solve. `value` is the FP32+FP64-refinement
 path; the plain FP64 GPU path runs the same iterations from the same start and
 its rate and the relative final-chi2 difference are reported in `config`.
 
+Further legs (reported in `legs`, not in `value`): the implicit (matrix-free)
+Schur mode on the same workload, and the 10,000-pose sphere pose graph
+(EdgeSE3, PCG tolerance 1e-8).
+
 Prints one JSON line. Only N=1 is supported (one GPU).
 """
 import argparse
@@ -22,109 +26,175 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LM iter/s at 1.5k-cam/500k-pt BA, FP32+refine vs FP64, rel chi2 diff ≤1e-6"
 PEAK_HBM = 8.0e12
+GATE = 1e-6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ba_1500cam_500kpt.md")
+TRACE_FILE = os.path.join(ROOT, "profiles", "kernel_stats_ba_1500cam_500kpt.md")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel, info, prec_bytes, lin_cam_rows):
-    """Compulsory HBM bytes of one launch (every array touched once).
-    info = structure_info(): [Cf, Pf, E, nS, nlc, windows, od pairs, od partials, maxslots, nF, lin rows, npos, nq]."""
-    Cf, Pf, E, nF = int(info[0]), int(info[1]), int(info[2]), int(info[9])
-    pairs, slots, nq = int(info[6]), int(info[7]), int(info[12])
-    hb = 18 * prec_bytes  # one 6x3 Hpl block
-    if kernel == "linearize":
-        # slot: cam 4 + pos 4 + meas 16 + w 8 + Hpl write; point: ids 12 + X 24 + Hll 48 + b_l 24;
-        # chunk camera partials 27 f64
-        return E * (32 + hb) + Pf * 108 + lin_cam_rows * 216
-    if kernel == "schur_windows":
-        # observation: Hpl block + q index 4 + point index 4; point: Hll 48 (6 unique f64);
-        # pair list 4 per pair product; partial: 36 values written + slot index 4
-        return nq * (hb + 8) + Pf * 48 + pairs * 4 + slots * (36 * prec_bytes + 4)
-    if kernel == "pcg_spmv":
-        # matrix blocks (both triangles) + column indices + row pointers; vectors s, d_old read and d, q
-        # written once (6 Cf doubles each); per-row dot partials
-        return nF * (36 * prec_bytes + 4) + (Cf + 1) * 4 + 4 * 6 * Cf * 8 + Cf * 8
-    if kernel == "refine":
-        # slot: cam 4 + pos 4 + meas 16 + w 8; point: ids 12 + X 24 + x_l write 24; chunk camera partials 6 f64
-        return E * 32 + Pf * 60 + lin_cam_rows * 48
-    if kernel == "chi2":
-        return E * 32 + Pf * 24
-    return None
+# --------------------------------------------------------------- byte models
+# Compulsory HBM bytes of each stage of one LM trial of the FP32+refine path:
+# every input read once and every output written once, in this build's data
+# layout (FP32 Hpl and off-diagonal Hschur blocks; FP64 state, measurements,
+# weights, point blocks and Schur diagonal). Intermediates a stage writes and
+# reads back (partial sums, window pair lists, L^-1) are not counted: they are
+# what the measured traffic exposes as excess.
+def stage_bytes(info, C):
+    Cf, Pf, E, nS = int(info[0]), int(info[1]), int(info[2]), int(info[3])
+    npos, nq = int(info[11]), int(info[12])
+    nO = nS - Cf  # off-diagonal upper Hschur blocks
+    cam_in = C * (96 + 32)  # camera [R|t] table and intrinsics
+    obs_in = E * (4 + 4 + 16 + 8)  # camera and point index, measurement, weight
+    return {
+        # K1+K2: errors, Jacobians, Hll / b_l per point, Hpp / b_p per camera, Hpl per observation
+        "linearize": obs_in + cam_in + npos * (4 + 24) + nq * 72 + Pf * (48 + 24) + Cf * (288 + 48),
+        # K4+K5: Hpl + its camera index, Hll and b_l, Hpp and b_p in; Hschur (FP32 off-diagonal
+        # upper blocks, FP64 diagonal), bschur and the block-Jacobi inverses out
+        "schur": nq * (72 + 4) + Pf * (48 + 24) + npos * 4 + Cf * (288 + 48) + nO * (144 + 4) + Cf * (288 + 48 + 144),
+        # K7, one iteration (SpMV + update): each stored block once, its column index, the FP64
+        # diagonal, the preconditioner, and the vectors (SpMV: s, d_old in, d, q out; update:
+        # d, q, r, x in, x, r, s out)
+        "pcg_iteration": nO * (144 + 4) + Cf * (288 + 4) + Cf * 144 + 11 * 48 * Cf,
+        # K7 SpMV launch alone
+        "pcg_spmv": nO * (144 + 4) + Cf * (288 + 4) + 4 * 48 * Cf,
+        # K9 / K10: FP64 residual of the reduced system from the state, back-substitution x_l
+        "refine": obs_in + cam_in + npos * (4 + 24) + Pf * 24 + Cf * 96,
+        # K1: new chi2
+        "chi2": obs_in + cam_in + npos * 24,
+    }
 
 
-KERNEL_SYMBOL = {"schur_windows": "k_schur_win<float, 512>", "pcg_spmv": "k_pcg_dir_spmv<float, 6, false>",
-                 "linearize": "k_linearize<float>", "refine": "k_refine", "chi2": "k_chi2"}
+SPMV_SYMBOL = "k_pcg_dir_spmv<float, 6, false, true>"
+STAGE_KERNELS = {
+    "linearize": ["k_linearize<float>"],
+    "schur": ["k_point_linv", "k_schur_win<float, 512>", "k_schur_offdiag<float>", "k_schur_diag_pm",
+              "k_schur_diag_finish<float>"],
+    "pcg_iteration": [SPMV_SYMBOL, "k_pcg_update<float, 6>"],
+    "refine": ["k_refine"],
+    "chi2": ["k_chi2"],
+}
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of the FP32-path kernel from the committed rocprofv3
-    counter summary of this workload (FETCH_SIZE + WRITE_SIZE, each calibrated
-    against a plain copy), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_ba_1500cam_500kpt.md")
-    key = f"traffic_bytes_per_launch[{KERNEL_SYMBOL.get(kernel)}]:"
+def profile_table(path, key):
+    """{kernel symbol: value} from `key[symbol]: value` lines of a committed
+    rocprofv3 summary, or {}."""
+    out = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith(key):
-                    return int(float(line.split(":", 1)[1]))
+                if line.startswith(key + "["):
+                    sym = line[len(key) + 1:line.index("]:")]
+                    out[sym] = float(line.split("]:", 1)[1])
     except OSError:
         pass
-    return None
+    return out
 
 
-def run_gpu(prob, precision, refine, steps, warmup):
+def run_gpu(prob, precision, refine, steps, warmup, schur_mode=None, replay=False):
     """Times `steps` LM iterations (profiling off), then repeats them from the
     same start with per-kernel HIP-event timing for the breakdown."""
-    import torch
     import g2o_amd
-    ba = g2o_amd.BundleAdjuster(prob, device=0, precision=precision, refine_steps=refine)
+    kw = {} if schur_mode is None else {"schur_mode": schur_mode}
+    ba = g2o_amd.BundleAdjuster(prob, device=0, precision=precision, refine_steps=refine, **kw)
     if warmup > 0:
         ba.optimize(warmup)
         ba.set_state(prob.cams, prob.pts)
-    torch.cuda.synchronize()
+    # optimize() returns after its stream has drained
     t0 = time.perf_counter()
     stats = ba.optimize(steps)
-    torch.cuda.synchronize()
     t1 = time.perf_counter()
     ba.set_state(prob.cams, prob.pts)
     ba.set_profiling(True)
     pstats = ba.optimize(steps)
     prof = ba.profile()
     ba.set_profiling(False)
-    # full PCG iterations of the profiled run (the SpMV slot also holds the
-    # cheap launches that only find the solve finished)
     prof["pcg_spmv_iterations"] = sum(st["iterations_linear_solver"] for st in pstats)
-    return ba, stats, t1 - t0, prof
+    rep = ba.replay_pcg(200) if replay else None
+    return ba, stats, t1 - t0, prof, pstats, rep
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(prob, iters):
-    """The oracle's LM run from the same start: (baseline record, its chi2
-    after each iteration)."""
+    """The oracle's LM run from the same start on all host threads, then one
+    iteration single-threaded: (baseline record, chi2 after each iteration)."""
     from oracle import oracle
-    n_threads = oracle.lib().orc_num_threads()
+    L = oracle.lib()
+    n_threads = L.orc_num_threads()
     t = time.perf_counter()
     ora = oracle.OracleBA(prob)
     setup = time.perf_counter() - t
     t = time.perf_counter()
     n, trace, times = ora.optimize(iters, solver=1, tol=1e-6)
     dt = time.perf_counter() - t
-    return ({"value": n / dt, "unit": "LM iter/s", "cores": int(n_threads), "kind": "port",
-             "sample": f"{n} LM iterations (FP64 explicit Schur + block-Jacobi PCG, tol 1e-6) of the same "
-                       f"workload from the same start on the host; structure setup {setup:.1f} s excluded; "
-                       f"chi2 after: {trace[-1][1]:.6e}"},
-            [float(c) for c in trace[:n, 1]])
+    names = ["timeResiduals", "timeQuadraticForm", "timeSchurComplement", "timeLinearSolver", "timeUpdate",
+             "timeIteration"]
+    phases = {k: round(float(v) / max(n, 1), 4) for k, v in zip(names, times)}
+    rec = {"value": n / dt, "unit": "LM iter/s", "cores": int(n_threads), "kind": "port",
+           "sample": f"{n} LM iterations (FP64 explicit Schur + block-Jacobi PCG, tol 1e-6) of the same "
+                     f"workload from the same start on the host; structure setup {setup:.1f} s excluded; "
+                     f"chi2 after: {trace[-1][1]:.6e}",
+           "cpu_model": cpu_model(),
+           "seconds_per_iteration_by_phase": phases}
+    # one iteration on one thread (a bounded sample of the same run)
+    L.orc_set_num_threads(1)
+    try:
+        ora1 = oracle.OracleBA(prob)
+        t = time.perf_counter()
+        n1, _, _ = ora1.optimize(1, solver=1, tol=1e-6)
+        rec["single_thread_iter_per_s"] = n1 / (time.perf_counter() - t)
+    finally:
+        L.orc_set_num_threads(int(n_threads))
+    return rec, [float(c) for c in trace[:n, 1]]
+
+
+def rel_list(a, b):
+    return [abs(x - y) / abs(y) for x, y in zip(a, b)]
+
+
+def pose_graph_leg(steps):
+    import g2o_amd
+    from g2o_amd import datasets
+    t = time.perf_counter()
+    pg = datasets.make_sphere_pose_graph(seed=1001)
+    gen = time.perf_counter() - t
+    opt = g2o_amd.PoseGraphOptimizer(pg, device=0, pcg_tolerance=1e-8)
+    opt.optimize(1)  # first-use setup
+    opt.set_state(pg.poses)
+    t = time.perf_counter()
+    st = opt.optimize(steps)
+    dt = time.perf_counter() - t
+    info = opt.structure_info()
+    pcg = sum(s["iterations_linear_solver"] for s in st)
+    return {"workload": f"sphere pose graph {pg.num_poses} poses / {pg.num_edges} EdgeSE3, seed 1001 "
+                        f"({gen:.1f} s to generate)",
+            "path": "FP64, 6x6 blocks without Schur, block-Jacobi PCG tol 1e-8",
+            "iter_per_s": round(len(st) / dt, 3), "ms_per_step": round(dt / len(st) * 1e3, 3), "steps": len(st),
+            "pcg_iterations": pcg, "us_per_pcg_iteration_incl_linearisation": round(dt / max(pcg, 1) * 1e6, 2),
+            "hessian_upper_blocks": int(info[2]), "final_chi2": st[-1]["chi2"]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--refine", type=int, default=4)
     ap.add_argument("--cpu-iters", type=int, default=None, help="oracle LM iterations (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="skip the implicit-Schur and pose-graph legs")
     ap.add_argument("--cams", type=int, default=1500)
     ap.add_argument("--points", type=int, default=500_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -146,7 +216,6 @@ def main():
         print("bench.py measures one GPU only: the solver path is single-GPU (no multi-GPU run in this build)",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np  # noqa: F401
     import g2o_amd
     from g2o_amd import datasets
 
@@ -154,39 +223,59 @@ def main():
     prob = datasets.make_walk_ba(seed=1002, num_cams=args.cams, num_pts=args.points)
     log(f"workload: C={prob.num_cams} P={prob.num_pts} E={prob.num_edges} ({time.perf_counter() - t:.1f} s)")
 
-    ba32, s32, dt32, prof32 = run_gpu(prob, g2o_amd.FP32, args.refine, args.steps, args.warmup)
+    ba32, s32, dt32, prof32, p32, rep = run_gpu(prob, g2o_amd.FP32, args.refine, args.steps, args.warmup,
+                                                replay=True)
     log(f"fp32+refine: {len(s32)} iters in {dt32 * 1e3:.1f} ms; chi2 {s32[-1]['chi2']:.9e}")
     log(f"profile fp32: {prof32}")
-    ba64, s64, dt64, prof64 = run_gpu(prob, g2o_amd.FP64, 0, args.steps, args.warmup)
+    ba64, s64, dt64, prof64, p64, _ = run_gpu(prob, g2o_amd.FP64, 0, args.steps, args.warmup)
     log(f"fp64: {len(s64)} iters in {dt64 * 1e3:.1f} ms; chi2 {s64[-1]['chi2']:.9e}")
     log(f"profile fp64: {prof64}")
     n32 = len(s32)
     value = n32 / dt32
     rel = abs(s32[-1]["chi2"] - s64[-1]["chi2"]) / s64[-1]["chi2"]
+    # the timed and the profiled runs of one path start from the same state
+    # and differ only in the order of the LDS sums: their gap is the path's
+    # own run-to-run rounding, amplified by the LM iteration
+    noise64 = abs(p64[-1]["chi2"] - s64[-1]["chi2"]) / s64[-1]["chi2"] if len(p64) == len(s64) else None
+    noise32 = abs(p32[-1]["chi2"] - s32[-1]["chi2"]) / s32[-1]["chi2"] if len(p32) == len(s32) else None
 
     info = ba32.structure_info()
-    lin_cam_rows = int(info[10])
-    # per-kernel rooflines of the FP32 path. `roofline` is the Schur window
-    # kernel: the largest launch (a third of the bytes of a trial), whose
-    # HIP-event duration is its kernel time. The PCG iteration kernel has the
-    # largest total time, but its launches are ~12 us, so the events around
-    # each one also time its launch latency (its rocprof kernel duration is in
-    # the kernel-trace summary).
-    roofs = {}
-    for k in ("pcg_spmv", "schur_windows", "linearize", "refine", "chi2"):
-        tot, cnt = prof32[k]
-        if k == "pcg_spmv":
-            cnt = prof32["pcg_spmv_iterations"]
-        if not cnt:
-            continue
-        avg_s = tot / cnt / 1e3
-        bytes_ = algorithmic_bytes(k, info, 4, lin_cam_rows)
-        ach = bytes_ / avg_s
-        roofs[k] = {"bound": "hbm", "kernel": KERNEL_SYMBOL[k], "achieved": round(ach / 1e9, 1),
-                    "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": round(ach / PEAK_HBM, 4),
-                    "traffic": measured_traffic(k), "bytes_per_launch": int(bytes_),
-                    "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt, "total_ms": round(tot, 3)}
-    roof = roofs.get("schur_windows")
+    C = prob.num_cams
+    alg = stage_bytes(info, C)
+    traffic = profile_table(TRAFFIC_FILE, "traffic_bytes_per_launch")
+    trace_us = profile_table(TRACE_FILE, "avg_us")
+    trials = sum(s["levenberg_iterations"] for s in p32)
+    iters_pcg = prof32["pcg_spmv_iterations"]
+    stage_time = {  # seconds per launch / per trial / per iteration, from HIP events of the profiled run
+        "linearize": prof32["linearize"][0] / 1e3 / max(prof32["linearize"][1], 1),
+        "schur": prof32["schur"][0] / 1e3 / max(prof32["schur"][1], 1),
+        "pcg_iteration": prof32["pcg"][0] / 1e3 / max(iters_pcg, 1),
+        "refine": prof32["refine"][0] / 1e3 / max(prof32["refine"][1], 1),
+        "chi2": prof32["chi2"][0] / 1e3 / max(prof32["chi2"][1], 1),
+    }
+    stages = {}
+    for k, sec in stage_time.items():
+        meas = [traffic.get(sym) for sym in STAGE_KERNELS[k]]
+        tr = None if any(m is None for m in meas) else int(sum(meas))
+        ach = alg[k] / sec if sec > 0 else 0.0
+        stages[k] = {"bound": "hbm", "algorithmic_bytes": int(alg[k]), "time_us": round(sec * 1e6, 2),
+                     "achieved": round(ach / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": round(ach / PEAK_HBM, 4), "traffic": tr,
+                     "traffic_over_algorithmic": None if tr is None else round(tr / alg[k], 2),
+                     "kernels": STAGE_KERNELS[k]}
+    stages["pcg_iteration"]["time_note"] = ("HIP events around each whole PCG solve (launches and the gaps "
+                                            "between them) over its iterations")
+    # roofline: the kernel with the largest total time, the direction + SpMV
+    # launch of the PCG, timed back to back between HIP events on its stream
+    us_spmv, us_iter = rep
+    ach = alg["pcg_spmv"] / (us_spmv * 1e-6)
+    roof = {"bound": "hbm", "kernel": SPMV_SYMBOL, "achieved": round(ach / 1e9, 1), "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s", "frac": round(ach / PEAK_HBM, 4), "traffic": traffic.get(SPMV_SYMBOL),
+            "bytes_per_launch": int(alg["pcg_spmv"]), "avg_launch_us": round(us_spmv, 2),
+            "avg_launch_us_trace": trace_us.get(SPMV_SYMBOL),
+            "launches_in_timed_run": iters_pcg, "total_ms_profiled_run": round(prof32["pcg_spmv"][0], 3),
+            "note": "avg_launch_us: 200 back-to-back launches between HIP events (g2ohip_replay_pcg), "
+                    "including the dispatch gap; avg_launch_us_trace: rocprofv3 kernel trace"}
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -198,39 +287,67 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 refinement, chi2, gain ratio, exp-map update)",
+        "dtype": "f32 (f64 refinement, Schur diagonal, chi2, gain ratio, exp-map update)",
         "data": "synthetic",
         "config": {
             "workload": f"BA random walk {prob.num_cams} cams / {prob.num_pts} pts / {prob.num_edges} obs, "
                         "Huber 2, 5% outliers, seed 1002",
-            "path": f"explicit Schur, FP32 blocks + FP64 refinement x{args.refine}",
+            "path": f"explicit Schur, FP32 off-diagonal blocks + FP64 diagonal, FP64 refinement (<= {args.refine} "
+                    "corrections, until the FP64 residual meets the stop rule)",
             "fp64_path_iter_per_s": round(len(s64) / dt64, 3),
             "fp64_path_ms_per_step": round(dt64 / len(s64) * 1e3, 3),
             "speedup_vs_fp64_path": round(value / (len(s64) / dt64), 3),
             "rel_final_chi2_diff_vs_fp64": rel,
+            "accuracy_gate": {"bound": GATE, "pass": bool(rel <= GATE),
+                              "fp64_rerun_rel_final_chi2_diff": noise64,
+                              "fp32_rerun_rel_final_chi2_diff": noise32},
             "final_chi2_fp32_refine": s32[-1]["chi2"],
             "final_chi2_fp64": s64[-1]["chi2"],
             "lm_trials": [s["levenberg_iterations"] for s in s32],
             "pcg_iterations": [s["iterations_linear_solver"] for s in s32],
+            "refine_steps": [s["refine_steps"] for s in s32],
+            "fp64_fallback": [s["fp64_fallback"] for s in s32],
             "kernel_ms_fp32": {k: round(v[0], 3) for k, v in prof32.items() if k in g2o_amd.KERNEL_NAMES},
             "kernel_ms_fp64": {k: round(v[0], 3) for k, v in prof64.items() if k in g2o_amd.KERNEL_NAMES},
+            "trials_profiled_run": trials,
+            "us_per_pcg_iteration_replay": round(us_iter, 2),
             "hschur_blocks": int(info[3]),
             "schur_windows": int(info[5]),
             "schur_offdiag_pairs": int(info[6]),
             "schur_partials": int(info[7]),
         },
         "roofline": roof,
-        "roofline_by_kernel": roofs,
+        "roofline_by_stage": stages,
+        "mfma": {"used": False, "utilisation": 0.0,
+                 "note": "the cfg2 path runs no MFMA: its Schur products are rank-3 6x6 updates per point "
+                         "(K = 3), which fill about a fifth of a 16x16x4 MFMA tile for tracks of mean length "
+                         "6, no better than the vector units; the SpMV and PCG are latency bound"},
     }
+    if not args.no_legs:
+        log("leg: implicit Schur ...")
+        bi, si, dti, profi, _, _ = run_gpu(prob, g2o_amd.FP32, args.refine, args.steps, args.warmup,
+                                           schur_mode=g2o_amd.SCHUR_IMPLICIT)
+        pcg_i = profi["pcg_spmv_iterations"]
+        out["legs"] = {"implicit_schur": {
+            "path": "implicit (matrix-free) Schur inside the PCG, FP32 Hpl + FP64 refinement",
+            "iter_per_s": round(len(si) / dti, 3), "ms_per_step": round(dti / len(si) * 1e3, 3),
+            "rel_final_chi2_diff_vs_fp64": abs(si[-1]["chi2"] - s64[-1]["chi2"]) / s64[-1]["chi2"],
+            "pcg_iterations": pcg_i,
+            "us_per_pcg_iteration": round(profi["pcg"][0] * 1e3 / max(pcg_i, 1), 2),
+            "kernel_ms": {k: round(v[0], 3) for k, v in profi.items() if k in g2o_amd.KERNEL_NAMES}}}
+        log(f"implicit: {out['legs']['implicit_schur']}")
+        del bi
+        log("leg: pose graph ...")
+        out["legs"]["pose_graph"] = pose_graph_leg(args.steps)
+        log(f"pose graph: {out['legs']['pose_graph']}")
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"], chi_o = cpu_baseline(prob, args.cpu_iters or args.steps)
-        # the same LM iterations on the oracle: chi2 error of the FP32+refine
-        # path against the FP64 CPU oracle, per iteration
-        k = min(len(chi_o), len(s32))
+        k = min(len(chi_o), len(s32), len(s64))
         if k:
-            out["config"]["rel_chi2_diff_vs_oracle_per_iter"] = [abs(s32[i]["chi2"] - chi_o[i]) / chi_o[i]
-                                                                 for i in range(k)]
+            out["config"]["rel_chi2_diff_vs_oracle_per_iter"] = {
+                "fp32_refine": rel_list([s["chi2"] for s in s32[:k]], chi_o[:k]),
+                "fp64": rel_list([s["chi2"] for s in s64[:k]], chi_o[:k])}
     print(json.dumps(out), flush=True)
 
 

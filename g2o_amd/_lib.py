@@ -32,7 +32,7 @@ EXPORTS = [
     "g2ohip_graph_destroy", "g2ohip_graph_last_error", "g2ohip_graph_log", "g2ohip_graph_load", "g2ohip_graph_info",
     "g2ohip_graph_get_ba", "g2ohip_graph_get_ba_params", "g2ohip_graph_get_pg", "g2ohip_graph_set_ba_state", "g2ohip_graph_set_pg_state",
     "g2ohip_graph_add_ba", "g2ohip_graph_add_pg", "g2ohip_graph_save", "g2ohip_graph_load_bal", "g2ohip_graph_get_bal",
-    "g2ohip_ba_adjoint", "g2ohip_ba_newton", "g2ohip_linear_solve_pcg_device",
+    "g2ohip_ba_adjoint", "g2ohip_ba_newton", "g2ohip_linear_solve_pcg_device", "g2ohip_replay_pcg",
 ]
 K_LINEARIZE, K_SCHUR, K_PCG, K_REFINE, K_CHI2, K_UPDATE, K_SCHUR_WINDOWS, K_PCG_SPMV = range(8)
 KERNEL_NAMES = ["linearize", "schur", "pcg", "refine", "chi2", "update", "schur_windows", "pcg_spmv"]
@@ -108,6 +108,7 @@ def lib():
         "g2ohip_get_profile": (ct.c_int, [_vp, ct.c_int, _dp, _ip]),
         "g2ohip_linear_solve_pcg": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _dp, _dp, _dp,
                                                ct.c_double, ct.c_int, _dp, _ip]),
+        "g2ohip_replay_pcg": (ct.c_int, [_vp, ct.c_int, _dp, _dp]),
         "g2ohip_linear_solve_pcg_device": (ct.c_int, [_vp, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                                       ct.c_double, ct.c_int, _dp, _ip]),
         "g2ohip_ba_adjoint": (ct.c_int, [_vp, _dp, _dp, _dp, _dp, ct.c_double, ct.c_int, _ip]),

@@ -346,6 +346,10 @@ int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launc
   });
 }
 
+int g2ohip_replay_pcg(g2ohip_ctx* ctx, int reps, double* us_spmv, double* us_iteration) {
+  return guarded(ctx, [&]() -> int { return ctx->replay_pcg(reps, us_spmv, us_iteration); });
+}
+
 int g2ohip_linear_solve_pcg(g2ohip_ctx* ctx, int nb, int bd, int nblk, const int* brow, const int* bcol,
                             const double* blocks, const double* b, double* x, double tol, int max_iter,
                             double* residual_io, int* iterations) {

@@ -626,6 +626,7 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   }
   last_info[0] = total_it;
   last_info[2] = pcg_residual_mem;
+  last_solve_prec = which == 0 ? G2OHIP_FP32 : G2OHIP_FP64;
   have_solution = true;
   return G2OHIP_OK;
 }
@@ -1030,6 +1031,60 @@ int g2ohip_ctx::marginals(int nblk, const int* rows, const int* cols, double* ou
       std::fill(o, o + 36, 0.0);
   }
   return G2OHIP_OK;
+}
+
+// ------------------------------------------------------------------ timing
+// Kernel time of the PCG iteration on the last solve's operator: `reps`
+// direction + SpMV launches back to back between two HIP events on the
+// context's stream (us_spmv), then `reps` iterations of both launches
+// (us_iter). The launches continue the last solve's recurrences on its
+// buffers with a stop threshold that never triggers, so every launch does a
+// full iteration's work; the buffers are scratch once the solution has been
+// copied out, and the next solve re-initialises them.
+int g2ohip_ctx::replay_pcg(int reps, double* us_spmv, double* us_iter) {
+  if (mode != 0 || !have_solution || opt.schur_mode != G2OHIP_SCHUR_EXPLICIT)
+    return fail(G2OHIP_ERR_INVALID, "needs a solve of the explicit bundle-adjustment system first");
+  if (reps < 1) return fail(G2OHIP_ERR_INVALID, "reps must be positive");
+  const bool single = last_solve_prec == G2OHIP_FP32;
+  const int nb = st.Cf;
+  DBuf<PcgScalars> sc;
+  HK(sc.alloc(1));
+  PcgScalars init{};
+  init.dn[0] = init.dn[1] = 1.0;
+  init.d0 = -1e300;  // never reached: every launch runs its whole iteration
+  PcgControl ctl{0.0, -1.0, -1e300};
+  hipEvent_t e[3];
+  for (auto& ev : e) HK(hipEventCreate(&ev));
+  auto body = [&](auto& B, const double* diag) -> int {
+    HK(hipMemcpyAsync(sc.p, &init, sizeof(init), hipMemcpyHostToDevice, stream));
+    HK(hipStreamSynchronize(stream));
+    using MT = typename std::remove_reference<decltype(*B.F_val.p)>::type;
+    HK(hipEventRecord(e[0], stream));
+    for (int k = 0; k < reps; ++k)
+      HK((launch_pcg_dir_spmv<MT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, B.d2.p, B.d.p, B.q.p, d_dn_part.p,
+                                     d_dq_part.p, sc.p, ctl, 1, 0, stream, diag)));
+    HK(hipEventRecord(e[1], stream));
+    for (int k = 0; k < reps; ++k) {
+      double* dn = (k & 1) ? B.d2.p : B.d.p;
+      double* dold = (k & 1) ? B.d.p : B.d2.p;
+      HK((launch_pcg_dir_spmv<MT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, dold, dn, B.q.p, d_dn_part.p,
+                                     d_dq_part.p, sc.p, ctl, 1 + k, 0, stream, diag)));
+      HK((launch_pcg_update<MT, 6>(nb, B.Jinv.p, B.x.p, B.r.p, dn, B.q.p, B.s.p, d_dq_part.p, pcg_spmv_blocks(nb),
+                                   d_dn_part.p, sc.p, 1 + k, stream)));
+    }
+    HK(hipEventRecord(e[2], stream));
+    HK(hipEventSynchronize(e[2]));
+    float a = 0, b = 0;
+    HK(hipEventElapsedTime(&a, e[0], e[1]));
+    HK(hipEventElapsedTime(&b, e[1], e[2]));
+    if (us_spmv) *us_spmv = 1e3 * a / reps;
+    if (us_iter) *us_iter = 1e3 * b / reps;
+    return G2OHIP_OK;
+  };
+  const int rc = single ? body(f32, d_Sdiag.p) : body(f64, nullptr);
+  for (auto& ev : e) (void)hipEventDestroy(ev);
+  have_solution = rc == G2OHIP_OK && have_solution;
+  return rc;
 }
 
 // --------------------------------------------------------------- inspection

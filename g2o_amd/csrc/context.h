@@ -87,6 +87,7 @@ struct g2ohip_ctx {
   bool have_lin = false;      // the blocks hold a Gauss-Newton system (kept across state updates,
                               // as the reference's solver keeps its last Hpp after optimize)
   int system_prec = 0;        // precision of the last build_system
+  int last_solve_prec = 0;    // precision of the last BA solve_impl
   bool have_solution = false;
   bool have_exact = false;    // the FP64 blocks hold the exact Hessian (adjoint / Newton)
   double lambda = 0.0;
@@ -186,6 +187,7 @@ struct g2ohip_ctx {
   int optimize_gauss_newton(int iterations, g2ohip_iter_stats* stats, int* done);
   int optimize_dogleg(int iterations, g2ohip_iter_stats* stats, int* done);
   int get_hessian(double* Hpp, double* Hll, double* Hpl_edge);
+  int replay_pcg(int reps, double* us_spmv, double* us_iter);
   int get_schur(int* rows, int* cols, double* blocks, double* bschur);
   int linear_solve_pcg_device(int nb, int bd, const int* F_ptr, const int* F_col, const double* F_val,
                               const int* F_diag, const double* b, double* x, double tol, int max_iter,

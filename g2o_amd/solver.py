@@ -244,6 +244,13 @@ class BundleAdjuster:
     def set_profiling(self, on: bool = True):
         self.ctx.call("g2ohip_set_profiling", int(on))
 
+    def replay_pcg(self, reps: int = 200):
+        """(us per SpMV launch, us per PCG iteration) on the last solve's
+        operator, back-to-back launches between HIP events (g2ohip_replay_pcg)."""
+        a, b = ct.c_double(), ct.c_double()
+        self.ctx.call("g2ohip_replay_pcg", int(reps), ct.byref(a), ct.byref(b))
+        return a.value, b.value
+
     def profile(self) -> dict:
         """{kernel name: (total ms, launches)} since set_profiling(True)."""
         out = {}

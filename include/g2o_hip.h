@@ -250,6 +250,14 @@ enum g2ohip_kernel {
   G2OHIP_K_COUNT = 8
 };
 int g2ohip_set_profiling(g2ohip_ctx* ctx, int on);
+/* Time of the PCG iteration kernels on the operator of the last explicit
+ * bundle-adjustment solve: `reps` direction + SpMV launches back to back
+ * between two HIP events on the context's stream give *us_spmv (microseconds
+ * per launch); `reps` whole iterations (SpMV + update launches) give
+ * *us_iteration. The launches continue that solve's recurrences on its
+ * scratch vectors (the solution x was already copied out; the next solve
+ * starts afresh). */
+int g2ohip_replay_pcg(g2ohip_ctx* ctx, int reps, double* us_spmv, double* us_iteration);
 int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launches);
 
 /* ------------------------- LinearSolver contract (g2o/core/linear_solver.h) */
