@@ -33,8 +33,10 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize_exact(DevGraph g, con
                                                                 const double* __restrict__ pts,
                                                                 double* __restrict__ Hpl, double* __restrict__ Hll,
                                                                 double* __restrict__ hpp_part) {
-  __shared__ int s_cam[LIN_CAMS];
-  __shared__ double s_cacc[LIN_CAMS * 27];
+  // camera accumulators (27 per chunk camera) and the chunk's camera list
+  extern __shared__ double s_dyn[];
+  double* const s_cacc = s_dyn;
+  int* const s_cam = reinterpret_cast<int*>(s_dyn + 27 * g.lc_maxc);
   __shared__ double s_pacc[LIN_PTS * 6];
   const int ch = blockIdx.x, tid = threadIdx.x;
   const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
@@ -87,7 +89,9 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize_exact(DevGraph g, con
 hipError_t launch_linearize_exact(const DevGraph& g, const double* camRt, const double* pts, double* Hpl,
                                   double* Hll, double* hpp_part, hipStream_t st) {
   if (g.nlc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_linearize_exact, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hpl, Hll, hpp_part);
+  const size_t lds = (27 * sizeof(double) + sizeof(int)) * static_cast<size_t>(g.lc_maxc);
+  if (!lds_fits(k_linearize_exact, lds)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_linearize_exact, dim3(g.nlc), dim3(LIN_THREADS), lds, st, g, camRt, pts, Hpl, Hll, hpp_part);
   return hipGetLastError();
 }
 

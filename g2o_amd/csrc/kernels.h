@@ -6,8 +6,27 @@
 namespace g2ohip {
 
 constexpr int LIN_THREADS = 256;
-constexpr int LIN_CAMS = 128;  // free cameras per linearisation chunk
 constexpr int LIN_PTS = 256;   // points per linearisation chunk
+// The chunk kernels size their LDS camera accumulators by the plan's largest
+// chunk. lds_fits: the kernel's static LDS plus `dynamic` bytes stay within
+// what one workgroup may allocate on the current device; the launchers
+// return hipErrorInvalidValue instead of launching above it. The kernel's
+// static size and the device limit are looked up once per process.
+template <typename Kernel>
+inline bool lds_fits(Kernel kernel, size_t dynamic) {
+  static size_t static_bytes = 0;
+  static int limit = -1;
+  if (limit < 0) {
+    hipFuncAttributes a{};
+    int dev = 0, lim = 0;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return false;
+    static_bytes = a.sharedSizeBytes;
+    limit = lim;
+  }
+  return static_bytes + dynamic <= static_cast<size_t>(limit);
+}
 constexpr int RED_BLOCKS = 512;  // fixed grid of the grid-stride reductions
 // fixed grid of the chi2 pass: each thread's slots form a chain of dependent
 // gathers, so the pass wants more waves in flight than the other reductions

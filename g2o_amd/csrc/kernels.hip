@@ -248,14 +248,14 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
                                                           double* __restrict__ Hll, double* __restrict__ b,
                                                           double* __restrict__ hpp_part,
                                                           double* __restrict__ chunk_chi) {
-  __shared__ int s_cam[LIN_CAMS];
   __shared__ double s_red[LIN_THREADS / 64];
   // camera accumulators sized for the plan's largest chunk (27 per camera),
-  // then the point accumulators: a smaller footprint keeps more workgroups
-  // resident per CU
+  // then the point accumulators and the chunk's camera list: a smaller
+  // footprint keeps more workgroups resident per CU
   extern __shared__ double s_dyn[];
   double* const s_cacc = s_dyn;
   double* const s_pacc = s_dyn + 27 * g.lc_maxc;
+  int* const s_cam = reinterpret_cast<int*>(s_pacc + 9 * LIN_PTS);
   const int ch = blockIdx.x, tid = threadIdx.x;
   const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
   const int s0 = g.pos_ptr[p0], s1 = g.pos_ptr[p1];
@@ -336,7 +336,9 @@ template <typename HT>
 hipError_t launch_linearize(const DevGraph& g, const double* camRt, const double* pts, HT* Hpl, double* Hll,
                             double* b, double* hpp_part, double* chunk_chi, hipStream_t st) {
   if (g.nlc == 0) return hipSuccess;
-  const size_t lds = sizeof(double) * (27 * static_cast<size_t>(g.lc_maxc) + 9 * LIN_PTS);
+  const size_t lds = sizeof(double) * (27 * static_cast<size_t>(g.lc_maxc) + 9 * LIN_PTS) +
+                     sizeof(int) * static_cast<size_t>(g.lc_maxc);
+  if (!lds_fits(k_linearize<HT>, lds)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_linearize<HT>, dim3(g.nlc), dim3(LIN_THREADS), lds, st, g, camRt, pts, Hpl, Hll, b, hpp_part,
                      chunk_chi);
   return hipGetLastError();
@@ -747,8 +749,10 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
                                                               const double* __restrict__ b,
                                                               double* __restrict__ cam_part, PcgScalars* sc, int it) {
   if (sc && sc->done) return;
-  __shared__ int s_cam[LIN_CAMS];
-  __shared__ double s_cacc[LIN_CAMS * 6];
+  // camera accumulators (6 per chunk camera) and the chunk's camera list
+  extern __shared__ double s_dyn[];
+  double* const s_cacc = s_dyn;
+  int* const s_cam = reinterpret_cast<int*>(s_dyn + 6 * g.lc_maxc);
   __shared__ double s_v[LIN_PTS * 3];
   const int ch = blockIdx.x, tid = threadIdx.x;
   const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
@@ -810,7 +814,9 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
 template <typename HT>
 hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const double* p, const double* b,
                                 double* cam_part, const PcgScalars* sc, int it, hipStream_t st) {
-  hipLaunchKernelGGL(k_implicit_pass<HT>, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, Hpl, Dinv, p, b, cam_part,
+  const size_t lds = (6 * sizeof(double) + sizeof(int)) * static_cast<size_t>(g.lc_maxc);
+  if (!lds_fits(k_implicit_pass<HT>, lds)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_implicit_pass<HT>, dim3(g.nlc), dim3(LIN_THREADS), lds, st, g, Hpl, Dinv, p, b, cam_part,
                      const_cast<PcgScalars*>(sc), it);
   return hipGetLastError();
 }
@@ -891,8 +897,10 @@ __global__ __launch_bounds__(LIN_THREADS) void k_refine(DevGraph g, const double
                                                        const double* __restrict__ pts, double lambda,
                                                        const double* __restrict__ xp, double* __restrict__ xl,
                                                        double* __restrict__ cam_part, int* __restrict__ err) {
-  __shared__ int s_cam[LIN_CAMS];
-  __shared__ double s_cacc[LIN_CAMS * 6];
+  // camera accumulators (6 per chunk camera) and the chunk's camera list
+  extern __shared__ double s_dyn[];
+  double* const s_cacc = s_dyn;
+  int* const s_cam = reinterpret_cast<int*>(s_dyn + 6 * g.lc_maxc);
   __shared__ double s_pacc[LIN_PTS * 9];
   const int ch = blockIdx.x, tid = threadIdx.x;
   const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
@@ -992,7 +1000,10 @@ __global__ __launch_bounds__(LIN_THREADS) void k_refine(DevGraph g, const double
 hipError_t launch_refine(const DevGraph& g, const double* camRt, const double* pts, double lambda, const double* xp,
                          double* xl, double* cam_part, int* err, hipStream_t st) {
   if (g.nlc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_refine, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, lambda, xp, xl, cam_part, err);
+  const size_t lds = (6 * sizeof(double) + sizeof(int)) * static_cast<size_t>(g.lc_maxc);
+  if (!lds_fits(k_refine, lds)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_refine, dim3(g.nlc), dim3(LIN_THREADS), lds, st, g, camRt, pts, lambda, xp, xl, cam_part,
+                     err);
   return hipGetLastError();
 }
 
@@ -1237,8 +1248,10 @@ __global__ __launch_bounds__(LIN_THREADS) void k_schur_diag_pm(DevGraph g, const
                                                                const double* __restrict__ Hll,
                                                                const double* __restrict__ b, double lambda,
                                                                double* __restrict__ part, int* __restrict__ err) {
-  __shared__ int s_cam[LIN_CAMS];
-  __shared__ double s_cacc[LIN_CAMS * 27];
+  // camera accumulators (27 per chunk camera) and the chunk's camera list
+  extern __shared__ double s_dyn[];
+  double* const s_cacc = s_dyn;
+  int* const s_cam = reinterpret_cast<int*>(s_dyn + 27 * g.lc_maxc);
   __shared__ double s_pd[LIN_PTS * 9];
   const int ch = blockIdx.x, tid = threadIdx.x;
   const int p0 = g.lc_pos0[ch], p1 = g.lc_pos0[ch + 1];
@@ -1361,8 +1374,10 @@ hipError_t launch_schur_diag_pm(const DevGraph& g, const double* camRt, const do
                                 const double* Hpp, const double* b, double lambda, double* part, HT* F_val, HT* Jinv,
                                 double* bschur, double* Sdiag, int* err, hipStream_t st) {
   if (g.Cf == 0) return hipSuccess;
+  const size_t lds = (27 * sizeof(double) + sizeof(int)) * static_cast<size_t>(g.lc_maxc);
+  if (!lds_fits(k_schur_diag_pm, lds)) return hipErrorInvalidValue;
   if (g.nlc > 0)
-    hipLaunchKernelGGL(k_schur_diag_pm, dim3(g.nlc), dim3(LIN_THREADS), 0, st, g, camRt, pts, Hll, b, lambda, part,
+    hipLaunchKernelGGL(k_schur_diag_pm, dim3(g.nlc), dim3(LIN_THREADS), lds, st, g, camRt, pts, Hll, b, lambda, part,
                        err);
   hipLaunchKernelGGL(k_schur_diag_finish<HT>, dim3(g.Cf), dim3(64), 0, st, g, part, Hpp, b, lambda, F_val, Jinv,
                      bschur, Sdiag, err);

@@ -159,8 +159,9 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   for (int i = 0; i < st.npos; ++i) {
     int n = st.pos_ptr[i + 1] - st.pos_ptr[i];
     int nf = st.pos_nfree[i];
-    if (n > lc.max_slots || nf > lc.max_cams)
-      return "a point has more observations than one workgroup handles (" + std::to_string(n) + ")";
+    if (nf > lc.max_point_cams)
+      return "a point is observed by more than " + std::to_string(lc.max_point_cams) + " free cameras (" +
+             std::to_string(nf) + ")";
     int newc = 0;
     for (int q = st.pos_ptr[i]; q < st.pos_ptr[i] + nf; ++q) {
       int c = st.camH[st.slot_cam[q]];

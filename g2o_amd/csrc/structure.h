@@ -22,11 +22,17 @@
 
 namespace g2ohip {
 
+// Packing targets of the linearisation chunks: a chunk closes before it
+// would exceed them. A single point larger than a target gets a chunk of its
+// own; only max_point_cams (the LDS camera accumulators of one chunk) is a
+// hard limit.
 struct LinPlanCaps {
   int max_slots = 1024;
   int max_cams = 128;
   int max_pts = 256;
+  int max_point_cams = 400;
 };
+
 
 // Window plan of the off-diagonal blocks of the explicit Schur complement.
 // The points, in processing order, are cut into windows of at most obs_max
