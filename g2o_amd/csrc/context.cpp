@@ -496,7 +496,7 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
       const int pb = prof_begin();
       const hipError_t e = launch_pcg_dir_spmv<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, d_old, d, B.q.p,
                                                       d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream,
-                                                      std::is_same<HT, float>::value ? d_Sdiag.p : nullptr);
+                                                      std::is_same<HT, float>::value ? d_Sdiag.p : nullptr, st.nF);
       prof_end(G2OHIP_K_PCG_SPMV, pb);
       return e;
     };
@@ -1062,13 +1062,13 @@ int g2ohip_ctx::replay_pcg(int reps, double* us_spmv, double* us_iter) {
     HK(hipEventRecord(e[0], stream));
     for (int k = 0; k < reps; ++k)
       HK((launch_pcg_dir_spmv<MT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, B.d2.p, B.d.p, B.q.p, d_dn_part.p,
-                                     d_dq_part.p, sc.p, ctl, 1, 0, stream, diag)));
+                                     d_dq_part.p, sc.p, ctl, 1, 0, stream, diag, st.nF)));
     HK(hipEventRecord(e[1], stream));
     for (int k = 0; k < reps; ++k) {
       double* dn = (k & 1) ? B.d2.p : B.d.p;
       double* dold = (k & 1) ? B.d.p : B.d2.p;
       HK((launch_pcg_dir_spmv<MT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, dold, dn, B.q.p, d_dn_part.p,
-                                     d_dq_part.p, sc.p, ctl, 1 + k, 0, stream, diag)));
+                                     d_dq_part.p, sc.p, ctl, 1 + k, 0, stream, diag, st.nF)));
       HK((launch_pcg_update<MT, 6>(nb, B.Jinv.p, B.x.p, B.r.p, dn, B.q.p, B.s.p, d_dq_part.p, pcg_spmv_blocks(nb),
                                    d_dn_part.p, sc.p, 1 + k, stream)));
     }
@@ -1208,7 +1208,7 @@ int g2ohip_ctx::linear_solve_pcg(int nb, int bd, int nblk, const int* brow, cons
     const int mi = max_iter < 0 ? static_cast<int>(n) : max_iter;
     auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
       return launch_pcg_dir_spmv<double, BD>(nb, dFptr.p, dFcol.p, dFval.p, ds.p, d_old, d, dq.p, ddn.p, ddq.p,
-                                             dsc.p, ctl, it, last, stream);
+                                             dsc.p, ctl, it, last, stream, nullptr, Fptr[nb]);
     };
     PcgRun pr;
     bool prog_failed = false;

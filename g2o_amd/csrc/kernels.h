@@ -201,7 +201,8 @@ hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double*
 // operator) then pcg_update. Vectors are FP64, the matrix and the
 // preconditioner blocks MT. Partials: part_rs has pcg_row_blocks(nb)
 // entries, part_dq pcg_spmv_blocks(nb) (explicit) or
-// implicit_reduce_blocks(Cf) (implicit). d and d_old swap every iteration.
+// implicit_reduce_blocks(Cf) (implicit); both are at most nb. d and d_old
+// swap every iteration.
 int pcg_row_blocks(int nb);
 int pcg_spmv_blocks(int nb);
 int implicit_reduce_blocks(int Cf);
@@ -209,12 +210,14 @@ template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
                            double* part_rs, PcgScalars* sc, hipStream_t st);
 // Ddiag (optional): FP64 diagonal blocks [nb][BD*BD] applied in place of
-// F_val's (zeroed) diagonal blocks.
+// F_val's (zeroed) diagonal blocks. nnz: the number of blocks F_ptr[nb]
+// (selects the launch shape for short or long rows; < 0 = not known, the
+// long-row shape, which is correct for every row length).
 template <typename MT, int BD>
 hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st,
-                               const double* Ddiag = nullptr);
+                               const double* Ddiag = nullptr, long long nnz = -1);
 template <int BD>
 hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
                           PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);

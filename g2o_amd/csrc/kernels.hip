@@ -1412,35 +1412,104 @@ template hipError_t launch_schur_diag<double>(const DevGraph&, const int*, const
 // direction d = s + beta d_old is formed on the fly from s and the previous
 // direction (double-buffered by iteration parity), so no kernel has to wait
 // for a separate direction pass. Dot products are per-workgroup FP64 partials
-// summed with compensation, identically by every wavefront that needs them.
+// that the next kernel sums with compensation, in a fixed order, in
+// wavefront 0 of each of its workgroups (which broadcasts the scalar through
+// LDS). Few, wide workgroups keep that redundant summation small: every
+// workgroup of one kernel reads every partial of the other, so its cost
+// grows with the product of the two grid sizes.
 
-// Compensated sum of part[0..n); every lane of the calling wavefront returns
-// the same value (pairs of lanes combine with the same commutative ops).
-// B loads in flight per lane (one batch covers 64 B entries), summed in
-// index order.
-template <int B = 8>
-__device__ double wave_sum_array(const double* __restrict__ part, int n) {
+// Cross-lane moves within rows of 16 lanes (DPP) and reads of single lanes:
+// the scalar reductions run on the VALU without an LDS round trip.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+// DPP controls: quad_perm [1,0,3,2] and [2,3,0,1] (partners 1 and 2 apart),
+// half-row mirror (the other quad of 8 lanes), row mirror (the other half of
+// a row of 16).
+constexpr int DPP_QUAD_X1 = 0xB1, DPP_QUAD_X2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+
+// Sum over the wavefront with the same bits in every lane: each DPP step
+// adds two equal-shaped partial sums (commutative, so both partners agree),
+// then the four row sums are added in a fixed order.
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_move<DPP_QUAD_X1>(v);
+  v += dpp_move<DPP_QUAD_X2>(v);
+  v += dpp_move<DPP_HALF_MIRROR>(v);
+  v += dpp_move<DPP_ROW_MIRROR>(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
+// (s, c) += (os, oc) with the rounding error of s + os kept in c; symmetric
+// in its two operands, so partner lanes compute the same bits.
+__device__ __forceinline__ void neumaier_combine(double& s, double& c, double os, double oc) {
+  const double t = s + os;
+  const double e = (fabs(s) >= fabs(os)) ? ((s - t) + os) : ((os - t) + s);
+  c = (c + oc) + e;
+  s = t;
+}
+template <int CTRL>
+__device__ __forceinline__ void neumaier_dpp(double& s, double& c) {
+  neumaier_combine(s, c, dpp_move<CTRL>(s), dpp_move<CTRL>(c));
+}
+__device__ __forceinline__ double wave_total_compensated(double s, double c) {
+  neumaier_dpp<DPP_QUAD_X1>(s, c);
+  neumaier_dpp<DPP_QUAD_X2>(s, c);
+  neumaier_dpp<DPP_HALF_MIRROR>(s, c);
+  neumaier_dpp<DPP_ROW_MIRROR>(s, c);
+  double S = lane_value(s, 0), C = lane_value(c, 0);
+  neumaier_combine(S, C, lane_value(s, 16), lane_value(c, 16));
+  neumaier_combine(S, C, lane_value(s, 32), lane_value(c, 32));
+  neumaier_combine(S, C, lane_value(s, 48), lane_value(c, 48));
+  return S + C;
+}
+
+// The other kernel's partials, loaded by the caller before anything that
+// depends on them so that their latency overlaps its other loads: one batch
+// of PCG_PART_BATCH loads per lane covers 64 * PCG_PART_BATCH partials.
+constexpr int PCG_PART_BATCH = 8;
+struct PcgPartials {
+  double v[PCG_PART_BATCH];
+};
+__device__ __forceinline__ PcgPartials pcg_part_load(const double* __restrict__ part, int n) {
+  const int lane = threadIdx.x & 63;
+  PcgPartials p;
+#pragma unroll
+  for (int j = 0; j < PCG_PART_BATCH; ++j) p.v[j] = part[min(lane + 64 * j, n - 1)];
+  return p;
+}
+// Compensated sum of part[0..n), the same value in every lane: lane l takes
+// partials l, l + 64, ... in index order, then the lanes combine.
+__device__ double pcg_part_sum(const PcgPartials& p, const double* __restrict__ part, int n) {
   const int lane = threadIdx.x & 63;
   double s = 0, c = 0;
-  for (int i0 = lane; i0 < n; i0 += 64 * B) {
-    // loads from clamped indices, then a select: a load under a lane
-    // condition becomes a branch that waits for it before the next one
-    double v[B];
 #pragma unroll
-    for (int j = 0; j < B; ++j) v[j] = part[min(i0 + 64 * j, n - 1)];
+  for (int j = 0; j < PCG_PART_BATCH; ++j) neumaier_add(s, c, lane + 64 * j < n ? p.v[j] : 0.0);
+  for (int i0 = lane + 64 * PCG_PART_BATCH; i0 < n; i0 += 64 * PCG_PART_BATCH) {
+    double v[PCG_PART_BATCH];
 #pragma unroll
-    for (int j = 0; j < B; ++j) v[j] = i0 + 64 * j < n ? v[j] : 0.0;
+    for (int j = 0; j < PCG_PART_BATCH; ++j) v[j] = part[min(i0 + 64 * j, n - 1)];
 #pragma unroll
-    for (int j = 0; j < B; ++j) neumaier_add(s, c, v[j]);
+    for (int j = 0; j < PCG_PART_BATCH; ++j) neumaier_add(s, c, i0 + 64 * j < n ? v[j] : 0.0);
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const double os = __shfl_xor(s, off, 64), oc = __shfl_xor(c, off, 64);
-    const double t = s + os;
-    const double e = (fabs(s) >= fabs(os)) ? ((s - t) + os) : ((os - t) + s);
-    c = (c + oc) + e;
-    s = t;
-  }
-  return s + c;
+  return wave_total_compensated(s, c);
+}
+
+// Orders LDS traffic within one wavefront (written by some lanes, read by
+// others of the same wavefront).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ double pcg_dir_val(double s, double d_old, double beta, bool first) {
@@ -1454,9 +1523,25 @@ __device__ __forceinline__ double pcg_dir_val(double s, double d_old, double bet
 // matrix) moves at half width.
 
 // Row-parallel vector kernels (init, update) give each block row 8 lanes
-// (lane k < BD owns component k), 8 rows per 64-lane workgroup; the
-// preconditioner product gathers the row's r components by lane shuffles.
-constexpr int PCG_RPW = 8;  // block rows per 64-lane workgroup
+// (lane k < BD owns component k), PCG_RPW rows per workgroup of 8 * PCG_RPW
+// threads; the preconditioner product gathers the row's r components by lane
+// shuffles. One partial per workgroup.
+constexpr int PCG_RPW = 64;
+constexpr int PCG_ROW_THREADS = 8 * PCG_RPW;
+constexpr int PCG_ROW_WAVES = PCG_ROW_THREADS / 64;
+
+// Sum of one value per wavefront of a PCG_ROW_THREADS workgroup, in
+// wavefront order, by thread 0.
+__device__ __forceinline__ void pcg_block_partial(double wave_value, double* s_w, double* part) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) s_w[wv] = wave_value;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < PCG_ROW_WAVES; ++w) t += s_w[w];
+    part[blockIdx.x] = t;
+  }
+}
 
 // Stores of the solve's progress into host-mapped memory (system scope), so
 // the host can follow the iteration without synchronising the stream.
@@ -1491,13 +1576,15 @@ hipError_t launch_publish_scalars(const double* scal, const int* err, TrialScala
 // x = 0, r = b, s = M^-1 r, part[block] = r . s   (linear_solver_pcg.hpp:120-130);
 // workgroup 0 also resets the solve's scalars.
 template <typename MT, int BD>
-__global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restrict__ b, const MT* __restrict__ J,
-                                                 double* __restrict__ x, double* __restrict__ r,
-                                                 double* __restrict__ s, double* __restrict__ part,
-                                                 PcgScalars* __restrict__ sc) {
-  const int lane = threadIdx.x;
-  if (blockIdx.x == 0 && lane == 0) *sc = PcgScalars{};
-  const int i = blockIdx.x * PCG_RPW + (lane >> 3), k = lane & 7;
+__global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_init(int nb, const double* __restrict__ b,
+                                                              const MT* __restrict__ J, double* __restrict__ x,
+                                                              double* __restrict__ r, double* __restrict__ s,
+                                                              double* __restrict__ part,
+                                                              PcgScalars* __restrict__ sc) {
+  __shared__ double s_w[PCG_ROW_WAVES];
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *sc = PcgScalars{};
+  const int i = blockIdx.x * PCG_RPW + (threadIdx.x >> 3), k = threadIdx.x & 7;
   const bool on = i < nb && k < BD;
   // loads from clamped indices (all in flight together), values selected
   const int ic = on ? i : 0, kc = on ? k : 0;
@@ -1519,30 +1606,29 @@ __global__ __launch_bounds__(64) void k_pcg_init(int nb, const double* __restric
     s[o] = v;
     dot = rv * v;
   }
-  dot = wave_sum(dot);
-  if (lane == 0) part[blockIdx.x] = dot;
+  pcg_block_partial(wave_total(dot), s_w, part);
 }
 
-// Head of iteration `it`: dn = r . s from the previous update's partials; the
-// stop test `dn <= d0` of linear_solver_pcg.hpp:141 (d0 fixed at it == 0 as
-// tol * dn0 floored by the residual memory, or d0_abs when >= 0); then
-// beta = dn / dn_old. Returns false (and records the iteration count) when
-// the solve is over. Every workgroup evaluates this identically; workgroup 0
+// Head of iteration `it`: dn = r . s from the previous update's partials
+// (loaded by the caller); the stop test `dn <= d0` of
+// linear_solver_pcg.hpp:141 (d0 fixed at it == 0 as tol * dn0 floored by the
+// residual memory, or d0_abs when >= 0); then beta = dn / dn_old. Returns
+// false (and records the iteration count) when the solve is over. Run by one
+// wavefront per workgroup, identically in every workgroup; workgroup 0
 // publishes the progress (iteration reached, done) to host-mapped memory.
-__device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalars* sc, PcgProgress* prog,
-                         const int* abort, double tol, double resid_mem, double d0_abs, int it, int last,
-                         double* beta) {
-  // every load of the head is issued before the first branch: the scalars,
-  // the abort word and the partials are one dependency level, not four
+__device__ bool pcg_head(const PcgPartials& pr, const double* __restrict__ part_rs, int n_rs, PcgScalars* sc,
+                         PcgProgress* prog, const int* abort, double tol, double resid_mem, double d0_abs, int it,
+                         int last, double* beta) {
   const int done0 = sc->done;
   const int ab = abort ? *abort : 0;
   const int err0 = sc->err;
   const double d0_prev = sc->d0, dn0_prev = sc->dn0;
   const double dn_old = sc->dn[(it + 1) & 1];
-  const double dn = wave_sum_array<4>(part_rs, n_rs);
+  const double dn = pcg_part_sum(pr, part_rs, n_rs);
   if (done0) return false;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (ab) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (first) {
       sc->done = 1;
       sc->iters = it;
       if (prog) progress_done(prog, it, err0, ab, dn, d0_prev, dn0_prev);
@@ -1560,7 +1646,6 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
   } else {
     d0 = d0_prev;
   }
-  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (first) {
     if (it == 0) { sc->d0 = d0; sc->dn0 = dn; }
     sc->dn[it & 1] = dn;  // the other slot still holds dn_old for this launch
@@ -1581,110 +1666,147 @@ __device__ bool pcg_head(const double* __restrict__ part_rs, int n_rs, PcgScalar
 }
 
 // Explicit operator: d = s + beta d_old for the owned rows and q = A d over
-// the full block rows (both triangles). One workgroup of PCG_SPMV_WAVES
-// wavefronts per block row (the whole grid is resident at once on 256 CUs):
-// its lanes form groups of BD lanes striding over the row's blocks (lane k of
-// a group owns output row k). Each group's first PCG_SPMV_HOIST blocks - the
-// matrix entries and the gathered s, d_old, none of which depend on beta -
-// are loaded before the iteration head, so the dependency chains of the
-// matrix and of the scalars are in flight together.
-// part_dq[row] = d . q over the row.
-constexpr int PCG_SPMV_WAVES = 2;
-constexpr int PCG_SPMV_HOIST = 3;
+// the full block rows (both triangles). PCG_SPMV_ROWS block rows per
+// workgroup, one wavefront per row. A row is taken in chunks of CH blocks:
+// lane t of the wavefront loads the column of block t and gathers s and
+// d_old there once, forms d = s + beta d_old into LDS, and the lanes, as
+// groups of BD (lane k of a group owns output row k), then multiply their
+// blocks' rows by d read back from LDS. The matrix entries of the first
+// chunk and its gathered vectors are loaded before the iteration head (run
+// by wavefront 0, whose partial loads are issued first of all), so the
+// matrix, the gathers and the head's reduction are in flight together.
+// part_dq[workgroup] = d . q over its rows.
+// HS: block slots per group per chunk (CH = min(64, HS * groups)): 7 for BA
+// rows (tens of blocks), 1 for pose-graph rows (a handful).
+constexpr int PCG_SPMV_ROWS = 8;
+constexpr int PCG_SPMV_THREADS = 64 * PCG_SPMV_ROWS;
+template <int BD, int HS>
+struct SpmvShape {
+  static constexpr int GPW = 64 / BD;  // groups per wavefront
+  static constexpr int CH = 64 < GPW * HS ? 64 : GPW * HS;
+};
 // FIRST: iteration 0 (d = s; no d_old, no beta), a separate instantiation so
 // the other iterations carry no run-time branch on it.
 // D64: the diagonal blocks are applied from Ddiag (FP64, BD*BD per row) and
 // F_val's diagonal blocks hold zeros. The FP32 operator keeps its diagonal in
 // FP64 this way: Hpp_ii - sum V Dinv V^T is the largest entry of a row, and
 // rounding it to FP32 is the largest perturbation of the reduced system.
-template <typename MT, int BD, bool FIRST, bool D64>
-__global__ __launch_bounds__(64 * PCG_SPMV_WAVES) void k_pcg_dir_spmv(
+template <typename MT, int BD, bool FIRST, bool D64, int HS>
+__global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_dir_spmv(
     int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
     const double* __restrict__ Ddiag, const double* __restrict__ s, const double* __restrict__ d_old,
     double* __restrict__ d, double* __restrict__ q, const double* __restrict__ part_rs, int n_rs,
     double* __restrict__ part_dq, PcgScalars* sc, PcgProgress* prog, const int* abort, double tol, double resid_mem,
     double d0_abs, int it, int last) {
-  constexpr int GPW = 64 / BD;  // groups per wavefront
-  constexpr int NG = PCG_SPMV_WAVES * GPW;
-  constexpr int H = PCG_SPMV_HOIST;
-  __shared__ double s_part[PCG_SPMV_WAVES][64];
+  constexpr int GPW = SpmvShape<BD, HS>::GPW;
+  constexpr int CH = SpmvShape<BD, HS>::CH;
+  constexpr int R = PCG_SPMV_ROWS;
+  __shared__ double s_dd[R][CH][BD];
+  __shared__ double s_acc[R][64];
+  __shared__ double s_dot[R];
+  __shared__ double s_beta;
+  __shared__ int s_go;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int row = blockIdx.x;
+  PcgPartials pr;
+  if (wv == 0) pr = pcg_part_load(part_rs, n_rs);
+  const int row = blockIdx.x * R + wv;
+  const bool act = row < nb;
+  const int rw = act ? row : 0;
   const int gw = lane / BD, k = lane % BD;
-  const int grp = wv * GPW + gw;
-  // own-row operands of the final stage (lane kk of wavefront 0 owns
-  // component kk), issued with the matrix loads
+  // own-row operands of the final stage (lane kk owns component kk)
   const int kk = lane < BD ? lane : 0;
-  const size_t o_own = BD * static_cast<size_t>(row) + kk;
+  const size_t o_own = BD * static_cast<size_t>(rw) + kk;
   const double s_own = s[o_own];
   const double dold_own = FIRST ? 0.0 : d_old[o_own];
   double Dg[BD];
   if (D64) {
 #pragma unroll
-    for (int c = 0; c < BD; ++c) Dg[c] = Ddiag[static_cast<size_t>(row) * BD * BD + BD * kk + c];
+    for (int c = 0; c < BD; ++c) Dg[c] = Ddiag[static_cast<size_t>(rw) * BD * BD + BD * kk + c];
   }
-  const int p0 = F_ptr[row], p1 = F_ptr[row + 1];
-  MT v[H][BD];
-  double sv[H][BD], dv[H][BD];
-#pragma unroll
-  for (int h = 0; h < H; ++h) {
-    const int p = p0 + grp + h * NG;
-    const bool on = gw < GPW && p < p1;
-    // every load unconditional from a clamped (valid) index, values
-    // selected afterwards, so all of them are in flight together
-    const int pc = on ? p : p0;
-    const int col = F_col[pc];
-    const size_t j = on ? BD * static_cast<size_t>(col) : 0;
-    const MT* V = F_val + static_cast<size_t>(pc) * BD * BD + BD * k;
+  const int p0 = F_ptr[rw], p1 = act ? F_ptr[rw + 1] : p0;
+  const int len = p1 - p0;
+  int nc = min(CH, len);
+  double sg[BD], dg[BD];
+  MT v[HS][BD];
+  // every load unconditional from a clamped (valid) index, values selected
+  // afterwards, so all of them are in flight together
+  auto load_chunk = [&](int c0) {
+    const size_t j = BD * static_cast<size_t>(F_col[p0 + c0 + (lane < nc ? lane : 0)]);
 #pragma unroll
     for (int c = 0; c < BD; ++c) {
-      const MT vv = V[c];
-      v[h][c] = on ? vv : MT(0);
-      sv[h][c] = s[j + c];
-      if (!FIRST) dv[h][c] = d_old[j + c];
+      sg[c] = s[j + c];
+      if (!FIRST) dg[c] = d_old[j + c];
+    }
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+      const int bk = gw + h * GPW;
+      const bool on = gw < GPW && bk < nc;
+      const MT* V = F_val + static_cast<size_t>(p0 + c0 + (on ? bk : 0)) * BD * BD + BD * k;
+#pragma unroll
+      for (int c = 0; c < BD; ++c) {
+        const MT vv = V[c];
+        v[h][c] = on ? vv : MT(0);
+      }
+    }
+  };
+  load_chunk(0);
+  if (wv == 0) {
+    double beta = 0.0;
+    const bool go = pcg_head(pr, part_rs, n_rs, sc, prog, abort, tol, resid_mem, d0_abs, it, last, &beta);
+    if (lane == 0) {
+      s_go = go;
+      s_beta = beta;
     }
   }
-  double beta;
-  if (!pcg_head(part_rs, n_rs, sc, prog, abort, tol, resid_mem, d0_abs, it, last, &beta)) return;
-  double acc = 0;
-#pragma unroll
-  for (int h = 0; h < H; ++h)
-#pragma unroll
-    for (int c = 0; c < BD; ++c)
-      acc += static_cast<double>(v[h][c]) * (FIRST ? sv[h][c] : fma(beta, dv[h][c], sv[h][c]));
-  if (gw < GPW) {
-    // blocks past the hoisted ones (rows longer than H * NG blocks)
-    for (int p = p0 + grp + H * NG; p < p1; p += NG) {
-      const MT* V = F_val + static_cast<size_t>(p) * BD * BD + BD * k;
-      const size_t j = BD * static_cast<size_t>(F_col[p]);
-#pragma unroll
-      for (int c = 0; c < BD; ++c)
-        acc += static_cast<double>(V[c]) * (FIRST ? s[j + c] : fma(beta, d_old[j + c], s[j + c]));
-    }
-  }
-  s_part[wv][lane] = acc;
   __syncthreads();
-  if (threadIdx.x < 64) {
-    double dot = 0.0;
-    const double dd = FIRST ? s_own : fma(beta, dold_own, s_own);
-    double vs = 0;
-    if (lane < BD)
-      for (int w2 = 0; w2 < PCG_SPMV_WAVES; ++w2)
-        for (int g2 = 0; g2 < GPW; ++g2) vs += s_part[w2][g2 * BD + lane];
-    if (D64) {
-      // every lane of the wavefront takes part in the shuffles
-      double dg = 0.0;
+  if (!s_go) return;
+  const double beta = s_beta;
+  double acc = 0.0;
+  for (int c0 = 0; c0 < len; c0 += CH) {
+    if (c0 > 0) {
+      nc = min(CH, len - c0);
+      load_chunk(c0);
+      wave_sync();  // the previous chunk's reads of s_dd are done
+    }
+    if (lane < nc) {
 #pragma unroll
-      for (int c = 0; c < BD; ++c) dg = fma(Dg[c], __shfl(dd, c, 64), dg);
-      vs += dg;
+      for (int c = 0; c < BD; ++c) s_dd[wv][lane][c] = FIRST ? sg[c] : fma(beta, dg[c], sg[c]);
     }
-    if (lane < BD) {
-      q[o_own] = vs;
-      d[o_own] = dd;
-      dot = vs * dd;
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+      const int bk = gw + h * GPW;
+      const int bc = bk < nc ? bk : 0;
+#pragma unroll
+      for (int c = 0; c < BD; ++c) acc = fma(static_cast<double>(v[h][c]), s_dd[wv][bc][c], acc);
     }
-    dot = wave_sum(dot);
-    if (lane == 0) part_dq[row] = dot;
+  }
+  s_acc[wv][lane] = acc;
+  wave_sync();
+  double vs = 0.0;
+  if (lane < BD)
+    for (int g2 = 0; g2 < GPW; ++g2) vs += s_acc[wv][g2 * BD + lane];
+  const double dd = FIRST ? s_own : fma(beta, dold_own, s_own);
+  if (D64) {
+    // every lane of the wavefront takes part in the shuffles
+    double dgv = 0.0;
+#pragma unroll
+    for (int c = 0; c < BD; ++c) dgv = fma(Dg[c], __shfl(dd, c, 64), dgv);
+    vs += dgv;
+  }
+  double dot = 0.0;
+  if (act && lane < BD) {
+    q[o_own] = vs;
+    d[o_own] = dd;
+    dot = vs * dd;
+  }
+  dot = wave_total(dot);
+  if (lane == 0) s_dot[wv] = dot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < R; ++w) t += s_dot[w];
+    part_dq[blockIdx.x] = t;
   }
 }
 
@@ -1695,8 +1817,9 @@ __global__ __launch_bounds__(64) void k_pcg_dir(int nb, const double* __restrict
                                                 const double* __restrict__ part_rs, int n_rs, PcgScalars* sc,
                                                 PcgProgress* prog, const int* abort, double tol, double resid_mem,
                                                 double d0_abs, int it, int last) {
+  const PcgPartials pr = pcg_part_load(part_rs, n_rs);
   double beta;
-  if (!pcg_head(part_rs, n_rs, sc, prog, abort, tol, resid_mem, d0_abs, it, last, &beta)) return;
+  if (!pcg_head(pr, part_rs, n_rs, sc, prog, abort, tol, resid_mem, d0_abs, it, last, &beta)) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= nb) return;
   for (int k = 0; k < BD; ++k) {
@@ -1707,13 +1830,17 @@ __global__ __launch_bounds__(64) void k_pcg_dir(int nb, const double* __restrict
 
 // a = dn / d.q; x += a d; r -= a q; s = M^-1 r; part_rs[block] = r . s
 template <typename MT, int BD>
-__global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict__ J, double* __restrict__ x,
-                                                   double* __restrict__ r, const double* __restrict__ d,
-                                                   const double* __restrict__ q, double* __restrict__ s,
-                                                   const double* __restrict__ part_dq, int n_dq,
-                                                   double* __restrict__ part_rs, PcgScalars* sc, int it) {
-  const int lane = threadIdx.x;
-  const int i = blockIdx.x * PCG_RPW + (lane >> 3), k = lane & 7;
+__global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_update(
+    int nb, const MT* __restrict__ J, double* __restrict__ x, double* __restrict__ r, const double* __restrict__ d,
+    const double* __restrict__ q, double* __restrict__ s, const double* __restrict__ part_dq, int n_dq,
+    double* __restrict__ part_rs, PcgScalars* sc, int it) {
+  __shared__ double s_w[PCG_ROW_WAVES];
+  __shared__ double s_a;
+  __shared__ int s_go;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  PcgPartials pr;
+  if (wv == 0) pr = pcg_part_load(part_dq, n_dq);
+  const int i = blockIdx.x * PCG_RPW + (threadIdx.x >> 3), k = threadIdx.x & 7;
   const bool on = i < nb && k < BD;
   // operands first (independent of alpha), then the reduction of d . q;
   // loads from clamped indices so they are all in flight together
@@ -1723,15 +1850,22 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
   double jr[BD];
 #pragma unroll
   for (int c = 0; c < BD; ++c) jr[c] = static_cast<double>(J[static_cast<size_t>(ic) * BD * BD + BD * kc + c]);
-  const int done0 = sc->done;
-  const double dn = sc->dn[it & 1];
-  const double dq = wave_sum_array<24>(part_dq, n_dq);  // one batch up to 1536 rows
-  if (done0) return;
-  const double a = dn / dq;
-  // d . q <= 0: the operator is not positive definite along d (reported in
-  // sc->err for callers that need it; the iteration itself follows
-  // linear_solver_pcg.hpp, which does not check)
-  if (blockIdx.x == 0 && threadIdx.x == 0 && !(dq > 0.0)) sc->err = 1;
+  if (wv == 0) {
+    const int done0 = sc->done;
+    const double dn = sc->dn[it & 1];
+    const double dq = pcg_part_sum(pr, part_dq, n_dq);
+    // d . q <= 0: the operator is not positive definite along d (reported in
+    // sc->err for callers that need it; the iteration itself follows
+    // linear_solver_pcg.hpp, which does not check)
+    if (!done0 && blockIdx.x == 0 && lane == 0 && !(dq > 0.0)) sc->err = 1;
+    if (lane == 0) {
+      s_go = !done0;
+      s_a = dn / dq;
+    }
+  }
+  __syncthreads();
+  if (!s_go) return;
+  const double a = s_a;
   double rv = 0.0;
   if (on) {
     x[o] = fma(a, dv, xv);
@@ -1746,38 +1880,49 @@ __global__ __launch_bounds__(64) void k_pcg_update(int nb, const MT* __restrict_
     s[o] = v;
     dot = rv * v;
   }
-  dot = wave_sum(dot);
-  if (lane == 0) part_rs[blockIdx.x] = dot;
+  pcg_block_partial(wave_total(dot), s_w, part_rs);
 }
 
 int pcg_row_blocks(int nb) { return nblocks(nb, PCG_RPW); }
-int pcg_spmv_blocks(int nb) { return nb; }
+int pcg_spmv_blocks(int nb) { return nblocks(nb, PCG_SPMV_ROWS); }
 int implicit_reduce_blocks(int Cf) { return nblocks(Cf, 64); }
 
 template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
                            double* part_rs, PcgScalars* sc, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(64), 0, st, nb, b, Jinv, x, r, s, part_rs,
-                     sc);
+  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(PCG_ROW_THREADS), 0, st, nb, b, Jinv, x,
+                     r, s, part_rs, sc);
   return hipGetLastError();
 }
+
+// Rows averaging fewer blocks than this take the one-slot shape (HS = 1).
+constexpr int PCG_SHORT_ROW = 16;
 
 template <typename MT, int BD>
 hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* s,
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st,
-                               const double* Ddiag) {
-#define G2OHIP_SPMV_LAUNCH(FIRST, D64)                                                                            \
-  hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, FIRST, D64>), dim3(pcg_spmv_blocks(nb)), dim3(64 * PCG_SPMV_WAVES), 0, \
-                     st, nb, F_ptr, F_col, F_val, Ddiag, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, sc,   \
-                     ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last)
-  if (Ddiag) {
-    if (it == 0) G2OHIP_SPMV_LAUNCH(true, true);
-    else G2OHIP_SPMV_LAUNCH(false, true);
-  } else {
-    if (it == 0) G2OHIP_SPMV_LAUNCH(true, false);
-    else G2OHIP_SPMV_LAUNCH(false, false);
+                               const double* Ddiag, long long nnz) {
+  constexpr int HS_LONG = (64 + 64 / BD - 1) / (64 / BD);
+  const bool short_rows = nnz >= 0 && nnz < static_cast<long long>(PCG_SHORT_ROW) * nb;
+#define G2OHIP_SPMV_LAUNCH(FIRST, D64, HS)                                                                        \
+  hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, FIRST, D64, HS>), dim3(pcg_spmv_blocks(nb)), dim3(PCG_SPMV_THREADS), \
+                     0, st, nb, F_ptr, F_col, F_val, Ddiag, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, \
+                     sc, ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last)
+#define G2OHIP_SPMV_SHAPE(HS)                         \
+  if (Ddiag) {                                        \
+    if (it == 0) G2OHIP_SPMV_LAUNCH(true, true, HS);  \
+    else G2OHIP_SPMV_LAUNCH(false, true, HS);         \
+  } else {                                            \
+    if (it == 0) G2OHIP_SPMV_LAUNCH(true, false, HS); \
+    else G2OHIP_SPMV_LAUNCH(false, false, HS);        \
   }
+  if (short_rows) {
+    G2OHIP_SPMV_SHAPE(1)
+  } else {
+    G2OHIP_SPMV_SHAPE(HS_LONG)
+  }
+#undef G2OHIP_SPMV_SHAPE
 #undef G2OHIP_SPMV_LAUNCH
   return hipGetLastError();
 }
@@ -1796,8 +1941,8 @@ template <typename MT, int BD>
 hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const double* d, const double* q,
                              double* s, const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_update<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(64), 0, st, nb, Jinv, x, r, d, q, s,
-                     part_dq, n_dq, part_rs, sc, it);
+  hipLaunchKernelGGL((k_pcg_update<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(PCG_ROW_THREADS), 0, st, nb, Jinv, x,
+                     r, d, q, s, part_dq, n_dq, part_rs, sc, it);
   return hipGetLastError();
 }
 
@@ -1808,7 +1953,7 @@ hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const
   template hipError_t launch_pcg_dir_spmv<MT, BD>(int, const int*, const int*, const MT*, const double*,         \
                                                   const double*, double*, double*, const double*, double*,       \
                                                   PcgScalars*, const PcgControl&, int, int, hipStream_t,         \
-                                                  const double*);                                                 \
+                                                  const double*, long long);                                      \
   template hipError_t launch_pcg_update<MT, BD>(int, const MT*, double*, double*, const double*, const double*,  \
                                                 double*, const double*, int, double*, PcgScalars*, int,          \
                                                 hipStream_t);

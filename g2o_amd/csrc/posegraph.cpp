@@ -175,7 +175,8 @@ int g2ohip_ctx::pg_solve(int* ok) {
   double* const dirs[2] = {B.d.p, B.d2.p};
   auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
     return launch_pcg_dir_spmv<double, 6>(nb, d_pg_F_ptr.p, d_pg_F_col.p, d_pg_F_val.p, B.s.p, d_old, d, B.q.p,
-                                          d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream);
+                                          d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream, nullptr,
+                                          pgst.nF);
   };
   PcgRun pr;
   {
@@ -217,7 +218,8 @@ int g2ohip_ctx::pg_marginals(int nblk, const int* rows, const int* cols, double*
   double* const dirs[2] = {B.d.p, B.d2.p};
   auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
     return launch_pcg_dir_spmv<double, 6>(nb, d_pg_F_ptr.p, d_pg_F_col.p, d_pg_F_val.p, B.s.p, d_old, d, B.q.p,
-                                          d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream);
+                                          d_dn_part.p, d_dq_part.p, d_sc.p, ctl, it, last, stream, nullptr,
+                                          pgst.nF);
   };
   g2ohip::DBuf<double> rhs;
   HK(rhs.alloc(n));
