@@ -504,6 +504,7 @@ struct BA {
   vector<double> bschur, Dinv, x, coeff;
   double lambda = 0;
   double user_lambda = 0;  // "initialLambda" property (<= 0: tau * max diag)
+  std::vector<double> trials;  // per LM trial of the last optimize: LM_TRIAL_FIELDS values
   PCGState pcg;
   int solver_kind = 0;  // 0 Cholesky, 1 PCG
   double times[6] = {0, 0, 0, 0, 0, 0};
@@ -1042,6 +1043,7 @@ struct PG {
   int solver_kind = 1;
   vector<vector<double>> stackT;
   vector<vector<Quat>> stackQ;
+  vector<double> trials;  // per LM trial of the last optimize: LM_TRIAL_FIELDS values
 };
 
 // error e[6] and, if J != null, J[6*12] = d e / d [d_i(6) d_j(6)] (row-major)
@@ -1217,6 +1219,18 @@ void pg_update(PG& p, const double* x) {
   }
 }
 
+// Record of one LM trial (a pass of the do-loop of
+// optimization_algorithm_levenberg.cpp:97-144): lambda used, currentChi,
+// tempChi, the divisor of rho (computeScale() + 1e-3, or 1 after a failed
+// solve), rho, accepted (1/0).
+constexpr int LM_TRIAL_FIELDS = 6;
+inline void log_trial(std::vector<double>* log, double lambda, double chi, double temp, double scale, double rho,
+                      bool accepted) {
+  if (!log) return;
+  const double rec[LM_TRIAL_FIELDS] = {lambda, chi, temp, scale, rho, accepted ? 1.0 : 0.0};
+  log->insert(log->end(), rec, rec + LM_TRIAL_FIELDS);
+}
+
 // OptimizationAlgorithmLevenberg::solve (optimization_algorithm_levenberg.cpp:
 // 60-153) over a problem P providing chi2(), build() (linearise at the
 // state), lambda_init(), solve(lambda, &pcg_iters) -> ok, update(), scale()
@@ -1224,7 +1238,8 @@ void pg_update(PG& p, const double* x) {
 // trace[6 * it + (0..5)] = chi2 before, chi2 after, lambda, trials, PCG
 // iterations, status (1 = Terminate). Returns the iterations run.
 template <class P>
-int lm_optimize(P& p, int iters, double* trace, double* times) {
+int lm_optimize(P& p, int iters, double* trace, double* times, std::vector<double>* log = nullptr) {
+  if (log) log->clear();
   for (int k = 0; k < 6; ++k) p.times[k] = 0;
   double tall = now_s();
   double lambda = 0, ni = 2;
@@ -1257,6 +1272,7 @@ int lm_optimize(P& p, int iters, double* trace, double* times) {
       rho = currentChi - tempChi;
       double scale = ok ? p.scale(lambda) + 1e-3 : 1;
       rho /= scale;
+      log_trial(log, lambda, currentChi, tempChi, scale, rho, rho > 0 && std::isfinite(tempChi) && ok);
       if (rho > 0 && std::isfinite(tempChi) && ok) {
         double alpha = 1. - std::pow((2 * rho - 1), 3);
         alpha = std::min(alpha, 2. / 3.);
@@ -1631,6 +1647,7 @@ int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
   p->pcg.tol = tol;
   p->pcg.max_iter = maxiter;
   for (double& t : p->times) t = 0;
+  p->trials.clear();
   double tall = now_s();
   double lambda = 0, ni = 2;
   int it;
@@ -1662,6 +1679,7 @@ int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
       rho = currentChi - tempChi;
       double scale = ok ? ba_scale(*p) + 1e-3 : 1;
       rho /= scale;
+      log_trial(&p->trials, lambda, currentChi, tempChi, scale, rho, rho > 0 && std::isfinite(tempChi) && ok);
       if (rho > 0 && std::isfinite(tempChi) && ok) {
         double alpha = 1. - std::pow((2 * rho - 1), 3);
         alpha = std::min(alpha, 2. / 3.);
@@ -1696,6 +1714,17 @@ int orc_ba_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
 
 // OptimizationAlgorithmLevenberg's "initialLambda" property (<= 0: computeLambdaInit).
 void orc_ba_set_user_lambda(void* h, double v) { static_cast<BA*>(h)->user_lambda = v; }
+
+// The LM trials of the last orc_ba_optimize / orc_pg_optimize: up to `max`
+// records of LM_TRIAL_FIELDS doubles (see log_trial). Returns the count.
+static int copy_trials(const std::vector<double>& t, double* out, int max) {
+  const int n = static_cast<int>(t.size()) / LM_TRIAL_FIELDS;
+  const int m = std::min(n, max);
+  if (out) std::copy(t.begin(), t.begin() + static_cast<size_t>(m) * LM_TRIAL_FIELDS, out);
+  return n;
+}
+int orc_ba_trials(void* h, double* out, int max) { return copy_trials(static_cast<BA*>(h)->trials, out, max); }
+int orc_pg_trials(void* h, double* out, int max) { return copy_trials(static_cast<PG*>(h)->trials, out, max); }
 
 int orc_ba_gauss_newton(void* h, int iters, int solver, double tol, int maxiter, double* trace) {
   BA* p = static_cast<BA*>(h);
@@ -1941,7 +1970,7 @@ int orc_pg_optimize(void* h, int iters, int solver, double tol, int maxiter, dou
   p->pcg.tol = tol;
   p->pcg.max_iter = maxiter;
   PGLM lm{*p};
-  return lm_optimize(lm, iters, trace, times);
+  return lm_optimize(lm, iters, trace, times, &p->trials);
 }
 
 int orc_pg_gauss_newton(void* h, int iters, int solver, double tol, int maxiter, double* trace) {

@@ -70,6 +70,10 @@ def lib():
         L.orc_ba_set_meas.argtypes = [ct.c_void_p, _dp, _dp]
         L.orc_ba_set_info.argtypes = [ct.c_void_p, _dp]
         L.orc_ba_set_user_lambda.argtypes = [ct.c_void_p, ct.c_double]
+        L.orc_ba_trials.restype = ct.c_int
+        L.orc_ba_trials.argtypes = [ct.c_void_p, _dp, ct.c_int]
+        L.orc_pg_trials.restype = ct.c_int
+        L.orc_pg_trials.argtypes = [ct.c_void_p, _dp, ct.c_int]
         L.orc_ba_adjoint.restype = ct.c_int
         L.orc_ba_adjoint.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.orc_ba_newton.restype = ct.c_int
@@ -118,6 +122,13 @@ def _i(a):
 def _u(a):
     a = np.ascontiguousarray(a, dtype=np.uint8)
     return a, a.ctypes.data_as(_up)
+
+
+def _trials(fn, h):
+    n = fn(h, None, 0)
+    out = np.zeros((max(n, 1), 6))
+    fn(h, out.ctypes.data_as(_dp), n)
+    return out[:n]
 
 
 class OracleBA:
@@ -214,6 +225,11 @@ class OracleBA:
     def set_user_lambda(self, lam: float):
         lib().orc_ba_set_user_lambda(self.h, float(lam))
 
+    def trials(self):
+        """LM trials of the last optimize(): rows of (lambda used, currentChi,
+        tempChi, rho divisor, rho, accepted)."""
+        return _trials(lib().orc_ba_trials, self.h)
+
     def set_meas(self, meas, weight):
         m, pm = _d(meas)
         w, pw = _d(weight)
@@ -300,6 +316,10 @@ class OraclePG:
         n = lib().orc_pg_optimize(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp),
                                   times.ctypes.data_as(_dp))
         return n, trace[:n], times
+
+    def trials(self):
+        """LM trials of the last optimize() (columns as OracleBA.trials)."""
+        return _trials(lib().orc_pg_trials, self.h)
 
     def gauss_newton(self, iters: int, solver: int = 1, tol: float = 1e-8, maxiter: int = -1):
         """Rows: chi2 before, chi2 after, pcg iterations, status (1 = Fail)."""

@@ -389,3 +389,109 @@ def test_dogleg_oracle(O):
     assert n == 6
     # the pose-graph GN steps stay inside the radius: Dogleg is Gauss-Newton here
     np.testing.assert_allclose(tp[:, 1], tg[:, 1], rtol=1e-10)
+
+
+# ------------------------------------------------ LM control (lambda, rho, nu)
+def _scrambled_sphere(seed=23):
+    """60 poses on 3 sphere laps whose initial positions and rotations are
+    scrambled (1.5 m, 1.5 rad): the damped steps overshoot on SO(3), so the LM
+    rejects trials with rho well below zero (several in a row in one
+    iteration for this seed)."""
+    pg = datasets.make_sphere_pose_graph(seed=1001, nodes_per_level=20, laps=3)
+    rng = np.random.default_rng(seed)
+    P = datasets.perturb_poses(rng, pg.poses, 1.5, 1.5)
+    P[0] = pg.poses[0]
+    pg.poses = P
+    return pg
+
+
+def _split_trials(trace, trials):
+    """Trial rows of each LM iteration (trace column 3 = trials)."""
+    out, k = [], 0
+    for q in trace[:, 3].astype(int):
+        out.append(trials[k:k + q])
+        k += q
+    assert k == trials.shape[0]
+    return out
+
+
+def test_lm_rho_is_one_for_small_steps(O):
+    """computeScale (optimization_algorithm_levenberg.cpp:169-176) is the
+    decrease the quadratic model predicts: with (H + lambda I) x = b,
+    x . (lambda x + b) = 2 x.b - x.H.x. Over a step short enough that the
+    error is linear in it, the actual chi2 decrease equals that, so
+    rho = (chi2 - chi2_new) / (scale + 1e-3) -> 1 as lambda grows, and
+    rho > 0.94 makes the update factor the lower crop 1/3
+    (levenberg.cpp:125-133). A scale without the lambda term gives rho -> 2,
+    one of the wrong sign a rejection."""
+    prob = datasets.make_small_ba(seed=1000)
+    _, Hpp, Hll, _, _ = O.OracleBA(prob).linearize()
+    md = max(np.abs(np.einsum("cii->ci", Hpp)).max(), np.abs(np.einsum("pii->pi", Hll)).max())
+    for f in (1e2, 1e4, 1e6):
+        ora = O.OracleBA(prob)
+        ora.set_user_lambda(f * md)
+        _, tr, _ = ora.optimize(1, solver=0)
+        t = ora.trials()
+        assert t.shape[0] == 1 and t[0, 5] == 1 and t[0, 0] == f * md
+        # rho from the chi2 values themselves, and the 1/3 crop
+        assert t[0, 4] == pytest.approx((t[0, 1] - t[0, 2]) / t[0, 3], rel=1e-12)
+        assert abs(t[0, 4] - 1.0) < 1e-3
+        assert tr[0, 2] == f * md * (1.0 / 3.0)
+    # the scale divisor equals x . (lambda x + b) of the step the solver
+    # returns for this lambda and linearisation
+    ora = O.OracleBA(prob)
+    _, _, _, _, b = ora.linearize()
+    lam = 1e3 * md
+    _, dx, _ = ora.solve(lam, solver=0)
+    ora.set_user_lambda(lam)
+    ora.optimize(1, solver=0)
+    assert ora.trials()[0, 3] == pytest.approx(dx @ (lam * dx + b) + 1e-3, rel=1e-9)
+
+
+def test_lm_lambda_schedule_with_rejected_trials(O):
+    """OptimizationAlgorithmLevenberg::solve (levenberg.cpp:97-144) on a pose
+    graph with genuinely rejected trials: computeLambdaInit is 1e-5 times
+    the largest diagonal entry of the vertex Hessians; a rejected trial
+    multiplies lambda by nu and doubles nu; an accepted one multiplies lambda
+    by max(1/3, min(2/3, 1 - (2 rho - 1)^3)) and resets nu to 2 - so rho <=
+    1/2 gives exactly 2/3, rho >= 0.94 exactly 1/3, and in between the factor
+    falls with rho; each iteration starts from the previous accepted state."""
+    saved = O.lib().orc_num_threads()
+    O.lib().orc_set_num_threads(4)
+    try:
+        pg = _scrambled_sphere()
+        ora = O.OraclePG(pg)
+        _, rows, cols, blocks, _ = ora.linearize()
+        lam0 = 1e-5 * max(np.abs(np.diag(B)).max() for r, c, B in zip(rows, cols, blocks) if r == c)
+        n, tr, _ = ora.optimize(12, solver=0)
+        t = ora.trials()
+    finally:
+        O.lib().orc_set_num_threads(saved)
+    assert n == 12 and t[0, 0] == pytest.approx(lam0, rel=1e-13)
+    its = _split_trials(tr, t)
+    assert max(len(x) for x in its) >= 3 and (t[:, 4] < -1.0).any()
+    sf = []
+    for i, rec in enumerate(its):
+        assert rec[0, 1] == tr[i, 0]
+        nu = 2.0
+        for j in range(len(rec) - 1):
+            assert rec[j, 5] == 0 and rec[j, 4] <= 0
+            assert rec[j + 1, 0] == rec[j, 0] * nu
+            nu *= 2.0
+        acc = rec[-1]
+        assert acc[5] == 1 and acc[4] > 0 and tr[i, 1] == acc[2]
+        f = tr[i, 2] / acc[0]
+        if acc[4] <= 0.5:
+            assert tr[i, 2] == acc[0] * (2.0 / 3.0)
+        elif acc[4] >= 0.94:
+            assert tr[i, 2] == acc[0] * (1.0 / 3.0)
+        else:
+            assert 1.0 / 3.0 <= f <= 2.0 / 3.0
+        sf.append((acc[4], f))
+        if i + 1 < len(its):
+            assert its[i + 1][0, 0] == tr[i, 2]
+            # chi2 of the same state evaluated again (the oracle's OpenMP
+            # sums may round differently from one evaluation to the next)
+            assert its[i + 1][0, 1] == pytest.approx(acc[2], rel=1e-13)
+    sf.sort()
+    assert all(b[1] <= a[1] * (1 + 1e-12) for a, b in zip(sf, sf[1:]))

@@ -83,6 +83,31 @@ def test_lm_trace_matches_oracle():
     assert abs(stats[-1]["chi2"] - trace[-1, 1]) <= 1e-6 * trace[-1, 1]
 
 
+def test_lm_with_rejected_trials_matches_oracle():
+    """A 60-pose sphere whose initial poses are scrambled (1.5 m, 1.5 rad):
+    the LM rejects trials with rho well below zero, up to four in a row in
+    one iteration. The device takes the same decisions as the oracle: the
+    same trials per iteration and the same lambda after each (nu doubling on
+    rejection, reset on acceptance, levenberg.cpp:125-141)."""
+    pg = datasets.make_sphere_pose_graph(seed=1001, nodes_per_level=20, laps=3)
+    rng = np.random.default_rng(23)
+    P = datasets.perturb_poses(rng, pg.poses, 1.5, 1.5)
+    P[0] = pg.poses[0]
+    pg.poses = P
+    # PCG run to 1e-20 on r.M^-1 r: the oracle's steps then agree with its
+    # dense Cholesky solve to ~1e-9, so the two paths do not drift apart
+    ora = _oracle(pg)
+    n, trace, _ = ora.optimize(12, solver=1, tol=1e-20)
+    assert trace[:, 3].max() >= 3
+    pgo = g2o_amd.PoseGraphOptimizer(pg, device=0, pcg_tolerance=1e-20)
+    stats = pgo.optimize(12)
+    assert len(stats) == n
+    for s, t in zip(stats, trace):
+        assert s["levenberg_iterations"] == int(t[3])
+        assert abs(s["lambda_"] - t[2]) <= 1e-6 * t[2]
+        assert abs(s["chi2"] - t[1]) <= 1e-5 * t[1]
+
+
 @pytest.mark.parametrize("which", ["translation", "rotation"])
 def test_two_vertex_lm_like_reference(which):
     """optimization_slam3d.cpp:62-138 on the device."""
