@@ -1,6 +1,7 @@
 // extern "C" entry points of include/g2o_hip.h. Each wraps one context
 // operation; errors become status codes plus the context's last-error text.
 #include <cstdlib>
+#include <cstring>
 #include <new>
 #include <string>
 
@@ -69,6 +70,10 @@ int g2ohip_create(int device, g2ohip_ctx** out) try {
     // without them, or with G2OHIP_HOST_MAPPED=0 in the environment, the host
     // copies and synchronises the stream)
     static_assert(sizeof(g2ohip::PcgProgress) <= 128 && sizeof(g2ohip::TrialScalars) <= 128, "mapped layout");
+    if (const char* sp = std::getenv("G2OHIP_SCHUR_PATH")) {
+      if (std::strcmp(sp, "windows") == 0) c->schur_path = 1;
+      else if (std::strcmp(sp, "rows") == 0) c->schur_path = 2;
+    }
     const char* mapped_env = std::getenv("G2OHIP_HOST_MAPPED");
     const bool mapped = !(mapped_env && mapped_env[0] == '0');
     void* hp = nullptr;
@@ -132,8 +137,11 @@ int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
   return guarded(ctx, [&]() -> int {
     if (!ctx->have_structure || !info) return ctx->fail(G2OHIP_ERR_INVALID, "no structure");
     if (ctx->mode != 0) return ctx->fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
-    int rc = ctx->ensure_win(0);
-    if (rc) return rc;
+    const bool rows = ctx->use_schur_rows();
+    if (!rows) {
+      const int rc = ctx->ensure_win(0);
+      if (rc) return rc;
+    }
     const auto& st = ctx->st;
     int maxn = 0;
     for (int i = 0; i < st.npos; ++i) maxn = std::max(maxn, st.pos_ptr[i + 1] - st.pos_ptr[i]);
@@ -143,9 +151,9 @@ int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
     info[3] = st.nS;
     info[4] = st.nlc;
     const auto& wp = st.win[0];
-    info[5] = static_cast<long long>(wp.win_obs0.size()) - 1;
-    info[6] = wp.pairs;
-    info[7] = wp.od_ptr.back();
+    info[5] = rows ? 0 : static_cast<long long>(wp.win_obs0.size()) - 1;
+    info[6] = g2ohip::schur_pair_count(st);
+    info[7] = rows ? 0 : wp.od_ptr.back();
     info[8] = maxn;
     info[9] = st.nF;
     info[10] = static_cast<long long>(st.lc_cams.size());

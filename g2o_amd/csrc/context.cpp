@@ -175,6 +175,8 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   mode = 0;
   have_meas = have_state = have_system = have_solution = have_exact = have_lin = false;
   win_ready[0] = win_ready[1] = false;
+  schur_rows_chosen = -1;
+  row_nitems = row_max_blk = 0;
   pcg_residual_mem = -1.0;
   stack_depth = 0;
   if (device < 0) return G2OHIP_OK;
@@ -294,6 +296,72 @@ int g2ohip_ctx::ensure_win(int which) {
   win_ready[which] = true;
   return G2OHIP_OK;
 }
+
+// Window plan or camera rows for the off-diagonal Schur blocks. The window
+// plan stages one partial block per (window, block) in HBM: cheap when
+// tracks are short (a window's pairs hit few blocks), but a track of n
+// cameras adds n (n - 1) / 2 partials of its own, and its host-built pair
+// list grows the same way. The row kernel stages nothing. Rows are chosen
+// when a point has more free observations than a window holds, or when
+// tracks longer than SCHUR_LONG_TRACK cameras make most of the pair
+// products.
+bool g2ohip_ctx::use_schur_rows() {
+  if (schur_path == 1) return false;
+  if (schur_path == 2) return true;
+  if (schur_rows_chosen < 0) {
+    long long pairs = 0, long_pairs = 0;
+    for (int p = 0; p < st.npos; ++p) {
+      const long long n = st.pos_h[p] >= 0 ? st.pos_nfree[p] : 0;
+      pairs += n * (n - 1) / 2;
+      if (n > SCHUR_LONG_TRACK) long_pairs += n * (n - 1) / 2;
+    }
+    schur_rows_chosen = (max_point_obs(st) > SCHUR_WIN_OBS32 || 2 * long_pairs > pairs) ? 1 : 0;
+  }
+  return schur_rows_chosen == 1;
+}
+
+// Row items of the camera-row Schur kernel: each free camera's off-diagonal
+// upper blocks in runs of at most SCHUR_ROW_CAP.
+int g2ohip_ctx::ensure_rows() {
+  if (row_nitems > 0 || st.nS == st.Cf) return G2OHIP_OK;
+  std::vector<int4> items;
+  int mx = 0;
+  for (int a = 0; a < st.Cf; ++a) {
+    const int b0 = st.S_rowptr[a] + 1, b1 = st.S_rowptr[a + 1];
+    for (int k = b0; k < b1; k += SCHUR_ROW_CAP) {
+      const int e = std::min(b1, k + SCHUR_ROW_CAP);
+      items.push_back(make_int4(a, k, e, (k == b0 && e == b1) ? 1 : 0));
+      mx = std::max(mx, e - k);
+    }
+  }
+  HK(d_row_items.upload(items, stream));
+  HK(hipStreamSynchronize(stream));
+  row_nitems = static_cast<int>(items.size());
+  row_max_blk = mx;
+  return G2OHIP_OK;
+}
+
+template <typename HT>
+int g2ohip_ctx::schur_offdiag(PrecBufs<HT>& B) {
+  const int which = std::is_same<HT, float>::value ? 0 : 1;
+  if (use_schur_rows()) {
+    CK(ensure_rows());
+    ProfScope pr(this, G2OHIP_K_SCHUR_WINDOWS);
+    HK(launch_schur_rows<HT>(g, d_row_items.p, row_nitems, row_max_blk, d_q_slot.p, B.Hpl.p, d_linv.p, B.F_val.p,
+                             stream));
+    return G2OHIP_OK;
+  }
+  CK(ensure_win(which));
+  HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win[which].od_ptr.back(), 1))));
+  {
+    ProfScope pr(this, G2OHIP_K_SCHUR_WINDOWS);
+    HK(launch_schur_win<HT>(g, dwins[which], B.Hpl.p, d_linv.p, B.part.p, stream));
+  }
+  HK(launch_schur_offdiag<HT>(g, dwins[which], B.part.p, B.F_val.p, stream));
+  return G2OHIP_OK;
+}
+template int g2ohip_ctx::schur_offdiag<float>(PrecBufs<float>&);
+template int g2ohip_ctx::schur_offdiag<double>(PrecBufs<double>&);
 
 int g2ohip_ctx::set_measurements(const double* meas, const double* w, const double* intr, double huber) {
   if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
@@ -523,8 +591,6 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   // pass left the table at the trial state)
   HK(launch_cam_table(g, d_cams.p, d_camRt.p, stream, d_err.p));  // and d_err[0..3] = 0
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
-  const int wpi = std::is_same<HT, float>::value ? 0 : 1;
-  if (!implicit) CK(ensure_win(wpi));
   {
     ProfScope ps(this, G2OHIP_K_SCHUR);
     if (implicit) {
@@ -532,14 +598,9 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
       HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
     } else {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
-      HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win[wpi].od_ptr.back(), 1))));
       HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
       HK(launch_point_linv(g, d_Hll.p, lambda, d_linv.p, d_err.p, stream));
-      {
-        ProfScope pr(this, G2OHIP_K_SCHUR_WINDOWS);
-        HK(launch_schur_win<HT>(g, dwins[wpi], B.Hpl.p, d_linv.p, B.part.p, stream));
-      }
-      HK(launch_schur_offdiag<HT>(g, dwins[wpi], B.part.p, B.F_val.p, stream));
+      CK(schur_offdiag<HT>(B));
     }
     // diagonal blocks, block-Jacobi inverses and (explicit) the reduced
     // right-hand side in FP64, re-linearised from the state point-major (FP32
@@ -1254,15 +1315,12 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
   *iters = 0;
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
-  CK(ensure_win(1));
   HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
   HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
-  HK(B.part.alloc(36 * static_cast<size_t>(std::max(st.win[1].od_ptr.back(), 1))));
   HK(launch_point_dinv<double>(g, d_Hll.p, lam, B.Dinv.p, d_err.p, stream));
   HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
   HK(launch_point_linv(g, d_Hll.p, lam, d_linv.p, d_err.p, stream));
-  HK(launch_schur_win<double>(g, dwins[1], B.Hpl.p, d_linv.p, B.part.p, stream));
-  HK(launch_schur_offdiag<double>(g, dwins[1], B.part.p, B.F_val.p, stream));
+  CK(schur_offdiag<double>(B));
   HK(launch_schur_diag<double>(g, d_q_slot.p, d_camRt.p, d_pts.p, B.Hpl.p, d_Hll.p, d_Hpp.p, d_b.p, lam, B.F_val.p,
                                B.Jinv.p, d_bschur.p, d_err.p, stream));
   int err = 0;

@@ -113,6 +113,13 @@ struct g2ohip_ctx {
   g2ohip::WinPlanBufs win_bufs[2];
   g2ohip::DevWinPlan dwins[2]{};
   bool win_ready[2] = {false, false};
+  // off-diagonal Schur blocks: 0 = choose per structure, 1 = window plan
+  // (partials staged in HBM), 2 = camera rows (G2OHIP_SCHUR_PATH in the
+  // environment at context creation: "windows" or "rows")
+  int schur_path = 0;
+  int schur_rows_chosen = -1;  // the choice made for the current structure
+  g2ohip::DBuf<int4> d_row_items;
+  int row_nitems = 0, row_max_blk = 0;
 
   g2ohip::DBuf<double> d_cams, d_pts, d_camRt;
   std::vector<g2ohip::DBuf<double>*> stack_c, stack_p;
@@ -174,6 +181,12 @@ struct g2ohip_ctx {
   int set_information(const double* omega);
   int get_state(double* cams, double* pts);
   int ensure_win(int which);
+  bool use_schur_rows();
+  int ensure_rows();
+  // off-diagonal blocks of the reduced system into B.F_val (the Hll^-1
+  // factors d_linv must be current)
+  template <typename HT>
+  int schur_offdiag(g2ohip::PrecBufs<HT>& B);
   int compute_active_errors(double* chi2, double* err_out = nullptr);
   int build_system(bool read_chi = true);
   int solve(int* ok, bool defer_err = false);

@@ -11,21 +11,22 @@ constexpr int LIN_PTS = 256;   // points per linearisation chunk
 // chunk. lds_fits: the kernel's static LDS plus `dynamic` bytes stay within
 // what one workgroup may allocate on the current device; the launchers
 // return hipErrorInvalidValue instead of launching above it. The kernel's
-// static size and the device limit are looked up once per process.
+// static size is queried at each call (kernels of one signature share the
+// template instance, so it is not cached); the device limit is the same for
+// every CDNA4 device and looked up once.
 template <typename Kernel>
 inline bool lds_fits(Kernel kernel, size_t dynamic) {
-  static size_t static_bytes = 0;
   static int limit = -1;
+  hipFuncAttributes a{};
+  if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) != hipSuccess) return false;
   if (limit < 0) {
-    hipFuncAttributes a{};
     int dev = 0, lim = 0;
-    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+    if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       return false;
-    static_bytes = a.sharedSizeBytes;
     limit = lim;
   }
-  return static_bytes + dynamic <= static_cast<size_t>(limit);
+  return a.sharedSizeBytes + dynamic <= static_cast<size_t>(limit);
 }
 constexpr int RED_BLOCKS = 512;  // fixed grid of the grid-stride reductions
 // fixed grid of the chi2 pass: each thread's slots form a chain of dependent
@@ -155,6 +156,14 @@ hipError_t launch_schur_win(const DevGraph& g, const DevWinPlan& wp, const HT* H
 // writes -sum and its transpose into both triangles of F_val.
 template <typename HT>
 hipError_t launch_schur_offdiag(const DevGraph& g, const DevWinPlan& wp, const HT* part, HT* F_val, hipStream_t st);
+// Off-diagonal Schur blocks by camera row (no staging in HBM): items[i] =
+// {row a, first and end upper block k0..k1 (off-diagonal, S order), 1 if the
+// item covers the whole row}; max_blk the longest item (its LDS size).
+constexpr int SCHUR_ROW_CAP = 448;  // most blocks one row item accumulates
+constexpr int SCHUR_LONG_TRACK = 64;  // see g2ohip_ctx::use_schur_rows
+template <typename HT>
+hipError_t launch_schur_rows(const DevGraph& g, const int4* items, int nitems, int max_blk, const int* q_slot,
+                             const HT* Hpl, const double* Linv, HT* F_val, hipStream_t st);
 
 // ---- implicit Schur operator
 template <typename HT>

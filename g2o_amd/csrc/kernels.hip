@@ -716,6 +716,173 @@ template hipError_t launch_schur_offdiag<float>(const DevGraph&, const DevWinPla
 template hipError_t launch_schur_offdiag<double>(const DevGraph&, const DevWinPlan&, const double*, double*,
                                                  hipStream_t);
 
+// ================================================= row-owner Schur blocks
+// The off-diagonal Schur blocks (a, b), a < b, one workgroup per camera row
+// a (or per column chunk of a long row): it walks camera a's observations
+// (a, p) in batches, forms G_a = Hpl_ap L_p^-T once per observation, then
+// takes every later free camera b of point p - the point's free slots are
+// sorted by camera, so they are a contiguous run after (a, p) - and adds
+// G_a G_b^T into an LDS accumulator for block (a, b). Nothing is staged in
+// HBM, every observation of a track of n cameras is touched by the n - 1
+// rows it pairs with, and a track of any length costs n (n - 1) / 2 block
+// products. The LDS sums are FP64 atomic adds (their order, and so the
+// last bits of the blocks, may vary from run to run).
+constexpr int SR_THREADS = 256;
+constexpr int SR_OBS = 128;  // observations per batch (threads 0..127 set them up)
+
+// Exclusive prefix sum of SR_OBS counts by wavefront 0 (two per lane);
+// pre[SR_OBS] = total.
+__device__ __forceinline__ void sr_prefix(const int* __restrict__ cnt, int* __restrict__ pre) {
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < 64) {
+    const int c0 = cnt[2 * lane], c1 = cnt[2 * lane + 1];
+    int v = c0 + c1;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(v, off, 64);
+      if (lane >= off) v += o;
+    }
+    const int ex = v - c0 - c1;
+    pre[2 * lane] = ex;
+    pre[2 * lane + 1] = ex + c0;
+    if (lane == 63) pre[SR_OBS] = v;
+  }
+}
+
+template <typename HT>
+__global__ __launch_bounds__(SR_THREADS) void k_schur_rows(DevGraph g, const int4* __restrict__ items, int nitems,
+                                                           const int* __restrict__ q_slot,
+                                                           const HT* __restrict__ Hpl,
+                                                           const double* __restrict__ Linv,
+                                                           HT* __restrict__ F_val) {
+  extern __shared__ double s_dyn[];  // [nblk * 36] accumulators, then [nblk] column cameras
+  __shared__ HT s_Ga[SR_OBS][18];
+  __shared__ double s_L[SR_OBS][6];
+  __shared__ int s_lo[SR_OBS], s_cnt[SR_OBS], s_pre[SR_OBS + 1];
+  // consecutive rows on one XCD (workgroups go to the 8 XCDs round robin):
+  // neighbouring rows share points, so their Hpl reads hit the same L2
+  const int per = (nitems + 7) / 8;
+  const int item = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (item >= nitems) return;
+  const int4 it = items[item];
+  const int a = it.x, k0 = it.y, k1 = it.z;
+  const bool full = it.w != 0;
+  const int nblk = k1 - k0;
+  double* s_acc = s_dyn;
+  int* s_col = reinterpret_cast<int*>(s_dyn + 36 * static_cast<size_t>(nblk));
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 36 * nblk; e += SR_THREADS) s_acc[e] = 0.0;
+  for (int j = tid; j < nblk; j += SR_THREADS) s_col[j] = g.S_col[k0 + j];
+  const int cb_lo = g.S_col[k0], cb_hi = g.S_col[k1 - 1];
+  const int qa0 = g.cam_qptr[a], qa1 = g.cam_qptr[a + 1];
+  for (int qb = qa0; qb < qa1; qb += SR_OBS) {
+    __syncthreads();  // the previous batch's pairs are done (and the zeroing above)
+    if (tid < SR_OBS) {
+      const int q = qb + tid;
+      const bool valid = q < qa1;
+      const int qc = valid ? q : qa0;
+      const int s = q_slot[qc];
+      const int ph = g.q_ph[qc];
+      const int pos = g.slot_pos[s];
+      HT h[18];
+#pragma unroll
+      for (int k = 0; k < 18; ++k) h[k] = Hpl[18 * static_cast<size_t>(qc) + k];
+      double m[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) m[k] = Linv[6 * static_cast<size_t>(ph) + k];
+      const int se = g.pos_ptr[pos] + g.pos_nfree[pos];
+      int lo = s + 1;
+      while (lo < se && g.slot_camh[lo] == a) ++lo;  // a second edge to camera a
+      int hi = se;
+      if (!full) {
+        // the chunk's cameras cb_lo..cb_hi: a sub-run of the sorted slots
+        int l = lo, r = hi;
+        while (l < r) { const int md = (l + r) >> 1; if (g.slot_camh[md] < cb_lo) l = md + 1; else r = md; }
+        lo = l;
+        r = hi;
+        while (l < r) { const int md = (l + r) >> 1; if (g.slot_camh[md] <= cb_hi) l = md + 1; else r = md; }
+        hi = l;
+      }
+      s_lo[tid] = lo;
+      s_cnt[tid] = valid ? max(hi - lo, 0) : 0;
+      // G_a = Hpl L^-T (L^-1 packed lower triangular)
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        const double h0 = h[3 * r], h1 = h[3 * r + 1], h2 = h[3 * r + 2];
+        s_Ga[tid][3 * r] = static_cast<HT>(h0 * m[0]);
+        s_Ga[tid][3 * r + 1] = static_cast<HT>(h0 * m[1] + h1 * m[2]);
+        s_Ga[tid][3 * r + 2] = static_cast<HT>(h0 * m[3] + h1 * m[4] + h2 * m[5]);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_L[tid][k] = m[k];
+    }
+    __syncthreads();
+    sr_prefix(s_cnt, s_pre);
+    __syncthreads();
+    const int total = s_pre[SR_OBS];
+    for (int k = tid; k < total; k += SR_THREADS) {
+      // the batch observation t whose run holds pair k
+      int l = 0, r = SR_OBS - 1;
+      while (l < r) { const int md = (l + r + 1) >> 1; if (s_pre[md] <= k) l = md; else r = md - 1; }
+      const int t = l;
+      const int s2 = s_lo[t] + (k - s_pre[t]);
+      const int b = g.slot_camh[s2];
+      const int q2 = g.slot_q[s2];
+      HT hb[18];
+#pragma unroll
+      for (int e = 0; e < 18; ++e) hb[e] = Hpl[18 * static_cast<size_t>(q2) + e];
+      // column of b in the chunk
+      int jl = 0, jr = nblk - 1;
+      while (jl < jr) { const int md = (jl + jr) >> 1; if (s_col[md] < b) jl = md + 1; else jr = md; }
+      double m[6];
+#pragma unroll
+      for (int e = 0; e < 6; ++e) m[e] = s_L[t][e];
+      HT gb[18];
+#pragma unroll
+      for (int rr = 0; rr < 6; ++rr) {
+        const double h0 = hb[3 * rr], h1 = hb[3 * rr + 1], h2 = hb[3 * rr + 2];
+        gb[3 * rr] = static_cast<HT>(h0 * m[0]);
+        gb[3 * rr + 1] = static_cast<HT>(h0 * m[1] + h1 * m[2]);
+        gb[3 * rr + 2] = static_cast<HT>(h0 * m[3] + h1 * m[4] + h2 * m[5]);
+      }
+      double* acc = s_acc + 36 * jl;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const HT a0 = s_Ga[t][3 * i], a1 = s_Ga[t][3 * i + 1], a2 = s_Ga[t][3 * i + 2];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          const HT v = a0 * gb[3 * j] + a1 * gb[3 * j + 1] + a2 * gb[3 * j + 2];
+          atomicAdd(acc + 6 * i + j, static_cast<double>(v));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < 36 * nblk; e += SR_THREADS) {
+    const int j = e / 36, r = e - 36 * j;
+    const int k = k0 + j;
+    const HT v = static_cast<HT>(-s_acc[e]);
+    F_val[36 * static_cast<size_t>(g.up_pos[k]) + r] = v;
+    F_val[36 * static_cast<size_t>(g.up_posT[k]) + (r % 6) * 6 + r / 6] = v;
+  }
+}
+
+template <typename HT>
+hipError_t launch_schur_rows(const DevGraph& g, const int4* items, int nitems, int max_blk, const int* q_slot,
+                             const HT* Hpl, const double* Linv, HT* F_val, hipStream_t st) {
+  if (nitems == 0) return hipSuccess;
+  const size_t lds = static_cast<size_t>(max_blk) * (36 * sizeof(double) + sizeof(int));
+  if (!lds_fits(k_schur_rows<HT>, lds)) return hipErrorInvalidValue;
+  const int per = (nitems + 7) / 8;
+  hipLaunchKernelGGL(k_schur_rows<HT>, dim3(8 * per), dim3(SR_THREADS), lds, st, g, items, nitems, q_slot, Hpl,
+                     Linv, F_val);
+  return hipGetLastError();
+}
+template hipError_t launch_schur_rows<float>(const DevGraph&, const int4*, int, int, const int*, const float*,
+                                             const double*, float*, hipStream_t);
+template hipError_t launch_schur_rows<double>(const DevGraph&, const int4*, int, int, const int*, const double*,
+                                              const double*, double*, hipStream_t);
+
 // ========================================================== implicit Schur
 template <typename HT>
 __global__ void k_point_dinv(DevGraph g, const double* __restrict__ Hll, double lambda, HT* __restrict__ Dinv,

@@ -248,6 +248,22 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
   return "";
 }
 
+long long schur_pair_count(const BAStructure& st) {
+  long long pairs = 0;
+  for (int p = 0; p < st.npos; ++p) {
+    if (st.pos_h[p] < 0) continue;
+    // free slots sorted by camera: pairs of different cameras
+    const int s0 = st.pos_ptr[p], n = st.pos_nfree[p];
+    for (int i = 0; i < n;) {
+      int j = i;
+      while (j < n && st.slot_camh[s0 + j] == st.slot_camh[s0 + i]) ++j;
+      pairs += static_cast<long long>(j - i) * (n - j);
+      i = j;
+    }
+  }
+  return pairs;
+}
+
 int max_point_obs(const BAStructure& st) {
   int m = 0;
   for (int p = 0; p < st.npos; ++p)

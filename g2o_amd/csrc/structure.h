@@ -117,6 +117,9 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
 std::string build_win_plan(const BAStructure& st, WinPlan& wp, int obs_max, int pch);
 // Largest number of observations with a free camera of one free point.
 int max_point_obs(const BAStructure& st);
+// Off-diagonal Schur pair products: over free points, pairs of their
+// observations with two different free cameras.
+long long schur_pair_count(const BAStructure& st);
 
 // 3D pose graph (VertexSE3 / EdgeSE3, 6x6 blocks, no Schur): free poses get
 // indices 0..Nf-1 in input order; the Hessian pattern is the diagonal plus one

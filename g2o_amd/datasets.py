@@ -274,6 +274,120 @@ def make_walk_ba(seed: int = 1002, num_cams: int = 1500, num_pts: int = 500_000,
                      gt_cams=gt_cams, gt_pts=gt_pts_new, info=info)
 
 
+def truncated_zipf(rng, a: float, lo: int, hi: int, size: int) -> np.ndarray:
+    """Samples of P(n) proportional to n^-a on the integers lo..hi (the Zipf
+    law restricted to that range, by inverting its cumulative table)."""
+    n = np.arange(lo, hi + 1)
+    cdf = np.cumsum(n.astype(np.float64) ** -a)
+    cdf /= cdf[-1]
+    return n[np.searchsorted(cdf, rng.random(size), side="right").clip(0, n.size - 1)]
+
+
+def make_zipf_ba(seed: int = 1003, num_cams: int = 4000, num_pts: int = 1_200_000, zipf: float = 1.8,
+                 max_track: int = 300, min_track: int = 2, step: float = 0.1, near_ratio: float = 0.02,
+                 outlier_ratio: float = 0.0, huber: float = 2.0, sig_t: float = 0.05, sig_r: float = 0.002,
+                 sig_p: float = 0.1, shuffle: bool = True) -> BAProblem:
+    """Uneven-structure BA: cameras ``step`` metres apart along a gently
+    curving path (0.4 km for 4000), each looking to the left of its heading
+    (fx ~ U(400, 800) = fy, principal point at the centre of 640 x 480). Track
+    lengths are min_track - 1 + k with k following the Zipf law k^-``zipf`` on
+    1..max_track - min_track + 1 (numpy's zipf(1.8) + 1 kept within 2..300:
+    mean ~6.3, so ~7.5M observations for 1.2M points);
+    each track is a run of consecutive cameras, so the co-visibility of the
+    reduced camera system is banded. A point is placed beyond the middle of
+    its run at a lateral distance that keeps it inside 80 % of every run
+    camera's view: at least the run's half-length over the narrowest tangent,
+    else log-uniform on 2..60 m; ``near_ratio`` of the points sit 0.3..1 m from
+    their cameras instead (short tracks there), so Hll conditioning spans
+    many decades. Projections behind a camera or outside the image are
+    dropped (points left with < 2 are dropped). Pixel noise N(0, 1);
+    ``outlier_ratio`` of the observations replaced by uniform image points.
+    Initial poses perturbed by (sig_t, sig_r), points by sig_p; the first two
+    cameras fixed."""
+    rng = np.random.default_rng(seed)
+    C = num_cams
+    heading = np.cumsum(rng.normal(0, 0.002, size=C)) + 0.2 * np.sin(np.arange(C) / 700.0)
+    pos = np.zeros((C, 3))
+    pos[1:, 0] = np.cumsum(step * np.cos(heading[:-1]))
+    pos[1:, 1] = np.cumsum(step * np.sin(heading[:-1]))
+    pos[:, 2] = 1.5 + 0.05 * np.sin(np.arange(C) / 90.0)
+    yaw = heading + np.pi / 2 + rng.normal(0, 0.01, size=C)
+    pitch = rng.normal(0, 0.01, size=C)
+    fwd = np.stack([np.cos(yaw) * np.cos(pitch), np.sin(yaw) * np.cos(pitch), np.sin(pitch)], axis=1)
+    R = look_rotation(fwd)
+    t = -np.einsum("cij,cj->ci", R, pos)
+    gt_cams = pose_from_Rt(R, t)
+    fx = rng.uniform(400, 800, size=C)
+    intr = np.stack([fx, fx, np.full(C, IMG_W / 2), np.full(C, IMG_H / 2)], axis=1)
+
+    P = num_pts
+    near = rng.random(P) < near_ratio
+    n = min_track - 1 + truncated_zipf(rng, zipf, 1, max_track - min_track + 1, P)
+    n = np.where(near, np.minimum(n, 4), n)
+    n = np.minimum(n, C)
+    first = (rng.random(P) * (C - n + 1)).astype(np.int64)
+    mid = first + (n - 1) // 2
+    # narrowest half-view tangent (80 % of the half width) over the run
+    tan_min = 0.8 * (IMG_W / 2) / _run_max(fx, first, n)
+    half = 0.5 * step * (n - 1)
+    lateral = np.maximum(1.2 * half / tan_min, np.exp(rng.uniform(np.log(2.0), np.log(60.0), size=P)))
+    lateral = np.where(near, rng.uniform(0.3, 1.0, size=P), lateral)
+    height = rng.uniform(-0.5, 0.5, size=P) * lateral * (IMG_H / IMG_W)
+    gt_pts = pos[mid] + fwd[mid] * lateral[:, None] + np.array([0.0, 0.0, 1.0]) * height[:, None]
+    offs = np.repeat(np.cumsum(n) - n, n)
+    k = np.arange(n.sum()) - offs
+    ep = np.repeat(np.arange(P), n)
+    ec = np.repeat(first, n) + k
+    uv, d = project(gt_cams, intr, gt_pts, ec, ep)
+    keep = (d > 0.1) & (uv[:, 0] >= 0) & (uv[:, 1] >= 0) & (uv[:, 0] < IMG_W) & (uv[:, 1] < IMG_H)
+    cnt = np.bincount(ep[keep], minlength=P)
+    keep &= cnt[ep] >= 2
+    used = np.flatnonzero(cnt >= 2)
+    remap = -np.ones(P, dtype=np.int64)
+    perm = rng.permutation(used.size) if shuffle else np.arange(used.size)
+    remap[used] = perm
+    gt_pts_new = np.empty((used.size, 3))
+    gt_pts_new[perm] = gt_pts[used]
+    ec, ep, uv = ec[keep], remap[ep[keep]], uv[keep]
+    E = ec.size
+    meas = uv + rng.normal(0, 1.0, size=uv.shape)
+    out = rng.random(E) < outlier_ratio
+    meas[out] = np.stack([rng.uniform(0, IMG_W, out.sum()), rng.uniform(0, IMG_H, out.sum())], axis=1)
+    order = np.lexsort((ep, ec))
+    ec, ep, meas = ec[order], ep[order], meas[order]
+    cams = perturb_poses(rng, gt_cams, sig_t, sig_r)
+    cam_fixed = np.zeros(C, dtype=np.uint8)
+    cam_fixed[:2] = 1
+    cams[:2] = gt_cams[:2]
+    pts = gt_pts_new + rng.normal(0, sig_p, size=gt_pts_new.shape)
+    tracks = np.bincount(ep, minlength=used.size)
+    info = {"outliers": int(out.sum()), "mean_track": float(E / max(used.size, 1)),
+            "max_track": int(tracks.max()) if tracks.size else 0}
+    return BAProblem(cams, intr, pts, ec.astype(np.int32), ep.astype(np.int32), meas, np.ones(E), cam_fixed,
+                     np.zeros(used.size, dtype=np.uint8), huber, name=f"zipf_ba_{C}c_{used.size}p",
+                     gt_cams=gt_cams, gt_pts=gt_pts_new, info=info)
+
+
+def _run_max(v: np.ndarray, first: np.ndarray, n: np.ndarray) -> np.ndarray:
+    """max(v[first[i] : first[i] + n[i]]) for every i (sparse table)."""
+    levels = [v]
+    span = 1
+    while 2 * span <= int(n.max()):
+        prev = levels[-1]
+        nxt = prev.copy()
+        nxt[:-span] = np.maximum(prev[:-span], prev[span:])
+        levels.append(nxt)
+        span *= 2
+    lvl = np.floor(np.log2(n)).astype(np.int64)
+    out = np.empty(first.size)
+    for L in range(len(levels)):
+        m = lvl == L
+        if m.any():
+            w = 1 << L
+            out[m] = np.maximum(levels[L][first[m]], levels[L][first[m] + n[m] - w])
+    return out
+
+
 def make_metric_ba(seed: int = 1002) -> BAProblem:
     """The headline workload: 1,500 cameras on a 3 km walk, 500k points,
     ~3M observations, Huber delta 2, 5 % outliers."""

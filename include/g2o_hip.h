@@ -119,7 +119,9 @@ int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_
  * off-diagonal Schur pair products, off-diagonal Schur partial sums, max
  * slots per point, Hschur blocks in both triangles, (chunk, camera) partial
  * rows of the linearisation, points with active edges, observations with a
- * free camera and a free point (Hpl blocks). */
+ * free camera and a free point (Hpl blocks). Point windows and partial sums
+ * are 0 when the explicit Schur complement forms its off-diagonal blocks by
+ * camera rows instead (no partial sums staged). */
 int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info);
 
 /* Measurements of EdgeSE3ProjectXYZ (edge_project_xyz.cpp:44-49): meas[E*2]

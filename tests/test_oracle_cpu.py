@@ -377,12 +377,18 @@ def test_dogleg_oracle(O):
     assert np.all(dl[:, 1] <= dl[:, 0] * (1 + 1e-12))
     # each of the k - 1 rejected tries of an iteration halves the radius; the
     # accepted one halves it, keeps it or sets it to 3 |h_dl| <= 3 delta
-    many = np.flatnonzero(dl[:, 3] > 1)
-    assert many.size > 0
-    for i in many:
-        prev = dl[i - 1, 2] if i > 0 else 1e4
-        k = dl[i, 3]
-        assert prev * 0.5 ** k * (1 - 1e-12) <= dl[i, 2] <= prev * 0.5 ** (k - 1) * 3.0 * (1 + 1e-12)
+    def radius_rule(tr, delta0):
+        for i in np.flatnonzero(tr[:, 3] > 1):
+            prev = tr[i - 1, 2] if i > 0 else delta0
+            k = tr[i, 3]
+            assert prev * 0.5 ** k * (1 - 1e-12) <= tr[i, 2] <= prev * 0.5 ** (k - 1) * 3.0 * (1 + 1e-12)
+        return int((tr[:, 3] > 1).sum())
+
+    radius_rule(dl, 1e4)
+    # the scrambled sphere rejects seven tries in a row in its second
+    # iteration (whether the BA run above rejects any depends on rounding)
+    _, ds = O.OraclePG(_scrambled_sphere()).dogleg(12, solver=0)
+    assert radius_rule(ds, 1e4) > 0 and ds[:, 3].max() >= 4
     pg = datasets.make_sphere_pose_graph(nodes_per_level=40, laps=3)
     n, tp = O.OraclePG(pg).dogleg(6, solver=0)
     _, tg = O.OraclePG(pg).gauss_newton(6, solver=0)
