@@ -159,7 +159,7 @@ hipError_t launch_schur_offdiag(const DevGraph& g, const DevWinPlan& wp, const H
 // Off-diagonal Schur blocks by camera row (no staging in HBM): items[i] =
 // {row a, first and end upper block k0..k1 (off-diagonal, S order), 1 if the
 // item covers the whole row}; max_blk the longest item (its LDS size).
-constexpr int SCHUR_ROW_CAP = 448;  // most blocks one row item accumulates
+constexpr int SCHUR_ROW_CAP = 320;  // most blocks one row item accumulates (FP64: 146 KB of LDS)
 constexpr int SCHUR_LONG_TRACK = 64;  // see g2ohip_ctx::use_schur_rows
 template <typename HT>
 hipError_t launch_schur_rows(const DevGraph& g, const int4* items, int nitems, int max_blk, const int* q_slot,

@@ -727,24 +727,39 @@ template hipError_t launch_schur_offdiag<double>(const DevGraph&, const DevWinPl
 // rows it pairs with, and a track of any length costs n (n - 1) / 2 block
 // products. The LDS sums are FP64 atomic adds (their order, and so the
 // last bits of the blocks, may vary from run to run).
-constexpr int SR_THREADS = 256;
-constexpr int SR_OBS = 128;  // observations per batch (threads 0..127 set them up)
+// 16 wavefronts per workgroup: a long row's accumulators fill most of a
+// CU's LDS, so one workgroup per CU has to bring the waves that hide the
+// gathers' latency
+constexpr int SR_THREADS = 1024;
+constexpr int SR_OBS = 256;  // observations per batch (threads 0..255 set them up)
+static_assert(SR_OBS % 64 == 0 && SR_OBS <= SR_THREADS, "batch setup: one thread per observation");
 
-// Exclusive prefix sum of SR_OBS counts by wavefront 0 (two per lane);
+// Exclusive prefix sum of SR_OBS counts by wavefront 0 (four per lane);
 // pre[SR_OBS] = total.
 __device__ __forceinline__ void sr_prefix(const int* __restrict__ cnt, int* __restrict__ pre) {
+  constexpr int PL = SR_OBS / 64;
   const int lane = threadIdx.x & 63;
   if (threadIdx.x < 64) {
-    const int c0 = cnt[2 * lane], c1 = cnt[2 * lane + 1];
-    int v = c0 + c1;
+    int c[PL];
+    int v = 0;
+#pragma unroll
+    for (int i = 0; i < PL; ++i) {
+      c[i] = cnt[PL * lane + i];
+      v += c[i];
+    }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int o = __shfl_up(v, off, 64);
       if (lane >= off) v += o;
     }
-    const int ex = v - c0 - c1;
-    pre[2 * lane] = ex;
-    pre[2 * lane + 1] = ex + c0;
+    int ex = v;
+#pragma unroll
+    for (int i = 0; i < PL; ++i) ex -= c[i];
+#pragma unroll
+    for (int i = 0; i < PL; ++i) {
+      pre[PL * lane + i] = ex;
+      ex += c[i];
+    }
     if (lane == 63) pre[SR_OBS] = v;
   }
 }

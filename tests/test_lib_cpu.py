@@ -76,6 +76,20 @@ def test_structure_matches_reference_pattern(seed, cams, pts):
     assert (nq + 511) // 512 <= info[5] <= nq
 
 
+def test_long_tracks_take_the_camera_row_schur():
+    """Tracks longer than 64 cameras making most of the pair products (Zipf
+    law, tracks up to 300): the off-diagonal Schur blocks are formed by
+    camera rows, so the structure has no point windows and no partial sums;
+    a walk with geometric tracks keeps the window plan."""
+    z = datasets.make_zipf_ba(seed=5, num_cams=300, num_pts=20000)
+    info = g2o_amd.BundleAdjuster(z, device=-1).structure_info()
+    pairs, nS = _pairs_numpy(z)
+    assert info[5] == 0 and info[7] == 0
+    assert info[6] == pairs and info[3] == nS
+    w = datasets.make_walk_ba(seed=3, num_cams=40, num_pts=3000)
+    assert g2o_amd.BundleAdjuster(w, device=-1).structure_info()[5] > 0
+
+
 def test_structure_rejects_bad_indices():
     prob = datasets.make_small_ba(seed=1)
     prob.edge_cam = prob.edge_cam.copy()

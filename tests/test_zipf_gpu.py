@@ -1,0 +1,106 @@
+"""Bundle adjustment with Zipf track lengths (n - 1 ~ k^-1.8, tracks up to 300
+cameras, banded co-visibility) at a reduced shape: 300 cameras, ~19k points,
+~127k observations, 3.5M off-diagonal pair products of which 82 % come from
+tracks longer than 64 cameras. The explicit Schur complement then forms its
+off-diagonal blocks by camera rows (no window partials); blocks, reduced
+right-hand side, solve and LM trace are compared with the FP64 oracle, and
+the two off-diagonal paths with each other."""
+import numpy as np
+import pytest
+
+from g2o_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def zipf():
+    return datasets.make_zipf_ba(seed=5, num_cams=300, num_pts=20000)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _schur(prob, lam, **kw):
+    import g2o_amd
+    ba = g2o_amd.BundleAdjuster(prob, device=0, precision=g2o_amd.FP64, pcg_tolerance=1e-30, pcg_max_iter=2000,
+                                **kw)
+    ba.build_system()
+    ba.set_lambda(lam)
+    assert ba.solve()
+    return ba
+
+
+def test_zipf_schur_fp64_matches_oracle(zipf, oracle_lib):
+    ora = oracle_lib.OracleBA(zipf)
+    ora.linearize()
+    lam = 2.5
+    st, rows_o, cols_o, blocks_o, bs_o = ora.schur(lam)
+    assert st == 0
+    ba = _schur(zipf, lam)
+    info = ba.structure_info()
+    assert info[5] == 0 and info[7] == 0  # camera-row path
+    rows, cols, blocks, bs = ba.schur(ora.nschur)
+    assert np.array_equal(rows, rows_o) and np.array_equal(cols, cols_o)
+    _say(f"blocks rel {_rel(blocks, blocks_o):.2e} bschur rel {_rel(bs, bs_o):.2e}")
+    assert _rel(blocks, blocks_o) < 1e-11
+    assert _rel(bs, bs_o) < 1e-11
+    st, dx_o, _ = ora.solve(lam, solver=0)
+    assert st == 0
+    assert _rel(ba.x(), dx_o) < 1e-8
+
+
+def test_zipf_row_and_window_paths_agree(zipf, monkeypatch):
+    lam = 0.7
+    monkeypatch.setenv("G2OHIP_SCHUR_PATH", "rows")
+    a = _schur(zipf, lam)
+    monkeypatch.setenv("G2OHIP_SCHUR_PATH", "windows")
+    b = _schur(zipf, lam)
+    assert a.structure_info()[5] == 0 and b.structure_info()[5] > 0
+    n = int(a.structure_info()[3])
+    ra, ca, ba_, sa = a.schur(n)
+    rb, cb, bb, sb = b.schur(n)
+    assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+    assert _rel(ba_, bb) < 1e-13
+    assert _rel(a.x(), b.x()) < 1e-9
+
+
+def test_zipf_lm_trace_fp64(zipf, oracle_lib):
+    """PCG run to 1e-12 on both sides: with Hll conditioning over many decades
+    the 1e-6 default leaves step differences that the LM amplifies (5e-8 in
+    chi2 after 8 iterations); at 1e-12 the traces agree to ~6e-9, the level
+    of the run-to-run rounding of the LDS sums amplified by the LM."""
+    import g2o_amd
+    ora = oracle_lib.OracleBA(zipf)
+    n_o, trace, _ = ora.optimize(8, solver=1, tol=1e-12)
+    ba = g2o_amd.BundleAdjuster(zipf, device=0, precision=g2o_amd.FP64, pcg_tolerance=1e-12)
+    stats = ba.optimize(8)
+    assert len(stats) == n_o
+    dev = max(abs(s["chi2"] - t[1]) / t[1] for s, t in zip(stats, trace))
+    _say(f"max relative chi2 deviation per iteration {dev:.2e}")
+    assert dev < 1e-7
+    for s, t in zip(stats, trace):
+        assert s["levenberg_iterations"] == int(t[3])
+        assert abs(s["iterations_linear_solver"] - int(t[4])) <= 2
+
+
+def test_zipf_lm_fp32_refine(zipf, oracle_lib):
+    """FP32 blocks (FP64 diagonal), FP64 refinement, PCG tolerance 1e-6 as in
+    the LM default: the trace stays within the FP32+refinement accuracy
+    bound of 1e-6 relative chi2 of the FP64 oracle."""
+    import g2o_amd
+    ora = oracle_lib.OracleBA(zipf)
+    n_o, trace, _ = ora.optimize(8, solver=1, tol=1e-6)
+    ba = g2o_amd.BundleAdjuster(zipf, device=0, precision=g2o_amd.FP32, refine_steps=4)
+    stats = ba.optimize(8)
+    assert len(stats) == n_o
+    dev = max(abs(s["chi2"] - t[1]) / t[1] for s, t in zip(stats, trace))
+    _say(f"max relative chi2 deviation per iteration {dev:.2e}")
+    assert dev < 1e-6
+    for s, t in zip(stats, trace):
+        assert s["levenberg_iterations"] == int(t[3])
