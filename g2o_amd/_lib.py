@@ -24,7 +24,7 @@ EXPORTS = [
     "g2ohip_ba_set_information",
     "g2ohip_ba_set_state", "g2ohip_ba_get_state", "g2ohip_compute_active_errors", "g2ohip_get_errors",
     "g2ohip_build_system", "g2ohip_set_lambda", "g2ohip_restore_diagonal", "g2ohip_solve", "g2ohip_get_x",
-    "g2ohip_get_b", "g2ohip_update", "g2ohip_push", "g2ohip_pop", "g2ohip_discard_top", "g2ohip_lambda_init",
+    "g2ohip_get_b", "g2ohip_multiply_hessian", "g2ohip_update", "g2ohip_push", "g2ohip_pop", "g2ohip_discard_top", "g2ohip_lambda_init",
     "g2ohip_compute_scale", "g2ohip_optimize", "g2ohip_set_algorithm", "g2ohip_set_force_stop_flag", "g2ohip_compute_marginals", "g2ohip_ba_get_hessian",
     "g2ohip_ba_get_schur", "g2ohip_last_solve_info", "g2ohip_set_profiling", "g2ohip_get_profile",
     "g2ohip_linear_solve_pcg", "g2ohip_pg_set_structure", "g2ohip_pg_structure_info", "g2ohip_pg_set_measurements",
@@ -91,6 +91,7 @@ def lib():
         "g2ohip_solve": (ct.c_int, [_vp, _ip]),
         "g2ohip_get_x": (ct.c_int, [_vp, _dp]),
         "g2ohip_get_b": (ct.c_int, [_vp, _dp]),
+        "g2ohip_multiply_hessian": (ct.c_int, [_vp, _dp, _dp]),
         "g2ohip_update": (ct.c_int, [_vp]),
         "g2ohip_push": (ct.c_int, [_vp]),
         "g2ohip_pop": (ct.c_int, [_vp]),

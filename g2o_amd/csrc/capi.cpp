@@ -252,6 +252,13 @@ int g2ohip_get_b(g2ohip_ctx* ctx, double* b) {
   });
 }
 
+int g2ohip_multiply_hessian(g2ohip_ctx* ctx, const double* src, double* dest) {
+  return guarded(ctx, [&]() -> int {
+    if (!src || !dest) return ctx->fail(G2OHIP_ERR_INVALID, "null vector");
+    return ctx->multiply_hessian(src, dest);
+  });
+}
+
 int g2ohip_update(g2ohip_ctx* ctx) {
   return guarded(ctx, [&]() -> int {
     return ctx->update();

@@ -713,6 +713,25 @@ int g2ohip_ctx::solve(int* ok, bool defer_err) {
   return G2OHIP_OK;
 }
 
+// BlockSolverBase::multiplyHessian (block_solver.h:149-151): the pose rows of
+// dest = Hpp src, with Hpp the undamped pose-pose blocks of the last
+// build_system (block diagonal for bundle adjustment, the whole Hessian of a
+// pose graph).
+int g2ohip_ctx::multiply_hessian(const double* src, double* dest) {
+  if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+  const int nb = mode == 1 ? pgst.Nf : st.Cf;
+  const size_t n = 6 * static_cast<size_t>(nb);
+  DBuf<double> dx, dy;
+  std::vector<double> hx(src, src + n);
+  HK(dx.upload(hx, stream));
+  HK(dy.alloc(n));
+  if (mode == 1) HK(launch_block_mv6(nb, d_pg_F_ptr.p, d_pg_F_col.p, d_pg_F_val.p, dx.p, dy.p, stream));
+  else HK(launch_block_mv6(nb, nullptr, nullptr, d_Hpp.p, dx.p, dy.p, stream));
+  HK(hipMemcpyAsync(dest, dy.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HK(hipStreamSynchronize(stream));
+  return G2OHIP_OK;
+}
+
 // --------------------------------------------------------- update / stack
 int g2ohip_ctx::update() {
   if (mode == 1) {

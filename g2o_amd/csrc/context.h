@@ -201,6 +201,7 @@ struct g2ohip_ctx {
   int optimize_dogleg(int iterations, g2ohip_iter_stats* stats, int* done);
   int get_hessian(double* Hpp, double* Hll, double* Hpl_edge);
   int replay_pcg(int reps, double* us_spmv, double* us_iter);
+  int multiply_hessian(const double* src, double* dest);
   int get_schur(int* rows, int* cols, double* blocks, double* bschur);
   int linear_solve_pcg_device(int nb, int bd, const int* F_ptr, const int* F_col, const double* F_val,
                               const int* F_diag, const double* b, double* x, double tol, int max_iter,

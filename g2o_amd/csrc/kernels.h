@@ -234,6 +234,9 @@ template <typename MT, int BD>
 hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const double* d, const double* q,
                              double* s, const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st);
+// y = A x over 6x6 blocks (block-CSR, or block diagonal when F_ptr is null)
+hipError_t launch_block_mv6(int nb, const int* F_ptr, const int* F_col, const double* F_val, const double* x,
+                            double* y, hipStream_t st);
 // ---- exact Hessian (adjoint, Newton polish), adjoint.hip
 hipError_t launch_linearize_exact(const DevGraph& g, const double* camRt, const double* pts, double* Hpl,
                                   double* Hll, double* hpp_part, hipStream_t st);

@@ -2144,6 +2144,32 @@ G2OHIP_PCG_INST(double, 6)
 G2OHIP_PCG_INST(double, 3)
 #undef G2OHIP_PCG_INST
 
+// y = A x over 6x6 blocks: block-CSR (F_ptr, F_col, F_val) or, with F_ptr
+// null, block diagonal (block i at F_val + 36 i); one thread per row of y.
+__global__ void k_block_mv6(int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col,
+                            const double* __restrict__ F_val, const double* __restrict__ x, double* __restrict__ y) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * nb) return;
+  const int i = t / 6, r = t - 6 * i;
+  const int k0 = F_ptr ? F_ptr[i] : i, k1 = F_ptr ? F_ptr[i + 1] : i + 1;
+  double acc = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const int c = F_ptr ? F_col[k] : i;
+    const double* B = F_val + 36 * static_cast<size_t>(k) + 6 * r;
+    const double* v = x + 6 * static_cast<size_t>(c);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc = fma(B[j], v[j], acc);
+  }
+  y[t] = acc;
+}
+
+hipError_t launch_block_mv6(int nb, const int* F_ptr, const int* F_col, const double* F_val, const double* x,
+                            double* y, hipStream_t st) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_block_mv6, dim3(nblocks(6 * nb, 256)), dim3(256), 0, st, nb, F_ptr, F_col, F_val, x, y);
+  return hipGetLastError();
+}
+
 template <typename HT>
 __global__ void k_to_double(int n, const HT* __restrict__ src, double* __restrict__ dst, double scale_add) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

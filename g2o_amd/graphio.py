@@ -143,25 +143,26 @@ class G2oGraph:
         fx fy cx cy, one row for every camera or a single row for all; None
         takes them from the file's camera parameters (EDGE_PROJECT_XYZ2UV
         edges), since EDGE_SE3_PROJECT_XYZ keeps its intrinsics out of the
-        format. The path takes scalar information per edge, so every Omega must
-        be w * I."""
+        format. Information matrices that are all w * I become the edge
+        weights; otherwise the weights are 1 and ``omega`` holds each edge's
+        full 2x2 matrix (O00 O01 O11)."""
         if intr is None:
             intr = self.camera_intrinsics()
         a = self.ba_arrays()
         om = a["omega"]
-        if om.size and (np.any(om[:, 1] != 0.0) or np.any(om[:, 0] != om[:, 2])):
-            bad = int(np.flatnonzero((om[:, 1] != 0.0) | (om[:, 0] != om[:, 2]))[0])
-            raise G2oIOError(f"edge {bad}: information matrix {om[bad].tolist()} (upper triangle) is not w * I; "
-                             "the path takes scalar information per projection edge")
+        scalar = not (om.size and (np.any(om[:, 1] != 0.0) or np.any(om[:, 0] != om[:, 2])))
         C = a["cams"].shape[0]
         intr = np.asarray(intr, np.float64).reshape(-1, 4)
         if intr.shape[0] == 1:
             intr = np.repeat(intr, C, axis=0)
         if intr.shape[0] != C:
             raise G2oIOError(f"intrinsics for {intr.shape[0]} cameras, graph has {C}")
-        return BAProblem(a["cams"], intr.copy(), a["pts"], a["edge_cam"], a["edge_pt"], a["meas"],
-                         om[:, 0].copy(), a["cam_fixed"], a["pt_fixed"], float(huber), name=name,
-                         info=dict(cam_id=a["cam_id"], pt_id=a["pt_id"]))
+        prob = BAProblem(a["cams"], intr.copy(), a["pts"], a["edge_cam"], a["edge_pt"], a["meas"],
+                         om[:, 0].copy() if scalar else np.ones(om.shape[0]), a["cam_fixed"], a["pt_fixed"],
+                         float(huber), name=name, info=dict(cam_id=a["cam_id"], pt_id=a["pt_id"]))
+        if not scalar:
+            prob.omega = om.copy()
+        return prob
 
     def set_ba_state(self, cams, pts):
         cams = None if cams is None else np.ascontiguousarray(cams, np.float64)
@@ -174,7 +175,8 @@ class G2oGraph:
         self._check(self._lib.g2ohip_graph_set_ba_state(self._h, _ptr(cams, ct.c_double), _ptr(pts, ct.c_double)))
 
     def add_ba(self, prob: BAProblem, cam_id=None, pt_id=None):
-        """Adds a BAProblem's vertices and edges (weights become w * I)."""
+        """Adds a BAProblem's vertices and edges; each edge's information is
+        weight * omega (omega = I when the problem has none)."""
         C, P, E = prob.num_cams, prob.num_pts, prob.num_edges
         cams = np.ascontiguousarray(prob.cams, np.float64)
         pts = np.ascontiguousarray(prob.pts, np.float64)
@@ -182,7 +184,9 @@ class G2oGraph:
         ep = np.ascontiguousarray(prob.edge_pt, np.int32)
         meas = np.ascontiguousarray(prob.meas, np.float64)
         w = np.ones(E) if prob.weight is None else np.asarray(prob.weight, np.float64)
-        om = np.ascontiguousarray(np.stack([w, np.zeros(E), w], axis=1))
+        o = getattr(prob, "omega", None)
+        base = np.tile([1.0, 0.0, 1.0], (E, 1)) if o is None else np.asarray(o, np.float64).reshape(E, 3)
+        om = np.ascontiguousarray(base * w[:, None])
         cf = np.ascontiguousarray(prob.cam_fixed, np.uint8)
         pf = np.ascontiguousarray(prob.pt_fixed, np.uint8)
         ci = None if cam_id is None else np.ascontiguousarray(cam_id, np.int32)

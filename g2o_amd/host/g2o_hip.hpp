@@ -64,8 +64,24 @@ class BlockSolverHIP {
     long long info[13];
     if (!_dev.ok(g2ohip_ba_structure_info(_dev.get(), info))) return false;
     _size = static_cast<size_t>(6 * info[0] + 3 * info[1]);
+    // hessianIndex of each vertex (sparse_optimizer.cpp:175-200): the free
+    // vertices that have an edge, in input order
+    std::vector<uint8_t> camUsed(numCams, 0), pointUsed(numPoints, 0);
+    for (int k = 0; k < numEdges; ++k) camUsed[edgeCam[k]] = pointUsed[edgePoint[k]] = 1;
+    _camIndex.assign(numCams, -1);
+    _pointIndex.assign(numPoints, -1);
+    int cf = 0, pf = 0;
+    for (int c = 0; c < numCams; ++c)
+      if (camUsed[c] && !(camFixed && camFixed[c])) _camIndex[c] = cf++;
+    for (int p = 0; p < numPoints; ++p)
+      if (pointUsed[p] && !(pointFixed && pointFixed[p])) _pointIndex[p] = pf++;
+    _hpp.assign(36 * static_cast<size_t>(cf), 0.0);
+    _hll.assign(9 * static_cast<size_t>(pf), 0.0);
+    _hessianStale = true;
     return true;
   }
+  // BaseEdge::setInformation for every edge: omega[E*3] = O00 O01 O11
+  bool setInformation(const double* omega) { return _dev.ok(g2ohip_ba_set_information(_dev.get(), omega)); }
   bool setMeasurements(const double* meas, const double* weight, const double* intrinsics, double huberDelta) {
     return _dev.ok(g2ohip_ba_set_measurements(_dev.get(), meas, weight, intrinsics, huberDelta));
   }
@@ -90,11 +106,37 @@ class BlockSolverHIP {
   bool init(bool /*online*/ = false) { return _dev.ok(g2ohip_set_options(_dev.get(), &_opt)); }
   // the block structure is built once by setGraph
   bool buildStructure(bool /*zeroBlocks*/ = false) { return _size > 0; }
-  bool buildSystem() { return _dev.ok(g2ohip_build_system(_dev.get())); }
+  bool buildSystem() {
+    _hessianStale = true;
+    return _dev.ok(g2ohip_build_system(_dev.get()));
+  }
   bool solve() {
     int ok = 0;
     return _dev.ok(g2ohip_solve(_dev.get(), &ok)) && ok != 0;
   }
+  // BlockSolverBase::multiplyHessian: the pose rows of dest = Hpp src (both
+  // of vectorSize() doubles; the landmark rows of dest are left as they are)
+  bool multiplyHessian(double* dest, const double* src) {
+    return _dev.ok(g2ohip_multiply_hessian(_dev.get(), src, dest));
+  }
+  // The vertex's Hessian block (row-major 6x6 for a camera, 3x3 for a point)
+  // of the last buildSystem, or nullptr for a vertex without one (fixed, or
+  // without edges). g2o's BlockSolver::buildStructure maps each vertex's
+  // _hessian onto its diagonal block (block_solver.hpp:144-161), which
+  // OptimizationAlgorithmLevenberg::computeLambdaInit reads
+  // (optimization_algorithm_levenberg.cpp:155-167); here the blocks are
+  // mirrored on the host, copied on the first access after a buildSystem.
+  const double* vertexHessian(bool camera, int index) {
+    const std::vector<int>& idx = camera ? _camIndex : _pointIndex;
+    if (index < 0 || index >= static_cast<int>(idx.size()) || idx[index] < 0) return nullptr;
+    if (_hessianStale) {
+      if (!_dev.ok(g2ohip_ba_get_hessian(_dev.get(), _hpp.data(), _hll.data(), nullptr))) return nullptr;
+      _hessianStale = false;
+    }
+    return camera ? &_hpp[36 * static_cast<size_t>(idx[index])] : &_hll[9 * static_cast<size_t>(idx[index])];
+  }
+  int numCameras() const { return static_cast<int>(_camIndex.size()); }
+  int numPoints() const { return static_cast<int>(_pointIndex.size()); }
   bool setLambda(double lambda, bool /*backup*/ = false) { return _dev.ok(g2ohip_set_lambda(_dev.get(), lambda)); }
   void restoreDiagonal() { _dev.ok(g2ohip_restore_diagonal(_dev.get())); }
   bool schur() const { return true; }
@@ -146,6 +188,9 @@ class BlockSolverHIP {
   g2ohip_options _opt{};
   size_t _size = 0;
   std::vector<double> _x, _b;
+  std::vector<int> _camIndex, _pointIndex;  // hessianIndex per vertex, -1 = none
+  std::vector<double> _hpp, _hll;           // vertex Hessian mirror
+  bool _hessianStale = true;
 };
 
 // LinearSolver<Matrix{3,6}d>::solve for a host matrix given by its upper
@@ -213,6 +258,10 @@ class PoseGraphSolverHIP {
   }
   bool setLambda(double lambda, bool = false) { return _dev.ok(g2ohip_set_lambda(_dev.get(), lambda)); }
   void restoreDiagonal() { _dev.ok(g2ohip_restore_diagonal(_dev.get())); }
+  // BlockSolverBase::multiplyHessian: dest = H src (the whole Hessian of the pose graph)
+  bool multiplyHessian(double* dest, const double* src) {
+    return _dev.ok(g2ohip_multiply_hessian(_dev.get(), src, dest));
+  }
   size_t vectorSize() const { return _size; }
   int optimize(int iterations, std::vector<g2ohip_iter_stats>* stats = nullptr) {
     std::vector<g2ohip_iter_stats> s(iterations > 0 ? iterations : 1);

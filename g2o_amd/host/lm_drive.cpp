@@ -19,6 +19,7 @@
 // command-line tool does with -o, saves the optimised graph. A bundle-
 // adjustment file needs the intrinsics (the format holds none); a file of
 // VERTEX_SE3:QUAT / EDGE_SE3:QUAT runs the pose-graph solver.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -37,6 +38,22 @@ bool read_vec(FILE* f, std::vector<T>& v, size_t n) {
   return n == 0 || std::fread(v.data(), sizeof(T), n, f) == n;
 }
 
+// OptimizationAlgorithmLevenberg::computeLambdaInit (optimization_algorithm_
+// levenberg.cpp:155-167) as g2o runs it: the largest diagonal entry over the
+// vertex Hessians the block solver mapped, times tau, or the user's
+// initialLambda.
+double lambda_init_from_vertices(g2o_hip::BlockSolverHIP& s) {
+  if (s.options().user_lambda_init > 0) return s.options().user_lambda_init;
+  double maxDiagonal = 0;
+  for (int c = 0; c < s.numCameras(); ++c)
+    if (const double* H = s.vertexHessian(true, c))
+      for (int j = 0; j < 6; ++j) maxDiagonal = std::max(std::fabs(H[7 * j]), maxDiagonal);
+  for (int p = 0; p < s.numPoints(); ++p)
+    if (const double* H = s.vertexHessian(false, p))
+      for (int j = 0; j < 3; ++j) maxDiagonal = std::max(std::fabs(H[4 * j]), maxDiagonal);
+  return s.options().tau * maxDiagonal;
+}
+
 int run_solver_loop(g2o_hip::BlockSolverHIP& s, int iterations) {
   s.init();
   double lambda = 0, ni = 2;
@@ -45,7 +62,12 @@ int run_solver_loop(g2o_hip::BlockSolverHIP& s, int iterations) {
     const double chiBefore = currentChi;
     if (!s.buildSystem()) return 1;
     if (i == 0) {
-      lambda = s.computeLambdaInit();
+      // the host loop takes lambda from the vertex Hessians, as g2o does; the
+      // device's own computeLambdaInit must give the same value
+      lambda = lambda_init_from_vertices(s);
+      const double dev = s.computeLambdaInit();
+      std::fprintf(stderr, "lambda_init vertex_hessians %.17g device %.17g\n", lambda, dev);
+      if (!(lambda == dev)) return 1;
       ni = 2;
     }
     double rho = 0;
@@ -162,20 +184,35 @@ int run_g2o_file(int argc, char** argv) {
   }
   const int C = (int)ba.camId.size(), P = (int)ba.pointId.size(), E = (int)ba.edgeCam.size();
   std::vector<double> w(E), intr(4 * static_cast<size_t>(C));
+  // scalar information (O = w I) as edge weights; otherwise the full 2x2
+  // matrices through setInformation with unit weights
+  bool scalar = true;
   for (int e = 0; e < E; ++e) {
     const double* o = &ba.omega[3 * static_cast<size_t>(e)];
-    if (o[1] != 0.0 || o[0] != o[2]) {
-      std::fprintf(stderr, "edge %d: information matrix is not w * I\n", e);
-      return 1;
-    }
+    if (o[1] != 0.0 || o[0] != o[2]) scalar = false;
     w[e] = o[0];
   }
+  if (!scalar) std::fill(w.begin(), w.end(), 1.0);
   if (from_params) {
     std::vector<int> ids, edgeParam;
     std::vector<double> params;
     if (!io.getCameraParameters(ids, params, edgeParam)) return 1;
-    // EdgeProjectXYZ2UV: fx = fy = f, principal point (cx, cy)
+    // EdgeProjectXYZ2UV: fx = fy = f, principal point (cx, cy). g2o resolves
+    // the parameters per edge; the solver holds them per camera, so a camera
+    // seen through two different parameter sets cannot be represented
+    std::vector<int> camParam(C, -1);
     for (int e = 0; e < E; ++e) {
+      int& cp = camParam[ba.edgeCam[e]];
+      if (cp >= 0 && cp != edgeParam[e]) {
+        const double* a = &params[4 * static_cast<size_t>(cp)];
+        const double* b = &params[4 * static_cast<size_t>(edgeParam[e])];
+        if (a[0] != b[0] || a[1] != b[1] || a[2] != b[2]) {
+          std::fprintf(stderr, "camera %d is observed through two different camera parameter sets\n",
+                       ba.camId[ba.edgeCam[e]]);
+          return 1;
+        }
+      }
+      cp = edgeParam[e];
       const double* p = &params[4 * static_cast<size_t>(edgeParam[e])];
       double* in = &intr[4 * static_cast<size_t>(ba.edgeCam[e])];
       in[0] = in[1] = p[0];
@@ -189,6 +226,7 @@ int run_g2o_file(int argc, char** argv) {
   g2o_hip::BlockSolverHIP s(std::atoi(argv[4]));
   if (!s.setGraph(C, P, E, ba.edgeCam.data(), ba.edgePoint.data(), ba.camFixed.data(), ba.pointFixed.data()) ||
       !s.setMeasurements(ba.meas.data(), w.data(), intr.data(), 0.0) ||
+      (!scalar && !s.setInformation(ba.omega.data())) ||
       !s.setEstimates(ba.cams.data(), ba.points.data())) {
     std::fprintf(stderr, "setup failed: %s\n", s.lastError().c_str());
     return 1;

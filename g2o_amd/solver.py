@@ -216,6 +216,16 @@ class BundleAdjuster:
         self.ctx.call("g2ohip_get_b", _dp(v))
         return v
 
+    def multiply_hessian(self, src) -> np.ndarray:
+        """BlockSolverBase::multiplyHessian: the pose rows of Hpp src (the
+        landmark rows of the result are zero)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        if s.size != 6 * self.Cf + 3 * self.Pf:
+            raise ValueError("src must hold 6 Cf + 3 Pf values")
+        v = np.zeros_like(s)
+        self.ctx.call("g2ohip_multiply_hessian", _dp(s), _dp(v))
+        return v
+
     def lambda_init(self) -> float:
         v = ct.c_double()
         self.ctx.call("g2ohip_lambda_init", ct.byref(v))
@@ -366,6 +376,16 @@ class PoseGraphOptimizer:
     def x(self) -> np.ndarray:
         v = np.zeros(6 * self.Nf)
         self.ctx.call("g2ohip_get_x", _dp(v))
+        return v
+
+    def multiply_hessian(self, src) -> np.ndarray:
+        """BlockSolverBase::multiplyHessian: H src with the undamped Hessian
+        of the last build_system."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        if s.size != 6 * self.Nf:
+            raise ValueError("src must hold 6 Nf values")
+        v = np.zeros_like(s)
+        self.ctx.call("g2ohip_multiply_hessian", _dp(s), _dp(v))
         return v
 
     def lambda_init(self) -> float:

@@ -164,8 +164,14 @@ int g2ohip_solve(g2ohip_ctx* ctx, int* ok);
 /* Solver::x() / Solver::b() (solver.h:101-108), host copies [6Cf + 3Pf] */
 int g2ohip_get_x(g2ohip_ctx* ctx, double* x);
 int g2ohip_get_b(g2ohip_ctx* ctx, double* b);
-/* BlockSolverBase::multiplyHessian (block_solver.h:90-97) is not needed by LM
- * and not provided. */
+/* BlockSolverBase::multiplyHessian (block_solver.h:90-97, 149-151): dest =
+ * Hpp src, the product with the pose-pose blocks alone, as the reference's
+ * _Hpp->multiplySymmetricUpperTriangle: block diagonal for bundle adjustment
+ * (no camera-camera edges), the whole Hessian of a pose graph. src and dest
+ * hold the system's length ([6Cf + 3Pf], or [6Nf]); only the pose rows of
+ * dest are written. Hpp is that of the last build_system, undamped
+ * (set_lambda's damping lives inside the solve). */
+int g2ohip_multiply_hessian(g2ohip_ctx* ctx, const double* src, double* dest);
 
 /* SparseOptimizer::update(solver.x()) (sparse_optimizer.cpp:457-468) */
 int g2ohip_update(g2ohip_ctx* ctx);

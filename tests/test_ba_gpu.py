@@ -298,3 +298,22 @@ def test_profile_counts(walk):
     _say(f"profile {prof}")
     assert prof["linearize"][1] >= 1 and prof["schur"][1] >= 1 and prof["pcg"][1] >= 1
     assert all(v[0] >= 0 for v in prof.values())
+
+
+@pytest.mark.parametrize("case", ["small", "walk"])
+def test_multiply_hessian_is_the_pose_block_product(case, request):
+    """BlockSolverBase::multiplyHessian (block_solver.h:149-151): the pose rows
+    of Hpp src with the undamped pose blocks of the last buildSystem (block
+    diagonal for bundle adjustment); the landmark rows stay zero."""
+    prob = request.getfixturevalue(case)
+    import g2o_amd
+    ba = _mk(prob, precision=g2o_amd.FP64)
+    ba.build_system()
+    ba.set_lambda(123.0)  # the damping is not part of the product
+    Hpp, _, _ = ba.hessian()
+    rng = np.random.default_rng(4)
+    src = rng.normal(size=6 * ba.Cf + 3 * ba.Pf)
+    out = ba.multiply_hessian(src)
+    ref = np.einsum("cij,cj->ci", Hpp, src[:6 * ba.Cf].reshape(-1, 6)).ravel()
+    assert _rel(out[:6 * ba.Cf], ref) < 1e-14
+    assert np.all(out[6 * ba.Cf:] == 0)

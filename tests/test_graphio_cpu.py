@@ -181,6 +181,51 @@ def test_ba_round_trip(tmp_path):
     assert np.array_equal(r.pts, p.pts * 2)
 
 
+def test_full_information_round_trip(tmp_path):
+    """EdgeSE3ProjectXYZ::read/write take the upper triangle of the 2x2
+    information (edge_project_xyz.cpp:34-42): a problem with per-edge full
+    information saves weight * omega and loads it back as omega (weights 1)."""
+    p = datasets.make_small_ba(seed=1000)
+    p.omega = datasets.random_information(p.num_edges, seed=3)
+    path = str(tmp_path / "ba_info.g2o")
+    gio.save_ba_problem(path, p)
+    q = gio.G2oGraph.load(path).to_ba_problem(p.intr)
+    assert np.array_equal(q.weight, np.ones(p.num_edges))
+    np.testing.assert_allclose(q.omega, p.omega * p.weight[:, None], rtol=1e-15)
+    # scalar information stays a weight
+    p2 = datasets.make_small_ba(seed=1000)
+    p2.weight = np.linspace(0.5, 2.0, p2.num_edges)
+    gio.save_ba_problem(path, p2)
+    q2 = gio.G2oGraph.load(path).to_ba_problem(p2.intr)
+    assert q2.omega is None and np.array_equal(q2.weight, p2.weight)
+
+
+def test_cpp_driver_rejects_a_camera_with_two_parameter_sets(tmp_path):
+    """The solver holds intrinsics per camera while g2o resolves
+    EDGE_PROJECT_XYZ2UV's camera parameters per edge: a camera observed through
+    two different parameter sets is refused before anything runs."""
+    driver = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "g2o_amd", "host",
+                          "lm_drive")
+    if not os.path.exists(driver):
+        pytest.skip("driver not built")
+    p = datasets.make_small_ba(seed=1000)
+    plain = tmp_path / "plain.g2o"
+    gio.save_ba_problem(str(plain), p)
+    lines = ["PARAMS_CAMERAPARAMETERS 0 500 320 240 0.5", "PARAMS_CAMERAPARAMETERS 1 510 320 240 0.5"]
+    n = 0
+    for ln in plain.read_text().splitlines():
+        if ln.startswith("EDGE_SE3_PROJECT_XYZ:EXPMAP "):
+            t = ln.split(" ", 3)
+            ln = f"EDGE_PROJECT_XYZ2UV:EXPMAP {t[1]} {t[2]} {n % 2} {t[3]}"
+            n += 1
+        lines.append(ln)
+    src = tmp_path / "two_params.g2o"
+    src.write_text("\n".join(lines) + "\n")
+    out = subprocess.run([driver, str(src), "optimize", "2", "0"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1
+    assert "two different camera parameter sets" in out.stderr
+
+
 def test_pose_graph_round_trip(tmp_path):
     pg = datasets.make_sphere_pose_graph(nodes_per_level=40, laps=3)
     path = str(tmp_path / "pg.g2o")
@@ -247,8 +292,9 @@ def test_errors(tmp_path):
     path = tmp_path / "aniso.g2o"
     path.write_text("VERTEX_SE3:EXPMAP 0 0 0 0 0 0 0 1\nVERTEX_TRACKXYZ 1 0 0 5\n"
                     "EDGE_SE3_PROJECT_XYZ:EXPMAP 1 0 1 2 1 0.5 1\n")
-    with pytest.raises(gio.G2oIOError, match="not w \\* I"):
-        gio.G2oGraph.load(str(path)).to_ba_problem([500, 500, 320, 240])
+    # an anisotropic information matrix is kept as the edge's full Omega
+    aniso = gio.G2oGraph.load(str(path)).to_ba_problem([500, 500, 320, 240])
+    assert aniso.omega.tolist() == [[1.0, 0.5, 1.0]] and aniso.weight.tolist() == [1.0]
     p = datasets.make_small_ba(seed=1000)
     g = gio.G2oGraph()
     g.add_ba(p)

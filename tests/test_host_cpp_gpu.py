@@ -29,7 +29,18 @@ def write_problem(path, prob, precision, refine):
 def run(path, mode, iters):
     out = subprocess.run([DRIVER, path, mode, str(iters), "0"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
+    if mode == "solver":
+        # the host loop took lambda_0 from the mapped vertex Hessians, as g2o's
+        # computeLambdaInit does, and it equals the device's
+        _lambda_line(out.stderr)
     return np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()])
+
+
+def _lambda_line(stderr):
+    line = next(ln for ln in stderr.splitlines() if ln.startswith("lambda_init vertex_hessians"))
+    v = line.split()
+    assert float(v[2]) == float(v[4]) and float(v[2]) > 0
+    return float(v[2])
 
 
 @pytest.mark.parametrize("precision,refine", [(g2o_amd.FP64, 0), (g2o_amd.FP32, 1)])
@@ -104,6 +115,33 @@ def test_pose_graph_file_through_the_cpp_driver(tmp_path, mode, algorithm):
     saved = gio.G2oGraph.load(dst).to_pose_graph()
     poses = opt.state()
     assert np.max(np.abs(saved.poses[:, :3] - poses[:, :3])) < 1e-9 * max(1.0, np.max(np.abs(poses[:, :3])))
+
+
+def test_full_information_file_through_the_solver_contract_loop(tmp_path):
+    """A file whose projection edges carry full 2x2 information matrices
+    (EdgeSE3ProjectXYZ::read, edge_project_xyz.cpp:34-42) through GraphIO and
+    the host LM loop over BlockSolverHIP: lambda_0 from the vertex Hessians
+    mapped after buildSystem, and the trace equal to the Python mirror's on the
+    same problem."""
+    from g2o_amd import graphio as gio
+    prob = datasets.make_small_ba(seed=1000)
+    prob.omega = datasets.random_information(prob.num_edges, seed=9)
+    fx, fy, cx, cy = (float(v) for v in prob.intr[0])
+    src = str(tmp_path / "info.g2o")
+    gio.save_ba_problem(src, prob)
+    out = subprocess.run([DRIVER, src, "solver", "6", "0", repr(fx), repr(fy), repr(cx), repr(cy)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lam0 = _lambda_line(out.stderr)
+    a = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()])
+    ba = g2o_amd.BundleAdjuster(gio.G2oGraph.load(src).to_ba_problem(prob.intr), device=0,
+                                precision=g2o_amd.FP64)
+    ba.build_system()
+    assert ba.lambda_init() == lam0
+    ba.set_state(prob.cams, prob.pts)
+    st = ba.optimize(6)
+    np.testing.assert_allclose(a[:, 2], [s["chi2"] for s in st], rtol=1e-10)
+    np.testing.assert_array_equal(a[:, 4], [s["levenberg_iterations"] for s in st])
 
 
 def test_xyz2uv_file_through_the_cpp_driver(tmp_path):

@@ -152,3 +152,22 @@ def test_sphere_10k_poses_matches_oracle():
     gt = prob.gt_poses
     err = np.linalg.norm(pgo.state()[:, :3] - gt[:, :3], axis=1)
     assert err.max() < 0.5
+
+
+def test_multiply_hessian_is_the_hessian_product():
+    """BlockSolverBase::multiplyHessian on the pose graph: H src with the whole
+    undamped Hessian of the last buildSystem (both triangles)."""
+    prob = datasets.make_sphere_pose_graph(seed=3, nodes_per_level=40, laps=4)
+    pgo = g2o_amd.PoseGraphOptimizer(prob, device=0)
+    pgo.build_system()
+    rows, cols, blocks, _ = pgo.system()
+    n = 6 * pgo.Nf
+    H = np.zeros((n, n))
+    for r, c, B in zip(rows, cols, blocks):
+        H[6 * r:6 * r + 6, 6 * c:6 * c + 6] = B
+        if r != c:
+            H[6 * c:6 * c + 6, 6 * r:6 * r + 6] = B.T
+    src = np.random.default_rng(5).normal(size=n)
+    out = pgo.multiply_hessian(src)
+    ref = H @ src
+    assert np.max(np.abs(out - ref)) <= 1e-12 * np.max(np.abs(ref))
