@@ -119,21 +119,28 @@ class BlockSolverHIP {
   bool multiplyHessian(double* dest, const double* src) {
     return _dev.ok(g2ohip_multiply_hessian(_dev.get(), src, dest));
   }
-  // The vertex's Hessian block (row-major 6x6 for a camera, 3x3 for a point)
-  // of the last buildSystem, or nullptr for a vertex without one (fixed, or
-  // without edges). g2o's BlockSolver::buildStructure maps each vertex's
+  // Vertex Hessians. g2o's BlockSolver::buildStructure maps each vertex's
   // _hessian onto its diagonal block (block_solver.hpp:144-161), which
   // OptimizationAlgorithmLevenberg::computeLambdaInit reads
-  // (optimization_algorithm_levenberg.cpp:155-167); here the blocks are
-  // mirrored on the host, copied on the first access after a buildSystem.
-  const double* vertexHessian(bool camera, int index) {
+  // (optimization_algorithm_levenberg.cpp:155-167). Here the blocks live in a
+  // host mirror with a stable address per vertex from setGraph on
+  // (vertexHessianMemory: row-major 6x6 for a camera, 3x3 for a point, nullptr
+  // for a vertex without a block - fixed, or without edges), filled from the
+  // last buildSystem by syncVertexHessians; vertexHessian syncs on demand.
+  double* vertexHessianMemory(bool camera, int index) {
     const std::vector<int>& idx = camera ? _camIndex : _pointIndex;
     if (index < 0 || index >= static_cast<int>(idx.size()) || idx[index] < 0) return nullptr;
-    if (_hessianStale) {
-      if (!_dev.ok(g2ohip_ba_get_hessian(_dev.get(), _hpp.data(), _hll.data(), nullptr))) return nullptr;
-      _hessianStale = false;
-    }
     return camera ? &_hpp[36 * static_cast<size_t>(idx[index])] : &_hll[9 * static_cast<size_t>(idx[index])];
+  }
+  bool syncVertexHessians() {
+    if (!_hessianStale) return true;
+    if (!_dev.ok(g2ohip_ba_get_hessian(_dev.get(), _hpp.data(), _hll.data(), nullptr))) return false;
+    _hessianStale = false;
+    return true;
+  }
+  const double* vertexHessian(bool camera, int index) {
+    double* m = vertexHessianMemory(camera, index);
+    return m && syncVertexHessians() ? m : nullptr;
   }
   int numCameras() const { return static_cast<int>(_camIndex.size()); }
   int numPoints() const { return static_cast<int>(_pointIndex.size()); }
