@@ -169,3 +169,45 @@ def test_differentiable_ba_200_cams_finite_differences():
     relwo = np.linalg.norm(dw - dw_o) / np.linalg.norm(dw_o)
     print(f"vs oracle: rel err meas {relo:.3e}, weights {relwo:.3e}", flush=True)
     assert relo <= 1e-6 and relwo <= 1e-6
+
+
+def test_dropped_observations_have_zero_gradient():
+    """The differentiable solve conditions the problem first: outlier edges
+    (robust chi2 above the 95 % bound) and landmarks left ill-conditioned are
+    dropped before the implicit differentiation, as g2o workflows move outlier
+    edges to level 1. The full pipeline's loss therefore does not depend on a
+    dropped observation while the selection stays the same: central
+    differences through the whole pipeline (re-running the conditioning and
+    the solve) on a sample of dropped measurements give zero, next to the
+    kept edges' adjoint gradient. Prints the kept fraction."""
+    import time
+    from g2o_amd import diff
+    prob = datasets.make_diff_ba()
+    t = time.perf_counter()
+    sub, ba = diff.prepare(prob)
+    cams0, _ = ba.state()
+    L0, gcam = centre_loss(cams0, prob.gt_cams)
+    dz, _, _ = ba.adjoint(gcam)
+    kept = sub.info["edge_index"]
+    dropped = np.setdiff1d(np.arange(prob.num_edges), kept)
+    print(f"\nkept {kept.size} of {prob.num_edges} observations ({kept.size / prob.num_edges:.1%}), "
+          f"{sub.num_pts} of {prob.num_pts} points; prepare {time.perf_counter() - t:.1f} s", flush=True)
+    assert 0.5 < kept.size / prob.num_edges < 1.0
+    rng = np.random.default_rng(8)
+    sample = rng.choice(dropped, size=3, replace=False)
+    h = 1e-3
+    fd = []
+    for k in sample:
+        vals = []
+        for sgn in (1.0, -1.0):
+            p = prob.copy()
+            p.meas = prob.meas.copy()
+            p.meas[k, 0] += sgn * h
+            s2, b2 = diff.prepare(p)
+            assert np.array_equal(s2.info["edge_index"], kept)  # the same selection
+            vals.append(centre_loss(b2.state()[0], prob.gt_cams)[0])
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    scale = float(np.median(np.abs(dz[:, 0])))
+    print(f"FD of the loss on dropped measurements {np.abs(fd).max():.2e}; median |dL/dz| of kept ones {scale:.2e}",
+          flush=True)
+    assert np.abs(fd).max() <= 1e-3 * scale
