@@ -67,7 +67,7 @@ def stage_bytes(info, C):
     }
 
 
-SPMV_SYMBOL = "k_pcg_dir_spmv<float, 6, false, true>"
+SPMV_SYMBOL = "k_pcg_dir_spmv<float, 6, false, true, 7>"
 STAGE_KERNELS = {
     "linearize": ["k_linearize<float>"],
     "schur": ["k_point_linv", "k_schur_win<float, 512>", "k_schur_offdiag<float>", "k_schur_diag_pm",
@@ -374,6 +374,8 @@ def main():
             "iter_per_s": round(len(si) / dti, 3), "ms_per_step": round(dti / len(si) * 1e3, 3),
             "rel_final_chi2_diff_vs_fp64": abs(si[-1]["chi2"] - s64[-1]["chi2"]) / s64[-1]["chi2"],
             "pcg_iterations": pcg_i,
+            "fp64_fallback": [s["fp64_fallback"] for s in si],
+            "refine_steps": [s["refine_steps"] for s in si],
             "us_per_pcg_iteration": round(profi["pcg"][0] * 1e3 / max(pcg_i, 1), 2),
             "kernel_ms": {k: round(v[0], 3) for k, v in profi.items() if k in g2o_amd.KERNEL_NAMES}}}
         log(f"implicit: {out['legs']['implicit_schur']}")

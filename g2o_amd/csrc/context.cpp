@@ -552,9 +552,14 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
     auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
       hipError_t e = launch_pcg_dir<6>(nb, B.s.p, d_old, d, d_dn_part.p, d_sc.p, ctl, it, last, stream);
       if (e != hipSuccess || last) return e;
-      e = launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, d, nullptr, d_cam_part.p, d_sc.p, it, stream);
+      // FP32: each camera's own block from the FP64 diagonal (as the explicit
+      // FP32 operator keeps it), the FP32 passes carry the cross terms only
+      constexpr bool f32 = std::is_same<HT, float>::value;
+      e = launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, d, nullptr, d_cam_part.p, d_sc.p, it, stream, f32,
+                                   f32 ? d_linv.p : nullptr);
       if (e != hipSuccess) return e;
-      return launch_implicit_reduce<HT>(g, d_cam_part.p, d_Hpp.p, lambda, d, B.q.p, d_dq_part.p, d_sc.p, it, stream);
+      return launch_implicit_reduce<HT>(g, d_cam_part.p, d_Hpp.p, lambda, d, B.q.p, d_dq_part.p, d_sc.p, it, stream,
+                                        f32 ? d_Sdiag.p : nullptr);
     };
     HK((pcg_drive<HT, 6>(stream, h_pinned, progress(), nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
                          d_dn_part.p, d_dq_part.p, implicit_reduce_blocks(nb), d_sc.p, max_iter, op, out,
@@ -596,6 +601,10 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
     if (implicit) {
       HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
       HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
+      if (std::is_same<HT, float>::value) {  // the FP32 operator's FP64 point factors
+        HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
+        HK(launch_point_linv(g, d_Hll.p, lambda, d_linv.p, d_err.p, stream));
+      }
     } else {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
       HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
@@ -605,13 +614,19 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
     // diagonal blocks, block-Jacobi inverses and (explicit) the reduced
     // right-hand side in FP64, re-linearised from the state point-major (FP32
     // blocks are too coarse for the cancellation in Hpp - sum V Dinv V^T)
-    // FP32 blocks: the diagonal blocks stay FP64 (d_Sdiag) for the SpMV
-    double* sdiag = (!implicit && std::is_same<HT, float>::value) ? d_Sdiag.p : nullptr;
-    HK(launch_schur_diag_pm<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, implicit ? nullptr : d_b.p, lambda,
+    // FP32 blocks: the diagonal blocks stay FP64 (d_Sdiag) for the SpMV and
+    // for the implicit operator
+    // (the FP32 implicit path takes its reduced right-hand side from here
+    // too: b_p - sum V Dinv b_l cancels towards convergence, and an FP32 Hpl
+    // would leave it mostly rounding error)
+    constexpr bool f32 = std::is_same<HT, float>::value;
+    double* sdiag = f32 ? d_Sdiag.p : nullptr;
+    const bool rhs_here = !implicit || f32;
+    HK(launch_schur_diag_pm<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, rhs_here ? d_b.p : nullptr, lambda,
                                 d_diag_part.p, implicit ? nullptr : B.F_val.p, B.Jinv.p,
-                                implicit ? nullptr : d_bschur.p, sdiag, d_err.p, stream));
+                                rhs_here ? d_bschur.p : nullptr, sdiag, d_err.p, stream));
   }
-  if (implicit) {
+  if (implicit && !std::is_same<HT, float>::value) {
     // matrix-free reduced operator: right-hand side by one pass
     HK(launch_implicit_pass<HT>(g, B.Hpl.p, B.Dinv.p, nullptr, d_b.p, d_cam_part.p, nullptr, 0, stream));
     HK(launch_implicit_rhs_reduce(g, d_cam_part.p, d_b.p, d_bschur.p, stream));

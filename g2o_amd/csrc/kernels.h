@@ -170,11 +170,15 @@ template <typename HT>
 hipError_t launch_point_dinv(const DevGraph& g, const double* Hll, double lambda, HT* Dinv, int* err, hipStream_t st);
 template <typename HT>
 hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const double* p, const double* b,
-                                double* cam_part, const PcgScalars* sc, int it, hipStream_t st);
+                                double* cam_part, const PcgScalars* sc, int it, hipStream_t st,
+                                bool exclude_own = false, const double* Linv = nullptr);
+// Sdiag (optional): FP64 diagonal blocks of the damped reduced system,
+// applied in place of (Hpp + lambda I) when the passes excluded each
+// camera's own terms (exclude_own)
 template <typename HT>
 hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, const double* Hpp, double lambda,
                                   const double* p, double* q, double* dq_part, PcgScalars* sc, int it,
-                                  hipStream_t st);
+                                  hipStream_t st, const double* Sdiag = nullptr);
 hipError_t launch_implicit_rhs_reduce(const DevGraph& g, const double* cam_part, const double* b, double* bschur,
                                       hipStream_t st);
 

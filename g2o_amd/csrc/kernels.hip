@@ -929,8 +929,27 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
                                                               const HT* __restrict__ Dinv,
                                                               const double* __restrict__ p,
                                                               const double* __restrict__ b,
-                                                              double* __restrict__ cam_part, PcgScalars* sc, int it) {
+                                                              double* __restrict__ cam_part, PcgScalars* sc, int it,
+                                                              int exclude_own, const double* __restrict__ Linv) {
   if (sc && sc->done) return;
+  // y = (Hll + lambda I)^-1 v: from the FP64 factor L^-1 (packed lower
+  // triangle, k_point_linv) as L^-T (L^-1 v) when given - the Gram form keeps
+  // the product accurate for ill-conditioned points, where rounding the
+  // inverse's entries to FP32 would not - else from the stored inverse
+  auto dinv_apply = [&](int ph, const double* v, double* y) {
+    if (Linv) {
+      const double* m = Linv + 6 * static_cast<size_t>(ph);
+      const double u0 = m[0] * v[0], u1 = m[1] * v[0] + m[2] * v[1], u2 = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+      y[0] = m[0] * u0 + m[1] * u1 + m[3] * u2;
+      y[1] = m[2] * u1 + m[4] * u2;
+      y[2] = m[5] * u2;
+    } else {
+      const HT* D = Dinv + 9 * static_cast<size_t>(ph);
+      for (int r = 0; r < 3; ++r)
+        y[r] = static_cast<double>(D[3 * r]) * v[0] + static_cast<double>(D[3 * r + 1]) * v[1] +
+               static_cast<double>(D[3 * r + 2]) * v[2];
+    }
+  };
   // camera accumulators (6 per chunk camera) and the chunk's camera list
   extern __shared__ double s_dyn[];
   double* const s_cacc = s_dyn;
@@ -963,7 +982,6 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
   for (int lp = tid; lp < np; lp += LIN_THREADS) {
     const int ph = g.pos_h[p0 + lp];
     if (ph < 0) continue;
-    const HT* D = Dinv + 9 * static_cast<size_t>(ph);
     double v[3];
     if (p) {
       v[0] = s_v[3 * lp]; v[1] = s_v[3 * lp + 1]; v[2] = s_v[3 * lp + 2];
@@ -971,16 +989,39 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
       const double* bl = b + 6 * static_cast<size_t>(g.Cf) + 3 * static_cast<size_t>(ph);
       v[0] = bl[0]; v[1] = bl[1]; v[2] = bl[2];
     }
-    for (int r = 0; r < 3; ++r)
-      s_v[3 * lp + r] = static_cast<double>(D[3 * r]) * v[0] + static_cast<double>(D[3 * r + 1]) * v[1] +
-                        static_cast<double>(D[3 * r + 2]) * v[2];
+    double y[3];
+    dinv_apply(ph, v, y);
+    for (int r = 0; r < 3; ++r) s_v[3 * lp + r] = y[r];
   }
   __syncthreads();
   for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
     const int pos = g.slot_pos[s];
     const int chh = g.camH[g.slot_cam[s]];
     if (g.pos_h[pos] < 0 || chh < 0) continue;
-    const double* y = s_v + 3 * (pos - p0);
+    double y[3] = {s_v[3 * (pos - p0)], s_v[3 * (pos - p0) + 1], s_v[3 * (pos - p0) + 2]};
+    if (exclude_own && p) {
+      // only the other cameras' terms: y = Dinv (sum_j V_j^T p_j - sum over
+      // this camera's slots of the point V^T p_i); the camera's own block is
+      // applied from the FP64 diagonal (k_implicit_reduce). The point's
+      // slots are sorted by camera, so its slots of this camera are adjacent.
+      const double* pc = p + 6 * static_cast<size_t>(chh);
+      const int sb = g.pos_ptr[pos], se = g.pos_ptr[pos + 1];
+      int a = s, e = s + 1;
+      while (a > sb && g.camH[g.slot_cam[a - 1]] == chh) --a;
+      while (e < se && g.camH[g.slot_cam[e]] == chh) ++e;
+      double own[3] = {0.0, 0.0, 0.0};
+      for (int s2 = a; s2 < e; ++s2) {
+        const HT* h2 = Hpl + 18 * static_cast<size_t>(g.slot_q[s2]);
+        for (int c = 0; c < 3; ++c) {
+          double v = 0;
+          for (int r = 0; r < 6; ++r) v += static_cast<double>(h2[3 * r + c]) * pc[r];
+          own[c] += v;
+        }
+      }
+      double yo[3];
+      dinv_apply(g.pos_h[pos], own, yo);
+      for (int r = 0; r < 3; ++r) y[r] -= yo[r];
+    }
     const int lc = lower_bound_i(s_cam, nc, chh);
     for (int r = 0; r < 6; ++r) {
       const HT* h = Hpl + 18 * static_cast<size_t>(g.slot_q[s]) + 3 * r;
@@ -995,22 +1036,29 @@ __global__ __launch_bounds__(LIN_THREADS) void k_implicit_pass(DevGraph g, const
 
 template <typename HT>
 hipError_t launch_implicit_pass(const DevGraph& g, const HT* Hpl, const HT* Dinv, const double* p, const double* b,
-                                double* cam_part, const PcgScalars* sc, int it, hipStream_t st) {
+                                double* cam_part, const PcgScalars* sc, int it, hipStream_t st, bool exclude_own,
+                                const double* Linv) {
   const size_t lds = (6 * sizeof(double) + sizeof(int)) * static_cast<size_t>(g.lc_maxc);
   if (!lds_fits(k_implicit_pass<HT>, lds)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_implicit_pass<HT>, dim3(g.nlc), dim3(LIN_THREADS), lds, st, g, Hpl, Dinv, p, b, cam_part,
-                     const_cast<PcgScalars*>(sc), it);
+                     const_cast<PcgScalars*>(sc), it, exclude_own ? 1 : 0, Linv);
   return hipGetLastError();
 }
 template hipError_t launch_implicit_pass<float>(const DevGraph&, const float*, const float*, const double*,
-                                                const double*, double*, const PcgScalars*, int, hipStream_t);
+                                                const double*, double*, const PcgScalars*, int, hipStream_t, bool,
+                                                const double*);
 template hipError_t launch_implicit_pass<double>(const DevGraph&, const double*, const double*, const double*,
-                                                 const double*, double*, const PcgScalars*, int, hipStream_t);
+                                                 const double*, double*, const PcgScalars*, int, hipStream_t, bool,
+                                                 const double*);
 
-// q = (Hpp + lambda I) p - sum of chunk partials; dq_part[block] = sum of q_c . p_c
+// q = (Hpp + lambda I) p - sum of chunk partials, or with Sdiag (the FP64
+// diagonal blocks of the damped reduced system, the passes then carrying
+// only the other cameras' terms) q = Sdiag p - sum of chunk partials;
+// dq_part[block] = sum of q_c . p_c
 template <typename HT>
 __global__ __launch_bounds__(64) void k_implicit_reduce(DevGraph g, const double* __restrict__ cam_part,
                                                         const double* __restrict__ Hpp, double lambda,
+                                                        const double* __restrict__ Sdiag,
                                                         const double* __restrict__ p, double* __restrict__ q,
                                                         double* __restrict__ dq_part, const PcgScalars* sc) {
   if (sc->done) return;
@@ -1019,9 +1067,10 @@ __global__ __launch_bounds__(64) void k_implicit_reduce(DevGraph g, const double
   if (c < g.Cf) {
     double pc[6], out[6];
     for (int r = 0; r < 6; ++r) pc[r] = p[6 * static_cast<size_t>(c) + r];
+    const double* A = Sdiag ? Sdiag : Hpp;
     for (int r = 0; r < 6; ++r) {
-      double v = lambda * pc[r];
-      for (int k = 0; k < 6; ++k) v += Hpp[36 * static_cast<size_t>(c) + 6 * r + k] * pc[k];
+      double v = Sdiag ? 0.0 : lambda * pc[r];
+      for (int k = 0; k < 6; ++k) v += A[36 * static_cast<size_t>(c) + 6 * r + k] * pc[k];
       out[r] = v;
     }
     for (int q2 = g.camp_ptr[c]; q2 < g.camp_ptr[c + 1]; ++q2) {
@@ -1040,16 +1089,18 @@ __global__ __launch_bounds__(64) void k_implicit_reduce(DevGraph g, const double
 template <typename HT>
 hipError_t launch_implicit_reduce(const DevGraph& g, const double* cam_part, const double* Hpp, double lambda,
                                   const double* p, double* q, double* dq_part, PcgScalars* sc, int it,
-                                  hipStream_t st) {
+                                  hipStream_t st, const double* Sdiag) {
   (void)it;
   hipLaunchKernelGGL(k_implicit_reduce<HT>, dim3(implicit_reduce_blocks(g.Cf)), dim3(64), 0, st, g, cam_part, Hpp,
-                     lambda, p, q, dq_part, sc);
+                     lambda, Sdiag, p, q, dq_part, sc);
   return hipGetLastError();
 }
 template hipError_t launch_implicit_reduce<float>(const DevGraph&, const double*, const double*, double,
-                                                  const double*, double*, double*, PcgScalars*, int, hipStream_t);
+                                                  const double*, double*, double*, PcgScalars*, int, hipStream_t,
+                                                  const double*);
 template hipError_t launch_implicit_reduce<double>(const DevGraph&, const double*, const double*, double,
-                                                   const double*, double*, double*, PcgScalars*, int, hipStream_t);
+                                                   const double*, double*, double*, PcgScalars*, int, hipStream_t,
+                                                   const double*);
 
 __global__ void k_implicit_rhs_reduce(DevGraph g, const double* __restrict__ cam_part, const double* __restrict__ b,
                                       double* __restrict__ bschur) {

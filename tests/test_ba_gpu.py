@@ -151,10 +151,18 @@ def test_lm_trace_fp64(case, request, oracle_lib):
     _say(f"{case}: gpu chi2 {[s['chi2'] for s in stats]}")
     _say(f"{case}: cpu chi2 {list(trace[:, 1])}")
     assert len(stats) == n_o
+    # PCG counts: equal, or one apart where the stop test dn <= d0 compares two
+    # values within rounding of each other (the device's sums round
+    # differently from the oracle's, and the linearisation's LDS sums
+    # differently from run to run); the first solve's count is exact
+    assert stats[0]["iterations_linear_solver"] == int(trace[0, 4])
+    off = 0
     for s, t in zip(stats, trace):
         assert abs(s["chi2"] - t[1]) <= 1e-8 * t[1]
         assert s["levenberg_iterations"] == int(t[3])
-        assert s["iterations_linear_solver"] == int(t[4])
+        assert abs(s["iterations_linear_solver"] - int(t[4])) <= 1
+        off += s["iterations_linear_solver"] != int(t[4])
+    assert off <= 1
     c, p = ba.state()
     co, po = ora.state()
     # inexact PCG (tol 1e-6) lets rounding differences grow slowly over 8 iterations

@@ -137,7 +137,9 @@ def test_full_information_file_through_the_solver_contract_loop(tmp_path):
     ba = g2o_amd.BundleAdjuster(gio.G2oGraph.load(src).to_ba_problem(prob.intr), device=0,
                                 precision=g2o_amd.FP64)
     ba.build_system()
-    assert ba.lambda_init() == lam0
+    # another context: its linearisation's LDS sums may round the diagonal
+    # differently in the last bit
+    assert ba.lambda_init() == pytest.approx(lam0, rel=1e-14)
     ba.set_state(prob.cams, prob.pts)
     st = ba.optimize(6)
     np.testing.assert_allclose(a[:, 2], [s["chi2"] for s in st], rtol=1e-10)
