@@ -79,7 +79,8 @@ def solve(meas: torch.Tensor, weight: torch.Tensor, ba: BundleAdjuster, init_cam
 
 def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine_steps: int = 1,
             lm_iterations: int = 30, chi2_max: float | None = None, min_obs: int = 3, max_cond: float = 1e6,
-            newton_steps: int = 12, polish_iterations: int = 100, polish_tolerance: float = 1e-12):
+            newton_steps: int = 12, polish_iterations: int = 100, polish_tolerance: float = 1e-12,
+            keep_edges: np.ndarray | None = None):
     """Conditions a robust BA problem for implicit differentiation and solves it.
 
     The implicit function theorem needs a strict stationary point with a
@@ -95,6 +96,11 @@ def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine
     exact-Hessian Newton steps until the gradient reaches its rounding floor
     (info['gradient_max']).
 
+    keep_edges (indices into the input's edges, e.g. an earlier call's
+    info['edge_index']): the selection is taken as given instead of from the
+    robust chi2 and the conditioning test; the robust LM on every edge still
+    runs first and provides the start state.
+
     Returns (sub_problem, BundleAdjuster at the stationary point). The
     sub-problem's info['edge_index'] maps its edges to the input's."""
     ba = BundleAdjuster(prob, device=device, precision=precision, refine_steps=refine_steps)
@@ -104,7 +110,12 @@ def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine
     chi = prob.weight * (e * e).sum(1)
     if chi2_max is None:
         chi2_max = 5.991 * prob.huber ** 2 if prob.huber > 0 else np.inf
-    sub = select_edges(prob, chi <= chi2_max, min_obs)
+    if keep_edges is not None:
+        mask = np.zeros(prob.num_edges, dtype=bool)
+        mask[np.asarray(keep_edges)] = True
+    else:
+        mask = chi <= chi2_max
+    sub = select_edges(prob, mask, min_obs)
     sub.cams, sub.pts = cams, pts[sub.info["pt_index"]]
     ba = BundleAdjuster(sub, device=device, precision=precision, refine_steps=refine_steps,
                         pcg_tolerance=polish_tolerance)
@@ -119,7 +130,7 @@ def prepare(prob: BAProblem, device: int = 0, precision: int = _lib.FP32, refine
     free = np.flatnonzero(sub.pt_fixed == 0)
     weak = np.zeros(sub.num_pts, dtype=bool)
     weak[free] = ev[:, 0] <= ev[:, 2] / max_cond
-    if weak.any():
+    if weak.any() and keep_edges is None:
         cams, pts = ba.state()
         eidx = sub.info["edge_index"]
         sub2 = select_edges(sub, ~weak[sub.edge_pt], min_obs)
