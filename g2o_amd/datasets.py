@@ -543,12 +543,39 @@ class BALProblem:
     """BAL data (g2o/examples/bal/bal_example.cpp:70-207): cams [C, 9] =
     angle-axis rotation, translation, f, k1, k2 (world to camera, P = R X + t,
     the camera looking down its -z axis); pts [P, 3]; observations in file
-    order, meas [E, 2] pixels relative to the image centre."""
+    order, meas [E, 2] pixels relative to the image centre. Optional, for the
+    solver (the BAL file format has none of them): weight [E] (information
+    weight * I2, None = 1), cam_fixed / pt_fixed 0/1 masks (None = all free,
+    as bal_example.cpp leaves them), huber (RobustKernelHuber delta, 0 = none).
+    gt_cams / gt_pts: the generator's ground truth, if any."""
     cams: np.ndarray
     pts: np.ndarray
     edge_cam: np.ndarray
     edge_pt: np.ndarray
     meas: np.ndarray
+    weight: np.ndarray | None = None
+    cam_fixed: np.ndarray | None = None
+    pt_fixed: np.ndarray | None = None
+    huber: float = 0.0
+    name: str = ""
+    gt_cams: np.ndarray | None = None
+    gt_pts: np.ndarray | None = None
+
+    @property
+    def num_cams(self) -> int:
+        return int(self.cams.shape[0])
+
+    @property
+    def num_pts(self) -> int:
+        return int(self.pts.shape[0])
+
+    @property
+    def num_edges(self) -> int:
+        return int(self.edge_cam.shape[0])
+
+    def copy(self) -> "BALProblem":
+        return BALProblem(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                             for k, v in self.__dict__.items()})
 
 
 def make_bal(seed: int = 11, num_cams: int = 8, num_pts: int = 300, k=(0.0, 0.0), noise: float = 0.5) -> BALProblem:
@@ -576,6 +603,56 @@ def make_bal(seed: int = 11, num_cams: int = 8, num_pts: int = 300, k=(0.0, 0.0)
     obs = np.array(obs)
     return BALProblem(np.array(cams), pts, obs[:, 0].astype(np.int32), obs[:, 1].astype(np.int32),
                       obs[:, 2:4].copy())
+
+
+def rot_to_angle_axis(R: np.ndarray) -> np.ndarray:
+    """Angle-axis vectors (..., 3) of rotation matrices (..., 3, 3), through
+    the unit quaternion (robust up to angles of pi)."""
+    q = rot_to_quat(R).reshape(-1, 4)
+    s = np.linalg.norm(q[:, :3], axis=1)
+    ang = 2.0 * np.arctan2(s, q[:, 3])
+    scale = np.where(s > 1e-12, ang / np.where(s > 1e-12, s, 1.0), 2.0)
+    return (q[:, :3] * scale[:, None]).reshape(np.shape(R)[:-2] + (3,))
+
+
+def bal_project(cams9: np.ndarray, pts: np.ndarray, ec: np.ndarray, ep: np.ndarray) -> np.ndarray:
+    """BAL projection f r(p) p, p = -P.xy / P.z, P = R(w) X + t, per observation."""
+    c = cams9[ec]
+    P = np.einsum("eij,ej->ei", axis_angle_to_rot(c[:, 0:3]), pts[ep]) + c[:, 3:6]
+    p = -P[:, 0:2] / P[:, 2:3]
+    r2 = np.sum(p * p, axis=1)
+    r = 1 + c[:, 7] * r2 + c[:, 8] * r2 * r2
+    return c[:, 6:7] * r[:, None] * p
+
+
+def make_walk_bal(seed: int = 1005, num_cams: int = 1500, num_pts: int = 500_000, mean_track: float = 6.0,
+                  k_sigma=(0.05, 0.01), noise: float = 1.0, sig_f: float = 0.005) -> BALProblem:
+    """BAL-model counterpart of make_walk_ba: the same random-walk cameras,
+    tracks and initial perturbations (no outliers, no robust kernel, nothing
+    fixed, as bal_example.cpp runs), each camera turned to look down its -z
+    axis (a half turn about x) and given a focal length f = fx and radial
+    distortion k1 ~ N(0, k_sigma[0]), k2 ~ N(0, k_sigma[1]). Measurements are
+    the distorted projections of the ground truth plus N(0, noise) pixels,
+    relative to the image centre. The initial cameras start from f (1 +
+    N(0, sig_f)) and k1 = k2 = 0."""
+    ba = make_walk_ba(seed=seed, num_cams=num_cams, num_pts=num_pts, mean_track=mean_track, outlier_ratio=0.0,
+                      huber=0.0)
+    rng = np.random.default_rng(seed + 7)
+    C = ba.num_cams
+    flip = np.diag([1.0, -1.0, -1.0])
+
+    def to_bal(cams7, f, k):
+        R = flip @ quat_to_rot(cams7[:, 3:7])
+        t = cams7[:, 0:3] @ flip.T
+        return np.concatenate([rot_to_angle_axis(R), t, f[:, None], k], axis=1)
+
+    f = ba.intr[:, 0].copy()
+    k = np.stack([rng.normal(0, k_sigma[0], C), rng.normal(0, k_sigma[1], C)], axis=1)
+    gt = to_bal(ba.gt_cams, f, k)
+    meas = bal_project(gt, ba.gt_pts, ba.edge_cam, ba.edge_pt) + rng.normal(0, noise, size=(ba.num_edges, 2))
+    init = to_bal(ba.cams, f * (1 + rng.normal(0, sig_f, C)), np.zeros((C, 2)))
+    return BALProblem(init, ba.pts.copy(), ba.edge_cam.copy(), ba.edge_pt.copy(), meas,
+                      name=f"walk_bal_{C}c_{ba.num_pts}p", gt_cams=gt, gt_pts=ba.gt_pts.copy())
 
 
 def random_information(num_edges: int, seed: int, lo: float = 0.3, hi: float = 3.0) -> np.ndarray:

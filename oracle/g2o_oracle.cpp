@@ -21,6 +21,9 @@
 //   g2o/core/optimization_algorithm_levenberg.cpp:60-176  LM control
 //   g2o/solvers/pcg/linear_solver_pcg.hpp:87-205 block-Jacobi PCG and its stop rule
 //   g2o/solvers/eigen/linear_solver_eigen.h:94-191 (semantics: FP64 Cholesky, false on non-PD)
+//   BAL: g2o/examples/bal/bal_example.cpp:70-207 (9-DoF camera, additive
+//        updates, the templated error functor with dual-number Jacobians
+//        in place of auto_differentiation.h:155-302), BlockSolver<9, 3>
 //   pose graph: g2o/types/slam3d/edge_se3.cpp:60-88, vertex_se3.h:99-108,
 //               isometry3d_mappings.cpp:80-128, isometry3d_gradients.h:206-279
 //
@@ -279,9 +282,11 @@ inline bool inv3(const double m[9], double out[9]) {
   return std::isfinite(id);
 }
 
-// n x n inverse (row-major) by Gauss-Jordan elimination with partial pivoting.
+// n x n inverse (row-major, n <= 9) by Gauss-Jordan elimination with
+// partial pivoting.
 inline bool invN(int n, const double* m, double* out) {
-  double a[6 * 12];
+  double a[9 * 18];
+  if (n > 9) return false;
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < 2 * n; ++j)
       a[i * 2 * n + j] = (j < n) ? m[i * n + j] : ((j - n == i) ? 1.0 : 0.0);
@@ -380,7 +385,7 @@ void sym_mult(const BlockSym& A, const FullRows& F, const double* src, double* d
   const int bd = A.bd;
 #pragma omp parallel for schedule(static) if (A.nb > 256)
   for (int i = 0; i < A.nb; ++i) {
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // bd <= 9
     for (int p = F.ptr[i]; p < F.ptr[i + 1]; ++p) {
       int k = F.blk[p];
       const double* m = &A.val[static_cast<size_t>(k) * bd * bd];
@@ -1336,6 +1341,376 @@ struct PGLM {
   void discard() { p.stackT.pop_back(); p.stackQ.pop_back(); }
 };
 
+
+// ------------------------------------------------------------- BAL (9-DoF)
+// g2o/examples/bal/bal_example.cpp:70-207: VertexCameraBAL holds 9 values
+// [w (angle-axis), t, f, k1, k2] with oplus = plain addition (:86-89),
+// VertexPointBAL a 3-vector (+), EdgeObservationBAL the error
+// prediction - measurement of the templated functor (:154-204), whose
+// Jacobians the reference takes by automatic differentiation
+// (G2O_MAKE_AUTO_AD_FUNCTIONS, g2o/core/auto_differentiation.h:155-302).
+// Here the same functor is evaluated on forward-mode dual numbers carrying
+// the derivatives along the 12 variables (9 camera + 3 point): exact
+// derivatives of the same arithmetic, independent of the closed form the HIP
+// kernels use. Solved by BlockSolver<BlockSolverTraits<9, 3>> (:227-250):
+// the Schur complement on 9x9 camera blocks, Cholesky or block-Jacobi PCG.
+
+template <int N>
+struct Jet {
+  double a = 0;
+  double v[N] = {};
+};
+template <int N> inline Jet<N> operator+(const Jet<N>& x, const Jet<N>& y) {
+  Jet<N> r; r.a = x.a + y.a; for (int i = 0; i < N; ++i) r.v[i] = x.v[i] + y.v[i]; return r;
+}
+template <int N> inline Jet<N> operator-(const Jet<N>& x, const Jet<N>& y) {
+  Jet<N> r; r.a = x.a - y.a; for (int i = 0; i < N; ++i) r.v[i] = x.v[i] - y.v[i]; return r;
+}
+template <int N> inline Jet<N> operator-(const Jet<N>& x) {
+  Jet<N> r; r.a = -x.a; for (int i = 0; i < N; ++i) r.v[i] = -x.v[i]; return r;
+}
+template <int N> inline Jet<N> operator*(const Jet<N>& x, const Jet<N>& y) {
+  Jet<N> r; r.a = x.a * y.a; for (int i = 0; i < N; ++i) r.v[i] = x.a * y.v[i] + x.v[i] * y.a; return r;
+}
+template <int N> inline Jet<N> operator/(const Jet<N>& x, const Jet<N>& y) {
+  Jet<N> r; r.a = x.a / y.a;
+  for (int i = 0; i < N; ++i) r.v[i] = (x.v[i] - r.a * y.v[i]) / y.a;
+  return r;
+}
+template <int N> inline Jet<N> jsqrt(const Jet<N>& x) {
+  Jet<N> r; r.a = std::sqrt(x.a); for (int i = 0; i < N; ++i) r.v[i] = x.v[i] / (2 * r.a); return r;
+}
+template <int N> inline Jet<N> jsin(const Jet<N>& x) {
+  Jet<N> r; r.a = std::sin(x.a); const double c = std::cos(x.a); for (int i = 0; i < N; ++i) r.v[i] = c * x.v[i];
+  return r;
+}
+template <int N> inline Jet<N> jcos(const Jet<N>& x) {
+  Jet<N> r; r.a = std::cos(x.a); const double s = -std::sin(x.a); for (int i = 0; i < N; ++i) r.v[i] = s * x.v[i];
+  return r;
+}
+template <int N> inline Jet<N> operator+(double s, const Jet<N>& x) { Jet<N> r = x; r.a += s; return r; }
+template <int N> inline Jet<N> operator-(double s, const Jet<N>& x) { Jet<N> r = -x; r.a += s; return r; }
+template <int N> inline Jet<N> operator-(const Jet<N>& x, double s) { Jet<N> r = x; r.a -= s; return r; }
+inline double jsqrt(double x) { return std::sqrt(x); }
+inline double jsin(double x) { return std::sin(x); }
+inline double jcos(double x) { return std::cos(x); }
+inline double jval(double x) { return x; }
+template <int N> inline double jval(const Jet<N>& x) { return x.a; }
+
+// EdgeObservationBAL::operator() (bal_example.cpp:154-204): rotate by
+// Rodrigues' formula (first-order expansion at theta = 0), translate, divide
+// by -z, scale by f (1 + k1 r^2 + k2 r^4); e = prediction - z.
+template <typename T>
+void bal_error(const T* cam, const T* X, const double z[2], T e[2]) {
+  T p[3];
+  const T theta = jsqrt(cam[0] * cam[0] + cam[1] * cam[1] + cam[2] * cam[2]);
+  if (jval(theta) > 0) {
+    const T v[3] = {cam[0] / theta, cam[1] / theta, cam[2] / theta};
+    const T cth = jcos(theta), sth = jsin(theta);
+    const T vxp[3] = {v[1] * X[2] - v[2] * X[1], v[2] * X[0] - v[0] * X[2], v[0] * X[1] - v[1] * X[0]};
+    const T vdp = v[0] * X[0] + v[1] * X[1] + v[2] * X[2];
+    const T omc = 1.0 - cth;
+    for (int i = 0; i < 3; ++i) p[i] = X[i] * cth + vxp[i] * sth + v[i] * vdp * omc;
+  } else {
+    const T wxp[3] = {cam[1] * X[2] - cam[2] * X[1], cam[2] * X[0] - cam[0] * X[2], cam[0] * X[1] - cam[1] * X[0]};
+    for (int i = 0; i < 3; ++i) p[i] = X[i] + wxp[i];
+  }
+  for (int i = 0; i < 3; ++i) p[i] = p[i] + cam[3 + i];
+  const T px = -p[0] / p[2], py = -p[1] / p[2];
+  const T r2 = px * px + py * py;
+  const T rp = 1.0 + cam[7] * r2 + cam[8] * r2 * r2;
+  e[0] = cam[6] * rp * px - z[0];
+  e[1] = cam[6] * rp * py - z[1];
+}
+
+// e[2], Jc[2x9] = de/dcam, Jp[2x3] = de/dX (row-major), by dual numbers.
+void bal_edge(const double* cam, const double* X, const double* z, double* e, double* Jc, double* Jp) {
+  if (!Jc && !Jp) {
+    bal_error<double>(cam, X, z, e);
+    return;
+  }
+  using J12 = Jet<12>;
+  J12 cj[9], xj[3], ej[2];
+  for (int i = 0; i < 9; ++i) { cj[i].a = cam[i]; cj[i].v[i] = 1.0; }
+  for (int i = 0; i < 3; ++i) { xj[i].a = X[i]; xj[i].v[9 + i] = 1.0; }
+  bal_error<J12>(cj, xj, z, ej);
+  for (int r = 0; r < 2; ++r) {
+    e[r] = ej[r].a;
+    if (Jc) for (int c = 0; c < 9; ++c) Jc[9 * r + c] = ej[r].v[c];
+    if (Jp) for (int c = 0; c < 3; ++c) Jp[3 * r + c] = ej[r].v[9 + c];
+  }
+}
+
+struct BAL {
+  int C = 0, P = 0, E = 0;
+  vector<double> cams;   // 9 per camera
+  vector<double> pts;    // 3 per point
+  vector<int> ecam, ept;
+  vector<double> meas, w;
+  vector<uint8_t> cfix, pfix;
+  double huber = 0;
+  vector<int> camH, ptH;
+  int Cf = 0, Pf = 0;
+  vector<int> active;
+  vector<int> colPtr, colCam, edgeBlk;  // Hpl blocks per point column (distinct cameras)
+  vector<int> sPtr, sCol;               // Hschur pattern
+  vector<double> Hpp, Hll, Hpl, b, err;
+  vector<vector<double>> stackC, stackP;
+  BlockSym S;
+  vector<double> bschur, Dinv, x;
+  double lambda = 0;
+  vector<double> trials;
+  PCGState pcg;
+  int solver_kind = 0;
+  double times[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// BlockSolver<9, 3>::buildStructure (block_solver.hpp:110-269) and the index
+// mapping (sparse_optimizer.cpp:175-200), as for the 6-DoF problem.
+void bal_structure(BAL& p) {
+  p.camH.assign(p.C, -1);
+  p.ptH.assign(p.P, -1);
+  p.Cf = p.Pf = 0;
+  vector<char> cu(p.C, 0), pu(p.P, 0);
+  for (int k = 0; k < p.E; ++k) { cu[p.ecam[k]] = 1; pu[p.ept[k]] = 1; }
+  for (int c = 0; c < p.C; ++c) if (!p.cfix[c] && cu[c]) p.camH[c] = p.Cf++;
+  for (int i = 0; i < p.P; ++i) if (!p.pfix[i] && pu[i]) p.ptH[i] = p.Pf++;
+  p.active.clear();
+  for (int k = 0; k < p.E; ++k)
+    if (!(p.cfix[p.ecam[k]] && p.pfix[p.ept[k]])) p.active.push_back(k);
+  vector<vector<int>> colCams(p.Pf);
+  for (int k : p.active) {
+    int c = p.camH[p.ecam[k]], l = p.ptH[p.ept[k]];
+    if (c >= 0 && l >= 0) colCams[l].push_back(c);
+  }
+  p.colPtr.assign(p.Pf + 1, 0);
+  for (int l = 0; l < p.Pf; ++l) {
+    auto& v = colCams[l];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    p.colPtr[l + 1] = p.colPtr[l] + static_cast<int>(v.size());
+  }
+  p.colCam.resize(p.colPtr[p.Pf]);
+  for (int l = 0; l < p.Pf; ++l) std::copy(colCams[l].begin(), colCams[l].end(), p.colCam.begin() + p.colPtr[l]);
+  p.edgeBlk.assign(p.E, -1);
+  for (int k : p.active) {
+    int c = p.camH[p.ecam[k]], l = p.ptH[p.ept[k]];
+    if (c < 0 || l < 0) continue;
+    auto b0 = p.colCam.begin() + p.colPtr[l], b1 = p.colCam.begin() + p.colPtr[l + 1];
+    p.edgeBlk[k] = static_cast<int>(std::lower_bound(b0, b1, c) - p.colCam.begin());
+  }
+  vector<vector<int>> rows(p.Cf);
+  for (int c = 0; c < p.Cf; ++c) rows[c].push_back(c);
+  for (int l = 0; l < p.Pf; ++l)
+    for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a)
+      for (int bb = a; bb < p.colPtr[l + 1]; ++bb) rows[p.colCam[a]].push_back(p.colCam[bb]);
+  p.sPtr.assign(p.Cf + 1, 0);
+  for (int c = 0; c < p.Cf; ++c) {
+    auto& v = rows[c];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    p.sPtr[c + 1] = p.sPtr[c] + static_cast<int>(v.size());
+  }
+  p.sCol.resize(p.sPtr[p.Cf]);
+  for (int c = 0; c < p.Cf; ++c) std::copy(rows[c].begin(), rows[c].end(), p.sCol.begin() + p.sPtr[c]);
+  p.S.bd = 9;
+  p.S.nb = p.Cf;
+  p.S.row.resize(p.sCol.size());
+  p.S.col = p.sCol;
+  p.S.diag.assign(p.Cf, -1);
+  for (int c = 0; c < p.Cf; ++c)
+    for (int k = p.sPtr[c]; k < p.sPtr[c + 1]; ++k) {
+      p.S.row[k] = c;
+      if (p.sCol[k] == c) p.S.diag[c] = k;
+    }
+  p.S.val.assign(p.sCol.size() * 81, 0.0);
+  p.err.assign(2 * static_cast<size_t>(p.E), 0.0);
+}
+
+// computeActiveErrors + activeRobustChi2 (sparse_optimizer.cpp:78-130);
+// Omega = w I (bal_example.cpp:313 sets the identity).
+double bal_chi2(BAL& p) {
+  double chi = 0;
+  const int n = static_cast<int>(p.active.size());
+#pragma omp parallel for reduction(+ : chi) schedule(static)
+  for (int a = 0; a < n; ++a) {
+    const int k = p.active[a];
+    double e[2];
+    bal_edge(&p.cams[9 * static_cast<size_t>(p.ecam[k])], &p.pts[3 * static_cast<size_t>(p.ept[k])], &p.meas[2 * k], e,
+             nullptr, nullptr);
+    p.err[2 * k] = e[0];
+    p.err[2 * k + 1] = e[1];
+    const double c2 = p.w[k] * (e[0] * e[0] + e[1] * e[1]);
+    if (p.huber > 0) {
+      double rho[3];
+      huber(c2, p.huber, rho);
+      chi += rho[0];
+    } else {
+      chi += c2;
+    }
+  }
+  return chi;
+}
+
+// buildSystem (block_solver.hpp:492-565) with constructQuadraticForm
+// (base_fixed_sized_edge.hpp:48-130): Hpp += Jc^T O' Jc, Hll += Jp^T O' Jp,
+// Hpl += Jc^T O' Jp, b -= J^T O' e with O' = rho' w I.
+void bal_build_system(BAL& p) {
+  p.Hpp.assign(static_cast<size_t>(p.Cf) * 81, 0.0);
+  p.Hll.assign(static_cast<size_t>(p.Pf) * 9, 0.0);
+  p.Hpl.assign(p.colCam.size() * 27, 0.0);
+  p.b.assign(9 * static_cast<size_t>(p.Cf) + 3 * static_cast<size_t>(p.Pf), 0.0);
+  const size_t sp = 9 * static_cast<size_t>(p.Cf);
+  // serial accumulation over the active edges (a fixed order)
+  for (int k : p.active) {
+    const int ci = p.camH[p.ecam[k]], li = p.ptH[p.ept[k]];
+    double e[2], Jc[18], Jp[6];
+    bal_edge(&p.cams[9 * static_cast<size_t>(p.ecam[k])], &p.pts[3 * static_cast<size_t>(p.ept[k])], &p.meas[2 * k], e,
+             Jc, Jp);
+    double om = p.w[k];
+    if (p.huber > 0) {
+      double rho[3];
+      huber(p.w[k] * (e[0] * e[0] + e[1] * e[1]), p.huber, rho);
+      om *= rho[1];
+    }
+    if (li >= 0) {
+      for (int r = 0; r < 3; ++r) {
+        p.b[sp + 3 * li + r] -= om * (Jp[r] * e[0] + Jp[3 + r] * e[1]);
+        for (int c = 0; c < 3; ++c) p.Hll[9 * li + 3 * r + c] += om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]);
+      }
+    }
+    if (ci >= 0) {
+      for (int r = 0; r < 9; ++r) {
+        p.b[9 * ci + r] -= om * (Jc[r] * e[0] + Jc[9 + r] * e[1]);
+        for (int c = 0; c < 9; ++c) p.Hpp[81 * static_cast<size_t>(ci) + 9 * r + c] += om * (Jc[r] * Jc[c] + Jc[9 + r] * Jc[9 + c]);
+      }
+    }
+    if (ci >= 0 && li >= 0) {
+      double* h = &p.Hpl[static_cast<size_t>(p.edgeBlk[k]) * 27];
+      for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 3; ++c) h[3 * r + c] += om * (Jc[r] * Jp[c] + Jc[9 + r] * Jp[3 + c]);
+    }
+  }
+}
+
+// BlockSolver<9, 3>::solve with Schur (block_solver.hpp:331-477), lambda on
+// the diagonals (setLambda, :567-590) applied to copies.
+bool bal_solve(BAL& p) {
+  const double t0 = now_s();
+  const int Cf = p.Cf, Pf = p.Pf;
+  const size_t sp = 9 * static_cast<size_t>(Cf);
+  std::fill(p.S.val.begin(), p.S.val.end(), 0.0);
+  for (int c = 0; c < Cf; ++c) {
+    double* d = &p.S.val[static_cast<size_t>(p.S.diag[c]) * 81];
+    for (int i = 0; i < 81; ++i) d[i] = p.Hpp[81 * static_cast<size_t>(c) + i];
+    for (int i = 0; i < 9; ++i) d[10 * i] += p.lambda;
+  }
+  p.Dinv.assign(static_cast<size_t>(Pf) * 9, 0.0);
+  vector<double> coeff(sp, 0.0);
+  bool okD = true;
+  for (int l = 0; l < Pf; ++l) {
+    double D[9];
+    for (int i = 0; i < 9; ++i) D[i] = p.Hll[9 * static_cast<size_t>(l) + i];
+    for (int i = 0; i < 3; ++i) D[4 * i] += p.lambda;
+    double* Di = &p.Dinv[9 * static_cast<size_t>(l)];
+    okD = inv3(D, Di) && okD;
+    const double* bl = &p.b[sp + 3 * static_cast<size_t>(l)];
+    double db[3];
+    for (int r = 0; r < 3; ++r) db[r] = Di[3 * r] * bl[0] + Di[3 * r + 1] * bl[1] + Di[3 * r + 2] * bl[2];
+    for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a) {
+      const int i1 = p.colCam[a];
+      const double* Bi = &p.Hpl[static_cast<size_t>(a) * 27];
+      double BD[27];
+      for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 3; ++c)
+          BD[3 * r + c] = Bi[3 * r] * Di[c] + Bi[3 * r + 1] * Di[3 + c] + Bi[3 * r + 2] * Di[6 + c];
+      for (int r = 0; r < 9; ++r) coeff[9 * i1 + r] += Bi[3 * r] * db[0] + Bi[3 * r + 1] * db[1] + Bi[3 * r + 2] * db[2];
+      int sk = p.sPtr[i1];
+      for (int bb = a; bb < p.colPtr[l + 1]; ++bb) {
+        const int i2 = p.colCam[bb];
+        while (p.sCol[sk] < i2) ++sk;
+        const double* Bj = &p.Hpl[static_cast<size_t>(bb) * 27];
+        double* h = &p.S.val[static_cast<size_t>(sk) * 81];
+        for (int r = 0; r < 9; ++r)
+          for (int c = 0; c < 9; ++c)
+            h[9 * r + c] -= BD[3 * r] * Bj[3 * c] + BD[3 * r + 1] * Bj[3 * c + 1] + BD[3 * r + 2] * Bj[3 * c + 2];
+      }
+    }
+  }
+  if (!okD) return false;
+  p.bschur.assign(sp, 0.0);
+  for (size_t i = 0; i < sp; ++i) p.bschur[i] = p.b[i] - coeff[i];
+  const double t1 = now_s();
+  p.times[2] += t1 - t0;
+  p.x.assign(sp + 3 * static_cast<size_t>(Pf), 0.0);
+  const bool ok = (p.solver_kind == 0) ? chol_solve_blocks(p.S, p.bschur.data(), p.x.data())
+                                       : pcg_solve(p.S, p.bschur.data(), p.x.data(), p.pcg);
+  p.times[3] += now_s() - t1;
+  if (!ok) return false;
+  // back-substitution (block_solver.hpp:452-476)
+  for (int l = 0; l < Pf; ++l) {
+    double cl[3];
+    for (int r = 0; r < 3; ++r) cl[r] = p.b[sp + 3 * static_cast<size_t>(l) + r];
+    for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a) {
+      const double* B = &p.Hpl[static_cast<size_t>(a) * 27];
+      const double* xp = &p.x[9 * static_cast<size_t>(p.colCam[a])];
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 9; ++r) cl[c] -= B[3 * r + c] * xp[r];
+    }
+    const double* Di = &p.Dinv[9 * static_cast<size_t>(l)];
+    for (int r = 0; r < 3; ++r)
+      p.x[sp + 3 * static_cast<size_t>(l) + r] = Di[3 * r] * cl[0] + Di[3 * r + 1] * cl[1] + Di[3 * r + 2] * cl[2];
+  }
+  return true;
+}
+
+// VertexCameraBAL / VertexPointBAL::oplusImpl (bal_example.cpp:86-89, 115-118)
+void bal_update(BAL& p, const double* x) {
+  for (int c = 0; c < p.C; ++c) {
+    const int h = p.camH[c];
+    if (h < 0) continue;
+    for (int r = 0; r < 9; ++r) p.cams[9 * static_cast<size_t>(c) + r] += x[9 * static_cast<size_t>(h) + r];
+  }
+  const size_t off = 9 * static_cast<size_t>(p.Cf);
+  for (int i = 0; i < p.P; ++i) {
+    const int h = p.ptH[i];
+    if (h < 0) continue;
+    for (int r = 0; r < 3; ++r) p.pts[3 * static_cast<size_t>(i) + r] += x[off + 3 * static_cast<size_t>(h) + r];
+  }
+}
+
+// LM adapter of the BAL problem (lm_optimize)
+struct BALLM {
+  BAL& p;
+  double* times = p.times;
+  double chi2() { return bal_chi2(p); }
+  void build() { bal_build_system(p); }
+  // computeLambdaInit (optimization_algorithm_levenberg.cpp:155-167)
+  double lambda_init() {
+    double m = 0;
+    for (int c = 0; c < p.Cf; ++c)
+      for (int j = 0; j < 9; ++j) m = std::max(m, std::fabs(p.Hpp[81 * static_cast<size_t>(c) + 10 * j]));
+    for (int l = 0; l < p.Pf; ++l)
+      for (int j = 0; j < 3; ++j) m = std::max(m, std::fabs(p.Hll[9 * static_cast<size_t>(l) + 4 * j]));
+    return 1e-5 * m;
+  }
+  bool solve(double lambda, int* pcg_it) {
+    p.lambda = lambda;
+    const bool ok = bal_solve(p);
+    *pcg_it = p.solver_kind == 1 ? p.pcg.last_iters : 0;
+    return ok;
+  }
+  void update() { bal_update(p, p.x.data()); }
+  double scale(double lambda) {
+    double s = 0;
+    for (size_t j = 0; j < p.x.size(); ++j) s += p.x[j] * (lambda * p.x[j] + p.b[j]);
+    return s;
+  }
+  void push() { p.stackC.push_back(p.cams); p.stackP.push_back(p.pts); }
+  void pop() { p.cams = p.stackC.back(); p.pts = p.stackP.back(); discard(); }
+  void discard() { p.stackC.pop_back(); p.stackP.pop_back(); }
+};
+
 }  // namespace
 
 // =============================================================== C interface
@@ -2063,5 +2438,129 @@ int orc_pg_dogleg(void* h, int iters, int solver, double tol, int maxiter, doubl
   PGDL dl{*p};
   return dl_optimize(dl, iters, trace);
 }
+
+
+// ------------------------------------------------------------------ BAL (9-DoF)
+void* orc_bal_new(int C, int P, int E, const double* cams9, const double* pts3, const int* ecam, const int* ept,
+                  const double* meas2, const double* w, const unsigned char* cam_fixed,
+                  const unsigned char* pt_fixed, double huber_delta) {
+  BAL* p = new BAL();
+  p->C = C; p->P = P; p->E = E;
+  p->cams.assign(cams9, cams9 + 9 * static_cast<size_t>(C));
+  p->pts.assign(pts3, pts3 + 3 * static_cast<size_t>(P));
+  p->ecam.assign(ecam, ecam + E);
+  p->ept.assign(ept, ept + E);
+  p->meas.assign(meas2, meas2 + 2 * static_cast<size_t>(E));
+  if (w) p->w.assign(w, w + E); else p->w.assign(E, 1.0);
+  p->cfix.assign(C, 0);
+  p->pfix.assign(P, 0);
+  if (cam_fixed) p->cfix.assign(cam_fixed, cam_fixed + C);
+  if (pt_fixed) p->pfix.assign(pt_fixed, pt_fixed + P);
+  p->huber = huber_delta;
+  bal_structure(*p);
+  return p;
+}
+
+void orc_bal_free(void* h) { delete static_cast<BAL*>(h); }
+
+void orc_bal_dims(void* h, int* cf, int* pf, int* nschur) {
+  BAL* p = static_cast<BAL*>(h);
+  *cf = p->Cf; *pf = p->Pf; *nschur = static_cast<int>(p->sCol.size());
+}
+
+void orc_bal_get_state(void* h, double* cams9, double* pts3) {
+  BAL* p = static_cast<BAL*>(h);
+  std::copy(p->cams.begin(), p->cams.end(), cams9);
+  std::copy(p->pts.begin(), p->pts.end(), pts3);
+}
+
+void orc_bal_set_state(void* h, const double* cams9, const double* pts3) {
+  BAL* p = static_cast<BAL*>(h);
+  p->cams.assign(cams9, cams9 + 9 * static_cast<size_t>(p->C));
+  p->pts.assign(pts3, pts3 + 3 * static_cast<size_t>(p->P));
+}
+
+// chi2 at the state; err[E*2] (optional) the edges' errors (0 for inactive)
+double orc_bal_chi2(void* h, double* err) {
+  BAL* p = static_cast<BAL*>(h);
+  std::fill(p->err.begin(), p->err.end(), 0.0);
+  const double chi = bal_chi2(*p);
+  if (err) std::copy(p->err.begin(), p->err.end(), err);
+  return chi;
+}
+
+// One edge function: e[2], Jc[18], Jp[6] (row-major; Jacobians optional)
+void orc_bal_edge(const double* cam9, const double* X, const double* z, double* e, double* Jc, double* Jp) {
+  bal_edge(cam9, X, z, e, Jc, Jp);
+}
+
+// Linearise at the state: Hpp[Cf*81], Hll[Pf*9], Hpl_edge[E*27], b[9Cf+3Pf]
+// (each optional). Returns chi2.
+double orc_bal_linearize(void* h, double* Hpp, double* Hll, double* Hpl_edge, double* b) {
+  BAL* p = static_cast<BAL*>(h);
+  const double chi = bal_chi2(*p);
+  bal_build_system(*p);
+  if (Hpp) std::copy(p->Hpp.begin(), p->Hpp.end(), Hpp);
+  if (Hll) std::copy(p->Hll.begin(), p->Hll.end(), Hll);
+  if (b) std::copy(p->b.begin(), p->b.end(), b);
+  if (Hpl_edge) {
+    for (int k = 0; k < p->E; ++k) {
+      double* o = &Hpl_edge[27 * static_cast<size_t>(k)];
+      if (p->edgeBlk[k] < 0) { std::fill(o, o + 27, 0.0); continue; }
+      const double* s = &p->Hpl[27 * static_cast<size_t>(p->edgeBlk[k])];
+      std::copy(s, s + 27, o);
+    }
+  }
+  return chi;
+}
+
+// Schur complement of the last linearisation at damping lambda: upper blocks
+// (rows, cols, 81 values each) and bschur[9Cf]. Returns 0, or 2 if a point
+// block is singular.
+int orc_bal_schur(void* h, double lambda, int* rows, int* cols, double* blocks, double* bschur) {
+  BAL* p = static_cast<BAL*>(h);
+  p->lambda = lambda;
+  const int saved = p->solver_kind;
+  p->solver_kind = 0;
+  const bool ok = bal_solve(*p);
+  p->solver_kind = saved;
+  for (size_t k = 0; k < p->sCol.size(); ++k) {
+    if (rows) rows[k] = p->S.row[k];
+    if (cols) cols[k] = p->S.col[k];
+    if (blocks) std::copy(&p->S.val[81 * k], &p->S.val[81 * k] + 81, &blocks[81 * k]);
+  }
+  if (bschur) std::copy(p->bschur.begin(), p->bschur.end(), bschur);
+  return ok ? 0 : 2;
+}
+
+// One damped solve at the last linearisation (solver 0 Cholesky, 1 PCG with
+// the residual memory reset): dx[9Cf+3Pf]. Returns 0 or 2 (not PD).
+int orc_bal_solve(void* h, double lambda, int solver, double tol, int maxiter, double* dx, int* pcg_iters) {
+  BAL* p = static_cast<BAL*>(h);
+  p->lambda = lambda;
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  const bool ok = bal_solve(*p);
+  if (pcg_iters) *pcg_iters = p->pcg.last_iters;
+  if (!ok) return 2;
+  std::copy(p->x.begin(), p->x.end(), dx);
+  return 0;
+}
+
+// SparseOptimizer::optimize(iters) with OptimizationAlgorithmLevenberg over
+// BlockSolver<9, 3> (bal_example.cpp:227-250, 341): trace as orc_ba_optimize.
+int orc_bal_optimize(void* h, int iters, int solver, double tol, int maxiter, double* trace, double* times) {
+  BAL* p = static_cast<BAL*>(h);
+  p->solver_kind = solver;
+  p->pcg = PCGState();
+  p->pcg.tol = tol;
+  p->pcg.max_iter = maxiter;
+  BALLM lm{*p};
+  return lm_optimize(lm, iters, trace, times, &p->trials);
+}
+
+int orc_bal_trials(void* h, double* out, int max) { return copy_trials(static_cast<BAL*>(h)->trials, out, max); }
 
 }  // extern "C"

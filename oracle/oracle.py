@@ -103,6 +103,25 @@ def lib():
         L.orc_pg_dogleg.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp]
         L.orc_pg_gauss_newton.restype = ct.c_int
         L.orc_pg_gauss_newton.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp]
+        L.orc_bal_new.restype = ct.c_void_p
+        L.orc_bal_new.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _ip, _ip, _dp, _dp, _up, _up, ct.c_double]
+        L.orc_bal_free.argtypes = [ct.c_void_p]
+        L.orc_bal_dims.argtypes = [ct.c_void_p, _ip, _ip, _ip]
+        L.orc_bal_get_state.argtypes = [ct.c_void_p, _dp, _dp]
+        L.orc_bal_set_state.argtypes = [ct.c_void_p, _dp, _dp]
+        L.orc_bal_chi2.restype = ct.c_double
+        L.orc_bal_chi2.argtypes = [ct.c_void_p, _dp]
+        L.orc_bal_edge.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_bal_linearize.restype = ct.c_double
+        L.orc_bal_linearize.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_bal_schur.restype = ct.c_int
+        L.orc_bal_schur.argtypes = [ct.c_void_p, ct.c_double, _ip, _ip, _dp, _dp]
+        L.orc_bal_solve.restype = ct.c_int
+        L.orc_bal_solve.argtypes = [ct.c_void_p, ct.c_double, ct.c_int, ct.c_double, ct.c_int, _dp, _ip]
+        L.orc_bal_optimize.restype = ct.c_int
+        L.orc_bal_optimize.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int, _dp, _dp]
+        L.orc_bal_trials.restype = ct.c_int
+        L.orc_bal_trials.argtypes = [ct.c_void_p, _dp, ct.c_int]
         L.orc_num_threads.restype = ct.c_int
         L.orc_set_num_threads.argtypes = [ct.c_int]
         _lib = L
@@ -260,6 +279,101 @@ class OracleBA:
         lib().orc_ba_exact_hessian(self.h, Hpp.ctypes.data_as(_dp), Hll.ctypes.data_as(_dp),
                                    Hpl.ctypes.data_as(_dp))
         return Hpp, Hll, Hpl
+
+
+def bal_edge(cam9, X, z, jacobians: bool = True):
+    """EdgeObservationBAL's error (prediction - z) and, by dual numbers, its
+    Jacobians: returns e[2], Jc[2,9], Jp[2,3] (Jacobians None if not asked)."""
+    c, pc = _d(cam9)
+    x, px = _d(X)
+    zz, pz = _d(z)
+    e = np.zeros(2)
+    Jc = np.zeros((2, 9)) if jacobians else None
+    Jp = np.zeros((2, 3)) if jacobians else None
+    lib().orc_bal_edge(pc, px, pz, e.ctypes.data_as(_dp), None if Jc is None else Jc.ctypes.data_as(_dp),
+                       None if Jp is None else Jp.ctypes.data_as(_dp))
+    return e, Jc, Jp
+
+
+class OracleBAL:
+    """FP64 CPU restatement of g2o's BAL example: VertexCameraBAL (9 values,
+    additive update), EdgeObservationBAL, BlockSolver<9, 3> with Schur and LM."""
+
+    def __init__(self, prob):
+        L = lib()
+        C, P, E = prob.cams.shape[0], prob.pts.shape[0], prob.edge_cam.shape[0]
+        w = np.ones(E) if getattr(prob, "weight", None) is None else prob.weight
+        cf = np.zeros(C, np.uint8) if getattr(prob, "cam_fixed", None) is None else prob.cam_fixed
+        pf = np.zeros(P, np.uint8) if getattr(prob, "pt_fixed", None) is None else prob.pt_fixed
+        keep, args = [], []
+        for arr, conv in ((prob.cams, _d), (prob.pts, _d), (prob.edge_cam, _i), (prob.edge_pt, _i),
+                          (prob.meas, _d), (w, _d), (cf, _u), (pf, _u)):
+            a, p = conv(arr)
+            keep.append(a)
+            args.append(p)
+        self.C, self.P, self.E = C, P, E
+        self.h = L.orc_bal_new(C, P, E, *args, float(getattr(prob, "huber", 0.0) or 0.0))
+        cf_, pf_, ns = ct.c_int(), ct.c_int(), ct.c_int()
+        L.orc_bal_dims(self.h, ct.byref(cf_), ct.byref(pf_), ct.byref(ns))
+        self.Cf, self.Pf, self.nschur = cf_.value, pf_.value, ns.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_bal_free(self.h)
+            self.h = None
+
+    def state(self):
+        c = np.zeros((self.C, 9))
+        p = np.zeros((self.P, 3))
+        lib().orc_bal_get_state(self.h, _d(c)[1], _d(p)[1])
+        return c, p
+
+    def set_state(self, cams, pts):
+        c, pc = _d(cams)
+        p, pp = _d(pts)
+        lib().orc_bal_set_state(self.h, pc, pp)
+
+    def chi2(self) -> float:
+        return lib().orc_bal_chi2(self.h, None)
+
+    def errors(self):
+        e = np.zeros((self.E, 2))
+        lib().orc_bal_chi2(self.h, e.ctypes.data_as(_dp))
+        return e
+
+    def linearize(self):
+        Hpp = np.zeros((self.Cf, 9, 9))
+        Hll = np.zeros((self.Pf, 3, 3))
+        Hpl = np.zeros((self.E, 9, 3))
+        b = np.zeros(9 * self.Cf + 3 * self.Pf)
+        chi = lib().orc_bal_linearize(self.h, Hpp.ctypes.data_as(_dp), Hll.ctypes.data_as(_dp),
+                                      Hpl.ctypes.data_as(_dp), b.ctypes.data_as(_dp))
+        return chi, Hpp, Hll, Hpl, b
+
+    def schur(self, lam: float):
+        rows = np.zeros(self.nschur, dtype=np.int32)
+        cols = np.zeros(self.nschur, dtype=np.int32)
+        blocks = np.zeros((self.nschur, 9, 9))
+        bs = np.zeros(9 * self.Cf)
+        st = lib().orc_bal_schur(self.h, lam, rows.ctypes.data_as(_ip), cols.ctypes.data_as(_ip),
+                                 blocks.ctypes.data_as(_dp), bs.ctypes.data_as(_dp))
+        return st, rows, cols, blocks, bs
+
+    def solve(self, lam: float, solver: int = 0, tol: float = 1e-6, maxiter: int = -1):
+        dx = np.zeros(9 * self.Cf + 3 * self.Pf)
+        it = ct.c_int()
+        st = lib().orc_bal_solve(self.h, lam, solver, tol, maxiter, dx.ctypes.data_as(_dp), ct.byref(it))
+        return st, dx, it.value
+
+    def optimize(self, iters: int, solver: int = 1, tol: float = 1e-6, maxiter: int = -1):
+        trace = np.zeros((iters, 6))
+        times = np.zeros(6)
+        n = lib().orc_bal_optimize(self.h, iters, solver, tol, maxiter, trace.ctypes.data_as(_dp),
+                                   times.ctypes.data_as(_dp))
+        return n, trace[:n], times
+
+    def trials(self):
+        return _trials(lib().orc_bal_trials, self.h)
 
 
 class OraclePG:
