@@ -28,7 +28,9 @@ EXPORTS = [
     "g2ohip_compute_scale", "g2ohip_optimize", "g2ohip_set_algorithm", "g2ohip_set_force_stop_flag", "g2ohip_compute_marginals", "g2ohip_ba_get_hessian",
     "g2ohip_ba_get_schur", "g2ohip_last_solve_info", "g2ohip_set_profiling", "g2ohip_get_profile",
     "g2ohip_linear_solve_pcg", "g2ohip_pg_set_structure", "g2ohip_pg_structure_info", "g2ohip_pg_set_measurements",
-    "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_graph_create",
+    "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_bal_set_structure",
+    "g2ohip_bal_structure_info", "g2ohip_bal_set_measurements", "g2ohip_bal_set_state", "g2ohip_bal_get_state",
+    "g2ohip_bal_get_hessian", "g2ohip_bal_get_schur", "g2ohip_graph_create",
     "g2ohip_graph_destroy", "g2ohip_graph_last_error", "g2ohip_graph_log", "g2ohip_graph_load", "g2ohip_graph_info",
     "g2ohip_graph_get_ba", "g2ohip_graph_get_ba_params", "g2ohip_graph_get_pg", "g2ohip_graph_set_ba_state", "g2ohip_graph_set_pg_state",
     "g2ohip_graph_add_ba", "g2ohip_graph_add_pg", "g2ohip_graph_save", "g2ohip_graph_load_bal", "g2ohip_graph_get_bal",
@@ -120,6 +122,13 @@ def lib():
         "g2ohip_pg_set_state": (ct.c_int, [_vp, _dp]),
         "g2ohip_pg_get_state": (ct.c_int, [_vp, _dp]),
         "g2ohip_pg_get_system": (ct.c_int, [_vp, _ip, _ip, _dp, _dp]),
+        "g2ohip_bal_set_structure": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _up, _up]),
+        "g2ohip_bal_structure_info": (ct.c_int, [_vp, ct.POINTER(ct.c_longlong)]),
+        "g2ohip_bal_set_measurements": (ct.c_int, [_vp, _dp, _dp, ct.c_double]),
+        "g2ohip_bal_set_state": (ct.c_int, [_vp, _dp, _dp]),
+        "g2ohip_bal_get_state": (ct.c_int, [_vp, _dp, _dp]),
+        "g2ohip_bal_get_hessian": (ct.c_int, [_vp, _dp, _dp, _dp]),
+        "g2ohip_bal_get_schur": (ct.c_int, [_vp, _ip, _ip, _dp, _dp]),
         "g2ohip_graph_create": (ct.c_int, [ct.POINTER(_vp)]),
         "g2ohip_graph_destroy": (None, [_vp]),
         "g2ohip_graph_last_error": (ct.c_char_p, [_vp]),

@@ -136,7 +136,7 @@ int g2ohip_ba_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_
 int g2ohip_ba_structure_info(g2ohip_ctx* ctx, long long* info) {
   return guarded(ctx, [&]() -> int {
     if (!ctx->have_structure || !info) return ctx->fail(G2OHIP_ERR_INVALID, "no structure");
-    if (ctx->mode != 0) return ctx->fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+    if (ctx->mode != 0) return ctx->fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
     const bool rows = ctx->use_schur_rows();
     if (!rows) {
       const int rc = ctx->ensure_win(0);
@@ -439,6 +439,60 @@ int g2ohip_pg_get_state(g2ohip_ctx* ctx, double* poses) {
 int g2ohip_pg_get_system(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* b) {
   return guarded(ctx, [&]() -> int {
     return ctx->pg_get_system(rows, cols, blocks, b);
+  });
+}
+
+// -------------------------------------------------------------------- BAL
+int g2ohip_bal_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_edges, const int* edge_cam,
+                             const int* edge_pt, const uint8_t* cam_fixed, const uint8_t* pt_fixed) {
+  return guarded(ctx, [&]() -> int {
+    return ctx->bal_set_structure(num_cams, num_pts, num_edges, edge_cam, edge_pt, cam_fixed, pt_fixed);
+  });
+}
+
+int g2ohip_bal_structure_info(g2ohip_ctx* ctx, long long* info) {
+  return guarded(ctx, [&]() -> int {
+    if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    if (!ctx->have_structure || ctx->mode != 2) return ctx->fail(G2OHIP_ERR_INVALID, "no BAL structure");
+    const g2ohip::BAStructure& st = ctx->st;
+    info[0] = st.Cf;
+    info[1] = st.Pf;
+    info[2] = st.ns;
+    info[3] = st.nS;
+    info[4] = st.nF;
+    info[5] = ctx->bal_pairs;
+    info[6] = st.nq;
+    return G2OHIP_OK;
+  });
+}
+
+int g2ohip_bal_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* weight, double huber_delta) {
+  return guarded(ctx, [&]() -> int {
+    return ctx->bal_set_measurements(meas, weight, huber_delta);
+  });
+}
+
+int g2ohip_bal_set_state(g2ohip_ctx* ctx, const double* cams, const double* pts) {
+  return guarded(ctx, [&]() -> int {
+    return ctx->bal_set_state(cams, pts);
+  });
+}
+
+int g2ohip_bal_get_state(g2ohip_ctx* ctx, double* cams, double* pts) {
+  return guarded(ctx, [&]() -> int {
+    return ctx->bal_get_state(cams, pts);
+  });
+}
+
+int g2ohip_bal_get_hessian(g2ohip_ctx* ctx, double* Hpp, double* Hll, double* Hpl_edge) {
+  return guarded(ctx, [&]() -> int {
+    return ctx->bal_get_hessian(Hpp, Hll, Hpl_edge);
+  });
+}
+
+int g2ohip_bal_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* bschur) {
+  return guarded(ctx, [&]() -> int {
+    return ctx->bal_get_schur(rows, cols, blocks, bschur);
   });
 }
 

@@ -364,7 +364,7 @@ template int g2ohip_ctx::schur_offdiag<float>(PrecBufs<float>&);
 template int g2ohip_ctx::schur_offdiag<double>(PrecBufs<double>&);
 
 int g2ohip_ctx::set_measurements(const double* meas, const double* w, const double* intr, double huber) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
   if (!meas || !intr) return fail(G2OHIP_ERR_INVALID, "measurements and intrinsics are required");
   CK(require_device());
@@ -395,7 +395,7 @@ int g2ohip_ctx::set_measurements(const double* meas, const double* w, const doub
 // factor S (Omega = S^T S, upper triangular) per slot; all-identity input, or
 // omega == nullptr, drops it (the unwhitened fast path).
 int g2ohip_ctx::set_information(const double* omega) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
   CK(require_device());
   std::vector<double> S;
@@ -429,7 +429,7 @@ int g2ohip_ctx::set_information(const double* omega) {
 }
 
 int g2ohip_ctx::set_state(const double* cams, const double* pts) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_structure) return fail(G2OHIP_ERR_INVALID, "set_structure first");
   CK(require_device());
   std::vector<double> c(cams, cams + 7 * static_cast<size_t>(st.C));
@@ -451,7 +451,7 @@ int g2ohip_ctx::set_state(const double* cams, const double* pts) {
 }
 
 int g2ohip_ctx::get_state(double* cams, double* pts) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_state) return fail(G2OHIP_ERR_INVALID, "no state");
   if (cams) HK(hipMemcpyAsync(cams, d_cams.p, 7 * sizeof(double) * st.C, hipMemcpyDeviceToHost, stream));
   if (pts && st.P > 0) HK(hipMemcpyAsync(pts, d_pts.p, 3 * sizeof(double) * st.P, hipMemcpyDeviceToHost, stream));
@@ -461,6 +461,10 @@ int g2ohip_ctx::get_state(double* cams, double* pts) {
 
 // ------------------------------------------------------------- residuals
 int g2ohip_ctx::compute_active_errors(double* chi2, double* err_out) {
+  if (mode == 2) {
+    if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+    return bal_chi2(chi2, err_out);
+  }
   if (mode == 1) {
     if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
     return pg_chi2(chi2, err_out);
@@ -488,6 +492,10 @@ int g2ohip_ctx::compute_active_errors(double* chi2, double* err_out) {
 }
 
 int g2ohip_ctx::build_system(bool read_chi) {
+  if (mode == 2) {
+    if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
+    return bal_build();
+  }
   if (mode == 1) {
     if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
     return pg_build();
@@ -516,6 +524,10 @@ int g2ohip_ctx::build_system(bool read_chi) {
 }
 
 int g2ohip_ctx::lambda_init(double* out) {
+  if (mode == 2) {
+    if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+    return bal_lambda_init(out);
+  }
   if (mode == 1) {
     if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
     return pg_lambda_init(out);
@@ -532,6 +544,7 @@ int g2ohip_ctx::lambda_init(double* out) {
 int g2ohip_ctx::compute_scale(double* out) {
   if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
   const int n = static_cast<int>(system_size());
+  HK(d_red.alloc(RED_BLOCKS));  // a no-op once set_structure sized it
   HK(launch_scale(n, d_x.p, d_b.p, lambda, d_red.p, d_scal.p + 3, stream));
   return read_scalar(d_scal.p + 3, out);
 }
@@ -708,6 +721,10 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
 }
 
 int g2ohip_ctx::solve(int* ok, bool defer_err) {
+  if (mode == 2) {
+    if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+    return bal_solve(ok);
+  }
   if (mode == 1) {
     if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
     return pg_solve(ok);
@@ -734,6 +751,7 @@ int g2ohip_ctx::solve(int* ok, bool defer_err) {
 // pose graph).
 int g2ohip_ctx::multiply_hessian(const double* src, double* dest) {
   if (!have_system) return fail(G2OHIP_ERR_INVALID, "build_system first");
+  if (mode == 2) return fail(G2OHIP_ERR_UNSUPPORTED, "multiplyHessian is not provided for BAL problems");
   const int nb = mode == 1 ? pgst.Nf : st.Cf;
   const size_t n = 6 * static_cast<size_t>(nb);
   DBuf<double> dx, dy;
@@ -749,6 +767,10 @@ int g2ohip_ctx::multiply_hessian(const double* src, double* dest) {
 
 // --------------------------------------------------------- update / stack
 int g2ohip_ctx::update() {
+  if (mode == 2) {
+    if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
+    return bal_update();
+  }
   if (mode == 1) {
     if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
     return pg_update();
@@ -771,10 +793,10 @@ int g2ohip_ctx::push() {
   }
   DBuf<double>& c = *stack_c[stack_depth];
   DBuf<double>& p = *stack_p[stack_depth];
-  HK(c.alloc(7 * state7()));
-  HK(p.alloc(3 * std::max<size_t>(state3(), 1)));
-  HK(launch_copy(c.p, d_cams.p, 7 * sizeof(double) * state7(), stream));
-  if (state3()) HK(launch_copy(p.p, d_pts.p, 3 * sizeof(double) * state3(), stream));
+  HK(c.alloc(cams_len()));
+  HK(p.alloc(std::max<size_t>(pts_len(), 1)));
+  HK(launch_copy(c.p, d_cams.p, sizeof(double) * cams_len(), stream));
+  if (pts_len()) HK(launch_copy(p.p, d_pts.p, sizeof(double) * pts_len(), stream));
   stack_depth++;
   return G2OHIP_OK;
 }
@@ -782,8 +804,8 @@ int g2ohip_ctx::push() {
 int g2ohip_ctx::pop() {
   if (stack_depth <= 0) return fail(G2OHIP_ERR_INVALID, "state stack is empty");
   stack_depth--;
-  HK(launch_copy(d_cams.p, stack_c[stack_depth]->p, 7 * sizeof(double) * state7(), stream));
-  if (state3()) HK(launch_copy(d_pts.p, stack_p[stack_depth]->p, 3 * sizeof(double) * state3(), stream));
+  HK(launch_copy(d_cams.p, stack_c[stack_depth]->p, sizeof(double) * cams_len(), stream));
+  if (pts_len()) HK(launch_copy(d_pts.p, stack_p[stack_depth]->p, sizeof(double) * pts_len(), stream));
   return G2OHIP_OK;
 }
 
@@ -952,6 +974,7 @@ int g2ohip_ctx::optimize_gauss_newton(int iterations, g2ohip_iter_stats* stats, 
 // after the linearisation (undamped).
 int g2ohip_ctx::optimize_dogleg(int iterations, g2ohip_iter_stats* stats, int* done) {
   *done = 0;
+  if (mode == 2) return fail(G2OHIP_ERR_UNSUPPORTED, "Dogleg needs multiplyHessian, not provided for BAL problems");
   pcg_residual_mem = -1.0;
   fp32_off = false;
   double delta = 1e4, lam = 1e-7;
@@ -1098,6 +1121,7 @@ int g2ohip_ctx::optimize_dogleg(int iterations, g2ohip_iter_stats* stats, int* d
 // graphs: Hpp is the whole Hessian (pg_marginals).
 int g2ohip_ctx::marginals(int nblk, const int* rows, const int* cols, double* out, double tol) {
   if (!have_lin) return fail(G2OHIP_ERR_INVALID, "no linearised system: build_system or optimize first");
+  if (mode == 2) return fail(G2OHIP_ERR_UNSUPPORTED, "marginals are not provided for BAL problems");
   const int nb = mode == 0 ? st.Cf : pgst.Nf;
   if (nblk < 0 || (nblk > 0 && (!rows || !cols || !out))) return fail(G2OHIP_ERR_INVALID, "bad block list");
   for (int k = 0; k < nblk; ++k)
@@ -1184,7 +1208,7 @@ int g2ohip_ctx::replay_pcg(int reps, double* us_spmv, double* us_iter) {
 
 // --------------------------------------------------------------- inspection
 int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_system && !have_exact) return fail(G2OHIP_ERR_INVALID, "build_system first");
   HK(hipStreamSynchronize(stream));
   if (Hpp) HK(hipMemcpy(Hpp, d_Hpp.p, 36 * sizeof(double) * st.Cf, hipMemcpyDeviceToHost));
@@ -1215,7 +1239,7 @@ int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
 }
 
 int g2ohip_ctx::get_schur(int* rows, int* cols, double* blocks, double* bschur) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_solution) return fail(G2OHIP_ERR_INVALID, "solve first");
   if (opt.schur_mode != G2OHIP_SCHUR_EXPLICIT) return fail(G2OHIP_ERR_INVALID, "explicit Schur mode only");
   HK(hipStreamSynchronize(stream));
@@ -1390,7 +1414,7 @@ int g2ohip_ctx::solve_exact(double lam, double tol, int max_iter, int* iters, in
 // largest component exceeds max_step (> 0) counts as rejected. At most 30
 // trials per step; *ok = 0 if none was acceptable.
 int g2ohip_ctx::newton(int steps, double tol, double max_step, int* ok) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   CK(require_device());
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   *ok = 1;
@@ -1439,7 +1463,7 @@ int g2ohip_ctx::newton(int steps, double tol, double max_step, int* ok) {
 
 int g2ohip_ctx::adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,
                         int max_iter, int* iterations) {
-  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "bundle-adjustment call on a pose-graph context");
+  if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   CK(require_device());
   if (!(have_structure && have_meas && have_state)) return fail(G2OHIP_ERR_INVALID, "structure/measurements/state missing");
   if (!dL_dcam || !dL_dmeas) return fail(G2OHIP_ERR_INVALID, "dL_dcam and dL_dmeas are required");

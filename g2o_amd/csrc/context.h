@@ -81,7 +81,7 @@ struct g2ohip_ctx {
   const volatile int* force_stop = nullptr;
 
   g2ohip::BAStructure st;
-  int mode = 0;  // 0: bundle adjustment, 1: 3D pose graph (the last *_set_structure)
+  int mode = 0;  // 0: bundle adjustment, 1: 3D pose graph, 2: BAL (the last *_set_structure)
   bool have_structure = false, have_meas = false, have_state = false;
   bool have_system = false;   // build_system ran since the last state change
   bool have_lin = false;      // the blocks hold a Gauss-Newton system (kept across state updates,
@@ -158,6 +158,14 @@ struct g2ohip_ctx {
       d_pg_blk_ref, d_pg_up_pos, d_pg_up_posT, d_pg_F_diag, d_pg_F_ptr, d_pg_F_col;
   g2ohip::DBuf<double> d_pg_meas, d_pg_info, d_pg_rec, d_pg_F_val, d_pg_Hdiag, d_pg_part;
 
+  // BAL (9-DoF cameras, bal.cpp): the structure is `st` with the lists below
+  g2ohip::DevBAL bal{};
+  g2ohip::DBuf<int> d_bal_cs_ptr, d_bal_cs_slot, d_bal_od_ptr, d_bal_od_blk;
+  g2ohip::DBuf<int2> d_bal_od_pair;
+  g2ohip::DBuf<int4> d_bal_rec;
+  g2ohip::DBuf<double> d_bal_tab, d_bal_Hpp, d_bal_Hpl, d_bal_Dinv, d_bal_F_val, d_bal_part;
+  long long bal_pairs = 0;
+
   // kernel timing (HIP events on `stream`)
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -211,12 +219,29 @@ struct g2ohip_ctx {
   int adjoint(const double* dL_dcam, const double* dL_dpt, double* dL_dmeas, double* dL_dw, double tol,
               int max_iter, int* iterations);
 
-  // state sizes saved by push / pop: 7 doubles per camera or pose, 3 per point
-  size_t state7() const { return mode == 1 ? static_cast<size_t>(pgst.N) : static_cast<size_t>(st.C); }
-  size_t state3() const { return mode == 1 ? 0 : static_cast<size_t>(st.P); }
-  size_t system_size() const {
-    return mode == 1 ? 6 * static_cast<size_t>(pgst.Nf) : 6 * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);
+  // state saved by push / pop, in doubles: 7 per camera or pose (9 per BAL
+  // camera), 3 per point
+  size_t cams_len() const {
+    return mode == 1 ? 7 * static_cast<size_t>(pgst.N) : (mode == 2 ? 9 : 7) * static_cast<size_t>(st.C);
   }
+  size_t pts_len() const { return mode == 1 ? 0 : 3 * static_cast<size_t>(st.P); }
+  // camera block size of the reduced system
+  int cam_dim() const { return mode == 2 ? 9 : 6; }
+  size_t system_size() const {
+    return mode == 1 ? 6 * static_cast<size_t>(pgst.Nf)
+                     : cam_dim() * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);
+  }
+  int bal_set_structure(int C, int P, int E, const int* ec, const int* ep, const uint8_t* cf, const uint8_t* pf);
+  int bal_set_measurements(const double* meas, const double* w, double huber);
+  int bal_set_state(const double* cams9, const double* pts);
+  int bal_get_state(double* cams9, double* pts);
+  int bal_chi2(double* chi2, double* err_out);
+  int bal_build();
+  int bal_lambda_init(double* out);
+  int bal_solve(int* ok);
+  int bal_update();
+  int bal_get_hessian(double* Hpp, double* Hll, double* Hpl_edge);
+  int bal_get_schur(int* rows, int* cols, double* blocks, double* bschur);
   int pg_set_structure(int N, int E, const int* from, const int* to, const uint8_t* fixed);
   int pg_set_measurements(const double* meas7, const double* info36);
   int pg_set_state(const double* poses7);

@@ -216,7 +216,7 @@ hipError_t launch_schur_diag(const DevGraph& g, const int* q_slot, const double*
 // entries, part_dq pcg_spmv_blocks(nb) (explicit) or
 // implicit_reduce_blocks(Cf) (implicit); both are at most nb. d and d_old
 // swap every iteration.
-int pcg_row_blocks(int nb);
+int pcg_row_blocks(int nb, int bd = 6);
 int pcg_spmv_blocks(int nb);
 int implicit_reduce_blocks(int Cf);
 template <typename MT, int BD>
@@ -265,6 +265,41 @@ hipError_t launch_pg_damp(const DevPG& pg, const double* Hdiag, double lambda, d
 hipError_t launch_pg_update(const DevPG& pg, const double* x, double* poses, hipStream_t st);
 hipError_t launch_pg_chi2(const DevPG& pg, const double* poses, double* part, double* err_out, hipStream_t st);
 hipError_t launch_pg_max_diag(int Nf, const double* Hdiag, double* out, hipStream_t st);
+
+// ---- BAL (9-DoF cameras, bal_example.cpp), bal.hip
+constexpr int BAL_TAB = 24;  // doubles per camera-table row
+struct DevBAL {
+  int C, P, Cf, Pf, ns, npos, nq, nO;
+  double huber;
+  const int *camH, *ptH, *cam_raw;
+  const int4* slot_rec;                 // {raw camera, raw point, q or -1, free camera or -1}
+  const double *meas, *w;               // per slot
+  const int *pos_ptr, *pos_h;           // point positions: slot ranges, free point index
+  const int *cs_ptr, *cs_slot;          // per free camera: its slots (any point)
+  const int *cam_qptr, *q_ph;           // per free camera: its Hpl blocks; free point per block
+  const int *od_ptr, *od_blk;           // off-diagonal upper blocks: pair ranges, upper block index
+  const int2* od_pair;                  // (q of the row camera, q of the column camera)
+  const int *F_diag, *up_pos, *up_posT;
+};
+hipError_t launch_bal_cam_table(int C, const double* cams, double* tab, hipStream_t st);
+int bal_chi2_blocks();
+int bal_point_blocks(int npos);
+// chi2 into *out; err_out (optional) 2 per slot
+hipError_t launch_bal_chi2(const DevBAL& g, const double* tab, const double* pts, double* part, double* out,
+                           double* err_out, hipStream_t st);
+// Hpp [Cf][81], Hll [Pf][6], Hpl [nq][27], b; the chi2 of the state into
+// *chi_out (part: bal_point_blocks(npos) partials)
+hipError_t launch_bal_linearize(const DevBAL& g, const double* tab, const double* pts, double* Hpp, double* Hll,
+                                double* Hpl, double* b, double* part, double* chi_out, hipStream_t st);
+hipError_t launch_bal_max_diag(const DevBAL& g, const double* Hpp, const double* Hll, double* out, hipStream_t st);
+// Dinv [Pf][9], the reduced system's blocks into F_val and bschur [9 Cf];
+// err |= 1 if a point block is singular
+hipError_t launch_bal_schur(const DevBAL& g, const double* Hpp, const double* Hll, const double* Hpl,
+                            const double* b, double lambda, double* Dinv, double* F_val, double* bschur, int* err,
+                            hipStream_t st);
+hipError_t launch_bal_backsub(const DevBAL& g, const double* Hpl, const double* Dinv, const double* b, double* x,
+                              hipStream_t st);
+hipError_t launch_bal_update(const DevBAL& g, const double* x, double* cams, double* pts, hipStream_t st);
 
 template <typename HT>
 hipError_t launch_to_double(int n, const HT* src, double* dst, double scale_add, hipStream_t st);

@@ -1755,13 +1755,18 @@ __device__ __forceinline__ double pcg_dir_val(double s, double d_old, double bet
 // process keeps its FP64 orthogonality, and the bandwidth-heavy data (the
 // matrix) moves at half width.
 
-// Row-parallel vector kernels (init, update) give each block row 8 lanes
-// (lane k < BD owns component k), PCG_RPW rows per workgroup of 8 * PCG_RPW
-// threads; the preconditioner product gathers the row's r components by lane
-// shuffles. One partial per workgroup.
-constexpr int PCG_RPW = 64;
-constexpr int PCG_ROW_THREADS = 8 * PCG_RPW;
+// Row-parallel vector kernels (init, update) give each block row LPR lanes
+// (8 for blocks of up to 8, else 16; lane k < BD owns component k), RPW rows
+// per workgroup of PCG_ROW_THREADS threads; the preconditioner product gathers
+// the row's r components by lane shuffles. One partial per workgroup.
+constexpr int PCG_ROW_THREADS = 512;
 constexpr int PCG_ROW_WAVES = PCG_ROW_THREADS / 64;
+template <int BD>
+struct PcgRowShape {
+  static_assert(BD >= 1 && BD <= 16, "block size");
+  static constexpr int LPR = BD <= 8 ? 8 : 16;
+  static constexpr int RPW = PCG_ROW_THREADS / LPR;
+};
 
 // Sum of one value per wavefront of a PCG_ROW_THREADS workgroup, in
 // wavefront order, by thread 0.
@@ -1817,7 +1822,8 @@ __global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_init(int nb, const doub
   __shared__ double s_w[PCG_ROW_WAVES];
   const int lane = threadIdx.x & 63;
   if (blockIdx.x == 0 && threadIdx.x == 0) *sc = PcgScalars{};
-  const int i = blockIdx.x * PCG_RPW + (threadIdx.x >> 3), k = threadIdx.x & 7;
+  constexpr int LPR = PcgRowShape<BD>::LPR;
+  const int i = blockIdx.x * PcgRowShape<BD>::RPW + (threadIdx.x / LPR), k = threadIdx.x & (LPR - 1);
   const bool on = i < nb && k < BD;
   // loads from clamped indices (all in flight together), values selected
   const int ic = on ? i : 0, kc = on ? k : 0;
@@ -1833,7 +1839,7 @@ __global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_init(int nb, const doub
   }
   double v = 0.0;
 #pragma unroll
-  for (int c = 0; c < BD; ++c) v = fma(jr[c], __shfl(rv, (lane & ~7) + c, 64), v);
+  for (int c = 0; c < BD; ++c) v = fma(jr[c], __shfl(rv, (lane & ~(LPR - 1)) + c, 64), v);
   double dot = 0.0;
   if (on) {
     s[o] = v;
@@ -2073,7 +2079,8 @@ __global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_update(
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   PcgPartials pr;
   if (wv == 0) pr = pcg_part_load(part_dq, n_dq);
-  const int i = blockIdx.x * PCG_RPW + (threadIdx.x >> 3), k = threadIdx.x & 7;
+  constexpr int LPR = PcgRowShape<BD>::LPR;
+  const int i = blockIdx.x * PcgRowShape<BD>::RPW + (threadIdx.x / LPR), k = threadIdx.x & (LPR - 1);
   const bool on = i < nb && k < BD;
   // operands first (independent of alpha), then the reduction of d . q;
   // loads from clamped indices so they are all in flight together
@@ -2107,7 +2114,7 @@ __global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_update(
   }
   double v = 0.0;
 #pragma unroll
-  for (int c = 0; c < BD; ++c) v += (on ? jr[c] : 0.0) * __shfl(rv, (lane & ~7) + c, 64);
+  for (int c = 0; c < BD; ++c) v += (on ? jr[c] : 0.0) * __shfl(rv, (lane & ~(LPR - 1)) + c, 64);
   double dot = 0.0;
   if (on) {
     s[o] = v;
@@ -2116,14 +2123,14 @@ __global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_update(
   pcg_block_partial(wave_total(dot), s_w, part_rs);
 }
 
-int pcg_row_blocks(int nb) { return nblocks(nb, PCG_RPW); }
+int pcg_row_blocks(int nb, int bd) { return nblocks(nb, bd <= 8 ? PCG_ROW_THREADS / 8 : PCG_ROW_THREADS / 16); }
 int pcg_spmv_blocks(int nb) { return nblocks(nb, PCG_SPMV_ROWS); }
 int implicit_reduce_blocks(int Cf) { return nblocks(Cf, 64); }
 
 template <typename MT, int BD>
 hipError_t launch_pcg_init(int nb, const double* b, const MT* Jinv, double* x, double* r, double* s,
                            double* part_rs, PcgScalars* sc, hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(PCG_ROW_THREADS), 0, st, nb, b, Jinv, x,
+  hipLaunchKernelGGL((k_pcg_init<MT, BD>), dim3(pcg_row_blocks(nb, BD)), dim3(PCG_ROW_THREADS), 0, st, nb, b, Jinv, x,
                      r, s, part_rs, sc);
   return hipGetLastError();
 }
@@ -2136,11 +2143,13 @@ hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st,
                                const double* Ddiag, long long nnz) {
-  constexpr int HS_LONG = (64 + 64 / BD - 1) / (64 / BD);
+  // block slots per group for long rows: a chunk of 64 blocks, fewer for
+  // blocks above 8 (their register footprint per slot)
+  constexpr int HS_LONG = BD <= 8 ? (64 + 64 / BD - 1) / (64 / BD) : 5;
   const bool short_rows = nnz >= 0 && nnz < static_cast<long long>(PCG_SHORT_ROW) * nb;
 #define G2OHIP_SPMV_LAUNCH(FIRST, D64, HS)                                                                        \
   hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, FIRST, D64, HS>), dim3(pcg_spmv_blocks(nb)), dim3(PCG_SPMV_THREADS), \
-                     0, st, nb, F_ptr, F_col, F_val, Ddiag, s, d_old, d, q, part_rs, pcg_row_blocks(nb), part_dq, \
+                     0, st, nb, F_ptr, F_col, F_val, Ddiag, s, d_old, d, q, part_rs, pcg_row_blocks(nb, BD), part_dq, \
                      sc, ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last)
 #define G2OHIP_SPMV_SHAPE(HS)                         \
   if (Ddiag) {                                        \
@@ -2164,7 +2173,7 @@ template <int BD>
 hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
                           PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st) {
   hipLaunchKernelGGL((k_pcg_dir<BD>), dim3(nblocks(nb, 64)), dim3(64), 0, st, nb, s, d_old, d, part_rs,
-                     pcg_row_blocks(nb), sc, ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
+                     pcg_row_blocks(nb, BD), sc, ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last);
   return hipGetLastError();
 }
 template hipError_t launch_pcg_dir<6>(int, const double*, const double*, double*, const double*, PcgScalars*,
@@ -2174,7 +2183,7 @@ template <typename MT, int BD>
 hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const double* d, const double* q,
                              double* s, const double* part_dq, int n_dq, double* part_rs, PcgScalars* sc, int it,
                              hipStream_t st) {
-  hipLaunchKernelGGL((k_pcg_update<MT, BD>), dim3(pcg_row_blocks(nb)), dim3(PCG_ROW_THREADS), 0, st, nb, Jinv, x,
+  hipLaunchKernelGGL((k_pcg_update<MT, BD>), dim3(pcg_row_blocks(nb, BD)), dim3(PCG_ROW_THREADS), 0, st, nb, Jinv, x,
                      r, d, q, s, part_dq, n_dq, part_rs, sc, it);
   return hipGetLastError();
 }
@@ -2193,6 +2202,7 @@ hipError_t launch_pcg_update(int nb, const MT* Jinv, double* x, double* r, const
 G2OHIP_PCG_INST(float, 6)
 G2OHIP_PCG_INST(double, 6)
 G2OHIP_PCG_INST(double, 3)
+G2OHIP_PCG_INST(double, 9)
 #undef G2OHIP_PCG_INST
 
 // y = A x over 6x6 blocks: block-CSR (F_ptr, F_col, F_val) or, with F_ptr

@@ -23,8 +23,10 @@ void csr_from_counts(std::vector<T>& ptr) {
 }  // namespace
 
 std::string build_structure(BAStructure& st, int C, int P, int E, const int* edge_cam, const int* edge_pt,
-                            const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur) {
+                            const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur,
+                            const LinPlanCaps* caps) {
   st = BAStructure();
+  if (caps) st.lcaps = *caps;
   if (C <= 0 || P < 0 || E < 0) return "invalid sizes";
   st.C = C; st.P = P; st.E = E;
   for (int k = 0; k < E; ++k) {

@@ -108,9 +108,11 @@ struct BAStructure {
 };
 
 // Builds everything except the Schur plans. Returns an empty string on
-// success, else an error message.
+// success, else an error message. caps (optional): the linearisation chunks'
+// packing targets and per-point limit, instead of the defaults.
 std::string build_structure(BAStructure& st, int C, int P, int E, const int* edge_cam, const int* edge_pt,
-                            const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur);
+                            const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur,
+                            const LinPlanCaps* caps = nullptr);
 
 // Builds a Schur window plan (windows of at most obs_max observations,
 // partials of at most pch pairs; obs_max <= 65535).

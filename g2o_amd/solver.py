@@ -473,3 +473,154 @@ def linear_solve_pcg_device(ctx: Context, nb: int, bd: int, F_ptr, F_col, F_val,
     ctx.call("g2ohip_linear_solve_pcg_device", nb, bd, addr(F_ptr), addr(F_col), addr(F_val), addr(F_diag),
              addr(b), addr(x), float(tol), int(max_iter), ct.byref(r), ct.byref(it))
     return it.value, r.value
+
+
+class BALAdjuster:
+    """SparseOptimizer with OptimizationAlgorithmLevenberg and
+    BlockSolver<BlockSolverTraits<9, 3>> + LinearSolverPCG over g2o's BAL
+    types (g2o/examples/bal/bal_example.cpp:70-207, 227-341): VertexCameraBAL
+    [w, t, f, k1, k2] with additive updates, VertexPointBAL, EdgeObservationBAL
+    (``g2o_amd.datasets.BALProblem``). FP64, explicit Schur."""
+
+    def __init__(self, prob, device: int = 0, pcg_tolerance: float = 1e-6, pcg_max_iter: int = -1,
+                 max_trials: int = 10):
+        self.ctx = Context(device)
+        self.prob = prob
+        self.ctx.set_options(precision=L.FP64, schur_mode=L.SCHUR_EXPLICIT, pcg_tolerance=pcg_tolerance,
+                             pcg_max_iter=pcg_max_iter, max_trials=max_trials)
+        self.C, self.P, self.E = prob.num_cams, prob.num_pts, prob.num_edges
+        self._ec = np.ascontiguousarray(prob.edge_cam, dtype=np.int32)
+        self._ep = np.ascontiguousarray(prob.edge_pt, dtype=np.int32)
+        cf = prob.cam_fixed if prob.cam_fixed is not None else np.zeros(self.C, np.uint8)
+        pf = prob.pt_fixed if prob.pt_fixed is not None else np.zeros(self.P, np.uint8)
+        self._cf = np.ascontiguousarray(cf, dtype=np.uint8)
+        self._pf = np.ascontiguousarray(pf, dtype=np.uint8)
+        self.ctx.call("g2ohip_bal_set_structure", self.C, self.P, self.E, _ip(self._ec), _ip(self._ep),
+                      _up(self._cf), _up(self._pf))
+        info = self.structure_info()
+        self.Cf, self.Pf, self.nschur = int(info[0]), int(info[1]), int(info[3])
+        if device >= 0:
+            self.set_measurements(prob.meas, prob.weight, prob.huber)
+            self.set_state(prob.cams, prob.pts)
+
+    def set_options(self, **kw):
+        return self.ctx.set_options(**kw)
+
+    def set_algorithm(self, algorithm: int):
+        """SparseOptimizer::setAlgorithm: L.LEVENBERG or L.GAUSS_NEWTON."""
+        self.ctx.call("g2ohip_set_algorithm", int(algorithm))
+
+    def structure_info(self) -> np.ndarray:
+        """[free cameras, free points, active edges, Hschur upper blocks, blocks
+        in both triangles, off-diagonal pair products, Hpl blocks]"""
+        info = (ct.c_longlong * 7)()
+        self.ctx.check(self.ctx._lib.g2ohip_bal_structure_info(self.ctx.h, info))
+        return np.array(info[:], dtype=np.int64)
+
+    def set_measurements(self, meas, weight=None, huber: float = 0.0):
+        m = np.ascontiguousarray(meas, dtype=np.float64)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+        self._meas, self._w = m, w
+        self.ctx.call("g2ohip_bal_set_measurements", _dp(m), None if w is None else _dp(w), float(huber or 0.0))
+
+    def set_state(self, cams, pts):
+        c = np.ascontiguousarray(cams, dtype=np.float64)
+        p = np.ascontiguousarray(pts, dtype=np.float64)
+        self.ctx.call("g2ohip_bal_set_state", _dp(c), _dp(p))
+
+    def state(self):
+        c = np.zeros((self.C, 9))
+        p = np.zeros((self.P, 3))
+        self.ctx.call("g2ohip_bal_get_state", _dp(c), _dp(p))
+        return c, p
+
+    def compute_active_errors(self) -> float:
+        v = ct.c_double()
+        self.ctx.call("g2ohip_compute_active_errors", ct.byref(v))
+        return v.value
+
+    def errors(self) -> np.ndarray:
+        e = np.zeros((self.E, 2))
+        self.ctx.call("g2ohip_get_errors", _dp(e))
+        return e
+
+    def optimize(self, iterations: int):
+        stats = (L.IterStats * max(iterations, 1))()
+        done = ct.c_int()
+        self.ctx.call("g2ohip_optimize", iterations, stats, ct.byref(done))
+        return [{f: getattr(s, f) for f, _ in L.IterStats._fields_} for s in stats[:done.value]]
+
+    def build_system(self):
+        self.ctx.call("g2ohip_build_system")
+
+    def set_lambda(self, lam: float):
+        self.ctx.call("g2ohip_set_lambda", float(lam))
+
+    def solve(self) -> bool:
+        ok = ct.c_int()
+        self.ctx.call("g2ohip_solve", ct.byref(ok))
+        return bool(ok.value)
+
+    def x(self) -> np.ndarray:
+        v = np.zeros(9 * self.Cf + 3 * self.Pf)
+        self.ctx.call("g2ohip_get_x", _dp(v))
+        return v
+
+    def b(self) -> np.ndarray:
+        v = np.zeros(9 * self.Cf + 3 * self.Pf)
+        self.ctx.call("g2ohip_get_b", _dp(v))
+        return v
+
+    def update(self):
+        self.ctx.call("g2ohip_update")
+
+    def push(self):
+        self.ctx.call("g2ohip_push")
+
+    def pop(self):
+        self.ctx.call("g2ohip_pop")
+
+    def discard_top(self):
+        self.ctx.call("g2ohip_discard_top")
+
+    def lambda_init(self) -> float:
+        v = ct.c_double()
+        self.ctx.call("g2ohip_lambda_init", ct.byref(v))
+        return v.value
+
+    def compute_scale(self) -> float:
+        v = ct.c_double()
+        self.ctx.call("g2ohip_compute_scale", ct.byref(v))
+        return v.value
+
+    def hessian(self):
+        Hpp = np.zeros((self.Cf, 9, 9))
+        Hll = np.zeros((self.Pf, 3, 3))
+        Hpl = np.zeros((self.E, 9, 3))
+        self.ctx.call("g2ohip_bal_get_hessian", _dp(Hpp), _dp(Hll), _dp(Hpl))
+        return Hpp, Hll, Hpl
+
+    def schur(self):
+        rows = np.zeros(self.nschur, dtype=np.int32)
+        cols = np.zeros(self.nschur, dtype=np.int32)
+        blocks = np.zeros((self.nschur, 9, 9))
+        bs = np.zeros(9 * self.Cf)
+        self.ctx.call("g2ohip_bal_get_schur", _ip(rows), _ip(cols), _dp(blocks), _dp(bs))
+        return rows, cols, blocks, bs
+
+    def last_solve_info(self):
+        v = np.zeros(4)
+        self.ctx.call("g2ohip_last_solve_info", _dp(v))
+        return v
+
+    def set_profiling(self, on: bool = True):
+        self.ctx.call("g2ohip_set_profiling", int(on))
+
+    def profile(self) -> dict:
+        out = {}
+        for k, name in enumerate(L.KERNEL_NAMES):
+            t = ct.c_double()
+            n = ct.c_int()
+            self.ctx.call("g2ohip_get_profile", k, ct.byref(t), ct.byref(n))
+            out[name] = (t.value, n.value)
+        return out

@@ -321,6 +321,50 @@ int g2ohip_pg_get_state(g2ohip_ctx* ctx, double* poses);
  * b [6 * free poses]. NULL skips. */
 int g2ohip_pg_get_system(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* b);
 
+/* ------------------------------------------------ BAL (9-DoF cameras) */
+/* g2o's BAL example (g2o/examples/bal/bal_example.cpp:70-207, 227-341):
+ * VertexCameraBAL (9 values: angle-axis rotation w, translation t, focal
+ * length f, radial distortion k1 k2; oplusImpl adds the update, :86-89),
+ * VertexPointBAL (3 values, +), EdgeObservationBAL (error = f r(p) p - z with
+ * P = R(w) X + t, p = -P.xy / P.z, r = 1 + k1 |p|^2 + k2 |p|^4, :154-204),
+ * solved by BlockSolver<BlockSolverTraits<9, 3>> with Schur and
+ * LinearSolverPCG (:227-250) under OptimizationAlgorithmLevenberg. The
+ * reference differentiates the error automatically (G2O_MAKE_AUTO_AD_FUNCTIONS,
+ * g2o/core/auto_differentiation.h:155-302); the kernels use the closed form.
+ * After g2ohip_bal_set_structure the context is a BAL context: the Solver
+ * calls above (compute_active_errors, get_errors, build_system, set_lambda,
+ * solve, get_x / get_b with x = [9 Cf camera values, 3 Pf point values],
+ * update, push / pop / discard_top, lambda_init, compute_scale, optimize
+ * with Levenberg-Marquardt or Gauss-Newton) act on it. FP64 with the explicit
+ * Schur complement only (other options: G2OHIP_ERR_UNSUPPORTED); no Dogleg,
+ * multiplyHessian or marginals. */
+
+/* SparseOptimizer::initializeOptimization + BlockSolver<9, 3>::buildStructure
+ * (block_solver.hpp:110-269): edge_cam / edge_pt [E], fixed masks [C] / [P]
+ * (NULL = none fixed, as bal_example.cpp leaves them). A camera that observes
+ * one point twice is G2OHIP_ERR_UNSUPPORTED. */
+int g2ohip_bal_set_structure(g2ohip_ctx* ctx, int num_cams, int num_pts, int num_edges, const int* edge_cam,
+                             const int* edge_pt, const uint8_t* cam_fixed, const uint8_t* pt_fixed);
+/* info[0..6] = free cameras, free points, active edges, Hschur upper blocks,
+ * Hschur blocks in both triangles, off-diagonal Schur pair products, Hpl
+ * blocks */
+int g2ohip_bal_structure_info(g2ohip_ctx* ctx, long long* info);
+/* EdgeObservationBAL::setMeasurement / setInformation (bal_example.cpp:
+ * 305-315): meas[E*2] pixels relative to the image centre, weight[E] > 0
+ * (information weight * I2, NULL = the identity the example sets);
+ * huber_delta > 0 attaches RobustKernelHuber(delta) to every edge. */
+int g2ohip_bal_set_measurements(g2ohip_ctx* ctx, const double* meas, const double* weight, double huber_delta);
+/* VertexCameraBAL / VertexPointBAL estimates: cams[C*9], pts[P*3] */
+int g2ohip_bal_set_state(g2ohip_ctx* ctx, const double* cams, const double* pts);
+int g2ohip_bal_get_state(g2ohip_ctx* ctx, double* cams, double* pts);
+/* Blocks of the last build_system: Hpp[Cf*81], Hll[Pf*9], Hpl_edge[E*27]
+ * (each row-major, optional; an edge without a block gets zeros). */
+int g2ohip_bal_get_hessian(g2ohip_ctx* ctx, double* Hpp, double* Hll, double* Hpl_edge);
+/* The reduced camera system of the last solve (damped by its lambda): upper
+ * blocks rows[], cols[], blocks[81 each] in (row, col) order, count info[3]
+ * of g2ohip_bal_structure_info, and bschur[9 Cf]. */
+int g2ohip_bal_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, double* bschur);
+
 /* ------------------------------------------------------- g2o file format */
 /* Host-side graph container for the path's vertex and edge types, read from
  * and written to g2o's text format (OptimizableGraph::load / save,

@@ -2519,6 +2519,7 @@ double orc_bal_linearize(void* h, double* Hpp, double* Hll, double* Hpl_edge, do
 // block is singular.
 int orc_bal_schur(void* h, double lambda, int* rows, int* cols, double* blocks, double* bschur) {
   BAL* p = static_cast<BAL*>(h);
+  if (p->Hpp.empty()) bal_build_system(*p);  // not linearised yet
   p->lambda = lambda;
   const int saved = p->solver_kind;
   p->solver_kind = 0;
@@ -2537,6 +2538,7 @@ int orc_bal_schur(void* h, double lambda, int* rows, int* cols, double* blocks, 
 // the residual memory reset): dx[9Cf+3Pf]. Returns 0 or 2 (not PD).
 int orc_bal_solve(void* h, double lambda, int solver, double tol, int maxiter, double* dx, int* pcg_iters) {
   BAL* p = static_cast<BAL*>(h);
+  if (p->Hpp.empty()) bal_build_system(*p);  // not linearised yet
   p->lambda = lambda;
   p->solver_kind = solver;
   p->pcg = PCGState();

@@ -220,3 +220,29 @@ def test_bal_lm_reaches_statistical_chi2(O):
     ora2 = O.OracleBAL(bal)
     n2, tr2, _ = ora2.optimize(20, solver=1, tol=1e-12)
     assert abs(tr2[-1, 1] - tr[-1, 1]) <= 1e-6 * tr[-1, 1]
+
+
+def test_bal_structure_on_the_host(O):
+    """The C ABI's BAL structure (host-only context): free vertices, the
+    Hschur pattern of the oracle, one pair product per two observations of a
+    free point by free cameras; a camera seeing a point twice is rejected."""
+    import g2o_amd
+    from g2o_amd.solver import G2oHipError
+    bal = _small_problem(0.0)
+    ba = g2o_amd.BALAdjuster(bal, device=-1)
+    info = ba.structure_info()
+    ora = O.OracleBAL(bal)
+    assert (info[0], info[1], info[3]) == (ora.Cf, ora.Pf, ora.nschur)
+    cf, pf = bal.cam_fixed.astype(bool), bal.pt_fixed.astype(bool)
+    keep = ~cf[bal.edge_cam] & ~pf[bal.edge_pt]
+    n = np.bincount(bal.edge_pt[keep], minlength=bal.num_pts)
+    assert info[5] == int((n * (n - 1) // 2).sum())
+    assert info[6] == int(keep.sum())
+    dup = bal.copy()
+    k = int(np.flatnonzero(keep)[0])  # an edge between a free camera and a free point
+    dup.edge_cam = np.r_[dup.edge_cam, dup.edge_cam[k]].astype(np.int32)
+    dup.edge_pt = np.r_[dup.edge_pt, dup.edge_pt[k]].astype(np.int32)
+    dup.meas = np.r_[dup.meas, dup.meas[k:k + 1]]
+    with pytest.raises(G2oHipError) as ex:
+        g2o_amd.BALAdjuster(dup, device=-1)
+    assert ex.value.status == g2o_amd._lib.ERR_UNSUPPORTED
