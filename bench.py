@@ -10,8 +10,9 @@ path; the plain FP64 GPU path runs the same iterations from the same start and
 its rate and the relative final-chi2 difference are reported in `config`.
 
 Further legs (reported in `legs`, not in `value`): the implicit (matrix-free)
-Schur mode on the same workload, and the 10,000-pose sphere pose graph
-(EdgeSE3, PCG tolerance 1e-8).
+Schur mode on the same workload, the 10,000-pose sphere pose graph (EdgeSE3,
+PCG tolerance 1e-8), the Zipf-track BA (4,000 cameras, 1.2M points) and a
+BAL problem (9-DoF cameras with radial distortion) of the headline's shape.
 
 Prints one JSON line. Only N=1 is supported (one GPU).
 """
@@ -227,6 +228,82 @@ def zipf_leg(warmup, steps=15):
             "schur_offdiag_pairs": int(info[6]), "schur_windows": int(info[5])}
 
 
+def bal_stage_bytes(info, C, E):
+    """Compulsory HBM bytes of one LM trial of the BAL path (FP64 blocks)."""
+    Cf, Pf, ns, nS, nF, pairs, nq = (int(v) for v in info[:7])
+    nO = nS - Cf
+    slot_in = ns * (16 + 16 + 8)  # slot record, measurement, weight
+    return {
+        # errors and Jacobians from the state; Hll, b_l per point, Hpp, b_p per camera, Hpl per observation
+        "linearize": slot_in + C * (72 + 192) + Pf * (24 + 48 + 24) + Cf * (648 + 72) + nq * 216,
+        # Hpp, Hll, b and every Hpl block in, Dinv per point, the reduced system (both triangles) and bschur out;
+        # the pair lists are the path's own index data
+        "schur": Cf * (648 + 72) + Pf * (48 + 24 + 72) + nq * 216 + pairs * 8 + nF * 648 + Cf * 72,
+        # one SpMV over the full block rows plus the vectors
+        "pcg_spmv": nF * (648 + 4) + Cf * 4 * 72,
+    }
+
+
+def bal_leg(steps, warmup, cpu_iters):
+    """BAL problem (9-DoF cameras with radial distortion, bal_example.cpp) of
+    the headline's shape: the random-walk cameras and tracks of the headline
+    workload in the BAL camera model, FP64, explicit Schur, block-Jacobi PCG
+    on 9x9 blocks, nothing fixed. CPU: the oracle's LM from the same start."""
+    import g2o_amd
+    from g2o_amd import datasets
+    from oracle import oracle
+    t = time.perf_counter()
+    prob = datasets.make_walk_bal(seed=1005)
+    gen = time.perf_counter() - t
+    ba = g2o_amd.BALAdjuster(prob, device=0)
+    if warmup > 0:
+        ba.optimize(warmup)
+        ba.set_state(prob.cams, prob.pts)
+    t = time.perf_counter()
+    st = ba.optimize(steps)
+    dt = time.perf_counter() - t
+    ba.set_state(prob.cams, prob.pts)
+    ba.set_profiling(True)
+    pst = ba.optimize(steps)
+    prof = ba.profile()
+    ba.set_profiling(False)
+    info = ba.structure_info()
+    del ba
+    trials = sum(s["levenberg_iterations"] for s in pst)
+    pcg = sum(s["iterations_linear_solver"] for s in pst)
+    alg = bal_stage_bytes(info, prob.num_cams, prob.num_edges)
+    stages = {}
+    for k, (ms, n) in (("linearize", prof["linearize"]), ("schur", prof["schur"])):
+        sec = ms / 1e3 / max(n, 1)
+        stages[k] = {"time_us": round(sec * 1e6, 1), "algorithmic_bytes": int(alg[k]),
+                     "achieved_GBps": round(alg[k] / sec / 1e9, 1) if sec > 0 else None,
+                     "frac_of_8TBps": round(alg[k] / sec / PEAK_HBM, 4) if sec > 0 else None}
+    pcg_sec = prof["pcg"][0] / 1e3 / max(pcg, 1)
+    stages["pcg_iteration"] = {"time_us": round(pcg_sec * 1e6, 2), "spmv_algorithmic_bytes": int(alg["pcg_spmv"])}
+    rec = {}
+    if cpu_iters > 0:
+        L = oracle.lib()
+        ora = oracle.OracleBAL(prob)
+        t = time.perf_counter()
+        n, trace, _ = ora.optimize(cpu_iters, solver=1, tol=1e-6)
+        cdt = time.perf_counter() - t
+        k = min(n, len(st))
+        rec["cpu_baseline"] = {"value": round(n / cdt, 3), "unit": "LM iter/s", "cores": int(L.orc_num_threads()),
+                               "kind": "port", "sample": f"{n} oracle LM iterations from the same start",
+                               "rel_chi2_diff_vs_gpu_per_iter": rel_list([s["chi2"] for s in st[:k]],
+                                                                         [float(c) for c in trace[:k, 1]])}
+    return {"workload": f"BAL random walk {prob.num_cams} cameras / {prob.num_pts} points / {prob.num_edges} "
+                        f"observations, radial distortion, seed 1005 ({gen:.1f} s to generate)",
+            "path": "FP64, BlockSolver<9,3>: 9x9 Schur complement, block-Jacobi PCG tol 1e-6",
+            "steps": len(st), "iter_per_s": round(len(st) / dt, 3), "ms_per_step": round(dt / len(st) * 1e3, 3),
+            "chi2_start": st[0]["chi2_before"], "final_chi2": st[-1]["chi2"],
+            "rerun_rel_final_chi2_diff": abs(pst[-1]["chi2"] - st[-1]["chi2"]) / st[-1]["chi2"],
+            "lm_trials": trials, "pcg_iterations": pcg,
+            "kernel_ms_profiled_run": {kk: round(v[0], 3) for kk, v in prof.items() if kk in g2o_amd.KERNEL_NAMES},
+            "stages": stages,
+            "hschur_upper_blocks": int(info[3]), "schur_offdiag_pairs": int(info[5]), **rec}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +313,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=None, help="oracle LM iterations (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the implicit-Schur and pose-graph legs")
+    ap.add_argument("--leg", choices=["bal", "zipf_tracks", "pose_graph"], default=None,
+                    help="run only this leg (--steps LM iterations, 1 warm-up) and print its JSON")
     ap.add_argument("--cams", type=int, default=1500)
     ap.add_argument("--points", type=int, default=500_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -259,6 +338,16 @@ def main():
         sys.exit(2)
     import g2o_amd
     from g2o_amd import datasets
+
+    if args.leg:
+        if args.leg == "bal":
+            rec = bal_leg(args.steps, 1, 0 if args.no_cpu else 3)
+        elif args.leg == "zipf_tracks":
+            rec = zipf_leg(1, args.steps)
+        else:
+            rec = pose_graph_leg(args.steps)
+        print(json.dumps({"leg": args.leg, **rec}), flush=True)
+        return
 
     t = time.perf_counter()
     prob = datasets.make_walk_ba(seed=1002, num_cams=args.cams, num_pts=args.points)
@@ -386,6 +475,9 @@ def main():
         log("leg: Zipf-track BA ...")
         out["legs"]["zipf_tracks"] = zipf_leg(1)
         log(f"zipf: {out['legs']['zipf_tracks']}")
+        log("leg: BAL 9-DoF ...")
+        out["legs"]["bal"] = bal_leg(args.steps, 1, 3)
+        log(f"bal: {out['legs']['bal']}")
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"], chi_o = cpu_baseline(prob, args.cpu_iters or args.steps)

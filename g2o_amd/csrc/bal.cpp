@@ -130,6 +130,7 @@ int g2ohip_ctx::bal_set_structure(int C, int P, int E, const int* ec, const int*
   HK(d_bal_Hpp.alloc(81 * static_cast<size_t>(st.Cf)));
   HK(d_bal_Hpl.alloc(27 * static_cast<size_t>(std::max(st.nq, 1))));
   HK(d_bal_Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
+  HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
   HK(d_bal_F_val.alloc(81 * static_cast<size_t>(st.nF)));
   HK(d_bschur.alloc(9 * static_cast<size_t>(st.Cf)));
   HK(d_bal_part.alloc(std::max(bal_point_blocks(st.npos), bal_chi2_blocks())));
@@ -245,7 +246,7 @@ int g2ohip_ctx::bal_build() {
 
 int g2ohip_ctx::bal_lambda_init(double* out) {
   if (opt.user_lambda_init > 0) { *out = opt.user_lambda_init; return G2OHIP_OK; }
-  HK(launch_bal_max_diag(bal, d_bal_Hpp.p, d_Hll.p, d_scal.p + 2, stream));
+  HK(launch_bal_max_diag(bal, d_bal_Hpp.p, d_Hll.p, d_red.p, d_scal.p + 2, stream));
   double m;
   CK(read_scalar(d_scal.p + 2, &m));
   *out = opt.tau * m;
@@ -265,8 +266,9 @@ int g2ohip_ctx::bal_solve(int* ok) {
   HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.d2.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
   {
     ProfScope ps(this, G2OHIP_K_SCHUR);
-    HK(launch_bal_schur(bal, d_bal_Hpp.p, d_Hll.p, d_bal_Hpl.p, d_b.p, lambda, d_bal_Dinv.p, d_bal_F_val.p,
-                        d_bschur.p, d_err.p, stream));
+    if (bal_mfma) HK(d_bal_G.alloc(27 * static_cast<size_t>(std::max(st.nq, 1))));
+    HK(launch_bal_schur(bal, d_bal_Hpp.p, d_Hll.p, d_bal_Hpl.p, d_b.p, lambda, d_bal_Dinv.p, d_linv.p, d_bal_G.p,
+                        d_bal_F_val.p, d_bschur.p, d_err.p, bal_mfma, stream));
     HK((launch_jacobi<double, 9>(nb, d_bal_F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream)));
   }
   int err = 0;

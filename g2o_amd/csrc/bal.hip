@@ -13,10 +13,10 @@
 //    free point, camera-major (q order), so a camera's blocks are one run;
 //  * Hpp [Cf][81], Hll 6 unique per free point, b [9 Cf + 3 Pf];
 //  * Hschur: full block-CSR (both triangles) of 9x9 blocks, F_val[nF][81].
-// Every reduction has a fixed order: camera sums are wave reductions over a
-// camera's own observation list, point sums a thread's loop over the point's
-// slots, off-diagonal Schur blocks a wavefront's loop over the block's
-// observation pairs; there are no atomics.
+// Every reduction has a fixed order: camera sums are one workgroup's
+// reduction over the camera's own observation list, point sums a thread's
+// loop over the point's slots, off-diagonal Schur blocks a wavefront's loop
+// over the block's observation pairs; no floating-point atomics.
 #include <hip/hip_runtime.h>
 
 #include "devmath.h"
@@ -210,9 +210,10 @@ hipError_t launch_bal_chi2(const DevBAL& g, const double* tab, const double* pts
 //  * point pass, one thread per point position: Hll and b_l of a free point
 //    from its slots in order, and the chi2 of every slot (per-workgroup
 //    partials);
-//  * camera pass, one wavefront per free camera: Hpp and b_p from the
-//    camera's slots (lanes stride the list, then a wave reduction), and the
-//    Hpl block of each of its observations with a free point.
+//  * camera pass, one workgroup per free camera: Hpp and b_p from the
+//    camera's slots (threads stride the list, then wave and workgroup
+//    reductions), and the Hpl block of each of its observations with a free
+//    point.
 __global__ __launch_bounds__(256) void k_bal_lin_points(DevBAL g, const double* __restrict__ tab,
                                                         const double* __restrict__ pts, double* __restrict__ Hll,
                                                         double* __restrict__ b, double* __restrict__ part) {
@@ -263,15 +264,15 @@ int bal_point_blocks(int npos) { return bal_blocks(npos > 0 ? npos : 1, 256); }
 __global__ __launch_bounds__(256) void k_bal_lin_cams(DevBAL g, const double* __restrict__ tab,
                                                       const double* __restrict__ pts, double* __restrict__ Hpp,
                                                       double* __restrict__ b, double* __restrict__ Hpl) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= g.Cf) return;
+  __shared__ double s_acc[4][54];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c = blockIdx.x;
   const int j0 = g.cs_ptr[c], j1 = g.cs_ptr[c + 1];
   const double* tb = tab + BAL_TAB * static_cast<size_t>(g.cam_raw[c]);
   double acc[54];
 #pragma unroll
   for (int k = 0; k < 54; ++k) acc[k] = 0.0;
-  for (int j = j0 + lane; j < j1; j += 64) {
+  for (int j = j0 + threadIdx.x; j < j1; j += 256) {
     const int s = g.cs_slot[j];
     const int4 rec = g.slot_rec[s];
     double X[3];
@@ -300,15 +301,20 @@ __global__ __launch_bounds__(256) void k_bal_lin_cams(DevBAL g, const double* __
 #pragma unroll
   for (int k = 0; k < 54; ++k) acc[k] = bal_wave_sum(acc[k]);
   if (lane == 0) {
-    double* H = Hpp + 81 * static_cast<size_t>(c);
 #pragma unroll
-    for (int k = 0; k < 45; ++k) {
-      const int r = c_up9_r[k], cc = c_up9_c[k];
-      H[9 * r + cc] = acc[k];
-      H[9 * cc + r] = acc[k];
+    for (int k = 0; k < 54; ++k) s_acc[wv][k] = acc[k];
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < 54) {
+    const double v = (s_acc[0][t] + s_acc[1][t]) + (s_acc[2][t] + s_acc[3][t]);
+    if (t < 45) {
+      const int r = c_up9_r[t], cc = c_up9_c[t];
+      Hpp[81 * static_cast<size_t>(c) + 9 * r + cc] = v;
+      Hpp[81 * static_cast<size_t>(c) + 9 * cc + r] = v;
+    } else {
+      b[9 * static_cast<size_t>(c) + (t - 45)] = v;
     }
-#pragma unroll
-    for (int r = 0; r < 9; ++r) b[9 * static_cast<size_t>(c) + r] = acc[45 + r];
   }
 }
 
@@ -318,114 +324,159 @@ hipError_t launch_bal_linearize(const DevBAL& g, const double* tab, const double
   hipLaunchKernelGGL(k_bal_lin_points, dim3(nb), dim3(256), 0, st, g, tab, pts, Hll, b, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bal_lin_cams, dim3(bal_blocks(g.Cf, 4)), dim3(256), 0, st, g, tab, pts, Hpp, b, Hpl);
+  hipLaunchKernelGGL(k_bal_lin_cams, dim3(g.Cf), dim3(256), 0, st, g, tab, pts, Hpp, b, Hpl);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return launch_sum(part, nb, chi_out, st);
 }
 
 // ------------------------------------------------------------- LM scalars
 // computeLambdaInit (optimization_algorithm_levenberg.cpp:155-167): the
-// largest |diagonal entry| of Hpp and Hll; one workgroup.
-__global__ __launch_bounds__(256) void k_bal_max_diag(DevBAL g, const double* __restrict__ Hpp,
-                                                      const double* __restrict__ Hll, double* __restrict__ out) {
-  __shared__ double s_m[256];
-  double m = 0.0;
-  for (long long i = threadIdx.x; i < 9LL * g.Cf; i += 256)
-    m = fmax(m, fabs(Hpp[81 * (i / 9) + 10 * (i % 9)]));
-  for (long long i = threadIdx.x; i < 3LL * g.Pf; i += 256) {
-    const int k = static_cast<int>(i % 3);
-    m = fmax(m, fabs(Hll[6 * (i / 3) + (k == 0 ? 0 : (k == 1 ? 3 : 5))]));
-  }
+// largest |diagonal entry| of Hpp and Hll; a grid-stride pass writes one
+// maximum per workgroup, one workgroup takes the maximum of those.
+constexpr int BAL_MAX_BLOCKS = 256;
+__device__ __forceinline__ double bal_block_max(double m, double* s_m) {
   s_m[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) s_m[threadIdx.x] = fmax(s_m[threadIdx.x], s_m[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = s_m[0];
+  return s_m[0];
+}
+__global__ __launch_bounds__(256) void k_bal_max_diag(DevBAL g, const double* __restrict__ Hpp,
+                                                      const double* __restrict__ Hll, double* __restrict__ part) {
+  __shared__ double s_m[256];
+  double m = 0.0;
+  const long long stride = static_cast<long long>(gridDim.x) * 256;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < 9LL * g.Cf; i += stride)
+    m = fmax(m, fabs(Hpp[81 * (i / 9) + 10 * (i % 9)]));
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < 3LL * g.Pf; i += stride) {
+    const int k = static_cast<int>(i % 3);
+    m = fmax(m, fabs(Hll[6 * (i / 3) + (k == 0 ? 0 : (k == 1 ? 3 : 5))]));
+  }
+  m = bal_block_max(m, s_m);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+__global__ __launch_bounds__(256) void k_bal_max_final(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double s_m[256];
+  const double m = bal_block_max(threadIdx.x < BAL_MAX_BLOCKS ? part[threadIdx.x] : 0.0, s_m);
+  if (threadIdx.x == 0) *out = m;
 }
 
-hipError_t launch_bal_max_diag(const DevBAL& g, const double* Hpp, const double* Hll, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_bal_max_diag, dim3(1), dim3(256), 0, st, g, Hpp, Hll, out);
+hipError_t launch_bal_max_diag(const DevBAL& g, const double* Hpp, const double* Hll, double* part, double* out,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_bal_max_diag, dim3(BAL_MAX_BLOCKS), dim3(256), 0, st, g, Hpp, Hll, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bal_max_final, dim3(1), dim3(256), 0, st, part, out);
   return hipGetLastError();
 }
 
 // ------------------------------------------------------- Schur complement
-// (Hll + lambda I)^-1 per free point, 9 values row-major (err |= 1 if
-// singular; BlockSolver::solve inverts with Eigen's closed form,
-// block_solver.hpp:371).
+// (Hll + lambda I)^-1 per free point, 9 values row-major (BlockSolver::solve
+// inverts with Eigen's closed form, block_solver.hpp:371), and L^-1 of its
+// Cholesky factor L L^T = Hll + lambda I, 6 values [i00 i10 i11 i20 i21 i22];
+// err |= 1 unless positive definite.
 __global__ void k_bal_point_dinv(int Pf, const double* __restrict__ Hll, double lambda, double* __restrict__ Dinv,
-                                 int* __restrict__ err) {
+                                 double* __restrict__ Linv, int* __restrict__ err) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= Pf) return;
-  double h[6], out[9];
+  double h[6], out[9], li[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) h[k] = Hll[6 * static_cast<size_t>(l) + k];
-  if (!inv3_sym(h, lambda, out)) {
+  const bool ok = chol3_inv(h, lambda, li) && inv3_sym(h, lambda, out);
+  if (!ok) {
     atomicOr(err, 1);
 #pragma unroll
     for (int k = 0; k < 9; ++k) out[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) li[k] = 0.0;
   }
 #pragma unroll
   for (int k = 0; k < 9; ++k) Dinv[9 * static_cast<size_t>(l) + k] = out[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) Linv[6 * static_cast<size_t>(l) + k] = li[k];
 }
 
-// Diagonal blocks and reduced right-hand side, one wavefront per free camera
-// over its observations (block_solver.hpp:345-450): S_ii = Hpp_ii + lambda I -
-// sum W Dinv W^T, bschur_i = b_i - sum W Dinv b_l (W the 9x3 Hpl block).
+// Diagonal blocks and reduced right-hand side, one workgroup per free camera
+// over its observations (block_solver.hpp:345-450), in the Gram form: with
+// G = W L^-T (W the 9x3 Hpl block, L L^T = Hll + lambda I of the point),
+// S_ii = Hpp_ii + lambda I - sum G G^T and bschur_i = b_i - sum G (L^-1 b_l)
+// (W Dinv W^T = G G^T, W Dinv b_l = G L^-1 b_l). With G_out the G blocks are
+// also stored for the off-diagonal products.
 __global__ __launch_bounds__(256) void k_bal_schur_diag(DevBAL g, const double* __restrict__ Hpp,
-                                                        const double* __restrict__ Hpl, const double* __restrict__ Dinv,
+                                                        const double* __restrict__ Hpl, const double* __restrict__ Linv,
                                                         const double* __restrict__ b, double lambda,
-                                                        double* __restrict__ F_val, double* __restrict__ bschur) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= g.Cf) return;
+                                                        double* __restrict__ F_val, double* __restrict__ bschur,
+                                                        double* __restrict__ G_out) {
+  __shared__ double s_acc[4][54];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c = blockIdx.x;
   double acc[54];
 #pragma unroll
   for (int k = 0; k < 54; ++k) acc[k] = 0.0;
   const size_t boff = 9 * static_cast<size_t>(g.Cf);
-  for (int q = g.cam_qptr[c] + lane; q < g.cam_qptr[c + 1]; q += 64) {
+  for (int q = g.cam_qptr[c] + threadIdx.x; q < g.cam_qptr[c + 1]; q += 256) {
     const int ph = g.q_ph[q];
-    double W[27], D[9], bl[3];
+    double W[27], L[6], bl[3];
 #pragma unroll
     for (int k = 0; k < 27; ++k) W[k] = Hpl[27 * static_cast<size_t>(q) + k];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) D[k] = Dinv[9 * static_cast<size_t>(ph) + k];
+    for (int k = 0; k < 6; ++k) L[k] = Linv[6 * static_cast<size_t>(ph) + k];
 #pragma unroll
     for (int k = 0; k < 3; ++k) bl[k] = b[boff + 3 * static_cast<size_t>(ph) + k];
-    double db[3];
+    const double y[3] = {L[0] * bl[0], L[1] * bl[0] + L[2] * bl[1], L[3] * bl[0] + L[4] * bl[1] + L[5] * bl[2]};
+    double G[27];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) db[k] = D[3 * k] * bl[0] + D[3 * k + 1] * bl[1] + D[3 * k + 2] * bl[2];
-    double WD[27];
+    for (int r = 0; r < 9; ++r) {
+      G[3 * r] = W[3 * r] * L[0];
+      G[3 * r + 1] = W[3 * r] * L[1] + W[3 * r + 1] * L[2];
+      G[3 * r + 2] = W[3 * r] * L[3] + W[3 * r + 1] * L[4] + W[3 * r + 2] * L[5];
+    }
+    if (G_out) {
 #pragma unroll
-    for (int r = 0; r < 9; ++r)
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        WD[3 * r + k] = W[3 * r] * D[k] + W[3 * r + 1] * D[3 + k] + W[3 * r + 2] * D[6 + k];
+      for (int k = 0; k < 27; ++k) G_out[27 * static_cast<size_t>(q) + k] = G[k];
+    }
 #pragma unroll
     for (int k = 0; k < 45; ++k) {
       const int r = c_up9_r[k], cc = c_up9_c[k];
-      acc[k] += WD[3 * r] * W[3 * cc] + WD[3 * r + 1] * W[3 * cc + 1] + WD[3 * r + 2] * W[3 * cc + 2];
+      acc[k] += G[3 * r] * G[3 * cc] + G[3 * r + 1] * G[3 * cc + 1] + G[3 * r + 2] * G[3 * cc + 2];
     }
 #pragma unroll
-    for (int r = 0; r < 9; ++r) acc[45 + r] += W[3 * r] * db[0] + W[3 * r + 1] * db[1] + W[3 * r + 2] * db[2];
+    for (int r = 0; r < 9; ++r) acc[45 + r] += G[3 * r] * y[0] + G[3 * r + 1] * y[1] + G[3 * r + 2] * y[2];
   }
 #pragma unroll
   for (int k = 0; k < 54; ++k) acc[k] = bal_wave_sum(acc[k]);
   if (lane == 0) {
-    const double* H = Hpp + 81 * static_cast<size_t>(c);
-    double* S = F_val + 81 * static_cast<size_t>(g.F_diag[c]);
 #pragma unroll
-    for (int k = 0; k < 45; ++k) {
-      const int r = c_up9_r[k], cc = c_up9_c[k];
-      const double v = H[9 * r + cc] + (r == cc ? lambda : 0.0) - acc[k];
-      S[9 * r + cc] = v;
-      S[9 * cc + r] = v;
+    for (int k = 0; k < 54; ++k) s_acc[wv][k] = acc[k];
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < 54) {
+    const double v = (s_acc[0][t] + s_acc[1][t]) + (s_acc[2][t] + s_acc[3][t]);
+    if (t < 45) {
+      const int r = c_up9_r[t], cc = c_up9_c[t];
+      const double h = Hpp[81 * static_cast<size_t>(c) + 9 * r + cc] + (r == cc ? lambda : 0.0) - v;
+      double* S = F_val + 81 * static_cast<size_t>(g.F_diag[c]);
+      S[9 * r + cc] = h;
+      S[9 * cc + r] = h;
+    } else {
+      const int r = t - 45;
+      bschur[9 * static_cast<size_t>(c) + r] = b[9 * static_cast<size_t>(c) + r] - v;
     }
-#pragma unroll
-    for (int r = 0; r < 9; ++r) bschur[9 * static_cast<size_t>(c) + r] = b[9 * static_cast<size_t>(c) + r] - acc[45 + r];
   }
 }
+
+// Workgroups go to the 8 XCDs round robin; the blocks are taken so that each
+// XCD works through a contiguous range of them (rows in order), whose rows
+// share cameras and points and so hit in that XCD's L2. The grid is 8 x
+// per workgroups, per = the workgroups of four blocks an XCD takes.
+__device__ __forceinline__ int bal_xcd_block(int nO) {
+  const int per = (nO + 31) / 32;
+  return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+inline int bal_xcd_grid(int nO) { return 8 * ((nO + 31) / 32); }
 
 // Off-diagonal blocks, one wavefront per off-diagonal upper block (i < j):
 // S_ij = - sum over the observation pairs (a by camera i, b by camera j, of
@@ -436,7 +487,7 @@ __global__ __launch_bounds__(256) void k_bal_schur_offdiag(DevBAL g, const doubl
                                                            const double* __restrict__ Dinv,
                                                            double* __restrict__ F_val) {
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = bal_xcd_block(g.nO) * 4 + (threadIdx.x >> 6);
   if (k >= g.nO) return;
   const int e0 = lane, e1 = 64 + (lane < 17 ? lane : 0);
   const int r0 = e0 / 9, c0 = e0 % 9, r1 = e1 / 9, c1 = e1 % 9;
@@ -473,18 +524,78 @@ __global__ __launch_bounds__(256) void k_bal_schur_offdiag(DevBAL g, const doubl
   }
 }
 
+// The same blocks on the matrix cores. With G = W L^-T (9x3 per observation,
+// L^-1 of its point's factor; k_bal_schur_diag stores it), S_ij = - sum
+// over the pairs of G_a G_b^T is one GEMM of K = 3 x pairs per block,
+// accumulated by v_mfma_f64_16x16x4f64 (FP64 in and out). One wavefront per
+// off-diagonal upper block, one pair per instruction: lane t holds
+// A[t % 16][t / 16] = G_a[row][k] and B[t / 16][t % 16] = G_b[col][k] for
+// row / col t % 16 < 9 and k = t / 16 < 3, zero elsewhere (9 x 9 x 3 of the
+// 16 x 16 x 4 product is used). The accumulator holds D[t / 16 + 4 r][t % 16]
+// in its register r. The pair list is read 64 pairs at a time, one per lane,
+// and each pair's observation indices reach the wavefront by readlane.
+typedef double bal_d4 __attribute__((ext_vector_type(4)));
+constexpr int BAL_MFMA_UNROLL = 8;
+__global__ __launch_bounds__(256) void k_bal_schur_offdiag_mfma(DevBAL g, const double* __restrict__ G,
+                                                                double* __restrict__ F_val) {
+  const int lane = threadIdx.x & 63;
+  const int k = bal_xcd_block(g.nO) * 4 + (threadIdx.x >> 6);
+  if (k >= g.nO) return;  // uniform per wavefront
+  const int ri = lane & 15, kc = lane >> 4;
+  const bool on = ri < 9 && kc < 3;
+  const int off = on ? 3 * ri + kc : 0;  // the lane's entry of a 9x3 block
+  bal_d4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int p0 = g.od_ptr[k], p1 = g.od_ptr[k + 1];
+  for (int c0 = p0; c0 < p1; c0 += 64) {
+    const int n = min(64, p1 - c0);
+    const int2 mine = g.od_pair[c0 + (lane < n ? lane : 0)];
+    for (int u0 = 0; u0 < n; u0 += BAL_MFMA_UNROLL) {
+      double ga[BAL_MFMA_UNROLL], gb[BAL_MFMA_UNROLL];
+#pragma unroll
+      for (int u = 0; u < BAL_MFMA_UNROLL; ++u) {
+        const int src = min(u0 + u, n - 1);
+        const int qa = __builtin_amdgcn_readlane(mine.x, src);
+        const int qb = __builtin_amdgcn_readlane(mine.y, src);
+        ga[u] = G[27 * static_cast<size_t>(qa) + off];
+        gb[u] = G[27 * static_cast<size_t>(qb) + off];
+      }
+#pragma unroll
+      for (int u = 0; u < BAL_MFMA_UNROLL; ++u) {
+        if (u0 + u >= n) break;  // uniform per wavefront
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? ga[u] : 0.0, on ? gb[u] : 0.0, acc, 0, 0, 0);
+      }
+    }
+  }
+  const int blk = g.od_blk[k];
+  double* S = F_val + 81 * static_cast<size_t>(g.up_pos[blk]);
+  double* ST = F_val + 81 * static_cast<size_t>(g.up_posT[blk]);
+  const int col = ri;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = kc + 4 * r;
+    if (row < 9 && col < 9) {
+      S[9 * row + col] = -acc[r];
+      ST[9 * col + row] = -acc[r];
+    }
+  }
+}
+
 hipError_t launch_bal_schur(const DevBAL& g, const double* Hpp, const double* Hll, const double* Hpl,
-                            const double* b, double lambda, double* Dinv, double* F_val, double* bschur, int* err,
-                            hipStream_t st) {
+                            const double* b, double lambda, double* Dinv, double* Linv, double* G, double* F_val,
+                            double* bschur, int* err, bool mfma, hipStream_t st) {
   hipLaunchKernelGGL(k_bal_point_dinv, dim3(bal_blocks(g.Pf > 0 ? g.Pf : 1, 256)), dim3(256), 0, st, g.Pf, Hll,
-                     lambda, Dinv, err);
+                     lambda, Dinv, Linv, err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bal_schur_diag, dim3(bal_blocks(g.Cf, 4)), dim3(256), 0, st, g, Hpp, Hpl, Dinv, b, lambda,
-                     F_val, bschur);
+  hipLaunchKernelGGL(k_bal_schur_diag, dim3(g.Cf), dim3(256), 0, st, g, Hpp, Hpl, Linv, b, lambda, F_val, bschur,
+                     mfma ? G : nullptr);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (g.nO > 0) {
-    hipLaunchKernelGGL(k_bal_schur_offdiag, dim3(bal_blocks(g.nO, 4)), dim3(256), 0, st, g, Hpl, Dinv, F_val);
+    if (mfma) {
+      hipLaunchKernelGGL(k_bal_schur_offdiag_mfma, dim3(bal_xcd_grid(g.nO)), dim3(256), 0, st, g, G, F_val);
+    } else {
+      hipLaunchKernelGGL(k_bal_schur_offdiag, dim3(bal_xcd_grid(g.nO)), dim3(256), 0, st, g, Hpl, Dinv, F_val);
+    }
     e = hipGetLastError();
   }
   return e;

@@ -74,6 +74,7 @@ int g2ohip_create(int device, g2ohip_ctx** out) try {
       if (std::strcmp(sp, "windows") == 0) c->schur_path = 1;
       else if (std::strcmp(sp, "rows") == 0) c->schur_path = 2;
     }
+    if (const char* bs = std::getenv("G2OHIP_BAL_SCHUR")) c->bal_mfma = std::strcmp(bs, "valu") != 0;
     const char* mapped_env = std::getenv("G2OHIP_HOST_MAPPED");
     const bool mapped = !(mapped_env && mapped_env[0] == '0');
     void* hp = nullptr;

@@ -163,8 +163,11 @@ struct g2ohip_ctx {
   g2ohip::DBuf<int> d_bal_cs_ptr, d_bal_cs_slot, d_bal_od_ptr, d_bal_od_blk;
   g2ohip::DBuf<int2> d_bal_od_pair;
   g2ohip::DBuf<int4> d_bal_rec;
-  g2ohip::DBuf<double> d_bal_tab, d_bal_Hpp, d_bal_Hpl, d_bal_Dinv, d_bal_F_val, d_bal_part;
+  g2ohip::DBuf<double> d_bal_tab, d_bal_Hpp, d_bal_Hpl, d_bal_Dinv, d_bal_F_val, d_bal_part, d_bal_G;
   long long bal_pairs = 0;
+  // off-diagonal BAL Schur blocks on the matrix cores (default) or the vector
+  // units (G2OHIP_BAL_SCHUR=valu in the environment at context creation)
+  bool bal_mfma = true;
 
   // kernel timing (HIP events on `stream`)
   bool profiling = false;

@@ -291,12 +291,16 @@ hipError_t launch_bal_chi2(const DevBAL& g, const double* tab, const double* pts
 // *chi_out (part: bal_point_blocks(npos) partials)
 hipError_t launch_bal_linearize(const DevBAL& g, const double* tab, const double* pts, double* Hpp, double* Hll,
                                 double* Hpl, double* b, double* part, double* chi_out, hipStream_t st);
-hipError_t launch_bal_max_diag(const DevBAL& g, const double* Hpp, const double* Hll, double* out, hipStream_t st);
-// Dinv [Pf][9], the reduced system's blocks into F_val and bschur [9 Cf];
-// err |= 1 if a point block is singular
+// part: 256 partial maxima
+hipError_t launch_bal_max_diag(const DevBAL& g, const double* Hpp, const double* Hll, double* part, double* out,
+                               hipStream_t st);
+// Dinv [Pf][9] and Linv [Pf][6], the reduced system's blocks into F_val and
+// bschur [9 Cf]; err |= 1 if a point block is not positive definite. mfma:
+// off-diagonal blocks on the matrix cores from G = Hpl L^-T ([nq][27], formed
+// by the diagonal pass), else on the vector units (G unused).
 hipError_t launch_bal_schur(const DevBAL& g, const double* Hpp, const double* Hll, const double* Hpl,
-                            const double* b, double lambda, double* Dinv, double* F_val, double* bschur, int* err,
-                            hipStream_t st);
+                            const double* b, double lambda, double* Dinv, double* Linv, double* G, double* F_val,
+                            double* bschur, int* err, bool mfma, hipStream_t st);
 hipError_t launch_bal_backsub(const DevBAL& g, const double* Hpl, const double* Dinv, const double* b, double* x,
                               hipStream_t st);
 hipError_t launch_bal_update(const DevBAL& g, const double* x, double* cams, double* pts, hipStream_t st);

@@ -1453,6 +1453,7 @@ struct BAL {
   int Cf = 0, Pf = 0;
   vector<int> active;
   vector<int> colPtr, colCam, edgeBlk;  // Hpl blocks per point column (distinct cameras)
+  vector<int> ptPtr, ptEdge;            // active edges grouped by point (P + 1 CSR), edge order
   vector<int> sPtr, sCol;               // Hschur pattern
   vector<double> Hpp, Hll, Hpl, b, err;
   vector<vector<double>> stackC, stackP;
@@ -1525,6 +1526,12 @@ void bal_structure(BAL& p) {
     }
   p.S.val.assign(p.sCol.size() * 81, 0.0);
   p.err.assign(2 * static_cast<size_t>(p.E), 0.0);
+  p.ptPtr.assign(p.P + 1, 0);
+  for (int k : p.active) p.ptPtr[p.ept[k] + 1]++;
+  for (int i = 0; i < p.P; ++i) p.ptPtr[i + 1] += p.ptPtr[i];
+  p.ptEdge.resize(p.active.size());
+  vector<int> fill(p.ptPtr.begin(), p.ptPtr.end() - 1);
+  for (int k : p.active) p.ptEdge[fill[p.ept[k]]++] = k;
 }
 
 // computeActiveErrors + activeRobustChi2 (sparse_optimizer.cpp:78-130);
@@ -1561,36 +1568,64 @@ void bal_build_system(BAL& p) {
   p.Hpl.assign(p.colCam.size() * 27, 0.0);
   p.b.assign(9 * static_cast<size_t>(p.Cf) + 3 * static_cast<size_t>(p.Pf), 0.0);
   const size_t sp = 9 * static_cast<size_t>(p.Cf);
-  // serial accumulation over the active edges (a fixed order)
-  for (int k : p.active) {
-    const int ci = p.camH[p.ecam[k]], li = p.ptH[p.ept[k]];
-    double e[2], Jc[18], Jp[6];
-    bal_edge(&p.cams[9 * static_cast<size_t>(p.ecam[k])], &p.pts[3 * static_cast<size_t>(p.ept[k])], &p.meas[2 * k], e,
-             Jc, Jp);
-    double om = p.w[k];
-    if (p.huber > 0) {
-      double rho[3];
-      huber(p.w[k] * (e[0] * e[0] + e[1] * e[1]), p.huber, rho);
-      om *= rho[1];
-    }
-    if (li >= 0) {
-      for (int r = 0; r < 3; ++r) {
-        p.b[sp + 3 * li + r] -= om * (Jp[r] * e[0] + Jp[3 + r] * e[1]);
-        for (int c = 0; c < 3; ++c) p.Hll[9 * li + 3 * r + c] += om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]);
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  // points are split over the threads (a point's Hll, b_l and Hpl blocks are
+  // its own); the camera sums go through per-thread accumulators added in
+  // thread order, as the per-vertex locks of the reference's OpenMP build
+  // serialise them (base_edge.h:42-58)
+  vector<vector<double>> hppT(nth);
+#pragma omp parallel
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    auto& hpp = hppT[tid];
+    hpp.assign(static_cast<size_t>(p.Cf) * 90, 0.0);  // 81 Hpp + 9 b_p per camera
+#pragma omp for schedule(static)
+    for (int i = 0; i < p.P; ++i) {
+      for (int a = p.ptPtr[i]; a < p.ptPtr[i + 1]; ++a) {
+        const int k = p.ptEdge[a];
+        const int ci = p.camH[p.ecam[k]], li = p.ptH[p.ept[k]];
+        double e[2], Jc[18], Jp[6];
+        bal_edge(&p.cams[9 * static_cast<size_t>(p.ecam[k])], &p.pts[3 * static_cast<size_t>(p.ept[k])],
+                 &p.meas[2 * k], e, Jc, Jp);
+        double om = p.w[k];
+        if (p.huber > 0) {
+          double rho[3];
+          huber(p.w[k] * (e[0] * e[0] + e[1] * e[1]), p.huber, rho);
+          om *= rho[1];
+        }
+        if (li >= 0) {
+          for (int r = 0; r < 3; ++r) {
+            p.b[sp + 3 * li + r] -= om * (Jp[r] * e[0] + Jp[3 + r] * e[1]);
+            for (int c = 0; c < 3; ++c) p.Hll[9 * li + 3 * r + c] += om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]);
+          }
+        }
+        if (ci >= 0) {
+          double* h = &hpp[90 * static_cast<size_t>(ci)];
+          for (int r = 0; r < 9; ++r) {
+            h[81 + r] -= om * (Jc[r] * e[0] + Jc[9 + r] * e[1]);
+            for (int c = 0; c < 9; ++c) h[9 * r + c] += om * (Jc[r] * Jc[c] + Jc[9 + r] * Jc[9 + c]);
+          }
+        }
+        if (ci >= 0 && li >= 0) {
+          double* h = &p.Hpl[static_cast<size_t>(p.edgeBlk[k]) * 27];
+          for (int r = 0; r < 9; ++r)
+            for (int c = 0; c < 3; ++c) h[3 * r + c] += om * (Jc[r] * Jp[c] + Jc[9 + r] * Jp[3 + c]);
+        }
       }
-    }
-    if (ci >= 0) {
-      for (int r = 0; r < 9; ++r) {
-        p.b[9 * ci + r] -= om * (Jc[r] * e[0] + Jc[9 + r] * e[1]);
-        for (int c = 0; c < 9; ++c) p.Hpp[81 * static_cast<size_t>(ci) + 9 * r + c] += om * (Jc[r] * Jc[c] + Jc[9 + r] * Jc[9 + c]);
-      }
-    }
-    if (ci >= 0 && li >= 0) {
-      double* h = &p.Hpl[static_cast<size_t>(p.edgeBlk[k]) * 27];
-      for (int r = 0; r < 9; ++r)
-        for (int c = 0; c < 3; ++c) h[3 * r + c] += om * (Jc[r] * Jp[c] + Jc[9 + r] * Jp[3 + c]);
     }
   }
+  for (int t = 0; t < nth; ++t)
+    for (int c = 0; c < p.Cf; ++c) {
+      const double* h = &hppT[t][90 * static_cast<size_t>(c)];
+      for (int i = 0; i < 81; ++i) p.Hpp[81 * static_cast<size_t>(c) + i] += h[i];
+      for (int r = 0; r < 9; ++r) p.b[9 * static_cast<size_t>(c) + r] += h[81 + r];
+    }
 }
 
 // BlockSolver<9, 3>::solve with Schur (block_solver.hpp:331-477), lambda on
@@ -1606,40 +1641,66 @@ bool bal_solve(BAL& p) {
     for (int i = 0; i < 9; ++i) d[10 * i] += p.lambda;
   }
   p.Dinv.assign(static_cast<size_t>(Pf) * 9, 0.0);
-  vector<double> coeff(sp, 0.0);
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  // points split over the threads, each with its own Hschur and coefficient
+  // accumulators, added in thread order
+  vector<vector<double>> Sth(nth), cth(nth);
   bool okD = true;
-  for (int l = 0; l < Pf; ++l) {
-    double D[9];
-    for (int i = 0; i < 9; ++i) D[i] = p.Hll[9 * static_cast<size_t>(l) + i];
-    for (int i = 0; i < 3; ++i) D[4 * i] += p.lambda;
-    double* Di = &p.Dinv[9 * static_cast<size_t>(l)];
-    okD = inv3(D, Di) && okD;
-    const double* bl = &p.b[sp + 3 * static_cast<size_t>(l)];
-    double db[3];
-    for (int r = 0; r < 3; ++r) db[r] = Di[3 * r] * bl[0] + Di[3 * r + 1] * bl[1] + Di[3 * r + 2] * bl[2];
-    for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a) {
-      const int i1 = p.colCam[a];
-      const double* Bi = &p.Hpl[static_cast<size_t>(a) * 27];
-      double BD[27];
-      for (int r = 0; r < 9; ++r)
-        for (int c = 0; c < 3; ++c)
-          BD[3 * r + c] = Bi[3 * r] * Di[c] + Bi[3 * r + 1] * Di[3 + c] + Bi[3 * r + 2] * Di[6 + c];
-      for (int r = 0; r < 9; ++r) coeff[9 * i1 + r] += Bi[3 * r] * db[0] + Bi[3 * r + 1] * db[1] + Bi[3 * r + 2] * db[2];
-      int sk = p.sPtr[i1];
-      for (int bb = a; bb < p.colPtr[l + 1]; ++bb) {
-        const int i2 = p.colCam[bb];
-        while (p.sCol[sk] < i2) ++sk;
-        const double* Bj = &p.Hpl[static_cast<size_t>(bb) * 27];
-        double* h = &p.S.val[static_cast<size_t>(sk) * 81];
+#pragma omp parallel reduction(&& : okD)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    auto& Sl = Sth[tid];
+    auto& coeff = cth[tid];
+    Sl.assign(p.S.val.size(), 0.0);
+    coeff.assign(sp, 0.0);
+#pragma omp for schedule(static)
+    for (int l = 0; l < Pf; ++l) {
+      double D[9];
+      for (int i = 0; i < 9; ++i) D[i] = p.Hll[9 * static_cast<size_t>(l) + i];
+      for (int i = 0; i < 3; ++i) D[4 * i] += p.lambda;
+      double* Di = &p.Dinv[9 * static_cast<size_t>(l)];
+      okD = inv3(D, Di) && okD;
+      const double* bl = &p.b[sp + 3 * static_cast<size_t>(l)];
+      double db[3];
+      for (int r = 0; r < 3; ++r) db[r] = Di[3 * r] * bl[0] + Di[3 * r + 1] * bl[1] + Di[3 * r + 2] * bl[2];
+      for (int a = p.colPtr[l]; a < p.colPtr[l + 1]; ++a) {
+        const int i1 = p.colCam[a];
+        const double* Bi = &p.Hpl[static_cast<size_t>(a) * 27];
+        double BD[27];
         for (int r = 0; r < 9; ++r)
-          for (int c = 0; c < 9; ++c)
-            h[9 * r + c] -= BD[3 * r] * Bj[3 * c] + BD[3 * r + 1] * Bj[3 * c + 1] + BD[3 * r + 2] * Bj[3 * c + 2];
+          for (int c = 0; c < 3; ++c)
+            BD[3 * r + c] = Bi[3 * r] * Di[c] + Bi[3 * r + 1] * Di[3 + c] + Bi[3 * r + 2] * Di[6 + c];
+        for (int r = 0; r < 9; ++r)
+          coeff[9 * i1 + r] += Bi[3 * r] * db[0] + Bi[3 * r + 1] * db[1] + Bi[3 * r + 2] * db[2];
+        int sk = p.sPtr[i1];
+        for (int bb = a; bb < p.colPtr[l + 1]; ++bb) {
+          const int i2 = p.colCam[bb];
+          while (p.sCol[sk] < i2) ++sk;
+          const double* Bj = &p.Hpl[static_cast<size_t>(bb) * 27];
+          double* h = &Sl[static_cast<size_t>(sk) * 81];
+          for (int r = 0; r < 9; ++r)
+            for (int c = 0; c < 9; ++c)
+              h[9 * r + c] -= BD[3 * r] * Bj[3 * c] + BD[3 * r + 1] * Bj[3 * c + 1] + BD[3 * r + 2] * Bj[3 * c + 2];
+        }
       }
     }
   }
   if (!okD) return false;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < static_cast<long>(p.S.val.size()); ++i)
+    for (int t = 0; t < nth; ++t) p.S.val[i] += Sth[t][i];
   p.bschur.assign(sp, 0.0);
-  for (size_t i = 0; i < sp; ++i) p.bschur[i] = p.b[i] - coeff[i];
+  for (size_t i = 0; i < sp; ++i) {
+    double c = 0;
+    for (int t = 0; t < nth; ++t) c += cth[t][i];
+    p.bschur[i] = p.b[i] - c;
+  }
   const double t1 = now_s();
   p.times[2] += t1 - t0;
   p.x.assign(sp + 3 * static_cast<size_t>(Pf), 0.0);
@@ -1648,6 +1709,7 @@ bool bal_solve(BAL& p) {
   p.times[3] += now_s() - t1;
   if (!ok) return false;
   // back-substitution (block_solver.hpp:452-476)
+#pragma omp parallel for schedule(static)
   for (int l = 0; l < Pf; ++l) {
     double cl[3];
     for (int r = 0; r < 3; ++r) cl[r] = p.b[sp + 3 * static_cast<size_t>(l) + r];
