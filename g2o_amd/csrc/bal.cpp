@@ -264,11 +264,16 @@ int g2ohip_ctx::bal_solve(int* ok) {
   HK(hipMemsetAsync(d_err.p, 0, sizeof(int) * 4, stream));
   HK(B.Jinv.alloc(81 * static_cast<size_t>(nb)));
   HK(B.x.alloc(n)); HK(B.r.alloc(n)); HK(B.d.alloc(n)); HK(B.d2.alloc(n)); HK(B.q.alloc(n)); HK(B.s.alloc(n));
+  if (bal_mfma) HK(d_bal_G.alloc(27 * static_cast<size_t>(std::max(st.nq, 1))));
   {
     ProfScope ps(this, G2OHIP_K_SCHUR);
-    if (bal_mfma) HK(d_bal_G.alloc(27 * static_cast<size_t>(std::max(st.nq, 1))));
-    HK(launch_bal_schur(bal, d_bal_Hpp.p, d_Hll.p, d_bal_Hpl.p, d_b.p, lambda, d_bal_Dinv.p, d_linv.p, d_bal_G.p,
-                        d_bal_F_val.p, d_bschur.p, d_err.p, bal_mfma, stream));
+    HK(launch_bal_schur_diag(bal, d_bal_Hpp.p, d_Hll.p, d_bal_Hpl.p, d_b.p, lambda, d_bal_Dinv.p, d_linv.p,
+                             bal_mfma ? d_bal_G.p : nullptr, d_bal_F_val.p, d_bschur.p, d_err.p, stream));
+    {
+      // the off-diagonal products alone, also under their own timer
+      ProfScope po(this, G2OHIP_K_SCHUR_WINDOWS);
+      HK(launch_bal_schur_offdiag(bal, d_bal_Hpl.p, d_bal_Dinv.p, d_bal_G.p, d_bal_F_val.p, bal_mfma, stream));
+    }
     HK((launch_jacobi<double, 9>(nb, d_bal_F_val.p, d_F_diag.p, B.Jinv.p, d_err.p, stream)));
   }
   int err = 0;

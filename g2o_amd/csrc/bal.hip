@@ -580,25 +580,25 @@ __global__ __launch_bounds__(256) void k_bal_schur_offdiag_mfma(DevBAL g, const 
   }
 }
 
-hipError_t launch_bal_schur(const DevBAL& g, const double* Hpp, const double* Hll, const double* Hpl,
-                            const double* b, double lambda, double* Dinv, double* Linv, double* G, double* F_val,
-                            double* bschur, int* err, bool mfma, hipStream_t st) {
+hipError_t launch_bal_schur_diag(const DevBAL& g, const double* Hpp, const double* Hll, const double* Hpl,
+                                 const double* b, double lambda, double* Dinv, double* Linv, double* G,
+                                 double* F_val, double* bschur, int* err, hipStream_t st) {
   hipLaunchKernelGGL(k_bal_point_dinv, dim3(bal_blocks(g.Pf > 0 ? g.Pf : 1, 256)), dim3(256), 0, st, g.Pf, Hll,
                      lambda, Dinv, Linv, err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bal_schur_diag, dim3(g.Cf), dim3(256), 0, st, g, Hpp, Hpl, Linv, b, lambda, F_val, bschur,
-                     mfma ? G : nullptr);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  if (g.nO > 0) {
-    if (mfma) {
-      hipLaunchKernelGGL(k_bal_schur_offdiag_mfma, dim3(bal_xcd_grid(g.nO)), dim3(256), 0, st, g, G, F_val);
-    } else {
-      hipLaunchKernelGGL(k_bal_schur_offdiag, dim3(bal_xcd_grid(g.nO)), dim3(256), 0, st, g, Hpl, Dinv, F_val);
-    }
-    e = hipGetLastError();
-  }
-  return e;
+  hipLaunchKernelGGL(k_bal_schur_diag, dim3(g.Cf), dim3(256), 0, st, g, Hpp, Hpl, Linv, b, lambda, F_val, bschur, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_bal_schur_offdiag(const DevBAL& g, const double* Hpl, const double* Dinv, const double* G,
+                                    double* F_val, bool mfma, hipStream_t st) {
+  if (g.nO == 0) return hipSuccess;
+  if (mfma)
+    hipLaunchKernelGGL(k_bal_schur_offdiag_mfma, dim3(bal_xcd_grid(g.nO)), dim3(256), 0, st, g, G, F_val);
+  else
+    hipLaunchKernelGGL(k_bal_schur_offdiag, dim3(bal_xcd_grid(g.nO)), dim3(256), 0, st, g, Hpl, Dinv, F_val);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------ back-substitution

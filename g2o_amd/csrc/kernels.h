@@ -294,13 +294,16 @@ hipError_t launch_bal_linearize(const DevBAL& g, const double* tab, const double
 // part: 256 partial maxima
 hipError_t launch_bal_max_diag(const DevBAL& g, const double* Hpp, const double* Hll, double* part, double* out,
                                hipStream_t st);
-// Dinv [Pf][9] and Linv [Pf][6], the reduced system's blocks into F_val and
-// bschur [9 Cf]; err |= 1 if a point block is not positive definite. mfma:
-// off-diagonal blocks on the matrix cores from G = Hpl L^-T ([nq][27], formed
-// by the diagonal pass), else on the vector units (G unused).
-hipError_t launch_bal_schur(const DevBAL& g, const double* Hpp, const double* Hll, const double* Hpl,
-                            const double* b, double lambda, double* Dinv, double* Linv, double* G, double* F_val,
-                            double* bschur, int* err, bool mfma, hipStream_t st);
+// Dinv [Pf][9] and Linv [Pf][6] of the damped point blocks (err |= 1 unless
+// positive definite), the diagonal blocks of the reduced system into F_val,
+// bschur [9 Cf], and (G not null) G = Hpl L^-T [nq][27].
+hipError_t launch_bal_schur_diag(const DevBAL& g, const double* Hpp, const double* Hll, const double* Hpl,
+                                 const double* b, double lambda, double* Dinv, double* Linv, double* G,
+                                 double* F_val, double* bschur, int* err, hipStream_t st);
+// The off-diagonal blocks into F_val (both triangles): on the matrix cores
+// from G (mfma), else on the vector units from Hpl and Dinv.
+hipError_t launch_bal_schur_offdiag(const DevBAL& g, const double* Hpl, const double* Dinv, const double* G,
+                                    double* F_val, bool mfma, hipStream_t st);
 hipError_t launch_bal_backsub(const DevBAL& g, const double* Hpl, const double* Dinv, const double* b, double* x,
                               hipStream_t st);
 hipError_t launch_bal_update(const DevBAL& g, const double* x, double* cams, double* pts, hipStream_t st);
