@@ -26,7 +26,8 @@ EXPORTS = [
     "g2ohip_build_system", "g2ohip_set_lambda", "g2ohip_restore_diagonal", "g2ohip_solve", "g2ohip_get_x",
     "g2ohip_get_b", "g2ohip_multiply_hessian", "g2ohip_update", "g2ohip_push", "g2ohip_pop", "g2ohip_discard_top", "g2ohip_lambda_init",
     "g2ohip_compute_scale", "g2ohip_optimize", "g2ohip_set_algorithm", "g2ohip_set_force_stop_flag", "g2ohip_compute_marginals", "g2ohip_ba_get_hessian",
-    "g2ohip_ba_get_schur", "g2ohip_last_solve_info", "g2ohip_set_profiling", "g2ohip_get_profile",
+    "g2ohip_ba_get_schur", "g2ohip_last_solve_info", "g2ohip_lm_trials", "g2ohip_set_profiling",
+    "g2ohip_get_profile",
     "g2ohip_linear_solve_pcg", "g2ohip_pg_set_structure", "g2ohip_pg_structure_info", "g2ohip_pg_set_measurements",
     "g2ohip_pg_set_state", "g2ohip_pg_get_state", "g2ohip_pg_get_system", "g2ohip_bal_set_structure",
     "g2ohip_bal_structure_info", "g2ohip_bal_set_measurements", "g2ohip_bal_set_state", "g2ohip_bal_get_state",
@@ -107,6 +108,7 @@ def lib():
         "g2ohip_ba_get_hessian": (ct.c_int, [_vp, _dp, _dp, _dp]),
         "g2ohip_ba_get_schur": (ct.c_int, [_vp, _ip, _ip, _dp, _dp]),
         "g2ohip_last_solve_info": (ct.c_int, [_vp, _dp]),
+        "g2ohip_lm_trials": (ct.c_int, [_vp, _dp, ct.c_int, _ip]),
         "g2ohip_set_profiling": (ct.c_int, [_vp, ct.c_int]),
         "g2ohip_get_profile": (ct.c_int, [_vp, ct.c_int, _dp, _ip]),
         "g2ohip_linear_solve_pcg": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _dp, _dp, _dp,

@@ -1,5 +1,6 @@
 // extern "C" entry points of include/g2o_hip.h. Each wraps one context
 // operation; errors become status codes plus the context's last-error text.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -346,6 +347,19 @@ int g2ohip_last_solve_info(g2ohip_ctx* ctx, double* info) {
   return guarded(ctx, [&]() -> int {
     if (!info) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
     for (int i = 0; i < 4; ++i) info[i] = ctx->last_info[i];
+    return G2OHIP_OK;
+  });
+}
+
+int g2ohip_lm_trials(g2ohip_ctx* ctx, double* out, int max, int* count) {
+  return guarded(ctx, [&]() -> int {
+    if (!count) return ctx->fail(G2OHIP_ERR_INVALID, "null output");
+    const int n = static_cast<int>(ctx->trials.size() / G2OHIP_TRIAL_FIELDS);
+    *count = n;
+    if (out && max > 0) {
+      const size_t m = static_cast<size_t>(std::min(n, max)) * G2OHIP_TRIAL_FIELDS;
+      std::copy(ctx->trials.begin(), ctx->trials.begin() + m, out);
+    }
     return G2OHIP_OK;
   });
 }

@@ -825,6 +825,7 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
   *done = 0;
   pcg_residual_mem = -1.0;  // LinearSolverPCG::init() via algorithm->init()
   fp32_off = false;
+  trials.clear();
   double ni = 2.0;
   for (int i = 0; i < iterations; ++i) {
     if (force_stop && *force_stop) break;
@@ -890,6 +891,12 @@ int g2ohip_ctx::optimize(int iterations, g2ohip_iter_stats* stats, int* done) {
         have_solution = true;
       }
       rho = (currentChi - tempChi) / scale;
+      {
+        const bool acc = rho > 0 && std::isfinite(tempChi) && ok;
+        const double rec[G2OHIP_TRIAL_FIELDS] = {lambda, currentChi, tempChi, scale, rho, acc ? 1.0 : 0.0,
+                                                 last_info[0], last_info[1], last_info[3], last_info[2]};
+        trials.insert(trials.end(), rec, rec + G2OHIP_TRIAL_FIELDS);
+      }
       if (rho > 0 && std::isfinite(tempChi) && ok) {
         double alpha = 1. - std::pow((2 * rho - 1), 3);
         alpha = std::min(alpha, 2. / 3.);

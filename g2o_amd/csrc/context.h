@@ -94,6 +94,8 @@ struct g2ohip_ctx {
   int algorithm = 0;  // g2ohip_algorithm
   double pcg_residual_mem = -1.0;  // LinearSolverPCG::_residual
   double last_info[4] = {0, 0, -1, 0};
+  // one record per LM trial of the last optimize (g2ohip_lm_trials)
+  std::vector<double> trials;
   // FP32 blocks with FP64 refinement: once the FP32 solution's FP64 residual
   // misses the first solve's threshold by more than 10x, or a correction
   // stops above it (the FP32-rounded reduced system no longer resolves its

@@ -77,6 +77,20 @@ class Context:
         return o
 
 
+TRIAL_FIELDS = ["lambda", "chi2_current", "chi2_trial", "scale", "rho", "accepted", "pcg_iterations",
+                "refine_steps", "fp64_fallback", "pcg_residual_mem"]
+
+
+def _lm_trials(ctx):
+    """The LM trials of the last optimize(): float64[n, 10] with the columns
+    TRIAL_FIELDS."""
+    n = ct.c_int()
+    ctx.call("g2ohip_lm_trials", None, 0, ct.byref(n))
+    out = np.zeros((max(n.value, 1), len(TRIAL_FIELDS)))
+    ctx.call("g2ohip_lm_trials", _dp(out), n.value, ct.byref(n))
+    return out[:n.value]
+
+
 def _marginals(ctx, blocks, tol):
     """SparseOptimizer::computeMarginals for the (row, col) pairs in blocks
     (free-vertex indices); returns float64[n, 6, 6]."""
@@ -299,6 +313,10 @@ class BundleAdjuster:
         self.ctx.call("g2ohip_last_solve_info", _dp(v))
         return v
 
+    def lm_trials(self) -> np.ndarray:
+        """Per LM trial of the last optimize(): columns TRIAL_FIELDS."""
+        return _lm_trials(self.ctx)
+
 
 class PoseGraphOptimizer:
     """SparseOptimizer with OptimizationAlgorithmLevenberg over VertexSE3 /
@@ -407,6 +425,10 @@ class PoseGraphOptimizer:
         done = ct.c_int()
         self.ctx.call("g2ohip_optimize", iterations, stats, ct.byref(done))
         return [{f: getattr(s, f) for f, _ in L.IterStats._fields_} for s in stats[:done.value]]
+
+    def lm_trials(self) -> np.ndarray:
+        """Per LM trial of the last optimize(): columns TRIAL_FIELDS."""
+        return _lm_trials(self.ctx)
 
     def set_profiling(self, on: bool = True):
         self.ctx.call("g2ohip_set_profiling", int(on))
@@ -612,6 +634,10 @@ class BALAdjuster:
         v = np.zeros(4)
         self.ctx.call("g2ohip_last_solve_info", _dp(v))
         return v
+
+    def lm_trials(self) -> np.ndarray:
+        """Per LM trial of the last optimize(): columns TRIAL_FIELDS."""
+        return _lm_trials(self.ctx)
 
     def set_profiling(self, on: bool = True):
         self.ctx.call("g2ohip_set_profiling", int(on))

@@ -240,6 +240,15 @@ int g2ohip_ba_get_schur(g2ohip_ctx* ctx, int* rows, int* cols, double* blocks, d
  * memory (LinearSolverPCG::_residual), info[3] 1 if the solve fell back to
  * FP64 (see g2ohip_options.precision) */
 int g2ohip_last_solve_info(g2ohip_ctx* ctx, double* info);
+/* The LM trials of the last g2ohip_optimize with the Levenberg algorithm, one
+ * record of G2OHIP_TRIAL_FIELDS doubles each (the do-loop of
+ * optimization_algorithm_levenberg.cpp:97-144): lambda used, currentChi,
+ * tempChi, the divisor of rho (computeScale() + 1e-3, or 1 after a failed
+ * solve), rho, accepted (1/0), PCG iterations, refinement corrections, FP64
+ * fallback (1/0), PCG residual memory after the solve. Copies at most max
+ * records into out (may be NULL) and stores the count in *count. */
+#define G2OHIP_TRIAL_FIELDS 10
+int g2ohip_lm_trials(g2ohip_ctx* ctx, double* out, int max, int* count);
 
 /* ------------------------------------------------------- kernel timing */
 /* Per-kernel durations from HIP events recorded on the context's stream around
