@@ -10,6 +10,8 @@
 //   LinearSolverHIP  - g2o::LinearSolver<MatrixType>::solve on a host
 //                      symmetric block matrix (linear_solver.h:47-145) with
 //                      LinearSolverPCG semantics (linear_solver_pcg.h:45-50);
+//   BALSolverHIP     - the same Solver contract for BlockSolver<
+//                      BlockSolverTraits<9,3>> over the BAL example's types;
 //   PoseGraphSolverHIP - the same Solver contract for VertexSE3 / EdgeSE3
 //                      graphs without Schur;
 //   GraphIO          - OptimizableGraph::load / save (optimizable_graph.cpp:
@@ -50,6 +52,7 @@ class DeviceContext {
 
 class BlockSolverHIP {
  public:
+  static constexpr int kCamDim = 6;
   explicit BlockSolverHIP(int device = 0) : _dev(device) {
     g2ohip_default_options(&_opt);
     if (_dev.get()) _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
@@ -200,6 +203,137 @@ class BlockSolverHIP {
   bool _hessianStale = true;
 };
 
+// The Solver contract of BlockSolver<BlockSolverTraits<9, 3>> over g2o's BAL
+// types (g2o/examples/bal/bal_example.cpp:70-207, 227-341): VertexCameraBAL
+// (9 values, additive update), VertexPointBAL, EdgeObservationBAL; FP64,
+// explicit Schur, LinearSolverPCG. Same calls as BlockSolverHIP; the vertex
+// Hessian mirror holds 9x9 camera blocks.
+class BALSolverHIP {
+ public:
+  static constexpr int kCamDim = 9;
+  explicit BALSolverHIP(int device = 0) : _dev(device) {
+    g2ohip_default_options(&_opt);
+    if (_dev.get()) _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
+  }
+  bool setGraph(int numCams, int numPoints, int numEdges, const int* edgeCam, const int* edgePoint,
+                const uint8_t* camFixed = nullptr, const uint8_t* pointFixed = nullptr) {
+    if (!_dev.ok(g2ohip_bal_set_structure(_dev.get(), numCams, numPoints, numEdges, edgeCam, edgePoint, camFixed,
+                                          pointFixed)))
+      return false;
+    long long info[7];
+    if (!_dev.ok(g2ohip_bal_structure_info(_dev.get(), info))) return false;
+    _size = static_cast<size_t>(9 * info[0] + 3 * info[1]);
+    std::vector<uint8_t> camUsed(numCams, 0), pointUsed(numPoints, 0);
+    for (int k = 0; k < numEdges; ++k) camUsed[edgeCam[k]] = pointUsed[edgePoint[k]] = 1;
+    _camIndex.assign(numCams, -1);
+    _pointIndex.assign(numPoints, -1);
+    int cf = 0, pf = 0;
+    for (int c = 0; c < numCams; ++c)
+      if (camUsed[c] && !(camFixed && camFixed[c])) _camIndex[c] = cf++;
+    for (int q = 0; q < numPoints; ++q)
+      if (pointUsed[q] && !(pointFixed && pointFixed[q])) _pointIndex[q] = pf++;
+    _hpp.assign(81 * static_cast<size_t>(cf), 0.0);
+    _hll.assign(9 * static_cast<size_t>(pf), 0.0);
+    _hessianStale = true;
+    return true;
+  }
+  // meas[E*2]; weight[E] (NULL = identity information); Huber delta (0 = none)
+  bool setMeasurements(const double* meas, const double* weight = nullptr, double huberDelta = 0.0) {
+    return _dev.ok(g2ohip_bal_set_measurements(_dev.get(), meas, weight, huberDelta));
+  }
+  bool setEstimates(const double* cams9, const double* points) {
+    return _dev.ok(g2ohip_bal_set_state(_dev.get(), cams9, points));
+  }
+  bool getEstimates(double* cams9, double* points) {
+    return _dev.ok(g2ohip_bal_get_state(_dev.get(), cams9, points));
+  }
+  bool setOptions(const g2ohip_options& opt) {
+    _opt = opt;
+    return _dev.ok(g2ohip_set_options(_dev.get(), &_opt));
+  }
+  const g2ohip_options& options() const { return _opt; }
+  bool setAlgorithm(int algorithm) { return _dev.ok(g2ohip_set_algorithm(_dev.get(), algorithm)); }
+
+  bool init(bool /*online*/ = false) { return _dev.ok(g2ohip_set_options(_dev.get(), &_opt)); }
+  bool buildStructure(bool /*zeroBlocks*/ = false) { return _size > 0; }
+  bool buildSystem() {
+    _hessianStale = true;
+    return _dev.ok(g2ohip_build_system(_dev.get()));
+  }
+  bool solve() {
+    int ok = 0;
+    return _dev.ok(g2ohip_solve(_dev.get(), &ok)) && ok != 0;
+  }
+  // vertex Hessians as BlockSolverHIP: row-major 9x9 for a camera, 3x3 for a point
+  double* vertexHessianMemory(bool camera, int index) {
+    const std::vector<int>& idx = camera ? _camIndex : _pointIndex;
+    if (index < 0 || index >= static_cast<int>(idx.size()) || idx[index] < 0) return nullptr;
+    return camera ? &_hpp[81 * static_cast<size_t>(idx[index])] : &_hll[9 * static_cast<size_t>(idx[index])];
+  }
+  bool syncVertexHessians() {
+    if (!_hessianStale) return true;
+    if (!_dev.ok(g2ohip_bal_get_hessian(_dev.get(), _hpp.data(), _hll.data(), nullptr))) return false;
+    _hessianStale = false;
+    return true;
+  }
+  const double* vertexHessian(bool camera, int index) {
+    double* m = vertexHessianMemory(camera, index);
+    return m && syncVertexHessians() ? m : nullptr;
+  }
+  int numCameras() const { return static_cast<int>(_camIndex.size()); }
+  int numPoints() const { return static_cast<int>(_pointIndex.size()); }
+  bool setLambda(double lambda, bool /*backup*/ = false) { return _dev.ok(g2ohip_set_lambda(_dev.get(), lambda)); }
+  void restoreDiagonal() { _dev.ok(g2ohip_restore_diagonal(_dev.get())); }
+  bool schur() const { return true; }
+  size_t vectorSize() const { return _size; }
+  const double* x() {
+    _x.resize(_size);
+    return _dev.ok(g2ohip_get_x(_dev.get(), _x.data())) ? _x.data() : nullptr;
+  }
+  const double* b() {
+    _b.resize(_size);
+    return _dev.ok(g2ohip_get_b(_dev.get(), _b.data())) ? _b.data() : nullptr;
+  }
+  double computeActiveErrors() {
+    double chi = -1;
+    return _dev.ok(g2ohip_compute_active_errors(_dev.get(), &chi)) ? chi : -1;
+  }
+  bool update() { return _dev.ok(g2ohip_update(_dev.get())); }
+  bool push() { return _dev.ok(g2ohip_push(_dev.get())); }
+  bool pop() { return _dev.ok(g2ohip_pop(_dev.get())); }
+  bool discardTop() { return _dev.ok(g2ohip_discard_top(_dev.get())); }
+  double computeLambdaInit() {
+    double l = -1;
+    return _dev.ok(g2ohip_lambda_init(_dev.get(), &l)) ? l : -1;
+  }
+  double computeScale() {
+    double s = 0;
+    return _dev.ok(g2ohip_compute_scale(_dev.get(), &s)) ? s : 0;
+  }
+  int iterationsLinearSolver() {
+    double info[4];
+    return _dev.ok(g2ohip_last_solve_info(_dev.get(), info)) ? static_cast<int>(info[0]) : -1;
+  }
+  int optimize(int iterations, std::vector<g2ohip_iter_stats>* stats = nullptr) {
+    std::vector<g2ohip_iter_stats> s(iterations > 0 ? iterations : 1);
+    int done = 0;
+    if (!_dev.ok(g2ohip_optimize(_dev.get(), iterations, s.data(), &done))) return -1;
+    if (stats) stats->assign(s.begin(), s.begin() + done);
+    return done;
+  }
+  std::string lastError() const { return _dev.lastError(); }
+  g2ohip_ctx* context() const { return _dev.get(); }
+
+ private:
+  DeviceContext _dev;
+  g2ohip_options _opt{};
+  size_t _size = 0;
+  std::vector<double> _x, _b;
+  std::vector<int> _camIndex, _pointIndex;
+  std::vector<double> _hpp, _hll;
+  bool _hessianStale = true;
+};
+
 // LinearSolver<Matrix{3,6}d>::solve for a host matrix given by its upper
 // blocks (the unit-test path: upload, then the device PCG).
 class LinearSolverHIP {
@@ -295,6 +429,10 @@ class GraphIO {
     std::vector<double> cams, points, meas, omega;  // omega: O00 O01 O11 per edge
     std::vector<uint8_t> camFixed, pointFixed;
   };
+  struct BAL {  // bal_example.cpp's arrays: cameras [C*9], points [P*3], observations in file order
+    std::vector<double> cams, points, meas;
+    std::vector<int> edgeCam, edgePoint;
+  };
   struct PoseGraph {
     std::vector<int> poseId, edgeFrom, edgeTo;
     std::vector<double> poses, meas, information;
@@ -332,6 +470,14 @@ class GraphIO {
     return ok(g2ohip_graph_get_ba(_g, o.camId.data(), o.cams.data(), o.camFixed.data(), o.pointId.data(),
                                   o.points.data(), o.pointFixed.data(), o.edgeCam.data(), o.edgePoint.data(),
                                   o.meas.data(), o.omega.data()));
+  }
+  bool getBAL(BAL& o) {
+    long long n[3] = {0, 0, 0};
+    if (!ok(g2ohip_graph_get_bal(_g, n, nullptr, nullptr, nullptr, nullptr, nullptr))) return false;
+    o.cams.resize(9 * n[0]); o.points.resize(3 * n[1]);
+    o.edgeCam.resize(n[2]); o.edgePoint.resize(n[2]); o.meas.resize(2 * n[2]);
+    return ok(g2ohip_graph_get_bal(_g, nullptr, o.cams.data(), o.points.data(), o.edgeCam.data(),
+                                   o.edgePoint.data(), o.meas.data()));
   }
   bool getPoseGraph(PoseGraph& o) {
     const std::vector<long long> n = info();

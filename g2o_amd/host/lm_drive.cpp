@@ -10,6 +10,8 @@
 // problem.bin: int32 C, P, E, precision, refine_steps; f64 cams[C*7],
 // intr[C*4], pts[P*3]; int32 edge_cam[E], edge_pt[E]; f64 meas[E*2],
 // weight[E]; u8 cam_fixed[C], pt_fixed[P]; f64 huber.
+//   or: lm_drive <problem.bal> <solver|optimize|gn> <iterations> <device>
+// (BAL: 9-DoF cameras, BlockSolver<9,3>, FP64)
 //   or: lm_drive <graph.g2o> <solver|optimize|gn|dogleg> <iterations> <device>
 //                [fx fy cx cy] [out.g2o]
 // (no intrinsics when every projection edge is an EDGE_PROJECT_XYZ2UV naming
@@ -42,19 +44,22 @@ bool read_vec(FILE* f, std::vector<T>& v, size_t n) {
 // levenberg.cpp:155-167) as g2o runs it: the largest diagonal entry over the
 // vertex Hessians the block solver mapped, times tau, or the user's
 // initialLambda.
-double lambda_init_from_vertices(g2o_hip::BlockSolverHIP& s) {
+template <typename Solver>
+double lambda_init_from_vertices(Solver& s) {
+  constexpr int D = Solver::kCamDim;
   if (s.options().user_lambda_init > 0) return s.options().user_lambda_init;
   double maxDiagonal = 0;
   for (int c = 0; c < s.numCameras(); ++c)
     if (const double* H = s.vertexHessian(true, c))
-      for (int j = 0; j < 6; ++j) maxDiagonal = std::max(std::fabs(H[7 * j]), maxDiagonal);
+      for (int j = 0; j < D; ++j) maxDiagonal = std::max(std::fabs(H[(D + 1) * j]), maxDiagonal);
   for (int p = 0; p < s.numPoints(); ++p)
     if (const double* H = s.vertexHessian(false, p))
       for (int j = 0; j < 3; ++j) maxDiagonal = std::max(std::fabs(H[4 * j]), maxDiagonal);
   return s.options().tau * maxDiagonal;
 }
 
-int run_solver_loop(g2o_hip::BlockSolverHIP& s, int iterations) {
+template <typename Solver>
+int run_solver_loop(Solver& s, int iterations) {
   s.init();
   double lambda = 0, ni = 2;
   for (int i = 0; i < iterations; ++i) {
@@ -117,7 +122,8 @@ int algorithm_of(const std::string& mode) {
   return mode == "gn" ? G2OHIP_GAUSS_NEWTON : mode == "dogleg" ? G2OHIP_DOGLEG : G2OHIP_LEVENBERG;
 }
 
-int run(g2o_hip::BlockSolverHIP& s, const char* mode, int iters) {
+template <typename Solver>
+int run(Solver& s, const char* mode, int iters) {
   if (std::string(mode) == "solver") {
     if (run_solver_loop(s, iters)) {
       std::fprintf(stderr, "solver loop failed: %s\n", s.lastError().c_str());
@@ -157,6 +163,31 @@ int run_pose_graph_file(g2o_hip::GraphIO& io, int argc, char** argv) {
     }
   }
   return 0;
+}
+
+// A BAL file (bal_example.cpp:265-335) through BALSolverHIP: the host loop
+// (solver) or the device loop (optimize, gn), FP64, nothing fixed and the
+// identity information, as bal_example.cpp sets up its graph.
+int run_bal_file(int argc, char** argv) {
+  g2o_hip::GraphIO io;
+  if (!io.loadBAL(argv[1])) {
+    std::fprintf(stderr, "cannot read %s: %s\n", argv[1], io.lastError().c_str());
+    return 1;
+  }
+  g2o_hip::GraphIO::BAL bal;
+  if (!io.getBAL(bal)) {
+    std::fprintf(stderr, "BAL arrays: %s\n", io.lastError().c_str());
+    return 1;
+  }
+  const int C = static_cast<int>(bal.cams.size() / 9), P = static_cast<int>(bal.points.size() / 3);
+  const int E = static_cast<int>(bal.edgeCam.size());
+  g2o_hip::BALSolverHIP s(std::atoi(argv[4]));
+  if (!s.setGraph(C, P, E, bal.edgeCam.data(), bal.edgePoint.data()) || !s.setMeasurements(bal.meas.data()) ||
+      !s.setEstimates(bal.cams.data(), bal.points.data())) {
+    std::fprintf(stderr, "setup failed: %s\n", s.lastError().c_str());
+    return 1;
+  }
+  return run(s, argv[2], std::atoi(argv[3]));
 }
 
 int run_g2o_file(int argc, char** argv) {
@@ -251,6 +282,7 @@ int main(int argc, char** argv) {
   }
   const std::string in(argv[1]);
   if (in.size() > 4 && in.compare(in.size() - 4, 4, ".g2o") == 0) return run_g2o_file(argc, argv);
+  if (in.size() > 4 && in.compare(in.size() - 4, 4, ".bal") == 0) return run_bal_file(argc, argv);
   FILE* f = std::fopen(argv[1], "rb");
   if (!f) return 2;
   int hdr[5];

@@ -169,3 +169,25 @@ def test_xyz2uv_file_through_the_cpp_driver(tmp_path):
     np.testing.assert_allclose(a[:, 2], [s["chi2"] for s in st], rtol=1e-10)
     saved = gio.G2oGraph.load(dst)
     assert saved.info()["xyz2uv_edges"] == prob.num_edges
+
+
+def test_bal_file_through_the_cpp_solver_contract(tmp_path, oracle_lib):
+    """A BAL file through BALSolverHIP (BlockSolver<9,3>): the reference's LM
+    loop on the Solver calls, with lambda_0 from the mapped 9x9 / 3x3 vertex
+    Hessians, equals the device loop and the oracle, iteration by
+    iteration."""
+    from g2o_amd import graphio
+    assert os.path.exists(DRIVER), "build first (make -C g2o_amd/csrc)"
+    prob = datasets.make_walk_bal(seed=47, num_cams=30, num_pts=1500)
+    path = str(tmp_path / "walk.bal")
+    graphio.save_bal(path, prob)
+    a = run(path, "solver", 8)
+    b = run(path, "optimize", 8)
+    assert a.shape == b.shape and a.shape[0] == 8
+    assert np.allclose(a[:, 1:4], b[:, 1:4], rtol=1e-9)
+    assert np.array_equal(a[:, 4], b[:, 4])
+    ora = oracle_lib.OracleBAL(prob)
+    n, trace, _ = ora.optimize(8, solver=1, tol=1e-6)
+    assert n == 8
+    assert np.allclose(b[:, 2], trace[:, 1], rtol=1e-8)
+    assert np.array_equal(b[:, 4], trace[:, 3])

@@ -106,6 +106,13 @@ def test_lm_with_rejected_trials_matches_oracle():
         assert s["levenberg_iterations"] == int(t[3])
         assert abs(s["lambda_"] - t[2]) <= 1e-6 * t[2]
         assert abs(s["chi2"] - t[1]) <= 1e-5 * t[1]
+    # trial by trial: lambda used, chi2 before and after, rho, the decision
+    tg, to = pgo.lm_trials(), ora.trials()
+    assert tg.shape[0] == to.shape[0] and (to[:, 5] == 0).sum() >= 3
+    assert np.array_equal(tg[:, 5], to[:, 5])
+    assert np.allclose(tg[:, 0], to[:, 0], rtol=1e-6)
+    assert np.allclose(tg[:, 1], to[:, 1], rtol=1e-5)
+    assert np.allclose(tg[:, 4], to[:, 4], rtol=1e-4, atol=1e-6)
 
 
 @pytest.mark.parametrize("which", ["translation", "rotation"])
