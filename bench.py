@@ -69,7 +69,7 @@ def stage_bytes(info, C):
     }
 
 
-SPMV_SYMBOL = "k_pcg_dir_spmv<float, 6, false, true, 7>"
+SPMV_SYMBOL = "k_pcg_dir_spmv<float, 6, false, true, 13>"
 STAGE_KERNELS = {
     "linearize": ["k_linearize<float>"],
     "schur": ["k_point_linv", "k_schur_win<float, 512>", "k_schur_offdiag<float>", "k_schur_diag_pm",

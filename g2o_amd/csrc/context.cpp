@@ -690,10 +690,10 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
       // solution misses the FP64 threshold by more than 10x, or the
       // correction stops above it
       if (pc.dn0 > 10.0 * pr.d0 || pc.dn_final > pr.d0 || pc.breakdown) refine_stalled = true;
-      // residual memory as the FP64 solver leaves it: at most half the
-      // threshold the solve stopped on, also when a correction stopped at
-      // its iteration cap above it
-      dn_final = std::min(pc.dn_final, pr.d0);
+      // the residual memory stays the main solve's: half the recurrence's
+      // r.M^-1 r at its exit, as the FP64 solver leaves it after the same
+      // iterations (a correction that runs ends far below it and would
+      // tighten the next solve's threshold beyond the FP64 path's)
       if (pc.iters == 0) break;  // the FP64 residual meets the threshold
       HK(launch_to_double<double>(6 * nb, B.x.p, d_x.p, 1.0, stream));
       xl_current = false;

@@ -1906,23 +1906,25 @@ __device__ bool pcg_head(const PcgPartials& pr, const double* __restrict__ part_
 
 // Explicit operator: d = s + beta d_old for the owned rows and q = A d over
 // the full block rows (both triangles). PCG_SPMV_ROWS block rows per
-// workgroup, one wavefront per row. A row is taken in chunks of CH blocks:
-// lane t of the wavefront loads the column of block t and gathers s and
-// d_old there once, forms d = s + beta d_old into LDS, and the lanes, as
+// workgroup, one wavefront per row. A row is taken in chunks of CH blocks
+// (up to 128): lane t of the wavefront loads the columns of blocks t and
+// t + 64 and gathers s and d_old there once, forms d = s + beta d_old into
+// LDS, and the lanes, as
 // groups of BD (lane k of a group owns output row k), then multiply their
 // blocks' rows by d read back from LDS. The matrix entries of the first
 // chunk and its gathered vectors are loaded before the iteration head (run
 // by wavefront 0, whose partial loads are issued first of all), so the
 // matrix, the gathers and the head's reduction are in flight together.
 // part_dq[workgroup] = d . q over its rows.
-// HS: block slots per group per chunk (CH = min(64, HS * groups)): 7 for BA
-// rows (tens of blocks), 1 for pose-graph rows (a handful).
+// HS: block slots per group per chunk (CH = min(128, HS * groups)): 13 for
+// BA rows (tens of blocks), 1 for pose-graph rows (a handful).
 constexpr int PCG_SPMV_ROWS = 8;
 constexpr int PCG_SPMV_THREADS = 64 * PCG_SPMV_ROWS;
 template <int BD, int HS>
 struct SpmvShape {
   static constexpr int GPW = 64 / BD;  // groups per wavefront
-  static constexpr int CH = 64 < GPW * HS ? 64 : GPW * HS;
+  static constexpr int CH = 128 < GPW * HS ? 128 : GPW * HS;
+  static constexpr int LB = (CH + 63) / 64;  // column slots per lane
 };
 // FIRST: iteration 0 (d = s; no d_old, no beta), a separate instantiation so
 // the other iterations carry no run-time branch on it.
@@ -1939,6 +1941,7 @@ __global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_dir_spmv(
     double d0_abs, int it, int last) {
   constexpr int GPW = SpmvShape<BD, HS>::GPW;
   constexpr int CH = SpmvShape<BD, HS>::CH;
+  constexpr int LB = SpmvShape<BD, HS>::LB;
   constexpr int R = PCG_SPMV_ROWS;
   __shared__ double s_dd[R][CH][BD];
   __shared__ double s_acc[R][64];
@@ -1965,16 +1968,20 @@ __global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_dir_spmv(
   const int p0 = F_ptr[rw], p1 = act ? F_ptr[rw + 1] : p0;
   const int len = p1 - p0;
   int nc = min(CH, len);
-  double sg[BD], dg[BD];
+  double sg[LB][BD], dg[LB][BD];
   MT v[HS][BD];
   // every load unconditional from a clamped (valid) index, values selected
   // afterwards, so all of them are in flight together
   auto load_chunk = [&](int c0) {
-    const size_t j = BD * static_cast<size_t>(F_col[p0 + c0 + (lane < nc ? lane : 0)]);
 #pragma unroll
-    for (int c = 0; c < BD; ++c) {
-      sg[c] = s[j + c];
-      if (!FIRST) dg[c] = d_old[j + c];
+    for (int lb = 0; lb < LB; ++lb) {
+      const int t = lane + 64 * lb;
+      const size_t j = BD * static_cast<size_t>(F_col[p0 + c0 + (t < nc ? t : 0)]);
+#pragma unroll
+      for (int c = 0; c < BD; ++c) {
+        sg[lb][c] = s[j + c];
+        if (!FIRST) dg[lb][c] = d_old[j + c];
+      }
     }
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
@@ -2007,9 +2014,13 @@ __global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_dir_spmv(
       load_chunk(c0);
       wave_sync();  // the previous chunk's reads of s_dd are done
     }
-    if (lane < nc) {
 #pragma unroll
-      for (int c = 0; c < BD; ++c) s_dd[wv][lane][c] = FIRST ? sg[c] : fma(beta, dg[c], sg[c]);
+    for (int lb = 0; lb < LB; ++lb) {
+      const int t = lane + 64 * lb;
+      if (t < nc) {
+#pragma unroll
+        for (int c = 0; c < BD; ++c) s_dd[wv][t][c] = FIRST ? sg[lb][c] : fma(beta, dg[lb][c], sg[lb][c]);
+      }
     }
     wave_sync();
 #pragma unroll
@@ -2143,9 +2154,12 @@ hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st,
                                const double* Ddiag, long long nnz) {
-  // block slots per group for long rows: a chunk of 64 blocks, fewer for
-  // blocks above 8 (their register footprint per slot)
-  constexpr int HS_LONG = BD <= 8 ? (64 + 64 / BD - 1) / (64 / BD) : 5;
+  // block slots per group for long rows: FP32 blocks a chunk of 128 blocks
+  // (two column slots per lane, so a BA row of ~70 blocks is one round trip),
+  // FP64 blocks 64 (the register footprint of 128 spills), fewer for blocks
+  // above 8
+  constexpr int CH_LONG = sizeof(MT) == 4 ? 128 : 64;
+  constexpr int HS_LONG = BD <= 8 ? (CH_LONG + 64 / BD - 1) / (64 / BD) : 5;
   const bool short_rows = nnz >= 0 && nnz < static_cast<long long>(PCG_SHORT_ROW) * nb;
 #define G2OHIP_SPMV_LAUNCH(FIRST, D64, HS)                                                                        \
   hipLaunchKernelGGL((k_pcg_dir_spmv<MT, BD, FIRST, D64, HS>), dim3(pcg_spmv_blocks(nb)), dim3(PCG_SPMV_THREADS), \
