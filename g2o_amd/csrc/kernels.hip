@@ -83,6 +83,28 @@ __device__ __forceinline__ int lower_bound_i(const int* a, int n, int key) {
   return lo;
 }
 
+// Sum of v over the run of consecutive lanes that share this lane's key (the
+// keys of one run sit in consecutive lanes): a segmented inclusive scan, after
+// which the last lane of each run holds the run's total; returns true there.
+// Every lane of the wavefront must take part (inactive lanes pass a key of
+// their own).
+template <int N>
+__device__ __forceinline__ bool run_sum(int key, double (&v)[N]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int k2 = __shfl_up(key, off, 64);
+    const bool take = lane >= off && k2 == key;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const double t = __shfl_up(v[n], off, 64);
+      v[n] = take ? v[n] + t : v[n];
+    }
+  }
+  const int kn = __shfl_down(key, 1, 64);
+  return lane == 63 || kn != key;
+}
+
 __constant__ int c_up6_r[21] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5};
 __constant__ int c_up6_c[21] = {0, 1, 2, 3, 4, 5, 1, 2, 3, 4, 5, 2, 3, 4, 5, 3, 4, 5, 4, 5, 5};
 
@@ -266,13 +288,16 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
   for (int i = tid; i < np * 9; i += LIN_THREADS) s_pacc[i] = 0.0;
   __syncthreads();
   double chi = 0;
-  for (int sb = s0 + tid; sb < s1; sb += LIN_PF * LIN_THREADS) {
+  // point-major slots: the lanes of one point are consecutive, so its Hll / b_l
+  // terms are reduced across the run before one LDS atomic per run and entry;
+  // the camera-side terms go to the chunk's camera accumulators directly
+  for (int base = s0; base < s1; base += LIN_PF * LIN_THREADS) {
     SlotOps op[LIN_PF];
 #pragma unroll
-    for (int u = 0; u < LIN_PF; ++u) load_slot_ops(g, camRt, pts, min(sb + u * LIN_THREADS, s1 - 1), op[u]);
+    for (int u = 0; u < LIN_PF; ++u) load_slot_ops(g, camRt, pts, min(base + tid + u * LIN_THREADS, s1 - 1), op[u]);
 #pragma unroll
     for (int u = 0; u < LIN_PF; ++u) {
-      if (sb + u * LIN_THREADS >= s1) break;
+      const bool valid = base + tid + u * LIN_THREADS < s1;
       const SlotOps& o = op[u];
       const int lp = o.pos - p0, ph = o.ph, chh = o.chh;
       const double* Rt = o.Rt;
@@ -288,34 +313,46 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize(DevGraph g, const dou
       whiten_obs(g, o.S, e, Jp, Jc);
       double rho[3];
       huber_rho(w * (e[0] * e[0] + e[1] * e[1]), g.huber, rho);
-      chi += rho[0];
       const double om = rho[1] * w;
       const double wr0 = -om * e[0], wr1 = -om * e[1];
-      if (ph >= 0) {
-        double* pa = s_pacc + 9 * lp;
-        int k = 0;
-        for (int r = 0; r < 3; ++r)
-          for (int c = r; c < 3; ++c) atomicAdd(&pa[k++], om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]));
-        for (int r = 0; r < 3; ++r) atomicAdd(&pa[6 + r], Jp[r] * wr0 + Jp[3 + r] * wr1);
-      }
-      if (chh >= 0) {
-        const int lc = lower_bound_i(s_cam, nc, chh);
-        double* ca = s_cacc + 27 * lc;
-        for (int k = 0; k < 21; ++k) {
-          const int r = c_up6_r[k], c = c_up6_c[k];
-          atomicAdd(&ca[k], om * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c]));
-        }
-        for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], Jc[r] * wr0 + Jc[6 + r] * wr1);
-        if (ph >= 0) {
-          // the 18 entries as 9 two-element stores (a block is 8-byte aligned)
-          HT v[18];
-          for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 3; ++c) v[3 * r + c] = static_cast<HT>(om * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]));
-          using HT2 = typename std::conditional<sizeof(HT) == 4, float2, double2>::type;
-          HT2* h2 = reinterpret_cast<HT2*>(Hpl + 18 * static_cast<size_t>(o.q));
+      if (valid) {
+        chi += rho[0];
+        if (chh >= 0) {
+          double* ca = s_cacc + 27 * lower_bound_i(s_cam, nc, chh);
+          for (int k = 0; k < 21; ++k) {
+            const int r = c_up6_r[k], c = c_up6_c[k];
+            atomicAdd(&ca[k], om * (Jc[r] * Jc[c] + Jc[6 + r] * Jc[6 + c]));
+          }
+          for (int r = 0; r < 6; ++r) atomicAdd(&ca[21 + r], Jc[r] * wr0 + Jc[6 + r] * wr1);
+          if (ph >= 0) {
+            // the 18 entries as 9 two-element stores (a block is 8-byte aligned)
+            HT v[18];
+            for (int r = 0; r < 6; ++r)
+              for (int c = 0; c < 3; ++c)
+                v[3 * r + c] = static_cast<HT>(om * (Jc[r] * Jp[c] + Jc[6 + r] * Jp[3 + c]));
+            using HT2 = typename std::conditional<sizeof(HT) == 4, float2, double2>::type;
+            HT2* h2 = reinterpret_cast<HT2*>(Hpl + 18 * static_cast<size_t>(o.q));
 #pragma unroll
-          for (int k = 0; k < 9; ++k) h2[k] = HT2{v[2 * k], v[2 * k + 1]};
+            for (int k = 0; k < 9; ++k) h2[k] = HT2{v[2 * k], v[2 * k + 1]};
+          }
         }
+      }
+      const bool pt_on = valid && ph >= 0;
+      double pv[9];
+      {
+        int k = 0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = r; c < 3; ++c) pv[k++] = pt_on ? om * (Jp[r] * Jp[c] + Jp[3 + r] * Jp[3 + c]) : 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) pv[6 + r] = pt_on ? Jp[r] * wr0 + Jp[3 + r] * wr1 : 0.0;
+      }
+      const int key = pt_on ? lp : -1 - (threadIdx.x & 63);
+      if (run_sum<9>(key, pv) && pt_on) {
+        double* pa = s_pacc + 9 * lp;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) atomicAdd(&pa[k], pv[k]);
       }
     }
   }
