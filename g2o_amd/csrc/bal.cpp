@@ -333,7 +333,8 @@ int g2ohip_ctx::bal_get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
     if (st.Pf) HK(hipMemcpy(h.data(), d_Hll.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     static const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int l = 0; l < st.Pf; ++l)
-      for (int k = 0; k < 9; ++k) Hll[9 * static_cast<size_t>(l) + k] = h[6 * static_cast<size_t>(l) + full[k]];
+      for (int k = 0; k < 9; ++k)
+        Hll[9 * static_cast<size_t>(st.ph_pub[l]) + k] = h[6 * static_cast<size_t>(l) + full[k]];
   }
   if (Hpl_edge) {
     std::vector<double> h(27 * static_cast<size_t>(st.nq));

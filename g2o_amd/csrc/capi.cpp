@@ -239,6 +239,7 @@ int g2ohip_get_x(g2ohip_ctx* ctx, double* x) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
         hipMemcpy(x, ctx->d_x.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       return ctx->fail(G2OHIP_ERR_HIP, "copy of x failed");
+    ctx->points_to_pub(x);
     return G2OHIP_OK;
   });
 }
@@ -250,6 +251,7 @@ int g2ohip_get_b(g2ohip_ctx* ctx, double* b) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
         hipMemcpy(b, ctx->d_b.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       return ctx->fail(G2OHIP_ERR_HIP, "copy of b failed");
+    ctx->points_to_pub(b);
     return G2OHIP_OK;
   });
 }

@@ -1214,6 +1214,14 @@ int g2ohip_ctx::replay_pcg(int reps, double* us_spmv, double* us_iter) {
 }
 
 // --------------------------------------------------------------- inspection
+void g2ohip_ctx::points_to_pub(double* v) const {
+  if (mode == 1 || st.Pf == 0) return;
+  const size_t off = cam_dim() * static_cast<size_t>(st.Cf);
+  std::vector<double> tmp(v + off, v + off + 3 * static_cast<size_t>(st.Pf));
+  for (int i = 0; i < st.Pf; ++i)
+    for (int r = 0; r < 3; ++r) v[off + 3 * static_cast<size_t>(st.ph_pub[i]) + r] = tmp[3 * static_cast<size_t>(i) + r];
+}
+
 int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
   if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_system && !have_exact) return fail(G2OHIP_ERR_INVALID, "build_system first");
@@ -1224,7 +1232,8 @@ int g2ohip_ctx::get_hessian(double* Hpp, double* Hll, double* Hpl_edge) {
     if (st.Pf) HK(hipMemcpy(h.data(), d_Hll.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int p = 0; p < st.Pf; ++p)
-      for (int k = 0; k < 9; ++k) Hll[9 * static_cast<size_t>(p) + k] = h[6 * static_cast<size_t>(p) + map[k]];
+      for (int k = 0; k < 9; ++k)
+        Hll[9 * static_cast<size_t>(st.ph_pub[p]) + k] = h[6 * static_cast<size_t>(p) + map[k]];
   }
   if (Hpl_edge) {
     std::vector<double> h(18 * static_cast<size_t>(st.nq));

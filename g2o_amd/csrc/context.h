@@ -232,6 +232,9 @@ struct g2ohip_ctx {
   size_t pts_len() const { return mode == 1 ? 0 : 3 * static_cast<size_t>(st.P); }
   // camera block size of the reduced system
   int cam_dim() const { return mode == 2 ? 9 : 6; }
+  // the point part of a system-sized vector (x, b), from the device's point
+  // order to hessianIndex order
+  void points_to_pub(double* v) const;
   size_t system_size() const {
     return mode == 1 ? 6 * static_cast<size_t>(pgst.Nf)
                      : cam_dim() * static_cast<size_t>(st.Cf) + 3 * static_cast<size_t>(st.Pf);

@@ -85,6 +85,20 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
     return a.p < b.p;
   });
   st.npos = static_cast<int>(order.size());
+  // device point index: the free points renumbered in processing order, so
+  // every per-point array (Hll, b_l, x_l, the point factors) is read in the
+  // order the point-major kernels walk it; ph_pub maps it back to the
+  // hessianIndex above for the interface's vectors and blocks
+  st.ph_pub.assign(st.Pf, -1);
+  {
+    int nxt = 0;
+    for (const PK& o : order)
+      if (st.ptH[o.p] >= 0) {
+        st.ph_pub[nxt] = st.ptH[o.p];
+        st.ptH[o.p] = nxt++;
+      }
+    if (nxt != st.Pf) return "internal error: free point without an active edge";
+  }
   st.pos_pt.resize(st.npos);
   st.pos_h.resize(st.npos);
   st.pos_ptr.assign(st.npos + 1, 0);

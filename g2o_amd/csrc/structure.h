@@ -66,7 +66,8 @@ struct WinPlan {
 struct BAStructure {
   int C = 0, P = 0, E = 0;
   int Cf = 0, Pf = 0;
-  std::vector<int> camH, ptH;        // free index or -1
+  std::vector<int> camH, ptH;        // free index or -1 (points: in processing order)
+  std::vector<int> ph_pub;           // Pf: device point index -> hessianIndex (input order)
   std::vector<int> free_cam_raw;     // Cf -> raw camera
   // processing positions: points with at least one active edge
   int npos = 0;
