@@ -29,6 +29,12 @@ METRIC = "LM iter/s at 1.5k-cam/500k-pt BA, FP32+refine vs FP64, rel chi2 diff â
 PEAK_HBM = 8.0e12
 PEAK_F64_MFMA = 78.6e12  # dense FP64 matrix peak of MI355X
 GATE = 1e-6
+# LM runs per path from the same start behind the accuracy gate: two FP64
+# runs that differ only in the order of their FP64 LDS sums end 20 iterations
+# later up to ~1e-6 apart, now and then 1.7e-5 (the LM's gain ratio divides
+# chi2 differences by a predicted decrease ~1e3 times smaller), so the gate
+# compares the paths' median final chi2 over this many runs each
+GATE_RUNS = 7
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ba_1500cam_500kpt.md")
 TRACE_FILE = os.path.join(ROOT, "profiles", "kernel_stats_ba_1500cam_500kpt.md")
 
@@ -116,6 +122,31 @@ def run_gpu(prob, precision, refine, steps, warmup, schur_mode=None, replay=Fals
     prof["pcg_spmv_iterations"] = sum(st["iterations_linear_solver"] for st in pstats)
     rep = ba.replay_pcg(200) if replay else None
     return ba, stats, t1 - t0, prof, pstats, rep
+
+
+def final_chi2_runs(ba, prob, steps, n):
+    """Final chi2 of n more LM runs of `steps` iterations from the problem's start."""
+    out = []
+    for _ in range(n):
+        ba.set_state(prob.cams, prob.pts)
+        out.append(ba.optimize(steps)[-1]["chi2"])
+    return out
+
+
+def accuracy_gate(f32, f64):
+    """The gate on the median final chi2 of each path's runs, with every run's value."""
+    import numpy as np
+    a32, a64 = np.asarray(f32), np.asarray(f64)
+    m32, m64 = float(np.median(a32)), float(np.median(a64))
+    rel = abs(m32 - m64) / m64
+    return {"bound": GATE, "pass": bool(rel <= GATE), "statistic": f"median final chi2 over {len(a32)} FP32+refine "
+            f"and {len(a64)} FP64 runs of the same iterations from the same start (the timed, the profiled and "
+            f"{len(a32) - 2} more runs per path)",
+            "rel_median_final_chi2_diff": rel,
+            "fp32_final_chi2_runs": [float(v) for v in a32], "fp64_final_chi2_runs": [float(v) for v in a64],
+            "fp32_runs_rel_sd": float(a32.std() / m64), "fp64_runs_rel_sd": float(a64.std() / m64),
+            "note": "two FP64 runs differ only in the order of their FP64 LDS sums; the LM amplifies that "
+                    "rounding to ~1e-7..1e-6 over 20 iterations (rel sd of the FP64 runs above)"}
 
 
 def cpu_model():
@@ -379,6 +410,14 @@ def main():
     # own run-to-run rounding, amplified by the LM iteration
     noise64 = abs(p64[-1]["chi2"] - s64[-1]["chi2"]) / s64[-1]["chi2"] if len(p64) == len(s64) else None
     noise32 = abs(p32[-1]["chi2"] - s32[-1]["chi2"]) / s32[-1]["chi2"] if len(p32) == len(s32) else None
+    more = max(GATE_RUNS - 2, 0)
+    gate = accuracy_gate([s32[-1]["chi2"], p32[-1]["chi2"]] + final_chi2_runs(ba32, prob, args.steps, more),
+                         [s64[-1]["chi2"], p64[-1]["chi2"]] + final_chi2_runs(ba64, prob, args.steps, more))
+    gate["single_pair_rel_diff"] = rel
+    gate["fp64_rerun_rel_final_chi2_diff"] = noise64
+    gate["fp32_rerun_rel_final_chi2_diff"] = noise32
+    log(f"accuracy gate: median rel diff {gate['rel_median_final_chi2_diff']:.2e} (timed pair {rel:.2e}), "
+        f"run sd fp32 {gate['fp32_runs_rel_sd']:.1e}, fp64 {gate['fp64_runs_rel_sd']:.1e}")
 
     info = ba32.structure_info()
     C = prob.num_cams
@@ -438,10 +477,9 @@ def main():
             "fp64_path_iter_per_s": round(len(s64) / dt64, 3),
             "fp64_path_ms_per_step": round(dt64 / len(s64) * 1e3, 3),
             "speedup_vs_fp64_path": round(value / (len(s64) / dt64), 3),
-            "rel_final_chi2_diff_vs_fp64": rel,
-            "accuracy_gate": {"bound": GATE, "pass": bool(rel <= GATE),
-                              "fp64_rerun_rel_final_chi2_diff": noise64,
-                              "fp32_rerun_rel_final_chi2_diff": noise32},
+            "rel_final_chi2_diff_vs_fp64": gate["rel_median_final_chi2_diff"],
+            "rel_final_chi2_diff_vs_fp64_timed_pair": rel,
+            "accuracy_gate": gate,
             "final_chi2_fp32_refine": s32[-1]["chi2"],
             "final_chi2_fp64": s64[-1]["chi2"],
             "lm_trials": [s["levenberg_iterations"] for s in s32],
