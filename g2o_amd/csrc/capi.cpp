@@ -76,6 +76,16 @@ int g2ohip_create(int device, g2ohip_ctx** out) try {
       else if (std::strcmp(sp, "rows") == 0) c->schur_path = 2;
     }
     if (const char* bs = std::getenv("G2OHIP_BAL_SCHUR")) c->bal_mfma = std::strcmp(bs, "valu") != 0;
+    // the concurrent Schur diagonal pass (G2OHIP_SCHUR_FORK=0: one stream)
+    const char* fork_env = std::getenv("G2OHIP_SCHUR_FORK");
+    if (!(fork_env && fork_env[0] == '0')) {
+      if (hipStreamCreate(&c->side) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        delete c;
+        return G2OHIP_ERR_HIP;
+      }
+    }
     const char* mapped_env = std::getenv("G2OHIP_HOST_MAPPED");
     const bool mapped = !(mapped_env && mapped_env[0] == '0');
     void* hp = nullptr;

@@ -34,6 +34,9 @@ g2ohip_ctx::~g2ohip_ctx() {
   for (auto* b : stack_p) delete b;
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (h_prog) (void)hipHostFree(h_prog);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -597,6 +600,24 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
 // BlockSolver::solve with Schur (block_solver.hpp:331-477).
 // defer_err: the final error word (back-substitution) is left for the
 // caller to read with its next scalars (d_err).
+// Diagonal blocks, block-Jacobi inverses and (explicit) the reduced
+// right-hand side in FP64, re-linearised from the state point-major (FP32
+// blocks are too coarse for the cancellation in Hpp - sum V Dinv V^T), on
+// stream `st`. FP32 blocks: the diagonal blocks stay FP64 (d_Sdiag) for the
+// SpMV and for the implicit operator (the FP32 implicit path takes its
+// reduced right-hand side from here too: b_p - sum V Dinv b_l cancels towards
+// convergence, and an FP32 Hpl would leave it mostly rounding error).
+template <typename HT>
+int g2ohip_ctx::schur_diag(PrecBufs<HT>& B, bool implicit, hipStream_t st) {
+  constexpr bool f32 = std::is_same<HT, float>::value;
+  double* sdiag = f32 ? d_Sdiag.p : nullptr;
+  const bool rhs_here = !implicit || f32;
+  HK(launch_schur_diag_pm<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, rhs_here ? d_b.p : nullptr, lambda,
+                              d_diag_part.p, implicit ? nullptr : B.F_val.p, B.Jinv.p,
+                              rhs_here ? d_bschur.p : nullptr, sdiag, d_err.p, st));
+  return G2OHIP_OK;
+}
+
 template <typename HT>
 int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   const int which = std::is_same<HT, float>::value ? 0 : 1;
@@ -611,6 +632,13 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
   HK(B.Jinv.alloc(36 * static_cast<size_t>(nb)));
   {
     ProfScope ps(this, G2OHIP_K_SCHUR);
+    // explicit operator: the FP64 diagonal pass (below) shares no data with
+    // the off-diagonal blocks and runs beside them on the side stream
+    const bool fork = !implicit && side != nullptr;
+    if (fork) {
+      HK(hipEventRecord(ev_fork, stream));
+      HK(hipStreamWaitEvent(side, ev_fork, 0));
+    }
     if (implicit) {
       HK(B.Dinv.alloc(9 * static_cast<size_t>(std::max(st.Pf, 1))));
       HK(launch_point_dinv<HT>(g, d_Hll.p, lambda, B.Dinv.p, d_err.p, stream));
@@ -621,23 +649,15 @@ int g2ohip_ctx::solve_impl(int* ok, bool defer_err) {
     } else {
       HK(B.F_val.alloc(36 * static_cast<size_t>(std::max(st.nF, 1))));
       HK(d_linv.alloc(6 * static_cast<size_t>(std::max(st.Pf, 1))));
+      if (fork) CK(schur_diag<HT>(B, implicit, side));
       HK(launch_point_linv(g, d_Hll.p, lambda, d_linv.p, d_err.p, stream));
       CK(schur_offdiag<HT>(B));
     }
-    // diagonal blocks, block-Jacobi inverses and (explicit) the reduced
-    // right-hand side in FP64, re-linearised from the state point-major (FP32
-    // blocks are too coarse for the cancellation in Hpp - sum V Dinv V^T)
-    // FP32 blocks: the diagonal blocks stay FP64 (d_Sdiag) for the SpMV and
-    // for the implicit operator
-    // (the FP32 implicit path takes its reduced right-hand side from here
-    // too: b_p - sum V Dinv b_l cancels towards convergence, and an FP32 Hpl
-    // would leave it mostly rounding error)
-    constexpr bool f32 = std::is_same<HT, float>::value;
-    double* sdiag = f32 ? d_Sdiag.p : nullptr;
-    const bool rhs_here = !implicit || f32;
-    HK(launch_schur_diag_pm<HT>(g, d_camRt.p, d_pts.p, d_Hll.p, d_Hpp.p, rhs_here ? d_b.p : nullptr, lambda,
-                                d_diag_part.p, implicit ? nullptr : B.F_val.p, B.Jinv.p,
-                                rhs_here ? d_bschur.p : nullptr, sdiag, d_err.p, stream));
+    if (!fork) CK(schur_diag<HT>(B, implicit, stream));
+    if (fork) {
+      HK(hipEventRecord(ev_join, side));
+      HK(hipStreamWaitEvent(stream, ev_join, 0));
+    }
   }
   if (implicit && !std::is_same<HT, float>::value) {
     // matrix-free reduced operator: right-hand side by one pass

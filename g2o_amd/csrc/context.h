@@ -76,6 +76,10 @@ size_t noalloc_bytes(int nb, int bd);
 struct g2ohip_ctx {
   int device = -1;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for the Schur diagonal pass, which runs
+  // beside the off-diagonal blocks; fork / join events on the two streams
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string last_error;
   g2ohip_options opt{};
   const volatile int* force_stop = nullptr;
@@ -200,6 +204,9 @@ struct g2ohip_ctx {
   // factors d_linv must be current)
   template <typename HT>
   int schur_offdiag(g2ohip::PrecBufs<HT>& B);
+  // the FP64 diagonal pass of the reduced system on stream st
+  template <typename HT>
+  int schur_diag(g2ohip::PrecBufs<HT>& B, bool implicit, hipStream_t st);
   int compute_active_errors(double* chi2, double* err_out = nullptr);
   int build_system(bool read_chi = true);
   int solve(int* ok, bool defer_err = false);
