@@ -76,6 +76,7 @@ int g2ohip_create(int device, g2ohip_ctx** out) try {
       else if (std::strcmp(sp, "rows") == 0) c->schur_path = 2;
     }
     if (const char* bs = std::getenv("G2OHIP_BAL_SCHUR")) c->bal_mfma = std::strcmp(bs, "valu") != 0;
+    if (const char* pf = std::getenv("G2OHIP_PCG_FUSED")) c->pcg_fused = pf[0] != '0';
     // the concurrent Schur diagonal pass (G2OHIP_SCHUR_FORK=0: one stream)
     const char* fork_env = std::getenv("G2OHIP_SCHUR_FORK");
     if (!(fork_env && fork_env[0] == '0')) {

@@ -244,6 +244,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_scal.alloc(8));
   HK(d_dq_part.alloc(std::max(st.Cf, 1)));
   HK(d_dn_part.alloc(std::max(st.Cf, 1)));
+  HK(d_fpart.alloc(8 * static_cast<size_t>(pcg_fused_blocks(std::max(st.Cf, 1)))));
   HK(d_err.alloc(4));
   HK(d_sc.alloc(1));
   HK(hipMemsetAsync(d_b.p, 0, nx * sizeof(double), stream));
@@ -580,6 +581,22 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
     HK((pcg_drive<HT, 6>(stream, h_pinned, progress(), nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
                          d_dn_part.p, d_dq_part.p, implicit_reduce_blocks(nb), d_sc.p, max_iter, op, out,
                          &prog_failed, ctl.abort)));
+  } else if (pcg_fused) {
+    HK(B.s2.alloc(n)); HK(B.w.alloc(n)); HK(B.w2.alloc(n));
+    double* const s2[2] = {B.s.p, B.s2.p};
+    double* const w2[2] = {B.w.p, B.w2.p};
+    auto init = [&]() { return launch_pcg_init<HT, 6>(nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, d_dn_part.p, d_sc.p,
+                                                      stream); };
+    auto step = [&](int it, int last) -> hipError_t {
+      const int pb = prof_begin();
+      const hipError_t e = launch_pcg_fused<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p,
+                                                   std::is_same<HT, float>::value ? d_Sdiag.p : nullptr, B.Jinv.p,
+                                                   B.x.p, B.r.p, B.q.p, s2, w2, dirs, d_dn_part.p, d_fpart.p,
+                                                   d_sc.p, ctl, it, last, stream, st.nF);
+      prof_end(G2OHIP_K_PCG_SPMV, pb);
+      return e;
+    };
+    HK(pcg_loop(stream, h_pinned, progress(), d_sc.p, max_iter, init, step, out, &prog_failed, ctl.abort));
   } else {
     auto op = [&](int it, int last, const double* d_old, double* d) -> hipError_t {
       const int pb = prof_begin();

@@ -50,6 +50,7 @@ struct PrecBufs {
   DBuf<HT> Dinv;      // Pf * 9, (Hll + lambda I)^-1 (implicit operator, exact back-substitution)
   DBuf<HT> part;      // 36 per window-plan slot: off-diagonal Schur partials
   DBuf<double> x, r, d, d2, q, s;  // PCG vectors, FP64 (d, d2: directions of odd / even iterations)
+  DBuf<double> s2, w, w2;          // one-launch iteration: the second s buffer, w = M^-1 q (two buffers)
 };
 
 // Outcome of one PCG solve (read back from PcgScalars).
@@ -80,6 +81,9 @@ struct g2ohip_ctx {
   // beside the off-diagonal blocks; fork / join events on the two streams
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // explicit-operator PCG in one launch per iteration (G2OHIP_PCG_FUSED=0:
+  // direction + product and update as two launches)
+  bool pcg_fused = true;
   std::string last_error;
   g2ohip_options opt{};
   const volatile int* force_stop = nullptr;
@@ -136,6 +140,7 @@ struct g2ohip_ctx {
   g2ohip::DBuf<double> d_Hll, d_b, d_Hpp, d_hpp_part, d_diag_part, d_chunk_chi, d_coef_part, d_bschur, d_x, d_red, d_scal,
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;
+  g2ohip::DBuf<double> d_fpart;  // one-launch PCG partials: [2][4][pcg_fused_blocks(Cf)]
   g2ohip::DBuf<int> d_err;
   // host-mapped PCG progress record (PcgControl::prog); prog_ok drops to
   // false if it ever fails to reach the host, and the PCG then polls the

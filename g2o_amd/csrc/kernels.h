@@ -231,6 +231,16 @@ hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const
                                const double* d_old, double* d, double* q, const double* part_rs, double* part_dq,
                                PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st,
                                const double* Ddiag = nullptr, long long nnz = -1);
+// One launch per PCG iteration on the explicit operator (k_pcg_fused): the
+// update of iteration it-1 and the direction + product of iteration it.
+// s, w, d: two buffers each (launch it writes index it & 1); part_init:
+// k_pcg_init's partials; part: 2 x 4 x pcg_fused_blocks(nb) doubles.
+template <typename MT, int BD>
+hipError_t launch_pcg_fused(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* Ddiag,
+                            const MT* J, double* x, double* r, double* q, double* const s[2], double* const w[2],
+                            double* const d[2], const double* part_init, double* part, PcgScalars* sc,
+                            const PcgControl& ctl, int it, int last, hipStream_t st, long long nnz);
+int pcg_fused_blocks(int nb);
 template <int BD>
 hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
                           PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);

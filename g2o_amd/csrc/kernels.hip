@@ -2115,6 +2115,290 @@ __global__ __launch_bounds__(64) void k_pcg_dir(int nb, const double* __restrict
   }
 }
 
+// ---------------------------------------------- one launch per iteration
+// LinearSolverPCG::solve (linear_solver_pcg.hpp:137-153) on the explicit
+// operator with the update of iteration L-1 folded into launch L, so an
+// iteration is one launch instead of two. With w = M^-1 q (M symmetric),
+// r' = r - alpha q and s' = M^-1 r' = s - alpha w, so
+//   dn' = r'.s' = r.s - 2 alpha s.q + alpha^2 q.w
+// follows from dot products of the vectors launch L-1 already holds. Launch
+// L >= 1 sums launch L-1's partials d.q, s.q, q.w and r.s, forms
+// alpha = r.s / d.q and dn', applies x += alpha d, r -= alpha q,
+// s -= alpha w to its own rows, takes the stop test on dn' and
+// beta = dn' / r.s, gathers s - alpha w and d at the columns it needs for
+// d' = s' + beta d, then forms q' = A d' and w' = M^-1 q' for its rows and
+// the partials d'.q', s'.q', q'.w', r'.s'. The r.s passed on is summed from
+// the updated vectors, so dn's formula never feeds on its own output.
+// Launch 0 takes dn = r.s of k_pcg_init's partials and d = s.
+// s, w and d alternate between two buffers by launch parity (launch L
+// writes index L & 1 and gathers from the other); x, r and q are updated in
+// place for the workgroup's own rows; the partials alternate as well
+// ([parity][4][workgroups]). A solve that ended at launch L'
+// (sc->done = L' + 1) makes later launches return at once, while the other
+// workgroups of launch L' itself still apply their rows' update.
+template <int NB>
+struct PcgPartialsN {
+  double v[NB];
+};
+template <int NB>
+__device__ __forceinline__ PcgPartialsN<NB> pcg_part_load_n(const double* __restrict__ part, int n) {
+  const int lane = threadIdx.x & 63;
+  PcgPartialsN<NB> p;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) p.v[j] = part[min(lane + 64 * j, n - 1)];
+  return p;
+}
+template <int NB>
+__device__ double pcg_part_sum_n(const PcgPartialsN<NB>& p, const double* __restrict__ part, int n) {
+  const int lane = threadIdx.x & 63;
+  double s = 0, c = 0;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) neumaier_add(s, c, lane + 64 * j < n ? p.v[j] : 0.0);
+  for (int i = lane + 64 * NB; i < n; i += 64) neumaier_add(s, c, part[i]);
+  return wave_total_compensated(s, c);
+}
+constexpr int PCG_FUSED_PB = 3;  // partial batches per lane loaded ahead (192 workgroups)
+
+template <typename MT, int BD, bool FIRST, bool D64, int HS>
+__global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_fused(
+    int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
+    const double* __restrict__ Ddiag, const MT* __restrict__ J, double* __restrict__ x, double* __restrict__ r,
+    double* __restrict__ q, const double* __restrict__ s_prev, double* __restrict__ s_next,
+    const double* __restrict__ w_prev, double* __restrict__ w_next, const double* __restrict__ d_prev,
+    double* __restrict__ d_next, const double* __restrict__ part_prev, int n_prev, double* __restrict__ part_next,
+    PcgScalars* sc, PcgProgress* prog, const int* abort, double tol, double resid_mem, double d0_abs, int it,
+    int last) {
+  constexpr int GPW = SpmvShape<BD, HS>::GPW;
+  constexpr int CH = SpmvShape<BD, HS>::CH;
+  constexpr int LB = SpmvShape<BD, HS>::LB;
+  constexpr int R = PCG_SPMV_ROWS;
+  constexpr int NP = PCG_FUSED_PB;
+  __shared__ double s_dd[R][CH][BD];
+  __shared__ double s_acc[R][64];
+  __shared__ double s_dot[R][4];
+  __shared__ double s_ab[2];
+  __shared__ int s_state;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // launch L-1's partials (FIRST: k_pcg_init's r.s), first of all
+  PcgPartialsN<NP> p_dq, p_sq, p_qw, p_rs;
+  if (wv == 0) {
+    if (FIRST) {
+      p_rs = pcg_part_load_n<NP>(part_prev, n_prev);
+    } else {
+      p_dq = pcg_part_load_n<NP>(part_prev, n_prev);
+      p_sq = pcg_part_load_n<NP>(part_prev + n_prev, n_prev);
+      p_qw = pcg_part_load_n<NP>(part_prev + 2 * static_cast<size_t>(n_prev), n_prev);
+      p_rs = pcg_part_load_n<NP>(part_prev + 3 * static_cast<size_t>(n_prev), n_prev);
+    }
+  }
+  const int row = blockIdx.x * R + wv;
+  const bool act = row < nb;
+  const int rw = act ? row : 0;
+  const int gw = lane / BD, k = lane % BD;
+  // own-row operands (lane kk < BD owns component kk)
+  const int kk = lane < BD ? lane : 0;
+  const size_t o_own = BD * static_cast<size_t>(rw) + kk;
+  const double s_own = s_prev[o_own];
+  const double r_own = r[o_own];
+  const double w_own = FIRST ? 0.0 : w_prev[o_own];
+  const double d_own = FIRST ? 0.0 : d_prev[o_own];
+  const double q_own = FIRST ? 0.0 : q[o_own];
+  const double x_own = FIRST ? 0.0 : x[o_own];
+  double Jr[BD];
+#pragma unroll
+  for (int c = 0; c < BD; ++c) Jr[c] = static_cast<double>(J[static_cast<size_t>(rw) * BD * BD + BD * kk + c]);
+  double Dg[BD];
+  if (D64) {
+#pragma unroll
+    for (int c = 0; c < BD; ++c) Dg[c] = Ddiag[static_cast<size_t>(rw) * BD * BD + BD * kk + c];
+  }
+  const int p0 = F_ptr[rw], p1 = act ? F_ptr[rw + 1] : p0;
+  const int len = p1 - p0;
+  int nc = min(CH, len);
+  double sg[LB][BD], wg[LB][BD], dg[LB][BD];
+  MT v[HS][BD];
+  auto load_chunk = [&](int c0) {
+#pragma unroll
+    for (int lb = 0; lb < LB; ++lb) {
+      const int t = lane + 64 * lb;
+      const size_t j = BD * static_cast<size_t>(F_col[p0 + c0 + (t < nc ? t : 0)]);
+#pragma unroll
+      for (int c = 0; c < BD; ++c) {
+        sg[lb][c] = s_prev[j + c];
+        if (!FIRST) {
+          wg[lb][c] = w_prev[j + c];
+          dg[lb][c] = d_prev[j + c];
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+      const int bk = gw + h * GPW;
+      const bool on = gw < GPW && bk < nc;
+      const MT* V = F_val + static_cast<size_t>(p0 + c0 + (on ? bk : 0)) * BD * BD + BD * k;
+#pragma unroll
+      for (int c = 0; c < BD; ++c) {
+        const MT vv = V[c];
+        v[h][c] = on ? vv : MT(0);
+      }
+    }
+  };
+  load_chunk(0);
+  if (wv == 0) {
+    const int dit = sc->done;
+    const int ab = abort ? *abort : 0;
+    const int err0 = sc->err;
+    const double d0_prev = sc->d0, dn0_prev = sc->dn0;
+    double rs, alpha = 0.0, dn, dq = 1.0;
+    if (FIRST) {
+      rs = pcg_part_sum_n<NP>(p_rs, part_prev, n_prev);
+      dn = rs;
+    } else {
+      dq = pcg_part_sum_n<NP>(p_dq, part_prev, n_prev);
+      const double sq = pcg_part_sum_n<NP>(p_sq, part_prev + n_prev, n_prev);
+      const double qw = pcg_part_sum_n<NP>(p_qw, part_prev + 2 * static_cast<size_t>(n_prev), n_prev);
+      rs = pcg_part_sum_n<NP>(p_rs, part_prev + 3 * static_cast<size_t>(n_prev), n_prev);
+      alpha = rs / dq;
+      dn = (rs - 2.0 * alpha * sq) + alpha * alpha * qw;
+    }
+    const bool first = blockIdx.x == 0 && lane == 0;
+    int state;
+    double beta = 0.0;
+    if (dit != 0 && dit - 1 < it) {
+      state = 0;  // ended at an earlier launch
+    } else if (ab) {
+      state = 0;
+      if (first) {
+        sc->done = it + 1;
+        sc->iters = it;
+        if (prog) progress_done(prog, it, err0, ab, dn, d0_prev, dn0_prev);
+      }
+    } else {
+      double d0;
+      if (FIRST) {
+        if (d0_abs >= 0.0) {
+          d0 = d0_abs;
+        } else {
+          d0 = tol * dn;
+          if (resid_mem > 0.0 && resid_mem > d0) d0 = resid_mem;
+        }
+      } else {
+        d0 = d0_prev;
+      }
+      if (first) {
+        if (FIRST) { sc->d0 = d0; sc->dn0 = dn; }
+        sc->dn[it & 1] = dn;
+        if (!(dn >= 0.0)) sc->err = 1;  // indefinite preconditioner (reported, not acted on)
+        // d.q <= 0: the operator is not positive definite along d (reported
+        // in sc->err; the iteration itself follows linear_solver_pcg.hpp)
+        if (!FIRST && !(dq > 0.0)) sc->err = 1;
+      }
+      if (dn <= d0 || last) {
+        state = 1;  // this launch's update, then the end
+        if (first) {
+          sc->done = it + 1;
+          sc->iters = it;
+          if (prog) progress_done(prog, it, err0 | (dn >= 0.0 ? 0 : 1), 0, dn, d0, FIRST ? dn : dn0_prev);
+        }
+      } else {
+        state = 2;
+        beta = FIRST ? 0.0 : dn / rs;
+        if (first && prog) progress_store(&prog->it, it);
+      }
+    }
+    if (lane == 0) {
+      s_state = state;
+      s_ab[0] = alpha;
+      s_ab[1] = beta;
+    }
+  }
+  __syncthreads();
+  const int state = s_state;
+  if (state == 0) return;
+  const double alpha = s_ab[0], beta = s_ab[1];
+  // this workgroup's rows: the update of launch L-1
+  const bool own = act && lane < BD;
+  double r_new = r_own, s_new = s_own;
+  if (!FIRST) {
+    r_new = fma(-alpha, q_own, r_own);
+    s_new = fma(-alpha, w_own, s_own);
+    if (own) {
+      x[o_own] = fma(alpha, d_own, x_own);
+      r[o_own] = r_new;
+      s_next[o_own] = s_new;
+    }
+  }
+  if (state == 1) return;
+  double acc = 0.0;
+  for (int c0 = 0; c0 < len; c0 += CH) {
+    if (c0 > 0) {
+      nc = min(CH, len - c0);
+      load_chunk(c0);
+      wave_sync();  // the previous chunk's reads of s_dd are done
+    }
+#pragma unroll
+    for (int lb = 0; lb < LB; ++lb) {
+      const int t = lane + 64 * lb;
+      if (t < nc) {
+#pragma unroll
+        for (int c = 0; c < BD; ++c)
+          s_dd[wv][t][c] = FIRST ? sg[lb][c] : fma(beta, dg[lb][c], fma(-alpha, wg[lb][c], sg[lb][c]));
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+      const int bk = gw + h * GPW;
+      const int bc = bk < nc ? bk : 0;
+#pragma unroll
+      for (int c = 0; c < BD; ++c) acc = fma(static_cast<double>(v[h][c]), s_dd[wv][bc][c], acc);
+    }
+  }
+  s_acc[wv][lane] = acc;
+  wave_sync();
+  double vs = 0.0;
+  if (lane < BD)
+    for (int g2 = 0; g2 < GPW; ++g2) vs += s_acc[wv][g2 * BD + lane];
+  const double dd = FIRST ? s_own : fma(beta, d_own, s_new);
+  if (D64) {
+    // every lane of the wavefront takes part in the shuffles
+    double dgv = 0.0;
+#pragma unroll
+    for (int c = 0; c < BD; ++c) dgv = fma(Dg[c], __shfl(dd, c, 64), dgv);
+    vs += dgv;
+  }
+  // w' = M^-1 q' for the row (q' components from lanes 0..BD-1)
+  double wn = 0.0;
+#pragma unroll
+  for (int c = 0; c < BD; ++c) wn = fma(Jr[c], __shfl(vs, c, 64), wn);
+  double t_dq = 0.0, t_sq = 0.0, t_qw = 0.0, t_rs = 0.0;
+  if (own) {
+    q[o_own] = vs;
+    d_next[o_own] = dd;
+    w_next[o_own] = wn;
+    t_dq = dd * vs;
+    t_sq = s_new * vs;
+    t_qw = vs * wn;
+    t_rs = r_new * s_new;
+  }
+  t_dq = wave_total(t_dq);
+  t_sq = wave_total(t_sq);
+  t_qw = wave_total(t_qw);
+  t_rs = wave_total(t_rs);
+  if (lane == 0) {
+    s_dot[wv][0] = t_dq;
+    s_dot[wv][1] = t_sq;
+    s_dot[wv][2] = t_qw;
+    s_dot[wv][3] = t_rs;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int w = 0; w < R; ++w) t += s_dot[w][threadIdx.x];
+    part_next[static_cast<size_t>(threadIdx.x) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
 // a = dn / d.q; x += a d; r -= a q; s = M^-1 r; part_rs[block] = r . s
 template <typename MT, int BD>
 __global__ __launch_bounds__(PCG_ROW_THREADS) void k_pcg_update(
@@ -2255,6 +2539,52 @@ G2OHIP_PCG_INST(double, 6)
 G2OHIP_PCG_INST(double, 3)
 G2OHIP_PCG_INST(double, 9)
 #undef G2OHIP_PCG_INST
+
+template <typename MT, int BD>
+hipError_t launch_pcg_fused(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* Ddiag,
+                            const MT* J, double* x, double* r, double* q, double* const s[2], double* const w[2],
+                            double* const d[2], const double* part_init, double* part, PcgScalars* sc,
+                            const PcgControl& ctl, int it, int last, hipStream_t st, long long nnz) {
+  const int G = pcg_spmv_blocks(nb);
+  const double* s_prev = it == 0 ? s[0] : s[(it + 1) & 1];
+  const double* w_prev = w[(it + 1) & 1];
+  const double* d_prev = d[(it + 1) & 1];
+  const double* p_prev = it == 0 ? part_init : part + 4 * static_cast<size_t>(G) * ((it + 1) & 1);
+  const int n_prev = it == 0 ? pcg_row_blocks(nb, BD) : G;
+  double* p_next = part + 4 * static_cast<size_t>(G) * (it & 1);
+  constexpr int CH_LONG = sizeof(MT) == 4 ? 128 : 64;
+  constexpr int HS_LONG = BD <= 8 ? (CH_LONG + 64 / BD - 1) / (64 / BD) : 5;
+  const bool short_rows = nnz >= 0 && nnz < static_cast<long long>(PCG_SHORT_ROW) * nb;
+#define G2OHIP_FUSED_LAUNCH(FIRST, D64, HS)                                                                       \
+  hipLaunchKernelGGL((k_pcg_fused<MT, BD, FIRST, D64, HS>), dim3(G), dim3(PCG_SPMV_THREADS), 0, st, nb, F_ptr,     \
+                     F_col, F_val, Ddiag, J, x, r, q, s_prev, s[it & 1], w_prev, w[it & 1], d_prev, d[it & 1],     \
+                     p_prev, n_prev, p_next, sc, ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last)
+#define G2OHIP_FUSED_SHAPE(HS)                        \
+  if (Ddiag) {                                        \
+    if (it == 0) G2OHIP_FUSED_LAUNCH(true, true, HS);  \
+    else G2OHIP_FUSED_LAUNCH(false, true, HS);         \
+  } else {                                            \
+    if (it == 0) G2OHIP_FUSED_LAUNCH(true, false, HS); \
+    else G2OHIP_FUSED_LAUNCH(false, false, HS);        \
+  }
+  if (short_rows) {
+    G2OHIP_FUSED_SHAPE(1)
+  } else {
+    G2OHIP_FUSED_SHAPE(HS_LONG)
+  }
+#undef G2OHIP_FUSED_SHAPE
+#undef G2OHIP_FUSED_LAUNCH
+  return hipGetLastError();
+}
+template hipError_t launch_pcg_fused<float, 6>(int, const int*, const int*, const float*, const double*, const float*,
+                                               double*, double*, double*, double* const[2], double* const[2],
+                                               double* const[2], const double*, double*, PcgScalars*,
+                                               const PcgControl&, int, int, hipStream_t, long long);
+template hipError_t launch_pcg_fused<double, 6>(int, const int*, const int*, const double*, const double*,
+                                                const double*, double*, double*, double*, double* const[2],
+                                                double* const[2], double* const[2], const double*, double*,
+                                                PcgScalars*, const PcgControl&, int, int, hipStream_t, long long);
+int pcg_fused_blocks(int nb) { return pcg_spmv_blocks(nb); }
 
 // y = A x over 6x6 blocks: block-CSR (F_ptr, F_col, F_val) or, with F_ptr
 // null, block diagonal (block i at F_val + 36 i); one thread per row of y.
