@@ -62,12 +62,10 @@ def stage_bytes(info, C):
         # K4+K5: Hpl + its camera index, Hll and b_l, Hpp and b_p in; Hschur (FP32 off-diagonal
         # upper blocks, FP64 diagonal), bschur and the block-Jacobi inverses out
         "schur": nq * (72 + 4) + Pf * (48 + 24) + npos * 4 + Cf * (288 + 48) + nO * (144 + 4) + Cf * (288 + 48 + 144),
-        # K7, one iteration (SpMV + update): each stored block once, its column index, the FP64
-        # diagonal, the preconditioner, and the vectors (SpMV: s, d_old in, d, q out; update:
-        # d, q, r, x in, x, r, s out)
-        "pcg_iteration": nO * (144 + 4) + Cf * (288 + 4) + Cf * 144 + 11 * 48 * Cf,
-        # K7 SpMV launch alone
-        "pcg_spmv": nO * (144 + 4) + Cf * (288 + 4) + 4 * 48 * Cf,
+        # K7, one iteration = one launch of k_pcg_fused: each stored block once, its column index,
+        # the FP64 diagonal, the preconditioner, and the vectors (s, w, d and x, r, q in; x, r, q
+        # and s, w, d out)
+        "pcg_iteration": nO * (144 + 4) + Cf * (288 + 4) + Cf * 144 + 12 * 48 * Cf,
         # K9 / K10: FP64 residual of the reduced system from the state, back-substitution x_l
         "refine": obs_in + cam_in + npos * (4 + 24) + Pf * 24 + Cf * 96,
         # K1: new chi2
@@ -75,12 +73,12 @@ def stage_bytes(info, C):
     }
 
 
-SPMV_SYMBOL = "k_pcg_dir_spmv<float, 6, false, true, 13>"
+PCG_SYMBOL = "k_pcg_fused<float, 6, false, true, 13>"
 STAGE_KERNELS = {
     "linearize": ["k_linearize<float>"],
     "schur": ["k_point_linv", "k_schur_win<float, 512>", "k_schur_offdiag<float>", "k_schur_diag_pm",
               "k_schur_diag_finish<float>"],
-    "pcg_iteration": [SPMV_SYMBOL, "k_pcg_update<float, 6>"],
+    "pcg_iteration": [PCG_SYMBOL],
     "refine": ["k_refine"],
     "chi2": ["k_chi2"],
 }
@@ -445,14 +443,15 @@ def main():
                      "kernels": STAGE_KERNELS[k]}
     stages["pcg_iteration"]["time_note"] = ("HIP events around each whole PCG solve (launches and the gaps "
                                             "between them) over its iterations")
-    # roofline: the kernel with the largest total time, the direction + SpMV
-    # launch of the PCG, timed back to back between HIP events on its stream
-    us_spmv, us_iter = rep
-    ach = alg["pcg_spmv"] / (us_spmv * 1e-6)
-    roof = {"bound": "hbm", "kernel": SPMV_SYMBOL, "achieved": round(ach / 1e9, 1), "peak": PEAK_HBM / 1e9,
-            "unit": "GB/s", "frac": round(ach / PEAK_HBM, 4), "traffic": traffic.get(SPMV_SYMBOL),
-            "bytes_per_launch": int(alg["pcg_spmv"]), "avg_launch_us": round(us_spmv, 2),
-            "avg_launch_us_trace": trace_us.get(SPMV_SYMBOL),
+    # roofline: the kernel with the largest total time, the PCG iteration
+    # (one launch of k_pcg_fused), timed back to back between HIP events on
+    # its stream
+    us_launch, us_iter = rep
+    ach = alg["pcg_iteration"] / (us_launch * 1e-6)
+    roof = {"bound": "hbm", "kernel": PCG_SYMBOL, "achieved": round(ach / 1e9, 1), "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s", "frac": round(ach / PEAK_HBM, 4), "traffic": traffic.get(PCG_SYMBOL),
+            "bytes_per_launch": int(alg["pcg_iteration"]), "avg_launch_us": round(us_launch, 2),
+            "avg_launch_us_trace": trace_us.get(PCG_SYMBOL),
             "launches_in_timed_run": iters_pcg, "total_ms_profiled_run": round(prof32["pcg_spmv"][0], 3),
             "note": "avg_launch_us: 200 back-to-back launches between HIP events (g2ohip_replay_pcg), "
                     "including the dispatch gap; avg_launch_us_trace: rocprofv3 kernel trace"}

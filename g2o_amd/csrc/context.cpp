@@ -1200,7 +1200,8 @@ int g2ohip_ctx::marginals(int nblk, const int* rows, const int* cols, double* ou
 // Kernel time of the PCG iteration on the last solve's operator: `reps`
 // direction + SpMV launches back to back between two HIP events on the
 // context's stream (us_spmv), then `reps` iterations of both launches
-// (us_iter). The launches continue the last solve's recurrences on its
+// (us_iter); with the one-launch iteration, `reps` launches of k_pcg_fused
+// (both figures). The launches continue the last solve's recurrences on its
 // buffers with a stop threshold that never triggers, so every launch does a
 // full iteration's work; the buffers are scratch once the solution has been
 // copied out, and the next solve re-initialises them.
@@ -1222,6 +1223,23 @@ int g2ohip_ctx::replay_pcg(int reps, double* us_spmv, double* us_iter) {
     HK(hipMemcpyAsync(sc.p, &init, sizeof(init), hipMemcpyHostToDevice, stream));
     HK(hipStreamSynchronize(stream));
     using MT = typename std::remove_reference<decltype(*B.F_val.p)>::type;
+    if (pcg_fused) {
+      // one launch per iteration: both figures are the launch of k_pcg_fused
+      double* const s2[2] = {B.s.p, B.s2.p};
+      double* const w2[2] = {B.w.p, B.w2.p};
+      double* const dirs[2] = {B.d.p, B.d2.p};
+      HK(hipEventRecord(e[0], stream));
+      for (int k = 0; k < reps; ++k)
+        HK((launch_pcg_fused<MT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, diag, B.Jinv.p, B.x.p, B.r.p, B.q.p, s2,
+                                    w2, dirs, d_dn_part.p, d_fpart.p, sc.p, ctl, 1 + k, 0, stream, st.nF)));
+      HK(hipEventRecord(e[1], stream));
+      HK(hipEventSynchronize(e[1]));
+      float a = 0;
+      HK(hipEventElapsedTime(&a, e[0], e[1]));
+      if (us_spmv) *us_spmv = 1e3 * a / reps;
+      if (us_iter) *us_iter = 1e3 * a / reps;
+      return G2OHIP_OK;
+    }
     HK(hipEventRecord(e[0], stream));
     for (int k = 0; k < reps; ++k)
       HK((launch_pcg_dir_spmv<MT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, B.s.p, B.d2.p, B.d.p, B.q.p, d_dn_part.p,

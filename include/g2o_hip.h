@@ -263,7 +263,8 @@ enum g2ohip_kernel {
   G2OHIP_K_CHI2 = 4,
   G2OHIP_K_UPDATE = 5,
   G2OHIP_K_SCHUR_WINDOWS = 6, /* the off-diagonal Schur window kernel alone (within G2OHIP_K_SCHUR) */
-  G2OHIP_K_PCG_SPMV = 7,      /* the explicit-operator direction + SpMV kernel alone (within G2OHIP_K_PCG) */
+  G2OHIP_K_PCG_SPMV = 7,      /* the explicit-operator iteration launch (one-launch iteration) or its
+                                 direction + SpMV kernel alone (two launches), within G2OHIP_K_PCG */
   G2OHIP_K_COUNT = 8
 };
 int g2ohip_set_profiling(g2ohip_ctx* ctx, int on);
@@ -271,9 +272,10 @@ int g2ohip_set_profiling(g2ohip_ctx* ctx, int on);
  * bundle-adjustment solve: `reps` direction + SpMV launches back to back
  * between two HIP events on the context's stream give *us_spmv (microseconds
  * per launch); `reps` whole iterations (SpMV + update launches) give
- * *us_iteration. The launches continue that solve's recurrences on its
- * scratch vectors (the solution x was already copied out; the next solve
- * starts afresh). */
+ * *us_iteration. With the one-launch iteration (the default; not with
+ * G2OHIP_PCG_FUSED=0) both are the time of one such launch. The launches
+ * continue that solve's recurrences on its scratch vectors (the solution x
+ * was already copied out; the next solve starts afresh). */
 int g2ohip_replay_pcg(g2ohip_ctx* ctx, int reps, double* us_spmv, double* us_iteration);
 int g2ohip_get_profile(g2ohip_ctx* ctx, int kernel, double* total_ms, int* launches);
 
