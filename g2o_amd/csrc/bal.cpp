@@ -140,6 +140,7 @@ int g2ohip_ctx::bal_set_structure(int C, int P, int E, const int* ec, const int*
   HK(d_sc.alloc(1));
   HK(d_dq_part.alloc(std::max(pcg_spmv_blocks(st.Cf), 1)));
   HK(d_dn_part.alloc(std::max(pcg_row_blocks(st.Cf, 9), 1)));
+  HK(d_fpart.alloc(8 * static_cast<size_t>(pcg_fused_blocks(std::max(st.Cf, 1)))));
   HK(hipMemsetAsync(d_b.p, 0, nx * sizeof(double), stream));
   HK(hipMemsetAsync(d_x.p, 0, nx * sizeof(double), stream));
   bal = DevBAL{};
@@ -295,9 +296,25 @@ int g2ohip_ctx::bal_solve(int* ok) {
   {
     ProfScope ps(this, G2OHIP_K_PCG);
     bool prog_failed = false;
-    HK((pcg_drive<double, 9>(stream, h_pinned, progress(), nb, d_bschur.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs,
-                             B.q.p, d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, &pr,
-                             &prog_failed)));
+    if (pcg_fused && pcg_fused_fits(nb)) {
+      // one launch per iteration
+      HK(B.s2.alloc(n)); HK(B.w.alloc(n)); HK(B.w2.alloc(n));
+      double* const s2[2] = {B.s.p, B.s2.p};
+      double* const w2[2] = {B.w.p, B.w2.p};
+      auto init = [&]() {
+        return launch_pcg_init<double, 9>(nb, d_bschur.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, d_dn_part.p, d_sc.p, stream);
+      };
+      auto step = [&](int it, int last) {
+        return launch_pcg_fused<double, 9>(nb, d_F_ptr.p, d_F_col.p, d_bal_F_val.p, nullptr, B.Jinv.p, B.x.p, B.r.p,
+                                           B.q.p, s2, w2, dirs, d_dn_part.p, d_fpart.p, d_sc.p, ctl, it, last,
+                                           stream, st.nF);
+      };
+      HK(pcg_loop(stream, h_pinned, progress(), d_sc.p, max_iter, init, step, &pr, &prog_failed, ctl.abort));
+    } else {
+      HK((pcg_drive<double, 9>(stream, h_pinned, progress(), nb, d_bschur.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs,
+                               B.q.p, d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, &pr,
+                               &prog_failed)));
+    }
     if (prog_failed) prog_ok = false;
   }
   pcg_residual_mem = 0.5 * pr.dn_final;

@@ -581,7 +581,7 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
     HK((pcg_drive<HT, 6>(stream, h_pinned, progress(), nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
                          d_dn_part.p, d_dq_part.p, implicit_reduce_blocks(nb), d_sc.p, max_iter, op, out,
                          &prog_failed, ctl.abort)));
-  } else if (pcg_fused) {
+  } else if (pcg_fused && pcg_fused_fits(nb)) {
     HK(B.s2.alloc(n)); HK(B.w.alloc(n)); HK(B.w2.alloc(n));
     double* const s2[2] = {B.s.p, B.s2.p};
     double* const w2[2] = {B.w.p, B.w2.p};
@@ -1223,7 +1223,7 @@ int g2ohip_ctx::replay_pcg(int reps, double* us_spmv, double* us_iter) {
     HK(hipMemcpyAsync(sc.p, &init, sizeof(init), hipMemcpyHostToDevice, stream));
     HK(hipStreamSynchronize(stream));
     using MT = typename std::remove_reference<decltype(*B.F_val.p)>::type;
-    if (pcg_fused) {
+    if (pcg_fused && pcg_fused_fits(nb)) {
       // one launch per iteration: both figures are the launch of k_pcg_fused
       double* const s2[2] = {B.s.p, B.s2.p};
       double* const w2[2] = {B.w.p, B.w2.p};

@@ -269,6 +269,8 @@ struct g2ohip_ctx {
   int pg_chi2(double* chi2, double* err_out);
   int pg_build();
   int pg_solve(int* ok);
+  hipError_t pg_pcg_fused(const double* rhs, const g2ohip::PcgControl& ctl, int max_iter, g2ohip::PcgRun* out,
+                          bool* prog_failed);
   int pg_marginals(int nblk, const int* rows, const int* cols, double* out, double tol);
   int marginals(int nblk, const int* rows, const int* cols, double* out, double tol);
   int pg_update();

@@ -241,6 +241,10 @@ hipError_t launch_pcg_fused(int nb, const int* F_ptr, const int* F_col, const MT
                             double* const d[2], const double* part_init, double* part, PcgScalars* sc,
                             const PcgControl& ctl, int it, int last, hipStream_t st, long long nnz);
 int pcg_fused_blocks(int nb);
+// Every workgroup of a launch sums all partials of the previous one, so the
+// one-launch iteration pays off while the grid is small: up to 256
+// workgroups (2,048 block rows); larger systems take the two launches.
+bool pcg_fused_fits(int nb);
 template <int BD>
 hipError_t launch_pcg_dir(int nb, const double* s, const double* d_old, double* d, const double* part_rs,
                           PcgScalars* sc, const PcgControl& ctl, int it, int last, hipStream_t st);

@@ -2157,7 +2157,7 @@ __device__ double pcg_part_sum_n(const PcgPartialsN<NB>& p, const double* __rest
   for (int i = lane + 64 * NB; i < n; i += 64) neumaier_add(s, c, part[i]);
   return wave_total_compensated(s, c);
 }
-constexpr int PCG_FUSED_PB = 3;  // partial batches per lane loaded ahead (192 workgroups)
+constexpr int PCG_FUSED_PB = 4;  // partial batches per lane loaded ahead (256 workgroups)
 
 template <typename MT, int BD, bool FIRST, bool D64, int HS>
 __global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_fused(
@@ -2584,7 +2584,12 @@ template hipError_t launch_pcg_fused<double, 6>(int, const int*, const int*, con
                                                 const double*, double*, double*, double*, double* const[2],
                                                 double* const[2], double* const[2], const double*, double*,
                                                 PcgScalars*, const PcgControl&, int, int, hipStream_t, long long);
+template hipError_t launch_pcg_fused<double, 9>(int, const int*, const int*, const double*, const double*,
+                                                const double*, double*, double*, double*, double* const[2],
+                                                double* const[2], double* const[2], const double*, double*,
+                                                PcgScalars*, const PcgControl&, int, int, hipStream_t, long long);
 int pcg_fused_blocks(int nb) { return pcg_spmv_blocks(nb); }
+bool pcg_fused_fits(int nb) { return pcg_fused_blocks(nb) <= 64 * PCG_FUSED_PB; }
 
 // y = A x over 6x6 blocks: block-CSR (F_ptr, F_col, F_val) or, with F_ptr
 // null, block diagonal (block i at F_val + 36 i); one thread per row of y.

@@ -52,6 +52,7 @@ int g2ohip_ctx::pg_set_structure(int N, int E, const int* from, const int* to, c
   HK(d_scal.alloc(8));
   HK(d_dq_part.alloc(std::max(pcg_spmv_blocks(pgst.Nf), 1)));
   HK(d_dn_part.alloc(std::max(pcg_row_blocks(pgst.Nf), 1)));
+  HK(d_fpart.alloc(8 * static_cast<size_t>(pcg_fused_blocks(std::max(pgst.Nf, 1)))));
   HK(d_err.alloc(4));
   HK(d_sc.alloc(1));
   pg = DevPG{};
@@ -155,6 +156,30 @@ int g2ohip_ctx::pg_lambda_init(double* out) {
   return G2OHIP_OK;
 }
 
+// One PCG solve on the pose graph's full block rows (B.Jinv current), one
+// launch per iteration.
+hipError_t g2ohip_ctx::pg_pcg_fused(const double* rhs, const PcgControl& ctl, int max_iter, PcgRun* out,
+                                    bool* prog_failed) {
+  const int nb = pgst.Nf;
+  const size_t n = 6 * static_cast<size_t>(nb);
+  PrecBufs<double>& B = f64;
+  hipError_t e;
+  if ((e = B.s2.alloc(n)) != hipSuccess || (e = B.w.alloc(n)) != hipSuccess || (e = B.w2.alloc(n)) != hipSuccess)
+    return e;
+  double* const s2[2] = {B.s.p, B.s2.p};
+  double* const w2[2] = {B.w.p, B.w2.p};
+  double* const dirs[2] = {B.d.p, B.d2.p};
+  auto init = [&]() {
+    return launch_pcg_init<double, 6>(nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, d_dn_part.p, d_sc.p, stream);
+  };
+  auto step = [&](int it, int last) {
+    return launch_pcg_fused<double, 6>(nb, d_pg_F_ptr.p, d_pg_F_col.p, d_pg_F_val.p, nullptr, B.Jinv.p, B.x.p, B.r.p,
+                                       B.q.p, s2, w2, dirs, d_dn_part.p, d_fpart.p, d_sc.p, ctl, it, last, stream,
+                                       pgst.nF);
+  };
+  return pcg_loop(stream, h_pinned, progress(), d_sc.p, max_iter, init, step, out, prog_failed, ctl.abort);
+}
+
 // (H + lambda I) x = b by block-Jacobi PCG on the full block rows
 // (linear_solver_pcg.hpp:87-205, with the residual memory).
 int g2ohip_ctx::pg_solve(int* ok) {
@@ -182,9 +207,13 @@ int g2ohip_ctx::pg_solve(int* ok) {
   {
     ProfScope ps(this, G2OHIP_K_PCG);
     bool prog_failed = false;
-    HK((pcg_drive<double, 6>(stream, h_pinned, progress(), nb, d_b.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
-                             d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, &pr,
-                             &prog_failed)));
+    if (pcg_fused && pcg_fused_fits(nb)) {
+      HK(pg_pcg_fused(d_b.p, ctl, max_iter, &pr, &prog_failed));
+    } else {
+      HK((pcg_drive<double, 6>(stream, h_pinned, progress(), nb, d_b.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
+                               d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, max_iter, op, &pr,
+                               &prog_failed)));
+    }
     if (prog_failed) prog_ok = false;
   }
   pcg_residual_mem = 0.5 * pr.dn_final;
@@ -236,8 +265,13 @@ int g2ohip_ctx::pg_marginals(int nblk, const int* rows, const int* cols, double*
       {
         ProfScope ps(this, G2OHIP_K_PCG);
         bool prog_failed = false;
-        HK((pcg_drive<double, 6>(stream, h_pinned, progress(), nb, rhs.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs, B.q.p,
-                                 d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, n, op, &pr, &prog_failed)));
+        if (pcg_fused && pcg_fused_fits(nb)) {
+          HK(pg_pcg_fused(rhs.p, ctl, n, &pr, &prog_failed));
+        } else {
+          HK((pcg_drive<double, 6>(stream, h_pinned, progress(), nb, rhs.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, dirs,
+                                   B.q.p, d_dn_part.p, d_dq_part.p, pcg_spmv_blocks(nb), d_sc.p, n, op, &pr,
+                                   &prog_failed)));
+        }
         if (prog_failed) prog_ok = false;
       }
       HK(hipMemcpyAsync(x.data(), B.x.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));

@@ -272,8 +272,8 @@ int g2ohip_set_profiling(g2ohip_ctx* ctx, int on);
  * bundle-adjustment solve: `reps` direction + SpMV launches back to back
  * between two HIP events on the context's stream give *us_spmv (microseconds
  * per launch); `reps` whole iterations (SpMV + update launches) give
- * *us_iteration. With the one-launch iteration (the default; not with
- * G2OHIP_PCG_FUSED=0) both are the time of one such launch. The launches
+ * *us_iteration. With the one-launch iteration (the default up to 2,048
+ * cameras; not with G2OHIP_PCG_FUSED=0) both are the time of one such launch. The launches
  * continue that solve's recurrences on its scratch vectors (the solution x
  * was already copied out; the next solve starts afresh). */
 int g2ohip_replay_pcg(g2ohip_ctx* ctx, int reps, double* us_spmv, double* us_iteration);
