@@ -73,7 +73,7 @@ def stage_bytes(info, C):
     }
 
 
-PCG_SYMBOL = "k_pcg_fused<float, 6, false, true, 13>"
+PCG_SYMBOL = "k_pcg_fused<float, 6, false, true, 11>"
 STAGE_KERNELS = {
     "linearize": ["k_linearize<float>"],
     "schur": ["k_point_linv", "k_schur_win<float, 512>", "k_schur_offdiag<float>", "k_schur_diag_pm",

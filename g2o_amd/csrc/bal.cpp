@@ -117,6 +117,8 @@ int g2ohip_ctx::bal_set_structure(int C, int P, int E, const int* ec, const int*
   HK(d_bal_od_pair.upload(pairs, stream));
   HK(d_F_ptr.upload(st.F_ptr, stream));
   HK(d_F_col.upload(st.F_col, stream));
+  st.F_head = row_heads(st.F_ptr, st.F_col);
+  HK(d_F_head.upload(st.F_head, stream));
   HK(d_F_diag.upload(fdiag, stream));
   HK(d_up_pos.upload(st.up_pos, stream));
   HK(d_up_posT.upload(st.up_posT, stream));
@@ -305,7 +307,7 @@ int g2ohip_ctx::bal_solve(int* ok) {
         return launch_pcg_init<double, 9>(nb, d_bschur.p, B.Jinv.p, B.x.p, B.r.p, B.s.p, d_dn_part.p, d_sc.p, stream);
       };
       auto step = [&](int it, int last) {
-        return launch_pcg_fused<double, 9>(nb, d_F_ptr.p, d_F_col.p, d_bal_F_val.p, nullptr, B.Jinv.p, B.x.p, B.r.p,
+        return launch_pcg_fused<double, 9>(nb, d_F_head.p, d_F_col.p, d_bal_F_val.p, nullptr, B.Jinv.p, B.x.p, B.r.p,
                                            B.q.p, s2, w2, dirs, d_dn_part.p, d_fpart.p, d_sc.p, ctl, it, last,
                                            stream, st.nF);
       };

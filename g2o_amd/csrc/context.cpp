@@ -220,6 +220,8 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
   HK(d_up_posT.upload(st.up_posT, stream));
   HK(d_F_ptr.upload(st.F_ptr, stream));
   HK(d_F_col.upload(st.F_col, stream));
+  st.F_head = row_heads(st.F_ptr, st.F_col);
+  HK(d_F_head.upload(st.F_head, stream));
   std::vector<int> fdiag(st.Cf), iota(st.Cf);
   for (int c = 0; c < st.Cf; ++c) { fdiag[c] = st.up_pos[st.S_diag[c]]; iota[c] = c; }
   HK(d_F_diag.upload(fdiag, stream));
@@ -589,7 +591,7 @@ int g2ohip_ctx::run_pcg(PrecBufs<HT>& B, const double* rhs, const PcgControl& ct
                                                       stream); };
     auto step = [&](int it, int last) -> hipError_t {
       const int pb = prof_begin();
-      const hipError_t e = launch_pcg_fused<HT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p,
+      const hipError_t e = launch_pcg_fused<HT, 6>(nb, d_F_head.p, d_F_col.p, B.F_val.p,
                                                    std::is_same<HT, float>::value ? d_Sdiag.p : nullptr, B.Jinv.p,
                                                    B.x.p, B.r.p, B.q.p, s2, w2, dirs, d_dn_part.p, d_fpart.p,
                                                    d_sc.p, ctl, it, last, stream, st.nF);
@@ -1230,7 +1232,7 @@ int g2ohip_ctx::replay_pcg(int reps, double* us_spmv, double* us_iter) {
       double* const dirs[2] = {B.d.p, B.d2.p};
       HK(hipEventRecord(e[0], stream));
       for (int k = 0; k < reps; ++k)
-        HK((launch_pcg_fused<MT, 6>(nb, d_F_ptr.p, d_F_col.p, B.F_val.p, diag, B.Jinv.p, B.x.p, B.r.p, B.q.p, s2,
+        HK((launch_pcg_fused<MT, 6>(nb, d_F_head.p, d_F_col.p, B.F_val.p, diag, B.Jinv.p, B.x.p, B.r.p, B.q.p, s2,
                                     w2, dirs, d_dn_part.p, d_fpart.p, sc.p, ctl, 1 + k, 0, stream, st.nF)));
       HK(hipEventRecord(e[1], stream));
       HK(hipEventSynchronize(e[1]));

@@ -141,6 +141,7 @@ struct g2ohip_ctx {
       d_cam_part, d_res, d_xp, d_err64;
   g2ohip::DBuf<double> d_dq_part, d_dn_part;
   g2ohip::DBuf<double> d_fpart;  // one-launch PCG partials: [2][4][pcg_fused_blocks(Cf)]
+  g2ohip::DBuf<int> d_F_head;    // row heads of the reduced system (row_heads)
   g2ohip::DBuf<int> d_err;
   // host-mapped PCG progress record (PcgControl::prog); prog_ok drops to
   // false if it ever fails to reach the host, and the PCG then polls the
@@ -166,7 +167,8 @@ struct g2ohip_ctx {
   g2ohip::PGStructure pgst;
   g2ohip::DevPG pg{};
   g2ohip::DBuf<int> d_pg_H, d_pg_act, d_pg_from, d_pg_to, d_pg_swap, d_pg_end_ptr, d_pg_end_ref, d_pg_blk_ptr,
-      d_pg_blk_ref, d_pg_up_pos, d_pg_up_posT, d_pg_F_diag, d_pg_F_ptr, d_pg_F_col;
+      d_pg_blk_ref, d_pg_up_pos, d_pg_up_posT, d_pg_F_diag, d_pg_F_ptr, d_pg_F_col,
+      d_pg_F_head;
   g2ohip::DBuf<double> d_pg_meas, d_pg_info, d_pg_rec, d_pg_F_val, d_pg_Hdiag, d_pg_part;
 
   // BAL (9-DoF cameras, bal.cpp): the structure is `st` with the lists below

@@ -233,10 +233,16 @@ hipError_t launch_pcg_dir_spmv(int nb, const int* F_ptr, const int* F_col, const
                                const double* Ddiag = nullptr, long long nnz = -1);
 // One launch per PCG iteration on the explicit operator (k_pcg_fused): the
 // update of iteration it-1 and the direction + product of iteration it.
+// F_head: the row heads (row_heads in structure.h).
 // s, w, d: two buffers each (launch it writes index it & 1); part_init:
 // k_pcg_init's partials; part: 2 x 4 x pcg_fused_blocks(nb) doubles.
+// Row heads of a block-row matrix for k_pcg_fused: per row, PCG_HEAD_STRIDE
+// ints = first block, length, then the first PCG_HEAD_COLS column indices
+// (padded with the row's own index).
+constexpr int PCG_HEAD_COLS = 128;
+constexpr int PCG_HEAD_STRIDE = PCG_HEAD_COLS + 2;
 template <typename MT, int BD>
-hipError_t launch_pcg_fused(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* Ddiag,
+hipError_t launch_pcg_fused(int nb, const int* F_head, const int* F_col, const MT* F_val, const double* Ddiag,
                             const MT* J, double* x, double* r, double* q, double* const s[2], double* const w[2],
                             double* const d[2], const double* part_init, double* part, PcgScalars* sc,
                             const PcgControl& ctl, int it, int last, hipStream_t st, long long nnz);

@@ -2161,7 +2161,7 @@ constexpr int PCG_FUSED_PB = 4;  // partial batches per lane loaded ahead (256 w
 
 template <typename MT, int BD, bool FIRST, bool D64, int HS>
 __global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_fused(
-    int nb, const int* __restrict__ F_ptr, const int* __restrict__ F_col, const MT* __restrict__ F_val,
+    int nb, const int* __restrict__ F_head, const int* __restrict__ F_col, const MT* __restrict__ F_val,
     const double* __restrict__ Ddiag, const MT* __restrict__ J, double* __restrict__ x, double* __restrict__ r,
     double* __restrict__ q, const double* __restrict__ s_prev, double* __restrict__ s_next,
     const double* __restrict__ w_prev, double* __restrict__ w_next, const double* __restrict__ d_prev,
@@ -2212,8 +2212,14 @@ __global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_fused(
 #pragma unroll
     for (int c = 0; c < BD; ++c) Dg[c] = Ddiag[static_cast<size_t>(rw) * BD * BD + BD * kk + c];
   }
-  const int p0 = F_ptr[rw], p1 = act ? F_ptr[rw + 1] : p0;
-  const int len = p1 - p0;
+  // the row's head (first block, length, first columns) in one round trip:
+  // the matrix entries and the gathers both depend only on it
+  const int* hd = F_head + static_cast<size_t>(rw) * PCG_HEAD_STRIDE;
+  const int p0 = hd[0];
+  const int len = act ? hd[1] : 0;
+  int col0[LB];
+#pragma unroll
+  for (int lb = 0; lb < LB; ++lb) col0[lb] = hd[2 + min(lane + 64 * lb, PCG_HEAD_COLS - 1)];
   int nc = min(CH, len);
   double sg[LB][BD], wg[LB][BD], dg[LB][BD];
   MT v[HS][BD];
@@ -2221,7 +2227,7 @@ __global__ __launch_bounds__(PCG_SPMV_THREADS) void k_pcg_fused(
 #pragma unroll
     for (int lb = 0; lb < LB; ++lb) {
       const int t = lane + 64 * lb;
-      const size_t j = BD * static_cast<size_t>(F_col[p0 + c0 + (t < nc ? t : 0)]);
+      const size_t j = BD * static_cast<size_t>(c0 == 0 ? col0[lb] : F_col[p0 + c0 + (t < nc ? t : 0)]);
 #pragma unroll
       for (int c = 0; c < BD; ++c) {
         sg[lb][c] = s_prev[j + c];
@@ -2541,7 +2547,7 @@ G2OHIP_PCG_INST(double, 9)
 #undef G2OHIP_PCG_INST
 
 template <typename MT, int BD>
-hipError_t launch_pcg_fused(int nb, const int* F_ptr, const int* F_col, const MT* F_val, const double* Ddiag,
+hipError_t launch_pcg_fused(int nb, const int* F_head, const int* F_col, const MT* F_val, const double* Ddiag,
                             const MT* J, double* x, double* r, double* q, double* const s[2], double* const w[2],
                             double* const d[2], const double* part_init, double* part, PcgScalars* sc,
                             const PcgControl& ctl, int it, int last, hipStream_t st, long long nnz) {
@@ -2552,20 +2558,26 @@ hipError_t launch_pcg_fused(int nb, const int* F_ptr, const int* F_col, const MT
   const double* p_prev = it == 0 ? part_init : part + 4 * static_cast<size_t>(G) * ((it + 1) & 1);
   const int n_prev = it == 0 ? pcg_row_blocks(nb, BD) : G;
   double* p_next = part + 4 * static_cast<size_t>(G) * (it & 1);
-  constexpr int CH_LONG = sizeof(MT) == 4 ? 128 : 64;
+  // FP32 rows in chunks of 110 blocks (11 block slots per lane group): a
+  // bundle-adjustment row of up to 110 blocks in one round trip without
+  // spilling the extra gathers of this kernel
+  constexpr int CH_LONG = sizeof(MT) == 4 ? 110 : 64;
   constexpr int HS_LONG = BD <= 8 ? (CH_LONG + 64 / BD - 1) / (64 / BD) : 5;
   const bool short_rows = nnz >= 0 && nnz < static_cast<long long>(PCG_SHORT_ROW) * nb;
 #define G2OHIP_FUSED_LAUNCH(FIRST, D64, HS)                                                                       \
-  hipLaunchKernelGGL((k_pcg_fused<MT, BD, FIRST, D64, HS>), dim3(G), dim3(PCG_SPMV_THREADS), 0, st, nb, F_ptr,     \
+  hipLaunchKernelGGL((k_pcg_fused<MT, BD, FIRST, D64, HS>), dim3(G), dim3(PCG_SPMV_THREADS), 0, st, nb, F_head,    \
                      F_col, F_val, Ddiag, J, x, r, q, s_prev, s[it & 1], w_prev, w[it & 1], d_prev, d[it & 1],     \
                      p_prev, n_prev, p_next, sc, ctl.prog, ctl.abort, ctl.tol, ctl.resid_mem, ctl.d0_abs, it, last)
-#define G2OHIP_FUSED_SHAPE(HS)                        \
-  if (Ddiag) {                                        \
-    if (it == 0) G2OHIP_FUSED_LAUNCH(true, true, HS);  \
-    else G2OHIP_FUSED_LAUNCH(false, true, HS);         \
-  } else {                                            \
-    if (it == 0) G2OHIP_FUSED_LAUNCH(true, false, HS); \
-    else G2OHIP_FUSED_LAUNCH(false, false, HS);        \
+// the FP64 diagonal beside the matrix only for FP32 blocks
+#define G2OHIP_FUSED_SHAPE(HS)                          \
+  if constexpr (sizeof(MT) == 4) {                      \
+    if (!Ddiag) return hipErrorInvalidValue;            \
+    if (it == 0) G2OHIP_FUSED_LAUNCH(true, true, HS);   \
+    else G2OHIP_FUSED_LAUNCH(false, true, HS);          \
+  } else {                                              \
+    if (Ddiag) return hipErrorInvalidValue;             \
+    if (it == 0) G2OHIP_FUSED_LAUNCH(true, false, HS);  \
+    else G2OHIP_FUSED_LAUNCH(false, false, HS);         \
   }
   if (short_rows) {
     G2OHIP_FUSED_SHAPE(1)

@@ -39,6 +39,8 @@ int g2ohip_ctx::pg_set_structure(int N, int E, const int* from, const int* to, c
   HK(d_pg_F_diag.upload(pgst.F_diag, stream));
   HK(d_pg_F_ptr.upload(pgst.F_ptr, stream));
   HK(d_pg_F_col.upload(pgst.F_col, stream));
+  pgst.F_head = row_heads(pgst.F_ptr, pgst.F_col);
+  HK(d_pg_F_head.upload(pgst.F_head, stream));
   const int A = static_cast<int>(pgst.act.size());
   const size_t n = 6 * static_cast<size_t>(pgst.Nf);
   HK(d_cams.alloc(7 * static_cast<size_t>(N)));  // the poses
@@ -173,7 +175,7 @@ hipError_t g2ohip_ctx::pg_pcg_fused(const double* rhs, const PcgControl& ctl, in
     return launch_pcg_init<double, 6>(nb, rhs, B.Jinv.p, B.x.p, B.r.p, B.s.p, d_dn_part.p, d_sc.p, stream);
   };
   auto step = [&](int it, int last) {
-    return launch_pcg_fused<double, 6>(nb, d_pg_F_ptr.p, d_pg_F_col.p, d_pg_F_val.p, nullptr, B.Jinv.p, B.x.p, B.r.p,
+    return launch_pcg_fused<double, 6>(nb, d_pg_F_head.p, d_pg_F_col.p, d_pg_F_val.p, nullptr, B.Jinv.p, B.x.p, B.r.p,
                                        B.q.p, s2, w2, dirs, d_dn_part.p, d_fpart.p, d_sc.p, ctl, it, last, stream,
                                        pgst.nF);
   };

@@ -1,6 +1,8 @@
 // Host-side structure builder (see structure.h).
 #include "structure.h"
 
+#include "kernels.h"
+
 #include <algorithm>
 #include <cstdint>
 #include <numeric>
@@ -484,6 +486,19 @@ std::string build_pg_structure(PGStructure& st, int N, int E, const int* from, c
   }
   st.nF = static_cast<int>(st.F_col.size());
   return "";
+}
+
+std::vector<int> row_heads(const std::vector<int>& F_ptr, const std::vector<int>& F_col) {
+  const int nb = static_cast<int>(F_ptr.size()) - 1;
+  std::vector<int> h(static_cast<size_t>(std::max(nb, 1)) * PCG_HEAD_STRIDE, 0);
+  for (int r = 0; r < nb; ++r) {
+    int* row = h.data() + static_cast<size_t>(r) * PCG_HEAD_STRIDE;
+    const int p0 = F_ptr[r], len = F_ptr[r + 1] - p0;
+    row[0] = p0;
+    row[1] = len;
+    for (int t = 0; t < PCG_HEAD_COLS; ++t) row[2 + t] = t < len ? F_col[p0 + t] : r;
+  }
+  return h;
 }
 
 }  // namespace g2ohip

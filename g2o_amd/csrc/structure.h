@@ -104,6 +104,7 @@ struct BAStructure {
   // full symmetric block-CSR used by the PCG: both triangles
   int nF = 0;
   std::vector<int> F_ptr, F_col, F_up, F_trans;
+  std::vector<int> F_head;  // row_heads(F_ptr, F_col), uploaded for k_pcg_fused
   std::vector<int> up_pos, up_posT;  // upper block -> F rows (posT = -1 on diag)
   WinPlan win[2];  // Schur window plans: [0] FP32 blocks, [1] FP64 blocks
 };
@@ -111,6 +112,10 @@ struct BAStructure {
 // Builds everything except the Schur plans. Returns an empty string on
 // success, else an error message. caps (optional): the linearisation chunks'
 // packing targets and per-point limit, instead of the defaults.
+// The row-head table of k_pcg_fused (kernels.h PCG_HEAD_STRIDE per row):
+// first block, length and the first PCG_HEAD_COLS columns of each block row.
+std::vector<int> row_heads(const std::vector<int>& F_ptr, const std::vector<int>& F_col);
+
 std::string build_structure(BAStructure& st, int C, int P, int E, const int* edge_cam, const int* edge_pt,
                             const uint8_t* cam_fixed, const uint8_t* pt_fixed, bool schur,
                             const LinPlanCaps* caps = nullptr);
@@ -145,6 +150,7 @@ struct PGStructure {
   // block and of its transpose (-1 on the diagonal), diagonal positions
   int nF = 0;
   std::vector<int> F_ptr, F_col, up_pos, up_posT, F_diag;
+  std::vector<int> F_head;  // row_heads(F_ptr, F_col), uploaded for k_pcg_fused
 };
 
 std::string build_pg_structure(PGStructure& st, int N, int E, const int* from, const int* to, const uint8_t* fixed);
