@@ -535,7 +535,7 @@ __global__ __launch_bounds__(256) void k_bal_schur_offdiag(DevBAL g, const doubl
 // in its register r. The pair list is read 64 pairs at a time, one per lane,
 // and each pair's observation indices reach the wavefront by readlane.
 typedef double bal_d4 __attribute__((ext_vector_type(4)));
-constexpr int BAL_MFMA_UNROLL = 8;
+constexpr int BAL_MFMA_UNROLL = 16;
 __global__ __launch_bounds__(256) void k_bal_schur_offdiag_mfma(DevBAL g, const double* __restrict__ G,
                                                                 double* __restrict__ F_val) {
   const int lane = threadIdx.x & 63;
