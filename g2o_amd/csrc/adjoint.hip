@@ -49,11 +49,11 @@ __global__ __launch_bounds__(LIN_THREADS) void k_linearize_exact(DevGraph g, con
   __syncthreads();
   for (int s = s0 + tid; s < s1; s += LIN_THREADS) {
     const int pos = g.slot_pos[s], lp = pos - p0;
-    const int praw = g.pos_pt[pos], ph = g.pos_h[pos];
+    const int ph = g.pos_h[pos];
     const int cam = g.slot_cam[s], chh = g.camH[cam];
     const double* Rt = camRt + 12 * cam;
     const double* in = g.intr + 4 * cam;
-    const double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+    const double X[3] = {pts[3 * pos], pts[3 * pos + 1], pts[3 * pos + 2]};  // position-ordered state
     const double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
     double Y[3], e[2], H[81], rho[3];
     project_error(Rt, in, X, z, Y, e);
@@ -142,11 +142,11 @@ __global__ __launch_bounds__(256) void k_adjoint_out(DevGraph g, const double* _
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= g.ns) return;
   const int pos = g.slot_pos[s];
-  const int praw = g.pos_pt[pos], ph = g.pos_h[pos];
+  const int ph = g.pos_h[pos];
   const int cam = g.slot_cam[s], chh = g.camH[cam];
   const double* Rt = camRt + 12 * cam;
   const double* in = g.intr + 4 * cam;
-  const double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+  const double X[3] = {pts[3 * pos], pts[3 * pos + 1], pts[3 * pos + 2]};  // position-ordered state
   const double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
   const double w = g.slot_w[s];
   double Y[3], e[2];

@@ -202,7 +202,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
     std::vector<int4> rec(st.ns);
     for (int s = 0; s < st.ns; ++s) {
       const int pos = st.slot_pos[s], cam = st.slot_cam[s];
-      rec[s] = make_int4(pos, st.pos_pt[pos], cam, st.camH[cam]);
+      rec[s] = make_int4(pos, pos, cam, st.camH[cam]);  // the point state is in position order
     }
     HK(d_slot_rec.upload(rec, stream));
     HK(hipStreamSynchronize(stream));  // the host vectors go out of scope
@@ -446,7 +446,10 @@ int g2ohip_ctx::set_state(const double* cams, const double* pts) {
     if (!(n > 0)) return fail(G2OHIP_ERR_INVALID, "zero quaternion");
     for (int k = 0; k < 4; ++k) q[k] /= n;
   }
-  std::vector<double> p(pts, pts + 3 * static_cast<size_t>(st.P));
+  // the device keeps the points in position order (st.pt_dev)
+  std::vector<double> p(3 * static_cast<size_t>(st.P));
+  for (int i = 0; i < st.P; ++i)
+    for (int k = 0; k < 3; ++k) p[3 * static_cast<size_t>(st.pt_dev[i]) + k] = pts[3 * static_cast<size_t>(i) + k];
   HK(d_cams.upload(c, stream));
   if (st.P > 0) HK(d_pts.upload(p, stream));
   HK(hipStreamSynchronize(stream));
@@ -460,8 +463,12 @@ int g2ohip_ctx::get_state(double* cams, double* pts) {
   if (mode != 0) return fail(G2OHIP_ERR_INVALID, "6-DoF bundle-adjustment call on a pose-graph or BAL context");
   if (!have_state) return fail(G2OHIP_ERR_INVALID, "no state");
   if (cams) HK(hipMemcpyAsync(cams, d_cams.p, 7 * sizeof(double) * st.C, hipMemcpyDeviceToHost, stream));
-  if (pts && st.P > 0) HK(hipMemcpyAsync(pts, d_pts.p, 3 * sizeof(double) * st.P, hipMemcpyDeviceToHost, stream));
+  std::vector<double> p(pts ? 3 * static_cast<size_t>(st.P) : 0);
+  if (pts && st.P > 0) HK(hipMemcpyAsync(p.data(), d_pts.p, 3 * sizeof(double) * st.P, hipMemcpyDeviceToHost, stream));
   HK(hipStreamSynchronize(stream));
+  if (pts)
+    for (int i = 0; i < st.P; ++i)
+      for (int k = 0; k < 3; ++k) pts[3 * static_cast<size_t>(i) + k] = p[3 * static_cast<size_t>(st.pt_dev[i]) + k];
   return G2OHIP_OK;
 }
 

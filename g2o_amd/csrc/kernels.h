@@ -43,7 +43,7 @@ struct DevGraph {
   const int *slot_cam, *slot_pos, *slot_camh, *slot_ph;
   int nq;                                    // observations with a free camera and a free point
   const int *slot_q, *cam_qptr, *q_ph;       // camera-major order of Hpl
-  const int *q_pt, *cam_raw;                 // raw point per q, raw camera per free camera
+  const int *q_pt, *cam_raw;                 // point position per q, raw camera per free camera
   // per slot in one load: {position, raw point, raw camera, free camera}
   // (the indices above, without their dependent lookups; used by k_chi2)
   const int4* slot_rec;

@@ -141,8 +141,8 @@ __global__ __launch_bounds__(256) void k_chi2(DevGraph g, const double* __restri
   double chi = 0;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.ns; s += gridDim.x * blockDim.x) {
     const int4 rec = g.slot_rec[s];
-    const int cam = rec.z, praw = rec.y;
-    double X[3] = {pts[3 * praw], pts[3 * praw + 1], pts[3 * praw + 2]};
+    const int cam = rec.z, pp = rec.y;  // the point's index in the position-ordered state
+    double X[3] = {pts[3 * pp], pts[3 * pp + 1], pts[3 * pp + 2]};
     double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
     double Xc[3], e[2];
     project_error(camRt + 12 * cam, g.intr + 4 * cam, X, z, Xc, e);
@@ -187,8 +187,9 @@ __global__ void k_update(DevGraph g, const double* __restrict__ dx, double* __re
     se3_left_update(dx + 6 * h, in, out);
     for (int i = 0; i < 7; ++i) cams7[7 * t + i] = out[i];
   } else if (t < g.C + g.P) {
+    // the point state is in position order: index p holds position p's point
     int p = t - g.C;
-    int h = g.ptH[p];
+    int h = p < g.npos ? g.pos_h[p] : -1;
     if (h < 0) return;
     const double* d = dx + 6 * static_cast<size_t>(g.Cf) + 3 * static_cast<size_t>(h);
     pts[3 * p] += d[0];
@@ -215,7 +216,7 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 // instead of walking one slot's chain of dependent loads
 // (slot -> point / camera -> coordinates) at a time.
 struct SlotOps {
-  int pos, cam, ph, chh, praw, q;
+  int pos, cam, ph, chh, q;
   double Rt[12], in[4], X[3], z[2], w;
   double S[3];  // information factor (identity without g.slot_S)
 };
@@ -240,7 +241,6 @@ __device__ __forceinline__ void load_slot_ops(const DevGraph& g, const double* _
   } else {
     o.S[0] = 1.0; o.S[1] = 0.0; o.S[2] = 1.0;
   }
-  o.praw = g.pos_pt[o.pos];
   o.ph = g.pos_h[o.pos];
   o.chh = g.camH[o.cam];
 #pragma unroll
@@ -248,7 +248,7 @@ __device__ __forceinline__ void load_slot_ops(const DevGraph& g, const double* _
 #pragma unroll
   for (int k = 0; k < 4; ++k) o.in[k] = g.intr[4 * static_cast<size_t>(o.cam) + k];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) o.X[k] = pts[3 * static_cast<size_t>(o.praw) + k];
+  for (int k = 0; k < 3; ++k) o.X[k] = pts[3 * static_cast<size_t>(o.pos) + k];  // position-ordered state
 }
 // Slots whose operands one thread of a linearisation chunk loads together:
 // two for the linearisation; one (all of a slot's loads before its first
@@ -1392,13 +1392,13 @@ hipError_t launch_jacobi(int nb, const HT* F_val, const int* F_diag, HT* Jinv, i
 // reduces them over the workgroup: no atomics, fixed order. Writes the block
 // into F_val (if not null), its inverse into Jinv and b_i - sum V Dinv b_p
 // into bschur (if not null).
-// V += omega Jc^T Jp of the observation (slot, raw point) by the camera
-// whose [R | t] and intrinsics are Rt, in.
+// V += omega Jc^T Jp of the observation (slot, point position pp) by the
+// camera whose [R | t] and intrinsics are Rt, in.
 __device__ __forceinline__ void obs_V64(const DevGraph& g, const double* __restrict__ Rt,
                                        const double* __restrict__ in, const double* __restrict__ pts, int slot,
-                                       int praw, double* V) {
-  const double X[3] = {pts[3 * static_cast<size_t>(praw)], pts[3 * static_cast<size_t>(praw) + 1],
-                       pts[3 * static_cast<size_t>(praw) + 2]};
+                                       int pp, double* V) {
+  const double X[3] = {pts[3 * static_cast<size_t>(pp)], pts[3 * static_cast<size_t>(pp) + 1],
+                       pts[3 * static_cast<size_t>(pp) + 2]};
   const double z[2] = {g.slot_meas[2 * static_cast<size_t>(slot)], g.slot_meas[2 * static_cast<size_t>(slot) + 1]};
   const double w = g.slot_w[slot];
   double Xc[3], e[2], rho[3], Jp[6], Jc[12];
@@ -1443,7 +1443,7 @@ __global__ __launch_bounds__(256) void k_schur_diag(DevGraph g, const int* __res
     // indices of this observation and of its predecessor, loaded together
     const int ph = g.q_ph[q];
     const int ph_prev = q > qb ? g.q_ph[q - 1] : -1;
-    const int slot = q_slot[q], praw = g.q_pt[q];
+    const int slot = q_slot[q], pp = g.q_pt[q];
     if (ph_prev == ph) continue;  // summed by the run's first observation
     double V[18];
     for (int k = 0; k < 18; ++k) V[k] = 0;
@@ -1453,7 +1453,7 @@ __global__ __launch_bounds__(256) void k_schur_diag(DevGraph g, const int* __res
         for (int k = 0; k < 18; ++k) V[k] += static_cast<double>(h[k]);
       }
     } else {
-      obs_V64(g, Rt, in, pts, slot, praw, V);
+      obs_V64(g, Rt, in, pts, slot, pp, V);
       for (int q2 = q + 1; q2 < qe && g.q_ph[q2] == ph; ++q2) obs_V64(g, Rt, in, pts, q_slot[q2], g.q_pt[q2], V);
     }
     double D[9];
@@ -1570,7 +1570,6 @@ __global__ __launch_bounds__(LIN_THREADS) void k_schur_diag_pm(DevGraph g, const
       const int pos = o.pos, cam = o.cam, chh = o.chh;
       if (o.ph < 0 || chh < 0) continue;
       if (s > s0 && nb_pos[u][0] == pos && nb_cam[u][0] == cam) continue;  // the run's first slot sums it
-      const int praw = o.praw;
       const double* Rt = o.Rt;
       const double* in = o.in;
       double V[18];
@@ -1578,7 +1577,7 @@ __global__ __launch_bounds__(LIN_THREADS) void k_schur_diag_pm(DevGraph g, const
       obs_V64_ops(g, o, V);
       if (s + 1 < s1 && nb_pos[u][1] == pos && nb_cam[u][1] == cam)
         for (int s2 = s + 1; s2 < s1 && g.slot_pos[s2] == pos && g.slot_cam[s2] == cam; ++s2)
-          obs_V64(g, Rt, in, pts, s2, praw, V);
+          obs_V64(g, Rt, in, pts, s2, pos, V);
       const double* pd = s_pd + 9 * (pos - p0);
       const double D[9] = {pd[0], pd[1], pd[2], pd[1], pd[3], pd[4], pd[2], pd[4], pd[5]};
       double VD[18];

@@ -129,6 +129,15 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
     }
   }
   st.pos_ptr[st.npos] = s;
+  // device order of the point state: the points with a position at their
+  // position, then the rest in input order
+  st.pt_dev.assign(P, -1);
+  for (int i = 0; i < st.npos; ++i) st.pt_dev[st.pos_pt[i]] = i;
+  {
+    int nxt = st.npos;
+    for (int p = 0; p < P; ++p)
+      if (st.pt_dev[p] < 0) st.pt_dev[p] = nxt++;
+  }
   st.slot_camh.resize(st.ns);
   st.slot_ph.resize(st.ns);
   for (int q = 0; q < st.ns; ++q) {
@@ -152,7 +161,7 @@ std::string build_structure(BAStructure& st, int C, int P, int E, const int* edg
       const int d = fill[c]++;
       st.q_slot[d] = q;
       st.q_ph[d] = st.slot_ph[q];
-      st.q_pt[d] = st.pos_pt[st.slot_pos[q]];
+      st.q_pt[d] = st.slot_pos[q];
       st.slot_q[q] = d;
     }
   }

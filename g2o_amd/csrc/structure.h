@@ -72,6 +72,7 @@ struct BAStructure {
   // processing positions: points with at least one active edge
   int npos = 0;
   std::vector<int> pos_pt;           // raw point id
+  std::vector<int> pt_dev;           // P: raw point -> index in the device point state (positions first)
   std::vector<int> pos_h;            // free point index or -1
   std::vector<int> pos_ptr;          // npos+1 slot CSR
   std::vector<int> pos_nfree;        // slots with a free camera (they come first)
@@ -86,7 +87,7 @@ struct BAStructure {
   int nq = 0;
   std::vector<int> cam_qptr;         // Cf+1
   std::vector<int> q_slot, q_ph;     // nq
-  std::vector<int> q_pt;             // nq: raw point of each observation
+  std::vector<int> q_pt;             // nq: position of each observation's point
   std::vector<int> slot_q;           // ns -> q or -1
   std::vector<int> edge_slot;        // E -> slot or -1 (inactive edge)
   // linearisation chunks
