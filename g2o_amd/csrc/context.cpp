@@ -202,7 +202,7 @@ int g2ohip_ctx::set_structure(int C, int P, int E, const int* ec, const int* ep,
     std::vector<int4> rec(st.ns);
     for (int s = 0; s < st.ns; ++s) {
       const int pos = st.slot_pos[s], cam = st.slot_cam[s];
-      rec[s] = make_int4(pos, pos, cam, st.camH[cam]);  // the point state is in position order
+      rec[s] = make_int4(pos, st.pos_h[pos], cam, st.camH[cam]);
     }
     HK(d_slot_rec.upload(rec, stream));
     HK(hipStreamSynchronize(stream));  // the host vectors go out of scope

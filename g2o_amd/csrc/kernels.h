@@ -44,8 +44,9 @@ struct DevGraph {
   int nq;                                    // observations with a free camera and a free point
   const int *slot_q, *cam_qptr, *q_ph;       // camera-major order of Hpl
   const int *q_pt, *cam_raw;                 // point position per q, raw camera per free camera
-  // per slot in one load: {position, raw point, raw camera, free camera}
-  // (the indices above, without their dependent lookups; used by k_chi2)
+  // per slot in one load: {position (also the point's index in the
+  // position-ordered point state), free point index or -1, raw camera, free
+  // camera or -1}: the indices above without their dependent lookups
   const int4* slot_rec;
   const double *slot_meas, *slot_w;
   // per slot the upper-triangular factor [s0 s1 s2] of the edge's information

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_chi2(DevGraph g, const double* __restri
   double chi = 0;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.ns; s += gridDim.x * blockDim.x) {
     const int4 rec = g.slot_rec[s];
-    const int cam = rec.z, pp = rec.y;  // the point's index in the position-ordered state
+    const int cam = rec.z, pp = rec.x;  // the point's index in the position-ordered state
     double X[3] = {pts[3 * pp], pts[3 * pp + 1], pts[3 * pp + 2]};
     double z[2] = {g.slot_meas[2 * s], g.slot_meas[2 * s + 1]};
     double Xc[3], e[2];
@@ -229,8 +229,12 @@ __device__ __forceinline__ void whiten_obs(const DevGraph& g, const double* S, d
 }
 __device__ __forceinline__ void load_slot_ops(const DevGraph& g, const double* __restrict__ camRt,
                                               const double* __restrict__ pts, int s, SlotOps& o) {
-  o.pos = g.slot_pos[s];
-  o.cam = g.slot_cam[s];
+  // {position, free point index, camera, free camera index} in one load
+  const int4 rec = g.slot_rec[s];
+  o.pos = rec.x;
+  o.ph = rec.y;
+  o.cam = rec.z;
+  o.chh = rec.w;
   o.q = g.slot_q[s];
   o.z[0] = g.slot_meas[2 * static_cast<size_t>(s)];
   o.z[1] = g.slot_meas[2 * static_cast<size_t>(s) + 1];
@@ -241,8 +245,6 @@ __device__ __forceinline__ void load_slot_ops(const DevGraph& g, const double* _
   } else {
     o.S[0] = 1.0; o.S[1] = 0.0; o.S[2] = 1.0;
   }
-  o.ph = g.pos_h[o.pos];
-  o.chh = g.camH[o.cam];
 #pragma unroll
   for (int k = 0; k < 12; ++k) o.Rt[k] = camRt[12 * static_cast<size_t>(o.cam) + k];
 #pragma unroll
