@@ -426,6 +426,12 @@ class PoseGraphOptimizer:
         self.ctx.call("g2ohip_optimize", iterations, stats, ct.byref(done))
         return [{f: getattr(s, f) for f, _ in L.IterStats._fields_} for s in stats[:done.value]]
 
+    def last_solve_info(self):
+        """[PCG iterations, 0, PCG residual memory, 0] of the last solve."""
+        v = np.zeros(4)
+        self.ctx.call("g2ohip_last_solve_info", _dp(v))
+        return v
+
     def lm_trials(self) -> np.ndarray:
         """Per LM trial of the last optimize(): columns TRIAL_FIELDS."""
         return _lm_trials(self.ctx)
