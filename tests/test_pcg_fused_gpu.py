@@ -100,3 +100,62 @@ def test_posegraph_one_launch_equals_two_launch(monkeypatch, tol):
     xb, ib = _solve(b)
     x_ref, _ = _solve(g2o_amd.PoseGraphOptimizer(prob, device=0, pcg_tolerance=1e-28))
     _compare(xa, ia, xb, ib, x_ref)
+
+
+@pytest.mark.parametrize("nf", [2047, 2048, 2049])
+def test_posegraph_at_the_one_launch_size_limit(monkeypatch, nf):
+    """The one-launch iteration serves up to 2,048 block rows (256 workgroups
+    of 8 rows, whose partials each lane of the next launch loads four at a
+    time); at 2,049 both settings run the two-launch iteration and agree bit
+    for bit."""
+    import g2o_amd
+    prob = datasets.make_sphere_pose_graph(seed=6, nodes_per_level=70, laps=30)
+    assert prob.num_poses == 2100
+    fixed = np.zeros(prob.num_poses, np.uint8)
+    fixed[:prob.num_poses - nf] = 1
+    prob.fixed = fixed
+    a, b = _pair(monkeypatch, lambda: g2o_amd.PoseGraphOptimizer(prob, device=0, pcg_tolerance=1e-10))
+    assert a.Nf == nf
+    xa, ia = _solve(a)
+    xb, ib = _solve(b)
+    if nf > 2048:
+        _say(f"Nf {nf}: PCG iterations {ia} / {ib}")
+        assert ia == ib and np.array_equal(xa, xb)
+        return
+    x_ref, _ = _solve(g2o_amd.PoseGraphOptimizer(prob, device=0, pcg_tolerance=1e-28))
+    _compare(xa, ia, xb, ib, x_ref)
+
+
+@pytest.mark.parametrize("kind", ["ba", "bal"])
+def test_one_free_camera(monkeypatch, kind, oracle_lib):
+    """One free camera: one workgroup, and block-Jacobi PCG on a single
+    block is exact after one iteration, as in the oracle."""
+    import g2o_amd
+    if kind == "ba":
+        prob = datasets.make_small_ba(seed=12, num_cams=2, num_pts=60)
+        prob.cam_fixed = np.array([1, 0], np.uint8)
+        ora = oracle_lib.OracleBA(prob)
+        mk = lambda: g2o_amd.BundleAdjuster(prob, device=0, pcg_tolerance=1e-10)
+    else:
+        prob = datasets.make_bal(seed=12, num_cams=2, num_pts=60, k=(0.05, -0.01))
+        prob.cam_fixed = np.array([1, 0], np.uint8)
+        ora = oracle_lib.OracleBAL(prob)
+        mk = lambda: g2o_amd.BALAdjuster(prob, device=0, pcg_tolerance=1e-10)
+    a, b = _pair(monkeypatch, mk)
+    a.build_system()
+    lam = a.lambda_init()
+    xs = []
+    for opt in (a, b):
+        if opt is b:
+            opt.build_system()
+        opt.set_lambda(lam)
+        assert opt.solve()
+        xs.append((opt.x(), int(opt.last_solve_info()[0])))
+    ora.linearize()
+    st, x_o, _ = ora.solve(lam, solver=0)
+    st1, _, it_o = ora.solve(lam, solver=1, tol=1e-10)
+    assert st == 0 and st1 == 0
+    _say(f"{kind}: PCG iterations one-launch {xs[0][1]}, two-launch {xs[1][1]}, oracle {it_o}")
+    for x, it in xs:
+        assert it == it_o == 1
+        assert np.linalg.norm(x - x_o) <= 1e-9 * np.linalg.norm(x_o)
